@@ -1,0 +1,742 @@
+/*
+ * fmm_oracle.c -- CPU restatement of the Tapas FMM path (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product path never calls it.
+ *
+ * It restates, in plain C, the algorithm of the reference path
+ * (paths relative to the reference repository):
+ *   TP = cpp/include/tapas/, EX = sample/exafmm-dev-13274dd4ac68/
+ *
+ *   bounding box (scalar cube)     TP/hot/buildtree.h:72-87, TP/vec.h:39-43
+ *   body keys                      TP/hot/buildtree.h:504-539,
+ *                                  TP/sfc_morton.h:387-402, :208-211
+ *   key sort                       TP/common.h:248-273 (ties: any order)
+ *   cell generation                TP/hot/buildtree.h:388-459
+ *   cell region / centre           TP/sfc_morton.h:536-557, TP/hot.h:232-238
+ *   upward pass (P2M, M2M)         EX/examples/tapas_exafmm.cxx:87-122,
+ *                                  EX/examples/LaplaceSphericalCPU_tapas.h:138-206
+ *   dual tree traversal            EX/examples/tapas_exafmm.cxx:209-286,
+ *                                  TP/hot/mapper.h:29-169, TP/geometry.h:80-83
+ *   M2L                            EX/examples/LaplaceSphericalCPU_tapas.h:208-273
+ *   P2P                            EX/examples/LaplaceP2PCPU_tapas.h:48-74
+ *   downward pass (L2L, L2P)       EX/examples/tapas_exafmm.cxx:124-140,
+ *                                  EX/examples/LaplaceSphericalCPU_tapas.h:275-345
+ *   solid harmonics                EX/examples/LaplaceSphericalCPU_tapas.h:43-135
+ *   direct sum + error metric      EX/examples/tapas_exafmm.cxx:288-383,
+ *                                  EX/include/verify.h:18-92
+ *
+ * Arithmetic is written in the reference's operation order and must be
+ * compiled with -ffp-contract=off (no FMA), so that the key, region and
+ * multipole-acceptance decisions are bit-identical to the reference's
+ * x86-64 build.  Loop orders (tree build, traversal, accumulation) follow
+ * the reference too, so field sums round the same way.
+ *
+ * Parity pinning of each function is listed in tests/test_oracle_*.py.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "fmm_oracle.h"
+
+#define P ORC_P
+#define NTERM ORC_NTERM
+#define MAXDEPTH 19
+#define DEPTHBITS 5
+
+typedef struct { double re, im; } cx;
+
+static inline cx cx_make(double re, double im) { cx r; r.re = re; r.im = im; return r; }
+static inline cx cx_add(cx a, cx b) { return cx_make(a.re + b.re, a.im + b.im); }
+/* complex * complex: (ac - bd, ad + bc), no fused terms */
+static inline cx cx_mul(cx a, cx b) {
+  double ac = a.re * b.re, bd = a.im * b.im, ad = a.re * b.im, bc = a.im * b.re;
+  return cx_make(ac - bd, ad + bc);
+}
+static inline cx cx_scale(cx a, double s) { return cx_make(a.re * s, a.im * s); }
+static inline cx cx_rscale(double s, cx a) { return cx_make(s * a.re, s * a.im); }
+static inline cx cx_conj(cx a) { return cx_make(a.re, -a.im); }
+
+static inline int oddeven(int n) { return (n & 1) ? -1 : 1; }
+static inline int ipow2n(int n) { return (n >= 0) ? 1 : oddeven(n); }
+
+/* ------------------------------------------------------------------ */
+/* Geometry and Morton keys                                             */
+/* ------------------------------------------------------------------ */
+
+void orc_bbox(const double *x, const double *y, const double *z, int64_t n,
+              double *out_min, double *out_max) {
+  /* The reference assigns a scalar to a Vec inside a per-dimension loop;
+   * the broadcast constructor makes every component equal.  Body 0 thus
+   * contributes only its z, every later body all three coordinates. */
+  double mx = 0, mn = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const double pos[3] = {x[i], y[i], z[i]};
+    for (int d = 0; d < 3; d++) {
+      if (i == 0) {
+        mx = pos[d];
+        mn = pos[d];
+      } else {
+        /* std::max(a, b) returns a if !(a < b); std::min(a, b) returns a if !(b < a) */
+        mx = (pos[d] < mx) ? mx : pos[d];
+        mn = (mn < pos[d]) ? mn : pos[d];
+      }
+    }
+  }
+  for (int d = 0; d < 3; d++) { out_min[d] = mn; out_max[d] = mx; }
+}
+
+uint64_t orc_finest_key(const int anchor[3]) {
+  uint64_t k = 0;
+  int mask = 1 << (MAXDEPTH - 1);
+  for (int i = 0; i < MAXDEPTH; i++) {
+    for (int d = 2; d >= 0; d--) {
+      k = (k << 1) | (uint64_t)((anchor[d] & mask) >> (MAXDEPTH - i - 1));
+    }
+    mask >>= 1;
+  }
+  return (k << DEPTHBITS) | MAXDEPTH;
+}
+
+void orc_keys(const double *x, const double *y, const double *z, int64_t n,
+              const double *rmin, const double *rmax, uint64_t *keys) {
+  const int nfin = 1 << MAXDEPTH;
+  double pitch[3];
+  for (int d = 0; d < 3; d++) pitch[d] = (rmax[d] - rmin[d]) / (double)nfin;
+  for (int64_t i = 0; i < n; i++) {
+    double ofst[3] = {x[i], y[i], z[i]};
+    int anchor[3];
+    for (int d = 0; d < 3; d++) {
+      ofst[d] -= rmin[d];
+      ofst[d] /= pitch[d];
+    }
+    for (int d = 0; d < 3; d++) {
+      anchor[d] = (int)ofst[d];
+      if (anchor[d] == nfin) anchor[d]--;
+    }
+    keys[i] = orc_finest_key(anchor);
+  }
+}
+
+static inline int key_depth(uint64_t k) { return (int)(k & 31u); }
+static inline uint64_t key_next(uint64_t k) {
+  int d = key_depth(k);
+  return k + ((uint64_t)1 << (3 * (MAXDEPTH - d) + DEPTHBITS));
+}
+static inline uint64_t key_child(uint64_t k, int idx) {
+  int d = key_depth(k) + 1;
+  uint64_t kk = ((k >> DEPTHBITS) << DEPTHBITS) | (uint64_t)d;
+  return kk | ((uint64_t)idx << ((MAXDEPTH - d) * 3 + DEPTHBITS));
+}
+static inline int key_dir(uint64_t k, int dim, int depth) {
+  uint64_t dir = (k >> DEPTHBITS) >> ((MAXDEPTH - depth) * 3);
+  return (int)((dir >> dim) & 1);
+}
+
+void orc_cell_region(uint64_t key, const double *rmin, const double *rmax,
+                     double *cmin, double *cmax) {
+  double mx[3], mn[3];
+  for (int d = 0; d < 3; d++) { mx[d] = rmax[d]; mn[d] = rmin[d]; }
+  int depth = key_depth(key);
+  for (int dep = 1; dep <= depth; dep++) {
+    for (int dim = 0; dim < 3; dim++) {
+      double c = (mx[dim] + mn[dim]) / 2;
+      if (key_dir(key, dim, dep) == 1) mn[dim] = c;
+      else mx[dim] = c;
+    }
+  }
+  for (int d = 0; d < 3; d++) { cmin[d] = mn[d]; cmax[d] = mx[d]; }
+}
+
+/* ------------------------------------------------------------------ */
+/* Solid harmonics                                                      */
+/* ------------------------------------------------------------------ */
+
+static void cart2sph(const double dX[3], double *r, double *theta, double *phi) {
+  double n2 = 0;
+  for (int i = 0; i < 3; i++) n2 += dX[i] * dX[i];
+  *r = sqrt(n2);
+  *theta = (*r == 0) ? 0 : acos(dX[2] / *r);
+  *phi = atan2(dX[1], dX[0]);
+}
+
+/* r^n Y_n^m and its theta derivative; Ynm has P*P entries */
+static void eval_multipole(double rho, double alpha, double beta, cx *Ynm, cx *YnmTheta) {
+  double x = cos(alpha), y = sin(alpha);
+  double fact = 1, pn = 1, rhom = 1;
+  cx ei = cx_make(cos(beta), sin(beta));
+  cx eim = cx_make(1.0, 0.0);
+  for (int m = 0; m < P; m++) {
+    double p = pn;
+    int npn = m * m + 2 * m, nmn = m * m;
+    Ynm[npn] = cx_rscale(rhom * p, eim);
+    Ynm[nmn] = cx_conj(Ynm[npn]);
+    double p1 = p;
+    p = x * (2 * m + 1) * p1;
+    YnmTheta[npn] = cx_rscale(rhom * (p - (m + 1) * x * p1) / y, eim);
+    rhom *= rho;
+    double rhon = rhom;
+    for (int n = m + 1; n < P; n++) {
+      int npm = n * n + n + m, nmm = n * n + n - m;
+      rhon /= -(n + m);
+      Ynm[npm] = cx_rscale(rhon * p, eim);
+      Ynm[nmm] = cx_conj(Ynm[npm]);
+      double p2 = p1;
+      p1 = p;
+      p = (x * (2 * n + 1) * p1 - (n + m) * p2) / (n - m + 1);
+      YnmTheta[npm] = cx_rscale(rhon * ((n - m + 1) * p - (n + 1) * x * p1) / y, eim);
+      rhon *= rho;
+    }
+    rhom /= -(2 * m + 2) * (2 * m + 1);
+    pn = -pn * fact * y;
+    fact += 2;
+    eim = cx_mul(eim, ei);
+  }
+}
+
+/* r^{-n-1} Y_n^m */
+static void eval_local(double rho, double alpha, double beta, cx *Ynm) {
+  double x = cos(alpha), y = sin(alpha);
+  double fact = 1, pn = 1;
+  double invR = -1.0 / rho;
+  double rhom = -invR;
+  cx ei = cx_make(cos(beta), sin(beta));
+  cx eim = cx_make(1.0, 0.0);
+  for (int m = 0; m < P; m++) {
+    double p = pn;
+    int npn = m * m + 2 * m, nmn = m * m;
+    Ynm[npn] = cx_rscale(rhom * p, eim);
+    Ynm[nmn] = cx_conj(Ynm[npn]);
+    double p1 = p;
+    p = x * (2 * m + 1) * p1;
+    rhom *= invR;
+    double rhon = rhom;
+    for (int n = m + 1; n < P; n++) {
+      int npm = n * n + n + m, nmm = n * n + n - m;
+      Ynm[npm] = cx_rscale(rhon * p, eim);
+      Ynm[nmm] = cx_conj(Ynm[npm]);
+      double p2 = p1;
+      p1 = p;
+      p = (x * (2 * n + 1) * p1 - (n + m) * p2) / (n - m + 1);
+      rhon *= invR * (n - m + 1);
+    }
+    pn = -pn * fact * y;
+    fact += 2;
+    eim = cx_mul(eim, ei);
+  }
+}
+
+void orc_eval_multipole(double rho, double alpha, double beta, double *ynm, double *ynm_theta) {
+  cx Y[P * P], T[P * P];
+  memset(T, 0, sizeof(T));
+  eval_multipole(rho, alpha, beta, Y, T);
+  memcpy(ynm, Y, sizeof(Y));
+  if (ynm_theta) memcpy(ynm_theta, T, sizeof(T));
+}
+
+void orc_eval_local(double rho, double alpha, double beta, double *ynm) {
+  cx Y[P * P];
+  eval_local(rho, alpha, beta, Y);
+  memcpy(ynm, Y, sizeof(Y));
+}
+
+/* ------------------------------------------------------------------ */
+/* Expansion kernels on raw coefficient arrays                          */
+/* ------------------------------------------------------------------ */
+
+/* M[nms] += q * Y[n^2+n-m] over the bodies (x,y,z,q) of a leaf */
+static void p2m(cx *M, const double *cen, const double *x, const double *y, const double *z,
+                const double *q, int64_t nb) {
+  cx Ynm[P * P], YnmTheta[P * P];
+  for (int64_t i = 0; i < nb; i++) {
+    double dX[3] = {x[i] - cen[0], y[i] - cen[1], z[i] - cen[2]};
+    double rho, alpha, beta;
+    cart2sph(dX, &rho, &alpha, &beta);
+    eval_multipole(rho, alpha, beta, Ynm, YnmTheta);
+    for (int n = 0; n < P; n++) {
+      for (int m = 0; m <= n; m++) {
+        int nm = n * n + n - m;
+        int nms = n * (n + 1) / 2 + m;
+        M[nms] = cx_add(M[nms], cx_rscale(q[i], Ynm[nm]));
+      }
+    }
+  }
+}
+
+/* parent M += shifted child M; dX = parent centre - child centre */
+static void m2m(cx *Mp, const cx *Mc, const double dX[3]) {
+  cx Ynm[P * P], YnmTheta[P * P];
+  double rho, alpha, beta;
+  cart2sph(dX, &rho, &alpha, &beta);
+  eval_multipole(rho, alpha, beta, Ynm, YnmTheta);
+  for (int j = 0; j < P; j++) {
+    for (int k = 0; k <= j; k++) {
+      int jks = j * (j + 1) / 2 + k;
+      cx M = cx_make(0, 0);
+      for (int n = 0; n <= j; n++) {
+        int mlo = (-n > -j + k + n) ? -n : (-j + k + n);
+        int mhi = (k - 1 < n) ? (k - 1) : n;
+        for (int m = mlo; m <= mhi; m++) {
+          int jnkms = (j - n) * (j - n + 1) / 2 + k - m;
+          int nm = n * n + n - m;
+          M = cx_add(M, cx_scale(cx_mul(Mc[jnkms], Ynm[nm]), (double)(ipow2n(m) * oddeven(n))));
+        }
+        int mhi2 = (n < j + k - n) ? n : (j + k - n);
+        for (int m = k; m <= mhi2; m++) {
+          int jnkms = (j - n) * (j - n + 1) / 2 - k + m;
+          int nm = n * n + n - m;
+          M = cx_add(M, cx_scale(cx_mul(cx_conj(Mc[jnkms]), Ynm[nm]), (double)oddeven(k + n + m)));
+        }
+      }
+      Mp[jks] = cx_add(Mp[jks], M);
+    }
+  }
+}
+
+/* target L += M2L(source M); dX = target centre - source centre */
+static void m2l(cx *Li, const cx *Mj, const double dX[3]) {
+  cx Ynmi[P * P];
+  double rho, alpha, beta;
+  cart2sph(dX, &rho, &alpha, &beta);
+  eval_local(rho, alpha, beta, Ynmi);
+  for (int j = 0; j < P; j++) {
+    double Cnm = oddeven(j);
+    for (int k = 0; k <= j; k++) {
+      int jks = j * (j + 1) / 2 + k;
+      cx L = cx_make(0, 0);
+      for (int n = 0; n < P - j; n++) {
+        for (int m = -n; m < 0; m++) {
+          int nms = n * (n + 1) / 2 - m;
+          int jnkm = (j + n) * (j + n) + j + n + m - k;
+          L = cx_add(L, cx_mul(cx_scale(cx_conj(Mj[nms]), Cnm), Ynmi[jnkm]));
+        }
+        for (int m = 0; m <= n; m++) {
+          int nms = n * (n + 1) / 2 + m;
+          int jnkm = (j + n) * (j + n) + j + n + m - k;
+          double Cnm2 = Cnm * oddeven((k - m) * (k < m) + m);
+          L = cx_add(L, cx_mul(cx_scale(Mj[nms], Cnm2), Ynmi[jnkm]));
+        }
+      }
+      Li[jks] = cx_add(Li[jks], L);
+    }
+  }
+}
+
+/* child L += shifted parent L; dX = child centre - parent centre */
+static void l2l(cx *Lc, const cx *Lp, const double dX[3]) {
+  cx Ynm[P * P], YnmTheta[P * P];
+  double rho, alpha, beta;
+  cart2sph(dX, &rho, &alpha, &beta);
+  eval_multipole(rho, alpha, beta, Ynm, YnmTheta);
+  for (int j = 0; j < P; j++) {
+    for (int k = 0; k <= j; k++) {
+      int jks = j * (j + 1) / 2 + k;
+      cx L = cx_make(0, 0);
+      for (int n = j; n < P; n++) {
+        for (int m = j + k - n; m < 0; m++) {
+          int jnkm = (n - j) * (n - j) + n - j + m - k;
+          int nms = n * (n + 1) / 2 - m;
+          L = cx_add(L, cx_scale(cx_mul(cx_conj(Lp[nms]), Ynm[jnkm]), (double)oddeven(k)));
+        }
+        for (int m = 0; m <= n; m++) {
+          if (n - j >= abs(m - k)) {
+            int jnkm = (n - j) * (n - j) + n - j + m - k;
+            int nms = n * (n + 1) / 2 + m;
+            L = cx_add(L, cx_scale(cx_mul(Lp[nms], Ynm[jnkm]), (double)oddeven((m - k) * (m < k))));
+          }
+        }
+      }
+      Lc[jks] = cx_add(Lc[jks], L);
+    }
+  }
+}
+
+/* trg (pot, ax, ay, az) of one body: divide P2P sum by q, add local field */
+static void l2p(double *trg, const double bx[3], double q, const cx *L, const double cen[3]) {
+  cx Ynm[P * P], YnmTheta[P * P];
+  double dX[3] = {bx[0] - cen[0], bx[1] - cen[1], bx[2] - cen[2]};
+  double sph[3] = {0, 0, 0}, cart[3];
+  double r, theta, phi;
+  cart2sph(dX, &r, &theta, &phi);
+  eval_multipole(r, theta, phi, Ynm, YnmTheta);
+  for (int d = 0; d < 4; d++) trg[d] /= q;
+  for (int n = 0; n < P; n++) {
+    int nm = n * n + n;
+    int nms = n * (n + 1) / 2;
+    double re = cx_mul(L[nms], Ynm[nm]).re;
+    trg[0] += re;
+    sph[0] += re / r * n;
+    sph[1] += cx_mul(L[nms], YnmTheta[nm]).re;
+    for (int m = 1; m <= n; m++) {
+      nm = n * n + n + m;
+      nms = n * (n + 1) / 2 + m;
+      cx ly = cx_mul(L[nms], Ynm[nm]);
+      trg[0] += 2 * ly.re;
+      sph[0] += 2 * ly.re / r * n;
+      sph[1] += 2 * cx_mul(L[nms], YnmTheta[nm]).re;
+      sph[2] += 2 * cx_mul(ly, cx_make(0, 1)).re * m;
+    }
+  }
+  cart[0] = sin(theta) * cos(phi) * sph[0] + cos(theta) * cos(phi) / r * sph[1]
+          - sin(phi) / r / sin(theta) * sph[2];
+  cart[1] = sin(theta) * sin(phi) * sph[0] + cos(theta) * sin(phi) / r * sph[1]
+          + cos(phi) / r / sin(theta) * sph[2];
+  cart[2] = cos(theta) * sph[0] - sin(theta) / r * sph[1];
+  trg[1] += cart[0];
+  trg[2] += cart[1];
+  trg[3] += cart[2];
+}
+
+/* Raw-array kernel entry points (for per-kernel parity tests). */
+void orc_p2m(double *M, const double *cen, const double *x, const double *y, const double *z,
+             const double *q, int64_t nb) {
+  p2m((cx *)M, cen, x, y, z, q, nb);
+}
+void orc_m2m(double *Mp, const double *Mc, const double *dX) { m2m((cx *)Mp, (const cx *)Mc, dX); }
+void orc_m2l(double *Li, const double *Mj, const double *dX) { m2l((cx *)Li, (const cx *)Mj, dX); }
+void orc_l2l(double *Lc, const double *Lp, const double *dX) { l2l((cx *)Lc, (const cx *)Lp, dX); }
+void orc_l2p(double *trg, const double *bx, double q, const double *L, const double *cen) {
+  l2p(trg, bx, q, (const cx *)L, cen);
+}
+
+/* ------------------------------------------------------------------ */
+/* Tree, traversal and full evaluation                                  */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  uint64_t key;
+  int64_t nb, bid;     /* bid = first sorted body of a leaf; 0 for internal cells */
+  int32_t leaf, depth;
+  int32_t child[8];    /* -1 for leaves */
+  int32_t parent;
+  double cmin[3], cmax[3], center[3];
+} ocell;
+
+struct orc_ctx {
+  int64_t n;
+  int ncrit;
+  double theta;
+  double rmin[3], rmax[3];
+  /* sorted bodies */
+  uint64_t *keys;
+  int64_t *perm;        /* sorted position -> original index */
+  double *x, *y, *z, *q;
+  double *trg;          /* 4 per sorted body */
+  /* cells in creation (depth-first) order */
+  ocell *cells;
+  int64_t ncells, cap;
+  cx *M, *L;            /* NTERM per cell */
+  int max_depth;
+  /* counters */
+  uint64_t n_m2l, n_p2p_leafpairs, n_p2p_bodypairs, n_p2p_selfpairs;
+  /* optional interaction lists (cell indices) */
+  int record;
+  int64_t *m2l_list, n_m2l_cap;
+  int64_t *p2p_list, n_p2p_cap;
+};
+
+static int64_t lower_bound_u64(const uint64_t *a, int64_t lo, int64_t hi, uint64_t v) {
+  while (lo < hi) {
+    int64_t mid = lo + (hi - lo) / 2;
+    if (a[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+static int64_t new_cell(orc_ctx *c) {
+  if (c->ncells == c->cap) {
+    c->cap = c->cap ? c->cap * 2 : 1024;
+    c->cells = (ocell *)realloc(c->cells, sizeof(ocell) * c->cap);
+  }
+  return c->ncells++;
+}
+
+static int64_t generate_cell(orc_ctx *c, uint64_t k, int64_t pbeg, int64_t pend, int32_t parent) {
+  uint64_t k2 = key_next(k);
+  int64_t rb = lower_bound_u64(c->keys, pbeg, pend, k);
+  int64_t re = lower_bound_u64(c->keys, pbeg, pend, k2);
+  int64_t nb = re - rb;
+  int depth = key_depth(k);
+  /* A cell at the finest depth cannot be split further; the reference would
+   * form invalid child keys there.  Such a cell stays a leaf. */
+  int is_leaf = (nb <= c->ncrit) || depth == MAXDEPTH;
+  int64_t idx = new_cell(c);
+  ocell *cell = &c->cells[idx];
+  cell->key = k;
+  cell->nb = nb;
+  cell->bid = is_leaf ? rb : 0;
+  cell->leaf = is_leaf;
+  cell->depth = depth;
+  cell->parent = parent;
+  orc_cell_region(k, c->rmin, c->rmax, cell->cmin, cell->cmax);
+  for (int d = 0; d < 3; d++) cell->center[d] = (cell->cmax[d] + cell->cmin[d]) / 2;
+  for (int i = 0; i < 8; i++) cell->child[i] = -1;
+  if (depth > c->max_depth) c->max_depth = depth;
+  if (!is_leaf) {
+    for (int i = 0; i < 8; i++) {
+      int64_t ch = generate_cell(c, key_child(k, i), rb, re, (int32_t)idx);
+      c->cells[idx].child[i] = (int32_t)ch;
+    }
+  }
+  return idx;
+}
+
+typedef struct { uint64_t key; int64_t idx; } kidx;
+static int cmp_kidx(const void *a, const void *b) {
+  const kidx *x = (const kidx *)a, *y = (const kidx *)b;
+  if (x->key < y->key) return -1;
+  if (x->key > y->key) return 1;
+  return (x->idx < y->idx) ? -1 : (x->idx > y->idx);
+}
+
+static double cell_width(const ocell *c, int d) { return c->cmax[d] - c->cmin[d]; }
+
+static void upward(orc_ctx *c, int64_t ci) {
+  ocell *cell = &c->cells[ci];
+  cx *M = c->M + ci * NTERM;
+  for (int i = 0; i < NTERM; i++) { M[i] = cx_make(0, 0); c->L[ci * NTERM + i] = cx_make(0, 0); }
+  if (cell->leaf) {
+    int64_t b = cell->bid;
+    p2m(M, cell->center, c->x + b, c->y + b, c->z + b, c->q + b, cell->nb);
+  } else {
+    for (int i = 0; i < 8; i++) upward(c, cell->child[i]);
+    for (int i = 0; i < 8; i++) {
+      const ocell *ch = &c->cells[cell->child[i]];
+      double dX[3];
+      for (int d = 0; d < 3; d++) dX[d] = cell->center[d] - ch->center[d];
+      m2m(M, c->M + (int64_t)cell->child[i] * NTERM, dX);
+    }
+  }
+}
+
+static void push_pair(int64_t **list, int64_t *cap, uint64_t cnt, int64_t a, int64_t b) {
+  if ((int64_t)(2 * cnt + 2) > *cap) {
+    *cap = *cap ? *cap * 2 : 4096;
+    *list = (int64_t *)realloc(*list, sizeof(int64_t) * (*cap));
+  }
+  (*list)[2 * cnt] = a;
+  (*list)[2 * cnt + 1] = b;
+}
+
+static void p2p_leaves(orc_ctx *c, const ocell *ci, const ocell *cj) {
+  for (int64_t i = ci->bid; i < ci->bid + ci->nb; i++) {
+    for (int64_t j = cj->bid; j < cj->bid + cj->nb; j++) {
+      double dX[3] = {c->x[i] - c->x[j], c->y[i] - c->y[j], c->z[i] - c->z[j]};
+      double R2 = 0;
+      for (int d = 0; d < 3; d++) R2 += dX[d] * dX[d];
+      R2 += 0.0; /* softening EPS2 = 0 */
+      if (R2 != 0) {
+        double invR2 = 1.0 / R2;
+        double invR = c->q[i] * c->q[j] * sqrt(invR2);
+        for (int d = 0; d < 3; d++) dX[d] *= invR2 * invR;
+        c->trg[4 * i + 0] += invR;
+        c->trg[4 * i + 1] -= dX[0];
+        c->trg[4 * i + 2] -= dX[1];
+        c->trg[4 * i + 3] -= dX[2];
+        c->n_p2p_bodypairs++;
+      } else {
+        c->n_p2p_selfpairs++;
+      }
+    }
+  }
+}
+
+static void dtt(orc_ctx *c, int64_t ii, int64_t jj) {
+  const ocell *Ci = &c->cells[ii], *Cj = &c->cells[jj];
+  double d0 = Ci->center[0] - Cj->center[0];
+  double d1 = Ci->center[1] - Cj->center[1];
+  double d2 = Ci->center[2] - Cj->center[2];
+  double R2 = 0;
+  R2 += d0 * d0;
+  R2 += d1 * d1;
+  R2 += d2 * d2;
+  double Ri = 0, Rj = 0;
+  for (int d = 0; d < 3; d++) {
+    double wi = cell_width(Ci, d), wj = cell_width(Cj, d);
+    Ri = (Ri < wi) ? wi : Ri;
+    Rj = (Rj < wj) ? wj : Rj;
+  }
+  Ri = (Ri / 2 * 1.00001) / c->theta;
+  Rj = (Rj / 2 * 1.00001) / c->theta;
+  if (R2 > (Ri + Rj) * (Ri + Rj)) {
+    double dX[3] = {d0, d1, d2};
+    m2l(c->L + ii * NTERM, c->M + jj * NTERM, dX);
+    if (c->record) push_pair(&c->m2l_list, &c->n_m2l_cap, c->n_m2l, ii, jj);
+    c->n_m2l++;
+  } else if (Ci->leaf && Cj->leaf) {
+    if (Ci->nb > 0 && Cj->nb > 0) {
+      if (c->record) push_pair(&c->p2p_list, &c->n_p2p_cap, c->n_p2p_leafpairs, ii, jj);
+      c->n_p2p_leafpairs++;
+      p2p_leaves(c, Ci, Cj);
+    }
+  } else if (Cj->leaf) {
+    for (int i = 0; i < 8; i++) dtt(c, c->cells[ii].child[i], jj);
+  } else if (Ci->leaf) {
+    for (int j = 0; j < 8; j++) dtt(c, ii, c->cells[jj].child[j]);
+  } else if (Ri >= Rj) {
+    for (int i = 0; i < 8; i++) dtt(c, c->cells[ii].child[i], jj);
+  } else {
+    for (int j = 0; j < 8; j++) dtt(c, ii, c->cells[jj].child[j]);
+  }
+}
+
+static void downward(orc_ctx *c, int64_t ci) {
+  ocell *cell = &c->cells[ci];
+  if (cell->parent >= 0) {
+    const ocell *par = &c->cells[cell->parent];
+    double dX[3];
+    for (int d = 0; d < 3; d++) dX[d] = cell->center[d] - par->center[d];
+    l2l(c->L + ci * NTERM, c->L + (int64_t)cell->parent * NTERM, dX);
+  }
+  if (cell->leaf) {
+    for (int64_t b = cell->bid; b < cell->bid + cell->nb; b++) {
+      double bx[3] = {c->x[b], c->y[b], c->z[b]};
+      l2p(c->trg + 4 * b, bx, c->q[b], c->L + ci * NTERM, cell->center);
+    }
+  } else {
+    for (int i = 0; i < 8; i++) downward(c, cell->child[i]);
+  }
+}
+
+orc_ctx *orc_run(const double *x, const double *y, const double *z, const double *q, int64_t n,
+                 int ncrit, double theta, int phases, int record_lists) {
+  if (n <= 0 || ncrit <= 0 || !(theta > 0)) return NULL;
+  orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+  c->n = n;
+  c->ncrit = ncrit;
+  c->theta = theta;
+  c->record = record_lists;
+  orc_bbox(x, y, z, n, c->rmin, c->rmax);
+  uint64_t *k0 = (uint64_t *)malloc(sizeof(uint64_t) * n);
+  orc_keys(x, y, z, n, c->rmin, c->rmax, k0);
+  kidx *ki = (kidx *)malloc(sizeof(kidx) * n);
+  for (int64_t i = 0; i < n; i++) { ki[i].key = k0[i]; ki[i].idx = i; }
+  qsort(ki, (size_t)n, sizeof(kidx), cmp_kidx);
+  c->keys = k0;
+  c->perm = (int64_t *)malloc(sizeof(int64_t) * n);
+  c->x = (double *)malloc(sizeof(double) * n);
+  c->y = (double *)malloc(sizeof(double) * n);
+  c->z = (double *)malloc(sizeof(double) * n);
+  c->q = (double *)malloc(sizeof(double) * n);
+  c->trg = (double *)calloc((size_t)n * 4, sizeof(double));
+  for (int64_t i = 0; i < n; i++) {
+    int64_t o = ki[i].idx;
+    c->keys[i] = ki[i].key;
+    c->perm[i] = o;
+    c->x[i] = x[o]; c->y[i] = y[o]; c->z[i] = z[o]; c->q[i] = q[o];
+  }
+  free(ki);
+  generate_cell(c, 0, 0, n, -1);
+  c->M = (cx *)calloc((size_t)c->ncells * NTERM, sizeof(cx));
+  c->L = (cx *)calloc((size_t)c->ncells * NTERM, sizeof(cx));
+  if (phases >= 1) upward(c, 0);
+  if (phases >= 2) dtt(c, 0, 0);
+  if (phases >= 3) downward(c, 0);
+  return c;
+}
+
+void orc_free(orc_ctx *c) {
+  if (!c) return;
+  free(c->keys); free(c->perm); free(c->x); free(c->y); free(c->z); free(c->q); free(c->trg);
+  free(c->cells); free(c->M); free(c->L); free(c->m2l_list); free(c->p2p_list);
+  free(c);
+}
+
+void orc_region(const orc_ctx *c, double *rmin, double *rmax) {
+  for (int d = 0; d < 3; d++) { rmin[d] = c->rmin[d]; rmax[d] = c->rmax[d]; }
+}
+int64_t orc_ncells(const orc_ctx *c) { return c->ncells; }
+int orc_max_depth(const orc_ctx *c) { return c->max_depth; }
+
+void orc_sorted(const orc_ctx *c, uint64_t *keys, int64_t *perm) {
+  if (keys) memcpy(keys, c->keys, sizeof(uint64_t) * c->n);
+  if (perm) memcpy(perm, c->perm, sizeof(int64_t) * c->n);
+}
+
+void orc_cells(const orc_ctx *c, uint64_t *keys, int64_t *nb, int64_t *bid, int32_t *leaf,
+               double *center, double *width) {
+  for (int64_t i = 0; i < c->ncells; i++) {
+    const ocell *e = &c->cells[i];
+    if (keys) keys[i] = e->key;
+    if (nb) nb[i] = e->nb;
+    if (bid) bid[i] = e->bid;
+    if (leaf) leaf[i] = e->leaf;
+    for (int d = 0; d < 3; d++) {
+      if (center) center[3 * i + d] = e->center[d];
+      if (width) width[3 * i + d] = cell_width(e, d);
+    }
+  }
+}
+
+void orc_expansions(const orc_ctx *c, double *M, double *L) {
+  size_t bytes = sizeof(cx) * (size_t)c->ncells * NTERM;
+  if (M) memcpy(M, c->M, bytes);
+  if (L) memcpy(L, c->L, bytes);
+}
+
+/* fields in the caller's original body order */
+void orc_fields(const orc_ctx *c, double *trg4) {
+  for (int64_t i = 0; i < c->n; i++) {
+    int64_t o = c->perm[i];
+    for (int d = 0; d < 4; d++) trg4[4 * o + d] = c->trg[4 * i + d];
+  }
+}
+
+void orc_counts(const orc_ctx *c, uint64_t *out4) {
+  out4[0] = c->n_m2l;
+  out4[1] = c->n_p2p_leafpairs;
+  out4[2] = c->n_p2p_bodypairs;
+  out4[3] = c->n_p2p_selfpairs;
+}
+
+/* recorded lists as (target cell index, source cell index) pairs */
+void orc_lists(const orc_ctx *c, int64_t *m2l, int64_t *p2p) {
+  if (m2l && c->m2l_list) memcpy(m2l, c->m2l_list, sizeof(int64_t) * 2 * c->n_m2l);
+  if (p2p && c->p2p_list) memcpy(p2p, c->p2p_list, sizeof(int64_t) * 2 * c->n_p2p_leafpairs);
+}
+
+/* FP64 direct sum on a target subset, normalised by the target charge as the
+ * reference's CheckResult does (tapas_exafmm.cxx:338-367). */
+void orc_direct(const double *x, const double *y, const double *z, const double *q, int64_t n,
+                const int64_t *targets, int64_t nt, double *trg4) {
+  for (int64_t t = 0; t < nt; t++) {
+    int64_t i = targets[t];
+    double acc[4] = {0, 0, 0, 0};
+    for (int64_t j = 0; j < n; j++) {
+      double dX[3] = {x[i] - x[j], y[i] - y[j], z[i] - z[j]};
+      double R2 = 0;
+      for (int d = 0; d < 3; d++) R2 += dX[d] * dX[d];
+      if (R2 != 0) {
+        double invR2 = 1.0 / R2;
+        double invR = q[i] * q[j] * sqrt(invR2);
+        for (int d = 0; d < 3; d++) dX[d] *= invR2 * invR;
+        acc[0] += invR;
+        acc[1] -= dX[0];
+        acc[2] -= dX[1];
+        acc[3] -= dX[2];
+      }
+    }
+    for (int d = 0; d < 4; d++) trg4[4 * t + d] = acc[d] / q[i];
+  }
+}
+
+/* rel-L2 error of potential and acceleration (verify.h:18-92) */
+void orc_rel_l2(const double *trg, const double *ref, int64_t n, double *err2) {
+  double pd = 0, pn = 0, ad = 0, an = 0;
+  for (int64_t i = 0; i < n; i++) {
+    double dp = trg[4 * i] - ref[4 * i];
+    pd += dp * dp;
+    pn += ref[4 * i] * ref[4 * i];
+    double dx = trg[4 * i + 1] - ref[4 * i + 1];
+    double dy = trg[4 * i + 2] - ref[4 * i + 2];
+    double dz = trg[4 * i + 3] - ref[4 * i + 3];
+    ad += dx * dx + dy * dy + dz * dz;
+    an += ref[4 * i + 1] * ref[4 * i + 1] + ref[4 * i + 2] * ref[4 * i + 2] +
+          ref[4 * i + 3] * ref[4 * i + 3];
+  }
+  err2[0] = sqrt(pd / pn);
+  err2[1] = sqrt(ad / an);
+}
