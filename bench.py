@@ -1,0 +1,164 @@
+"""Benchmark of the MI355X TapasFMM path.
+
+One step = one full FMM evaluation (tree build + upward + traversal with M2L
+and P2P + downward) of the workload's bodies, which are resident in HBM
+before the timed region starts.  Prints one JSON line (rank 0).
+
+Workload (default): 4,194,304 bodies from a Plummer sphere (a = 1, radii
+truncated at 20a, isotropic), equal charges 1/N, expansion order P = 10,
+ncrit = 64, theta = 0.4.  The error of the evaluated fields is measured
+against the FP64 direct sum on 4096 random targets (outside the timed
+region).  The CPU baseline is the reference's own pipeline (oracle/_ref,
+1 MPI rank) timed on a bounded sample of the same distribution.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tapas-reduce_amd"))
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+P2P_FLOPS_PER_PAIR = 20
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=4194304)
+    ap.add_argument("--dist", default="plummer")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--ncrit", type=int, default=64)
+    ap.add_argument("--theta", type=float, default=0.4)
+    ap.add_argument("--targets", type=int, default=4096)
+    ap.add_argument("--cpu-sample", type=int, default=131072)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(args):
+    """Reference pipeline (oracle/_ref/tapas_ref) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import datasets
+    exe = oracle.ref_binary(count=False)
+    if exe is None:
+        return None
+    n = args.cpu_sample
+    x, y, z, q = datasets.make(args.dist, n, seed=1)
+    t0 = time.time()
+    r = oracle.run_ref(x, y, z, q, args.ncrit, args.theta, count=False, timeout=600)
+    wall = time.time() - t0
+    ph = r["times"]
+    t = ph["tree"] + ph["upward"] + ph["traverse"] + ph["downward"]
+    return {"value": n / t, "unit": "bodies/s", "cores": 1, "kind": "reference",
+            "sample": "%d-body %s (same generator and parameters), reference TapasFMM "
+                      "pipeline, 1 MPI rank; phases tree %.2fs upward %.2fs traverse %.2fs "
+                      "downward %.2fs (upward includes the reference's double upward pass "
+                      "on one rank); process wall %.1fs" % (n, args.dist, ph["tree"],
+                                                             ph["upward"], ph["traverse"],
+                                                             ph["downward"], wall)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU; this build has no multi-GPU path (--gpus %d refused)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import datasets
+    import tfmm
+
+    precision = tfmm.FP64 if args.precision == "fp64" else tfmm.MIXED
+    x, y, z, q = datasets.make(args.dist, args.n, seed=0)
+    dev = torch.device("cuda:0")
+    tx = torch.from_numpy(x).to(dev)
+    ty = torch.from_numpy(y).to(dev)
+    tz = torch.from_numpy(z).to(dev)
+    tq = torch.from_numpy(q).to(dev)
+    torch.cuda.synchronize()
+
+    ctx = tfmm.Context(0)
+    ctx.set_bodies_device(args.n, tx.data_ptr(), ty.data_ptr(), tz.data_ptr(), tq.data_ptr())
+    prm = tfmm.make_params(args.ncrit, args.theta, precision)
+    for _ in range(args.warmup):
+        ctx.evaluate(prm)
+    ctx.sync()
+    torch.cuda.synchronize()
+
+    phase = {k: [] for k in ("ms_tree", "ms_upward", "ms_traverse", "ms_downward", "ms_total",
+                             "ms_p2p", "ms_m2l", "ms_dtt")}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.evaluate(prm)
+        st = ctx.stats()  # waits for the step; phase times from HIP events
+        for k in phase:
+            phase[k].append(st[k])
+    ctx.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_step = elapsed * 1e3 / args.steps
+
+    # accuracy against the FP64 direct sum (untimed)
+    rng = np.random.default_rng(123)
+    targets = np.sort(rng.choice(args.n, size=min(args.targets, args.n), replace=False))
+    fields = ctx.fields()
+    direct = ctx.direct(targets)
+    f = fields[targets]
+
+    def rel(a, b):
+        return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+    err_pot = rel(f[:, 0], direct[:, 0])
+    err_acc = rel(f[:, 1:], direct[:, 1:])
+
+    pairs = st["n_p2p_calls"] - st["n_p2p_self"]
+    p2p_ms = float(np.mean(phase["ms_p2p"]))
+    achieved = P2P_FLOPS_PER_PAIR * pairs / (p2p_ms * 1e-3) / 1e12
+    peak = PEAK_FP32_TFLOPS if precision == tfmm.MIXED else PEAK_FP64_TFLOPS
+
+    cpu = None if args.no_cpu else cpu_baseline(args)
+    out = {
+        "metric": "FMM bodies/s at rel-L2 err <=1e-4, 4M Plummer; P2P % FP32 peak",
+        "value": args.n / (ms_step * 1e-3),
+        "unit": "bodies/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == tfmm.FP64 else "f32 P2P/M2L + f64 tree/L2L/L2P",
+        "data": "synthetic",
+        "config": {"workload": "%s N=%d P=10 ncrit=%d theta=%g %s" % (
+            args.dist, args.n, args.ncrit, args.theta, args.precision),
+            "n_bodies": args.n, "distribution": args.dist, "expansion": 10,
+            "ncrit": args.ncrit, "theta": args.theta, "precision": args.precision},
+        "error": {"rel_l2_pot": err_pot, "rel_l2_acc": err_acc, "targets": len(targets),
+                  "against": "FP64 direct sum"},
+        "phases_ms": {k: float(np.mean(v)) for k, v in phase.items()},
+        "counts": {"cells": st["ncells"], "leaves": st["nleaves"], "max_depth": st["max_depth"],
+                   "m2l_pairs": st["n_m2l_exec"], "p2p_leaf_pairs": st["n_p2p_leafpairs"],
+                   "p2p_body_pairs": pairs},
+        "roofline": {"kernel": "p2p", "bound": "valu", "achieved": achieved, "peak": peak,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "flops_per_pair": P2P_FLOPS_PER_PAIR,
+                     "p2p_pct_fp32_peak": 100.0 * achieved / PEAK_FP32_TFLOPS},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,145 @@
+/*
+ * tfmm.h -- C ABI of the MI355X (gfx950) TapasFMM library (libtfmm.so).
+ *
+ * Plain C types only: pointers, sizes and POD structs; no exceptions, no
+ * aborts across the boundary.  Every call returns TFMM_OK (0) or a negative
+ * status; tfmm_last_error() gives the message of the last failure.
+ *
+ * Each entry point replaces a piece of the reference's path (paths relative to
+ * the reference repository; TP = cpp/include/tapas, EX =
+ * sample/exafmm-dev-13274dd4ac68):
+ *
+ *   tfmm_set_bodies*     the body copy Partition makes of the caller's array
+ *                        (TP/hot/buildtree.h:72-73, TP/hot.h:1349-1352)
+ *   tfmm_build_tree      TapasFMM::Partition -> SamplingOctree::Build
+ *                        (TP/hot.h:1240-1284, TP/hot/buildtree.h:329-459)
+ *   tfmm_upward          TapasFMM::Map(FMM_Upward(), *root, theta)
+ *                        (EX/examples/tapas_exafmm.cxx:586, :87-122)
+ *   tfmm_traverse        TapasFMM::Map(FMM_DTT(), Product(*root, *root), mutual,
+ *                        nspawn, theta) (tapas_exafmm.cxx:606, :209-286); on the
+ *                        reference's GPU build the GPUMapper records the P2P
+ *                        products and launches them at the end
+ *                        (TP/hot/mapper.h:452-690, TP/vectormap_cuda.h:911-1004)
+ *   tfmm_downward        TapasFMM::Map(FMM_Downward(), *root)
+ *                        (tapas_exafmm.cxx:627, :124-140)
+ *   tfmm_get_fields      CopyBackResult: root->local_body_attr(i)
+ *                        (tapas_exafmm.cxx:388-398, TP/hot.h:1037-1050)
+ *   tfmm_get_sorted /    root->data().local_body_keys_, ht_ (TP/hot/shared_data.h:31-68)
+ *   tfmm_get_cells
+ *   tfmm_stats           RankCSV columns total/upward/traverse/downward/tree and
+ *                        the -DCOUNT numP2P/numM2L counters
+ *                        (tapas_exafmm.cxx:651-677, :39-50)
+ *   tfmm_direct          CheckResult's direct sum (tapas_exafmm.cxx:288-383)
+ */
+#ifndef TFMM_H_
+#define TFMM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TFMM_OK 0
+#define TFMM_EINVAL (-22) /* invalid argument or parameters */
+#define TFMM_ENOMEM (-12) /* device allocation failed */
+#define TFMM_EHIP (-5)    /* HIP runtime error */
+#define TFMM_ESTATE (-77) /* call out of order (e.g. upward before build_tree) */
+
+#define TFMM_P 10      /* expansion order (EXPANSION=10) */
+#define TFMM_NTERM 55  /* P*(P+1)/2 coefficients per cell */
+
+#define TFMM_FP64 0  /* FP64 throughout */
+#define TFMM_MIXED 1 /* FP32 P2P with compensated sums, FP32 batched M2L; FP64 tree/L2L/L2P */
+
+typedef struct tfmm_ctx tfmm_ctx;
+
+/* FMM parameters (the reference's -c/--ncrit, -t/--theta, -m/--mutual of
+ * EX/include/args.h:101-158). */
+typedef struct {
+  int32_t ncrit;     /* a cell is a leaf iff it holds <= ncrit bodies */
+  int32_t mutual;    /* must be 0: mutual interaction is not supported (as on the
+                        reference's CUDA path, tapas_exafmm.cxx:507-515) */
+  int32_t precision; /* TFMM_FP64 or TFMM_MIXED */
+  int32_t reserved;
+  double theta;      /* multipole acceptance parameter, 0 < theta */
+} tfmm_params;
+
+/* Layout of the caller's array-of-structs bodies (the reference's Body is
+ * 96 B with X at 0 and SRC at 24, EX/include/types.h:62-74). */
+typedef struct {
+  int64_t n;
+  size_t stride;    /* bytes between consecutive bodies */
+  size_t off_x;     /* byte offset of double X[3] */
+  size_t off_q;     /* byte offset of double SRC */
+} tfmm_body_desc;
+
+typedef struct {
+  int64_t n;
+  int64_t ncells;
+  int64_t nleaves;
+  int32_t nlevels;    /* number of non-empty levels (max depth + 1) */
+  int32_t max_depth;
+  double region_min[3];
+  double region_max[3];
+  uint64_t n_m2l;            /* all M2L pairs the traversal produced (numM2L) */
+  uint64_t n_m2l_exec;       /* M2L pairs executed (both cells non-empty) */
+  uint64_t n_p2p_leafpairs;  /* P2P leaf pairs (both non-empty) */
+  uint64_t n_p2p_calls;      /* body pairs visited (numP2P: includes r == 0 self pairs) */
+  uint64_t n_p2p_self;       /* r == 0 self pairs among them */
+  /* milliseconds of the last evaluation, measured with HIP events on the
+   * context stream; kernel entries are summed over their launches */
+  double ms_tree, ms_upward, ms_traverse, ms_downward, ms_total;
+  double ms_p2p, ms_m2l, ms_dtt;
+} tfmm_stats_t;
+
+int tfmm_create(int hip_device, tfmm_ctx **out);
+void tfmm_destroy(tfmm_ctx *ctx);
+const char *tfmm_last_error(const tfmm_ctx *ctx);
+/* the hipStream_t all work of the context is queued on */
+void *tfmm_stream(tfmm_ctx *ctx);
+int tfmm_sync(tfmm_ctx *ctx);
+
+/* Copy n host bodies (AoS as described) into the context. */
+int tfmm_set_bodies(tfmm_ctx *ctx, const void *host_bodies, const tfmm_body_desc *desc);
+/* Use n device-resident bodies (SoA FP64 arrays in HBM) without a copy; the
+ * arrays must stay valid until the next set_bodies call or destroy. */
+int tfmm_set_bodies_device(tfmm_ctx *ctx, int64_t n, const double *x, const double *y,
+                           const double *z, const double *q);
+
+int tfmm_build_tree(tfmm_ctx *ctx, const tfmm_params *params);
+int tfmm_upward(tfmm_ctx *ctx);
+int tfmm_traverse(tfmm_ctx *ctx);
+int tfmm_downward(tfmm_ctx *ctx);
+/* build_tree + upward + traverse + downward */
+int tfmm_evaluate(tfmm_ctx *ctx, const tfmm_params *params);
+
+/* Fields (pot, ax, ay, az) per body, 4 doubles each.  order 0: the caller's
+ * body order; 1: sorted (Morton) order.  dst is a host pointer, or a device
+ * pointer when on_device != 0. */
+int tfmm_get_fields(tfmm_ctx *ctx, double *dst, int order, int on_device);
+
+int tfmm_get_stats(tfmm_ctx *ctx, tfmm_stats_t *out);
+/* Sorted body keys and, per sorted body, its index in the caller's order. */
+int tfmm_get_sorted(tfmm_ctx *ctx, uint64_t *keys, int32_t *perm);
+/* Per cell (breadth-first order): key, first sorted body, body count,
+ * leaf flag, centre (3 doubles) and width (max over the dimensions).  Any
+ * pointer may be null. */
+int tfmm_get_cells(tfmm_ctx *ctx, uint64_t *key, int32_t *beg, int32_t *nb, int32_t *leaf,
+                   double *center, double *width);
+/* M and L of every cell: ncells * 55 complex (re, im) doubles each. */
+int tfmm_get_expansions(tfmm_ctx *ctx, double *M, double *L);
+/* Interaction lists in CSR form over target cells: off has ncells + 1
+ * entries; src the source cell of each entry (sorted by target, then source). */
+int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p2p_off,
+                   int32_t *p2p_src);
+
+/* FP64 direct sum on the current bodies for the given target indices (caller
+ * order), normalised by the target charge; out: nt * 4 doubles (host). */
+int tfmm_direct(tfmm_ctx *ctx, const int64_t *targets, int64_t nt, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TFMM_H_ */

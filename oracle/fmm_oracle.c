@@ -740,3 +740,7 @@ void orc_rel_l2(const double *trg, const double *ref, int64_t n, double *err2) {
   err2[0] = sqrt(pd / pn);
   err2[1] = sqrt(ad / an);
 }
+
+/* key algebra entry points (for the known-answer tests of the key functions) */
+uint64_t orc_key_child(uint64_t k, int idx) { return key_child(k, idx); }
+uint64_t orc_key_next(uint64_t k) { return key_next(k); }

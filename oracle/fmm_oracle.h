@@ -22,6 +22,8 @@ void orc_bbox(const double *x, const double *y, const double *z, int64_t n, doub
 uint64_t orc_finest_key(const int anchor[3]);
 void orc_keys(const double *x, const double *y, const double *z, int64_t n, const double *rmin,
               const double *rmax, uint64_t *keys);
+uint64_t orc_key_child(uint64_t k, int idx);
+uint64_t orc_key_next(uint64_t k);
 void orc_cell_region(uint64_t key, const double *rmin, const double *rmax, double *cmin,
                      double *cmax);
 

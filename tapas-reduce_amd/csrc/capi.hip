@@ -1,0 +1,685 @@
+// capi.hip -- the C ABI (include/tfmm.h): context, device buffers and the
+// phase orchestration of one FMM evaluation on one GPU.
+//
+// Phases follow the reference driver (sample/exafmm-dev-13274dd4ac68/examples/
+// tapas_exafmm.cxx:557-641): Partition (tree), Map(FMM_Upward),
+// Map(FMM_DTT, Product(root, root)), Map(FMM_Downward), CopyBackResult.
+// All work is queued on the context's stream; the host only waits where a
+// size decides the next launch (cells per level, frontier per traversal step).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tfmm.h"
+#include "tfmm_device.h"
+#include "tfmm_internal.h"
+#include "tfmm_kernels.h"
+
+using namespace tfmm;
+
+namespace {
+
+enum Ev {
+  EV_BEGIN,
+  EV_TREE,
+  EV_UP,
+  EV_DTT0,
+  EV_DTT1,
+  EV_M2L0,
+  EV_M2L1,
+  EV_P2P0,
+  EV_P2P1,
+  EV_TRAV,
+  EV_DOWN,
+  EV_COUNT
+};
+
+}  // namespace
+
+struct tfmm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // bodies (caller order)
+  int64_t n = 0;
+  const double *bx = nullptr, *by = nullptr, *bz = nullptr, *bq = nullptr;
+  DevBuf own_x, own_y, own_z, own_q;
+
+  tfmm_params prm{};
+  int stage = 0;  // 0 none, 1 tree, 2 upward, 3 traverse, 4 downward
+
+  // tree
+  DevBuf keys, keys_alt, idx, idx_alt, sort_tmp, sx, sy, sz, sq, perm, region, bbox_part;
+  DevBuf c_key, c_beg, c_nb, c_parent, c_child, c_depth, c_center, c_width;
+  size_t cell_cap = 0;
+  DevBuf flag, slot, scan_tmp;
+  std::vector<int32_t> lvl;
+  int64_t ncells = 0, nleaves = 0;
+  bool keys_in_alt = false;
+  double h_region[6] = {0, 0, 0, 0, 0, 0};
+
+  // expansions
+  DevBuf M, L;
+
+  // traversal
+  DevBuf front_a, front_b, m2l_pairs, p2p_pairs, cnt, lkeys, lkeys_alt, lidx, lidx_alt;
+  DevBuf m2l_off, m2l_src, p2p_off, p2p_src, pair_count;
+  uint32_t m2l_cap = 0, p2p_cap = 0;
+  uint32_t n_m2l_exec = 0, n_p2p = 0;
+  uint64_t n_m2l_all = 0, n_p2p_calls = 0, n_p2p_self = 0;
+  uint32_t dtt_steps = 0;
+
+  // fields
+  DevBuf acc, trg, out;
+
+  // host staging (pinned)
+  uint32_t *h_u32 = nullptr;
+  unsigned long long *h_u64 = nullptr;
+
+  hipEvent_t ev[EV_COUNT] = {};
+  float ms_dtt = 0, ms_m2l = 0, ms_p2p = 0;
+  bool timed_phases = false;
+};
+
+namespace {
+
+int fail(tfmm_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hipfail(tfmm_ctx *c, hipError_t e, const char *where) {
+  return fail(c, e == hipErrorOutOfMemory ? TFMM_ENOMEM : TFMM_EHIP,
+              std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define TRY(expr)                                     \
+  do {                                                \
+    hipError_t _e = (expr);                           \
+    if (_e != hipSuccess) return hipfail(c, _e, #expr); \
+  } while (0)
+
+hipError_t grow_copy(DevBuf &b, size_t bytes, size_t used, hipStream_t s) {
+  if (bytes <= b.cap) return hipSuccess;
+  size_t want = b.cap ? b.cap : 256;
+  while (want < bytes) want *= 2;
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) return e;
+  if (b.ptr && used) {
+    e = hipMemcpyAsync(p, b.ptr, used, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+  }
+  if (b.ptr) (void)hipFree(b.ptr);
+  b.ptr = p;
+  b.cap = want;
+  return hipSuccess;
+}
+
+Cells cells_of(tfmm_ctx *c) {
+  Cells k;
+  k.key = dp<uint64_t>(c->c_key);
+  k.beg = dp<int32_t>(c->c_beg);
+  k.nb = dp<int32_t>(c->c_nb);
+  k.parent = dp<int32_t>(c->c_parent);
+  k.child = dp<int32_t>(c->c_child);
+  k.depth = dp<int32_t>(c->c_depth);
+  k.center = dp<double>(c->c_center);
+  k.width = dp<double>(c->c_width);
+  return k;
+}
+
+Bodies sorted_bodies(tfmm_ctx *c) {
+  Bodies b;
+  b.x = dp<double>(c->sx);
+  b.y = dp<double>(c->sy);
+  b.z = dp<double>(c->sz);
+  b.q = dp<double>(c->sq);
+  b.n = c->n;
+  return b;
+}
+
+hipError_t reserve_cells(tfmm_ctx *c, size_t need, size_t used, hipStream_t s) {
+  if (need <= c->cell_cap) return hipSuccess;
+  size_t cap = c->cell_cap ? c->cell_cap : 4096;
+  while (cap < need) cap *= 2;
+  hipError_t e;
+  if ((e = grow_copy(c->c_key, cap * 8, used * 8, s)) != hipSuccess) return e;
+  if ((e = grow_copy(c->c_beg, cap * 4, used * 4, s)) != hipSuccess) return e;
+  if ((e = grow_copy(c->c_nb, cap * 4, used * 4, s)) != hipSuccess) return e;
+  if ((e = grow_copy(c->c_parent, cap * 4, used * 4, s)) != hipSuccess) return e;
+  if ((e = grow_copy(c->c_child, cap * 4, used * 4, s)) != hipSuccess) return e;
+  if ((e = grow_copy(c->c_depth, cap * 4, used * 4, s)) != hipSuccess) return e;
+  if ((e = grow_copy(c->c_center, cap * 24, used * 24, s)) != hipSuccess) return e;
+  if ((e = grow_copy(c->c_width, cap * 8, used * 8, s)) != hipSuccess) return e;
+  c->cell_cap = cap;
+  return hipSuccess;
+}
+
+int bits_for(int64_t v) {
+  int b = 1;
+  while (b < 62 && (((int64_t)1 << b) <= v)) b++;
+  return b;
+}
+
+int check_params(tfmm_ctx *c, const tfmm_params *p) {
+  if (!p) return fail(c, TFMM_EINVAL, "params is null");
+  if (p->ncrit < 1) return fail(c, TFMM_EINVAL, "ncrit must be >= 1");
+  if (!(p->theta > 0) || !(p->theta < 1e300))
+    return fail(c, TFMM_EINVAL, "theta must be a positive finite number");
+  if (p->mutual != 0)
+    return fail(c, TFMM_EINVAL, "mutual interaction is not supported (mutual must be 0)");
+  if (p->precision != TFMM_FP64 && p->precision != TFMM_MIXED)
+    return fail(c, TFMM_EINVAL, "precision must be TFMM_FP64 or TFMM_MIXED");
+  return TFMM_OK;
+}
+
+int do_build_tree(tfmm_ctx *c) {
+  hipStream_t s = c->stream;
+  const int64_t n = c->n;
+  TRY(dev_reserve(c->keys, n * 8));
+  TRY(dev_reserve(c->keys_alt, n * 8));
+  TRY(dev_reserve(c->idx, n * 4));
+  TRY(dev_reserve(c->idx_alt, n * 4));
+  TRY(dev_reserve(c->sort_tmp, radix_tmp_bytes(n)));
+  TRY(dev_reserve(c->sx, n * 8));
+  TRY(dev_reserve(c->sy, n * 8));
+  TRY(dev_reserve(c->sz, n * 8));
+  TRY(dev_reserve(c->sq, n * 8));
+  TRY(dev_reserve(c->perm, n * 4));
+  TRY(dev_reserve(c->region, 6 * 8));
+  const int nparts = (int)std::min<int64_t>(1024, (n + 255) / 256);
+  TRY(dev_reserve(c->bbox_part, 2 * 8 * (size_t)nparts));
+
+  TRY(launch_bbox(c->bx, c->by, c->bz, n, dp<double>(c->bbox_part), nparts,
+                  dp<double>(c->region), s));
+  TRY(launch_keys(c->bx, c->by, c->bz, n, dp<double>(c->region), dp<uint64_t>(c->keys),
+                  dp<uint32_t>(c->idx), s));
+  bool alt = false;
+  TRY(radix_sort_pairs(dp<uint64_t>(c->keys), dp<uint32_t>(c->idx), dp<uint64_t>(c->keys_alt),
+                       dp<uint32_t>(c->idx_alt), n, DEPTHBITS, DEPTHBITS + 3 * MAXDEPTH,
+                       c->sort_tmp.ptr, s, &alt));
+  c->keys_in_alt = alt;
+  const uint64_t *skeys = dp<uint64_t>(alt ? c->keys_alt : c->keys);
+  const uint32_t *order = dp<uint32_t>(alt ? c->idx_alt : c->idx);
+  TRY(launch_permute(order, c->bx, c->by, c->bz, c->bq, n, dp<double>(c->sx), dp<double>(c->sy),
+                     dp<double>(c->sz), dp<double>(c->sq), dp<int32_t>(c->perm), s));
+
+  // level-synchronous cell build
+  TRY(reserve_cells(c, 4096, 0, s));
+  TRY(launch_root(cells_of(c), n, dp<double>(c->region), s));
+  c->lvl.assign(1, 0);
+  int32_t lbeg = 0, lcnt = 1;
+  int64_t nleaves = 0;
+  while (true) {
+    TRY(dev_reserve(c->flag, (size_t)lcnt * 4));
+    TRY(dev_reserve(c->slot, (size_t)lcnt * 4));
+    TRY(dev_reserve(c->scan_tmp, scan_tmp_bytes(lcnt)));
+    TRY(launch_split_flags(cells_of(c), lbeg, lcnt, c->prm.ncrit, dp<uint32_t>(c->flag), s));
+    TRY(scan_exclusive_u32(dp<uint32_t>(c->flag), dp<uint32_t>(c->slot), lcnt,
+                           c->scan_tmp.ptr, s));
+    TRY(hipMemcpyAsync(c->h_u32, dp<uint32_t>(c->flag) + (lcnt - 1), 4, hipMemcpyDeviceToHost, s));
+    TRY(hipMemcpyAsync(c->h_u32 + 1, dp<uint32_t>(c->slot) + (lcnt - 1), 4,
+                       hipMemcpyDeviceToHost, s));
+    TRY(hipStreamSynchronize(s));
+    const int64_t splits = (int64_t)c->h_u32[0] + c->h_u32[1];
+    nleaves += lcnt - splits;
+    c->lvl.push_back(lbeg + lcnt);
+    if (splits == 0) break;
+    const int64_t next_beg = (int64_t)lbeg + lcnt;
+    const int64_t need = next_beg + 8 * splits;
+    if (need > INT32_MAX) return fail(c, TFMM_EINVAL, "too many cells");
+    TRY(reserve_cells(c, (size_t)need, (size_t)next_beg, s));
+    TRY(launch_children(cells_of(c), lbeg, lcnt, dp<uint32_t>(c->flag), dp<uint32_t>(c->slot),
+                        (int32_t)next_beg, skeys, dp<double>(c->region), s));
+    lbeg = (int32_t)next_beg;
+    lcnt = (int32_t)(8 * splits);
+  }
+  c->ncells = c->lvl.back();
+  c->nleaves = nleaves;
+  TRY(hipMemcpyAsync(c->h_region, c->region.ptr, 48, hipMemcpyDeviceToHost, s));
+  return TFMM_OK;
+}
+
+int do_upward(tfmm_ctx *c) {
+  hipStream_t s = c->stream;
+  const size_t bytes = (size_t)c->ncells * NTERM * 16;
+  TRY(dev_reserve(c->M, bytes));
+  TRY(dev_reserve(c->L, bytes));
+  TRY(hipMemsetAsync(c->M.ptr, 0, bytes, s));
+  const Cells k = cells_of(c);
+  TRY(launch_p2m(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->M), s));
+  const int nl = (int)c->lvl.size() - 1;
+  for (int d = nl - 2; d >= 0; d--)
+    TRY(launch_m2m(k, c->lvl[d], c->lvl[d + 1] - c->lvl[d], dp<double>(c->M), s));
+  return TFMM_OK;
+}
+
+int sort_list(tfmm_ctx *c, DevBuf &pairs, uint32_t npairs, DevBuf &off, DevBuf &src) {
+  hipStream_t s = c->stream;
+  const int sbits = bits_for(c->ncells);
+  TRY(dev_reserve(c->lkeys, (size_t)npairs * 8 + 8));
+  TRY(dev_reserve(c->lkeys_alt, (size_t)npairs * 8 + 8));
+  TRY(dev_reserve(c->lidx, (size_t)npairs * 4 + 4));
+  TRY(dev_reserve(c->lidx_alt, (size_t)npairs * 4 + 4));
+  TRY(dev_reserve(c->sort_tmp, radix_tmp_bytes(npairs)));
+  TRY(dev_reserve(off, (size_t)(c->ncells + 1) * 4));
+  TRY(dev_reserve(src, (size_t)npairs * 4 + 4));
+  TRY(launch_pack_pairs(dp<int2>(pairs), npairs, sbits, dp<uint64_t>(c->lkeys),
+                        dp<uint32_t>(c->lidx), s));
+  bool alt = false;
+  TRY(radix_sort_pairs(dp<uint64_t>(c->lkeys), dp<uint32_t>(c->lidx), dp<uint64_t>(c->lkeys_alt),
+                       dp<uint32_t>(c->lidx_alt), npairs, 0, 2 * sbits, c->sort_tmp.ptr, s,
+                       &alt));
+  const uint64_t *sk = dp<uint64_t>(alt ? c->lkeys_alt : c->lkeys);
+  TRY(launch_csr_offsets(sk, npairs, sbits, (int32_t)c->ncells, dp<int32_t>(off), s));
+  TRY(launch_unpack_pairs(sk, npairs, sbits, dp<int32_t>(src), s));
+  return TFMM_OK;
+}
+
+int do_dtt(tfmm_ctx *c) {
+  hipStream_t s = c->stream;
+  const Cells k = cells_of(c);
+  TRY(dev_reserve(c->cnt, 64));
+  if (c->m2l_cap == 0) c->m2l_cap = (uint32_t)std::min<int64_t>(UINT32_MAX / 2, std::max<int64_t>(1 << 16, 200 * c->ncells));
+  if (c->p2p_cap == 0) c->p2p_cap = (uint32_t)std::min<int64_t>(UINT32_MAX / 2, std::max<int64_t>(1 << 16, 40 * c->nleaves));
+  for (int attempt = 0; attempt < 3; attempt++) {
+    TRY(dev_reserve(c->m2l_pairs, (size_t)c->m2l_cap * 8));
+    TRY(dev_reserve(c->p2p_pairs, (size_t)c->p2p_cap * 8));
+    TRY(dev_reserve(c->front_a, 64));
+    TRY(hipMemsetAsync(c->cnt.ptr, 0, 64, s));
+    c->h_u32[8] = 0;  // the root pair (0, 0), staged in pinned memory
+    c->h_u32[9] = 0;
+    TRY(hipMemcpyAsync(c->front_a.ptr, c->h_u32 + 8, 8, hipMemcpyHostToDevice, s));
+    uint32_t nin = 1;
+    c->dtt_steps = 0;
+    while (nin > 0) {
+      const size_t out_need = (size_t)nin * 8 * 8;
+      TRY(dev_reserve(c->front_b, out_need));
+      TRY(hipMemsetAsync(c->cnt.ptr, 0, 4, s));
+      TRY(launch_dtt_step(dp<int2>(c->front_a), nullptr, nin, dp<int2>(c->front_b),
+                          (uint32_t)(out_need / 8), dp<int2>(c->m2l_pairs), c->m2l_cap,
+                          dp<int2>(c->p2p_pairs), c->p2p_cap, dp<uint32_t>(c->cnt), k,
+                          c->prm.theta, s));
+      TRY(hipMemcpyAsync(c->h_u32, c->cnt.ptr, 16, hipMemcpyDeviceToHost, s));
+      TRY(hipStreamSynchronize(s));
+      nin = c->h_u32[0];
+      std::swap(c->front_a, c->front_b);
+      c->dtt_steps++;
+    }
+    const uint32_t nm = c->h_u32[1], np = c->h_u32[2];
+    if (nm <= c->m2l_cap && np <= c->p2p_cap) {
+      c->n_m2l_exec = nm;
+      c->n_p2p = np;
+      c->n_m2l_all = c->h_u32[3];
+      return TFMM_OK;
+    }
+    c->m2l_cap = std::max(c->m2l_cap, nm + nm / 8 + 1024);
+    c->p2p_cap = std::max(c->p2p_cap, np + np / 8 + 1024);
+  }
+  return fail(c, TFMM_EHIP, "traversal lists did not converge");
+}
+
+int do_traverse(tfmm_ctx *c) {
+  hipStream_t s = c->stream;
+  const Cells k = cells_of(c);
+  TRY(hipEventRecord(c->ev[EV_DTT0], s));
+  int rc = do_dtt(c);
+  if (rc) return rc;
+  rc = sort_list(c, c->m2l_pairs, c->n_m2l_exec, c->m2l_off, c->m2l_src);
+  if (rc) return rc;
+  rc = sort_list(c, c->p2p_pairs, c->n_p2p, c->p2p_off, c->p2p_src);
+  if (rc) return rc;
+  TRY(dev_reserve(c->pair_count, 16));
+  TRY(hipMemsetAsync(c->pair_count.ptr, 0, 16, s));
+  TRY(launch_p2p_pair_count(k, dp<int2>(c->p2p_pairs), c->n_p2p,
+                            dp<unsigned long long>(c->pair_count), s));
+  TRY(hipEventRecord(c->ev[EV_DTT1], s));
+
+  const size_t lbytes = (size_t)c->ncells * NTERM * 16;
+  TRY(hipMemsetAsync(c->L.ptr, 0, lbytes, s));
+  TRY(hipEventRecord(c->ev[EV_M2L0], s));
+  Csr ml{dp<int32_t>(c->m2l_off), dp<int32_t>(c->m2l_src)};
+  TRY(launch_m2l_fp64(k, (int32_t)c->ncells, ml, dp<double>(c->M), dp<double>(c->L), s));
+  TRY(hipEventRecord(c->ev[EV_M2L1], s));
+
+  TRY(dev_reserve(c->acc, (size_t)c->n * 32));
+  TRY(hipEventRecord(c->ev[EV_P2P0], s));
+  Csr pl{dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src)};
+  TRY(launch_p2p_fp64(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<double>(c->acc), s));
+  TRY(hipEventRecord(c->ev[EV_P2P1], s));
+  TRY(hipMemcpyAsync(c->h_u64, c->pair_count.ptr, 16, hipMemcpyDeviceToHost, s));
+  return TFMM_OK;
+}
+
+int do_downward(tfmm_ctx *c) {
+  hipStream_t s = c->stream;
+  const Cells k = cells_of(c);
+  const int nl = (int)c->lvl.size() - 1;
+  for (int d = 0; d + 1 < nl; d++)
+    TRY(launch_l2l(k, c->lvl[d], c->lvl[d + 1] - c->lvl[d], dp<double>(c->L), s));
+  TRY(dev_reserve(c->trg, (size_t)c->n * 32));
+  TRY(dev_reserve(c->out, (size_t)c->n * 32));
+  TRY(launch_l2p(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->L), dp<double>(c->acc),
+                 dp<double>(c->trg), s));
+  TRY(launch_unpermute(dp<int32_t>(c->perm), dp<double>(c->trg), c->n, dp<double>(c->out), s));
+  return TFMM_OK;
+}
+
+int need_stage(tfmm_ctx *c, int st, const char *what) {
+  if (!c) return TFMM_EINVAL;
+  if (c->stage < st) return fail(c, TFMM_ESTATE, std::string(what) + " called out of order");
+  return TFMM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tfmm_create(int hip_device, tfmm_ctx **out) {
+  if (!out) return TFMM_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return TFMM_EHIP;
+  if (hip_device < 0 || hip_device >= ndev) return TFMM_EINVAL;
+  if (hipSetDevice(hip_device) != hipSuccess) return TFMM_EHIP;
+  tfmm_ctx *c = new tfmm_ctx();
+  c->device = hip_device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return TFMM_EHIP;
+  }
+  for (int i = 0; i < EV_COUNT; i++) {
+    if (hipEventCreate(&c->ev[i]) != hipSuccess) {
+      tfmm_destroy(c);
+      return TFMM_EHIP;
+    }
+  }
+  if (hipHostMalloc((void **)&c->h_u32, 64, 0) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_u64, 64, 0) != hipSuccess) {
+    tfmm_destroy(c);
+    return TFMM_ENOMEM;
+  }
+  *out = c;
+  return TFMM_OK;
+}
+
+void tfmm_destroy(tfmm_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  DevBuf *bufs[] = {&c->own_x, &c->own_y, &c->own_z, &c->own_q, &c->keys, &c->keys_alt, &c->idx,
+                    &c->idx_alt, &c->sort_tmp, &c->sx, &c->sy, &c->sz, &c->sq, &c->perm,
+                    &c->region, &c->bbox_part, &c->c_key, &c->c_beg, &c->c_nb, &c->c_parent,
+                    &c->c_child, &c->c_depth, &c->c_center, &c->c_width, &c->flag, &c->slot,
+                    &c->scan_tmp, &c->M, &c->L, &c->front_a, &c->front_b, &c->m2l_pairs,
+                    &c->p2p_pairs, &c->cnt, &c->lkeys, &c->lkeys_alt, &c->lidx, &c->lidx_alt,
+                    &c->m2l_off, &c->m2l_src, &c->p2p_off, &c->p2p_src, &c->pair_count, &c->acc,
+                    &c->trg, &c->out};
+  for (DevBuf *b : bufs) dev_release(*b);
+  for (int i = 0; i < EV_COUNT; i++)
+    if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  if (c->h_u32) (void)hipHostFree(c->h_u32);
+  if (c->h_u64) (void)hipHostFree(c->h_u64);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *tfmm_last_error(const tfmm_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+void *tfmm_stream(tfmm_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int tfmm_sync(tfmm_ctx *c) {
+  if (!c) return TFMM_EINVAL;
+  TRY(hipStreamSynchronize(c->stream));
+  return TFMM_OK;
+}
+
+int tfmm_set_bodies(tfmm_ctx *c, const void *host, const tfmm_body_desc *d) {
+  if (!c) return TFMM_EINVAL;
+  if (!d || !host) return fail(c, TFMM_EINVAL, "null bodies or descriptor");
+  if (d->n < 1 || d->n > INT32_MAX) return fail(c, TFMM_EINVAL, "n must be in [1, 2^31)");
+  if (d->stride < 8 || d->off_x + 24 > d->stride || d->off_q + 8 > d->stride)
+    return fail(c, TFMM_EINVAL, "body descriptor offsets exceed the stride");
+  TRY(hipSetDevice(c->device));
+  const int64_t n = d->n;
+  std::vector<double> x(n), y(n), z(n), q(n);
+  const char *base = reinterpret_cast<const char *>(host);
+  for (int64_t i = 0; i < n; i++) {
+    const char *b = base + (size_t)i * d->stride;
+    double X[3], Q;
+    memcpy(X, b + d->off_x, 24);
+    memcpy(&Q, b + d->off_q, 8);
+    if (!std::isfinite(X[0]) || !std::isfinite(X[1]) || !std::isfinite(X[2]) || !std::isfinite(Q))
+      return fail(c, TFMM_EINVAL, "non-finite body coordinate or charge");
+    x[i] = X[0];
+    y[i] = X[1];
+    z[i] = X[2];
+    q[i] = Q;
+  }
+  TRY(dev_reserve(c->own_x, n * 8));
+  TRY(dev_reserve(c->own_y, n * 8));
+  TRY(dev_reserve(c->own_z, n * 8));
+  TRY(dev_reserve(c->own_q, n * 8));
+  TRY(hipMemcpyAsync(c->own_x.ptr, x.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+  TRY(hipMemcpyAsync(c->own_y.ptr, y.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+  TRY(hipMemcpyAsync(c->own_z.ptr, z.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+  TRY(hipMemcpyAsync(c->own_q.ptr, q.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+  TRY(hipStreamSynchronize(c->stream));
+  c->n = n;
+  c->bx = dp<double>(c->own_x);
+  c->by = dp<double>(c->own_y);
+  c->bz = dp<double>(c->own_z);
+  c->bq = dp<double>(c->own_q);
+  c->stage = 0;
+  return TFMM_OK;
+}
+
+int tfmm_set_bodies_device(tfmm_ctx *c, int64_t n, const double *x, const double *y,
+                           const double *z, const double *q) {
+  if (!c) return TFMM_EINVAL;
+  if (n < 1 || n > INT32_MAX) return fail(c, TFMM_EINVAL, "n must be in [1, 2^31)");
+  if (!x || !y || !z || !q) return fail(c, TFMM_EINVAL, "null body array");
+  c->n = n;
+  c->bx = x;
+  c->by = y;
+  c->bz = z;
+  c->bq = q;
+  c->stage = 0;
+  return TFMM_OK;
+}
+
+int tfmm_build_tree(tfmm_ctx *c, const tfmm_params *p) {
+  if (!c) return TFMM_EINVAL;
+  int rc = check_params(c, p);
+  if (rc) return rc;
+  if (c->n < 1) return fail(c, TFMM_ESTATE, "no bodies set");
+  TRY(hipSetDevice(c->device));
+  c->prm = *p;
+  c->stage = 0;
+  TRY(hipEventRecord(c->ev[EV_BEGIN], c->stream));
+  rc = do_build_tree(c);
+  if (rc) return rc;
+  TRY(hipEventRecord(c->ev[EV_TREE], c->stream));
+  c->stage = 1;
+  return TFMM_OK;
+}
+
+int tfmm_upward(tfmm_ctx *c) {
+  int rc = need_stage(c, 1, "tfmm_upward");
+  if (rc) return rc;
+  rc = do_upward(c);
+  if (rc) return rc;
+  TRY(hipEventRecord(c->ev[EV_UP], c->stream));
+  c->stage = 2;
+  return TFMM_OK;
+}
+
+int tfmm_traverse(tfmm_ctx *c) {
+  int rc = need_stage(c, 2, "tfmm_traverse");
+  if (rc) return rc;
+  rc = do_traverse(c);
+  if (rc) return rc;
+  TRY(hipEventRecord(c->ev[EV_TRAV], c->stream));
+  c->stage = 3;
+  return TFMM_OK;
+}
+
+int tfmm_downward(tfmm_ctx *c) {
+  int rc = need_stage(c, 3, "tfmm_downward");
+  if (rc) return rc;
+  rc = do_downward(c);
+  if (rc) return rc;
+  TRY(hipEventRecord(c->ev[EV_DOWN], c->stream));
+  c->stage = 4;
+  return TFMM_OK;
+}
+
+int tfmm_evaluate(tfmm_ctx *c, const tfmm_params *p) {
+  int rc = tfmm_build_tree(c, p);
+  if (!rc) rc = tfmm_upward(c);
+  if (!rc) rc = tfmm_traverse(c);
+  if (!rc) rc = tfmm_downward(c);
+  return rc;
+}
+
+int tfmm_get_fields(tfmm_ctx *c, double *dst, int order, int on_device) {
+  int rc = need_stage(c, 4, "tfmm_get_fields");
+  if (rc) return rc;
+  if (!dst) return fail(c, TFMM_EINVAL, "null destination");
+  const void *src = order == 0 ? c->out.ptr : c->trg.ptr;
+  TRY(hipMemcpyAsync(dst, src, (size_t)c->n * 32,
+                     on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
+  TRY(hipStreamSynchronize(c->stream));
+  return TFMM_OK;
+}
+
+int tfmm_get_stats(tfmm_ctx *c, tfmm_stats_t *o) {
+  if (!c || !o) return TFMM_EINVAL;
+  TRY(hipStreamSynchronize(c->stream));
+  memset(o, 0, sizeof(*o));
+  o->n = c->n;
+  o->ncells = c->ncells;
+  o->nleaves = c->nleaves;
+  o->nlevels = c->stage >= 1 ? (int32_t)c->lvl.size() - 1 : 0;
+  o->max_depth = o->nlevels - 1;
+  for (int d = 0; d < 3; d++) {
+    o->region_min[d] = c->h_region[d];
+    o->region_max[d] = c->h_region[3 + d];
+  }
+  if (c->stage >= 3) {
+    o->n_m2l = c->n_m2l_all;
+    o->n_m2l_exec = c->n_m2l_exec;
+    o->n_p2p_leafpairs = c->n_p2p;
+    o->n_p2p_calls = c->h_u64[0];
+    o->n_p2p_self = c->h_u64[1];
+  }
+  auto el = [&](int a, int b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) return 0.0;
+    return (double)ms;
+  };
+  if (c->stage >= 1) o->ms_tree = el(EV_BEGIN, EV_TREE);
+  if (c->stage >= 2) o->ms_upward = el(EV_TREE, EV_UP);
+  if (c->stage >= 3) {
+    o->ms_traverse = el(EV_UP, EV_TRAV);
+    o->ms_dtt = el(EV_DTT0, EV_DTT1);
+    o->ms_m2l = el(EV_M2L0, EV_M2L1);
+    o->ms_p2p = el(EV_P2P0, EV_P2P1);
+  }
+  if (c->stage >= 4) {
+    o->ms_downward = el(EV_TRAV, EV_DOWN);
+    o->ms_total = el(EV_BEGIN, EV_DOWN);
+  }
+  return TFMM_OK;
+}
+
+int tfmm_get_sorted(tfmm_ctx *c, uint64_t *keys, int32_t *perm) {
+  int rc = need_stage(c, 1, "tfmm_get_sorted");
+  if (rc) return rc;
+  const void *k = c->keys_in_alt ? c->keys_alt.ptr : c->keys.ptr;
+  if (keys) TRY(hipMemcpyAsync(keys, k, (size_t)c->n * 8, hipMemcpyDeviceToHost, c->stream));
+  if (perm) TRY(hipMemcpyAsync(perm, c->perm.ptr, (size_t)c->n * 4, hipMemcpyDeviceToHost, c->stream));
+  TRY(hipStreamSynchronize(c->stream));
+  return TFMM_OK;
+}
+
+int tfmm_get_cells(tfmm_ctx *c, uint64_t *key, int32_t *beg, int32_t *nb, int32_t *leaf,
+                   double *center, double *width) {
+  int rc = need_stage(c, 1, "tfmm_get_cells");
+  if (rc) return rc;
+  const size_t nc = (size_t)c->ncells;
+  hipStream_t s = c->stream;
+  if (key) TRY(hipMemcpyAsync(key, c->c_key.ptr, nc * 8, hipMemcpyDeviceToHost, s));
+  if (beg) TRY(hipMemcpyAsync(beg, c->c_beg.ptr, nc * 4, hipMemcpyDeviceToHost, s));
+  if (nb) TRY(hipMemcpyAsync(nb, c->c_nb.ptr, nc * 4, hipMemcpyDeviceToHost, s));
+  if (center) TRY(hipMemcpyAsync(center, c->c_center.ptr, nc * 24, hipMemcpyDeviceToHost, s));
+  if (width) TRY(hipMemcpyAsync(width, c->c_width.ptr, nc * 8, hipMemcpyDeviceToHost, s));
+  if (leaf) {
+    std::vector<int32_t> ch(nc);
+    TRY(hipMemcpyAsync(ch.data(), c->c_child.ptr, nc * 4, hipMemcpyDeviceToHost, s));
+    TRY(hipStreamSynchronize(s));
+    for (size_t i = 0; i < nc; i++) leaf[i] = ch[i] < 0 ? 1 : 0;
+  }
+  TRY(hipStreamSynchronize(s));
+  return TFMM_OK;
+}
+
+int tfmm_get_expansions(tfmm_ctx *c, double *M, double *L) {
+  int rc = need_stage(c, 2, "tfmm_get_expansions");
+  if (rc) return rc;
+  const size_t bytes = (size_t)c->ncells * NTERM * 16;
+  if (M) TRY(hipMemcpyAsync(M, c->M.ptr, bytes, hipMemcpyDeviceToHost, c->stream));
+  if (L) {
+    if (c->stage < 3) return fail(c, TFMM_ESTATE, "L is formed by tfmm_traverse");
+    TRY(hipMemcpyAsync(L, c->L.ptr, bytes, hipMemcpyDeviceToHost, c->stream));
+  }
+  TRY(hipStreamSynchronize(c->stream));
+  return TFMM_OK;
+}
+
+int tfmm_get_lists(tfmm_ctx *c, int32_t *m2l_off, int32_t *m2l_src, int32_t *p2p_off,
+                   int32_t *p2p_src) {
+  int rc = need_stage(c, 3, "tfmm_get_lists");
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  const size_t no = (size_t)(c->ncells + 1) * 4;
+  if (m2l_off) TRY(hipMemcpyAsync(m2l_off, c->m2l_off.ptr, no, hipMemcpyDeviceToHost, s));
+  if (m2l_src && c->n_m2l_exec)
+    TRY(hipMemcpyAsync(m2l_src, c->m2l_src.ptr, (size_t)c->n_m2l_exec * 4, hipMemcpyDeviceToHost, s));
+  if (p2p_off) TRY(hipMemcpyAsync(p2p_off, c->p2p_off.ptr, no, hipMemcpyDeviceToHost, s));
+  if (p2p_src && c->n_p2p)
+    TRY(hipMemcpyAsync(p2p_src, c->p2p_src.ptr, (size_t)c->n_p2p * 4, hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  return TFMM_OK;
+}
+
+int tfmm_direct(tfmm_ctx *c, const int64_t *targets, int64_t nt, double *out) {
+  if (!c) return TFMM_EINVAL;
+  if (c->n < 1) return fail(c, TFMM_ESTATE, "no bodies set");
+  if (nt < 0 || (nt > 0 && (!targets || !out))) return fail(c, TFMM_EINVAL, "bad targets");
+  if (nt == 0) return TFMM_OK;
+  for (int64_t i = 0; i < nt; i++)
+    if (targets[i] < 0 || targets[i] >= c->n) return fail(c, TFMM_EINVAL, "target index out of range");
+  TRY(hipSetDevice(c->device));
+  DevBuf t, o;
+  hipError_t e = dev_reserve(t, (size_t)nt * 8);
+  if (e == hipSuccess) e = dev_reserve(o, (size_t)nt * 32);
+  if (e == hipSuccess) e = hipMemcpyAsync(t.ptr, targets, (size_t)nt * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = launch_direct_fp64(c->bx, c->by, c->bz, c->bq, c->n, dp<int64_t>(t), nt, dp<double>(o), c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, o.ptr, (size_t)nt * 32, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  dev_release(t);
+  dev_release(o);
+  if (e != hipSuccess) return hipfail(c, e, "tfmm_direct");
+  return TFMM_OK;
+}
+
+}  // extern "C"

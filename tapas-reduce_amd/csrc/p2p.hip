@@ -1,0 +1,104 @@
+// p2p.hip -- near-field P2P over a per-target-leaf CSR list of source leaves.
+//
+// Reference: P2P functor (sample/exafmm-dev-13274dd4ac68/examples/
+// LaplaceP2PCPU_tapas.h:48-74) applied by ProductMapImpl over leaf body
+// products (cpp/include/tapas/hot/mapper.h:126-169), mutual = 0:
+//   dX = x_i - x_j; R2 = |dX|^2; if R2 != 0:
+//     invR = q_i q_j / |dX|;  acc_i += (invR, -dX invR / R2)
+// (the reference's CUDA path accumulates with an atomicCAS loop per term,
+// TP/map.h:159-206; here one wave owns a target leaf for its whole source
+// list and every target body is written once).
+//
+// Wave mapping: a target leaf of nt <= 64 bodies is spread over g lanes per
+// body, g the largest power of two with g * nt <= 64; the g lanes of one
+// body split the source bodies and are summed by a butterfly at the end, so
+// leaves of 8..32 bodies still keep all 64 lanes busy.
+#include "tfmm_device.h"
+#include "tfmm_kernels.h"
+
+namespace tfmm {
+namespace {
+
+constexpr int P2P_CHUNK = 64;
+
+__global__ __launch_bounds__(64) void k_p2p_fp64(Cells c, Csr list, Bodies b, double *acc) {
+  const int32_t ti = blockIdx.x;
+  if (c.child[ti] >= 0) return;
+  const int32_t nt = c.nb[ti];
+  if (nt == 0) return;
+  const int32_t tb = c.beg[ti];
+  const int lane = threadIdx.x;
+  __shared__ double sx[P2P_CHUNK], sy[P2P_CHUNK], sz[P2P_CHUNK], sq[P2P_CHUNK];
+  const int32_t lb = list.off[ti], le = list.off[ti + 1];
+  // lanes per target body
+  int lg = 0;
+  while (lg < 6 && (nt << (lg + 1)) <= 64) lg++;
+  const int g = 1 << lg;
+  for (int32_t t0 = 0; t0 < nt; t0 += (64 >> lg)) {
+    const int32_t tl = t0 + (lane >> lg);
+    const int sub = lane & (g - 1);
+    const bool tvalid = tl < nt;
+    double xi = 0, yi = 0, zi = 0, qi = 0;
+    if (tvalid) {
+      xi = b.x[tb + tl];
+      yi = b.y[tb + tl];
+      zi = b.z[tb + tl];
+      qi = b.q[tb + tl];
+    }
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (int32_t s = lb; s < le; s++) {
+      const int32_t sj = list.src[s];
+      const int32_t sb = c.beg[sj], ns = c.nb[sj];
+      for (int32_t c0 = 0; c0 < ns; c0 += P2P_CHUNK) {
+        const int32_t cn = min(P2P_CHUNK, ns - c0);
+        __syncthreads();
+        if (lane < cn) {
+          sx[lane] = b.x[sb + c0 + lane];
+          sy[lane] = b.y[sb + c0 + lane];
+          sz[lane] = b.z[sb + c0 + lane];
+          sq[lane] = b.q[sb + c0 + lane];
+        }
+        __syncthreads();
+        if (tvalid) {
+          for (int jj = sub; jj < cn; jj += g) {
+            const double dx = xi - sx[jj], dy = yi - sy[jj], dz = zi - sz[jj];
+            const double R2 = dx * dx + dy * dy + dz * dz;
+            if (R2 != 0) {
+              const double invR2 = 1.0 / R2;
+              const double invR = qi * sq[jj] * sqrt(invR2);
+              const double f = invR2 * invR;
+              a0 += invR;
+              a1 -= dx * f;
+              a2 -= dy * f;
+              a3 -= dz * f;
+            }
+          }
+        }
+      }
+    }
+    for (int o = 1; o < g; o <<= 1) {
+      a0 += __shfl_xor(a0, o, 64);
+      a1 += __shfl_xor(a1, o, 64);
+      a2 += __shfl_xor(a2, o, 64);
+      a3 += __shfl_xor(a3, o, 64);
+    }
+    if (tvalid && sub == 0) {
+      double *o = acc + 4 * (int64_t)(tb + tl);
+      o[0] = a0;
+      o[1] = a1;
+      o[2] = a2;
+      o[3] = a3;
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t launch_p2p_fp64(Cells c, int32_t ncells, Csr list, Bodies b, double *acc,
+                           hipStream_t s) {
+  if (ncells <= 0) return hipSuccess;
+  k_p2p_fp64<<<ncells, 64, 0, s>>>(c, list, b, acc);
+  return hipGetLastError();
+}
+
+}  // namespace tfmm

@@ -1,0 +1,90 @@
+// tfmm_kernels.h -- device data structures and the launch wrappers of the
+// TapasFMM kernels (one wrapper per kernel family, defined in the .hip files).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tfmm {
+
+// Cells in breadth-first order (level-contiguous, siblings adjacent).
+struct Cells {
+  uint64_t *key;    // Morton key with depth bits (TP/sfc_morton.h)
+  int32_t *beg;     // first sorted body (valid for every cell)
+  int32_t *nb;      // number of bodies
+  int32_t *parent;  // -1 for the root
+  int32_t *child;   // index of the first of 8 children, -1 for leaves
+  int32_t *depth;
+  double *center;   // 3 per cell
+  double *width;    // max_d (max_d - min_d), the quantity the MAC radius uses
+};
+
+// Sorted bodies, SoA.
+struct Bodies {
+  const double *x, *y, *z, *q;
+  int64_t n;
+};
+
+// A CSR interaction list: entries [off[t], off[t+1]) of src are the sources
+// of target cell t.
+struct Csr {
+  const int32_t *off;  // ncells + 1
+  const int32_t *src;
+};
+
+// tree.hip
+hipError_t launch_bbox(const double *x, const double *y, const double *z, int64_t n,
+                       double *partial, int nparts, double *region, hipStream_t s);
+hipError_t launch_keys(const double *x, const double *y, const double *z, int64_t n,
+                       const double *region, uint64_t *keys, uint32_t *idx, hipStream_t s);
+hipError_t launch_permute(const uint32_t *order, const double *x, const double *y,
+                          const double *z, const double *q, int64_t n, double *sx, double *sy,
+                          double *sz, double *sq, int32_t *perm, hipStream_t s);
+hipError_t launch_root(Cells c, int64_t n, const double *region, hipStream_t s);
+hipError_t launch_split_flags(Cells c, int32_t lbeg, int32_t lcnt, int ncrit, uint32_t *flag,
+                              hipStream_t s);
+hipError_t launch_children(Cells c, int32_t lbeg, int32_t lcnt, const uint32_t *flag,
+                           const uint32_t *slot, int32_t next_beg, const uint64_t *keys,
+                           const double *region, hipStream_t s);
+
+// expansions.hip (FP64)
+hipError_t launch_p2m(Cells c, int32_t ncells, Bodies b, double *M, hipStream_t s);
+hipError_t launch_m2m(Cells c, int32_t lbeg, int32_t lcnt, double *M, hipStream_t s);
+hipError_t launch_l2l(Cells c, int32_t lbeg, int32_t lcnt, double *L, hipStream_t s);
+hipError_t launch_l2p(Cells c, int32_t ncells, Bodies b, const double *L, const double *p2p,
+                      double *trg, hipStream_t s);
+hipError_t launch_unpermute(const int32_t *perm, const double *trg_sorted, int64_t n,
+                            double *trg_orig, hipStream_t s);
+
+// traverse.hip
+struct DttCounters {
+  // [0] next-frontier size, [1] executed M2L pairs, [2] executed P2P leaf pairs,
+  // [3] all M2L pairs (also those with an empty cell)
+  uint32_t *cnt;
+};
+hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin_max, int2 *out,
+                           uint32_t out_cap, int2 *m2l, uint32_t m2l_cap, int2 *p2p,
+                           uint32_t p2p_cap, uint32_t *cnt, Cells c, double theta,
+                           hipStream_t s);
+hipError_t launch_pack_pairs(const int2 *pairs, uint32_t n, int sbits, uint64_t *keys,
+                             uint32_t *idx, hipStream_t s);
+hipError_t launch_unpack_pairs(const uint64_t *keys, uint32_t n, int sbits, int32_t *src,
+                               hipStream_t s);
+hipError_t launch_csr_offsets(const uint64_t *sorted_keys, uint32_t n, int sbits,
+                              int32_t ncells, int32_t *off, hipStream_t s);
+hipError_t launch_p2p_pair_count(Cells c, const int2 *p2p, uint32_t n,
+                                 unsigned long long *out, hipStream_t s);
+
+// m2l.hip
+hipError_t launch_m2l_fp64(Cells c, int32_t ncells, Csr list, const double *M, double *L,
+                           hipStream_t s);
+
+// p2p.hip
+hipError_t launch_p2p_fp64(Cells c, int32_t ncells, Csr list, Bodies b, double *acc,
+                           hipStream_t s);
+
+// direct.hip
+hipError_t launch_direct_fp64(const double *x, const double *y, const double *z,
+                              const double *q, int64_t n, const int64_t *targets, int64_t nt,
+                              double *out, hipStream_t s);
+
+}  // namespace tfmm

@@ -1,0 +1,192 @@
+// traverse.hip -- dual tree traversal on the GPU.
+//
+// The reference recursion FMM_DTT / tapas_splitCell
+// (sample/exafmm-dev-13274dd4ac68/examples/tapas_exafmm.cxx:209-286, with
+// mutual = 0) visits a deterministic set of (target, source) cell pairs:
+//   R2 = |centre_i - centre_j|^2  (TP/geometry.h:80-83, TP/vec.h:52-58)
+//   R  = (max_d width_d / 2 * 1.00001) / theta
+//   R2 > (Ri + Rj)^2          -> M2L(Ci, Cj)
+//   both leaves               -> P2P(Ci, Cj)   (skipped if either is empty)
+//   Cj leaf                   -> (children of Ci) x Cj
+//   Ci leaf                   -> Ci x (children of Cj)
+//   Ri >= Rj                  -> (children of Ci) x Cj
+//   else                      -> Ci x (children of Cj)
+// Here that recursion is expanded breadth first: one launch per frontier,
+// each pair either emits an M2L entry, a P2P entry or its 8 successors.  The
+// MAC is evaluated in FP64 without contraction, in the reference's operation
+// order, so every decision is bit-identical to the reference's.
+#include "tfmm_device.h"
+#include "tfmm_kernels.h"
+
+namespace tfmm {
+namespace {
+
+__device__ inline uint32_t wave_append(uint32_t *counter, bool pred, uint32_t per_lane,
+                                       uint32_t &slot) {
+  const uint64_t m = __ballot(pred);
+  const int lane = __lane_id();
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  uint32_t base = 0;
+  if (m) {
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m) * per_lane);
+    base = __shfl(base, leader, 64);
+  }
+  slot = base + (uint32_t)__popcll(m & lt) * per_lane;
+  return (uint32_t)__popcll(m);
+}
+
+__global__ __launch_bounds__(256) void k_dtt_step(const int2 *in, const uint32_t *nin_dev,
+                                                  uint32_t nin_max, int2 *out, uint32_t out_cap,
+                                                  int2 *m2l, uint32_t m2l_cap, int2 *p2p,
+                                                  uint32_t p2p_cap, uint32_t *cnt, Cells c,
+                                                  double theta) {
+#pragma clang fp contract(off)
+  const uint32_t nin = nin_dev ? *nin_dev : nin_max;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = t < nin && t < nin_max;
+  int32_t i = 0, j = 0;
+  bool far = false, p2p_pair = false, split_i = false, split_j = false, nonempty = false;
+  if (valid) {
+    const int2 pr = in[t];
+    i = pr.x;
+    j = pr.y;
+    const double d0 = c.center[3 * (int64_t)i] - c.center[3 * (int64_t)j];
+    const double d1 = c.center[3 * (int64_t)i + 1] - c.center[3 * (int64_t)j + 1];
+    const double d2 = c.center[3 * (int64_t)i + 2] - c.center[3 * (int64_t)j + 2];
+    double R2 = 0;
+    R2 += d0 * d0;
+    R2 += d1 * d1;
+    R2 += d2 * d2;
+    double Ri = c.width[i], Rj = c.width[j];
+    Ri = (Ri / 2 * 1.00001) / theta;
+    Rj = (Rj / 2 * 1.00001) / theta;
+    const bool leaf_i = c.child[i] < 0, leaf_j = c.child[j] < 0;
+    nonempty = c.nb[i] > 0 && c.nb[j] > 0;
+    if (R2 > (Ri + Rj) * (Ri + Rj)) {
+      far = true;
+    } else if (leaf_i && leaf_j) {
+      p2p_pair = nonempty;
+    } else if (leaf_j) {
+      split_i = true;
+    } else if (leaf_i) {
+      split_j = true;
+    } else if (Ri >= Rj) {
+      split_i = true;
+    } else {
+      split_j = true;
+    }
+  }
+  uint32_t slot;
+  // every M2L pair is counted; only pairs of two non-empty cells are executed
+  {
+    const uint64_t m = __ballot(far);
+    if (m && __lane_id() == (__ffsll((unsigned long long)m) - 1))
+      atomicAdd(&cnt[3], (uint32_t)__popcll(m));
+  }
+  wave_append(&cnt[1], far && nonempty, 1, slot);
+  if (far && nonempty && slot < m2l_cap) m2l[slot] = make_int2(i, j);
+  wave_append(&cnt[2], p2p_pair, 1, slot);
+  if (p2p_pair && slot < p2p_cap) p2p[slot] = make_int2(i, j);
+  const bool split = split_i || split_j;
+  wave_append(&cnt[0], split, 8, slot);
+  if (split && slot + 8 <= out_cap) {
+    if (split_i) {
+      const int32_t f = c.child[i];
+      for (int k = 0; k < 8; k++) out[slot + k] = make_int2(f + k, j);
+    } else {
+      const int32_t f = c.child[j];
+      for (int k = 0; k < 8; k++) out[slot + k] = make_int2(i, f + k);
+    }
+  }
+}
+
+__global__ void k_pack_pairs(const int2 *pairs, uint32_t n, int sbits, uint64_t *keys,
+                             uint32_t *idx) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int2 p = pairs[t];
+  keys[t] = ((uint64_t)(uint32_t)p.x << sbits) | (uint64_t)(uint32_t)p.y;
+  if (idx) idx[t] = t;
+}
+
+__global__ void k_unpack_pairs(const uint64_t *keys, uint32_t n, int sbits, int32_t *src) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  src[t] = (int32_t)(keys[t] & ((1ull << sbits) - 1ull));
+}
+
+// off[t] = first entry whose target is >= t (t = 0..ncells)
+__global__ void k_csr_offsets(const uint64_t *keys, uint32_t n, int sbits, int32_t ncells,
+                              int32_t *off) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ncells) return;
+  const uint64_t v = (uint64_t)(uint32_t)t << sbits;
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    if (keys[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  off[t] = (int32_t)lo;
+}
+
+__global__ void k_p2p_pair_count(Cells c, const int2 *p2p, uint32_t n,
+                                 unsigned long long *out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long v = 0, self = 0;
+  if (t < n) {
+    const int2 p = p2p[t];
+    v = (unsigned long long)c.nb[p.x] * (unsigned long long)c.nb[p.y];
+    if (p.x == p.y) self = (unsigned long long)c.nb[p.x];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    v += __shfl_xor(v, o, 64);
+    self += __shfl_xor(self, o, 64);
+  }
+  if (__lane_id() == 0) {
+    atomicAdd(&out[0], v);
+    atomicAdd(&out[1], self);
+  }
+}
+
+}  // namespace
+
+hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin_max, int2 *out,
+                           uint32_t out_cap, int2 *m2l, uint32_t m2l_cap, int2 *p2p,
+                           uint32_t p2p_cap, uint32_t *cnt, Cells c, double theta,
+                           hipStream_t s) {
+  if (nin_max == 0) return hipSuccess;
+  k_dtt_step<<<(nin_max + 255) / 256, 256, 0, s>>>(in, nin_dev, nin_max, out, out_cap, m2l,
+                                                  m2l_cap, p2p, p2p_cap, cnt, c, theta);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_pairs(const int2 *pairs, uint32_t n, int sbits, uint64_t *keys,
+                             uint32_t *idx, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_pack_pairs<<<(n + 255) / 256, 256, 0, s>>>(pairs, n, sbits, keys, idx);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack_pairs(const uint64_t *keys, uint32_t n, int sbits, int32_t *src,
+                               hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_unpack_pairs<<<(n + 255) / 256, 256, 0, s>>>(keys, n, sbits, src);
+  return hipGetLastError();
+}
+
+hipError_t launch_csr_offsets(const uint64_t *sorted_keys, uint32_t n, int sbits,
+                              int32_t ncells, int32_t *off, hipStream_t s) {
+  k_csr_offsets<<<(ncells + 1 + 255) / 256, 256, 0, s>>>(sorted_keys, n, sbits, ncells, off);
+  return hipGetLastError();
+}
+
+hipError_t launch_p2p_pair_count(Cells c, const int2 *p2p, uint32_t n,
+                                 unsigned long long *out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_p2p_pair_count<<<(n + 255) / 256, 256, 0, s>>>(c, p2p, n, out);
+  return hipGetLastError();
+}
+
+}  // namespace tfmm

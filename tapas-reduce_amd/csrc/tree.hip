@@ -1,0 +1,239 @@
+// tree.hip -- octree construction on the GPU: scalar-cube bounding box,
+// Morton keys, body permutation and the level-synchronous cell build.
+//
+// Reference semantics (cpp/include/tapas/):
+//   bounding box    hot/buildtree.h:72-87 (a Vec assigned from a scalar inside a
+//                   per-dimension loop: min/max over every coordinate of every
+//                   body except body 0's x and y) and vec.h:39-43
+//   keys            hot/buildtree.h:504-539, sfc_morton.h:387-402
+//   cells           hot/buildtree.h:388-459: a cell is a leaf iff nb <= ncrit;
+//                   every internal cell gets all 8 children, empty ones too;
+//                   region by halving (sfc_morton.h:536-557), centre (max+min)/2
+//                   (hot.h:232-238)
+// The cell set is built breadth first: level d+1 is produced from level d by
+// one scan over the "split" flags, so cells of one level are contiguous and
+// the 8 children of a cell are adjacent, which the level-synchronous upward
+// and downward passes rely on.
+#include "tfmm_device.h"
+#include "tfmm_kernels.h"
+
+namespace tfmm {
+namespace {
+
+constexpr int BB_THREADS = 256;
+
+__global__ __launch_bounds__(BB_THREADS) void k_bbox_partial(const double *x, const double *y,
+                                                             const double *z, int64_t n,
+                                                             double *pmin, double *pmax) {
+  double lo = INFINITY, hi = -INFINITY;
+  for (int64_t i = (int64_t)blockIdx.x * BB_THREADS + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * BB_THREADS) {
+    const double vz = z[i];
+    lo = fmin(lo, vz);
+    hi = fmax(hi, vz);
+    if (i != 0) {
+      const double vx = x[i], vy = y[i];
+      lo = fmin(lo, fmin(vx, vy));
+      hi = fmax(hi, fmax(vx, vy));
+    }
+  }
+  __shared__ double slo[BB_THREADS], shi[BB_THREADS];
+  slo[threadIdx.x] = lo;
+  shi[threadIdx.x] = hi;
+  __syncthreads();
+  for (int o = BB_THREADS / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      slo[threadIdx.x] = fmin(slo[threadIdx.x], slo[threadIdx.x + o]);
+      shi[threadIdx.x] = fmax(shi[threadIdx.x], shi[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pmin[blockIdx.x] = slo[0];
+    pmax[blockIdx.x] = shi[0];
+  }
+}
+
+__global__ __launch_bounds__(BB_THREADS) void k_bbox_final(const double *pmin,
+                                                           const double *pmax, int nparts,
+                                                           double *region) {
+  __shared__ double slo[BB_THREADS], shi[BB_THREADS];
+  double lo = INFINITY, hi = -INFINITY;
+  for (int i = threadIdx.x; i < nparts; i += BB_THREADS) {
+    lo = fmin(lo, pmin[i]);
+    hi = fmax(hi, pmax[i]);
+  }
+  slo[threadIdx.x] = lo;
+  shi[threadIdx.x] = hi;
+  __syncthreads();
+  for (int o = BB_THREADS / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      slo[threadIdx.x] = fmin(slo[threadIdx.x], slo[threadIdx.x + o]);
+      shi[threadIdx.x] = fmax(shi[threadIdx.x], shi[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) {
+    region[threadIdx.x] = slo[0];
+    region[3 + threadIdx.x] = shi[0];
+  }
+}
+
+__global__ void k_keys(const double *x, const double *y, const double *z, int64_t n,
+                       const double *region, uint64_t *keys, uint32_t *idx) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int nfin = 1 << MAXDEPTH;
+  const double p[3] = {x[i], y[i], z[i]};
+  int a[3];
+  for (int d = 0; d < 3; d++) {
+    const double pitch = (region[3 + d] - region[d]) / (double)nfin;
+    double o = p[d] - region[d];
+    o = o / pitch;
+    int v = (int)o;
+    if (v == nfin) v--;
+    a[d] = v;
+  }
+  keys[i] = finest_key(a[0], a[1], a[2]);
+  idx[i] = (uint32_t)i;
+}
+
+// Gather the caller's bodies into sorted order (SoA FP64) and keep the
+// permutation.  perm[sorted] = original index.
+__global__ void k_permute(const uint32_t *order, const double *x, const double *y,
+                          const double *z, const double *q, int64_t n, double *sx, double *sy,
+                          double *sz, double *sq, int32_t *perm) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t o = order[i];
+  sx[i] = x[o];
+  sy[i] = y[o];
+  sz[i] = z[o];
+  sq[i] = q[o];
+  perm[i] = (int32_t)o;
+}
+
+__global__ void k_root(Cells c, int64_t n, const double *region) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  c.key[0] = 0;
+  c.beg[0] = 0;
+  c.nb[0] = (int32_t)n;
+  c.parent[0] = -1;
+  c.child[0] = -1;
+  c.depth[0] = 0;
+  double mn[3] = {region[0], region[1], region[2]};
+  double mx[3] = {region[3], region[4], region[5]};
+  double w = 0;
+  for (int d = 0; d < 3; d++) {
+    c.center[3 * 0 + d] = (mx[d] + mn[d]) / 2;
+    const double wd = mx[d] - mn[d];
+    w = (w < wd) ? wd : w;
+  }
+  c.width[0] = w;
+}
+
+__global__ void k_split_flags(Cells c, int32_t lbeg, int32_t lcnt, int ncrit, uint32_t *flag) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= lcnt) return;
+  const int32_t ci = lbeg + i;
+  const bool split = c.nb[ci] > ncrit && c.depth[ci] < MAXDEPTH;
+  flag[i] = split ? 1u : 0u;
+  if (!split) c.child[ci] = -1;
+}
+
+__device__ inline int32_t lower_bound_keys(const uint64_t *k, int32_t lo, int32_t hi,
+                                           uint64_t v) {
+  while (lo < hi) {
+    const int32_t mid = lo + ((hi - lo) >> 1);
+    if (k[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_children(Cells c, int32_t lbeg, int32_t lcnt, const uint32_t *flag,
+                           const uint32_t *slot, int32_t next_beg, const uint64_t *keys,
+                           const double *region) {
+  // one thread per (parent, child) -> 8 threads per parent
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)lcnt * 8) return;
+  const int32_t i = (int32_t)(t >> 3);
+  const int ch = (int)(t & 7);
+  if (!flag[i]) return;
+  const int32_t p = lbeg + i;
+  const int32_t first = next_beg + 8 * (int32_t)slot[i];
+  const int32_t ci = first + ch;
+  const uint64_t pk = c.key[p];
+  const uint64_t ck = key_child(pk, ch);
+  const int32_t pb = c.beg[p], pe = c.beg[p] + c.nb[p];
+  const int32_t b = lower_bound_keys(keys, pb, pe, ck);
+  const int32_t e = (ch == 7) ? pe : lower_bound_keys(keys, pb, pe, key_child(pk, ch + 1));
+  c.key[ci] = ck;
+  c.beg[ci] = b;
+  c.nb[ci] = e - b;
+  c.parent[ci] = p;
+  c.child[ci] = -1;
+  c.depth[ci] = c.depth[p] + 1;
+  double rmin[3] = {region[0], region[1], region[2]};
+  double rmax[3] = {region[3], region[4], region[5]};
+  double mn[3], mx[3];
+  cell_region(ck, rmin, rmax, mn, mx);
+  double w = 0;
+  for (int d = 0; d < 3; d++) {
+    c.center[3 * (int64_t)ci + d] = (mx[d] + mn[d]) / 2;
+    const double wd = mx[d] - mn[d];
+    w = (w < wd) ? wd : w;
+  }
+  c.width[ci] = w;
+  if (ch == 0) c.child[p] = first;
+}
+
+}  // namespace
+
+hipError_t launch_bbox(const double *x, const double *y, const double *z, int64_t n,
+                       double *partial, int nparts, double *region, hipStream_t s) {
+  k_bbox_partial<<<nparts, BB_THREADS, 0, s>>>(x, y, z, n, partial, partial + nparts);
+  k_bbox_final<<<1, BB_THREADS, 0, s>>>(partial, partial + nparts, nparts, region);
+  return hipGetLastError();
+}
+
+hipError_t launch_keys(const double *x, const double *y, const double *z, int64_t n,
+                       const double *region, uint64_t *keys, uint32_t *idx, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_keys<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(x, y, z, n, region, keys, idx);
+  return hipGetLastError();
+}
+
+hipError_t launch_permute(const uint32_t *order, const double *x, const double *y,
+                          const double *z, const double *q, int64_t n, double *sx, double *sy,
+                          double *sz, double *sq, int32_t *perm, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_permute<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(order, x, y, z, q, n, sx, sy, sz, sq,
+                                                         perm);
+  return hipGetLastError();
+}
+
+hipError_t launch_root(Cells c, int64_t n, const double *region, hipStream_t s) {
+  k_root<<<1, 64, 0, s>>>(c, n, region);
+  return hipGetLastError();
+}
+
+hipError_t launch_split_flags(Cells c, int32_t lbeg, int32_t lcnt, int ncrit, uint32_t *flag,
+                              hipStream_t s) {
+  if (lcnt <= 0) return hipSuccess;
+  k_split_flags<<<(lcnt + 255) / 256, 256, 0, s>>>(c, lbeg, lcnt, ncrit, flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_children(Cells c, int32_t lbeg, int32_t lcnt, const uint32_t *flag,
+                           const uint32_t *slot, int32_t next_beg, const uint64_t *keys,
+                           const double *region, hipStream_t s) {
+  if (lcnt <= 0) return hipSuccess;
+  const int64_t nt = (int64_t)lcnt * 8;
+  k_children<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(c, lbeg, lcnt, flag, slot, next_beg,
+                                                           keys, region);
+  return hipGetLastError();
+}
+
+}  // namespace tfmm

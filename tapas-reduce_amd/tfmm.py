@@ -1,0 +1,251 @@
+"""Python binding of libtfmm.so (the C ABI in include/tfmm.h) via ctypes.
+
+This is the host-side mirror of the Tapas FMM driver sequence for Python
+callers (tests, bench): ``Partition`` (tree build), ``upward``, ``traverse``,
+``downward`` and the result accessors, the same phase split as the
+reference's tapas_exafmm.cxx:557-641.  It loads the in-tree HIP library and
+fails loudly if it is missing; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtfmm.so")
+
+TFMM_OK = 0
+FP64 = 0
+MIXED = 1
+P = 10
+NTERM = 55
+
+_lib = None
+
+
+class TfmmError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("ncrit", ctypes.c_int32), ("mutual", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("theta", ctypes.c_double)]
+
+
+class BodyDesc(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("stride", ctypes.c_size_t), ("off_x", ctypes.c_size_t),
+                ("off_q", ctypes.c_size_t)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("ncells", ctypes.c_int64), ("nleaves", ctypes.c_int64),
+                ("nlevels", ctypes.c_int32), ("max_depth", ctypes.c_int32),
+                ("region_min", ctypes.c_double * 3), ("region_max", ctypes.c_double * 3),
+                ("n_m2l", ctypes.c_uint64), ("n_m2l_exec", ctypes.c_uint64),
+                ("n_p2p_leafpairs", ctypes.c_uint64), ("n_p2p_calls", ctypes.c_uint64),
+                ("n_p2p_self", ctypes.c_uint64),
+                ("ms_tree", ctypes.c_double), ("ms_upward", ctypes.c_double),
+                ("ms_traverse", ctypes.c_double), ("ms_downward", ctypes.c_double),
+                ("ms_total", ctypes.c_double), ("ms_p2p", ctypes.c_double),
+                ("ms_m2l", ctypes.c_double), ("ms_dtt", ctypes.c_double)]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if isinstance(v, ctypes.Array) else v
+        return out
+
+
+EXPORTED = [
+    "tfmm_create", "tfmm_destroy", "tfmm_last_error", "tfmm_stream", "tfmm_sync",
+    "tfmm_set_bodies", "tfmm_set_bodies_device", "tfmm_build_tree", "tfmm_upward",
+    "tfmm_traverse", "tfmm_downward", "tfmm_evaluate", "tfmm_get_fields", "tfmm_get_stats",
+    "tfmm_get_sorted", "tfmm_get_cells", "tfmm_get_expansions", "tfmm_get_lists", "tfmm_direct",
+]
+
+
+def lib():
+    """Load libtfmm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TfmmError("libtfmm.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.tfmm_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        L.tfmm_destroy.argtypes = [vp]
+        L.tfmm_destroy.restype = None
+        L.tfmm_last_error.argtypes = [vp]
+        L.tfmm_last_error.restype = ctypes.c_char_p
+        L.tfmm_stream.argtypes = [vp]
+        L.tfmm_stream.restype = vp
+        L.tfmm_sync.argtypes = [vp]
+        L.tfmm_set_bodies.argtypes = [vp, vp, ctypes.POINTER(BodyDesc)]
+        L.tfmm_set_bodies_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp]
+        L.tfmm_build_tree.argtypes = [vp, ctypes.POINTER(Params)]
+        L.tfmm_evaluate.argtypes = [vp, ctypes.POINTER(Params)]
+        for f in ("tfmm_upward", "tfmm_traverse", "tfmm_downward"):
+            getattr(L, f).argtypes = [vp]
+        L.tfmm_get_fields.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
+        L.tfmm_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+        L.tfmm_get_sorted.argtypes = [vp, vp, vp]
+        L.tfmm_get_cells.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.tfmm_get_expansions.argtypes = [vp, vp, vp]
+        L.tfmm_get_lists.argtypes = [vp, vp, vp, vp, vp]
+        L.tfmm_direct.argtypes = [vp, vp, ctypes.c_int64, vp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def make_params(ncrit=64, theta=0.4, precision=FP64, mutual=0):
+    return Params(int(ncrit), int(mutual), int(precision), 0, float(theta))
+
+
+class Context:
+    """One tfmm_ctx: a device, a stream and the buffers of one problem."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = ctypes.c_void_p()
+        rc = L.tfmm_create(int(device), ctypes.byref(h))
+        if rc != TFMM_OK:
+            raise TfmmError("tfmm_create failed with status %d" % rc)
+        self._h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().tfmm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- helpers ---------------------------------------------------------
+    def _check(self, rc, what):
+        if rc != TFMM_OK:
+            msg = lib().tfmm_last_error(self._h)
+            raise TfmmError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    @property
+    def stream(self):
+        return lib().tfmm_stream(self._h)
+
+    def sync(self):
+        self._check(lib().tfmm_sync(self._h), "tfmm_sync")
+
+    # -- bodies ----------------------------------------------------------
+    def set_bodies(self, x, y, z, q):
+        """Copy host bodies (through the reference's 96-byte Body layout:
+        X at offset 0, SRC at 24)."""
+        n = len(x)
+        body = np.zeros((n, 12), np.float64)  # 96 bytes per body
+        body[:, 0] = x
+        body[:, 1] = y
+        body[:, 2] = z
+        body[:, 3] = q
+        desc = BodyDesc(n, 96, 0, 24)
+        self._check(lib().tfmm_set_bodies(self._h, _ptr(body), ctypes.byref(desc)),
+                    "tfmm_set_bodies")
+        self.n = n
+
+    def set_bodies_device(self, n, px, py, pz, pq):
+        """Use device-resident FP64 SoA arrays (raw device pointers)."""
+        self._check(lib().tfmm_set_bodies_device(self._h, int(n), px, py, pz, pq),
+                    "tfmm_set_bodies_device")
+        self.n = int(n)
+
+    # -- phases ----------------------------------------------------------
+    def build_tree(self, params):
+        self._check(lib().tfmm_build_tree(self._h, ctypes.byref(params)), "tfmm_build_tree")
+
+    def upward(self):
+        self._check(lib().tfmm_upward(self._h), "tfmm_upward")
+
+    def traverse(self):
+        self._check(lib().tfmm_traverse(self._h), "tfmm_traverse")
+
+    def downward(self):
+        self._check(lib().tfmm_downward(self._h), "tfmm_downward")
+
+    def evaluate(self, params):
+        self._check(lib().tfmm_evaluate(self._h, ctypes.byref(params)), "tfmm_evaluate")
+
+    # -- results ---------------------------------------------------------
+    def fields(self, order=0):
+        out = np.zeros((self.n, 4), np.float64)
+        self._check(lib().tfmm_get_fields(self._h, _ptr(out), int(order), 0), "tfmm_get_fields")
+        return out
+
+    def fields_to_device(self, dev_ptr, order=0):
+        self._check(lib().tfmm_get_fields(self._h, dev_ptr, int(order), 1), "tfmm_get_fields")
+
+    def stats(self):
+        s = Stats()
+        self._check(lib().tfmm_get_stats(self._h, ctypes.byref(s)), "tfmm_get_stats")
+        return s.as_dict()
+
+    def sorted(self):
+        k = np.zeros(self.n, np.uint64)
+        p = np.zeros(self.n, np.int32)
+        self._check(lib().tfmm_get_sorted(self._h, _ptr(k), _ptr(p)), "tfmm_get_sorted")
+        return k, p
+
+    def cells(self):
+        nc = self.stats()["ncells"]
+        out = dict(key=np.zeros(nc, np.uint64), beg=np.zeros(nc, np.int32),
+                   nb=np.zeros(nc, np.int32), leaf=np.zeros(nc, np.int32),
+                   center=np.zeros((nc, 3)), width=np.zeros(nc))
+        self._check(lib().tfmm_get_cells(self._h, _ptr(out["key"]), _ptr(out["beg"]),
+                                         _ptr(out["nb"]), _ptr(out["leaf"]),
+                                         _ptr(out["center"]), _ptr(out["width"])),
+                    "tfmm_get_cells")
+        return out
+
+    def expansions(self, want_l=True):
+        nc = self.stats()["ncells"]
+        M = np.zeros((nc, NTERM), np.complex128)
+        Lx = np.zeros((nc, NTERM), np.complex128) if want_l else None
+        self._check(lib().tfmm_get_expansions(self._h, _ptr(M), _ptr(Lx)),
+                    "tfmm_get_expansions")
+        return M, Lx
+
+    def lists(self):
+        st = self.stats()
+        nc = st["ncells"]
+        mo = np.zeros(nc + 1, np.int32)
+        ms = np.zeros(max(1, st["n_m2l_exec"]), np.int32)
+        po = np.zeros(nc + 1, np.int32)
+        ps = np.zeros(max(1, st["n_p2p_leafpairs"]), np.int32)
+        self._check(lib().tfmm_get_lists(self._h, _ptr(mo), _ptr(ms), _ptr(po), _ptr(ps)),
+                    "tfmm_get_lists")
+        return (mo, ms[:st["n_m2l_exec"]]), (po, ps[:st["n_p2p_leafpairs"]])
+
+    def direct(self, targets):
+        t = np.ascontiguousarray(targets, np.int64)
+        out = np.zeros((len(t), 4), np.float64)
+        self._check(lib().tfmm_direct(self._h, _ptr(t), len(t), _ptr(out)), "tfmm_direct")
+        return out
+
+
+def evaluate(x, y, z, q, ncrit=64, theta=0.4, precision=FP64, device=0):
+    """One full FMM evaluation of host bodies; returns (fields, stats)."""
+    with Context(device) as ctx:
+        ctx.set_bodies(x, y, z, q)
+        ctx.evaluate(make_params(ncrit, theta, precision))
+        return ctx.fields(), ctx.stats()

@@ -1,0 +1,98 @@
+"""Writes the golden fixtures of tests/golden/ by running the reference's own
+TapasFMM pipeline, compiled from the reference sources by oracle/Makefile
+(oracle/_ref/tapas_ref_count), on seeded bodies.
+
+Run in the build container (where /root/reference exists):
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Each fixture is an .npz of plain arrays (no pickles): the input bodies and
+parameters, and what the reference produced for them -- bounding box, sorted
+keys and body order, cells (key, nb, first body, leaf, centre, width), the
+-DCOUNT counters numM2L/numP2P, fields per body (caller order) and, for the
+small cases, M and L of every cell.  For the 1M-body case only a digest of
+keys and cells and the fields of 4096 sampled bodies are kept.
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tapas-reduce_amd"))
+import datasets  # noqa: E402
+import oracle  # noqa: E402
+
+
+def digest(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def cases():
+    """(name, bodies, ncrit, theta, full) for every fixture."""
+    out = []
+    out.append(("c0_cube16k", datasets.cube(16384, seed=0), 64, 0.4, True))
+    # (a root that is a leaf, n <= ncrit, makes the reference abort in its
+    #  inspector: "Cell::subcell(int) is called for a leaf cell"; not a fixture)
+    out.append(("cube_n2", datasets.cube(2, seed=2), 1, 0.4, True))
+    out.append(("cube_n65", datasets.cube(65, seed=3), 64, 0.4, True))
+    out.append(("cube_n200_c16", datasets.cube(200, seed=4), 16, 0.4, True))
+    out.append(("plummer3000_c32_t05", datasets.plummer(3000, seed=5, charges="uniform"), 32, 0.5,
+                True))
+    out.append(("ellipsoid5000", datasets.ellipsoid(5000, seed=6), 64, 0.4, True))
+    out.append(("two_plummer4000_c16", datasets.two_plummer(4000, seed=7), 16, 0.3, True))
+    # coincident bodies (r = 0 pairs between distinct bodies are skipped)
+    x, y, z, q = datasets.cube(400, seed=8)
+    x[200:230] = x[0:30]
+    y[200:230] = y[0:30]
+    z[200:230] = z[0:30]
+    out.append(("cube_dups400_c8", (x, y, z, q), 8, 0.4, True))
+    # body 0 holds the global x minimum and y maximum: the scalar-cube box of
+    # the reference ignores them, so body 0 falls outside the root region
+    x, y, z, q = datasets.cube(500, seed=9)
+    x[0] = -0.25
+    y[0] = 1.4
+    out.append(("bbox_quirk500", (x, y, z, q), 16, 0.4, True))
+    out.append(("c1_cube1m", datasets.cube(1 << 20, seed=0), 64, 0.4, False))
+    return out
+
+
+def main(only=None):
+    for name, (x, y, z, q), ncrit, theta, full in cases():
+        if only and name not in only:
+            continue
+        r = oracle.run_ref(x, y, z, q, ncrit, theta, count=True, dump_expansions=full)
+        cells = r["cells"]
+        d = dict(
+            ncrit=np.int64(ncrit), theta=np.float64(theta),
+            rmin=r["rmin"], rmax=r["rmax"], max_depth=np.int64(r["max_depth"]),
+            n_m2l=np.uint64(r["m2l"]), n_p2p_calls=np.uint64(r["p2p_calls"]),
+            ncells=np.int64(len(cells)),
+        )
+        if full:
+            d.update(x=x, y=y, z=z, q=q)
+            d.update(keys=r["keys"], ibody=r["ibody"], cell_key=cells["key"], cell_nb=cells["nb"],
+                     cell_bid=cells["bid"], cell_leaf=cells["leaf"], cell_center=cells["center"],
+                     cell_width=cells["width"], trg=r["trg"], M=r["M"], L=r["L"])
+        else:
+            # large case: the bodies are regenerated from the seeded generator
+            # by the tests; only their digest is kept
+            sample = np.arange(0, len(x), len(x) // 4096)[:4096]
+            d.update(input_sha=np.bytes_(digest(x, y, z, q)),keys_sha=np.bytes_(digest(r["keys"])),
+                     cells_sha=np.bytes_(digest(cells["key"], cells["nb"], cells["leaf"])),
+                     sample=sample, trg_sample=r["trg"][sample],
+                     nleaves=np.int64(int((cells["leaf"] == 1).sum())))
+        path = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(path, **d)
+        print("%-24s n=%-8d cells=%-7d m2l=%-9d p2p_calls=%-11d -> %s (%d KB)" % (
+            name, len(x), len(cells), r["m2l"], r["p2p_calls"], os.path.basename(path),
+            os.path.getsize(path) // 1024), flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:] or None)

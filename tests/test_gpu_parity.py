@@ -1,0 +1,194 @@
+"""Parity of the HIP library (through the C ABI) with the reference's outputs
+(tests/golden/) and with the CPU oracle, FP64 path.
+
+Bars: keys, body order (up to ties), cells, interaction counters and the
+traversal lists bit-exact; multipole/local expansions and fields within
+relative 1e-10 (max-norm for expansions, relative L2 for fields).
+"""
+import glob
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import datasets
+import oracle
+import tfmm
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+FULL = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
+              if not os.path.basename(p).startswith("c1_"))
+TOL = 1e-10
+
+
+def load(name):
+    return np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = tfmm.Context(0)
+    yield c
+    c.close()
+
+
+def evaluate(ctx, x, y, z, q, ncrit, theta, precision=tfmm.FP64):
+    print("case n=%d ncrit=%d theta=%g precision=%d" % (len(x), ncrit, theta, precision),
+          flush=True)
+    ctx.set_bodies(x, y, z, q)
+    ctx.evaluate(tfmm.make_params(ncrit, theta, precision))
+    return ctx.fields(), ctx.stats()
+
+
+def rel_max(a, b):
+    scale = np.abs(b).max()
+    return np.abs(a - b).max() / scale if scale > 0 else np.abs(a - b).max()
+
+
+def same_key_groups(keys, perm_a, perm_b):
+    if np.array_equal(perm_a, perm_b):
+        return True
+    bounds = np.flatnonzero(np.diff(keys)) + 1
+    return all(np.array_equal(np.sort(a), np.sort(b))
+               for a, b in zip(np.split(perm_a, bounds), np.split(perm_b, bounds)))
+
+
+@pytest.mark.parametrize("name", FULL)
+def test_gpu_matches_reference(ctx, name):
+    g = load(name)
+    f, st = evaluate(ctx, g["x"], g["y"], g["z"], g["q"], int(g["ncrit"]), float(g["theta"]))
+    assert np.array_equal(st["region_min"], g["rmin"]) and np.array_equal(st["region_max"], g["rmax"])
+    keys, perm = ctx.sorted()
+    assert np.array_equal(keys, g["keys"])
+    assert same_key_groups(keys, perm.astype(np.int64), g["ibody"])
+    c = ctx.cells()
+    order = np.argsort(c["key"])
+    assert st["ncells"] == int(g["ncells"]) and st["max_depth"] == int(g["max_depth"])
+    assert np.array_equal(c["key"][order], g["cell_key"])
+    assert np.array_equal(c["nb"][order], g["cell_nb"])
+    assert np.array_equal(c["leaf"][order], g["cell_leaf"])
+    nel = (g["cell_leaf"] == 1) & (g["cell_nb"] > 0)
+    assert np.array_equal(c["beg"][order][nel], g["cell_bid"][nel])
+    assert np.array_equal(c["center"][order], g["cell_center"])
+    assert np.array_equal(c["width"][order], g["cell_width"].max(axis=1))
+    assert st["n_m2l"] == int(g["n_m2l"])
+    assert st["n_p2p_calls"] == int(g["n_p2p_calls"])
+    M, L = ctx.expansions()
+    assert rel_max(M[order], g["M"]) < TOL
+    if np.abs(g["L"]).max() > 0:
+        assert rel_max(L[order], g["L"]) < TOL
+    ep, ea = oracle.rel_l2(f, g["trg"])
+    assert ep < TOL and ea < TOL, (ep, ea)
+
+
+def test_gpu_lists_match_oracle(ctx):
+    """Executed M2L and P2P lists equal the oracle's (as sets of cell-key pairs)."""
+    g = load("plummer3000_c32_t05")
+    evaluate(ctx, g["x"], g["y"], g["z"], g["q"], int(g["ncrit"]), float(g["theta"]))
+    c = ctx.cells()
+    (mo, ms), (po, ps) = ctx.lists()
+    o = oracle.Run(g["x"], g["y"], g["z"], g["q"], int(g["ncrit"]), float(g["theta"]), phases=2,
+                   record_lists=True)
+    oc = o.cells()
+    om, op = o.lists()
+    onb = oc["nb"]
+
+    def gpu_pairs(off, src):
+        tgt = np.repeat(np.arange(len(off) - 1), np.diff(off))
+        return set(zip(c["key"][tgt].tolist(), c["key"][src].tolist()))
+
+    def orc_pairs(lst, nonempty):
+        if nonempty:
+            lst = lst[(onb[lst[:, 0]] > 0) & (onb[lst[:, 1]] > 0)]
+        return set(zip(oc["key"][lst[:, 0]].tolist(), oc["key"][lst[:, 1]].tolist()))
+
+    assert gpu_pairs(mo, ms) == orc_pairs(om, True)
+    assert gpu_pairs(po, ps) == orc_pairs(op, False)
+    # sorted by target, then source: deterministic order
+    tgt = np.repeat(np.arange(len(mo) - 1), np.diff(mo))
+    assert np.all(np.diff(tgt.astype(np.int64) * (1 << 32) + ms) > 0)
+
+
+def test_gpu_c1_matches_reference(ctx):
+    """1M uniform bodies: key/cell digests, counters and 4096 sampled fields
+    against the reference, and the FMM against the GPU FP64 direct sum."""
+    g = load("c1_cube1m")
+    x, y, z, q = datasets.cube(1 << 20, seed=0)
+    h = hashlib.sha256()
+    for a in (x, y, z, q):
+        h.update(a.tobytes())
+    assert h.hexdigest() == g["input_sha"].item().decode()
+    f, st = evaluate(ctx, x, y, z, q, 64, 0.4)
+    keys, _ = ctx.sorted()
+    assert hashlib.sha256(keys.tobytes()).hexdigest() == g["keys_sha"].item().decode()
+    c = ctx.cells()
+    order = np.argsort(c["key"])
+    h = hashlib.sha256()
+    for a in (c["key"][order], c["nb"][order].astype(np.int64), c["leaf"][order]):
+        h.update(np.ascontiguousarray(a).tobytes())
+    assert h.hexdigest() == g["cells_sha"].item().decode()
+    assert st["n_m2l"] == int(g["n_m2l"]) == 5656496
+    assert st["n_p2p_leafpairs"] == 2421064
+    assert st["n_p2p_calls"] == int(g["n_p2p_calls"])
+    s = g["sample"]
+    ep, ea = oracle.rel_l2(f[s], g["trg_sample"])
+    assert ep < TOL and ea < TOL, (ep, ea)
+    d = ctx.direct(s)
+    ep, ea = oracle.rel_l2(f[s], d)
+    assert ep < 1e-4 and ea < 1e-4, (ep, ea)
+
+
+def test_gpu_direct_matches_oracle(ctx):
+    x, y, z, q = datasets.cube(3000, seed=3)
+    ctx.set_bodies(x, y, z, q)
+    t = np.arange(0, 3000, 11)
+    got = ctx.direct(t)
+    ref = oracle.direct(x, y, z, q, t)
+    ep, ea = oracle.rel_l2(got, ref)
+    assert ep < 1e-13 and ea < 1e-13
+
+
+@pytest.mark.parametrize("n,ncrit", [(1, 64), (7, 64), (64, 64), (100, 1)])
+def test_gpu_small_and_root_leaf(ctx, n, ncrit):
+    """Root-leaf trees (n <= ncrit) and ncrit = 1; checked against the oracle."""
+    x, y, z, q = datasets.cube(n, seed=n)
+    f, st = evaluate(ctx, x, y, z, q, ncrit, 0.4)
+    o = oracle.Run(x, y, z, q, ncrit, 0.4)
+    assert st["ncells"] == o.ncells
+    assert st["n_m2l"] == o.counts()["m2l"]
+    ep, ea = oracle.rel_l2(f, o.fields())
+    if n == 1:
+        assert np.all(f == 0)
+    else:
+        assert ep < TOL and ea < TOL
+
+
+def test_gpu_repeat_is_deterministic(ctx):
+    x, y, z, q = datasets.plummer(20000, seed=4, charges="uniform")
+    f1, _ = evaluate(ctx, x, y, z, q, 32, 0.4)
+    f2, _ = evaluate(ctx, x, y, z, q, 32, 0.4)
+    assert np.array_equal(f1, f2)
+
+
+def test_gpu_phases_in_order_and_errors(ctx):
+    x, y, z, q = datasets.cube(500, seed=6)
+    ctx.set_bodies(x, y, z, q)
+    with pytest.raises(tfmm.TfmmError):
+        ctx.upward()  # before build_tree
+    with pytest.raises(tfmm.TfmmError):
+        ctx.build_tree(tfmm.make_params(64, 0.4, mutual=1))
+    with pytest.raises(tfmm.TfmmError):
+        ctx.build_tree(tfmm.make_params(0, 0.4))
+    with pytest.raises(tfmm.TfmmError):
+        ctx.build_tree(tfmm.make_params(64, -1.0))
+    p = tfmm.make_params(16, 0.4)
+    ctx.build_tree(p)
+    ctx.upward()
+    ctx.traverse()
+    ctx.downward()
+    o = oracle.Run(x, y, z, q, 16, 0.4)
+    ep, ea = oracle.rel_l2(ctx.fields(), o.fields())
+    assert ep < TOL and ea < TOL
