@@ -92,6 +92,9 @@ typedef struct {
    * context stream; kernel entries are summed over their launches */
   double ms_tree, ms_upward, ms_traverse, ms_downward, ms_total;
   double ms_p2p, ms_m2l, ms_dtt;
+  /* M2L batching: offset groups, MFMA work items (<= 16 pairs of one group
+   * and one 128-cell target tile each) and pairs outside the groups */
+  uint64_t n_m2l_groups, n_m2l_items, n_m2l_fallback;
 } tfmm_stats_t;
 
 int tfmm_create(int hip_device, tfmm_ctx **out);

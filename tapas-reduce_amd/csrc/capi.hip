@@ -61,6 +61,7 @@ struct tfmm_ctx {
   int64_t ncells = 0, nleaves = 0;
   bool keys_in_alt = false;
   double h_region[6] = {0, 0, 0, 0, 0, 0};
+  double w0 = 0;  // root width
 
   // expansions
   DevBuf M, L;
@@ -72,6 +73,15 @@ struct tfmm_ctx {
   uint32_t n_m2l_exec = 0, n_p2p = 0;
   uint64_t n_m2l_all = 0, n_p2p_calls = 0, n_p2p_self = 0;
   uint32_t dtt_steps = 0;
+
+  // batched M2L (offset groups)
+  DevBuf g_used, g_gid, g_y64, g_y32, g_dl, g_lt, g_cnt;
+  DevBuf gkeys, gvals, gkeys_alt, gvals_alt, fkeys, fvals, fkeys_alt, fvals_alt;
+  DevBuf it_flag, it_excl, it_seg, it_iflag, it_iexcl, items, tile_off, mq, codes;
+  DevBuf fb_off, fb_src, relpos;
+  bool codes_ready = false;
+  uint32_t n_groups = 0, n_grouped = 0, n_fallback = 0, n_items = 0;
+  bool grouped_keys_in_alt = false;
 
   // fields
   DevBuf acc, trg, out;
@@ -326,33 +336,156 @@ int do_dtt(tfmm_ctx *c) {
   return fail(c, TFMM_EHIP, "traversal lists did not converge");
 }
 
+// Group the executed M2L pairs by (target level, source level, offset), build
+// the MFMA work items and the per-target list of the pairs outside the groups.
+int build_m2l_batches(tfmm_ctx *c) {
+  hipStream_t s = c->stream;
+  const Cells k = cells_of(c);
+  const uint32_t n = c->n_m2l_exec;
+  const int max_depth = (int)c->lvl.size() - 2;
+  const int32_t nslots = m2l_group_slots(max_depth);
+  TRY(dev_reserve(c->g_used, (size_t)nslots * 4));
+  TRY(dev_reserve(c->g_gid, (size_t)nslots * 4));
+  TRY(dev_reserve(c->scan_tmp, scan_tmp_bytes(std::max<int64_t>(nslots, n))));
+  TRY(hipMemsetAsync(c->g_used.ptr, 0, (size_t)nslots * 4, s));
+  TRY(launch_m2l_mark_slots(k, dp<int2>(c->m2l_pairs), n, dp<uint32_t>(c->g_used), s));
+  TRY(scan_exclusive_u32(dp<uint32_t>(c->g_used), dp<uint32_t>(c->g_gid), nslots,
+                         c->scan_tmp.ptr, s));
+  TRY(hipMemcpyAsync(c->h_u32, dp<uint32_t>(c->g_gid) + nslots - 1, 4, hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(c->h_u32 + 1, dp<uint32_t>(c->g_used) + nslots - 1, 4,
+                     hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  const uint32_t ng = c->h_u32[0] + c->h_u32[1];
+  c->n_groups = ng;
+  const size_t ngr = std::max<uint32_t>(ng, 1);
+  TRY(dev_reserve(c->g_y64, ngr * 112 * 8));
+  TRY(dev_reserve(c->g_y32, ngr * 112 * 4));
+  TRY(dev_reserve(c->g_dl, ngr));
+  TRY(dev_reserve(c->g_lt, ngr));
+  double w0 = 0;
+  TRY(hipMemcpyAsync(&c->h_u64[2], dp<double>(c->c_width), 8, hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  memcpy(&w0, &c->h_u64[2], 8);
+  c->w0 = w0;
+  TRY(launch_m2l_group_harmonics(dp<uint32_t>(c->g_used), dp<uint32_t>(c->g_gid), nslots, w0,
+                                 dp<double>(c->g_y64), dp<float>(c->g_y32), dp<int8_t>(c->g_dl),
+                                 dp<int8_t>(c->g_lt), s));
+  // split grouped / fallback pairs
+  const size_t nn = (size_t)std::max<uint32_t>(n, 1);
+  TRY(dev_reserve(c->gkeys, nn * 8));
+  TRY(dev_reserve(c->gvals, nn * 4));
+  TRY(dev_reserve(c->gkeys_alt, nn * 8));
+  TRY(dev_reserve(c->gvals_alt, nn * 4));
+  TRY(dev_reserve(c->fkeys, nn * 8));
+  TRY(dev_reserve(c->fvals, nn * 4));
+  TRY(dev_reserve(c->fkeys_alt, nn * 8));
+  TRY(dev_reserve(c->fvals_alt, nn * 4));
+  TRY(dev_reserve(c->g_cnt, 16));
+  TRY(hipMemsetAsync(c->g_cnt.ptr, 0, 16, s));
+  const int gbits = bits_for(ngr);
+  const int sbits = bits_for(c->ncells);
+  TRY(launch_m2l_classify(k, dp<int2>(c->m2l_pairs), n, dp<uint32_t>(c->g_gid), gbits, sbits,
+                          dp<uint64_t>(c->gkeys), dp<uint32_t>(c->gvals), dp<uint64_t>(c->fkeys),
+                          dp<uint32_t>(c->fvals), dp<uint32_t>(c->g_cnt), s));
+  TRY(hipMemcpyAsync(c->h_u32, c->g_cnt.ptr, 8, hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  const uint32_t ngp = c->h_u32[0], nfb = c->h_u32[1];
+  c->n_grouped = ngp;
+  c->n_fallback = nfb;
+  const int32_t tile = m2l_tile_size();
+  const int32_t ntiles = (int32_t)((c->ncells + tile - 1) / tile);
+  const int tbits = bits_for(ntiles);
+  TRY(dev_reserve(c->sort_tmp, radix_tmp_bytes(std::max<uint32_t>(ngp, nfb))));
+  bool alt = false;
+  TRY(radix_sort_pairs(dp<uint64_t>(c->gkeys), dp<uint32_t>(c->gvals), dp<uint64_t>(c->gkeys_alt),
+                       dp<uint32_t>(c->gvals_alt), ngp, 0, tbits + gbits + 7, c->sort_tmp.ptr, s,
+                       &alt));
+  c->grouped_keys_in_alt = alt;
+  const uint64_t *gk = dp<uint64_t>(alt ? c->gkeys_alt : c->gkeys);
+  // work items: runs of <= 16 pairs of one group within one tile
+  const size_t ng1 = (size_t)std::max<uint32_t>(ngp, 1);
+  TRY(dev_reserve(c->it_flag, ng1 * 4));
+  TRY(dev_reserve(c->it_excl, ng1 * 4));
+  TRY(dev_reserve(c->it_seg, ng1 * 4));
+  TRY(dev_reserve(c->it_iflag, ng1 * 4));
+  TRY(dev_reserve(c->it_iexcl, ng1 * 4));
+  TRY(dev_reserve(c->items, ng1 * 16));
+  uint32_t nitems = 0;
+  TRY(launch_m2l_items(gk, ngp, gbits, dp<uint32_t>(c->it_flag), dp<uint32_t>(c->it_excl),
+                       dp<uint32_t>(c->it_seg), dp<uint32_t>(c->it_iflag),
+                       dp<uint32_t>(c->it_iexcl), c->scan_tmp.ptr, c->h_u32, dp<int4>(c->items),
+                       ng1, &nitems, s));
+  c->n_items = nitems;
+  TRY(dev_reserve(c->tile_off, (size_t)(ntiles + 1) * 4));
+  TRY(launch_m2l_tile_offsets(dp<int4>(c->items), nitems, ntiles, dp<int32_t>(c->tile_off), s));
+  // fallback pairs: per-target list sorted by (target, source)
+  bool falt = false;
+  TRY(radix_sort_pairs(dp<uint64_t>(c->fkeys), dp<uint32_t>(c->fvals), dp<uint64_t>(c->fkeys_alt),
+                       dp<uint32_t>(c->fvals_alt), nfb, 0, 2 * sbits, c->sort_tmp.ptr, s, &falt));
+  const uint64_t *fk = dp<uint64_t>(falt ? c->fkeys_alt : c->fkeys);
+  TRY(dev_reserve(c->fb_off, (size_t)(c->ncells + 1) * 4));
+  TRY(dev_reserve(c->fb_src, (size_t)std::max<uint32_t>(nfb, 1) * 4));
+  TRY(launch_csr_offsets(fk, nfb, sbits, (int32_t)c->ncells, dp<int32_t>(c->fb_off), s));
+  TRY(launch_unpack_pairs(fk, nfb, sbits, dp<int32_t>(c->fb_src), s));
+  if (!c->codes_ready) {
+    TRY(dev_reserve(c->codes, 72 * 64 * 4));
+    hipError_t e = m2l_upload_codes(dp<uint32_t>(c->codes), s);
+    if (e == hipErrorInvalidValue) return fail(c, TFMM_EHIP, "M2L operator table is inconsistent");
+    TRY(e);
+    c->codes_ready = true;
+  }
+  return TFMM_OK;
+}
+
 int do_traverse(tfmm_ctx *c) {
   hipStream_t s = c->stream;
   const Cells k = cells_of(c);
+  const bool mixed = c->prm.precision == TFMM_MIXED;
   TRY(hipEventRecord(c->ev[EV_DTT0], s));
   int rc = do_dtt(c);
   if (rc) return rc;
-  rc = sort_list(c, c->m2l_pairs, c->n_m2l_exec, c->m2l_off, c->m2l_src);
-  if (rc) return rc;
   rc = sort_list(c, c->p2p_pairs, c->n_p2p, c->p2p_off, c->p2p_src);
+  if (rc) return rc;
+  rc = build_m2l_batches(c);
   if (rc) return rc;
   TRY(dev_reserve(c->pair_count, 16));
   TRY(hipMemsetAsync(c->pair_count.ptr, 0, 16, s));
   TRY(launch_p2p_pair_count(k, dp<int2>(c->p2p_pairs), c->n_p2p,
                             dp<unsigned long long>(c->pair_count), s));
+  // M~ in the MFMA operand layout
+  TRY(dev_reserve(c->mq, (size_t)c->ncells * 112 * (mixed ? 4 : 8)));
+  TRY(launch_m2l_scale_m(k, (int32_t)c->ncells, dp<double>(c->M), c->w0, c->mq.ptr, mixed, s));
+  if (mixed) {
+    TRY(dev_reserve(c->relpos, (size_t)c->n * 16));
+    TRY(launch_leaf_relpos(k, (int32_t)c->ncells, sorted_bodies(c), dp<float4>(c->relpos), s));
+  }
   TRY(hipEventRecord(c->ev[EV_DTT1], s));
 
   const size_t lbytes = (size_t)c->ncells * NTERM * 16;
   TRY(hipMemsetAsync(c->L.ptr, 0, lbytes, s));
   TRY(hipEventRecord(c->ev[EV_M2L0], s));
-  Csr ml{dp<int32_t>(c->m2l_off), dp<int32_t>(c->m2l_src)};
-  TRY(launch_m2l_fp64(k, (int32_t)c->ncells, ml, dp<double>(c->M), dp<double>(c->L), s));
+  const int32_t tile = m2l_tile_size();
+  const int32_t ntiles = (int32_t)((c->ncells + tile - 1) / tile);
+  const uint64_t *gk = dp<uint64_t>(c->grouped_keys_in_alt ? c->gkeys_alt : c->gkeys);
+  const uint32_t *gv = dp<uint32_t>(c->grouped_keys_in_alt ? c->gvals_alt : c->gvals);
+  TRY(launch_m2l_tiles(dp<int4>(c->items), dp<int32_t>(c->tile_off), ntiles, gk, gv,
+                       mixed ? (const void *)c->g_y32.ptr : (const void *)c->g_y64.ptr,
+                       dp<int8_t>(c->g_dl), c->mq.ptr, dp<uint32_t>(c->codes), k,
+                       (int32_t)c->ncells, c->w0, dp<double>(c->L), mixed, s));
+  if (c->n_fallback) {
+    Csr fl{dp<int32_t>(c->fb_off), dp<int32_t>(c->fb_src)};
+    TRY(launch_m2l_fp64(k, (int32_t)c->ncells, fl, dp<double>(c->M), dp<double>(c->L), s));
+  }
   TRY(hipEventRecord(c->ev[EV_M2L1], s));
 
   TRY(dev_reserve(c->acc, (size_t)c->n * 32));
   TRY(hipEventRecord(c->ev[EV_P2P0], s));
   Csr pl{dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src)};
-  TRY(launch_p2p_fp64(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<double>(c->acc), s));
+  if (mixed)
+    TRY(launch_p2p_fp32(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
+                        dp<double>(c->acc), s));
+  else
+    TRY(launch_p2p_fp64(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<double>(c->acc), s));
   TRY(hipEventRecord(c->ev[EV_P2P1], s));
   TRY(hipMemcpyAsync(c->h_u64, c->pair_count.ptr, 16, hipMemcpyDeviceToHost, s));
   return TFMM_OK;
@@ -421,7 +554,11 @@ void tfmm_destroy(tfmm_ctx *c) {
                     &c->scan_tmp, &c->M, &c->L, &c->front_a, &c->front_b, &c->m2l_pairs,
                     &c->p2p_pairs, &c->cnt, &c->lkeys, &c->lkeys_alt, &c->lidx, &c->lidx_alt,
                     &c->m2l_off, &c->m2l_src, &c->p2p_off, &c->p2p_src, &c->pair_count, &c->acc,
-                    &c->trg, &c->out};
+                    &c->trg, &c->out, &c->g_used, &c->g_gid, &c->g_y64, &c->g_y32, &c->g_dl,
+                    &c->g_lt, &c->g_cnt, &c->gkeys, &c->gvals, &c->gkeys_alt, &c->gvals_alt,
+                    &c->fkeys, &c->fvals, &c->fkeys_alt, &c->fvals_alt, &c->it_flag,
+                    &c->it_excl, &c->it_seg, &c->it_iflag, &c->it_iexcl, &c->items,
+                    &c->tile_off, &c->mq, &c->codes, &c->fb_off, &c->fb_src, &c->relpos};
   for (DevBuf *b : bufs) dev_release(*b);
   for (int i = 0; i < EV_COUNT; i++)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
@@ -579,6 +716,9 @@ int tfmm_get_stats(tfmm_ctx *c, tfmm_stats_t *o) {
     o->n_p2p_leafpairs = c->n_p2p;
     o->n_p2p_calls = c->h_u64[0];
     o->n_p2p_self = c->h_u64[1];
+    o->n_m2l_groups = c->n_groups;
+    o->n_m2l_items = c->n_items;
+    o->n_m2l_fallback = c->n_fallback;
   }
   auto el = [&](int a, int b) {
     float ms = 0;
@@ -650,9 +790,24 @@ int tfmm_get_lists(tfmm_ctx *c, int32_t *m2l_off, int32_t *m2l_src, int32_t *p2p
   if (rc) return rc;
   hipStream_t s = c->stream;
   const size_t no = (size_t)(c->ncells + 1) * 4;
-  if (m2l_off) TRY(hipMemcpyAsync(m2l_off, c->m2l_off.ptr, no, hipMemcpyDeviceToHost, s));
-  if (m2l_src && c->n_m2l_exec)
-    TRY(hipMemcpyAsync(m2l_src, c->m2l_src.ptr, (size_t)c->n_m2l_exec * 4, hipMemcpyDeviceToHost, s));
+  if (m2l_off || m2l_src) {
+    std::vector<int2> pr(c->n_m2l_exec);
+    if (c->n_m2l_exec)
+      TRY(hipMemcpyAsync(pr.data(), c->m2l_pairs.ptr, pr.size() * 8, hipMemcpyDeviceToHost, s));
+    TRY(hipStreamSynchronize(s));
+    std::sort(pr.begin(), pr.end(), [](const int2 &a, const int2 &b) {
+      return a.x != b.x ? a.x < b.x : a.y < b.y;
+    });
+    if (m2l_off) {
+      size_t e = 0;
+      for (int64_t t = 0; t <= c->ncells; t++) {
+        while (e < pr.size() && pr[e].x < t) e++;
+        m2l_off[t] = (int32_t)e;
+      }
+    }
+    if (m2l_src)
+      for (size_t i = 0; i < pr.size(); i++) m2l_src[i] = pr[i].y;
+  }
   if (p2p_off) TRY(hipMemcpyAsync(p2p_off, c->p2p_off.ptr, no, hipMemcpyDeviceToHost, s));
   if (p2p_src && c->n_p2p)
     TRY(hipMemcpyAsync(p2p_src, c->p2p_src.ptr, (size_t)c->n_p2p * 4, hipMemcpyDeviceToHost, s));

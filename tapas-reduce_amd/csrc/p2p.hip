@@ -102,3 +102,131 @@ hipError_t launch_p2p_fp64(Cells c, int32_t ncells, Csr list, Bodies b, double *
 }
 
 }  // namespace tfmm
+
+// ---------------------------------------------------------------------------
+// Mixed precision: FP32 interactions, compensated accumulation.
+// ---------------------------------------------------------------------------
+namespace tfmm {
+namespace {
+
+// body position relative to its leaf's centre (FP32) and charge
+__global__ __launch_bounds__(64) void k_leaf_relpos(Cells c, Bodies b, float4 *rel) {
+  const int32_t ci = blockIdx.x;
+  if (c.child[ci] >= 0) return;
+  const double cx = c.center[3 * (int64_t)ci], cy = c.center[3 * (int64_t)ci + 1],
+               cz = c.center[3 * (int64_t)ci + 2];
+  const int32_t beg = c.beg[ci], end = beg + c.nb[ci];
+  for (int32_t i = beg + threadIdx.x; i < end; i += 64)
+    rel[i] = make_float4((float)(b.x[i] - cx), (float)(b.y[i] - cy), (float)(b.z[i] - cz),
+                         (float)b.q[i]);
+}
+
+// Kahan step: (s, c) += v
+__device__ inline void kahan_add(float &s, float &c, float v) {
+  const float y = v - c;
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+// Target leaf per wave; positions are relative to the own leaf centre, so a
+// source leaf is shifted by the FP64 centre difference (rounded once) and
+// dX = x_i - x_j keeps the FP32 precision of the leaf size, not of the
+// coordinates.  Each source leaf's contribution is summed in FP32 and then
+// added to the running per-target sums with Kahan compensation.
+__global__ __launch_bounds__(64) void k_p2p_fp32(Cells c, Csr list, Bodies b,
+                                                const float4 *rel, double *acc) {
+  const int32_t ti = blockIdx.x;
+  if (c.child[ti] >= 0) return;
+  const int32_t nt = c.nb[ti];
+  if (nt == 0) return;
+  const int32_t tb = c.beg[ti];
+  const int lane = threadIdx.x;
+  __shared__ float4 sb[P2P_CHUNK];
+  const int32_t lb = list.off[ti], le = list.off[ti + 1];
+  const double tcx = c.center[3 * (int64_t)ti], tcy = c.center[3 * (int64_t)ti + 1],
+               tcz = c.center[3 * (int64_t)ti + 2];
+  int lg = 0;
+  while (lg < 6 && (nt << (lg + 1)) <= 64) lg++;
+  const int g = 1 << lg;
+  for (int32_t t0 = 0; t0 < nt; t0 += (64 >> lg)) {
+    const int32_t tl = t0 + (lane >> lg);
+    const int sub = lane & (g - 1);
+    const bool tvalid = tl < nt;
+    float4 xi = make_float4(0, 0, 0, 0);
+    if (tvalid) xi = rel[tb + tl];
+    float s0 = 0, s1 = 0, s2 = 0, s3 = 0, c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+    for (int32_t s = lb; s < le; s++) {
+      const int32_t sj = list.src[s];
+      const int32_t sbeg = c.beg[sj], ns = c.nb[sj];
+      const float ox = (float)(tcx - c.center[3 * (int64_t)sj]);
+      const float oy = (float)(tcy - c.center[3 * (int64_t)sj + 1]);
+      const float oz = (float)(tcz - c.center[3 * (int64_t)sj + 2]);
+      for (int32_t c0i = 0; c0i < ns; c0i += P2P_CHUNK) {
+        const int32_t cn = min(P2P_CHUNK, ns - c0i);
+        __syncthreads();
+        if (lane < cn) {
+          float4 v = rel[sbeg + c0i + lane];
+          v.x -= ox;
+          v.y -= oy;
+          v.z -= oz;
+          sb[lane] = v;
+        }
+        __syncthreads();
+        float p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+        if (tvalid) {
+          for (int jj = sub; jj < cn; jj += g) {
+            const float4 xj = sb[jj];
+            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const float rinv = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;
+            const float qr = xj.w * rinv;
+            const float f = qr * rinv * rinv;
+            p0 += qr;
+            p1 -= dx * f;
+            p2 -= dy * f;
+            p3 -= dz * f;
+          }
+        }
+        kahan_add(s0, c0, p0);
+        kahan_add(s1, c1, p1);
+        kahan_add(s2, c2, p2);
+        kahan_add(s3, c3, p3);
+      }
+    }
+    double d0 = (double)s0 - c0, d1 = (double)s1 - c1, d2 = (double)s2 - c2, d3 = (double)s3 - c3;
+    for (int o = 1; o < g; o <<= 1) {
+      d0 += __shfl_xor(d0, o, 64);
+      d1 += __shfl_xor(d1, o, 64);
+      d2 += __shfl_xor(d2, o, 64);
+      d3 += __shfl_xor(d3, o, 64);
+    }
+    if (tvalid && sub == 0) {
+      // the FP64 path stores q_i * sum (the reference's P2P sum, divided by
+      // q_i in L2P); keep that convention
+      const double qi = b.q[tb + tl];
+      double *o = acc + 4 * (int64_t)(tb + tl);
+      o[0] = d0 * qi;
+      o[1] = d1 * qi;
+      o[2] = d2 * qi;
+      o[3] = d3 * qi;
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t launch_leaf_relpos(Cells c, int32_t ncells, Bodies b, float4 *rel, hipStream_t s) {
+  if (ncells <= 0) return hipSuccess;
+  k_leaf_relpos<<<ncells, 64, 0, s>>>(c, b, rel);
+  return hipGetLastError();
+}
+
+hipError_t launch_p2p_fp32(Cells c, int32_t ncells, Csr list, Bodies b, const float4 *rel,
+                           double *acc, hipStream_t s) {
+  if (ncells <= 0) return hipSuccess;
+  k_p2p_fp32<<<ncells, 64, 0, s>>>(c, list, b, rel, acc);
+  return hipGetLastError();
+}
+
+}  // namespace tfmm

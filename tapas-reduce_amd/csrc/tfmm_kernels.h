@@ -88,3 +88,36 @@ hipError_t launch_direct_fp64(const double *x, const double *y, const double *z,
                               double *out, hipStream_t s);
 
 }  // namespace tfmm
+
+namespace tfmm {
+// m2l_batched.hip
+int32_t m2l_group_slots(int max_depth);
+int m2l_tile_size();
+hipError_t m2l_upload_codes(uint32_t *dev_codes, hipStream_t s);
+hipError_t launch_m2l_mark_slots(Cells c, const int2 *pairs, uint32_t n, uint32_t *used,
+                                 hipStream_t s);
+hipError_t launch_m2l_classify(Cells c, const int2 *pairs, uint32_t n,
+                               const uint32_t *gid_of_slot, int gbits, int sbits,
+                               uint64_t *gkeys, uint32_t *gvals, uint64_t *fkeys,
+                               uint32_t *fvals, uint32_t *counters, hipStream_t s);
+hipError_t launch_m2l_group_harmonics(const uint32_t *used, const uint32_t *gid_of_slot,
+                                      int32_t nslots, double w0, double *Y64, float *Y32,
+                                      int8_t *gdl, int8_t *glt, hipStream_t s);
+hipError_t launch_m2l_scale_m(Cells c, int32_t ncells, const double *M, double w0, void *Mq,
+                              bool fp32, hipStream_t s);
+hipError_t launch_m2l_items(const uint64_t *keys, uint32_t n, int gbits, uint32_t *flag,
+                            uint32_t *excl, uint32_t *segstart, uint32_t *iflag,
+                            uint32_t *iexcl, void *scan_tmp, uint32_t *h_pinned, int4 *items,
+                            size_t items_cap, uint32_t *nitems_out, hipStream_t s);
+hipError_t launch_m2l_tile_offsets(const int4 *items, uint32_t nitems, int32_t ntiles,
+                                   int32_t *off, hipStream_t s);
+hipError_t launch_m2l_tiles(const int4 *items, const int32_t *tile_off, int32_t ntiles,
+                            const uint64_t *keys, const uint32_t *src, const void *Yg,
+                            const int8_t *gdl, const void *Mq, const uint32_t *codes, Cells c,
+                            int32_t ncells, double w0, double *L, bool fp32, hipStream_t s);
+
+// p2p.hip (mixed precision)
+hipError_t launch_leaf_relpos(Cells c, int32_t ncells, Bodies b, float4 *rel, hipStream_t s);
+hipError_t launch_p2p_fp32(Cells c, int32_t ncells, Csr list, Bodies b, const float4 *rel,
+                           double *acc, hipStream_t s);
+}  // namespace tfmm

@@ -198,3 +198,39 @@ def test_gpu_phases_in_order_and_errors(ctx):
     o = oracle.Run(x, y, z, q, 16, 0.4)
     ep, ea = oracle.rel_l2(ctx.fields(), o.fields())
     assert ep < TOL and ea < TOL
+
+
+# ---------------------------------------------------------------------------
+# mixed precision (FP32 P2P with compensated sums, FP32 MFMA M2L)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("dist,n,ncrit", [("plummer", 200000, 64), ("cube", 100000, 32),
+                                          ("two_plummer", 150000, 64)])
+def test_gpu_mixed_error_against_direct(ctx, dist, n, ncrit):
+    """Mixed precision within rel-L2 1e-4 of the FP64 direct sum, and within
+    1e-5 of the FP64 FMM (the part of the error the reduced precision adds)."""
+    x, y, z, q = datasets.make(dist, n, seed=21)
+    f64, st64 = evaluate(ctx, x, y, z, q, ncrit, 0.4, tfmm.FP64)
+    f32, st32 = evaluate(ctx, x, y, z, q, ncrit, 0.4, tfmm.MIXED)
+    assert st32["n_m2l"] == st64["n_m2l"] and st32["n_p2p_calls"] == st64["n_p2p_calls"]
+    t = np.sort(np.random.default_rng(1).choice(n, 2048, replace=False))
+    d = ctx.direct(t)
+    e64 = oracle.rel_l2(f64[t], d)
+    e32 = oracle.rel_l2(f32[t], d)
+    emix = oracle.rel_l2(f32, f64)
+    print("fp64 vs direct", e64, "mixed vs direct", e32, "mixed vs fp64", emix)
+    assert max(e32) < 1e-4
+    assert max(emix) < 1e-5
+
+
+def test_gpu_m2l_groups_cover_pairs(ctx):
+    """Every executed M2L pair is either in an offset group or in the per-pair
+    fallback list; uniform trees need no fallback."""
+    x, y, z, q = datasets.cube(1 << 17, seed=3)
+    _, st = evaluate(ctx, x, y, z, q, 64, 0.4)
+    assert st["n_m2l_fallback"] == 0
+    assert st["n_m2l_items"] * 16 >= st["n_m2l_exec"]
+    x, y, z, q = datasets.plummer(1 << 17, seed=3)
+    _, st = evaluate(ctx, x, y, z, q, 64, 0.4)
+    assert st["n_m2l_items"] * 16 + st["n_m2l_fallback"] >= st["n_m2l_exec"]
+    assert st["n_m2l_fallback"] < 0.01 * st["n_m2l_exec"]
