@@ -112,6 +112,10 @@ int tfmm_set_bodies_device(tfmm_ctx *ctx, int64_t n, const double *x, const doub
                            const double *z, const double *q);
 
 int tfmm_build_tree(tfmm_ctx *ctx, const tfmm_params *params);
+/* Change theta or the precision of the following phases without rebuilding
+ * the tree (the reference passes theta to Map(FMM_DTT, ...), not to
+ * Partition).  A different ncrit is rejected: it changes the tree. */
+int tfmm_set_params(tfmm_ctx *ctx, const tfmm_params *params);
 int tfmm_upward(tfmm_ctx *ctx);
 int tfmm_traverse(tfmm_ctx *ctx);
 int tfmm_downward(tfmm_ctx *ctx);

@@ -648,6 +648,17 @@ int tfmm_build_tree(tfmm_ctx *c, const tfmm_params *p) {
   return TFMM_OK;
 }
 
+int tfmm_set_params(tfmm_ctx *c, const tfmm_params *p) {
+  if (!c) return TFMM_EINVAL;
+  int rc = check_params(c, p);
+  if (rc) return rc;
+  if (c->stage >= 1 && p->ncrit != c->prm.ncrit)
+    return fail(c, TFMM_EINVAL, "ncrit changes the tree: call tfmm_build_tree");
+  c->prm = *p;
+  if (c->stage > 2) c->stage = 2;  // the traversal has to be redone
+  return TFMM_OK;
+}
+
 int tfmm_upward(tfmm_ctx *c) {
   int rc = need_stage(c, 1, "tfmm_upward");
   if (rc) return rc;

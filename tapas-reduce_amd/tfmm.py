@@ -62,7 +62,8 @@ class Stats(ctypes.Structure):
 
 EXPORTED = [
     "tfmm_create", "tfmm_destroy", "tfmm_last_error", "tfmm_stream", "tfmm_sync",
-    "tfmm_set_bodies", "tfmm_set_bodies_device", "tfmm_build_tree", "tfmm_upward",
+    "tfmm_set_bodies", "tfmm_set_bodies_device", "tfmm_build_tree", "tfmm_set_params",
+    "tfmm_upward",
     "tfmm_traverse", "tfmm_downward", "tfmm_evaluate", "tfmm_get_fields", "tfmm_get_stats",
     "tfmm_get_sorted", "tfmm_get_cells", "tfmm_get_expansions", "tfmm_get_lists", "tfmm_direct",
 ]
@@ -88,6 +89,7 @@ def lib():
         L.tfmm_set_bodies_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp]
         L.tfmm_build_tree.argtypes = [vp, ctypes.POINTER(Params)]
         L.tfmm_evaluate.argtypes = [vp, ctypes.POINTER(Params)]
+        L.tfmm_set_params.argtypes = [vp, ctypes.POINTER(Params)]
         for f in ("tfmm_upward", "tfmm_traverse", "tfmm_downward"):
             getattr(L, f).argtypes = [vp]
         L.tfmm_get_fields.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
@@ -175,6 +177,9 @@ class Context:
     # -- phases ----------------------------------------------------------
     def build_tree(self, params):
         self._check(lib().tfmm_build_tree(self._h, ctypes.byref(params)), "tfmm_build_tree")
+
+    def set_params(self, params):
+        self._check(lib().tfmm_set_params(self._h, ctypes.byref(params)), "tfmm_set_params")
 
     def upward(self):
         self._check(lib().tfmm_upward(self._h), "tfmm_upward")
