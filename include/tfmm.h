@@ -146,6 +146,44 @@ int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p
  * order), normalised by the target charge; out: nt * 4 doubles (host). */
 int tfmm_direct(tfmm_ctx *ctx, const int64_t *targets, int64_t nt, double *out);
 
+/* ---------------------------------------------------------------------------
+ * Learned parameter picker.  No counterpart in the reference (its ncrit and
+ * theta are fixed command-line values, EX/include/args.h:101-158): an FP32
+ * MLP, 16 tree statistics -> 64 -> 64 (ReLU) -> 5 logits over ncrit in
+ * {16, 32, 64, 128, 256} and theta = 0.2 + 0.4 sigmoid(o5), trained and
+ * evaluated by the library's own GPU kernels.
+ * ------------------------------------------------------------------------- */
+typedef struct tfmm_picker tfmm_picker;
+
+int64_t tfmm_picker_nparam(void);
+/* Glorot-uniform initial weights from the seed */
+int tfmm_picker_create(int hip_device, uint64_t seed, tfmm_picker **out);
+void tfmm_picker_destroy(tfmm_picker *p);
+const char *tfmm_picker_last_error(const tfmm_picker *p);
+int tfmm_picker_set_weights(tfmm_picker *p, const float *w, int64_t n);
+int tfmm_picker_get_weights(tfmm_picker *p, float *w, int64_t n);
+/* Full-batch Adam on n samples (16 features each, ncrit class 0..4, theta in
+ * [0.2, 0.6]); loss_per_step (optional) receives the mean loss of each step. */
+int tfmm_picker_train(tfmm_picker *p, const float *X, const int32_t *cls, const float *theta,
+                      int n, int steps, float lr, float *loss_per_step);
+/* Mean gradient of the loss over the batch (tfmm_picker_nparam() floats) and
+ * per-sample losses; the weights are not changed. */
+int tfmm_picker_gradient(tfmm_picker *p, const float *X, const int32_t *cls, const float *theta,
+                         int n, float *grad, float *loss);
+/* Raw network outputs (6 per sample) */
+int tfmm_picker_predict(tfmm_picker *p, const float *X, int n, float *out);
+
+/* The 16 tree statistics of the built tree: log2 N / 23, max depth / 20,
+ * fraction of non-empty leaves per depth bin (<=3, 4..9, >=10), mean, spread
+ * and third moment of leaf occupancy nb / ncrit, fraction of empty leaves,
+ * mean leaf depth / 20, fraction of bodies in leaves at depth >= 8. */
+int tfmm_tree_features(tfmm_ctx *ctx, float *feat16);
+/* Pick ncrit and theta for the current bodies: builds a trial tree at
+ * ncrit = 64, reads its statistics and runs the picker's network; with a null
+ * picker (or a non-finite output) the fallback ncrit = 64, theta = 0.4 is
+ * returned.  The precision field of *io is kept. */
+int tfmm_pick_params(tfmm_ctx *ctx, tfmm_picker *picker, tfmm_params *io);
+
 #ifdef __cplusplus
 }
 #endif

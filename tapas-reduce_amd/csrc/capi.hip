@@ -83,6 +83,9 @@ struct tfmm_ctx {
   uint32_t n_groups = 0, n_grouped = 0, n_fallback = 0, n_items = 0;
   bool grouped_keys_in_alt = false;
 
+  // picker
+  DevBuf feat_acc, feat_x, feat_out;
+
   // fields
   DevBuf acc, trg, out;
 
@@ -558,7 +561,8 @@ void tfmm_destroy(tfmm_ctx *c) {
                     &c->g_lt, &c->g_cnt, &c->gkeys, &c->gvals, &c->gkeys_alt, &c->gvals_alt,
                     &c->fkeys, &c->fvals, &c->fkeys_alt, &c->fvals_alt, &c->it_flag,
                     &c->it_excl, &c->it_seg, &c->it_iflag, &c->it_iexcl, &c->items,
-                    &c->tile_off, &c->mq, &c->codes, &c->fb_off, &c->fb_src, &c->relpos};
+                    &c->tile_off, &c->mq, &c->codes, &c->fb_off, &c->fb_src, &c->relpos,
+                    &c->feat_acc, &c->feat_x, &c->feat_out};
   for (DevBuf *b : bufs) dev_release(*b);
   for (int i = 0; i < EV_COUNT; i++)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
@@ -845,6 +849,68 @@ int tfmm_direct(tfmm_ctx *c, const int64_t *targets, int64_t nt, double *out) {
   dev_release(t);
   dev_release(o);
   if (e != hipSuccess) return hipfail(c, e, "tfmm_direct");
+  return TFMM_OK;
+}
+
+int tfmm_tree_features(tfmm_ctx *c, float *feat) {
+  int rc = need_stage(c, 1, "tfmm_tree_features");
+  if (rc) return rc;
+  if (!feat) return fail(c, TFMM_EINVAL, "null feature array");
+  hipStream_t s = c->stream;
+  TRY(dev_reserve(c->feat_acc, 16 * 8));
+  TRY(hipMemsetAsync(c->feat_acc.ptr, 0, 16 * 8, s));
+  TRY(launch_tree_stats(cells_of(c), (int32_t)c->ncells, c->prm.ncrit, dp<double>(c->feat_acc), s));
+  double a[16];
+  TRY(hipMemcpyAsync(a, c->feat_acc.ptr, 16 * 8, hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  const double nl = a[8] > 0 ? a[8] : 1;
+  const double mean = a[10] / nl;
+  const double var = std::max(0.0, a[11] / nl - mean * mean);
+  feat[0] = (float)(std::log2((double)c->n) / 23.0);
+  feat[1] = (float)(((int)c->lvl.size() - 2) / 20.0);
+  for (int b = 0; b < 8; b++) feat[2 + b] = (float)(a[b] / nl);
+  feat[10] = (float)mean;
+  feat[11] = (float)std::sqrt(var);
+  feat[12] = (float)(a[12] / nl);
+  feat[13] = (float)(a[9] / std::max(1.0, a[8] + a[9]));
+  feat[14] = (float)(a[13] / nl / 20.0);
+  feat[15] = (float)(a[14] / (double)c->n);
+  return TFMM_OK;
+}
+
+int tfmm_pick_params(tfmm_ctx *c, tfmm_picker *picker, tfmm_params *io) {
+  if (!c) return TFMM_EINVAL;
+  if (!io) return fail(c, TFMM_EINVAL, "null params");
+  static const int32_t kNcrit[5] = {16, 32, 64, 128, 256};
+  io->ncrit = 64;  // fallback
+  io->theta = 0.4;
+  io->mutual = 0;
+  if (!picker) return TFMM_OK;
+  if (c->n < 1) return fail(c, TFMM_ESTATE, "no bodies set");
+  if (c->stage < 1 || c->prm.ncrit != 64) {
+    tfmm_params trial = *io;
+    int rc = tfmm_build_tree(c, &trial);
+    if (rc) return rc;
+  }
+  float feat[16];
+  int rc = tfmm_tree_features(c, feat);
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  TRY(dev_reserve(c->feat_x, 16 * 4));
+  TRY(dev_reserve(c->feat_out, 8 * 4));
+  TRY(hipMemcpyAsync(c->feat_x.ptr, feat, 64, hipMemcpyHostToDevice, s));
+  TRY(launch_mlp_forward(picker_weights(picker), dp<float>(c->feat_x), 1, dp<float>(c->feat_out), s));
+  float o[6];
+  TRY(hipMemcpyAsync(o, c->feat_out.ptr, 24, hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  int best = -1;
+  for (int i = 0; i < 5; i++) {
+    if (!std::isfinite(o[i])) return TFMM_OK;  // fallback stays
+    if (best < 0 || o[i] > o[best]) best = i;
+  }
+  if (!std::isfinite(o[5])) return TFMM_OK;
+  io->ncrit = kNcrit[best];
+  io->theta = 0.2 + 0.4 / (1.0 + std::exp(-(double)o[5]));
   return TFMM_OK;
 }
 

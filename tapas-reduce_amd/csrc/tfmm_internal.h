@@ -37,3 +37,9 @@ hipError_t radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
                             hipStream_t s, bool *result_in_alt);
 
 }  // namespace tfmm
+
+struct tfmm_picker;
+namespace tfmm {
+// device weights of a picker (null for a null picker)
+const float *picker_weights(tfmm_picker *p);
+}  // namespace tfmm

@@ -121,3 +121,14 @@ hipError_t launch_leaf_relpos(Cells c, int32_t ncells, Bodies b, float4 *rel, hi
 hipError_t launch_p2p_fp32(Cells c, int32_t ncells, Csr list, Bodies b, const float4 *rel,
                            double *acc, hipStream_t s);
 }  // namespace tfmm
+
+namespace tfmm {
+// mlp.hip
+int mlp_nparam();
+hipError_t launch_mlp_grad(const float *W, const float *X, const int32_t *cls,
+                           const float *theta, int n, float *G, float *loss, hipStream_t s);
+hipError_t launch_mlp_adam(float *W, const float *G, int n, float *m, float *v, float lr,
+                           int step, float *gmean, hipStream_t s);
+hipError_t launch_mlp_forward(const float *W, const float *X, int n, float *out, hipStream_t s);
+hipError_t launch_tree_stats(Cells c, int32_t ncells, int ncrit, double *acc, hipStream_t s);
+}  // namespace tfmm

@@ -66,7 +66,12 @@ EXPORTED = [
     "tfmm_upward",
     "tfmm_traverse", "tfmm_downward", "tfmm_evaluate", "tfmm_get_fields", "tfmm_get_stats",
     "tfmm_get_sorted", "tfmm_get_cells", "tfmm_get_expansions", "tfmm_get_lists", "tfmm_direct",
+    "tfmm_picker_nparam", "tfmm_picker_create", "tfmm_picker_destroy", "tfmm_picker_last_error",
+    "tfmm_picker_set_weights", "tfmm_picker_get_weights", "tfmm_picker_train",
+    "tfmm_picker_gradient", "tfmm_picker_predict", "tfmm_tree_features", "tfmm_pick_params",
 ]
+
+NCRIT_CLASSES = (16, 32, 64, 128, 256)
 
 
 def lib():
@@ -99,6 +104,21 @@ def lib():
         L.tfmm_get_expansions.argtypes = [vp, vp, vp]
         L.tfmm_get_lists.argtypes = [vp, vp, vp, vp, vp]
         L.tfmm_direct.argtypes = [vp, vp, ctypes.c_int64, vp]
+        L.tfmm_picker_nparam.restype = ctypes.c_int64
+        L.tfmm_picker_nparam.argtypes = []
+        L.tfmm_picker_create.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(vp)]
+        L.tfmm_picker_destroy.argtypes = [vp]
+        L.tfmm_picker_destroy.restype = None
+        L.tfmm_picker_last_error.argtypes = [vp]
+        L.tfmm_picker_last_error.restype = ctypes.c_char_p
+        L.tfmm_picker_set_weights.argtypes = [vp, vp, ctypes.c_int64]
+        L.tfmm_picker_get_weights.argtypes = [vp, vp, ctypes.c_int64]
+        L.tfmm_picker_train.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                        vp]
+        L.tfmm_picker_gradient.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp, vp]
+        L.tfmm_picker_predict.argtypes = [vp, vp, ctypes.c_int, vp]
+        L.tfmm_tree_features.argtypes = [vp, vp]
+        L.tfmm_pick_params.argtypes = [vp, vp, ctypes.POINTER(Params)]
         _lib = L
     return _lib
 
@@ -243,11 +263,99 @@ class Context:
                     "tfmm_get_lists")
         return (mo, ms[:st["n_m2l_exec"]]), (po, ps[:st["n_p2p_leafpairs"]])
 
+    def tree_features(self):
+        f = np.zeros(16, np.float32)
+        self._check(lib().tfmm_tree_features(self._h, _ptr(f)), "tfmm_tree_features")
+        return f
+
+    def pick_params(self, picker=None, precision=FP64):
+        p = make_params(64, 0.4, precision)
+        self._check(lib().tfmm_pick_params(self._h, None if picker is None else picker.handle,
+                                           ctypes.byref(p)), "tfmm_pick_params")
+        return p
+
     def direct(self, targets):
         t = np.ascontiguousarray(targets, np.int64)
         out = np.zeros((len(t), 4), np.float64)
         self._check(lib().tfmm_direct(self._h, _ptr(t), len(t), _ptr(out)), "tfmm_direct")
         return out
+
+
+class Picker:
+    """The learned (ncrit, theta) picker (an FP32 MLP trained on the GPU)."""
+
+    def __init__(self, device=0, seed=0):
+        h = ctypes.c_void_p()
+        rc = lib().tfmm_picker_create(int(device), int(seed), ctypes.byref(h))
+        if rc != TFMM_OK:
+            raise TfmmError("tfmm_picker_create failed with status %d" % rc)
+        self._h = h
+        self.nparam = int(lib().tfmm_picker_nparam())
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().tfmm_picker_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != TFMM_OK:
+            msg = lib().tfmm_picker_last_error(self._h)
+            raise TfmmError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def weights(self):
+        w = np.zeros(self.nparam, np.float32)
+        self._check(lib().tfmm_picker_get_weights(self._h, _ptr(w), self.nparam), "get_weights")
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._check(lib().tfmm_picker_set_weights(self._h, _ptr(w), len(w)), "set_weights")
+
+    @staticmethod
+    def _batch(X, cls, theta):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, 16)
+        c = None if cls is None else np.ascontiguousarray(cls, np.int32)
+        t = None if theta is None else np.ascontiguousarray(theta, np.float32)
+        return X, c, t
+
+    def train(self, X, cls, theta, steps, lr=1e-2):
+        X, c, t = self._batch(X, cls, theta)
+        loss = np.zeros(steps, np.float32)
+        self._check(lib().tfmm_picker_train(self._h, _ptr(X), _ptr(c), _ptr(t), len(X), int(steps),
+                                            float(lr), _ptr(loss)), "train")
+        return loss
+
+    def gradient(self, X, cls, theta):
+        X, c, t = self._batch(X, cls, theta)
+        g = np.zeros(self.nparam, np.float32)
+        loss = np.zeros(len(X), np.float32)
+        self._check(lib().tfmm_picker_gradient(self._h, _ptr(X), _ptr(c), _ptr(t), len(X), _ptr(g),
+                                               _ptr(loss)), "gradient")
+        return g, loss
+
+    def predict(self, X):
+        X, _, _ = self._batch(X, None, None)
+        out = np.zeros((len(X), 6), np.float32)
+        self._check(lib().tfmm_picker_predict(self._h, _ptr(X), len(X), _ptr(out)), "predict")
+        return out
+
+
+def decode(out):
+    """(ncrit, theta) from raw picker outputs."""
+    out = np.atleast_2d(out)
+    cls = np.argmax(out[:, :5], axis=1)
+    theta = 0.2 + 0.4 / (1.0 + np.exp(-out[:, 5].astype(np.float64)))
+    return np.array(NCRIT_CLASSES)[cls], theta
 
 
 def evaluate(x, y, z, q, ncrit=64, theta=0.4, precision=FP64, device=0):
