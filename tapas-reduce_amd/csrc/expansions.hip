@@ -42,10 +42,10 @@ __global__ __launch_bounds__(64) void k_p2m(Cells c, Bodies b, double *M) {
       dz = b.z[i] - cz;
       q = b.q[i];
     }
-    double rho, alpha, beta;
-    cart2sph(dx, dy, dz, rho, alpha, beta);
-    const cd ei = mk(cos(beta), sin(beta));
-    multipole_all<false>(rho, cos(alpha), sin(alpha), ei, [&](int n, int m, cd Y, cd) {
+    double rho, ca, sa;
+    cd ei;
+    sph_frame(dx, dy, dz, rho, ca, sa, ei);
+    multipole_all<false>(rho, ca, sa, ei, [&](int n, int m, cd Y, cd) {
       const double re = wave_sum(q * Y.re);
       const double im = wave_sum(-q * Y.im);
       if (lane == tri(n, m)) {
@@ -73,10 +73,10 @@ __global__ __launch_bounds__(64) void k_m2m(Cells c, int32_t lbeg, double *M) {
     const double dx = c.center[3 * (int64_t)p] - c.center[3 * (int64_t)ch];
     const double dy = c.center[3 * (int64_t)p + 1] - c.center[3 * (int64_t)ch + 1];
     const double dz = c.center[3 * (int64_t)p + 2] - c.center[3 * (int64_t)ch + 2];
-    double rho, alpha, beta;
-    cart2sph(dx, dy, dz, rho, alpha, beta);
-    const cd ei = mk(cos(beta), sin(beta));
-    multipole_all<false>(rho, cos(alpha), sin(alpha), ei,
+    double rho, ca, sa;
+    cd ei;
+    sph_frame(dx, dy, dz, rho, ca, sa, ei);
+    multipole_all<false>(rho, ca, sa, ei,
                          [&](int n, int m, cd Y, cd) { Ys[lane][tri(n, m)] = Y; });
   }
   __syncthreads();
@@ -123,10 +123,10 @@ __global__ __launch_bounds__(64) void k_l2l(Cells c, int32_t lbeg, double *L) {
     const double dx = c.center[3 * (int64_t)ch] - c.center[3 * (int64_t)p];
     const double dy = c.center[3 * (int64_t)ch + 1] - c.center[3 * (int64_t)p + 1];
     const double dz = c.center[3 * (int64_t)ch + 2] - c.center[3 * (int64_t)p + 2];
-    double rho, alpha, beta;
-    cart2sph(dx, dy, dz, rho, alpha, beta);
-    const cd ei = mk(cos(beta), sin(beta));
-    multipole_all<false>(rho, cos(alpha), sin(alpha), ei,
+    double rho, ca, sa;
+    cd ei;
+    sph_frame(dx, dy, dz, rho, ca, sa, ei);
+    multipole_all<false>(rho, ca, sa, ei,
                          [&](int n, int m, cd Y, cd) { Ys[lane][tri(n, m)] = Y; });
   }
   __syncthreads();
@@ -168,13 +168,13 @@ __global__ __launch_bounds__(64) void k_l2p(Cells c, Bodies b, const double *L,
   for (int32_t i = beg + lane; i < end; i += 64) {
     const double dx = b.x[i] - cx, dy = b.y[i] - cy, dz = b.z[i] - cz;
     const double q = b.q[i];
-    double r, theta, phi;
-    cart2sph(dx, dy, dz, r, theta, phi);
-    const cd ei = mk(cos(phi), sin(phi));
+    double r, ct, st;
+    cd ei;
+    sph_frame(dx, dy, dz, r, ct, st, ei);
     double t0 = p2p[4 * (int64_t)i] / q, t1 = p2p[4 * (int64_t)i + 1] / q,
            t2 = p2p[4 * (int64_t)i + 2] / q, t3 = p2p[4 * (int64_t)i + 3] / q;
     double s0 = 0, s1 = 0, s2 = 0;
-    multipole_all<true>(r, cos(theta), sin(theta), ei, [&](int n, int m, cd Y, cd YT) {
+    multipole_all<true>(r, ct, st, ei, [&](int n, int m, cd Y, cd YT) {
       const cd l = Ls[tri(n, m)];
       const cd ly = cmul(l, Y);
       const double lyt = cmul(l, YT).re;
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(64) void k_l2p(Cells c, Bodies b, const double *L,
         s2 += 2 * (-ly.im) * m;
       }
     });
-    const double st = sin(theta), ct = cos(theta), sp = sin(phi), cp = cos(phi);
+    const double sp = ei.im, cp = ei.re;
     t1 += st * cp * s0 + ct * cp / r * s1 - sp / r / st * s2;
     t2 += st * sp * s0 + ct * sp / r * s1 + cp / r / st * s2;
     t3 += ct * s0 - st / r * s1;

@@ -40,10 +40,10 @@ __global__ __launch_bounds__(64) void k_m2l_fp64(Cells c, Csr list, const double
       const double dx = cx - c.center[3 * (int64_t)sj];
       const double dy = cy - c.center[3 * (int64_t)sj + 1];
       const double dz = cz - c.center[3 * (int64_t)sj + 2];
-      double rho, alpha, beta;
-      cart2sph(dx, dy, dz, rho, alpha, beta);
-      const cd ei = mk(cos(beta), sin(beta));
-      local_column(lane, rho, cos(alpha), sin(alpha), ei,
+      double rho, ca, sa;
+      cd ei;
+      sph_frame(dx, dy, dz, rho, ca, sa, ei);
+      local_column(lane, rho, ca, sa, ei,
                    [&](int n, int m, cd Y) { Ys[tri(n, m)] = Y; });
     }
     __syncthreads();

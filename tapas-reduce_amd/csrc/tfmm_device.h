@@ -104,37 +104,28 @@ __device__ inline void cart2sph(double dx, double dy, double dz, double &r, doub
   phi = atan2(dy, dx);
 }
 
-// Column m of the regular solid harmonics r^n Y_n^m (n = m..P-1) and of their
-// theta derivative, by the same recurrences as evalMultipole
-// (LaplaceSphericalCPU_tapas.h:63-99) restarted at column m, so that any
-// lane can produce one column independently.  Results go to Y[tri(n,m)]
-// (and T[tri(n,m)] when T is not null).
-template <class Store>
-__device__ inline void multipole_column(int m, double rho, double x, double y, cd ei,
-                                        Store store) {
-  double fact = 1, pn = 1, rhom = 1;
-  cd eim = mk(1.0, 0.0);
-  for (int i = 0; i < m; i++) {
-    rhom *= rho;
-    rhom /= -(2 * i + 2) * (2 * i + 1);
-    pn = -pn * fact * y;
-    fact += 2;
-    eim = cmul(eim, ei);
+// What the harmonic recurrences actually consume -- cos(theta), sin(theta)
+// and e^{i phi} -- formed without inverse trigonometry:
+//   cos(theta) = z / r,  sin(theta) = r_xy / r,  e^{i phi} = (x + i y) / r_xy.
+// The reference forms theta = acos(z/r), phi = atan2(y, x) and then takes
+// cos/sin of them again; the values agree to rounding.  Degenerate cases
+// follow the reference: r = 0 gives theta = 0 (cos 1, sin 0) and r_xy = 0
+// gives phi = atan2(0, x) (e^{i phi} = +-1).
+__device__ inline void sph_frame(double dx, double dy, double dz, double &r, double &ct,
+                                 double &st, cd &eip) {
+  const double rxy2 = dx * dx + dy * dy;
+  r = sqrt(rxy2 + dz * dz);
+  if (r == 0) {
+    ct = 1;
+    st = 0;
+    eip = mk(1, 0);
+    return;
   }
-  double p = pn;
-  double p1 = p;
-  store(m, m, cscale(eim, rhom * p), cscale(eim, rhom * (x * (2 * m + 1) * p1 - (m + 1) * x * p1) / y));
-  p = x * (2 * m + 1) * p1;
-  double rhon = rhom * rho;
-  for (int n = m + 1; n < P; n++) {
-    rhon /= -(n + m);
-    const double p2 = p1;
-    p1 = p;
-    const cd yv = cscale(eim, rhon * p);
-    p = (x * (2 * n + 1) * p1 - (n + m) * p2) / (n - m + 1);
-    store(n, m, yv, cscale(eim, rhon * ((n - m + 1) * p - (n + 1) * x * p1) / y));
-    rhon *= rho;
-  }
+  ct = dz / r;
+  const double rxy = sqrt(rxy2);
+  st = rxy / r;
+  if (rxy > 0) eip = mk(dx / rxy, dy / rxy);
+  else eip = mk(dx < 0 ? -1.0 : 1.0, 0.0);
 }
 
 // Column m of the singular solid harmonics r^{-n-1} Y_n^m (evalLocal,
