@@ -33,7 +33,13 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=4194304)
     ap.add_argument("--dist", default="plummer")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--picker", default=os.path.join(ROOT, "tapas-reduce_amd", "picker",
+                                                     "weights.npz"),
+                    help="trained picker weights; the fallback ncrit/theta is used if absent")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--fp64-steps", type=int, default=3,
+                    help="steps of the plain FP64 form timed beside the headline (0: skip)")
     ap.add_argument("--ncrit", type=int, default=64)
     ap.add_argument("--theta", type=float, default=0.4)
     ap.add_argument("--targets", type=int, default=4096)
@@ -66,6 +72,33 @@ def cpu_baseline(args):
                                                              ph["downward"], wall)}
 
 
+def time_config(ctx, prm, steps, warmup):
+    import numpy as np
+    for _ in range(warmup):
+        ctx.evaluate(prm)
+    ctx.sync()
+    phase = {k: [] for k in ("ms_tree", "ms_upward", "ms_traverse", "ms_downward", "ms_total",
+                             "ms_p2p", "ms_m2l", "ms_dtt")}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.evaluate(prm)
+        st = ctx.stats()  # waits for the step; phase times from HIP events
+        for k in phase:
+            phase[k].append(st[k])
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    return elapsed * 1e3 / steps, {k: float(np.mean(v)) for k, v in phase.items()}, st
+
+
+def errors(ctx, targets, direct):
+    import numpy as np
+    f = ctx.fields()[targets]
+
+    def rel(a, b):
+        return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+    return rel(f[:, 0], direct[:, 0]), rel(f[:, 1:], direct[:, 1:])
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -89,42 +122,47 @@ def main():
 
     ctx = tfmm.Context(0)
     ctx.set_bodies_device(args.n, tx.data_ptr(), ty.data_ptr(), tz.data_ptr(), tq.data_ptr())
-    prm = tfmm.make_params(args.ncrit, args.theta, precision)
-    for _ in range(args.warmup):
-        ctx.evaluate(prm)
-    ctx.sync()
-    torch.cuda.synchronize()
 
-    phase = {k: [] for k in ("ms_tree", "ms_upward", "ms_traverse", "ms_downward", "ms_total",
-                             "ms_p2p", "ms_m2l", "ms_dtt")}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.evaluate(prm)
-        st = ctx.stats()  # waits for the step; phase times from HIP events
-        for k in phase:
-            phase[k].append(st[k])
-    ctx.sync()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_step = elapsed * 1e3 / args.steps
+    # parameters: the learned picker (mixed precision) or the fixed ncrit/theta
+    picked = None
+    ncrit, theta = args.ncrit, args.theta
+    if precision == tfmm.MIXED and not args.no_learned and os.path.exists(args.picker):
+        w = np.load(args.picker, allow_pickle=False)["weights"]
+        pk = tfmm.Picker(0)
+        pk.set_weights(w)
+        t0 = time.perf_counter()
+        p = ctx.pick_params(pk, precision)
+        pick_ms = (time.perf_counter() - t0) * 1e3
+        ncrit, theta = int(p.ncrit), float(p.theta)
+        picked = {"ncrit": ncrit, "theta": theta, "ms": pick_ms,
+                  "note": "trial tree at ncrit 64 + tree statistics + FP32 MLP inference "
+                          "kernel, timed outside the steps"}
+        pk.close()
+    prm = tfmm.make_params(ncrit, theta, precision)
 
-    # accuracy against the FP64 direct sum (untimed)
+    # accuracy reference: FP64 direct sum on random targets (untimed)
     rng = np.random.default_rng(123)
     targets = np.sort(rng.choice(args.n, size=min(args.targets, args.n), replace=False))
-    fields = ctx.fields()
     direct = ctx.direct(targets)
-    f = fields[targets]
 
-    def rel(a, b):
-        return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
-
-    err_pot = rel(f[:, 0], direct[:, 0])
-    err_acc = rel(f[:, 1:], direct[:, 1:])
+    ms_step, phases, st = time_config(ctx, prm, args.steps, args.warmup)
+    err_pot, err_acc = errors(ctx, targets, direct)
 
     pairs = st["n_p2p_calls"] - st["n_p2p_self"]
-    p2p_ms = float(np.mean(phase["ms_p2p"]))
+    p2p_ms = phases["ms_p2p"]
     achieved = P2P_FLOPS_PER_PAIR * pairs / (p2p_ms * 1e-3) / 1e12
     peak = PEAK_FP32_TFLOPS if precision == tfmm.MIXED else PEAK_FP64_TFLOPS
+    m2l_flops = 9680.0 * st["n_m2l_exec"]
+    m2l_tflops = m2l_flops / (phases["ms_m2l"] * 1e-3) / 1e12 if phases["ms_m2l"] > 0 else 0.0
+
+    fp64 = None
+    if args.fp64_steps > 0 and precision == tfmm.MIXED:
+        prm64 = tfmm.make_params(args.ncrit, args.theta, tfmm.FP64)
+        ms64, ph64, st64 = time_config(ctx, prm64, args.fp64_steps, 1)
+        e64 = errors(ctx, targets, direct)
+        fp64 = {"value": args.n / (ms64 * 1e-3), "ms_per_step": ms64, "ncrit": args.ncrit,
+                "theta": args.theta, "rel_l2_pot": e64[0], "rel_l2_acc": e64[1],
+                "phases_ms": ph64, "speedup_of_headline": ms64 / ms_step}
 
     cpu = None if args.no_cpu else cpu_baseline(args)
     out = {
@@ -138,22 +176,27 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if precision == tfmm.FP64 else "f32 P2P/M2L + f64 tree/L2L/L2P",
+        "dtype": "f64" if precision == tfmm.FP64 else "f32 P2P/M2L + f64 tree/P2M/M2M/L2L/L2P",
         "data": "synthetic",
-        "config": {"workload": "%s N=%d P=10 ncrit=%d theta=%g %s" % (
-            args.dist, args.n, args.ncrit, args.theta, args.precision),
+        "config": {"workload": "%s N=%d P=10 ncrit=%d theta=%g %s%s" % (
+            args.dist, args.n, ncrit, round(theta, 4), args.precision,
+            " (learned)" if picked else ""),
             "n_bodies": args.n, "distribution": args.dist, "expansion": 10,
-            "ncrit": args.ncrit, "theta": args.theta, "precision": args.precision},
+            "ncrit": ncrit, "theta": theta, "precision": args.precision},
         "error": {"rel_l2_pot": err_pot, "rel_l2_acc": err_acc, "targets": len(targets),
                   "against": "FP64 direct sum"},
-        "phases_ms": {k: float(np.mean(v)) for k, v in phase.items()},
+        "picker": picked,
+        "phases_ms": phases,
         "counts": {"cells": st["ncells"], "leaves": st["nleaves"], "max_depth": st["max_depth"],
-                   "m2l_pairs": st["n_m2l_exec"], "p2p_leaf_pairs": st["n_p2p_leafpairs"],
-                   "p2p_body_pairs": pairs},
+                   "m2l_pairs": st["n_m2l_exec"], "m2l_groups": st["n_m2l_groups"],
+                   "m2l_items": st["n_m2l_items"], "m2l_fallback": st["n_m2l_fallback"],
+                   "p2p_leaf_pairs": st["n_p2p_leafpairs"], "p2p_body_pairs": pairs},
         "roofline": {"kernel": "p2p", "bound": "valu", "achieved": achieved, "peak": peak,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                      "flops_per_pair": P2P_FLOPS_PER_PAIR,
-                     "p2p_pct_fp32_peak": 100.0 * achieved / PEAK_FP32_TFLOPS},
+                     "p2p_pct_fp32_peak": 100.0 * achieved / PEAK_FP32_TFLOPS,
+                     "m2l_tflops_9680_per_pair": m2l_tflops},
+        "fp64_plain": fp64,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
