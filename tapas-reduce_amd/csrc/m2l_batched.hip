@@ -298,12 +298,16 @@ __global__ void k_tile_offsets(const int4 *items, uint32_t nitems, int32_t ntile
 // ---------------------------------------------------------------------------
 template <class T>
 struct Mfma;
+// Operands (both types): lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16].  Result i of lane l is D[row(l / 16, i)][l % 16], with
+// the row order differing between the two instructions (checked on gfx950).
 template <>
 struct Mfma<float> {
   typedef f4 V;
   static __device__ inline V step(float a, float b, V c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ inline int row(int q, int i) { return 4 * q + i; }
 };
 template <>
 struct Mfma<double> {
@@ -311,6 +315,7 @@ struct Mfma<double> {
   static __device__ inline V step(double a, double b, V c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ inline int row(int q, int i) { return q + 4 * i; }
 };
 
 // A-operand code of one (block, k-step, lane): T[r][c] = s1 Y~[i1] + s2 Y~[i2],
@@ -390,12 +395,12 @@ __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
     column_block<T, 4>(acc, mq, valid, ys, Cs, lane, dl);
     column_block<T, 5>(acc, mq, valid, ys, Cs, lane, dl);
     column_block<T, 6>(acc, mq, valid, ys, Cs, lane, dl);
-    // D of block row rb: lane holds rows 16 rb + 4 q + i of column j
+    // D of block row rb: lane holds rows 16 rb + row(q, i) of column j
 #pragma unroll
     for (int rb = 0; rb < 7; rb++) {
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int row = rb * 16 + q * 4 + i;
+        const int row = rb * 16 + Mfma<T>::row(q, i);
         if (valid && row < 110) atomicAdd(&Lt[tslot][row], acc[rb][i]);
       }
     }
