@@ -319,16 +319,17 @@ struct Mfma<double> {
 };
 
 // A-operand code of one (block, k-step, lane): T[r][c] = s1 Y~[i1] + s2 Y~[i2],
-// times 2^(n dl) with n the column's degree.  Bits: i1 0-6, s1 7-8, i2 9-15,
-// s2 16-17, n 18-21 (s: 0 none, 1 plus, 2 minus).
-__device__ inline float sgn_of(uint32_t s) { return s == 0 ? 0.f : (s == 1 ? 1.f : -1.f); }
+// times 2^(n dl) with n the column's degree.  s1, s2 in {-2..2} (the +m and
+// -m terms of a k = 0 row can hit the same harmonic).  Bits: i1 0-6,
+// s1 + 2 7-9, i2 10-16, s2 + 2 17-19, n 20-23.
+__device__ inline float mult_of(uint32_t f) { return (float)f - 2.f; }
 
 // A value of code cw from the group's harmonics in LDS
 template <class T>
 __device__ inline T a_value(const T *ys, uint32_t cw, int dl) {
-  const T y1 = ys[cw & 127u] * (T)sgn_of((cw >> 7) & 3u);
-  const T y2 = ys[(cw >> 9) & 127u] * (T)sgn_of((cw >> 16) & 3u);
-  return ldexp(y1 + y2, (int)((cw >> 18) & 15u) * dl);
+  const T y1 = ys[cw & 127u] * (T)mult_of((cw >> 7) & 7u);
+  const T y2 = ys[(cw >> 10) & 127u] * (T)mult_of((cw >> 17) & 7u);
+  return ldexp(y1 + y2, (int)((cw >> 20) & 15u) * dl);
 }
 
 // one column block cb: B rows 16 cb .. 16 cb + 15 (4 k-steps) against the
@@ -368,8 +369,14 @@ __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
   __syncthreads();
   const int q = lane >> 4, j = lane & 15;
   const int32_t ib = tile_off[tile], ie = tile_off[tile + 1];
-  for (int32_t it = ib + wave; it < ie; it += TILE_WAVES) {
-    const int4 item = items[it];
+  // items go round-robin to the waves; the adds into the tile's expansions
+  // happen in wave order within each round, so every target sums its
+  // contributions in the same order on every run
+  const int32_t rounds = (ie - ib + TILE_WAVES - 1) / TILE_WAVES;
+  for (int32_t rd = 0; rd < rounds; rd++) {
+    const int32_t it = ib + rd * TILE_WAVES + wave;
+    const bool have = it < ie;
+    const int4 item = have ? items[it] : make_int4(0, 0, 0, 0);
     const int g = item.z;
     const T *yg = Yg + (int64_t)g * NR;
     __builtin_amdgcn_wave_barrier();
@@ -396,13 +403,18 @@ __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
     column_block<T, 5>(acc, mq, valid, ys, Cs, lane, dl);
     column_block<T, 6>(acc, mq, valid, ys, Cs, lane, dl);
     // D of block row rb: lane holds rows 16 rb + row(q, i) of column j
+    for (int w = 0; w < TILE_WAVES; w++) {
+      if (w == wave && valid) {
 #pragma unroll
-    for (int rb = 0; rb < 7; rb++) {
+        for (int rb = 0; rb < 7; rb++) {
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int row = rb * 16 + Mfma<T>::row(q, i);
-        if (valid && row < 110) atomicAdd(&Lt[tslot][row], acc[rb][i]);
+          for (int i = 0; i < 4; i++) {
+            const int row = rb * 16 + Mfma<T>::row(q, i);
+            if (row < 110) Lt[tslot][row] += acc[rb][i];
+          }
+        }
       }
+      __syncthreads();
     }
   }
   __syncthreads();
@@ -415,10 +427,12 @@ __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
     const int jk = row >> 1;
     int jj = 0;
     while (tri(jj + 1, 0) <= jk) jj++;
+    const double v = (double)Lt[sl][row];
+    if (v == 0) continue;  // nothing arrived (and a zero-width tree must not form 0 * inf)
     const double isig = 1.0 / ldexp(w0, -c.depth[ci]);
     double sc = isig;
     for (int i = 0; i < jj; i++) sc *= isig;
-    L[2 * ((int64_t)ci * NTERM + jk) + (row & 1)] += (double)Lt[sl][row] * sc;
+    L[2 * ((int64_t)ci * NTERM + jk) + (row & 1)] += v * sc;
   }
 }
 
@@ -472,7 +486,10 @@ static bool build_codes(std::vector<uint32_t> &codes) {
       }
     }
   }
-  auto enc = [](int s) -> uint32_t { return s == 0 ? 0u : (s > 0 ? 1u : 2u); };
+  auto enc = [&](int s) -> uint32_t {
+    if (s < -2 || s > 2) ok = false;
+    return (uint32_t)(s + 2);
+  };
   auto degree_of_col = [](int cc) {
     if (cc >= 110) return 0;
     const int t = cc >> 1;
@@ -488,9 +505,9 @@ static bool build_codes(std::vector<uint32_t> &codes) {
         const int cc = 16 * BLK_CB[b] + 4 * kk + l / 16;
         const Ent e = T[(size_t)r * NR + cc];
         uint32_t w = 0;
-        if (e.i1 >= 0 && e.s1 != 0) w |= (uint32_t)e.i1 | (enc(e.s1) << 7);
-        if (e.i2 >= 0 && e.s2 != 0) w |= ((uint32_t)e.i2 << 9) | (enc(e.s2) << 16);
-        w |= (uint32_t)degree_of_col(cc) << 18;
+        w |= (e.i1 >= 0 ? (uint32_t)e.i1 : 0u) | (enc(e.i1 >= 0 ? e.s1 : 0) << 7);
+        w |= ((e.i2 >= 0 ? (uint32_t)e.i2 : 0u) << 10) | (enc(e.i2 >= 0 ? e.s2 : 0) << 17);
+        w |= (uint32_t)degree_of_col(cc) << 20;
         codes[(size_t)(b * 4 + kk) * 64 + l] = w;
       }
     }
