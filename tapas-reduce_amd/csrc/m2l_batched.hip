@@ -134,9 +134,12 @@ __global__ void k_mark_slots(Cells c, const int2 *pairs, uint32_t n, uint32_t *u
 
 // Split the executed M2L pairs into grouped ones (sort key: tile | group |
 // slot-in-tile, value: source cell) and the rest (key: target | source).
-__global__ void k_classify(Cells c, const int2 *pairs, uint32_t n, const uint32_t *gid_of_slot,
-                           int gbits, int sbits, uint64_t *gkeys, uint32_t *gvals,
-                           uint64_t *fkeys, uint32_t *fvals, uint32_t *counters) {
+__global__ __launch_bounds__(1024) void k_classify(Cells c, const int2 *pairs, uint32_t n,
+                                                  const uint32_t *gid_of_slot, int gbits,
+                                                  int sbits, uint64_t *gkeys, uint32_t *gvals,
+                                                  uint64_t *fkeys, uint32_t *fvals,
+                                                  uint32_t *counters) {
+  __shared__ uint32_t scratch[17];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n;
   int2 p = make_int2(0, 0);
@@ -147,36 +150,16 @@ __global__ void k_classify(Cells c, const int2 *pairs, uint32_t n, const uint32_
   }
   const bool grouped = valid && slot >= 0;
   const bool fallback = valid && slot < 0;
-  const int lane = __lane_id();
-  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  {
-    const uint64_t m = __ballot(grouped);
-    uint32_t base = 0;
-    if (m) {
-      const int leader = __ffsll((unsigned long long)m) - 1;
-      if (lane == leader) base = atomicAdd(&counters[0], (uint32_t)__popcll(m));
-      base = __shfl(base, leader, 64);
-    }
-    if (grouped) {
-      const uint32_t at = base + (uint32_t)__popcll(m & lt);
-      const uint64_t tile = (uint32_t)p.x / TILE, in_tile = (uint32_t)p.x % TILE;
-      gkeys[at] = (tile << (gbits + 7)) | ((uint64_t)gid_of_slot[slot] << 7) | in_tile;
-      gvals[at] = (uint32_t)p.y;
-    }
+  const uint32_t at = block_append<16>(&counters[0], grouped, 1, scratch);
+  if (grouped) {
+    const uint64_t tile = (uint32_t)p.x / TILE, in_tile = (uint32_t)p.x % TILE;
+    gkeys[at] = (tile << (gbits + 7)) | ((uint64_t)gid_of_slot[slot] << 7) | in_tile;
+    gvals[at] = (uint32_t)p.y;
   }
-  {
-    const uint64_t m = __ballot(fallback);
-    uint32_t base = 0;
-    if (m) {
-      const int leader = __ffsll((unsigned long long)m) - 1;
-      if (lane == leader) base = atomicAdd(&counters[1], (uint32_t)__popcll(m));
-      base = __shfl(base, leader, 64);
-    }
-    if (fallback) {
-      const uint32_t at = base + (uint32_t)__popcll(m & lt);
-      fkeys[at] = ((uint64_t)(uint32_t)p.x << sbits) | (uint64_t)(uint32_t)p.y;
-      fvals[at] = 0;
-    }
+  const uint32_t af = block_append<16>(&counters[1], fallback, 1, scratch);
+  if (fallback) {
+    fkeys[af] = ((uint64_t)(uint32_t)p.x << sbits) | (uint64_t)(uint32_t)p.y;
+    fvals[af] = 0;
   }
 }
 
@@ -332,56 +315,109 @@ __device__ inline T a_value(const T *ys, uint32_t cw, int dl) {
   return ldexp(y1 + y2, (int)((cw >> 20) & 15u) * dl);
 }
 
-// one column block cb: B rows 16 cb .. 16 cb + 15 (4 k-steps) against the
-// row blocks listed for cb
-template <class T, int CB>
-__device__ inline void column_block(typename Mfma<T>::V *acc, const T *mq, bool valid,
-                                    const T *ys, const uint32_t *codes_lds, int lane, int dl) {
-  T b[4];
+// Work split of one item over the 4 waves of a tile: each wave owns fixed
+// (row block, column block) pairs of T, so the waves write disjoint rows of
+// the tile's expansions and need no atomics and no barriers between items.
+// Row block 0 has 7 column blocks and is split between waves 0 and 1 (wave 1
+// accumulates its part in a separate array, added at the end).
+//   wave 0: (0,0) (0,1) (0,2) (0,3) (6,0)
+//   wave 1: (0,4) (0,5) (0,6) (4,0) (5,0)
+//   wave 2: (1,0) (1,1) (1,2) (1,3)
+//   wave 3: (2,0) (2,1) (3,0) (3,1)
+constexpr int WAVE_NBLK[TILE_WAVES] = {5, 5, 4, 4};
+constexpr int WAVE_BLK[TILE_WAVES][5] = {
+    {0, 1, 2, 3, 17}, {4, 5, 6, 15, 16}, {7, 8, 9, 10, -1}, {11, 12, 13, 14, -1}};
+
+template <class T, int W>
+__device__ inline void wave_item(typename Mfma<T>::V *acc, const uint32_t *code, const T *ys,
+                                 const T *mq, bool valid, int dl) {
+  typedef typename Mfma<T>::V V;
+  // B fragments of the column blocks this wave touches (4 k-steps each)
+  T b[7][4];
 #pragma unroll
-  for (int kk = 0; kk < 4; kk++) b[kk] = valid ? mq[4 * CB + kk] : (T)0;
-  // k-step outer, blocks inner: consecutive MFMAs feed different accumulators
+  for (int cb = 0; cb < 7; cb++) {
+    bool need = false;
+#pragma unroll
+    for (int e = 0; e < WAVE_NBLK[W]; e++) need |= BLK_CB[WAVE_BLK[W][e]] == cb;
+    if (need) {
+#pragma unroll
+      for (int kk = 0; kk < 4; kk++) b[cb][kk] = valid ? mq[4 * cb + kk] : (T)0;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < WAVE_NBLK[W]; e++) acc[e] = (V){0, 0, 0, 0};
 #pragma unroll
   for (int kk = 0; kk < 4; kk++) {
 #pragma unroll
-    for (int blk = 0; blk < NBLK; blk++) {
-      if (BLK_CB[blk] != CB) continue;
-      const T a = a_value(ys, codes_lds[(blk * 4 + kk) * 64 + lane], dl);
-      acc[BLK_RB[blk]] = Mfma<T>::step(a, b[kk], acc[BLK_RB[blk]]);
+    for (int e = 0; e < WAVE_NBLK[W]; e++) {
+      const int blk = WAVE_BLK[W][e];
+      const T a = a_value(ys, code[e * 4 + kk], dl);
+      acc[e] = Mfma<T>::step(a, b[BLK_CB[blk]][kk], acc[e]);
     }
   }
-  __builtin_amdgcn_sched_barrier(0);
 }
 
-template <class T>
-__global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
-    const int4 *items, const int32_t *tile_off, const uint64_t *keys, const uint32_t *src,
-    const T *Yg, const int8_t *gdl, const T *Mq, const uint32_t *codes, Cells c, int32_t ncells,
-    double w0, double *L) {
+// accumulate the wave's results of one item into the tile (rows of the
+// wave's row blocks; the second half of row block 0 goes to Lt0b)
+template <class T, int W>
+__device__ inline void wave_store(const typename Mfma<T>::V *acc, T (*Lt)[NR + 1],
+                                  T (*Lt0b)[17], uint32_t tslot, int q) {
+  // blocks of one row block are summed first
+  typename Mfma<T>::V sum[5];
+  int rbs[5];
+  int nrb = 0;
+#pragma unroll
+  for (int e = 0; e < WAVE_NBLK[W]; e++) {
+    const int rb = BLK_RB[WAVE_BLK[W][e]];
+    bool found = false;
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      if (f < nrb && rbs[f] == rb) {
+        sum[f] += acc[e];
+        found = true;
+      }
+    }
+    if (!found) {
+      rbs[nrb] = rb;
+      sum[nrb] = acc[e];
+      nrb++;
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < 5; f++) {
+    if (f >= nrb) break;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int r = Mfma<T>::row(q, i);
+      if (W == 1 && rbs[f] == 0) {
+        Lt0b[tslot][r] += sum[f][i];
+      } else {
+        const int row = rbs[f] * 16 + r;
+        if (row < 110) Lt[tslot][row] += sum[f][i];
+      }
+    }
+  }
+}
+
+template <class T, int W>
+__device__ inline void run_wave(const int4 *items, int32_t ib, int32_t ie, const uint64_t *keys,
+                                const uint32_t *src, const T *Yg, const int8_t *gdl,
+                                const T *Mq, const uint32_t *codes, T (*Lt)[NR + 1],
+                                T (*Lt0b)[17], T *ys, int lane) {
   typedef typename Mfma<T>::V V;
-  __shared__ T Lt[TILE][NR + 1];
-  __shared__ T Ys[TILE_WAVES][NR];
-  __shared__ uint32_t Cs[NSLOT * 64];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tile = blockIdx.x;
-  for (int i = threadIdx.x; i < TILE * (NR + 1); i += 64 * TILE_WAVES) (&Lt[0][0])[i] = 0;
-  for (int i = threadIdx.x; i < NSLOT * 64; i += 64 * TILE_WAVES) Cs[i] = codes[i];
-  __syncthreads();
+  uint32_t code[5 * 4];
+#pragma unroll
+  for (int e = 0; e < WAVE_NBLK[W]; e++)
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) code[e * 4 + kk] = codes[(WAVE_BLK[W][e] * 4 + kk) * 64 + lane];
   const int q = lane >> 4, j = lane & 15;
-  const int32_t ib = tile_off[tile], ie = tile_off[tile + 1];
-  // items go round-robin to the waves; the adds into the tile's expansions
-  // happen in wave order within each round, so every target sums its
-  // contributions in the same order on every run
-  const int32_t rounds = (ie - ib + TILE_WAVES - 1) / TILE_WAVES;
-  for (int32_t rd = 0; rd < rounds; rd++) {
-    const int32_t it = ib + rd * TILE_WAVES + wave;
-    const bool have = it < ie;
-    const int4 item = have ? items[it] : make_int4(0, 0, 0, 0);
+  for (int32_t it = ib; it < ie; it++) {
+    const int4 item = items[it];
     const int g = item.z;
     const T *yg = Yg + (int64_t)g * NR;
     __builtin_amdgcn_wave_barrier();
-    Ys[wave][lane] = yg[lane];
-    if (lane < NR - 64) Ys[wave][64 + lane] = yg[64 + lane];
+    ys[lane] = yg[lane];
+    if (lane < NR - 64) ys[64 + lane] = yg[64 + lane];
     const int dl = gdl[g];
     const bool valid = j < item.y;
     uint32_t s = 0, tslot = 0;
@@ -389,33 +425,38 @@ __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
       s = src[item.x + j];
       tslot = (uint32_t)(keys[item.x + j] & 127u);
     }
-    const T *mq = Mq + (int64_t)s * NR + q * 28;
     __builtin_amdgcn_wave_barrier();
-    V acc[7];
-#pragma unroll
-    for (int r = 0; r < 7; r++) acc[r] = (V){0, 0, 0, 0};
-    const T *ys = Ys[wave];
-    column_block<T, 0>(acc, mq, valid, ys, Cs, lane, dl);
-    column_block<T, 1>(acc, mq, valid, ys, Cs, lane, dl);
-    column_block<T, 2>(acc, mq, valid, ys, Cs, lane, dl);
-    column_block<T, 3>(acc, mq, valid, ys, Cs, lane, dl);
-    column_block<T, 4>(acc, mq, valid, ys, Cs, lane, dl);
-    column_block<T, 5>(acc, mq, valid, ys, Cs, lane, dl);
-    column_block<T, 6>(acc, mq, valid, ys, Cs, lane, dl);
-    // D of block row rb: lane holds rows 16 rb + row(q, i) of column j
-    for (int w = 0; w < TILE_WAVES; w++) {
-      if (w == wave && valid) {
-#pragma unroll
-        for (int rb = 0; rb < 7; rb++) {
-#pragma unroll
-          for (int i = 0; i < 4; i++) {
-            const int row = rb * 16 + Mfma<T>::row(q, i);
-            if (row < 110) Lt[tslot][row] += acc[rb][i];
-          }
-        }
-      }
-      __syncthreads();
-    }
+    V acc[5];
+    wave_item<T, W>(acc, code, ys, Mq + (int64_t)s * NR + q * 28, valid, dl);
+    if (valid) wave_store<T, W>(acc, Lt, Lt0b, tslot, q);
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
+    const int4 *items, const int32_t *tile_off, const uint64_t *keys, const uint32_t *src,
+    const T *Yg, const int8_t *gdl, const T *Mq, const uint32_t *codes, Cells c, int32_t ncells,
+    double w0, double *L) {
+  __shared__ T Lt[TILE][NR + 1];
+  __shared__ T Lt0b[TILE][17];
+  __shared__ T Ys[TILE_WAVES][NR];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tile = blockIdx.x;
+  for (int i = threadIdx.x; i < TILE * (NR + 1); i += 64 * TILE_WAVES) (&Lt[0][0])[i] = 0;
+  for (int i = threadIdx.x; i < TILE * 17; i += 64 * TILE_WAVES) (&Lt0b[0][0])[i] = 0;
+  __syncthreads();
+  const int32_t ib = tile_off[tile], ie = tile_off[tile + 1];
+  // every wave walks all items of the tile, in order, for its own blocks
+  switch (wave) {
+    case 0: run_wave<T, 0>(items, ib, ie, keys, src, Yg, gdl, Mq, codes, Lt, Lt0b, Ys[0], lane); break;
+    case 1: run_wave<T, 1>(items, ib, ie, keys, src, Yg, gdl, Mq, codes, Lt, Lt0b, Ys[1], lane); break;
+    case 2: run_wave<T, 2>(items, ib, ie, keys, src, Yg, gdl, Mq, codes, Lt, Lt0b, Ys[2], lane); break;
+    default: run_wave<T, 3>(items, ib, ie, keys, src, Yg, gdl, Mq, codes, Lt, Lt0b, Ys[3], lane); break;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < TILE * 16; e += 64 * TILE_WAVES) {
+    const int sl = e >> 4, r = e & 15;
+    Lt[sl][r] += Lt0b[sl][r];
   }
   __syncthreads();
   // L_j = L~_j / sigma_t^(j+1), added to the target's FP64 local expansion
@@ -549,8 +590,8 @@ hipError_t launch_m2l_classify(Cells c, const int2 *pairs, uint32_t n,
                                uint64_t *gkeys, uint32_t *gvals, uint64_t *fkeys,
                                uint32_t *fvals, uint32_t *counters, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  k_classify<<<(n + 255) / 256, 256, 0, s>>>(c, pairs, n, gid_of_slot, gbits, sbits, gkeys,
-                                              gvals, fkeys, fvals, counters);
+  k_classify<<<(n + 1023) / 1024, 1024, 0, s>>>(c, pairs, n, gid_of_slot, gbits, sbits, gkeys,
+                                                  gvals, fkeys, fvals, counters);
   return hipGetLastError();
 }
 

@@ -194,6 +194,31 @@ __device__ inline cd yfull(const cd *Y, int n, int mm) {
   return mm >= 0 ? Y[tri(n, mm)] : cconj(Y[tri(n, -mm)]);
 }
 
+// Append from every thread of a block with pred set: one global atomic per
+// block (same-address atomics per wave serialise in L2 and dominated the
+// traversal).  Returns the slot of this thread (valid where pred is set);
+// every thread of the block must call it.  scratch: >= NW + 1 uints of LDS.
+template <int NW>
+__device__ inline uint32_t block_append(uint32_t *counter, bool pred, uint32_t per_lane,
+                                        uint32_t *scratch) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t m = __ballot(pred);
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  if (lane == 0) scratch[wid] = (uint32_t)__popcll(m) * per_lane;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int w = 0; w < NW; w++) tot += scratch[w];
+    scratch[NW] = tot ? atomicAdd(counter, tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t pre = scratch[NW];
+  for (int w = 0; w < wid; w++) pre += scratch[w];
+  const uint32_t slot = pre + (uint32_t)__popcll(m & lt) * per_lane;
+  __syncthreads();
+  return slot;
+}
+
 __device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);

@@ -21,22 +21,10 @@
 namespace tfmm {
 namespace {
 
-__device__ inline uint32_t wave_append(uint32_t *counter, bool pred, uint32_t per_lane,
-                                       uint32_t &slot) {
-  const uint64_t m = __ballot(pred);
-  const int lane = __lane_id();
-  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  uint32_t base = 0;
-  if (m) {
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m) * per_lane);
-    base = __shfl(base, leader, 64);
-  }
-  slot = base + (uint32_t)__popcll(m & lt) * per_lane;
-  return (uint32_t)__popcll(m);
-}
+constexpr int DTT_THREADS = 1024;
+constexpr int DTT_WAVES = DTT_THREADS / 64;
 
-__global__ __launch_bounds__(256) void k_dtt_step(const int2 *in, const uint32_t *nin_dev,
+__global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const uint32_t *nin_dev,
                                                   uint32_t nin_max, int2 *out, uint32_t out_cap,
                                                   int2 *m2l, uint32_t m2l_cap, int2 *p2p,
                                                   uint32_t p2p_cap, uint32_t *cnt, Cells c,
@@ -77,19 +65,16 @@ __global__ __launch_bounds__(256) void k_dtt_step(const int2 *in, const uint32_t
       split_j = true;
     }
   }
+  __shared__ uint32_t scratch[DTT_WAVES + 1];
   uint32_t slot;
   // every M2L pair is counted; only pairs of two non-empty cells are executed
-  {
-    const uint64_t m = __ballot(far);
-    if (m && __lane_id() == (__ffsll((unsigned long long)m) - 1))
-      atomicAdd(&cnt[3], (uint32_t)__popcll(m));
-  }
-  wave_append(&cnt[1], far && nonempty, 1, slot);
+  block_append<DTT_WAVES>(&cnt[3], far, 1, scratch);
+  slot = block_append<DTT_WAVES>(&cnt[1], far && nonempty, 1, scratch);
   if (far && nonempty && slot < m2l_cap) m2l[slot] = make_int2(i, j);
-  wave_append(&cnt[2], p2p_pair, 1, slot);
+  slot = block_append<DTT_WAVES>(&cnt[2], p2p_pair, 1, scratch);
   if (p2p_pair && slot < p2p_cap) p2p[slot] = make_int2(i, j);
   const bool split = split_i || split_j;
-  wave_append(&cnt[0], split, 8, slot);
+  slot = block_append<DTT_WAVES>(&cnt[0], split, 8, scratch);
   if (split && slot + 8 <= out_cap) {
     if (split_i) {
       const int32_t f = c.child[i];
@@ -131,8 +116,8 @@ __global__ void k_csr_offsets(const uint64_t *keys, uint32_t n, int sbits, int32
   off[t] = (int32_t)lo;
 }
 
-__global__ void k_p2p_pair_count(Cells c, const int2 *p2p, uint32_t n,
-                                 unsigned long long *out) {
+__global__ __launch_bounds__(256) void k_p2p_pair_count(Cells c, const int2 *p2p, uint32_t n,
+                                                        unsigned long long *out) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long v = 0, self = 0;
   if (t < n) {
@@ -144,9 +129,15 @@ __global__ void k_p2p_pair_count(Cells c, const int2 *p2p, uint32_t n,
     v += __shfl_xor(v, o, 64);
     self += __shfl_xor(self, o, 64);
   }
+  __shared__ unsigned long long sv[4], ss[4];
   if (__lane_id() == 0) {
-    atomicAdd(&out[0], v);
-    atomicAdd(&out[1], self);
+    sv[threadIdx.x >> 6] = v;
+    ss[threadIdx.x >> 6] = self;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&out[0], sv[0] + sv[1] + sv[2] + sv[3]);
+    atomicAdd(&out[1], ss[0] + ss[1] + ss[2] + ss[3]);
   }
 }
 
@@ -157,8 +148,8 @@ hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin
                            uint32_t p2p_cap, uint32_t *cnt, Cells c, double theta,
                            hipStream_t s) {
   if (nin_max == 0) return hipSuccess;
-  k_dtt_step<<<(nin_max + 255) / 256, 256, 0, s>>>(in, nin_dev, nin_max, out, out_cap, m2l,
-                                                  m2l_cap, p2p, p2p_cap, cnt, c, theta);
+  k_dtt_step<<<(nin_max + DTT_THREADS - 1) / DTT_THREADS, DTT_THREADS, 0, s>>>(
+      in, nin_dev, nin_max, out, out_cap, m2l, m2l_cap, p2p, p2p_cap, cnt, c, theta);
   return hipGetLastError();
 }
 
