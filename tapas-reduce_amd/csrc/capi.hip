@@ -77,7 +77,7 @@ struct tfmm_ctx {
   // batched M2L (offset groups)
   DevBuf g_used, g_gid, g_y64, g_y32, g_dl, g_lt, g_cnt;
   DevBuf gkeys, gvals, gkeys_alt, gvals_alt, fkeys, fvals, fkeys_alt, fvals_alt;
-  DevBuf it_flag, it_excl, it_seg, it_iflag, it_iexcl, items, tile_off, mq, codes;
+  DevBuf it_flag, it_excl, it_seg, it_iflag, it_iexcl, items, cols, tile_off, mq, codes;
   DevBuf fb_off, fb_src, relpos;
   bool codes_ready = false;
   uint32_t n_groups = 0, n_grouped = 0, n_fallback = 0, n_items = 0;
@@ -412,12 +412,19 @@ int build_m2l_batches(tfmm_ctx *c) {
   TRY(dev_reserve(c->it_seg, ng1 * 4));
   TRY(dev_reserve(c->it_iflag, ng1 * 4));
   TRY(dev_reserve(c->it_iexcl, ng1 * 4));
-  TRY(dev_reserve(c->items, ng1 * 16));
-  uint32_t nitems = 0;
-  TRY(launch_m2l_items(gk, ngp, gbits, dp<uint32_t>(c->it_flag), dp<uint32_t>(c->it_excl),
-                       dp<uint32_t>(c->it_seg), dp<uint32_t>(c->it_iflag),
-                       dp<uint32_t>(c->it_iexcl), c->scan_tmp.ptr, c->h_u32, dp<int4>(c->items),
-                       ng1, &nitems, s));
+  uint32_t nitems = 0, nseg = 0;
+  TRY(launch_m2l_count_items(gk, ngp, dp<uint32_t>(c->it_flag), dp<uint32_t>(c->it_excl),
+                             dp<uint32_t>(c->it_seg), dp<uint32_t>(c->it_iflag),
+                             dp<uint32_t>(c->it_iexcl), c->scan_tmp.ptr, c->h_u32, &nseg,
+                             &nitems, s));
+  const size_t ni1 = std::max<uint32_t>(nitems, 1);
+  TRY(dev_reserve(c->items, ni1 * 16));
+  TRY(dev_reserve(c->cols, ni1 * 16 * 8));
+  const uint32_t *gvs = dp<uint32_t>(alt ? c->gvals_alt : c->gvals);
+  TRY(launch_m2l_write_items(gk, gvs, ngp, gbits, dp<uint32_t>(c->it_flag),
+                             dp<uint32_t>(c->it_excl), dp<uint32_t>(c->it_seg), nseg,
+                             dp<uint32_t>(c->it_iflag), dp<uint32_t>(c->it_iexcl),
+                             dp<int4>(c->items), dp<uint2>(c->cols), s));
   c->n_items = nitems;
   TRY(dev_reserve(c->tile_off, (size_t)(ntiles + 1) * 4));
   TRY(launch_m2l_tile_offsets(dp<int4>(c->items), nitems, ntiles, dp<int32_t>(c->tile_off), s));
@@ -431,7 +438,7 @@ int build_m2l_batches(tfmm_ctx *c) {
   TRY(launch_csr_offsets(fk, nfb, sbits, (int32_t)c->ncells, dp<int32_t>(c->fb_off), s));
   TRY(launch_unpack_pairs(fk, nfb, sbits, dp<int32_t>(c->fb_src), s));
   if (!c->codes_ready) {
-    TRY(dev_reserve(c->codes, 72 * 64 * 4));
+    TRY(dev_reserve(c->codes, m2l_codes_bytes()));
     hipError_t e = m2l_upload_codes(dp<uint32_t>(c->codes), s);
     if (e == hipErrorInvalidValue) return fail(c, TFMM_EHIP, "M2L operator table is inconsistent");
     TRY(e);
@@ -469,9 +476,7 @@ int do_traverse(tfmm_ctx *c) {
   TRY(hipEventRecord(c->ev[EV_M2L0], s));
   const int32_t tile = m2l_tile_size();
   const int32_t ntiles = (int32_t)((c->ncells + tile - 1) / tile);
-  const uint64_t *gk = dp<uint64_t>(c->grouped_keys_in_alt ? c->gkeys_alt : c->gkeys);
-  const uint32_t *gv = dp<uint32_t>(c->grouped_keys_in_alt ? c->gvals_alt : c->gvals);
-  TRY(launch_m2l_tiles(dp<int4>(c->items), dp<int32_t>(c->tile_off), ntiles, gk, gv,
+  TRY(launch_m2l_tiles(dp<int32_t>(c->tile_off), ntiles, dp<uint2>(c->cols),
                        mixed ? (const void *)c->g_y32.ptr : (const void *)c->g_y64.ptr,
                        dp<int8_t>(c->g_dl), c->mq.ptr, dp<uint32_t>(c->codes), k,
                        (int32_t)c->ncells, c->w0, dp<double>(c->L), mixed, s));
@@ -561,7 +566,7 @@ void tfmm_destroy(tfmm_ctx *c) {
                     &c->g_lt, &c->g_cnt, &c->gkeys, &c->gvals, &c->gkeys_alt, &c->gvals_alt,
                     &c->fkeys, &c->fvals, &c->fkeys_alt, &c->fvals_alt, &c->it_flag,
                     &c->it_excl, &c->it_seg, &c->it_iflag, &c->it_iexcl, &c->items,
-                    &c->tile_off, &c->mq, &c->codes, &c->fb_off, &c->fb_src, &c->relpos,
+                    &c->tile_off, &c->cols, &c->mq, &c->codes, &c->fb_off, &c->fb_src, &c->relpos,
                     &c->feat_acc, &c->feat_x, &c->feat_out};
   for (DevBuf *b : bufs) dev_release(*b);
   for (int i = 0; i < EV_COUNT; i++)

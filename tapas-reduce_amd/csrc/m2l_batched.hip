@@ -25,13 +25,22 @@
 // first, so one tile is one Morton range of one level); the pairs of a tile
 // are sorted by group, and each run of up to 16 pairs of one group is a work
 // item = 18 blocks x 4 MFMA 16x16x4 steps.  One workgroup (4 waves) owns a
-// tile: it accumulates the tile's local expansions in LDS (LDS atomics, since
-// waves work on different items) and writes each target once, without
-// global atomics.  Pairs whose offset falls outside the group table (level
+// tile and accumulates its local expansions in LDS; each wave owns fixed row
+// blocks of T, so the waves never write the same LDS word, walk the tile's
+// items independently, and the accumulation order is fixed (deterministic).
+// The item loop is software pipelined two deep: while an item is multiplied,
+// the next item's operands and the column descriptors of the one after are
+// in flight.  Pairs whose offset falls outside the group table (level
 // difference >= 2) go to the per-pair kernel of m2l.hip.
+//
+// Per-group factors: the level factor 2^(n dl), n the source degree and
+// dl = l_t - l_s, is split as 2^(p dl) 2^(-j dl) with p = j + n, the first
+// folded into the group's harmonics (index p) and the second applied to the
+// output row (degree j); both are exact powers of two.
 #include <algorithm>
 #include <vector>
 
+#include "m2l_codes.h"
 #include "tfmm_device.h"
 #include "tfmm_kernels.h"
 #include "tfmm_internal.h"
@@ -43,15 +52,14 @@ namespace {
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int NR = 112;  // padded real rows/columns
-constexpr int NBLK = 18;
-constexpr int NSLOT = NBLK * 4;
+using m2lc::BLK_CB;
+using m2lc::BLK_RB;
+using m2lc::NBLK;
+using m2lc::NMULT;
+using m2lc::NR;
+constexpr int NCODE = NBLK * 4 * 64;
 constexpr int TILE = 128;
 constexpr int TILE_WAVES = 4;
-
-// non-zero 16x16 blocks (row block, column block) of T
-constexpr int8_t BLK_RB[NBLK] = {0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 3, 3, 4, 5, 6};
-constexpr int8_t BLK_CB[NBLK] = {0, 1, 2, 3, 4, 5, 6, 0, 1, 2, 3, 0, 1, 0, 1, 0, 0, 0};
 
 // ---------------------------------------------------------------------------
 // group table
@@ -163,8 +171,9 @@ __global__ __launch_bounds__(1024) void k_classify(Cells c, const int2 *pairs, u
   }
 }
 
-// Harmonics of every used group, FP64, scaled: Y~_p^q = Y_p^q sigma_t^(p+1).
-// Stored as NR reals (re/im interleaved, index 2 tri(p, q)), in FP64 and FP32.
+// Harmonics of every used group, FP64, scaled: Y~_p^q = Y_p^q sigma_t^(p+1)
+// 2^(p dl), dl = l_t - l_s.  Stored as NR reals (re/im interleaved, index
+// 2 tri(p, q)), in FP64 and FP32.
 __global__ void k_group_harmonics(const uint32_t *used, const uint32_t *gid_of_slot,
                                   int32_t nslots, double w0, double *Y64, float *Y32,
                                   int8_t *gdl, int8_t *glt) {
@@ -185,7 +194,7 @@ __global__ void k_group_harmonics(const uint32_t *used, const uint32_t *gid_of_s
   float *yf = Y32 + (int64_t)g * NR;
   for (int m = 0; m < P; m++) {
     local_column(m, rho, x, y, ei, [&](int n, int mm, cd v) {
-      double sc = sig;  // sigma^(n+1)
+      double sc = ldexp(sig, n * (lt - ls));  // sigma^(n+1) 2^(n dl)
       for (int i = 0; i < n; i++) sc *= sig;
       const int t = tri(n, mm);
       yo[2 * t] = v.re * sc;
@@ -246,21 +255,33 @@ __global__ void k_item_flags(const uint32_t *flag, const uint32_t *excl,
   iflag[i] = ((i - segstart[seg]) % 16u == 0u) ? 1u : 0u;
 }
 
-__global__ void k_items(const uint64_t *keys, const uint32_t *flag, const uint32_t *excl,
-                        const uint32_t *segstart, uint32_t nseg, const uint32_t *iflag,
-                        const uint32_t *iexcl, uint32_t n, int gbits, int4 *items) {
+// Item descriptors (first pair, pairs, group, tile) and the item's 16 column
+// descriptors: cols[16 item + j] = (source cell, target slot in tile
+// | 128 if column j holds a pair | group << 8).  Every column carries the
+// group, so that every lane of the tile kernel can fetch the group's data.
+__global__ void k_items(const uint64_t *keys, const uint32_t *vals, const uint32_t *flag,
+                        const uint32_t *excl, const uint32_t *segstart, uint32_t nseg,
+                        const uint32_t *iflag, const uint32_t *iexcl, uint32_t n, int gbits,
+                        int4 *items, uint2 *cols) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || !iflag[i]) return;
   const uint32_t seg = excl[i] + flag[i] - 1;
   const uint32_t segend = (seg + 1 < nseg) ? segstart[seg + 1] : n;
   const uint32_t cnt = min(16u, segend - i);
   const uint64_t k = keys[i];
+  const uint32_t g = (uint32_t)((k >> 7) & ((1ull << gbits) - 1ull));
   int4 it;
   it.x = (int)i;                                         // first pair
   it.y = (int)cnt;                                       // pairs in the item
-  it.z = (int)((k >> 7) & ((1ull << gbits) - 1ull));     // group
+  it.z = (int)g;                                         // group
   it.w = (int)(k >> (gbits + 7));                        // tile
-  items[iexcl[i]] = it;
+  const uint32_t at = iexcl[i];
+  items[at] = it;
+  for (uint32_t jj = 0; jj < 16; jj++) {
+    uint2 cv = make_uint2(0u, g << 8);
+    if (jj < cnt) cv = make_uint2(vals[i + jj], (uint32_t)(keys[i + jj] & 127u) | 128u | (g << 8));
+    cols[(int64_t)at * 16 + jj] = cv;
+  }
 }
 
 __global__ void k_tile_offsets(const int4 *items, uint32_t nitems, int32_t ntiles,
@@ -301,20 +322,6 @@ struct Mfma<double> {
   static __device__ inline int row(int q, int i) { return q + 4 * i; }
 };
 
-// A-operand code of one (block, k-step, lane): T[r][c] = s1 Y~[i1] + s2 Y~[i2],
-// times 2^(n dl) with n the column's degree.  s1, s2 in {-2..2} (the +m and
-// -m terms of a k = 0 row can hit the same harmonic).  Bits: i1 0-6,
-// s1 + 2 7-9, i2 10-16, s2 + 2 17-19, n 20-23.
-__device__ inline float mult_of(uint32_t f) { return (float)f - 2.f; }
-
-// A value of code cw from the group's harmonics in LDS
-template <class T>
-__device__ inline T a_value(const T *ys, uint32_t cw, int dl) {
-  const T y1 = ys[cw & 127u] * (T)mult_of((cw >> 7) & 7u);
-  const T y2 = ys[(cw >> 10) & 127u] * (T)mult_of((cw >> 17) & 7u);
-  return ldexp(y1 + y2, (int)((cw >> 20) & 15u) * dl);
-}
-
 // Work split of one item over the 4 waves of a tile: each wave owns fixed
 // (row block, column block) pairs of T, so the waves write disjoint rows of
 // the tile's expansions and need no atomics and no barriers between items.
@@ -324,122 +331,149 @@ __device__ inline T a_value(const T *ys, uint32_t cw, int dl) {
 //   wave 1: (0,4) (0,5) (0,6) (4,0) (5,0)
 //   wave 2: (1,0) (1,1) (1,2) (1,3)
 //   wave 3: (2,0) (2,1) (3,0) (3,1)
+// Blocks of one row block accumulate into one MFMA accumulator (WAVE_F).
 constexpr int WAVE_NBLK[TILE_WAVES] = {5, 5, 4, 4};
 constexpr int WAVE_BLK[TILE_WAVES][5] = {
     {0, 1, 2, 3, 17}, {4, 5, 6, 15, 16}, {7, 8, 9, 10, -1}, {11, 12, 13, 14, -1}};
+constexpr int WAVE_NRB[TILE_WAVES] = {2, 3, 1, 2};
+constexpr int WAVE_RBS[TILE_WAVES][3] = {{0, 6, -1}, {0, 4, 5}, {1, -1, -1}, {2, 3, -1}};
+constexpr int WAVE_F[TILE_WAVES][5] = {
+    {0, 0, 0, 0, 1}, {0, 0, 0, 1, 2}, {0, 0, 0, 0, -1}, {0, 0, 1, 1, -1}};
 
+__host__ __device__ constexpr bool wave_needs_cb(int w, int cb) {
+  for (int e = 0; e < WAVE_NBLK[w]; e++)
+    if (BLK_CB[WAVE_BLK[w][e]] == cb) return true;
+  return false;
+}
+
+__device__ inline int row_degree(int r) {
+  if (r >= 110) return 0;
+  const int t = r >> 1;
+  int n = 0;
+  while (tri(n + 1, 0) <= t) n++;
+  return n;
+}
+
+// B fragments (4 k-steps) of the column blocks wave W multiplies, for the
+// source whose M~ starts at mq (already offset by the lane's k-row quad)
 template <class T, int W>
-__device__ inline void wave_item(typename Mfma<T>::V *acc, const uint32_t *code, const T *ys,
-                                 const T *mq, bool valid, int dl) {
-  typedef typename Mfma<T>::V V;
-  // B fragments of the column blocks this wave touches (4 k-steps each)
-  T b[7][4];
+__device__ inline void load_b(T (&b)[7][4], const T *mq) {
 #pragma unroll
   for (int cb = 0; cb < 7; cb++) {
-    bool need = false;
+    if (wave_needs_cb(W, cb)) {
 #pragma unroll
-    for (int e = 0; e < WAVE_NBLK[W]; e++) need |= BLK_CB[WAVE_BLK[W][e]] == cb;
-    if (need) {
-#pragma unroll
-      for (int kk = 0; kk < 4; kk++) b[cb][kk] = valid ? mq[4 * cb + kk] : (T)0;
-    }
-  }
-#pragma unroll
-  for (int e = 0; e < WAVE_NBLK[W]; e++) acc[e] = (V){0, 0, 0, 0};
-#pragma unroll
-  for (int kk = 0; kk < 4; kk++) {
-#pragma unroll
-    for (int e = 0; e < WAVE_NBLK[W]; e++) {
-      const int blk = WAVE_BLK[W][e];
-      const T a = a_value(ys, code[e * 4 + kk], dl);
-      acc[e] = Mfma<T>::step(a, b[BLK_CB[blk]][kk], acc[e]);
+      for (int kk = 0; kk < 4; kk++) b[cb][kk] = mq[4 * cb + kk];
     }
   }
 }
 
-// accumulate the wave's results of one item into the tile (rows of the
-// wave's row blocks; the second half of row block 0 goes to Lt0b)
-template <class T, int W>
-__device__ inline void wave_store(const typename Mfma<T>::V *acc, T (*Lt)[NR + 1],
-                                  T (*Lt0b)[17], uint32_t tslot, int q) {
-  // blocks of one row block are summed first
-  typename Mfma<T>::V sum[5];
-  int rbs[5];
-  int nrb = 0;
-#pragma unroll
-  for (int e = 0; e < WAVE_NBLK[W]; e++) {
-    const int rb = BLK_RB[WAVE_BLK[W][e]];
-    bool found = false;
-#pragma unroll
-    for (int f = 0; f < 5; f++) {
-      if (f < nrb && rbs[f] == rb) {
-        sum[f] += acc[e];
-        found = true;
-      }
-    }
-    if (!found) {
-      rbs[nrb] = rb;
-      sum[nrb] = acc[e];
-      nrb++;
-    }
-  }
-#pragma unroll
-  for (int f = 0; f < 5; f++) {
-    if (f >= nrb) break;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int r = Mfma<T>::row(q, i);
-      if (W == 1 && rbs[f] == 0) {
-        Lt0b[tslot][r] += sum[f][i];
-      } else {
-        const int row = rbs[f] * 16 + r;
-        if (row < 110) Lt[tslot][row] += sum[f][i];
-      }
-    }
-  }
+// A entry of code cw: sum of two elements of the wave's operand table
+template <class T>
+__device__ inline T a_value(const T *tbl, uint32_t cw) {
+  const char *base = reinterpret_cast<const char *>(tbl);
+  return *reinterpret_cast<const T *>(base + (cw & 0xffffu)) +
+         *reinterpret_cast<const T *>(base + (cw >> 16));
 }
 
 template <class T, int W>
-__device__ inline void run_wave(const int4 *items, int32_t ib, int32_t ie, const uint64_t *keys,
-                                const uint32_t *src, const T *Yg, const int8_t *gdl,
-                                const T *Mq, const uint32_t *codes, T (*Lt)[NR + 1],
-                                T (*Lt0b)[17], T *ys, int lane) {
+__device__ inline void run_wave(const uint2 *cols, int32_t ib, int32_t ie, const T *Yg,
+                                const int8_t *gdl, const T *Mq, const uint32_t *codes,
+                                T (*Lt)[NR + 1], T (*Lt0b)[17], T *tbl, int lane) {
   typedef typename Mfma<T>::V V;
-  uint32_t code[5 * 4];
+  constexpr int NB = WAVE_NBLK[W], NF = WAVE_NRB[W];
+  if (ib >= ie) return;
+  uint32_t code[NB * 4];
 #pragma unroll
-  for (int e = 0; e < WAVE_NBLK[W]; e++)
+  for (int e = 0; e < NB; e++)
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) code[e * 4 + kk] = codes[(WAVE_BLK[W][e] * 4 + kk) * 64 + lane];
   const int q = lane >> 4, j = lane & 15;
+  int jd[NF][4];  // degree of each output row this lane holds
+#pragma unroll
+  for (int f = 0; f < NF; f++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) jd[f][i] = row_degree(WAVE_RBS[W][f] * 16 + Mfma<T>::row(q, i));
+  const bool hi = lane < NR - 64;
+
+  // pipeline: columns of items it and it + 1, operands of item it
+  uint2 cc = cols[(int64_t)ib * 16 + j];
+  uint2 cn = cols[(int64_t)min(ib + 1, ie - 1) * 16 + j];  // clamped: loads stay in range
+  T bc[7][4];
+  load_b<T, W>(bc, Mq + (int64_t)cc.x * NR + q * 28);
+  T y0 = Yg[(int64_t)(cc.y >> 8) * NR + lane];
+  T y1 = hi ? Yg[(int64_t)(cc.y >> 8) * NR + 64 + lane] : (T)0;
+  int dl = gdl[cc.y >> 8];
   for (int32_t it = ib; it < ie; it++) {
-    const int4 item = items[it];
-    const int g = item.z;
-    const T *yg = Yg + (int64_t)g * NR;
-    __builtin_amdgcn_wave_barrier();
-    ys[lane] = yg[lane];
-    if (lane < NR - 64) ys[64 + lane] = yg[64 + lane];
-    const int dl = gdl[g];
-    const bool valid = j < item.y;
-    uint32_t s = 0, tslot = 0;
-    if (valid) {
-      s = src[item.x + j];
-      tslot = (uint32_t)(keys[item.x + j] & 127u);
+    // in flight during this item: columns of it + 2, operands of it + 1
+    const uint2 cnn = cols[(int64_t)min(it + 2, ie - 1) * 16 + j];
+    T bn[7][4];
+    load_b<T, W>(bn, Mq + (int64_t)cn.x * NR + q * 28);
+    const T yn0 = Yg[(int64_t)(cn.y >> 8) * NR + lane];
+    const T yn1 = hi ? Yg[(int64_t)(cn.y >> 8) * NR + 64 + lane] : (T)0;
+    const int dln = gdl[cn.y >> 8];
+
+    // the item's operand table: harmonics at multiples +1, -1, +2, -2
+    tbl[lane] = y0;
+    tbl[NR + lane] = -y0;
+    tbl[2 * NR + lane] = y0 + y0;
+    tbl[3 * NR + lane] = -(y0 + y0);
+    if (hi) {
+      tbl[64 + lane] = y1;
+      tbl[NR + 64 + lane] = -y1;
+      tbl[2 * NR + 64 + lane] = y1 + y1;
+      tbl[3 * NR + 64 + lane] = -(y1 + y1);
     }
     __builtin_amdgcn_wave_barrier();
-    V acc[5];
-    wave_item<T, W>(acc, code, ys, Mq + (int64_t)s * NR + q * 28, valid, dl);
-    if (valid) wave_store<T, W>(acc, Lt, Lt0b, tslot, q);
+    V acc[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) acc[f] = (V){0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+#pragma unroll
+      for (int e = 0; e < NB; e++) {
+        const T a = a_value(tbl, code[e * 4 + kk]);
+        acc[WAVE_F[W][e]] = Mfma<T>::step(a, bc[BLK_CB[WAVE_BLK[W][e]]][kk], acc[WAVE_F[W][e]]);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (cc.y & 128u) {
+      const uint32_t ts = cc.y & 127u;
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int r = Mfma<T>::row(q, i);
+          const T v = ldexp(acc[f][i], -jd[f][i] * dl);
+          if (W == 1 && WAVE_RBS[W][f] == 0) {
+            Lt0b[ts][r] += v;
+          } else {
+            const int row = WAVE_RBS[W][f] * 16 + r;
+            if (row < 110) Lt[ts][row] += v;
+          }
+        }
+      }
+    }
+    // advance the pipeline
+    cc = cn;
+    cn = cnn;
+#pragma unroll
+    for (int cb = 0; cb < 7; cb++)
+#pragma unroll
+      for (int kk = 0; kk < 4; kk++)
+        if (wave_needs_cb(W, cb)) bc[cb][kk] = bn[cb][kk];
+    y0 = yn0;
+    y1 = yn1;
+    dl = dln;
   }
 }
 
 template <class T>
 __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
-    const int4 *items, const int32_t *tile_off, const uint64_t *keys, const uint32_t *src,
-    const T *Yg, const int8_t *gdl, const T *Mq, const uint32_t *codes, Cells c, int32_t ncells,
-    double w0, double *L) {
+    const int32_t *tile_off, const uint2 *cols, const T *Yg, const int8_t *gdl, const T *Mq,
+    const uint32_t *codes, Cells c, int32_t ncells, double w0, double *L) {
   __shared__ T Lt[TILE][NR + 1];
   __shared__ T Lt0b[TILE][17];
-  __shared__ T Ys[TILE_WAVES][NR];
+  __shared__ T Tb[TILE_WAVES][NMULT * NR];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tile = blockIdx.x;
   for (int i = threadIdx.x; i < TILE * (NR + 1); i += 64 * TILE_WAVES) (&Lt[0][0])[i] = 0;
@@ -448,10 +482,10 @@ __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
   const int32_t ib = tile_off[tile], ie = tile_off[tile + 1];
   // every wave walks all items of the tile, in order, for its own blocks
   switch (wave) {
-    case 0: run_wave<T, 0>(items, ib, ie, keys, src, Yg, gdl, Mq, codes, Lt, Lt0b, Ys[0], lane); break;
-    case 1: run_wave<T, 1>(items, ib, ie, keys, src, Yg, gdl, Mq, codes, Lt, Lt0b, Ys[1], lane); break;
-    case 2: run_wave<T, 2>(items, ib, ie, keys, src, Yg, gdl, Mq, codes, Lt, Lt0b, Ys[2], lane); break;
-    default: run_wave<T, 3>(items, ib, ie, keys, src, Yg, gdl, Mq, codes, Lt, Lt0b, Ys[3], lane); break;
+    case 0: run_wave<T, 0>(cols, ib, ie, Yg, gdl, Mq, codes, Lt, Lt0b, Tb[0], lane); break;
+    case 1: run_wave<T, 1>(cols, ib, ie, Yg, gdl, Mq, codes, Lt, Lt0b, Tb[1], lane); break;
+    case 2: run_wave<T, 2>(cols, ib, ie, Yg, gdl, Mq, codes, Lt, Lt0b, Tb[2], lane); break;
+    default: run_wave<T, 3>(cols, ib, ie, Yg, gdl, Mq, codes, Lt, Lt0b, Tb[3], lane); break;
   }
   __syncthreads();
   for (int e = threadIdx.x; e < TILE * 16; e += 64 * TILE_WAVES) {
@@ -483,98 +517,18 @@ __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
 // host side
 // ---------------------------------------------------------------------------
 
-// A-operand codes: for block b, k-step kk and lane l, the entry
-// T[r][c] with r = 16 rb + l % 16, c = 16 cb + 4 kk + l / 16.
-static bool build_codes(std::vector<uint32_t> &codes) {
-  // T entries as up to two (flat index, sign) terms
-  struct Ent { int i1 = -1, s1 = 0, i2 = -1, s2 = 0; };
-  std::vector<Ent> T((size_t)NR * NR);
-  bool ok = true;
-  auto add = [&](int r, int cc, int yi, int sg) {
-    Ent &e = T[(size_t)r * NR + cc];
-    if (e.i1 < 0) { e.i1 = yi; e.s1 = sg; }
-    else if (e.i1 == yi) { e.s1 += sg; }
-    else if (e.i2 < 0) { e.i2 = yi; e.s2 = sg; }
-    else if (e.i2 == yi) { e.s2 += sg; }
-    else { ok = false; }  // more than two harmonics in one entry
-  };
-  auto oddeven = [](int n) { return (n & 1) ? -1 : 1; };
-  for (int j = 0; j < P; j++) {
-    for (int k = 0; k <= j; k++) {
-      const int jk = tri(j, k);
-      const int Cnm = oddeven(j);
-      for (int n = 0; n < P - j; n++) {
-        for (int m = -n; m <= n; m++) {
-          const int p = j + n, qq = m - k;
-          // Y_p^qq = cy + i dy, stored for qq >= 0; conj for qq < 0
-          const int base = 2 * tri(p, qq >= 0 ? qq : -qq);
-          const int cy = base, dy = base + 1, dys = qq >= 0 ? 1 : -1;
-          const int mm = m >= 0 ? m : -m;
-          const int cn = 2 * tri(n, mm);
-          int s = (m < 0) ? Cnm : Cnm * oddeven((k - m) * (k < m) + m);
-          if (m >= 0) {
-            add(2 * jk, cn, cy, s);
-            add(2 * jk, cn + 1, dy, -s * dys);
-            add(2 * jk + 1, cn, dy, s * dys);
-            add(2 * jk + 1, cn + 1, cy, s);
-          } else {
-            add(2 * jk, cn, cy, s);
-            add(2 * jk, cn + 1, dy, s * dys);
-            add(2 * jk + 1, cn, dy, s * dys);
-            add(2 * jk + 1, cn + 1, cy, -s);
-          }
-        }
-      }
-    }
-  }
-  auto enc = [&](int s) -> uint32_t {
-    if (s < -2 || s > 2) ok = false;
-    return (uint32_t)(s + 2);
-  };
-  auto degree_of_col = [](int cc) {
-    if (cc >= 110) return 0;
-    const int t = cc >> 1;
-    int n = 0;
-    while (tri(n + 1, 0) <= t) n++;
-    return n;
-  };
-  codes.assign((size_t)NSLOT * 64, 0u);
-  for (int b = 0; b < NBLK; b++) {
-    for (int kk = 0; kk < 4; kk++) {
-      for (int l = 0; l < 64; l++) {
-        const int r = 16 * BLK_RB[b] + (l % 16);
-        const int cc = 16 * BLK_CB[b] + 4 * kk + l / 16;
-        const Ent e = T[(size_t)r * NR + cc];
-        uint32_t w = 0;
-        w |= (e.i1 >= 0 ? (uint32_t)e.i1 : 0u) | (enc(e.i1 >= 0 ? e.s1 : 0) << 7);
-        w |= ((e.i2 >= 0 ? (uint32_t)e.i2 : 0u) << 10) | (enc(e.i2 >= 0 ? e.s2 : 0) << 17);
-        w |= (uint32_t)degree_of_col(cc) << 20;
-        codes[(size_t)(b * 4 + kk) * 64 + l] = w;
-      }
-    }
-  }
-  // every non-zero entry of T must lie in one of the listed blocks
-  for (int r = 0; r < NR; r++) {
-    for (int cc = 0; cc < NR; cc++) {
-      const Ent &e = T[(size_t)r * NR + cc];
-      if ((e.i1 < 0 || e.s1 == 0) && (e.i2 < 0 || e.s2 == 0)) continue;
-      bool in = false;
-      for (int b = 0; b < NBLK; b++)
-        in |= (r / 16 == BLK_RB[b] && cc / 16 == BLK_CB[b]);
-      ok &= in;
-    }
-  }
-  return ok;
-}
-
+// Operand codes of both element types, FP32 first (NCODE words each).
 hipError_t m2l_upload_codes(uint32_t *dev_codes, hipStream_t s) {
-  std::vector<uint32_t> codes;
-  if (!build_codes(codes)) return hipErrorInvalidValue;
-  hipError_t e = hipMemcpyAsync(dev_codes, codes.data(), codes.size() * 4,
-                                hipMemcpyHostToDevice, s);
+  std::vector<uint32_t> c32, c64;
+  if (!m2lc::build_codes(c32, 4) || !m2lc::build_codes(c64, 8)) return hipErrorInvalidValue;
+  if (c32.size() != (size_t)NCODE || c64.size() != (size_t)NCODE) return hipErrorInvalidValue;
+  c32.insert(c32.end(), c64.begin(), c64.end());
+  hipError_t e = hipMemcpyAsync(dev_codes, c32.data(), c32.size() * 4, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return e;
   return hipStreamSynchronize(s);
 }
+
+size_t m2l_codes_bytes() { return (size_t)2 * NCODE * 4; }
 
 int32_t m2l_group_slots(int max_depth) { return (max_depth + 1) * SLOTS_PER_LEVEL; }
 
@@ -612,11 +566,12 @@ hipError_t launch_m2l_scale_m(Cells c, int32_t ncells, const double *M, double w
   return hipGetLastError();
 }
 
-hipError_t launch_m2l_items(const uint64_t *keys, uint32_t n, int gbits, uint32_t *flag,
-                            uint32_t *excl, uint32_t *segstart, uint32_t *iflag,
-                            uint32_t *iexcl, void *scan_tmp, uint32_t *h_pinned, int4 *items,
-                            size_t items_cap, uint32_t *nitems_out, hipStream_t s) {
+hipError_t launch_m2l_count_items(const uint64_t *keys, uint32_t n, uint32_t *flag,
+                                  uint32_t *excl, uint32_t *segstart, uint32_t *iflag,
+                                  uint32_t *iexcl, void *scan_tmp, uint32_t *h_pinned,
+                                  uint32_t *nseg_out, uint32_t *nitems_out, hipStream_t s) {
   *nitems_out = 0;
+  *nseg_out = 0;
   if (n == 0) return hipSuccess;
   const unsigned nb = (n + 255) / 256;
   k_seg_flags<<<nb, 256, 0, s>>>(keys, n, flag);
@@ -639,13 +594,19 @@ hipError_t launch_m2l_items(const uint64_t *keys, uint32_t n, int gbits, uint32_
   if (e != hipSuccess) return e;
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) return e;
-  const uint32_t nitems = h_pinned[0] + h_pinned[1];
-  if (nitems > items_cap) {
-    *nitems_out = nitems;
-    return hipErrorOutOfMemory;
-  }
-  k_items<<<nb, 256, 0, s>>>(keys, flag, excl, segstart, nseg, iflag, iexcl, n, gbits, items);
-  *nitems_out = nitems;
+  *nseg_out = nseg;
+  *nitems_out = h_pinned[0] + h_pinned[1];
+  return hipGetLastError();
+}
+
+hipError_t launch_m2l_write_items(const uint64_t *keys, const uint32_t *vals, uint32_t n,
+                                  int gbits, const uint32_t *flag, const uint32_t *excl,
+                                  const uint32_t *segstart, uint32_t nseg, const uint32_t *iflag,
+                                  const uint32_t *iexcl, int4 *items, uint2 *cols,
+                                  hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_items<<<(n + 255) / 256, 256, 0, s>>>(keys, vals, flag, excl, segstart, nseg, iflag, iexcl,
+                                          n, gbits, items, cols);
   return hipGetLastError();
 }
 
@@ -655,21 +616,19 @@ hipError_t launch_m2l_tile_offsets(const int4 *items, uint32_t nitems, int32_t n
   return hipGetLastError();
 }
 
-hipError_t launch_m2l_tiles(const int4 *items, const int32_t *tile_off, int32_t ntiles,
-                            const uint64_t *keys, const uint32_t *src, const void *Yg,
-                            const int8_t *gdl, const void *Mq, const uint32_t *codes, Cells c,
-                            int32_t ncells, double w0, double *L, bool fp32, hipStream_t s) {
+hipError_t launch_m2l_tiles(const int32_t *tile_off, int32_t ntiles, const uint2 *cols,
+                            const void *Yg, const int8_t *gdl, const void *Mq,
+                            const uint32_t *codes, Cells c, int32_t ncells, double w0, double *L,
+                            bool fp32, hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
   if (fp32)
-    k_m2l_tiles<float><<<ntiles, 64 * TILE_WAVES, 0, s>>>(items, tile_off, keys, src,
-                                                          (const float *)Yg, gdl,
+    k_m2l_tiles<float><<<ntiles, 64 * TILE_WAVES, 0, s>>>(tile_off, cols, (const float *)Yg, gdl,
                                                           (const float *)Mq, codes, c, ncells,
                                                           w0, L);
   else
-    k_m2l_tiles<double><<<ntiles, 64 * TILE_WAVES, 0, s>>>(items, tile_off, keys, src,
-                                                           (const double *)Yg, gdl,
-                                                           (const double *)Mq, codes, c, ncells,
-                                                           w0, L);
+    k_m2l_tiles<double><<<ntiles, 64 * TILE_WAVES, 0, s>>>(tile_off, cols, (const double *)Yg,
+                                                           gdl, (const double *)Mq,
+                                                           codes + NCODE, c, ncells, w0, L);
   return hipGetLastError();
 }
 

@@ -94,6 +94,7 @@ namespace tfmm {
 int32_t m2l_group_slots(int max_depth);
 int m2l_tile_size();
 hipError_t m2l_upload_codes(uint32_t *dev_codes, hipStream_t s);
+size_t m2l_codes_bytes();
 hipError_t launch_m2l_mark_slots(Cells c, const int2 *pairs, uint32_t n, uint32_t *used,
                                  hipStream_t s);
 hipError_t launch_m2l_classify(Cells c, const int2 *pairs, uint32_t n,
@@ -105,16 +106,21 @@ hipError_t launch_m2l_group_harmonics(const uint32_t *used, const uint32_t *gid_
                                       int8_t *gdl, int8_t *glt, hipStream_t s);
 hipError_t launch_m2l_scale_m(Cells c, int32_t ncells, const double *M, double w0, void *Mq,
                               bool fp32, hipStream_t s);
-hipError_t launch_m2l_items(const uint64_t *keys, uint32_t n, int gbits, uint32_t *flag,
-                            uint32_t *excl, uint32_t *segstart, uint32_t *iflag,
-                            uint32_t *iexcl, void *scan_tmp, uint32_t *h_pinned, int4 *items,
-                            size_t items_cap, uint32_t *nitems_out, hipStream_t s);
+hipError_t launch_m2l_count_items(const uint64_t *keys, uint32_t n, uint32_t *flag,
+                                  uint32_t *excl, uint32_t *segstart, uint32_t *iflag,
+                                  uint32_t *iexcl, void *scan_tmp, uint32_t *h_pinned,
+                                  uint32_t *nseg_out, uint32_t *nitems_out, hipStream_t s);
+hipError_t launch_m2l_write_items(const uint64_t *keys, const uint32_t *vals, uint32_t n,
+                                  int gbits, const uint32_t *flag, const uint32_t *excl,
+                                  const uint32_t *segstart, uint32_t nseg, const uint32_t *iflag,
+                                  const uint32_t *iexcl, int4 *items, uint2 *cols,
+                                  hipStream_t s);
 hipError_t launch_m2l_tile_offsets(const int4 *items, uint32_t nitems, int32_t ntiles,
                                    int32_t *off, hipStream_t s);
-hipError_t launch_m2l_tiles(const int4 *items, const int32_t *tile_off, int32_t ntiles,
-                            const uint64_t *keys, const uint32_t *src, const void *Yg,
-                            const int8_t *gdl, const void *Mq, const uint32_t *codes, Cells c,
-                            int32_t ncells, double w0, double *L, bool fp32, hipStream_t s);
+hipError_t launch_m2l_tiles(const int32_t *tile_off, int32_t ntiles, const uint2 *cols,
+                            const void *Yg, const int8_t *gdl, const void *Mq,
+                            const uint32_t *codes, Cells c, int32_t ncells, double w0, double *L,
+                            bool fp32, hipStream_t s);
 
 // p2p.hip (mixed precision)
 hipError_t launch_leaf_relpos(Cells c, int32_t ncells, Bodies b, float4 *rel, hipStream_t s);
