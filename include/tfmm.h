@@ -95,6 +95,8 @@ typedef struct {
   /* M2L batching: offset groups, MFMA work items (<= 16 pairs of one group
    * and one 128-cell target tile each) and pairs outside the groups */
   uint64_t n_m2l_groups, n_m2l_items, n_m2l_fallback;
+  /* the batched M2L tile kernel alone (its one launch, inside ms_m2l) */
+  double ms_m2l_tiles;
 } tfmm_stats_t;
 
 int tfmm_create(int hip_device, tfmm_ctx **out);

@@ -29,6 +29,7 @@ enum Ev {
   EV_DTT0,
   EV_DTT1,
   EV_M2L0,
+  EV_M2LT,
   EV_M2L1,
   EV_P2P0,
   EV_P2P1,
@@ -68,7 +69,7 @@ struct tfmm_ctx {
 
   // traversal
   DevBuf front_a, front_b, m2l_pairs, p2p_pairs, cnt, lkeys, lkeys_alt, lidx, lidx_alt;
-  DevBuf m2l_off, m2l_src, p2p_off, p2p_src, pair_count;
+  DevBuf m2l_off, m2l_src, p2p_off, p2p_src;
   uint32_t m2l_cap = 0, p2p_cap = 0;
   uint32_t n_m2l_exec = 0, n_p2p = 0;
   uint64_t n_m2l_all = 0, n_p2p_calls = 0, n_p2p_self = 0;
@@ -78,7 +79,7 @@ struct tfmm_ctx {
   DevBuf g_used, g_gid, g_y64, g_y32, g_dl, g_lt, g_cnt;
   DevBuf gkeys, gvals, gkeys_alt, gvals_alt, fkeys, fvals, fkeys_alt, fvals_alt;
   DevBuf it_flag, it_excl, it_seg, it_iflag, it_iexcl, items, cols, tile_off, mq, codes;
-  DevBuf fb_off, fb_src, relpos;
+  DevBuf fb_off, fb_src, fb_part, relpos, leaf_of, tops;
   bool codes_ready = false;
   uint32_t n_groups = 0, n_grouped = 0, n_fallback = 0, n_items = 0;
   bool grouped_keys_in_alt = false;
@@ -269,8 +270,14 @@ int do_upward(tfmm_ctx *c) {
   const Cells k = cells_of(c);
   TRY(launch_p2m(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->M), s));
   const int nl = (int)c->lvl.size() - 1;
+  // translation operators of every level with children (used again by L2L)
+  const size_t opsz = translation_ops_doubles();
+  TRY(dev_reserve(c->tops, (size_t)std::max(nl - 1, 1) * opsz * 8));
+  for (int d = 0; d + 1 < nl; d++)
+    TRY(launch_translation_ops(k, c->lvl[d + 1], dp<double>(c->tops) + (size_t)d * opsz, s));
   for (int d = nl - 2; d >= 0; d--)
-    TRY(launch_m2m(k, c->lvl[d], c->lvl[d + 1] - c->lvl[d], dp<double>(c->M), s));
+    TRY(launch_m2m(k, c->lvl[d], c->lvl[d + 1] - c->lvl[d], dp<double>(c->tops) + (size_t)d * opsz,
+                   dp<double>(c->M), s));
   return TFMM_OK;
 }
 
@@ -458,10 +465,6 @@ int do_traverse(tfmm_ctx *c) {
   if (rc) return rc;
   rc = build_m2l_batches(c);
   if (rc) return rc;
-  TRY(dev_reserve(c->pair_count, 16));
-  TRY(hipMemsetAsync(c->pair_count.ptr, 0, 16, s));
-  TRY(launch_p2p_pair_count(k, dp<int2>(c->p2p_pairs), c->n_p2p,
-                            dp<unsigned long long>(c->pair_count), s));
   // M~ in the MFMA operand layout
   TRY(dev_reserve(c->mq, (size_t)c->ncells * 112 * (mixed ? 4 : 8)));
   TRY(launch_m2l_scale_m(k, (int32_t)c->ncells, dp<double>(c->M), c->w0, c->mq.ptr, mixed, s));
@@ -480,9 +483,12 @@ int do_traverse(tfmm_ctx *c) {
                        mixed ? (const void *)c->g_y32.ptr : (const void *)c->g_y64.ptr,
                        dp<int8_t>(c->g_dl), c->mq.ptr, dp<uint32_t>(c->codes), k,
                        (int32_t)c->ncells, c->w0, dp<double>(c->L), mixed, s));
+  TRY(hipEventRecord(c->ev[EV_M2LT], s));
   if (c->n_fallback) {
     Csr fl{dp<int32_t>(c->fb_off), dp<int32_t>(c->fb_src)};
-    TRY(launch_m2l_fp64(k, (int32_t)c->ncells, fl, dp<double>(c->M), dp<double>(c->L), s));
+    TRY(dev_reserve(c->fb_part, m2l_pairs_scratch_bytes((int32_t)c->n_fallback)));
+    TRY(launch_m2l_fp64(k, (int32_t)c->ncells, fl, (int32_t)c->n_fallback, dp<double>(c->M),
+                        dp<double>(c->L), c->fb_part.ptr, s));
   }
   TRY(hipEventRecord(c->ev[EV_M2L1], s));
 
@@ -495,7 +501,7 @@ int do_traverse(tfmm_ctx *c) {
   else
     TRY(launch_p2p_fp64(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<double>(c->acc), s));
   TRY(hipEventRecord(c->ev[EV_P2P1], s));
-  TRY(hipMemcpyAsync(c->h_u64, c->pair_count.ptr, 16, hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(c->h_u64, static_cast<char *>(c->cnt.ptr) + 32, 16, hipMemcpyDeviceToHost, s));
   return TFMM_OK;
 }
 
@@ -503,13 +509,16 @@ int do_downward(tfmm_ctx *c) {
   hipStream_t s = c->stream;
   const Cells k = cells_of(c);
   const int nl = (int)c->lvl.size() - 1;
+  const size_t opsz = translation_ops_doubles();
   for (int d = 0; d + 1 < nl; d++)
-    TRY(launch_l2l(k, c->lvl[d], c->lvl[d + 1] - c->lvl[d], dp<double>(c->L), s));
+    TRY(launch_l2l(k, c->lvl[d], c->lvl[d + 1] - c->lvl[d], dp<double>(c->tops) + (size_t)d * opsz,
+                   dp<double>(c->L), s));
   TRY(dev_reserve(c->trg, (size_t)c->n * 32));
   TRY(dev_reserve(c->out, (size_t)c->n * 32));
+  TRY(dev_reserve(c->leaf_of, (size_t)c->n * 4));
   TRY(launch_l2p(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->L), dp<double>(c->acc),
-                 dp<double>(c->trg), s));
-  TRY(launch_unpermute(dp<int32_t>(c->perm), dp<double>(c->trg), c->n, dp<double>(c->out), s));
+                 dp<int32_t>(c->perm), dp<int32_t>(c->leaf_of), dp<double>(c->trg),
+                 dp<double>(c->out), s));
   return TFMM_OK;
 }
 
@@ -561,12 +570,12 @@ void tfmm_destroy(tfmm_ctx *c) {
                     &c->c_child, &c->c_depth, &c->c_center, &c->c_width, &c->flag, &c->slot,
                     &c->scan_tmp, &c->M, &c->L, &c->front_a, &c->front_b, &c->m2l_pairs,
                     &c->p2p_pairs, &c->cnt, &c->lkeys, &c->lkeys_alt, &c->lidx, &c->lidx_alt,
-                    &c->m2l_off, &c->m2l_src, &c->p2p_off, &c->p2p_src, &c->pair_count, &c->acc,
+                    &c->m2l_off, &c->m2l_src, &c->p2p_off, &c->p2p_src, &c->acc,
                     &c->trg, &c->out, &c->g_used, &c->g_gid, &c->g_y64, &c->g_y32, &c->g_dl,
                     &c->g_lt, &c->g_cnt, &c->gkeys, &c->gvals, &c->gkeys_alt, &c->gvals_alt,
                     &c->fkeys, &c->fvals, &c->fkeys_alt, &c->fvals_alt, &c->it_flag,
                     &c->it_excl, &c->it_seg, &c->it_iflag, &c->it_iexcl, &c->items,
-                    &c->tile_off, &c->cols, &c->mq, &c->codes, &c->fb_off, &c->fb_src, &c->relpos,
+                    &c->tile_off, &c->cols, &c->mq, &c->codes, &c->fb_off, &c->fb_src, &c->fb_part, &c->relpos, &c->leaf_of, &c->tops,
                     &c->feat_acc, &c->feat_x, &c->feat_out};
   for (DevBuf *b : bufs) dev_release(*b);
   for (int i = 0; i < EV_COUNT; i++)
@@ -751,6 +760,7 @@ int tfmm_get_stats(tfmm_ctx *c, tfmm_stats_t *o) {
     o->ms_traverse = el(EV_UP, EV_TRAV);
     o->ms_dtt = el(EV_DTT0, EV_DTT1);
     o->ms_m2l = el(EV_M2L0, EV_M2L1);
+    o->ms_m2l_tiles = el(EV_M2L0, EV_M2LT);
     o->ms_p2p = el(EV_P2P0, EV_P2P1);
   }
   if (c->stage >= 4) {

@@ -1,12 +1,13 @@
-// m2l.hip -- FP64 M2L over a per-target CSR list.
+// m2l.hip -- FP64 M2L of individual pairs (those outside the batched
+// groups: level difference >= 2).
 //
 // Reference: M2L (sample/exafmm-dev-13274dd4ac68/examples/
 // LaplaceSphericalCPU_tapas.h:208-273) with evalLocal (:102-135), mutual = 0:
 //   L_i[j,k] += sum_{n < P-j} sum_{|m| <= n} s(j,k,n,m) M_j[n,m] Y_{j+n}^{m-k}(dX)
-// dX = centre_i - centre_j.  One wave64 per target cell owns the target's L
-// for the whole source list (no atomics); per source, lanes 0..9 build one
-// column of the singular harmonics each into LDS, then lane t < 55 forms
-// output coefficient t.
+// dX = centre_i - centre_j.  One wave64 per pair: lanes 0..9 build one column
+// of the singular harmonics each into LDS, then lane t < 55 forms output
+// coefficient t into a per-pair partial.  A second kernel adds the partials
+// of each target in list order (deterministic, no atomics).
 #include "tfmm_device.h"
 #include "tfmm_kernels.h"
 
@@ -15,65 +16,91 @@ namespace {
 
 __device__ inline int sgn(int n) { return (n & 1) ? -1 : 1; }
 
-__global__ __launch_bounds__(64) void k_m2l_fp64(Cells c, Csr list, const double *M,
-                                                 double *L) {
-  const int32_t ti = blockIdx.x;
-  const int32_t b = list.off[ti], e = list.off[ti + 1];
-  if (b == e) return;
-  __shared__ cd Ys[NTERM];
-  __shared__ cd Ms[NTERM];
-  const int lane = threadIdx.x;
+constexpr int PAIR_WAVES = 4;
+
+__global__ __launch_bounds__(64 * PAIR_WAVES) void k_m2l_pairs_fp64(Cells c, Csr list,
+                                                                   int32_t npairs,
+                                                                   const int32_t *tgt,
+                                                                   const double *M, cd *part) {
+  __shared__ cd Ys[PAIR_WAVES][NTERM];
+  __shared__ cd Ms[PAIR_WAVES][NTERM];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int32_t pi = blockIdx.x * PAIR_WAVES + w;
+  if (pi >= npairs) return;  // whole wave leaves together; no block barrier follows
+  const int32_t ti = tgt[pi], sj = list.src[pi];
   int j = 0, k = 0;
   if (lane < NTERM) {
     while (tri(j + 1, 0) <= lane) j++;
     k = lane - tri(j, 0);
   }
-  const double cx = c.center[3 * (int64_t)ti], cy = c.center[3 * (int64_t)ti + 1],
-               cz = c.center[3 * (int64_t)ti + 2];
   const cd *Mall = reinterpret_cast<const cd *>(M);
+  if (lane < NTERM) Ms[w][lane] = Mall[(int64_t)sj * NTERM + lane];
+  if (lane < P) {
+    const double dx = c.center[3 * (int64_t)ti] - c.center[3 * (int64_t)sj];
+    const double dy = c.center[3 * (int64_t)ti + 1] - c.center[3 * (int64_t)sj + 1];
+    const double dz = c.center[3 * (int64_t)ti + 2] - c.center[3 * (int64_t)sj + 2];
+    double rho, ca, sa;
+    cd ei;
+    sph_frame(dx, dy, dz, rho, ca, sa, ei);
+    local_column(lane, rho, ca, sa, ei, [&](int n, int m, cd Y) { Ys[w][tri(n, m)] = Y; });
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  if (lane >= NTERM) return;
   const double Cnm = sgn(j);
   cd acc = mk(0, 0);
-  for (int32_t s = b; s < e; s++) {
-    const int32_t sj = list.src[s];
-    if (lane < NTERM) Ms[lane] = Mall[(int64_t)sj * NTERM + lane];
-    if (lane < P) {
-      const double dx = cx - c.center[3 * (int64_t)sj];
-      const double dy = cy - c.center[3 * (int64_t)sj + 1];
-      const double dz = cz - c.center[3 * (int64_t)sj + 2];
-      double rho, ca, sa;
-      cd ei;
-      sph_frame(dx, dy, dz, rho, ca, sa, ei);
-      local_column(lane, rho, ca, sa, ei,
-                   [&](int n, int m, cd Y) { Ys[tri(n, m)] = Y; });
+  for (int n = 0; n < P - j; n++) {
+    for (int m = -n; m < 0; m++) {
+      const cd t = cmul(cscale(cconj(Ms[w][tri(n, -m)]), Cnm), yfull(Ys[w], j + n, m - k));
+      acc = cadd(acc, t);
     }
-    __syncthreads();
-    if (lane < NTERM) {
-      for (int n = 0; n < P - j; n++) {
-        for (int m = -n; m < 0; m++) {
-          const cd t = cmul(cscale(cconj(Ms[tri(n, -m)]), Cnm), yfull(Ys, j + n, m - k));
-          acc = cadd(acc, t);
-        }
-        for (int m = 0; m <= n; m++) {
-          const double Cnm2 = Cnm * sgn((k - m) * (k < m) + m);
-          const cd t = cmul(cscale(Ms[tri(n, m)], Cnm2), yfull(Ys, j + n, m - k));
-          acc = cadd(acc, t);
-        }
-      }
+    for (int m = 0; m <= n; m++) {
+      const double Cnm2 = Cnm * sgn((k - m) * (k < m) + m);
+      const cd t = cmul(cscale(Ms[w][tri(n, m)], Cnm2), yfull(Ys[w], j + n, m - k));
+      acc = cadd(acc, t);
     }
-    __syncthreads();
   }
-  if (lane < NTERM) {
-    cd *Li = reinterpret_cast<cd *>(L) + (int64_t)ti * NTERM;
-    Li[lane] = cadd(Li[lane], acc);
-  }
+  part[(int64_t)pi * NTERM + lane] = acc;
+}
+
+// target of every list entry
+__global__ void k_list_targets(const int32_t *off, int32_t ncells, int32_t *tgt) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncells) return;
+  for (int32_t e = off[t]; e < off[t + 1]; e++) tgt[e] = t;
+}
+
+// L_t += partials of t's pairs, in list order; one wave per target
+__global__ __launch_bounds__(256) void k_m2l_reduce(Csr list, int32_t ncells, const cd *part,
+                                                    double *L) {
+  const int lane = threadIdx.x & 63;
+  const int32_t ti = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (ti >= ncells || lane >= NTERM) return;
+  const int32_t b = list.off[ti], e = list.off[ti + 1];
+  if (b == e) return;
+  cd acc = mk(0, 0);
+  for (int32_t p = b; p < e; p++) acc = cadd(acc, part[(int64_t)p * NTERM + lane]);
+  cd *Li = reinterpret_cast<cd *>(L) + (int64_t)ti * NTERM;
+  Li[lane] = cadd(Li[lane], acc);
 }
 
 }  // namespace
 
-hipError_t launch_m2l_fp64(Cells c, int32_t ncells, Csr list, const double *M, double *L,
-                           hipStream_t s) {
-  if (ncells <= 0) return hipSuccess;
-  k_m2l_fp64<<<ncells, 64, 0, s>>>(c, list, M, L);
+size_t m2l_pairs_scratch_bytes(int32_t npairs) {
+  return (size_t)npairs * (NTERM * sizeof(cd) + sizeof(int32_t));
+}
+
+// scratch: m2l_pairs_scratch_bytes(npairs) bytes (partials, then targets)
+hipError_t launch_m2l_fp64(Cells c, int32_t ncells, Csr list, int32_t npairs, const double *M,
+                           double *L, void *scratch, hipStream_t s) {
+  if (ncells <= 0 || npairs <= 0) return hipSuccess;
+  cd *part = static_cast<cd *>(scratch);
+  int32_t *tgt = reinterpret_cast<int32_t *>(part + (size_t)npairs * NTERM);
+  k_list_targets<<<(ncells + 255) / 256, 256, 0, s>>>(list.off, ncells, tgt);
+  k_m2l_pairs_fp64<<<(npairs + PAIR_WAVES - 1) / PAIR_WAVES, 64 * PAIR_WAVES, 0, s>>>(
+      c, list, npairs, tgt, M, part);
+  k_m2l_reduce<<<(ncells + 3) / 4, 256, 0, s>>>(list, ncells, part, L);
   return hipGetLastError();
 }
 

@@ -113,6 +113,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const uint64_t *keys_in, const uint32_t *vals_in, uint64_t *keys_out, uint32_t *vals_out,
     int64_t n, int shift, int nbits, const uint32_t *offsets, int nblocks) {
   __shared__ uint32_t whist[RS_WAVES][RS_BINS];
+  __shared__ uint32_t dstart[RS_BINS];
+  __shared__ uint64_t sk[RS_TILE];
+  __shared__ uint32_t sv[RS_TILE];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const uint32_t mask = (1u << nbits) - 1u;
   for (int i = lane; i < RS_BINS; i += 64) whist[wid][i] = 0;
@@ -144,25 +147,43 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     rank[r] = base + before;
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < RS_BINS; d += RS_THREADS) {
-    uint32_t s = 0;
+  // per digit: offsets of the waves (exclusive, in wave order) and the tile's
+  // count; then the tile's digit starts (exclusive scan over digits)
+  static_assert(RS_BINS == RS_THREADS, "one digit per thread");
+  uint32_t cnt = 0;
+  {
+    const int d = threadIdx.x;
     for (int w = 0; w < RS_WAVES; w++) {
       const uint32_t t = whist[w][d];
-      whist[w][d] = s;
-      s += t;
+      whist[w][d] = cnt;
+      cnt += t;
     }
   }
+  uint32_t unused;
+  const uint32_t start = block_excl_scan(cnt, &unused);
+  dstart[threadIdx.x] = start;
   __syncthreads();
+  // stage the tile in LDS in digit order (stable), then write each digit's
+  // run to consecutive global positions
 #pragma unroll
   for (int r = 0; r < RS_ROWS; r++) {
     const int64_t j = wbase + r * 64 + lane;
     if (j < n) {
       const uint32_t d = digit_of(k[r], shift, mask);
-      const uint64_t pos =
-          (uint64_t)offsets[(int64_t)d * nblocks + blockIdx.x] + whist[wid][d] + rank[r];
-      keys_out[pos] = k[r];
-      vals_out[pos] = v[r];
+      const uint32_t lpos = dstart[d] + whist[wid][d] + rank[r];
+      sk[lpos] = k[r];
+      sv[lpos] = v[r];
     }
+  }
+  __syncthreads();
+  const int64_t tbase = (int64_t)blockIdx.x * RS_TILE;
+  const int tn = (int)((n - tbase) < RS_TILE ? (n - tbase) : RS_TILE);
+  for (int i = threadIdx.x; i < tn; i += RS_THREADS) {
+    const uint64_t key = sk[i];
+    const uint32_t d = digit_of(key, shift, mask);
+    const uint64_t pos = (uint64_t)offsets[(int64_t)d * nblocks + blockIdx.x] + (uint32_t)i - dstart[d];
+    keys_out[pos] = key;
+    vals_out[pos] = sv[i];
   }
 }
 

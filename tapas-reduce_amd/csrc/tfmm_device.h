@@ -156,33 +156,43 @@ __device__ inline void local_column(int m, double rho, double x, double y, cd ei
   }
 }
 
+// Reciprocals of the integer divisors of the recurrences: the divisions of
+// the reference's recurrences become multiplications (a difference of at
+// most one rounding per step).
+__host__ __device__ constexpr double recip_int(int k) { return 1.0 / (double)k; }
+
 // All columns of the regular harmonics in the reference's order (m outer,
 // n inner), one serial recurrence per lane (evalMultipole,
 // LaplaceSphericalCPU_tapas.h:63-99).  store(n, m, Y, dY/dtheta) is called
-// once per (n, m >= 0); the theta derivative is only formed if WithTheta.
+// once per (n, m >= 0), 55 calls in a fixed order; the theta derivative is
+// only formed if WithTheta (inv_y = 1 / sin(theta) is then required).
 template <bool WithTheta, class Store>
-__device__ inline void multipole_all(double rho, double x, double y, cd ei, Store store) {
+__device__ inline void multipole_all(double rho, double x, double y, cd ei, Store store,
+                                     double inv_y = 0.0) {
   double fact = 1, pn = 1, rhom = 1;
   cd eim = mk(1.0, 0.0);
+#pragma unroll
   for (int m = 0; m < P; m++) {
     double p = pn;
     double p1 = p;
     p = x * (2 * m + 1) * p1;
     store(m, m, cscale(eim, rhom * p1),
-          WithTheta ? cscale(eim, rhom * (p - (m + 1) * x * p1) / y) : mk(0, 0));
+          WithTheta ? cscale(eim, rhom * (p - (m + 1) * x * p1) * inv_y) : mk(0, 0));
     rhom *= rho;
     double rhon = rhom;
+#pragma unroll
     for (int n = m + 1; n < P; n++) {
-      rhon /= -(n + m);
+      rhon *= -recip_int(n + m);
       const cd yv = cscale(eim, rhon * p);
       const double p2 = p1;
       p1 = p;
-      p = (x * (2 * n + 1) * p1 - (n + m) * p2) / (n - m + 1);
-      store(n, m, yv, WithTheta ? cscale(eim, rhon * ((n - m + 1) * p - (n + 1) * x * p1) / y)
-                                : mk(0, 0));
+      p = (x * (2 * n + 1) * p1 - (n + m) * p2) * recip_int(n - m + 1);
+      store(n, m, yv,
+            WithTheta ? cscale(eim, rhon * ((n - m + 1) * p - (n + 1) * x * p1) * inv_y)
+                      : mk(0, 0));
       rhon *= rho;
     }
-    rhom /= -(2 * m + 2) * (2 * m + 1);
+    rhom *= -recip_int((2 * m + 2) * (2 * m + 1));
     pn = -pn * fact * y;
     fact += 2;
     eim = cmul(eim, ei);

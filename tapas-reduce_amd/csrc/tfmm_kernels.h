@@ -48,13 +48,17 @@ hipError_t launch_children(Cells c, int32_t lbeg, int32_t lcnt, const uint32_t *
 
 // expansions.hip (FP64)
 hipError_t launch_p2m(Cells c, int32_t ncells, Bodies b, double *M, hipStream_t s);
-hipError_t launch_m2m(Cells c, int32_t lbeg, int32_t lcnt, double *M, hipStream_t s);
-hipError_t launch_l2l(Cells c, int32_t lbeg, int32_t lcnt, double *L, hipStream_t s);
+// translate.hip (FP64 M2M / L2L on MFMA)
+size_t translation_ops_doubles();
+hipError_t launch_translation_ops(Cells c, int32_t first_child_cell, double *ops,
+                                  hipStream_t s);
+hipError_t launch_m2m(Cells c, int32_t lbeg, int32_t lcnt, const double *ops, double *M,
+                      hipStream_t s);
+hipError_t launch_l2l(Cells c, int32_t lbeg, int32_t lcnt, const double *ops, double *L,
+                      hipStream_t s);
 hipError_t launch_l2p(Cells c, int32_t ncells, Bodies b, const double *L, const double *p2p,
-                      double *trg, hipStream_t s);
-hipError_t launch_unpermute(const int32_t *perm, const double *trg_sorted, int64_t n,
-                            double *trg_orig, hipStream_t s);
-
+                      const int32_t *perm, int32_t *leaf_of, double *trg, double *out,
+                      hipStream_t s);
 // traverse.hip
 struct DttCounters {
   // [0] next-frontier size, [1] executed M2L pairs, [2] executed P2P leaf pairs,
@@ -71,12 +75,10 @@ hipError_t launch_unpack_pairs(const uint64_t *keys, uint32_t n, int sbits, int3
                                hipStream_t s);
 hipError_t launch_csr_offsets(const uint64_t *sorted_keys, uint32_t n, int sbits,
                               int32_t ncells, int32_t *off, hipStream_t s);
-hipError_t launch_p2p_pair_count(Cells c, const int2 *p2p, uint32_t n,
-                                 unsigned long long *out, hipStream_t s);
-
 // m2l.hip
-hipError_t launch_m2l_fp64(Cells c, int32_t ncells, Csr list, const double *M, double *L,
-                           hipStream_t s);
+size_t m2l_pairs_scratch_bytes(int32_t npairs);
+hipError_t launch_m2l_fp64(Cells c, int32_t ncells, Csr list, int32_t npairs, const double *M,
+                           double *L, void *scratch, hipStream_t s);
 
 // p2p.hip
 hipError_t launch_p2p_fp64(Cells c, int32_t ncells, Csr list, Bodies b, double *acc,

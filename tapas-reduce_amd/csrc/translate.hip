@@ -1,0 +1,218 @@
+// translate.hip -- FP64 M2M and L2L as per-level matrix products on the
+// gfx950 matrix cores.
+//
+// Reference (sample/exafmm-dev-13274dd4ac68/examples/):
+//   M2M  LaplaceSphericalCPU_tapas.h:165-206  (dX = parent - child centre)
+//   L2L  LaplaceSphericalCPU_tapas.h:309-345  (dX = child - parent centre)
+//   called per cell by FMM_Upward / FMM_Downward, tapas_exafmm.cxx:87-140.
+// Both are real-linear maps of 55 complex (110 real) coefficients whose
+// entries depend only on dX, and within one level dX takes 8 values, one
+// per child octant (the cells of a level share their width).  So per level:
+//   M2M:  M_p (110)   += [T_0 .. T_7] (110 x 880) . [M_c0; ..; M_c7] (880)
+//   L2L:  L_c_o (110) += S_o (110 x 110) . L_p (110),   o = 0..7
+// The 8 children of a parent are adjacent cells, so the stacked children
+// coefficients of one parent are one contiguous column of 880 doubles.
+// The operators are built each evaluation by applying the reference's
+// formulas to unit coefficient vectors, with dX taken from the first parent
+// of the level (the centres of other parents differ from it by rounding
+// only), and the products run on v_mfma_f64_16x16x4 in FP64.
+#include "tfmm_device.h"
+#include "tfmm_kernels.h"
+
+namespace tfmm {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int NV = 2 * NTERM;  // 110 reals per cell
+constexpr int KM = 8 * NV;     // 880: stacked children
+
+__device__ inline int sgn(int n) { return (n & 1) ? -1 : 1; }
+__device__ inline int ipow2n(int n) { return (n >= 0) ? 1 : sgn(n); }
+
+__device__ inline void untri(int t, int &j, int &k) {
+  j = 0;
+  while (tri(j + 1, 0) <= t) j++;
+  k = t - tri(j, 0);
+}
+
+__device__ inline int degree_of_row(int r) {
+  int j, k;
+  untri(r >> 1, j, k);
+  return j;
+}
+
+// M2M output coefficient (j, k) of one child with coefficients Mc and
+// harmonics Y of dX (LaplaceSphericalCPU_tapas.h:178-200)
+__device__ inline cd m2m_entry(int j, int k, const cd *Mc, const cd *Y) {
+  cd acc = mk(0, 0);
+  for (int n = 0; n <= j; n++) {
+    const int mlo = max(-n, -j + k + n), mhi = min(k - 1, n);
+    for (int m = mlo; m <= mhi; m++) {
+      const cd t = cmul(Mc[tri(j - n, k - m)], yfull(Y, n, -m));
+      acc = cadd(acc, cscale(t, (double)(ipow2n(m) * sgn(n))));
+    }
+    const int mhi2 = min(n, j + k - n);
+    for (int m = k; m <= mhi2; m++) {
+      const cd t = cmul(cconj(Mc[tri(j - n, m - k)]), yfull(Y, n, -m));
+      acc = cadd(acc, cscale(t, (double)sgn(k + n + m)));
+    }
+  }
+  return acc;
+}
+
+// L2L output coefficient (j, k) from the parent's Lp and harmonics Y of dX
+// (LaplaceSphericalCPU_tapas.h:322-341)
+__device__ inline cd l2l_entry(int j, int k, const cd *Lp, const cd *Y) {
+  cd acc = mk(0, 0);
+  for (int n = j; n < P; n++) {
+    for (int m = j + k - n; m < 0; m++) {
+      const cd t = cmul(cconj(Lp[tri(n, -m)]), yfull(Y, n - j, m - k));
+      acc = cadd(acc, cscale(t, (double)sgn(k)));
+    }
+    for (int m = 0; m <= n; m++) {
+      if (n - j >= abs(m - k)) {
+        const cd t = cmul(Lp[tri(n, m)], yfull(Y, n - j, m - k));
+        acc = cadd(acc, cscale(t, (double)sgn((m - k) * (m < k))));
+      }
+    }
+  }
+  return acc;
+}
+
+// Operators of one level.  Block (column, octant, kind): the image of unit
+// input vector `column` under M2M (kind 0) or L2L (kind 1) of the octant.
+//   Tm: [110 rows][880 = octant * 110 + child column]
+//   Tl: [octant][110 rows][110 parent columns]
+__global__ __launch_bounds__(64) void k_build_ops(Cells c, int32_t first_child_cell, double *Tm,
+                                                  double *Tl) {
+  const int col = blockIdx.x, oct = blockIdx.y, kind = blockIdx.z;
+  const int lane = threadIdx.x;
+  __shared__ cd Y[NTERM];
+  __shared__ cd U[NTERM];
+  const int32_t p = c.parent[first_child_cell];
+  const int32_t ch = c.child[p] + oct;
+  if (lane == 0) {
+    const double sg = kind == 0 ? 1.0 : -1.0;  // M2M: parent - child; L2L: child - parent
+    const double dx = sg * (c.center[3 * (int64_t)p] - c.center[3 * (int64_t)ch]);
+    const double dy = sg * (c.center[3 * (int64_t)p + 1] - c.center[3 * (int64_t)ch + 1]);
+    const double dz = sg * (c.center[3 * (int64_t)p + 2] - c.center[3 * (int64_t)ch + 2]);
+    double rho, ca, sa;
+    cd ei;
+    sph_frame(dx, dy, dz, rho, ca, sa, ei);
+    multipole_all<false>(rho, ca, sa, ei, [&](int n, int m, cd v, cd) { Y[tri(n, m)] = v; });
+  }
+  if (lane < NTERM) U[lane] = mk(lane == (col >> 1) && !(col & 1) ? 1.0 : 0.0,
+                                 lane == (col >> 1) && (col & 1) ? 1.0 : 0.0);
+  __syncthreads();
+  if (lane >= NTERM) return;
+  int j, k;
+  untri(lane, j, k);
+  if (kind == 0) {
+    const cd e = m2m_entry(j, k, U, Y);
+    Tm[(int64_t)(2 * lane) * KM + oct * NV + col] = e.re;
+    Tm[(int64_t)(2 * lane + 1) * KM + oct * NV + col] = e.im;
+  } else {
+    const cd e = l2l_entry(j, k, U, Y);
+    Tl[((int64_t)oct * NV + 2 * lane) * NV + col] = e.re;
+    Tl[((int64_t)oct * NV + 2 * lane + 1) * NV + col] = e.im;
+  }
+}
+
+// MFMA f64 16x16x4 operands: lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; result i of lane l is D[l / 16 + 4 i][l % 16].
+
+// M2M of one level: block = 16 consecutive cells of the level (columns),
+// wave w = row block w of the parents' coefficients.  Parent degrees j only
+// take child degrees <= j, so row block w stops at the last needed column.
+__global__ __launch_bounds__(7 * 64) void k_m2m_mfma(Cells c, int32_t lbeg, int32_t lcnt,
+                                                     const double *Tm, double *M) {
+  const int lane = threadIdx.x & 63, rb = threadIdx.x >> 6;
+  const int col = lane & 15, kq = lane >> 4;
+  const int32_t p = lbeg + (int32_t)blockIdx.x * 16 + col;
+  const bool valid = p < lbeg + lcnt && c.child[p] >= 0;
+  const int32_t c0 = valid ? c.child[p] : 0;
+  const double *B = M + (int64_t)c0 * NV;
+  const int arow = rb * 16 + col;
+  const double *A = Tm + (int64_t)min(arow, NV - 1) * KM;
+  const int dmax = degree_of_row(min(rb * 16 + 15, NV - 1));
+  const int kend = min(NV, (dmax + 1) * (dmax + 2));  // child columns of degree <= dmax
+  d4 acc = {0, 0, 0, 0};
+  for (int oct = 0; oct < 8; oct++) {
+    for (int k0 = 0; k0 < kend; k0 += 4) {
+      const int kc = k0 + kq;
+      const bool in = kc < kend;
+      const double a = (in && arow < NV) ? A[oct * NV + kc] : 0.0;
+      const double b = in ? B[oct * NV + kc] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+  }
+  if (!valid) return;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int r = rb * 16 + kq + 4 * i;
+    if (r < NV) M[(int64_t)p * NV + r] += acc[i];
+  }
+}
+
+// L2L of one level: block = 16 consecutive cells of the parent level,
+// wave o = child octant o; every wave covers all 7 row blocks of the child
+// coefficients.  Child degrees j only take parent degrees >= j.
+__global__ __launch_bounds__(8 * 64) void k_l2l_mfma(Cells c, int32_t lbeg, int32_t lcnt,
+                                                     const double *Tl, double *L) {
+  const int lane = threadIdx.x & 63, oct = threadIdx.x >> 6;
+  const int col = lane & 15, kq = lane >> 4;
+  const int32_t p = lbeg + (int32_t)blockIdx.x * 16 + col;
+  const bool valid = p < lbeg + lcnt && c.child[p] >= 0;
+  const double *B = L + (int64_t)(valid ? p : 0) * NV;
+  const double *S = Tl + (int64_t)oct * NV * NV;
+  const int32_t cidx = valid ? c.child[p] + oct : 0;
+  for (int rb = 0; rb < 7; rb++) {
+    const int arow = rb * 16 + col;
+    const int dmin = degree_of_row(rb * 16);
+    const int kbeg = (dmin * (dmin + 1)) & ~3;  // parent columns of degree >= dmin
+    d4 acc = {0, 0, 0, 0};
+    for (int k0 = kbeg; k0 < NV; k0 += 4) {
+      const int kc = k0 + kq;
+      const bool in = kc < NV;
+      const double a = (in && arow < NV) ? S[(int64_t)arow * NV + kc] : 0.0;
+      const double b = in ? B[kc] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    if (valid) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int r = rb * 16 + kq + 4 * i;
+        if (r < NV) L[(int64_t)cidx * NV + r] += acc[i];
+      }
+    }
+  }
+}
+
+}  // namespace
+
+size_t translation_ops_doubles() { return (size_t)NV * KM + (size_t)8 * NV * NV; }
+
+// Operators of the level whose children start at cell first_child_cell;
+// ops: translation_ops_doubles() doubles (M2M operator, then L2L operators).
+hipError_t launch_translation_ops(Cells c, int32_t first_child_cell, double *ops,
+                                  hipStream_t s) {
+  k_build_ops<<<dim3(NV, 8, 2), 64, 0, s>>>(c, first_child_cell, ops, ops + (size_t)NV * KM);
+  return hipGetLastError();
+}
+
+hipError_t launch_m2m(Cells c, int32_t lbeg, int32_t lcnt, const double *ops, double *M,
+                      hipStream_t s) {
+  if (lcnt <= 0) return hipSuccess;
+  k_m2m_mfma<<<(lcnt + 15) / 16, 7 * 64, 0, s>>>(c, lbeg, lcnt, ops, M);
+  return hipGetLastError();
+}
+
+hipError_t launch_l2l(Cells c, int32_t lbeg, int32_t lcnt, const double *ops, double *L,
+                      hipStream_t s) {
+  if (lcnt <= 0) return hipSuccess;
+  k_l2l_mfma<<<(lcnt + 15) / 16, 8 * 64, 0, s>>>(c, lbeg, lcnt, ops + (size_t)NV * KM, L);
+  return hipGetLastError();
+}
+
+}  // namespace tfmm

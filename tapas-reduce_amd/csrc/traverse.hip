@@ -12,7 +12,10 @@
 //   Ri >= Rj                  -> (children of Ci) x Cj
 //   else                      -> Ci x (children of Cj)
 // Here that recursion is expanded breadth first: one launch per frontier,
-// each pair either emits an M2L entry, a P2P entry or its 8 successors.  The
+// each pair either emits an M2L entry, a P2P entry or its 8 successors.
+// Counters (cnt, uint32 words): 0 next frontier, 1 executed M2L pairs, 2 P2P
+// pairs, 3 all M2L pairs; words 8-11 hold two uint64 sums: body-pair
+// interactions of the P2P pairs and those of the self pairs.  The
 // MAC is evaluated in FP64 without contraction, in the reference's operation
 // order, so every decision is bit-identical to the reference's.
 #include "tfmm_device.h"
@@ -73,6 +76,34 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
   if (far && nonempty && slot < m2l_cap) m2l[slot] = make_int2(i, j);
   slot = block_append<DTT_WAVES>(&cnt[2], p2p_pair, 1, scratch);
   if (p2p_pair && slot < p2p_cap) p2p[slot] = make_int2(i, j);
+  // body-pair interactions of the P2P pairs (all, and those of self pairs)
+  {
+    unsigned long long v = 0, self = 0;
+    if (p2p_pair) {
+      v = (unsigned long long)c.nb[i] * (unsigned long long)c.nb[j];
+      if (i == j) self = (unsigned long long)c.nb[i];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      v += __shfl_xor(v, o, 64);
+      self += __shfl_xor(self, o, 64);
+    }
+    __shared__ unsigned long long sv[DTT_WAVES], ss[DTT_WAVES];
+    if ((threadIdx.x & 63) == 0) {
+      sv[threadIdx.x >> 6] = v;
+      ss[threadIdx.x >> 6] = self;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long tv = 0, ts = 0;
+      for (int w = 0; w < DTT_WAVES; w++) {
+        tv += sv[w];
+        ts += ss[w];
+      }
+      unsigned long long *pc = reinterpret_cast<unsigned long long *>(cnt + 8);
+      if (tv) atomicAdd(&pc[0], tv);
+      if (ts) atomicAdd(&pc[1], ts);
+    }
+  }
   const bool split = split_i || split_j;
   slot = block_append<DTT_WAVES>(&cnt[0], split, 8, scratch);
   if (split && slot + 8 <= out_cap) {
@@ -116,31 +147,6 @@ __global__ void k_csr_offsets(const uint64_t *keys, uint32_t n, int sbits, int32
   off[t] = (int32_t)lo;
 }
 
-__global__ __launch_bounds__(256) void k_p2p_pair_count(Cells c, const int2 *p2p, uint32_t n,
-                                                        unsigned long long *out) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long v = 0, self = 0;
-  if (t < n) {
-    const int2 p = p2p[t];
-    v = (unsigned long long)c.nb[p.x] * (unsigned long long)c.nb[p.y];
-    if (p.x == p.y) self = (unsigned long long)c.nb[p.x];
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    v += __shfl_xor(v, o, 64);
-    self += __shfl_xor(self, o, 64);
-  }
-  __shared__ unsigned long long sv[4], ss[4];
-  if (__lane_id() == 0) {
-    sv[threadIdx.x >> 6] = v;
-    ss[threadIdx.x >> 6] = self;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    atomicAdd(&out[0], sv[0] + sv[1] + sv[2] + sv[3]);
-    atomicAdd(&out[1], ss[0] + ss[1] + ss[2] + ss[3]);
-  }
-}
-
 }  // namespace
 
 hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin_max, int2 *out,
@@ -170,13 +176,6 @@ hipError_t launch_unpack_pairs(const uint64_t *keys, uint32_t n, int sbits, int3
 hipError_t launch_csr_offsets(const uint64_t *sorted_keys, uint32_t n, int sbits,
                               int32_t ncells, int32_t *off, hipStream_t s) {
   k_csr_offsets<<<(ncells + 1 + 255) / 256, 256, 0, s>>>(sorted_keys, n, sbits, ncells, off);
-  return hipGetLastError();
-}
-
-hipError_t launch_p2p_pair_count(Cells c, const int2 *p2p, uint32_t n,
-                                 unsigned long long *out, hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  k_p2p_pair_count<<<(n + 255) / 256, 256, 0, s>>>(c, p2p, n, out);
   return hipGetLastError();
 }
 
