@@ -276,8 +276,8 @@ int do_upward(tfmm_ctx *c) {
   for (int d = 0; d + 1 < nl; d++)
     TRY(launch_translation_ops(k, c->lvl[d + 1], dp<double>(c->tops) + (size_t)d * opsz, s));
   for (int d = nl - 2; d >= 0; d--)
-    TRY(launch_m2m(k, c->lvl[d], c->lvl[d + 1] - c->lvl[d], dp<double>(c->tops) + (size_t)d * opsz,
-                   dp<double>(c->M), s));
+    TRY(launch_m2m(k, c->lvl[d + 1], (c->lvl[d + 2] - c->lvl[d + 1]) / 8,
+                   dp<double>(c->tops) + (size_t)d * opsz, dp<double>(c->M), s));
   return TFMM_OK;
 }
 
@@ -511,8 +511,8 @@ int do_downward(tfmm_ctx *c) {
   const int nl = (int)c->lvl.size() - 1;
   const size_t opsz = translation_ops_doubles();
   for (int d = 0; d + 1 < nl; d++)
-    TRY(launch_l2l(k, c->lvl[d], c->lvl[d + 1] - c->lvl[d], dp<double>(c->tops) + (size_t)d * opsz,
-                   dp<double>(c->L), s));
+    TRY(launch_l2l(k, c->lvl[d + 1], (c->lvl[d + 2] - c->lvl[d + 1]) / 8,
+                   dp<double>(c->tops) + (size_t)d * opsz, dp<double>(c->L), s));
   TRY(dev_reserve(c->trg, (size_t)c->n * 32));
   TRY(dev_reserve(c->out, (size_t)c->n * 32));
   TRY(dev_reserve(c->leaf_of, (size_t)c->n * 4));

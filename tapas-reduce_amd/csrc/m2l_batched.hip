@@ -264,23 +264,23 @@ __global__ void k_items(const uint64_t *keys, const uint32_t *vals, const uint32
                         const uint32_t *iflag, const uint32_t *iexcl, uint32_t n, int gbits,
                         int4 *items, uint2 *cols) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !iflag[i]) return;
+  if (i >= n) return;
   const uint32_t seg = excl[i] + flag[i] - 1;
-  const uint32_t segend = (seg + 1 < nseg) ? segstart[seg + 1] : n;
-  const uint32_t cnt = min(16u, segend - i);
+  const uint32_t item = iexcl[i] + iflag[i] - 1;
+  const uint32_t jj = (i - segstart[seg]) & 15u;
   const uint64_t k = keys[i];
   const uint32_t g = (uint32_t)((k >> 7) & ((1ull << gbits) - 1ull));
-  int4 it;
-  it.x = (int)i;                                         // first pair
-  it.y = (int)cnt;                                       // pairs in the item
-  it.z = (int)g;                                         // group
-  it.w = (int)(k >> (gbits + 7));                        // tile
-  const uint32_t at = iexcl[i];
-  items[at] = it;
-  for (uint32_t jj = 0; jj < 16; jj++) {
-    uint2 cv = make_uint2(0u, g << 8);
-    if (jj < cnt) cv = make_uint2(vals[i + jj], (uint32_t)(keys[i + jj] & 127u) | 128u | (g << 8));
-    cols[(int64_t)at * 16 + jj] = cv;
+  cols[(int64_t)item * 16 + jj] = make_uint2(vals[i], (uint32_t)(k & 127u) | 128u | (g << 8));
+  if (iflag[i]) {  // first pair of the item: descriptor and padding columns
+    const uint32_t segend = (seg + 1 < nseg) ? segstart[seg + 1] : n;
+    const uint32_t cnt = min(16u, segend - i);
+    int4 it;
+    it.x = (int)i;                             // first pair
+    it.y = (int)cnt;                           // pairs in the item
+    it.z = (int)g;                             // group
+    it.w = (int)(k >> (gbits + 7));            // tile
+    items[item] = it;
+    for (uint32_t j2 = cnt; j2 < 16; j2++) cols[(int64_t)item * 16 + j2] = make_uint2(0u, g << 8);
   }
 }
 

@@ -52,9 +52,9 @@ hipError_t launch_p2m(Cells c, int32_t ncells, Bodies b, double *M, hipStream_t 
 size_t translation_ops_doubles();
 hipError_t launch_translation_ops(Cells c, int32_t first_child_cell, double *ops,
                                   hipStream_t s);
-hipError_t launch_m2m(Cells c, int32_t lbeg, int32_t lcnt, const double *ops, double *M,
+hipError_t launch_m2m(Cells c, int32_t first_child, int32_t npar, const double *ops, double *M,
                       hipStream_t s);
-hipError_t launch_l2l(Cells c, int32_t lbeg, int32_t lcnt, const double *ops, double *L,
+hipError_t launch_l2l(Cells c, int32_t first_child, int32_t npar, const double *ops, double *L,
                       hipStream_t s);
 hipError_t launch_l2p(Cells c, int32_t ncells, Bodies b, const double *L, const double *p2p,
                       const int32_t *perm, int32_t *leaf_of, double *trg, double *out,

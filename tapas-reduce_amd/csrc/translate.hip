@@ -36,12 +36,6 @@ __device__ inline void untri(int t, int &j, int &k) {
   k = t - tri(j, 0);
 }
 
-__device__ inline int degree_of_row(int r) {
-  int j, k;
-  untri(r >> 1, j, k);
-  return j;
-}
-
 // M2M output coefficient (j, k) of one child with coefficients Mc and
 // harmonics Y of dX (LaplaceSphericalCPU_tapas.h:178-200)
 __device__ inline cd m2m_entry(int j, int k, const cd *Mc, const cd *Y) {
@@ -121,33 +115,65 @@ __global__ __launch_bounds__(64) void k_build_ops(Cells c, int32_t first_child_c
 
 // MFMA f64 16x16x4 operands: lane l supplies A[l % 16][l / 16] and
 // B[l / 16][l % 16]; result i of lane l is D[l / 16 + 4 i][l % 16].
+//
+// Degree bounds of the 7 row blocks of 16 real coefficients: M2M row block
+// rb only takes child columns below m2m_kend(rb) (child degree <= parent
+// degree), L2L row block rb only parent columns from l2l_kbeg(rb) on
+// (parent degree >= child degree; rounded down to the k-step of 4).
+__host__ __device__ constexpr int deg_of_coef(int t) {
+  int n = 0;
+  while ((n + 1) * (n + 2) / 2 <= t) n++;
+  return n;
+}
+__host__ __device__ constexpr int m2m_kend(int rb) {
+  const int last = (16 * rb + 15 < NV - 1 ? 16 * rb + 15 : NV - 1) >> 1;
+  const int d = deg_of_coef(last);
+  return (d + 1) * (d + 2) < NV ? (d + 1) * (d + 2) : NV;
+}
+__host__ __device__ constexpr int l2l_kbeg(int rb) {
+  const int d = deg_of_coef((16 * rb) >> 1);
+  return (d * (d + 1)) & ~3;
+}
 
-// M2M of one level: block = 16 consecutive cells of the level (columns),
-// wave w = row block w of the parents' coefficients.  Parent degrees j only
-// take child degrees <= j, so row block w stops at the last needed column.
-__global__ __launch_bounds__(7 * 64) void k_m2m_mfma(Cells c, int32_t lbeg, int32_t lcnt,
+// M2M of one level: columns are the level's parents with children, in the
+// order of their children (the t-th such parent has children at
+// first_child + 8 t); block = 16 parents, wave rb = row block rb.
+__global__ __launch_bounds__(7 * 64) void k_m2m_mfma(Cells c, int32_t first_child, int32_t npar,
                                                      const double *Tm, double *M) {
   const int lane = threadIdx.x & 63, rb = threadIdx.x >> 6;
   const int col = lane & 15, kq = lane >> 4;
-  const int32_t p = lbeg + (int32_t)blockIdx.x * 16 + col;
-  const bool valid = p < lbeg + lcnt && c.child[p] >= 0;
-  const int32_t c0 = valid ? c.child[p] : 0;
+  const int32_t t = (int32_t)blockIdx.x * 16 + col;
+  const bool valid = t < npar;
+  const int32_t c0 = first_child + 8 * (valid ? t : 0);
   const double *B = M + (int64_t)c0 * NV;
   const int arow = rb * 16 + col;
   const double *A = Tm + (int64_t)min(arow, NV - 1) * KM;
-  const int dmax = degree_of_row(min(rb * 16 + 15, NV - 1));
-  const int kend = min(NV, (dmax + 1) * (dmax + 2));  // child columns of degree <= dmax
+  const bool arow_ok = arow < NV;
+  int kend = NV;
+  switch (rb) {  // constants per wave
+    case 0: kend = m2m_kend(0); break;
+    case 1: kend = m2m_kend(1); break;
+    case 2: kend = m2m_kend(2); break;
+    case 3: kend = m2m_kend(3); break;
+    case 4: kend = m2m_kend(4); break;
+    case 5: kend = m2m_kend(5); break;
+    default: kend = m2m_kend(6); break;
+  }
   d4 acc = {0, 0, 0, 0};
-  for (int oct = 0; oct < 8; oct++) {
-    for (int k0 = 0; k0 < kend; k0 += 4) {
-      const int kc = k0 + kq;
-      const bool in = kc < kend;
-      const double a = (in && arow < NV) ? A[oct * NV + kc] : 0.0;
-      const double b = in ? B[oct * NV + kc] : 0.0;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  for (int k0 = 0; k0 < kend; k0 += 4) {
+    const int kc = k0 + kq;
+    const bool in = kc < kend;
+    double a[8], b[8];
+#pragma unroll
+    for (int o = 0; o < 8; o++) {
+      a[o] = (in && arow_ok) ? A[o * NV + kc] : 0.0;
+      b[o] = in ? B[o * NV + kc] : 0.0;
     }
+#pragma unroll
+    for (int o = 0; o < 8; o++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[o], b[o], acc, 0, 0, 0);
   }
   if (!valid) return;
+  const int32_t p = c.parent[c0];
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int r = rb * 16 + kq + 4 * i;
@@ -155,36 +181,44 @@ __global__ __launch_bounds__(7 * 64) void k_m2m_mfma(Cells c, int32_t lbeg, int3
   }
 }
 
-// L2L of one level: block = 16 consecutive cells of the parent level,
-// wave o = child octant o; every wave covers all 7 row blocks of the child
-// coefficients.  Child degrees j only take parent degrees >= j.
-__global__ __launch_bounds__(8 * 64) void k_l2l_mfma(Cells c, int32_t lbeg, int32_t lcnt,
+// L2L of one level: the same parent columns; wave o = child octant o, all
+// 7 row blocks of the child's coefficients at once (one B fragment per
+// k-step feeds 7 independent accumulators).
+__global__ __launch_bounds__(8 * 64) void k_l2l_mfma(Cells c, int32_t first_child, int32_t npar,
                                                      const double *Tl, double *L) {
   const int lane = threadIdx.x & 63, oct = threadIdx.x >> 6;
   const int col = lane & 15, kq = lane >> 4;
-  const int32_t p = lbeg + (int32_t)blockIdx.x * 16 + col;
-  const bool valid = p < lbeg + lcnt && c.child[p] >= 0;
-  const double *B = L + (int64_t)(valid ? p : 0) * NV;
+  const int32_t t = (int32_t)blockIdx.x * 16 + col;
+  const bool valid = t < npar;
+  const int32_t c0 = first_child + 8 * (valid ? t : 0);
+  const int32_t p = c.parent[c0];
+  const double *B = L + (int64_t)p * NV;
   const double *S = Tl + (int64_t)oct * NV * NV;
-  const int32_t cidx = valid ? c.child[p] + oct : 0;
-  for (int rb = 0; rb < 7; rb++) {
-    const int arow = rb * 16 + col;
-    const int dmin = degree_of_row(rb * 16);
-    const int kbeg = (dmin * (dmin + 1)) & ~3;  // parent columns of degree >= dmin
-    d4 acc = {0, 0, 0, 0};
-    for (int k0 = kbeg; k0 < NV; k0 += 4) {
-      const int kc = k0 + kq;
-      const bool in = kc < NV;
-      const double a = (in && arow < NV) ? S[(int64_t)arow * NV + kc] : 0.0;
-      const double b = in ? B[kc] : 0.0;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-    }
-    if (valid) {
+  d4 acc[7];
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int r = rb * 16 + kq + 4 * i;
-        if (r < NV) L[(int64_t)cidx * NV + r] += acc[i];
-      }
+  for (int rb = 0; rb < 7; rb++) acc[rb] = (d4){0, 0, 0, 0};
+  for (int k0 = 0; k0 < NV; k0 += 4) {
+    const int kc = k0 + kq;
+    const bool in = kc < NV;
+    const double b = in ? B[kc] : 0.0;
+    double a[7];
+#pragma unroll
+    for (int rb = 0; rb < 7; rb++) {
+      const int arow = rb * 16 + col;
+      a[rb] = (in && arow < NV && k0 >= l2l_kbeg(rb)) ? S[(int64_t)arow * NV + kc] : 0.0;
+    }
+#pragma unroll
+    for (int rb = 0; rb < 7; rb++)
+      if (k0 >= l2l_kbeg(rb)) acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b, acc[rb], 0, 0, 0);
+  }
+  if (!valid) return;
+  const int64_t cidx = c0 + oct;
+#pragma unroll
+  for (int rb = 0; rb < 7; rb++) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int r = rb * 16 + kq + 4 * i;
+      if (r < NV) L[cidx * NV + r] += acc[rb][i];
     }
   }
 }
@@ -201,17 +235,19 @@ hipError_t launch_translation_ops(Cells c, int32_t first_child_cell, double *ops
   return hipGetLastError();
 }
 
-hipError_t launch_m2m(Cells c, int32_t lbeg, int32_t lcnt, const double *ops, double *M,
+// parents of one level: those whose children are the cells
+// [first_child, first_child + 8 npar) of the next level
+hipError_t launch_m2m(Cells c, int32_t first_child, int32_t npar, const double *ops, double *M,
                       hipStream_t s) {
-  if (lcnt <= 0) return hipSuccess;
-  k_m2m_mfma<<<(lcnt + 15) / 16, 7 * 64, 0, s>>>(c, lbeg, lcnt, ops, M);
+  if (npar <= 0) return hipSuccess;
+  k_m2m_mfma<<<(npar + 15) / 16, 7 * 64, 0, s>>>(c, first_child, npar, ops, M);
   return hipGetLastError();
 }
 
-hipError_t launch_l2l(Cells c, int32_t lbeg, int32_t lcnt, const double *ops, double *L,
+hipError_t launch_l2l(Cells c, int32_t first_child, int32_t npar, const double *ops, double *L,
                       hipStream_t s) {
-  if (lcnt <= 0) return hipSuccess;
-  k_l2l_mfma<<<(lcnt + 15) / 16, 8 * 64, 0, s>>>(c, lbeg, lcnt, ops + (size_t)NV * KM, L);
+  if (npar <= 0) return hipSuccess;
+  k_l2l_mfma<<<(npar + 15) / 16, 8 * 64, 0, s>>>(c, first_child, npar, ops + (size_t)NV * KM, L);
   return hipGetLastError();
 }
 

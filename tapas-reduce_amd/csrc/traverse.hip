@@ -68,51 +68,67 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
       split_j = true;
     }
   }
-  __shared__ uint32_t scratch[DTT_WAVES + 1];
-  uint32_t slot;
-  // every M2L pair is counted; only pairs of two non-empty cells are executed
-  block_append<DTT_WAVES>(&cnt[3], far, 1, scratch);
-  slot = block_append<DTT_WAVES>(&cnt[1], far && nonempty, 1, scratch);
-  if (far && nonempty && slot < m2l_cap) m2l[slot] = make_int2(i, j);
-  slot = block_append<DTT_WAVES>(&cnt[2], p2p_pair, 1, scratch);
-  if (p2p_pair && slot < p2p_cap) p2p[slot] = make_int2(i, j);
-  // body-pair interactions of the P2P pairs (all, and those of self pairs)
-  {
-    unsigned long long v = 0, self = 0;
-    if (p2p_pair) {
-      v = (unsigned long long)c.nb[i] * (unsigned long long)c.nb[j];
-      if (i == j) self = (unsigned long long)c.nb[i];
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      v += __shfl_xor(v, o, 64);
-      self += __shfl_xor(self, o, 64);
-    }
-    __shared__ unsigned long long sv[DTT_WAVES], ss[DTT_WAVES];
-    if ((threadIdx.x & 63) == 0) {
-      sv[threadIdx.x >> 6] = v;
-      ss[threadIdx.x >> 6] = self;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned long long tv = 0, ts = 0;
-      for (int w = 0; w < DTT_WAVES; w++) {
-        tv += sv[w];
-        ts += ss[w];
-      }
-      unsigned long long *pc = reinterpret_cast<unsigned long long *>(cnt + 8);
-      if (tv) atomicAdd(&pc[0], tv);
-      if (ts) atomicAdd(&pc[1], ts);
-    }
-  }
+  // Appends of the block to the four lists (0 next frontier, 8 entries per
+  // pair; 1 executed M2L; 2 P2P; 3 all M2L, counted only) and the body-pair
+  // sums: one global atomic per list per block, two block barriers.
   const bool split = split_i || split_j;
-  slot = block_append<DTT_WAVES>(&cnt[0], split, 8, scratch);
-  if (split && slot + 8 <= out_cap) {
+  const bool pred[4] = {split, far && nonempty, p2p_pair, far};
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  uint64_t m[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) m[q] = __ballot(pred[q]);
+  unsigned long long v = 0, self = 0;
+  if (p2p_pair) {
+    v = (unsigned long long)c.nb[i] * (unsigned long long)c.nb[j];
+    if (i == j) self = (unsigned long long)c.nb[i];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    v += __shfl_xor(v, o, 64);
+    self += __shfl_xor(self, o, 64);
+  }
+  __shared__ uint32_t wc[DTT_WAVES][4];
+  __shared__ unsigned long long ws[DTT_WAVES][2];
+  __shared__ uint32_t base[4];
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) wc[wid][q] = (uint32_t)__popcll(m[q]) * (q == 0 ? 8u : 1u);
+    ws[wid][0] = v;
+    ws[wid][1] = self;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int q = threadIdx.x;
+    uint32_t tot = 0;
+    for (int w = 0; w < DTT_WAVES; w++) tot += wc[w][q];
+    base[q] = tot ? atomicAdd(&cnt[q], tot) : 0u;
+  } else if (threadIdx.x == 64) {
+    unsigned long long tv = 0, ts = 0;
+    for (int w = 0; w < DTT_WAVES; w++) {
+      tv += ws[w][0];
+      ts += ws[w][1];
+    }
+    unsigned long long *pc = reinterpret_cast<unsigned long long *>(cnt + 8);
+    if (tv) atomicAdd(&pc[0], tv);
+    if (ts) atomicAdd(&pc[1], ts);
+  }
+  __syncthreads();
+  uint32_t slot[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    uint32_t pre = base[q];
+    for (int w = 0; w < wid; w++) pre += wc[w][q];
+    slot[q] = pre + (uint32_t)__popcll(m[q] & lt) * (q == 0 ? 8u : 1u);
+  }
+  if (far && nonempty && slot[1] < m2l_cap) m2l[slot[1]] = make_int2(i, j);
+  if (p2p_pair && slot[2] < p2p_cap) p2p[slot[2]] = make_int2(i, j);
+  if (split && slot[0] + 8 <= out_cap) {
     if (split_i) {
       const int32_t f = c.child[i];
-      for (int k = 0; k < 8; k++) out[slot + k] = make_int2(f + k, j);
+      for (int k = 0; k < 8; k++) out[slot[0] + k] = make_int2(f + k, j);
     } else {
       const int32_t f = c.child[j];
-      for (int k = 0; k < 8; k++) out[slot + k] = make_int2(i, f + k);
+      for (int k = 0; k < 8; k++) out[slot[0] + k] = make_int2(i, f + k);
     }
   }
 }
