@@ -431,7 +431,7 @@ int build_m2l_batches(tfmm_ctx *c) {
   TRY(launch_m2l_write_items(gk, gvs, ngp, gbits, dp<uint32_t>(c->it_flag),
                              dp<uint32_t>(c->it_excl), dp<uint32_t>(c->it_seg), nseg,
                              dp<uint32_t>(c->it_iflag), dp<uint32_t>(c->it_iexcl),
-                             dp<int4>(c->items), dp<uint2>(c->cols), s));
+                             dp<int8_t>(c->g_dl), dp<int4>(c->items), dp<uint2>(c->cols), s));
   c->n_items = nitems;
   TRY(dev_reserve(c->tile_off, (size_t)(ntiles + 1) * 4));
   TRY(launch_m2l_tile_offsets(dp<int4>(c->items), nitems, ntiles, dp<int32_t>(c->tile_off), s));
@@ -481,7 +481,7 @@ int do_traverse(tfmm_ctx *c) {
   const int32_t ntiles = (int32_t)((c->ncells + tile - 1) / tile);
   TRY(launch_m2l_tiles(dp<int32_t>(c->tile_off), ntiles, dp<uint2>(c->cols),
                        mixed ? (const void *)c->g_y32.ptr : (const void *)c->g_y64.ptr,
-                       dp<int8_t>(c->g_dl), c->mq.ptr, dp<uint32_t>(c->codes), k,
+                       c->mq.ptr, dp<uint32_t>(c->codes), k,
                        (int32_t)c->ncells, c->w0, dp<double>(c->L), mixed, s));
   TRY(hipEventRecord(c->ev[EV_M2LT], s));
   if (c->n_fallback) {

@@ -28,9 +28,10 @@
 // tile and accumulates its local expansions in LDS; each wave owns fixed row
 // blocks of T, so the waves never write the same LDS word, walk the tile's
 // items independently, and the accumulation order is fixed (deterministic).
-// The item loop is software pipelined two deep: while an item is multiplied,
-// the next item's operands and the column descriptors of the one after are
-// in flight.  Pairs whose offset falls outside the group table (level
+// The item loop is software pipelined (the next items' operands are in
+// flight while an item is multiplied), and tiles are dealt to the XCDs in
+// runs of consecutive tiles so that one XCD's L2 holds the sources its tiles
+// share.  Pairs whose offset falls outside the group table (level
 // difference >= 2) go to the per-pair kernel of m2l.hip.
 //
 // Per-group factors: the level factor 2^(n dl), n the source degree and
@@ -60,6 +61,7 @@ using m2lc::NR;
 constexpr int NCODE = NBLK * 4 * 64;
 constexpr int TILE = 128;
 constexpr int TILE_WAVES = 4;
+constexpr int WAVE_SYNC = 4;  // items between block barriers in the tile loop
 
 // ---------------------------------------------------------------------------
 // group table
@@ -257,12 +259,13 @@ __global__ void k_item_flags(const uint32_t *flag, const uint32_t *excl,
 
 // Item descriptors (first pair, pairs, group, tile) and the item's 16 column
 // descriptors: cols[16 item + j] = (source cell, target slot in tile
-// | 128 if column j holds a pair | group << 8).  Every column carries the
-// group, so that every lane of the tile kernel can fetch the group's data.
+// | 128 if column j holds a pair | (level difference + 1) << 8 | group << 10).
+// Every column carries the group, so that every lane of the tile kernel can
+// fetch the group's data.
 __global__ void k_items(const uint64_t *keys, const uint32_t *vals, const uint32_t *flag,
                         const uint32_t *excl, const uint32_t *segstart, uint32_t nseg,
                         const uint32_t *iflag, const uint32_t *iexcl, uint32_t n, int gbits,
-                        int4 *items, uint2 *cols) {
+                        const int8_t *gdl, int4 *items, uint2 *cols) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t seg = excl[i] + flag[i] - 1;
@@ -270,7 +273,8 @@ __global__ void k_items(const uint64_t *keys, const uint32_t *vals, const uint32
   const uint32_t jj = (i - segstart[seg]) & 15u;
   const uint64_t k = keys[i];
   const uint32_t g = (uint32_t)((k >> 7) & ((1ull << gbits) - 1ull));
-  cols[(int64_t)item * 16 + jj] = make_uint2(vals[i], (uint32_t)(k & 127u) | 128u | (g << 8));
+  const uint32_t gw = (uint32_t)(gdl[g] + 1) << 8 | (g << 10);
+  cols[(int64_t)item * 16 + jj] = make_uint2(vals[i], (uint32_t)(k & 127u) | 128u | gw);
   if (iflag[i]) {  // first pair of the item: descriptor and padding columns
     const uint32_t segend = (seg + 1 < nseg) ? segstart[seg + 1] : n;
     const uint32_t cnt = min(16u, segend - i);
@@ -280,7 +284,7 @@ __global__ void k_items(const uint64_t *keys, const uint32_t *vals, const uint32
     it.z = (int)g;                             // group
     it.w = (int)(k >> (gbits + 7));            // tile
     items[item] = it;
-    for (uint32_t j2 = cnt; j2 < 16; j2++) cols[(int64_t)item * 16 + j2] = make_uint2(0u, g << 8);
+    for (uint32_t j2 = cnt; j2 < 16; j2++) cols[(int64_t)item * 16 + j2] = make_uint2(0u, gw);
   }
 }
 
@@ -375,11 +379,123 @@ __device__ inline T a_value(const T *tbl, uint32_t cw) {
          *reinterpret_cast<const T *>(base + (cw >> 16));
 }
 
+// the operand table of one item: harmonics at multiples +1, -1, +2, -2
+template <class T>
+__device__ inline void write_table(T *tbl, T y0, T y1, int lane) {
+  tbl[lane] = y0;
+  tbl[NR + lane] = -y0;
+  tbl[2 * NR + lane] = y0 + y0;
+  tbl[3 * NR + lane] = -(y0 + y0);
+  if (lane < NR - 64) {
+    tbl[64 + lane] = y1;
+    tbl[NR + 64 + lane] = -y1;
+    tbl[2 * NR + 64 + lane] = y1 + y1;
+    tbl[3 * NR + 64 + lane] = -(y1 + y1);
+  }
+}
+
+template <class T, int W>
+__device__ inline void read_a(T (&a)[WAVE_NBLK[W] * 4], const T *tbl, const uint32_t *code) {
+#pragma unroll
+  for (int x = 0; x < WAVE_NBLK[W] * 4; x++) a[x] = a_value(tbl, code[x]);
+}
+
+// Pipeline state of one wave at item it: everything was loaded during item
+// it - 1, so nothing the item consumes is still in flight when it starts;
+// acc holds the products of item it - 1, stored during item it.
+template <class T, int W>
+struct WaveState {
+  uint2 c_cur;  // columns of it (sources of its B operands were c_b of it - 1)
+  uint2 c_b;    // columns of it + 1 (sources for the next B)
+  uint2 c_y;    // columns of it + 2 (group for the next harmonics)
+  uint2 c_st;   // columns of it - 1 (target slots and level difference of acc)
+  T b[7][4];    // B operands of it
+  T y0, y1;     // harmonics of it + 1
+  T a[WAVE_NBLK[W] * 4];  // A operands of it
+  typename Mfma<T>::V acc[WAVE_NRB[W]];  // products of it - 1
+};
+
+// add the products of one item (columns c) into the tile's expansions
+template <class T, int W>
+__device__ inline void store_item(const typename Mfma<T>::V (&acc)[WAVE_NRB[W]], uint2 c,
+                                  const int (&jd)[WAVE_NRB[W]][4], T (*Lt)[NR + 1],
+                                  T (*Lt0b)[17], int q) {
+  if (!(c.y & 128u)) return;
+  const uint32_t ts = c.y & 127u;
+  const int dl = (int)((c.y >> 8) & 3u) - 1;
+#pragma unroll
+  for (int f = 0; f < WAVE_NRB[W]; f++) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int r = Mfma<T>::row(q, i);
+      const T v = ldexp(acc[f][i], -jd[f][i] * dl);
+      if (W == 1 && WAVE_RBS[W][f] == 0) {
+        Lt0b[ts][r] += v;
+      } else {
+        const int row = WAVE_RBS[W][f] * 16 + r;
+        if (row < 110) Lt[ts][row] += v;
+      }
+    }
+  }
+}
+
+// One item of wave W with the state of it in cur; fills nxt for it + 1.
+// Order: the item's MFMAs are issued first; the loads, operand table and A
+// reads for the next item and the store of the previous item's products
+// are independent of them and overlap their execution.  Loads issued here
+// are consumed by the next call.  The state is double buffered by
+// unrolling the caller's loop by two (a register rotation would wait for
+// the loads just issued).
+template <class T, int W>
+__device__ inline void wave_item(WaveState<T, W> &cur, WaveState<T, W> &nxt, int32_t it,
+                                 int32_t ib, int32_t last, const uint2 *cols, const T *Yg,
+                                 const T *Mq, const uint32_t *code,
+                                 const int (&jd)[WAVE_NRB[W]][4], T (*Lt)[NR + 1],
+                                 T (*Lt0b)[17], T *tbl, int lane) {
+  constexpr int NB = WAVE_NBLK[W], NF = WAVE_NRB[W];
+  const int q = lane >> 4, j = lane & 15;
+  const bool hi = lane < NR - 64;
+  // the matrix work of it
+#pragma unroll
+  for (int f = 0; f < NF; f++) nxt.acc[f] = (typename Mfma<T>::V){0, 0, 0, 0};
+#pragma unroll
+  for (int kk = 0; kk < 4; kk++) {
+#pragma unroll
+    for (int e = 0; e < NB; e++)
+      nxt.acc[WAVE_F[W][e]] = Mfma<T>::step(cur.a[e * 4 + kk], cur.b[BLK_CB[WAVE_BLK[W][e]]][kk],
+                                            nxt.acc[WAVE_F[W][e]]);
+  }
+  nxt.c_st = cur.c_cur;
+  // loads for the next items (indices clamped: loads stay in range)
+  nxt.c_cur = cur.c_b;
+  nxt.c_b = cols[(int64_t)min(it + 2, last) * 16 + j];
+  nxt.c_y = cols[(int64_t)min(it + 3, last) * 16 + j];
+  load_b<T, W>(nxt.b, Mq + (int64_t)cur.c_b.x * NR + q * 28);
+  nxt.y0 = Yg[(int64_t)(cur.c_y.y >> 10) * NR + lane];
+  nxt.y1 = hi ? Yg[(int64_t)(cur.c_y.y >> 10) * NR + 64 + lane] : (T)0;
+  // operand table and A operands of it + 1 (shared by consecutive items of
+  // one group)
+  if ((cur.c_b.y >> 10) != (cur.c_cur.y >> 10)) {
+    __builtin_amdgcn_wave_barrier();
+    write_table<T>(tbl, cur.y0, cur.y1, lane);
+    __builtin_amdgcn_wave_barrier();
+    read_a<T, W>(nxt.a, tbl, code);
+  } else {
+#pragma unroll
+    for (int x = 0; x < NB * 4; x++) nxt.a[x] = cur.a[x];
+  }
+  // the products of it - 1
+  store_item<T, W>(cur.acc, cur.c_st, jd, Lt, Lt0b, q);
+  // keep the 4 waves within a few items of each other: they read the same
+  // source multipoles, which then come from L2 once instead of 4 times
+  if (((it - ib) & (WAVE_SYNC - 1)) == WAVE_SYNC - 1) __syncthreads();
+}
+
+// The item loop of wave W over the tile's items [ib, ie).
 template <class T, int W>
 __device__ inline void run_wave(const uint2 *cols, int32_t ib, int32_t ie, const T *Yg,
-                                const int8_t *gdl, const T *Mq, const uint32_t *codes,
+                                const T *Mq, const uint32_t *codes,
                                 T (*Lt)[NR + 1], T (*Lt0b)[17], T *tbl, int lane) {
-  typedef typename Mfma<T>::V V;
   constexpr int NB = WAVE_NBLK[W], NF = WAVE_NRB[W];
   if (ib >= ie) return;
   uint32_t code[NB * 4];
@@ -394,98 +510,70 @@ __device__ inline void run_wave(const uint2 *cols, int32_t ib, int32_t ie, const
 #pragma unroll
     for (int i = 0; i < 4; i++) jd[f][i] = row_degree(WAVE_RBS[W][f] * 16 + Mfma<T>::row(q, i));
   const bool hi = lane < NR - 64;
+  const int32_t last = ie - 1;
 
-  // pipeline: columns of items it and it + 1, operands of item it
-  uint2 cc = cols[(int64_t)ib * 16 + j];
-  uint2 cn = cols[(int64_t)min(ib + 1, ie - 1) * 16 + j];  // clamped: loads stay in range
-  T bc[7][4];
-  load_b<T, W>(bc, Mq + (int64_t)cc.x * NR + q * 28);
-  T y0 = Yg[(int64_t)(cc.y >> 8) * NR + lane];
-  T y1 = hi ? Yg[(int64_t)(cc.y >> 8) * NR + 64 + lane] : (T)0;
-  int dl = gdl[cc.y >> 8];
-  for (int32_t it = ib; it < ie; it++) {
-    // in flight during this item: columns of it + 2, operands of it + 1
-    const uint2 cnn = cols[(int64_t)min(it + 2, ie - 1) * 16 + j];
-    T bn[7][4];
-    load_b<T, W>(bn, Mq + (int64_t)cn.x * NR + q * 28);
-    const T yn0 = Yg[(int64_t)(cn.y >> 8) * NR + lane];
-    const T yn1 = hi ? Yg[(int64_t)(cn.y >> 8) * NR + 64 + lane] : (T)0;
-    const int dln = gdl[cn.y >> 8];
-
-    // the item's operand table: harmonics at multiples +1, -1, +2, -2
-    tbl[lane] = y0;
-    tbl[NR + lane] = -y0;
-    tbl[2 * NR + lane] = y0 + y0;
-    tbl[3 * NR + lane] = -(y0 + y0);
-    if (hi) {
-      tbl[64 + lane] = y1;
-      tbl[NR + 64 + lane] = -y1;
-      tbl[2 * NR + 64 + lane] = y1 + y1;
-      tbl[3 * NR + 64 + lane] = -(y1 + y1);
-    }
-    __builtin_amdgcn_wave_barrier();
-    V acc[NF];
+  WaveState<T, W> s0, s1;
+  s0.c_cur = cols[(int64_t)ib * 16 + j];
+  s0.c_b = cols[(int64_t)min(ib + 1, last) * 16 + j];
+  s0.c_y = cols[(int64_t)min(ib + 2, last) * 16 + j];
+  s0.c_st = make_uint2(0u, 0u);  // no previous item
 #pragma unroll
-    for (int f = 0; f < NF; f++) acc[f] = (V){0, 0, 0, 0};
-#pragma unroll
-    for (int kk = 0; kk < 4; kk++) {
-#pragma unroll
-      for (int e = 0; e < NB; e++) {
-        const T a = a_value(tbl, code[e * 4 + kk]);
-        acc[WAVE_F[W][e]] = Mfma<T>::step(a, bc[BLK_CB[WAVE_BLK[W][e]]][kk], acc[WAVE_F[W][e]]);
-      }
-    }
-    __builtin_amdgcn_wave_barrier();
-    if (cc.y & 128u) {
-      const uint32_t ts = cc.y & 127u;
-#pragma unroll
-      for (int f = 0; f < NF; f++) {
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int r = Mfma<T>::row(q, i);
-          const T v = ldexp(acc[f][i], -jd[f][i] * dl);
-          if (W == 1 && WAVE_RBS[W][f] == 0) {
-            Lt0b[ts][r] += v;
-          } else {
-            const int row = WAVE_RBS[W][f] * 16 + r;
-            if (row < 110) Lt[ts][row] += v;
-          }
-        }
-      }
-    }
-    // advance the pipeline
-    cc = cn;
-    cn = cnn;
-#pragma unroll
-    for (int cb = 0; cb < 7; cb++)
-#pragma unroll
-      for (int kk = 0; kk < 4; kk++)
-        if (wave_needs_cb(W, cb)) bc[cb][kk] = bn[cb][kk];
-    y0 = yn0;
-    y1 = yn1;
-    dl = dln;
+  for (int f = 0; f < NF; f++) s0.acc[f] = (typename Mfma<T>::V){0, 0, 0, 0};
+  load_b<T, W>(s0.b, Mq + (int64_t)s0.c_cur.x * NR + q * 28);
+  {
+    const uint32_t g = s0.c_cur.y >> 10;
+    const T y0 = Yg[(int64_t)g * NR + lane];
+    const T y1 = hi ? Yg[(int64_t)g * NR + 64 + lane] : (T)0;
+    write_table<T>(tbl, y0, y1, lane);
   }
+  s0.y0 = Yg[(int64_t)(s0.c_b.y >> 10) * NR + lane];
+  s0.y1 = hi ? Yg[(int64_t)(s0.c_b.y >> 10) * NR + 64 + lane] : (T)0;
+  __builtin_amdgcn_wave_barrier();
+  read_a<T, W>(s0.a, tbl, code);
+  for (int32_t it = ib; it < ie; it += 2) {
+    wave_item<T, W>(s0, s1, it, ib, last, cols, Yg, Mq, code, jd, Lt, Lt0b, tbl, lane);
+    if (it + 1 < ie)
+      wave_item<T, W>(s1, s0, it + 1, ib, last, cols, Yg, Mq, code, jd, Lt, Lt0b, tbl, lane);
+  }
+  // the products of the last item
+  if ((ie - ib) & 1) store_item<T, W>(s1.acc, s1.c_st, jd, Lt, Lt0b, q);
+  else store_item<T, W>(s0.acc, s0.c_st, jd, Lt, Lt0b, q);
+}
+
+// Workgroups are dispatched to the 8 XCDs round robin (workgroup b to XCD
+// b % 8) and every XCD has its own L2.  Consecutive tiles are neighbouring
+// target ranges of one level and share most of their source cells, so each
+// run of 256 tiles is dealt out as 8 blocks of 32 consecutive tiles, one per
+// XCD: the tiles an XCD runs at the same time read the same sources from
+// its L2.  The last partial run keeps the identity order (a bijection).
+constexpr int NXCD = 8;
+constexpr int XCD_RUN = 32;
+__device__ inline int tile_of_block(int b, int ntiles) {
+  constexpr int S = NXCD * XCD_RUN;
+  if (b >= (ntiles / S) * S) return b;
+  const int local = b % S;
+  return (b - local) + (local % NXCD) * XCD_RUN + local / NXCD;
 }
 
 template <class T>
 __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
-    const int32_t *tile_off, const uint2 *cols, const T *Yg, const int8_t *gdl, const T *Mq,
+    const int32_t *tile_off, const uint2 *cols, const T *Yg, const T *Mq,
     const uint32_t *codes, Cells c, int32_t ncells, double w0, double *L) {
   __shared__ T Lt[TILE][NR + 1];
   __shared__ T Lt0b[TILE][17];
   __shared__ T Tb[TILE_WAVES][NMULT * NR];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tile = blockIdx.x;
+  const int tile = tile_of_block(blockIdx.x, gridDim.x);
   for (int i = threadIdx.x; i < TILE * (NR + 1); i += 64 * TILE_WAVES) (&Lt[0][0])[i] = 0;
   for (int i = threadIdx.x; i < TILE * 17; i += 64 * TILE_WAVES) (&Lt0b[0][0])[i] = 0;
   __syncthreads();
   const int32_t ib = tile_off[tile], ie = tile_off[tile + 1];
   // every wave walks all items of the tile, in order, for its own blocks
   switch (wave) {
-    case 0: run_wave<T, 0>(cols, ib, ie, Yg, gdl, Mq, codes, Lt, Lt0b, Tb[0], lane); break;
-    case 1: run_wave<T, 1>(cols, ib, ie, Yg, gdl, Mq, codes, Lt, Lt0b, Tb[1], lane); break;
-    case 2: run_wave<T, 2>(cols, ib, ie, Yg, gdl, Mq, codes, Lt, Lt0b, Tb[2], lane); break;
-    default: run_wave<T, 3>(cols, ib, ie, Yg, gdl, Mq, codes, Lt, Lt0b, Tb[3], lane); break;
+    case 0: run_wave<T, 0>(cols, ib, ie, Yg, Mq, codes, Lt, Lt0b, Tb[0], lane); break;
+    case 1: run_wave<T, 1>(cols, ib, ie, Yg, Mq, codes, Lt, Lt0b, Tb[1], lane); break;
+    case 2: run_wave<T, 2>(cols, ib, ie, Yg, Mq, codes, Lt, Lt0b, Tb[2], lane); break;
+    default: run_wave<T, 3>(cols, ib, ie, Yg, Mq, codes, Lt, Lt0b, Tb[3], lane); break;
   }
   __syncthreads();
   for (int e = threadIdx.x; e < TILE * 16; e += 64 * TILE_WAVES) {
@@ -602,11 +690,11 @@ hipError_t launch_m2l_count_items(const uint64_t *keys, uint32_t n, uint32_t *fl
 hipError_t launch_m2l_write_items(const uint64_t *keys, const uint32_t *vals, uint32_t n,
                                   int gbits, const uint32_t *flag, const uint32_t *excl,
                                   const uint32_t *segstart, uint32_t nseg, const uint32_t *iflag,
-                                  const uint32_t *iexcl, int4 *items, uint2 *cols,
-                                  hipStream_t s) {
+                                  const uint32_t *iexcl, const int8_t *gdl, int4 *items,
+                                  uint2 *cols, hipStream_t s) {
   if (n == 0) return hipSuccess;
   k_items<<<(n + 255) / 256, 256, 0, s>>>(keys, vals, flag, excl, segstart, nseg, iflag, iexcl,
-                                          n, gbits, items, cols);
+                                          n, gbits, gdl, items, cols);
   return hipGetLastError();
 }
 
@@ -617,17 +705,17 @@ hipError_t launch_m2l_tile_offsets(const int4 *items, uint32_t nitems, int32_t n
 }
 
 hipError_t launch_m2l_tiles(const int32_t *tile_off, int32_t ntiles, const uint2 *cols,
-                            const void *Yg, const int8_t *gdl, const void *Mq,
+                            const void *Yg, const void *Mq,
                             const uint32_t *codes, Cells c, int32_t ncells, double w0, double *L,
                             bool fp32, hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
   if (fp32)
-    k_m2l_tiles<float><<<ntiles, 64 * TILE_WAVES, 0, s>>>(tile_off, cols, (const float *)Yg, gdl,
+    k_m2l_tiles<float><<<ntiles, 64 * TILE_WAVES, 0, s>>>(tile_off, cols, (const float *)Yg,
                                                           (const float *)Mq, codes, c, ncells,
                                                           w0, L);
   else
     k_m2l_tiles<double><<<ntiles, 64 * TILE_WAVES, 0, s>>>(tile_off, cols, (const double *)Yg,
-                                                           gdl, (const double *)Mq,
+                                                           (const double *)Mq,
                                                            codes + NCODE, c, ncells, w0, L);
   return hipGetLastError();
 }

@@ -21,6 +21,7 @@
 // two table elements whose byte offsets are packed into one 32-bit code
 // (low 16 bits, high 16 bits).  Absent terms point at element 110, a zero.
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 
 #include <vector>

@@ -115,12 +115,12 @@ hipError_t launch_m2l_count_items(const uint64_t *keys, uint32_t n, uint32_t *fl
 hipError_t launch_m2l_write_items(const uint64_t *keys, const uint32_t *vals, uint32_t n,
                                   int gbits, const uint32_t *flag, const uint32_t *excl,
                                   const uint32_t *segstart, uint32_t nseg, const uint32_t *iflag,
-                                  const uint32_t *iexcl, int4 *items, uint2 *cols,
-                                  hipStream_t s);
+                                  const uint32_t *iexcl, const int8_t *gdl, int4 *items,
+                                  uint2 *cols, hipStream_t s);
 hipError_t launch_m2l_tile_offsets(const int4 *items, uint32_t nitems, int32_t ntiles,
                                    int32_t *off, hipStream_t s);
 hipError_t launch_m2l_tiles(const int32_t *tile_off, int32_t ntiles, const uint2 *cols,
-                            const void *Yg, const int8_t *gdl, const void *Mq,
+                            const void *Yg, const void *Mq,
                             const uint32_t *codes, Cells c, int32_t ncells, double w0, double *L,
                             bool fp32, hipStream_t s);
 
