@@ -382,9 +382,9 @@ int build_m2l_batches(tfmm_ctx *c) {
                                  dp<int8_t>(c->g_lt), s));
   // split grouped / fallback pairs
   const size_t nn = (size_t)std::max<uint32_t>(n, 1);
-  TRY(dev_reserve(c->gkeys, nn * 8));
+  TRY(dev_reserve(c->gkeys, nn * 4));
   TRY(dev_reserve(c->gvals, nn * 4));
-  TRY(dev_reserve(c->gkeys_alt, nn * 8));
+  TRY(dev_reserve(c->gkeys_alt, nn * 4));
   TRY(dev_reserve(c->gvals_alt, nn * 4));
   TRY(dev_reserve(c->fkeys, nn * 8));
   TRY(dev_reserve(c->fvals, nn * 4));
@@ -394,8 +394,13 @@ int build_m2l_batches(tfmm_ctx *c) {
   TRY(hipMemsetAsync(c->g_cnt.ptr, 0, 16, s));
   const int gbits = bits_for(ngr);
   const int sbits = bits_for(c->ncells);
+  {
+    const int64_t nt = (c->ncells + m2l_tile_size() - 1) / m2l_tile_size();
+    if (bits_for(nt) + gbits > 32 || c->ncells >= ((int64_t)1 << m2l_src_bits()))
+      return fail(c, TFMM_EHIP, "tree too large for the batched M2L keys");
+  }
   TRY(launch_m2l_classify(k, dp<int2>(c->m2l_pairs), n, dp<uint32_t>(c->g_gid), gbits, sbits,
-                          dp<uint64_t>(c->gkeys), dp<uint32_t>(c->gvals), dp<uint64_t>(c->fkeys),
+                          dp<uint32_t>(c->gkeys), dp<uint32_t>(c->gvals), dp<uint64_t>(c->fkeys),
                           dp<uint32_t>(c->fvals), dp<uint32_t>(c->g_cnt), s));
   TRY(hipMemcpyAsync(c->h_u32, c->g_cnt.ptr, 8, hipMemcpyDeviceToHost, s));
   TRY(hipStreamSynchronize(s));
@@ -407,11 +412,11 @@ int build_m2l_batches(tfmm_ctx *c) {
   const int tbits = bits_for(ntiles);
   TRY(dev_reserve(c->sort_tmp, radix_tmp_bytes(std::max<uint32_t>(ngp, nfb))));
   bool alt = false;
-  TRY(radix_sort_pairs(dp<uint64_t>(c->gkeys), dp<uint32_t>(c->gvals), dp<uint64_t>(c->gkeys_alt),
-                       dp<uint32_t>(c->gvals_alt), ngp, 0, tbits + gbits + 7, c->sort_tmp.ptr, s,
-                       &alt));
+  TRY(radix_sort_pairs32(dp<uint32_t>(c->gkeys), dp<uint32_t>(c->gvals), dp<uint32_t>(c->gkeys_alt),
+                         dp<uint32_t>(c->gvals_alt), ngp, 0, tbits + gbits, c->sort_tmp.ptr, s,
+                         &alt));
   c->grouped_keys_in_alt = alt;
-  const uint64_t *gk = dp<uint64_t>(alt ? c->gkeys_alt : c->gkeys);
+  const uint32_t *gk = dp<uint32_t>(alt ? c->gkeys_alt : c->gkeys);
   // work items: runs of <= 16 pairs of one group within one tile
   const size_t ng1 = (size_t)std::max<uint32_t>(ngp, 1);
   TRY(dev_reserve(c->it_flag, ng1 * 4));

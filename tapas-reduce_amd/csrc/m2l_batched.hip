@@ -59,9 +59,10 @@ using m2lc::NBLK;
 using m2lc::NMULT;
 using m2lc::NR;
 constexpr int NCODE = NBLK * 4 * 64;
+constexpr int SRC_BITS = 25;  // source cell bits of a grouped pair's value
 constexpr int TILE = 128;
 constexpr int TILE_WAVES = 4;
-constexpr int WAVE_SYNC = 4;  // items between block barriers in the tile loop
+constexpr int WAVE_SYNC = 16;  // items between block barriers in the tile loop
 
 // ---------------------------------------------------------------------------
 // group table
@@ -142,11 +143,14 @@ __global__ void k_mark_slots(Cells c, const int2 *pairs, uint32_t n, uint32_t *u
   if (s >= 0) used[s] = 1u;
 }
 
-// Split the executed M2L pairs into grouped ones (sort key: tile | group |
-// slot-in-tile, value: source cell) and the rest (key: target | source).
+// Split the executed M2L pairs into grouped ones (sort key: tile << gbits |
+// group, value: source cell | slot in tile << SRC_BITS) and the rest (key:
+// target | source).  Within one (tile, group) segment every target occurs at
+// most once, so the order inside a segment does not affect any sum and the
+// key needs no slot bits.
 __global__ __launch_bounds__(1024) void k_classify(Cells c, const int2 *pairs, uint32_t n,
                                                   const uint32_t *gid_of_slot, int gbits,
-                                                  int sbits, uint64_t *gkeys, uint32_t *gvals,
+                                                  int sbits, uint32_t *gkeys, uint32_t *gvals,
                                                   uint64_t *fkeys, uint32_t *fvals,
                                                   uint32_t *counters) {
   __shared__ uint32_t scratch[17];
@@ -163,8 +167,8 @@ __global__ __launch_bounds__(1024) void k_classify(Cells c, const int2 *pairs, u
   const uint32_t at = block_append<16>(&counters[0], grouped, 1, scratch);
   if (grouped) {
     const uint64_t tile = (uint32_t)p.x / TILE, in_tile = (uint32_t)p.x % TILE;
-    gkeys[at] = (tile << (gbits + 7)) | ((uint64_t)gid_of_slot[slot] << 7) | in_tile;
-    gvals[at] = (uint32_t)p.y;
+    gkeys[at] = (uint32_t)(tile << gbits) | gid_of_slot[slot];
+    gvals[at] = (uint32_t)p.y | (uint32_t)(in_tile << SRC_BITS);
   }
   const uint32_t af = block_append<16>(&counters[1], fallback, 1, scratch);
   if (fallback) {
@@ -236,10 +240,10 @@ __global__ void k_scale_m(Cells c, int32_t ncells, const double *M, double w0, T
 // ---------------------------------------------------------------------------
 // work items
 // ---------------------------------------------------------------------------
-__global__ void k_seg_flags(const uint64_t *keys, uint32_t n, uint32_t *flag) {
+__global__ void k_seg_flags(const uint32_t *keys, uint32_t n, uint32_t *flag) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  flag[i] = (i == 0 || (keys[i] >> 7) != (keys[i - 1] >> 7)) ? 1u : 0u;
+  flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
 }
 
 __global__ void k_seg_starts(const uint32_t *flag, const uint32_t *excl, uint32_t n,
@@ -262,7 +266,7 @@ __global__ void k_item_flags(const uint32_t *flag, const uint32_t *excl,
 // | 128 if column j holds a pair | (level difference + 1) << 8 | group << 10).
 // Every column carries the group, so that every lane of the tile kernel can
 // fetch the group's data.
-__global__ void k_items(const uint64_t *keys, const uint32_t *vals, const uint32_t *flag,
+__global__ void k_items(const uint32_t *keys, const uint32_t *vals, const uint32_t *flag,
                         const uint32_t *excl, const uint32_t *segstart, uint32_t nseg,
                         const uint32_t *iflag, const uint32_t *iexcl, uint32_t n, int gbits,
                         const int8_t *gdl, int4 *items, uint2 *cols) {
@@ -271,10 +275,11 @@ __global__ void k_items(const uint64_t *keys, const uint32_t *vals, const uint32
   const uint32_t seg = excl[i] + flag[i] - 1;
   const uint32_t item = iexcl[i] + iflag[i] - 1;
   const uint32_t jj = (i - segstart[seg]) & 15u;
-  const uint64_t k = keys[i];
-  const uint32_t g = (uint32_t)((k >> 7) & ((1ull << gbits) - 1ull));
+  const uint32_t k = keys[i], v = vals[i];
+  const uint32_t g = k & ((1u << gbits) - 1u);
   const uint32_t gw = (uint32_t)(gdl[g] + 1) << 8 | (g << 10);
-  cols[(int64_t)item * 16 + jj] = make_uint2(vals[i], (uint32_t)(k & 127u) | 128u | gw);
+  cols[(int64_t)item * 16 + jj] =
+      make_uint2(v & ((1u << SRC_BITS) - 1u), (v >> SRC_BITS) | 128u | gw);
   if (iflag[i]) {  // first pair of the item: descriptor and padding columns
     const uint32_t segend = (seg + 1 < nseg) ? segstart[seg + 1] : n;
     const uint32_t cnt = min(16u, segend - i);
@@ -282,7 +287,7 @@ __global__ void k_items(const uint64_t *keys, const uint32_t *vals, const uint32
     it.x = (int)i;                             // first pair
     it.y = (int)cnt;                           // pairs in the item
     it.z = (int)g;                             // group
-    it.w = (int)(k >> (gbits + 7));            // tile
+    it.w = (int)(k >> gbits);                  // tile
     items[item] = it;
     for (uint32_t j2 = cnt; j2 < 16; j2++) cols[(int64_t)item * 16 + j2] = make_uint2(0u, gw);
   }
@@ -629,7 +634,7 @@ hipError_t launch_m2l_mark_slots(Cells c, const int2 *pairs, uint32_t n, uint32_
 
 hipError_t launch_m2l_classify(Cells c, const int2 *pairs, uint32_t n,
                                const uint32_t *gid_of_slot, int gbits, int sbits,
-                               uint64_t *gkeys, uint32_t *gvals, uint64_t *fkeys,
+                               uint32_t *gkeys, uint32_t *gvals, uint64_t *fkeys,
                                uint32_t *fvals, uint32_t *counters, hipStream_t s) {
   if (n == 0) return hipSuccess;
   k_classify<<<(n + 1023) / 1024, 1024, 0, s>>>(c, pairs, n, gid_of_slot, gbits, sbits, gkeys,
@@ -654,7 +659,7 @@ hipError_t launch_m2l_scale_m(Cells c, int32_t ncells, const double *M, double w
   return hipGetLastError();
 }
 
-hipError_t launch_m2l_count_items(const uint64_t *keys, uint32_t n, uint32_t *flag,
+hipError_t launch_m2l_count_items(const uint32_t *keys, uint32_t n, uint32_t *flag,
                                   uint32_t *excl, uint32_t *segstart, uint32_t *iflag,
                                   uint32_t *iexcl, void *scan_tmp, uint32_t *h_pinned,
                                   uint32_t *nseg_out, uint32_t *nitems_out, hipStream_t s) {
@@ -687,7 +692,7 @@ hipError_t launch_m2l_count_items(const uint64_t *keys, uint32_t n, uint32_t *fl
   return hipGetLastError();
 }
 
-hipError_t launch_m2l_write_items(const uint64_t *keys, const uint32_t *vals, uint32_t n,
+hipError_t launch_m2l_write_items(const uint32_t *keys, const uint32_t *vals, uint32_t n,
                                   int gbits, const uint32_t *flag, const uint32_t *excl,
                                   const uint32_t *segstart, uint32_t nseg, const uint32_t *iflag,
                                   const uint32_t *iexcl, const int8_t *gdl, int4 *items,
@@ -721,5 +726,6 @@ hipError_t launch_m2l_tiles(const int32_t *tile_off, int32_t ntiles, const uint2
 }
 
 int m2l_tile_size() { return TILE; }
+int m2l_src_bits() { return SRC_BITS; }
 
 }  // namespace tfmm

@@ -18,7 +18,8 @@ constexpr int RS_THREADS = 256;
 constexpr int RS_WAVES = RS_THREADS / 64;
 constexpr int RS_ROWS = 16;  // keys per lane
 constexpr int RS_TILE = RS_THREADS * RS_ROWS;
-constexpr int RS_BINS = 256;
+constexpr int RS_MAXBITS = 9;
+constexpr int RS_BINS = 1 << RS_MAXBITS;
 
 __device__ inline uint32_t wave_incl_scan(uint32_t v, int lane) {
 #pragma unroll
@@ -88,14 +89,20 @@ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // ---------------------------------------------------------------------------
 // radix sort
 // ---------------------------------------------------------------------------
+// LSD passes of up to RS_MAXBITS bits; a pass = per-tile digit histogram,
+// device-wide exclusive scan of the (digit, tile) counts, and a stable
+// scatter that ranks each tile in LDS (wave ballots), stages it in digit
+// order and writes every digit run to consecutive positions.
 
-__device__ inline uint32_t digit_of(uint64_t k, int shift, uint32_t mask) {
+template <class K>
+__device__ inline uint32_t digit_of(K k, int shift, uint32_t mask) {
   return (uint32_t)(k >> shift) & mask;
 }
 
-__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint64_t *keys, int64_t n,
-                                                           int shift, uint32_t mask,
-                                                           uint32_t *counts, int nblocks) {
+template <class K>
+__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const K *keys, int64_t n, int shift,
+                                                           uint32_t mask, uint32_t *counts,
+                                                           int nblocks) {
   __shared__ uint32_t hist[RS_BINS];
   for (int i = threadIdx.x; i < RS_BINS; i += RS_THREADS) hist[i] = 0;
   __syncthreads();
@@ -109,12 +116,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint64_t *keys,
     counts[(int64_t)d * nblocks + blockIdx.x] = hist[d];
 }
 
+template <class K>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
-    const uint64_t *keys_in, const uint32_t *vals_in, uint64_t *keys_out, uint32_t *vals_out,
-    int64_t n, int shift, int nbits, const uint32_t *offsets, int nblocks) {
+    const K *keys_in, const uint32_t *vals_in, K *keys_out, uint32_t *vals_out, int64_t n,
+    int shift, int nbits, const uint32_t *offsets, int nblocks) {
   __shared__ uint32_t whist[RS_WAVES][RS_BINS];
   __shared__ uint32_t dstart[RS_BINS];
-  __shared__ uint64_t sk[RS_TILE];
+  __shared__ K sk[RS_TILE];
   __shared__ uint32_t sv[RS_TILE];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const uint32_t mask = (1u << nbits) - 1u;
@@ -123,14 +131,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
 
   const int64_t wbase = (int64_t)blockIdx.x * RS_TILE + (int64_t)wid * (64 * RS_ROWS);
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  uint64_t k[RS_ROWS];
+  K k[RS_ROWS];
   uint32_t v[RS_ROWS];
   uint32_t rank[RS_ROWS];
 #pragma unroll
   for (int r = 0; r < RS_ROWS; r++) {
     const int64_t j = wbase + r * 64 + lane;
     const bool valid = j < n;
-    k[r] = valid ? keys_in[j] : 0ull;
+    k[r] = valid ? keys_in[j] : (K)0;
     v[r] = valid ? vals_in[j] : 0u;
     const uint32_t d = digit_of(k[r], shift, mask);
     uint64_t m = __ballot(valid);
@@ -148,20 +156,29 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
   }
   __syncthreads();
   // per digit: offsets of the waves (exclusive, in wave order) and the tile's
-  // count; then the tile's digit starts (exclusive scan over digits)
-  static_assert(RS_BINS == RS_THREADS, "one digit per thread");
-  uint32_t cnt = 0;
-  {
-    const int d = threadIdx.x;
+  // count; then the tile's digit starts (exclusive scan over digits).  Each
+  // thread owns RS_BINS / RS_THREADS consecutive digits.
+  constexpr int DPT = RS_BINS / RS_THREADS;
+  uint32_t cnt[DPT];
+  uint32_t mine = 0;
+#pragma unroll
+  for (int u = 0; u < DPT; u++) {
+    const int d = threadIdx.x * DPT + u;
+    cnt[u] = 0;
     for (int w = 0; w < RS_WAVES; w++) {
       const uint32_t t = whist[w][d];
-      whist[w][d] = cnt;
-      cnt += t;
+      whist[w][d] = cnt[u];
+      cnt[u] += t;
     }
+    mine += cnt[u];
   }
   uint32_t unused;
-  const uint32_t start = block_excl_scan(cnt, &unused);
-  dstart[threadIdx.x] = start;
+  uint32_t start = block_excl_scan(mine, &unused);
+#pragma unroll
+  for (int u = 0; u < DPT; u++) {
+    dstart[threadIdx.x * DPT + u] = start;
+    start += cnt[u];
+  }
   __syncthreads();
   // stage the tile in LDS in digit order (stable), then write each digit's
   // run to consecutive global positions
@@ -179,9 +196,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
   const int64_t tbase = (int64_t)blockIdx.x * RS_TILE;
   const int tn = (int)((n - tbase) < RS_TILE ? (n - tbase) : RS_TILE);
   for (int i = threadIdx.x; i < tn; i += RS_THREADS) {
-    const uint64_t key = sk[i];
+    const K key = sk[i];
     const uint32_t d = digit_of(key, shift, mask);
-    const uint64_t pos = (uint64_t)offsets[(int64_t)d * nblocks + blockIdx.x] + (uint32_t)i - dstart[d];
+    const uint64_t pos =
+        (uint64_t)offsets[(int64_t)d * nblocks + blockIdx.x] + (uint32_t)i - dstart[d];
     keys_out[pos] = key;
     vals_out[pos] = sv[i];
   }
@@ -245,9 +263,12 @@ size_t radix_tmp_bytes(int64_t n) {
   return 2 * (size_t)ncount * sizeof(uint32_t) + scan_tmp_bytes(ncount) + 512;
 }
 
-hipError_t radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
-                            uint32_t *vals_alt, int64_t n, int bit_lo, int bit_hi, void *tmp,
-                            hipStream_t s, bool *result_in_alt) {
+// Sorts the bits [bit_lo, bit_hi) of the keys with as few passes of at most
+// RS_MAXBITS bits as possible, digit widths balanced.
+template <class K>
+static hipError_t radix_sort_impl(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt,
+                                  int64_t n, int bit_lo, int bit_hi, void *tmp, hipStream_t s,
+                                  bool *result_in_alt) {
   *result_in_alt = false;
   if (n <= 1 || bit_hi <= bit_lo) return hipSuccess;
   const int nb = (int)ceil_div(n, RS_TILE);
@@ -256,23 +277,41 @@ hipError_t radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
   uint32_t *offs = counts + ncount;
   char *stmp = reinterpret_cast<char *>(offs + ncount);
   stmp = reinterpret_cast<char *>(((uintptr_t)stmp + 255) & ~(uintptr_t)255);
-  uint64_t *kin = keys, *kout = keys_alt;
+  K *kin = keys, *kout = keys_alt;
   uint32_t *vin = vals, *vout = vals_alt;
-  for (int lo = bit_lo; lo < bit_hi; lo += 8) {
-    const int nbits = (bit_hi - lo) < 8 ? (bit_hi - lo) : 8;
+  const int bits = bit_hi - bit_lo;
+  const int passes = (bits + RS_MAXBITS - 1) / RS_MAXBITS;
+  int lo = bit_lo;
+  for (int ps = 0; ps < passes; ps++) {
+    const int nbits = (bit_hi - lo + (passes - ps) - 1) / (passes - ps);  // balanced widths
     const uint32_t mask = (1u << nbits) - 1u;
     const int64_t nc = (int64_t)nb * (mask + 1);
-    k_radix_hist<<<nb, RS_THREADS, 0, s>>>(kin, n, lo, mask, counts, nb);
+    k_radix_hist<K><<<nb, RS_THREADS, 0, s>>>(kin, n, lo, mask, counts, nb);
     hipError_t e = scan_exclusive_u32(counts, offs, nc, stmp, s);
     if (e != hipSuccess) return e;
-    k_radix_scatter<<<nb, RS_THREADS, 0, s>>>(kin, vin, kout, vout, n, lo, nbits, offs, nb);
+    k_radix_scatter<K><<<nb, RS_THREADS, 0, s>>>(kin, vin, kout, vout, n, lo, nbits, offs, nb);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    uint64_t *tk = kin; kin = kout; kout = tk;
+    K *tk = kin; kin = kout; kout = tk;
     uint32_t *tv = vin; vin = vout; vout = tv;
     *result_in_alt = !*result_in_alt;
+    lo += nbits;
   }
   return hipSuccess;
+}
+
+hipError_t radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
+                            uint32_t *vals_alt, int64_t n, int bit_lo, int bit_hi, void *tmp,
+                            hipStream_t s, bool *result_in_alt) {
+  return radix_sort_impl<uint64_t>(keys, vals, keys_alt, vals_alt, n, bit_lo, bit_hi, tmp, s,
+                                   result_in_alt);
+}
+
+hipError_t radix_sort_pairs32(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt,
+                              uint32_t *vals_alt, int64_t n, int bit_lo, int bit_hi, void *tmp,
+                              hipStream_t s, bool *result_in_alt) {
+  return radix_sort_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, bit_lo, bit_hi, tmp, s,
+                                   result_in_alt);
 }
 
 }  // namespace tfmm

@@ -35,6 +35,9 @@ size_t radix_tmp_bytes(int64_t n);
 hipError_t radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
                             uint32_t *vals_alt, int64_t n, int bit_lo, int bit_hi, void *tmp,
                             hipStream_t s, bool *result_in_alt);
+hipError_t radix_sort_pairs32(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt,
+                              uint32_t *vals_alt, int64_t n, int bit_lo, int bit_hi, void *tmp,
+                              hipStream_t s, bool *result_in_alt);
 
 }  // namespace tfmm
 

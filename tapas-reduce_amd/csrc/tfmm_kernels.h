@@ -95,24 +95,25 @@ namespace tfmm {
 // m2l_batched.hip
 int32_t m2l_group_slots(int max_depth);
 int m2l_tile_size();
+int m2l_src_bits();
 hipError_t m2l_upload_codes(uint32_t *dev_codes, hipStream_t s);
 size_t m2l_codes_bytes();
 hipError_t launch_m2l_mark_slots(Cells c, const int2 *pairs, uint32_t n, uint32_t *used,
                                  hipStream_t s);
 hipError_t launch_m2l_classify(Cells c, const int2 *pairs, uint32_t n,
                                const uint32_t *gid_of_slot, int gbits, int sbits,
-                               uint64_t *gkeys, uint32_t *gvals, uint64_t *fkeys,
+                               uint32_t *gkeys, uint32_t *gvals, uint64_t *fkeys,
                                uint32_t *fvals, uint32_t *counters, hipStream_t s);
 hipError_t launch_m2l_group_harmonics(const uint32_t *used, const uint32_t *gid_of_slot,
                                       int32_t nslots, double w0, double *Y64, float *Y32,
                                       int8_t *gdl, int8_t *glt, hipStream_t s);
 hipError_t launch_m2l_scale_m(Cells c, int32_t ncells, const double *M, double w0, void *Mq,
                               bool fp32, hipStream_t s);
-hipError_t launch_m2l_count_items(const uint64_t *keys, uint32_t n, uint32_t *flag,
+hipError_t launch_m2l_count_items(const uint32_t *keys, uint32_t n, uint32_t *flag,
                                   uint32_t *excl, uint32_t *segstart, uint32_t *iflag,
                                   uint32_t *iexcl, void *scan_tmp, uint32_t *h_pinned,
                                   uint32_t *nseg_out, uint32_t *nitems_out, hipStream_t s);
-hipError_t launch_m2l_write_items(const uint64_t *keys, const uint32_t *vals, uint32_t n,
+hipError_t launch_m2l_write_items(const uint32_t *keys, const uint32_t *vals, uint32_t n,
                                   int gbits, const uint32_t *flag, const uint32_t *excl,
                                   const uint32_t *segstart, uint32_t nseg, const uint32_t *iflag,
                                   const uint32_t *iexcl, const int8_t *gdl, int4 *items,
