@@ -62,6 +62,11 @@ struct tfmm_ctx {
   int64_t ncells = 0, nleaves = 0;
   bool keys_in_alt = false;
   double h_region[6] = {0, 0, 0, 0, 0, 0};
+  // translation operators: valid for this root region and number of levels
+  void *tops_ptr = nullptr;
+  size_t tops_cap = 0;
+  int tops_levels = 0;
+  double tops_region[6] = {0, 0, 0, 0, 0, 0};
   double w0 = 0;  // root width
 
   // expansions
@@ -79,7 +84,7 @@ struct tfmm_ctx {
   DevBuf g_used, g_gid, g_y64, g_y32, g_dl, g_lt, g_cnt;
   DevBuf gkeys, gvals, gkeys_alt, gvals_alt, fkeys, fvals, fkeys_alt, fvals_alt;
   DevBuf it_flag, it_excl, it_seg, it_iflag, it_iexcl, items, cols, tile_off, mq, codes;
-  DevBuf fb_off, fb_src, fb_part, relpos, leaf_of, tops;
+  DevBuf fb_off, fb_src, fb_part, relpos, leaf_of, tops, slot_of;
   bool codes_ready = false;
   uint32_t n_groups = 0, n_grouped = 0, n_fallback = 0, n_items = 0;
   bool grouped_keys_in_alt = false;
@@ -271,10 +276,22 @@ int do_upward(tfmm_ctx *c) {
   TRY(launch_p2m(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->M), s));
   const int nl = (int)c->lvl.size() - 1;
   // translation operators of every level with children (used again by L2L)
+  // depend only on the root region and the number of levels: rebuilt only
+  // when either changed since the last evaluation
   const size_t opsz = translation_ops_doubles();
   TRY(dev_reserve(c->tops, (size_t)std::max(nl - 1, 1) * opsz * 8));
-  for (int d = 0; d + 1 < nl; d++)
-    TRY(launch_translation_ops(k, c->lvl[d + 1], dp<double>(c->tops) + (size_t)d * opsz, s));
+  TRY(hipStreamSynchronize(s));  // h_region of this tree
+  const bool same = c->tops_ptr == c->tops.ptr && c->tops_cap == c->tops.cap &&
+                    c->tops_levels >= nl &&
+                    memcmp(c->tops_region, c->h_region, sizeof(c->tops_region)) == 0;
+  if (!same) {
+    for (int d = 0; d + 1 < nl; d++)
+      TRY(launch_translation_ops(k, c->lvl[d + 1], dp<double>(c->tops) + (size_t)d * opsz, s));
+    c->tops_ptr = c->tops.ptr;
+    c->tops_cap = c->tops.cap;
+    c->tops_levels = nl;
+    memcpy(c->tops_region, c->h_region, sizeof(c->tops_region));
+  }
   for (int d = nl - 2; d >= 0; d--)
     TRY(launch_m2m(k, c->lvl[d + 1], (c->lvl[d + 2] - c->lvl[d + 1]) / 8,
                    dp<double>(c->tops) + (size_t)d * opsz, dp<double>(c->M), s));
@@ -358,7 +375,9 @@ int build_m2l_batches(tfmm_ctx *c) {
   TRY(dev_reserve(c->g_gid, (size_t)nslots * 4));
   TRY(dev_reserve(c->scan_tmp, scan_tmp_bytes(std::max<int64_t>(nslots, n))));
   TRY(hipMemsetAsync(c->g_used.ptr, 0, (size_t)nslots * 4, s));
-  TRY(launch_m2l_mark_slots(k, dp<int2>(c->m2l_pairs), n, dp<uint32_t>(c->g_used), s));
+  TRY(dev_reserve(c->slot_of, (size_t)std::max<uint32_t>(n, 1) * 4));
+  TRY(launch_m2l_mark_slots(k, dp<int2>(c->m2l_pairs), n, dp<uint32_t>(c->g_used),
+                            dp<int32_t>(c->slot_of), s));
   TRY(scan_exclusive_u32(dp<uint32_t>(c->g_used), dp<uint32_t>(c->g_gid), nslots,
                          c->scan_tmp.ptr, s));
   TRY(hipMemcpyAsync(c->h_u32, dp<uint32_t>(c->g_gid) + nslots - 1, 4, hipMemcpyDeviceToHost, s));
@@ -399,7 +418,7 @@ int build_m2l_batches(tfmm_ctx *c) {
     if (bits_for(nt) + gbits > 32 || c->ncells >= ((int64_t)1 << m2l_src_bits()))
       return fail(c, TFMM_EHIP, "tree too large for the batched M2L keys");
   }
-  TRY(launch_m2l_classify(k, dp<int2>(c->m2l_pairs), n, dp<uint32_t>(c->g_gid), gbits, sbits,
+  TRY(launch_m2l_classify(dp<int2>(c->m2l_pairs), dp<int32_t>(c->slot_of), n, dp<uint32_t>(c->g_gid), gbits, sbits,
                           dp<uint32_t>(c->gkeys), dp<uint32_t>(c->gvals), dp<uint64_t>(c->fkeys),
                           dp<uint32_t>(c->fvals), dp<uint32_t>(c->g_cnt), s));
   TRY(hipMemcpyAsync(c->h_u32, c->g_cnt.ptr, 8, hipMemcpyDeviceToHost, s));
@@ -580,7 +599,7 @@ void tfmm_destroy(tfmm_ctx *c) {
                     &c->g_lt, &c->g_cnt, &c->gkeys, &c->gvals, &c->gkeys_alt, &c->gvals_alt,
                     &c->fkeys, &c->fvals, &c->fkeys_alt, &c->fvals_alt, &c->it_flag,
                     &c->it_excl, &c->it_seg, &c->it_iflag, &c->it_iexcl, &c->items,
-                    &c->tile_off, &c->cols, &c->mq, &c->codes, &c->fb_off, &c->fb_src, &c->fb_part, &c->relpos, &c->leaf_of, &c->tops,
+                    &c->tile_off, &c->cols, &c->mq, &c->codes, &c->fb_off, &c->fb_src, &c->fb_part, &c->relpos, &c->leaf_of, &c->tops, &c->slot_of,
                     &c->feat_acc, &c->feat_x, &c->feat_out};
   for (DevBuf *b : bufs) dev_release(*b);
   for (int i = 0; i < EV_COUNT; i++)

@@ -135,11 +135,14 @@ __device__ inline int32_t pair_slot(const Cells &c, int32_t t, int32_t s) {
   return group_slot(lt, ls, o);
 }
 
-__global__ void k_mark_slots(Cells c, const int2 *pairs, uint32_t n, uint32_t *used) {
+// group-table slot of every pair (-1: outside the table), and the used slots
+__global__ void k_mark_slots(Cells c, const int2 *pairs, uint32_t n, uint32_t *used,
+                             int32_t *slot_of) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int2 p = pairs[i];
   const int32_t s = pair_slot(c, p.x, p.y);
+  slot_of[i] = s;
   if (s >= 0) used[s] = 1u;
 }
 
@@ -148,8 +151,8 @@ __global__ void k_mark_slots(Cells c, const int2 *pairs, uint32_t n, uint32_t *u
 // target | source).  Within one (tile, group) segment every target occurs at
 // most once, so the order inside a segment does not affect any sum and the
 // key needs no slot bits.
-__global__ __launch_bounds__(1024) void k_classify(Cells c, const int2 *pairs, uint32_t n,
-                                                  const uint32_t *gid_of_slot, int gbits,
+__global__ __launch_bounds__(1024) void k_classify(const int2 *pairs, const int32_t *slot_of,
+                                                  uint32_t n, const uint32_t *gid_of_slot, int gbits,
                                                   int sbits, uint32_t *gkeys, uint32_t *gvals,
                                                   uint64_t *fkeys, uint32_t *fvals,
                                                   uint32_t *counters) {
@@ -160,7 +163,7 @@ __global__ __launch_bounds__(1024) void k_classify(Cells c, const int2 *pairs, u
   int32_t slot = -1;
   if (valid) {
     p = pairs[i];
-    slot = pair_slot(c, p.x, p.y);
+    slot = slot_of[i];
   }
   const bool grouped = valid && slot >= 0;
   const bool fallback = valid && slot < 0;
@@ -626,19 +629,19 @@ size_t m2l_codes_bytes() { return (size_t)2 * NCODE * 4; }
 int32_t m2l_group_slots(int max_depth) { return (max_depth + 1) * SLOTS_PER_LEVEL; }
 
 hipError_t launch_m2l_mark_slots(Cells c, const int2 *pairs, uint32_t n, uint32_t *used,
-                                 hipStream_t s) {
+                                 int32_t *slot_of, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  k_mark_slots<<<(n + 255) / 256, 256, 0, s>>>(c, pairs, n, used);
+  k_mark_slots<<<(n + 255) / 256, 256, 0, s>>>(c, pairs, n, used, slot_of);
   return hipGetLastError();
 }
 
-hipError_t launch_m2l_classify(Cells c, const int2 *pairs, uint32_t n,
+hipError_t launch_m2l_classify(const int2 *pairs, const int32_t *slot_of, uint32_t n,
                                const uint32_t *gid_of_slot, int gbits, int sbits,
                                uint32_t *gkeys, uint32_t *gvals, uint64_t *fkeys,
                                uint32_t *fvals, uint32_t *counters, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  k_classify<<<(n + 1023) / 1024, 1024, 0, s>>>(c, pairs, n, gid_of_slot, gbits, sbits, gkeys,
-                                                  gvals, fkeys, fvals, counters);
+  k_classify<<<(n + 1023) / 1024, 1024, 0, s>>>(pairs, slot_of, n, gid_of_slot, gbits, sbits,
+                                                  gkeys, gvals, fkeys, fvals, counters);
   return hipGetLastError();
 }
 

@@ -9,10 +9,10 @@
 // TP/map.h:159-206; here one wave owns a target leaf for its whole source
 // list and every target body is written once).
 //
-// Wave mapping: a target leaf of nt <= 64 bodies is spread over g lanes per
-// body, g the largest power of two with g * nt <= 64; the g lanes of one
-// body split the source bodies and are summed by a butterfly at the end, so
-// leaves of 8..32 bodies still keep all 64 lanes busy.
+// Wave mapping: a target leaf of nt <= 64 bodies is spread over g = 64 / nt
+// lanes per body (lane = body * g + sub); the g lanes of one body split the
+// source bodies and their partial sums are added in lane order through LDS
+// at the end, so small leaves still keep (almost) all 64 lanes busy.
 #include "tfmm_device.h"
 #include "tfmm_kernels.h"
 
@@ -30,14 +30,14 @@ __global__ __launch_bounds__(64) void k_p2p_fp64(Cells c, Csr list, Bodies b, do
   const int lane = threadIdx.x;
   __shared__ double sx[P2P_CHUNK], sy[P2P_CHUNK], sz[P2P_CHUNK], sq[P2P_CHUNK];
   const int32_t lb = list.off[ti], le = list.off[ti + 1];
+  __shared__ double red[4][64];
   // lanes per target body
-  int lg = 0;
-  while (lg < 6 && (nt << (lg + 1)) <= 64) lg++;
-  const int g = 1 << lg;
-  for (int32_t t0 = 0; t0 < nt; t0 += (64 >> lg)) {
-    const int32_t tl = t0 + (lane >> lg);
-    const int sub = lane & (g - 1);
-    const bool tvalid = tl < nt;
+  const int g = nt >= 64 ? 1 : 64 / nt;
+  const int tpp = 64 / g;  // target bodies per pass
+  for (int32_t t0 = 0; t0 < nt; t0 += tpp) {
+    const int32_t tl = t0 + lane / g;
+    const int sub = lane % g;
+    const bool tvalid = lane < g * tpp && tl < nt;
     double xi = 0, yi = 0, zi = 0, qi = 0;
     if (tvalid) {
       xi = b.x[tb + tl];
@@ -76,13 +76,19 @@ __global__ __launch_bounds__(64) void k_p2p_fp64(Cells c, Csr list, Bodies b, do
         }
       }
     }
-    for (int o = 1; o < g; o <<= 1) {
-      a0 += __shfl_xor(a0, o, 64);
-      a1 += __shfl_xor(a1, o, 64);
-      a2 += __shfl_xor(a2, o, 64);
-      a3 += __shfl_xor(a3, o, 64);
-    }
+    __syncthreads();
+    red[0][lane] = a0;
+    red[1][lane] = a1;
+    red[2][lane] = a2;
+    red[3][lane] = a3;
+    __syncthreads();
     if (tvalid && sub == 0) {
+      for (int u = 1; u < g; u++) {
+        a0 += red[0][lane + u];
+        a1 += red[1][lane + u];
+        a2 += red[2][lane + u];
+        a3 += red[3][lane + u];
+      }
       double *o = acc + 4 * (int64_t)(tb + tl);
       o[0] = a0;
       o[1] = a1;
@@ -134,6 +140,13 @@ __device__ inline void kahan_add(float &s, float &c, float v) {
 // dX = x_i - x_j keeps the FP32 precision of the leaf size, not of the
 // coordinates.  Each source leaf's contribution is summed in FP32 and then
 // added to the running per-target sums with Kahan compensation.
+//
+// Latency: the metadata of 64 source leaves at a time (first body, count,
+// centre offset) is loaded by the 64 lanes together and read back with
+// readlane; the bodies of the next source chunk (<= 64 bodies) are loaded
+// into registers while the current chunk is computed from LDS.  The block is
+// one wave, whose LDS operations complete in issue order, so one staging
+// buffer suffices.
 __global__ __launch_bounds__(64) void k_p2p_fp32(Cells c, Csr list, Bodies b,
                                                 const float4 *rel, double *acc) {
   const int32_t ti = blockIdx.x;
@@ -143,36 +156,64 @@ __global__ __launch_bounds__(64) void k_p2p_fp32(Cells c, Csr list, Bodies b,
   const int32_t tb = c.beg[ti];
   const int lane = threadIdx.x;
   __shared__ float4 sb[P2P_CHUNK];
+  __shared__ double red[4][64];
   const int32_t lb = list.off[ti], le = list.off[ti + 1];
   const double tcx = c.center[3 * (int64_t)ti], tcy = c.center[3 * (int64_t)ti + 1],
                tcz = c.center[3 * (int64_t)ti + 2];
-  int lg = 0;
-  while (lg < 6 && (nt << (lg + 1)) <= 64) lg++;
-  const int g = 1 << lg;
-  for (int32_t t0 = 0; t0 < nt; t0 += (64 >> lg)) {
-    const int32_t tl = t0 + (lane >> lg);
-    const int sub = lane & (g - 1);
-    const bool tvalid = tl < nt;
+  const int g = nt >= 64 ? 1 : 64 / nt;  // lanes per target body
+  const int tpp = 64 / g;                // target bodies per pass
+  for (int32_t t0 = 0; t0 < nt; t0 += tpp) {
+    const int32_t tl = t0 + lane / g;
+    const int sub = lane % g;
+    const bool tvalid = lane < g * tpp && tl < nt;
     float4 xi = make_float4(0, 0, 0, 0);
     if (tvalid) xi = rel[tb + tl];
     float s0 = 0, s1 = 0, s2 = 0, s3 = 0, c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-    for (int32_t s = lb; s < le; s++) {
-      const int32_t sj = list.src[s];
-      const int32_t sbeg = c.beg[sj], ns = c.nb[sj];
-      const float ox = (float)(tcx - c.center[3 * (int64_t)sj]);
-      const float oy = (float)(tcy - c.center[3 * (int64_t)sj + 1]);
-      const float oz = (float)(tcz - c.center[3 * (int64_t)sj + 2]);
-      for (int32_t c0i = 0; c0i < ns; c0i += P2P_CHUNK) {
-        const int32_t cn = min(P2P_CHUNK, ns - c0i);
-        __syncthreads();
-        if (lane < cn) {
-          float4 v = rel[sbeg + c0i + lane];
-          v.x -= ox;
-          v.y -= oy;
-          v.z -= oz;
-          sb[lane] = v;
+    for (int32_t base = lb; base < le; base += 64) {
+      // metadata of source leaves base .. base + 63, one per lane
+      const int nsl = min(64, le - base);
+      int32_t mbeg = 0, mnb = 0;
+      float mox = 0, moy = 0, moz = 0;
+      if (lane < nsl) {
+        const int32_t sj = list.src[base + lane];
+        mbeg = c.beg[sj];
+        mnb = c.nb[sj];
+        mox = (float)(tcx - c.center[3 * (int64_t)sj]);
+        moy = (float)(tcy - c.center[3 * (int64_t)sj + 1]);
+        moz = (float)(tcz - c.center[3 * (int64_t)sj + 2]);
+      }
+      // chunk (k, off): bodies off .. off + 63 of source leaf k
+      int k = 0, off = 0;
+      int32_t kb = __builtin_amdgcn_readlane(mbeg, 0), kn = __builtin_amdgcn_readlane(mnb, 0);
+      float kox = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mox), 0));
+      float koy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(moy), 0));
+      float koz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(moz), 0));
+      float4 v = make_float4(0, 0, 0, 0);
+      if (lane < kn) v = rel[kb + lane];
+      while (true) {
+        const int cn = min(P2P_CHUNK, kn - off);
+        // stage the current chunk (shifted into the target leaf's frame)
+        __builtin_amdgcn_wave_barrier();
+        sb[lane] = make_float4(v.x - kox, v.y - koy, v.z - koz, v.w);
+        __builtin_amdgcn_wave_barrier();
+        // advance to the next chunk and start its load
+        off += P2P_CHUNK;
+        bool more = true;
+        if (off >= kn) {
+          off = 0;
+          k++;
+          if (k >= nsl) {
+            more = false;
+          } else {
+            kb = __builtin_amdgcn_readlane(mbeg, k);
+            kn = __builtin_amdgcn_readlane(mnb, k);
+            kox = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mox), k));
+            koy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(moy), k));
+            koz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(moz), k));
+          }
         }
-        __syncthreads();
+        if (more && off + lane < kn) v = rel[kb + off + lane];
+        // the current chunk
         float p0 = 0, p1 = 0, p2 = 0, p3 = 0;
         if (tvalid) {
           for (int jj = sub; jj < cn; jj += g) {
@@ -192,16 +233,23 @@ __global__ __launch_bounds__(64) void k_p2p_fp32(Cells c, Csr list, Bodies b,
         kahan_add(s1, c1, p1);
         kahan_add(s2, c2, p2);
         kahan_add(s3, c3, p3);
+        if (!more) break;
       }
     }
     double d0 = (double)s0 - c0, d1 = (double)s1 - c1, d2 = (double)s2 - c2, d3 = (double)s3 - c3;
-    for (int o = 1; o < g; o <<= 1) {
-      d0 += __shfl_xor(d0, o, 64);
-      d1 += __shfl_xor(d1, o, 64);
-      d2 += __shfl_xor(d2, o, 64);
-      d3 += __shfl_xor(d3, o, 64);
-    }
+    __syncthreads();
+    red[0][lane] = d0;
+    red[1][lane] = d1;
+    red[2][lane] = d2;
+    red[3][lane] = d3;
+    __syncthreads();
     if (tvalid && sub == 0) {
+      for (int u = 1; u < g; u++) {
+        d0 += red[0][lane + u];
+        d1 += red[1][lane + u];
+        d2 += red[2][lane + u];
+        d3 += red[3][lane + u];
+      }
       // the FP64 path stores q_i * sum (the reference's P2P sum, divided by
       // q_i in L2P); keep that convention
       const double qi = b.q[tb + tl];
