@@ -129,9 +129,7 @@ def main():
     picked = None
     ncrit, theta = args.ncrit, args.theta
     if precision == tfmm.MIXED and not args.no_learned and os.path.exists(args.picker):
-        w = np.load(args.picker, allow_pickle=False)["weights"]
-        pk = tfmm.Picker(0)
-        pk.set_weights(w)
+        pk = tfmm.Picker.load(args.picker)
         t0 = time.perf_counter()
         p = ctx.pick_params(pk, precision)
         pick_ms = (time.perf_counter() - t0) * 1e3

@@ -164,6 +164,10 @@ void tfmm_picker_destroy(tfmm_picker *p);
 const char *tfmm_picker_last_error(const tfmm_picker *p);
 int tfmm_picker_set_weights(tfmm_picker *p, const float *w, int64_t n);
 int tfmm_picker_get_weights(tfmm_picker *p, float *w, int64_t n);
+/* Upper bound on the theta tfmm_pick_params returns (default 0.6): the
+ * largest theta among the training labels, so that the regression head does
+ * not extrapolate past the configurations it was trained on. */
+int tfmm_picker_set_theta_max(tfmm_picker *p, float theta_max);
 /* Full-batch Adam on n samples (16 features each, ncrit class 0..4, theta in
  * [0.2, 0.6]); loss_per_step (optional) receives the mean loss of each step. */
 int tfmm_picker_train(tfmm_picker *p, const float *X, const int32_t *cls, const float *theta,
@@ -183,7 +187,7 @@ int tfmm_tree_features(tfmm_ctx *ctx, float *feat16);
 /* Pick ncrit and theta for the current bodies: builds a trial tree at
  * ncrit = 64, reads its statistics and runs the picker's network; with a null
  * picker (or a non-finite output) the fallback ncrit = 64, theta = 0.4 is
- * returned.  The precision field of *io is kept. */
+ * returned.  theta is capped by tfmm_picker_set_theta_max.  The precision field of *io is kept. */
 int tfmm_pick_params(tfmm_ctx *ctx, tfmm_picker *picker, tfmm_params *io);
 
 #ifdef __cplusplus

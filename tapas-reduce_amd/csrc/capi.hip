@@ -949,7 +949,7 @@ int tfmm_pick_params(tfmm_ctx *c, tfmm_picker *picker, tfmm_params *io) {
   }
   if (!std::isfinite(o[5])) return TFMM_OK;
   io->ncrit = kNcrit[best];
-  io->theta = 0.2 + 0.4 / (1.0 + std::exp(-(double)o[5]));
+  io->theta = std::min(0.2 + 0.4 / (1.0 + std::exp(-(double)o[5])), (double)picker_theta_max(picker));
   return TFMM_OK;
 }
 

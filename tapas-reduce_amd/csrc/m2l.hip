@@ -8,6 +8,8 @@
 // of the singular harmonics each into LDS, then lane t < 55 forms output
 // coefficient t into a per-pair partial.  A second kernel adds the partials
 // of each target in list order (deterministic, no atomics).
+#include <algorithm>
+
 #include "tfmm_device.h"
 #include "tfmm_kernels.h"
 
@@ -19,14 +21,15 @@ __device__ inline int sgn(int n) { return (n & 1) ? -1 : 1; }
 constexpr int PAIR_WAVES = 4;
 
 __global__ __launch_bounds__(64 * PAIR_WAVES) void k_m2l_pairs_fp64(Cells c, Csr list,
-                                                                   int32_t npairs,
+                                                                   int32_t p0, int32_t npairs,
                                                                    const int32_t *tgt,
                                                                    const double *M, cd *part) {
   __shared__ cd Ys[PAIR_WAVES][NTERM];
   __shared__ cd Ms[PAIR_WAVES][NTERM];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int32_t pi = blockIdx.x * PAIR_WAVES + w;
-  if (pi >= npairs) return;  // whole wave leaves together; no block barrier follows
+  const int32_t pl = blockIdx.x * PAIR_WAVES + w;  // pair within the chunk
+  if (pl >= npairs) return;  // whole wave leaves together; no block barrier follows
+  const int32_t pi = p0 + pl;
   const int32_t ti = tgt[pi], sj = list.src[pi];
   int j = 0, k = 0;
   if (lane < NTERM) {
@@ -61,7 +64,7 @@ __global__ __launch_bounds__(64 * PAIR_WAVES) void k_m2l_pairs_fp64(Cells c, Csr
       acc = cadd(acc, t);
     }
   }
-  part[(int64_t)pi * NTERM + lane] = acc;
+  part[(int64_t)pl * NTERM + lane] = acc;
 }
 
 // target of every list entry
@@ -71,36 +74,45 @@ __global__ void k_list_targets(const int32_t *off, int32_t ncells, int32_t *tgt)
   for (int32_t e = off[t]; e < off[t + 1]; e++) tgt[e] = t;
 }
 
-// L_t += partials of t's pairs, in list order; one wave per target
-__global__ __launch_bounds__(256) void k_m2l_reduce(Csr list, int32_t ncells, const cd *part,
-                                                    double *L) {
+// L_t += partials of t's pairs within the chunk [p0, p1), in list order; one
+// wave per target
+__global__ __launch_bounds__(256) void k_m2l_reduce(Csr list, int32_t ncells, int32_t p0,
+                                                    int32_t p1, const cd *part, double *L) {
   const int lane = threadIdx.x & 63;
   const int32_t ti = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (ti >= ncells || lane >= NTERM) return;
-  const int32_t b = list.off[ti], e = list.off[ti + 1];
-  if (b == e) return;
+  const int32_t b = max(list.off[ti], p0), e = min(list.off[ti + 1], p1);
+  if (b >= e) return;
   cd acc = mk(0, 0);
-  for (int32_t p = b; p < e; p++) acc = cadd(acc, part[(int64_t)p * NTERM + lane]);
+  for (int32_t p = b; p < e; p++) acc = cadd(acc, part[(int64_t)(p - p0) * NTERM + lane]);
   cd *Li = reinterpret_cast<cd *>(L) + (int64_t)ti * NTERM;
   Li[lane] = cadd(Li[lane], acc);
 }
 
 }  // namespace
 
+constexpr int32_t PAIR_CHUNK = 1 << 22;  // pairs per partials buffer (3.7 GB)
+
 size_t m2l_pairs_scratch_bytes(int32_t npairs) {
-  return (size_t)npairs * (NTERM * sizeof(cd) + sizeof(int32_t));
+  const size_t chunk = (size_t)std::min(npairs, PAIR_CHUNK);
+  return chunk * NTERM * sizeof(cd) + (size_t)npairs * sizeof(int32_t);
 }
 
-// scratch: m2l_pairs_scratch_bytes(npairs) bytes (partials, then targets)
+// scratch: m2l_pairs_scratch_bytes(npairs) bytes (partials of one chunk of
+// pairs, then the target of every pair); chunks are added in list order
 hipError_t launch_m2l_fp64(Cells c, int32_t ncells, Csr list, int32_t npairs, const double *M,
                            double *L, void *scratch, hipStream_t s) {
   if (ncells <= 0 || npairs <= 0) return hipSuccess;
+  const int32_t chunk = std::min(npairs, PAIR_CHUNK);
   cd *part = static_cast<cd *>(scratch);
-  int32_t *tgt = reinterpret_cast<int32_t *>(part + (size_t)npairs * NTERM);
+  int32_t *tgt = reinterpret_cast<int32_t *>(part + (size_t)chunk * NTERM);
   k_list_targets<<<(ncells + 255) / 256, 256, 0, s>>>(list.off, ncells, tgt);
-  k_m2l_pairs_fp64<<<(npairs + PAIR_WAVES - 1) / PAIR_WAVES, 64 * PAIR_WAVES, 0, s>>>(
-      c, list, npairs, tgt, M, part);
-  k_m2l_reduce<<<(ncells + 3) / 4, 256, 0, s>>>(list, ncells, part, L);
+  for (int32_t p0 = 0; p0 < npairs; p0 += chunk) {
+    const int32_t n = std::min(chunk, npairs - p0);
+    k_m2l_pairs_fp64<<<(n + PAIR_WAVES - 1) / PAIR_WAVES, 64 * PAIR_WAVES, 0, s>>>(
+        c, list, p0, n, tgt, M, part);
+    k_m2l_reduce<<<(ncells + 3) / 4, 256, 0, s>>>(list, ncells, p0, p0 + n, part, L);
+  }
   return hipGetLastError();
 }
 

@@ -69,10 +69,13 @@ constexpr int WAVE_SYNC = 16;  // items between block barriers in the tile loop
 // ---------------------------------------------------------------------------
 // Slot of a (target level, level difference, offset) in the direct-mapped
 // table: offsets are in units of half the finer width; same-level offsets are
-// even with |o| <= 14, adjacent-level offsets odd with |o| <= 15.
-constexpr int SAME_HALF = 7;                 // |o/2| <= 7
-constexpr int SAME_N = 2 * SAME_HALF + 1;    // 15
-constexpr int CROSS_N = 16;                  // (o + 15) / 2 in [0, 15]
+// even with |o| <= 22, adjacent-level offsets odd with |o| <= 23.  An M2L
+// pair's parents were not well separated, so its offset is at most about
+// 2 (2 / theta + 1) half widths: the table covers theta >= 0.2.
+constexpr int SAME_HALF = 11;                // |o/2| <= 11
+constexpr int SAME_N = 2 * SAME_HALF + 1;    // 23
+constexpr int CROSS_MAX = 23;                // |o| <= 23, odd
+constexpr int CROSS_N = CROSS_MAX + 1;       // (o + 23) / 2 in [0, 23]
 constexpr int SLOTS_SAME = SAME_N * SAME_N * SAME_N;
 constexpr int SLOTS_CROSS = CROSS_N * CROSS_N * CROSS_N;
 constexpr int SLOTS_PER_LEVEL = SLOTS_SAME + 2 * SLOTS_CROSS;
@@ -92,7 +95,7 @@ __device__ inline int32_t group_slot(int lt, int ls, const int o[3]) {
     int a[3];
     for (int d = 0; d < 3; d++) {
       if (!(o[d] & 1)) return -1;
-      a[d] = (o[d] + 15) / 2;
+      a[d] = (o[d] + CROSS_MAX) / 2;
       if (a[d] < 0 || a[d] >= CROSS_N) return -1;
     }
     idx = SLOTS_SAME + (dl == 1 ? 0 : SLOTS_CROSS) + (a[0] * CROSS_N + a[1]) * CROSS_N + a[2];
@@ -115,10 +118,10 @@ __device__ inline void slot_decode(int32_t slot, int &lt, int &ls, int o[3]) {
     idx -= SLOTS_SAME;
     ls = idx < SLOTS_CROSS ? lt + 1 : lt - 1;
     idx %= SLOTS_CROSS;
-    o[2] = (idx % CROSS_N) * 2 - 15;
+    o[2] = (idx % CROSS_N) * 2 - CROSS_MAX;
     idx /= CROSS_N;
-    o[1] = (idx % CROSS_N) * 2 - 15;
-    o[0] = (idx / CROSS_N) * 2 - 15;
+    o[1] = (idx % CROSS_N) * 2 - CROSS_MAX;
+    o[0] = (idx / CROSS_N) * 2 - CROSS_MAX;
   }
 }
 

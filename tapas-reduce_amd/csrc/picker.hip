@@ -14,6 +14,7 @@ using namespace tfmm;
 
 struct tfmm_picker {
   int device = 0;
+  float theta_max = 0.6f;  // largest theta tfmm_pick_params may return
   hipStream_t stream = nullptr;
   std::string err;
   DevBuf W, m, v, X, cls, theta, G, loss, gmean, out;
@@ -62,6 +63,7 @@ int check_labels(tfmm_picker *p, const int32_t *cls, const float *theta, int n) 
 
 namespace tfmm {
 const float *picker_weights(tfmm_picker *p) { return p ? dp<float>(p->W) : nullptr; }
+float picker_theta_max(const tfmm_picker *p) { return p ? p->theta_max : 0.6f; }
 }  // namespace tfmm
 
 extern "C" {
@@ -124,6 +126,14 @@ int tfmm_picker_set_weights(tfmm_picker *p, const float *w, int64_t n) {
   PTRY(hipMemsetAsync(p->v.ptr, 0, n * 4, p->stream));
   PTRY(hipStreamSynchronize(p->stream));
   p->step = 0;
+  return TFMM_OK;
+}
+
+int tfmm_picker_set_theta_max(tfmm_picker *p, float theta_max) {
+  if (!p) return TFMM_EINVAL;
+  if (!(theta_max >= 0.2f && theta_max <= 0.6f))
+    return pfail(p, TFMM_EINVAL, "theta_max outside [0.2, 0.6]");
+  p->theta_max = theta_max;
   return TFMM_OK;
 }
 

@@ -45,4 +45,5 @@ struct tfmm_picker;
 namespace tfmm {
 // device weights of a picker (null for a null picker)
 const float *picker_weights(tfmm_picker *p);
+float picker_theta_max(const tfmm_picker *p);
 }  // namespace tfmm

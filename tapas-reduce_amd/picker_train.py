@@ -3,15 +3,20 @@ on the GPU.
 
 Every problem (distribution, N log-uniform in [2^16, 2^23], seed) is
 evaluated in mixed precision at each (ncrit, theta) of the grid
-{16, 32, 64, 128, 256} x {0.2, 0.3, 0.4, 0.5, 0.6}; its error is the larger
-of the potential and acceleration rel-L2 errors against the FP64 direct sum
-on 1024 random targets, its cost the wall time of one evaluation (best of
-two).  The label of a problem is the cheapest configuration whose error is
-<= the target (1e-4); the features are the 16 statistics of its trial tree
-(ncrit = 64).  The network is then trained by the library's own kernels
-(tfmm_picker_train) and its weights written to picker/weights.npz.
+{16, 32, 64, 128, 256} x {0.2, 0.3, 0.36, 0.4, 0.42, ..., 0.5, 0.6}
+(finer where the error crosses 1e-4); its error is the larger of the
+potential and acceleration rel-L2 errors against the FP64 direct sum on 1024
+random targets, its cost the wall time of one evaluation (best of two).  The
+label of a problem is the cheapest configuration whose error is <= the
+target times a safety margin; the features are the 16 statistics of its
+trial tree (ncrit = 64).  The network is then trained by the library's own
+kernels (tfmm_picker_train) and its weights written to picker/weights.npz.
 
-    python tapas-reduce_amd/picker_train.py --problems 80 --out tapas-reduce_amd/picker
+Measuring is split into slices that can run as separate processes:
+
+    python tapas-reduce_amd/picker_train.py --first 0 --count 16 --measure-only
+    ...
+    python tapas-reduce_amd/picker_train.py --data picker/runs_000.npz picker/runs_016.npz ...
 """
 import argparse
 import json
@@ -27,7 +32,7 @@ import datasets  # noqa: E402
 import tfmm  # noqa: E402
 
 NCRITS = tfmm.NCRIT_CLASSES
-THETAS = (0.2, 0.3, 0.4, 0.5, 0.6)
+THETAS = (0.2, 0.3, 0.36, 0.4, 0.42, 0.44, 0.46, 0.48, 0.5, 0.6)
 DISTS = ("cube", "plummer", "ellipsoid", "two_plummer")
 
 
@@ -49,11 +54,17 @@ def measure(ctx, x, y, z, q, targets, direct, precision):
     for a, nc in enumerate(NCRITS):
         for b, th in enumerate(THETAS):
             best = 1e30
-            for _ in range(2):
-                t0 = time.perf_counter()
-                ctx.evaluate(tfmm.make_params(nc, th, precision))
-                ctx.sync()
-                best = min(best, time.perf_counter() - t0)
+            try:
+                for _ in range(2):
+                    t0 = time.perf_counter()
+                    ctx.evaluate(tfmm.make_params(nc, th, precision))
+                    ctx.sync()
+                    best = min(best, time.perf_counter() - t0)
+            except tfmm.TfmmError as e:  # e.g. lists too large for the device
+                print("  (ncrit %d, theta %g) not evaluated: %s" % (nc, th, e), flush=True)
+                errs[a, b] = np.inf
+                times[a, b] = np.inf
+                continue
             f = ctx.fields()[targets]
             ep = np.sqrt(((f[:, 0] - direct[:, 0]) ** 2).sum() / (direct[:, 0] ** 2).sum())
             ea = np.sqrt(((f[:, 1:] - direct[:, 1:]) ** 2).sum() / (direct[:, 1:] ** 2).sum())
@@ -73,10 +84,10 @@ def labels(errs, times, target):
     return int(a), float(THETAS[b])
 
 
-def generate(nprob, seed0, ntarget, precision, log):
+def generate(first, count, seed0, ntarget, precision, log):
     ctx = tfmm.Context(0)
     F, E, T, meta = [], [], [], []
-    for i in range(nprob):
+    for i in range(first, first + count):
         dist, n, seed = problem(i, seed0)
         x, y, z, q = datasets.make(dist, n, seed=seed)
         rng = np.random.default_rng(seed)
@@ -97,7 +108,11 @@ def generate(nprob, seed0, ntarget, precision, log):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--problems", type=int, default=80)
+    ap.add_argument("--first", type=int, default=0)
+    ap.add_argument("--count", type=int, default=96)
+    ap.add_argument("--measure-only", action="store_true")
+    ap.add_argument("--margin", type=float, default=0.6,
+                    help="labels use target-error * margin")
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--targets", type=int, default=1024)
     ap.add_argument("--target-error", type=float, default=1e-4)
@@ -105,7 +120,8 @@ def main():
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--holdout", type=float, default=0.2)
     ap.add_argument("--out", default=os.path.join(HERE, "picker"))
-    ap.add_argument("--data", default=None, help="reuse a saved dataset instead of measuring")
+    ap.add_argument("--data", nargs="*", default=None,
+                    help="train on saved measurement slices instead of measuring")
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
 
@@ -113,14 +129,23 @@ def main():
         print(msg, flush=True)
 
     if args.data:
-        d = np.load(args.data, allow_pickle=False)
-        F, E, T = d["features"], d["errors"], d["times"]
-        meta = [tuple(m) for m in json.loads(str(d["meta"]))]
+        F, E, T, meta = [], [], [], []
+        for path in args.data:
+            d = np.load(path, allow_pickle=False)
+            F.append(d["features"])
+            E.append(d["errors"])
+            T.append(d["times"])
+            meta += [tuple(m) for m in json.loads(str(d["meta"]))]
+        F, E, T = np.concatenate(F), np.concatenate(E), np.concatenate(T)
     else:
-        F, E, T, meta = generate(args.problems, args.seed, args.targets, tfmm.MIXED, log)
-        np.savez(os.path.join(args.out, "runs.npz"), features=F, errors=E, times=T,
-                 meta=json.dumps(meta), ncrits=np.array(NCRITS), thetas=np.array(THETAS))
-    lab = [labels(E[i], T[i], args.target_error) for i in range(len(F))]
+        F, E, T, meta = generate(args.first, args.count, args.seed, args.targets, tfmm.MIXED,
+                                 log)
+        np.savez(os.path.join(args.out, "runs_%03d.npz" % args.first), features=F, errors=E,
+                 times=T, meta=json.dumps(meta), ncrits=np.array(NCRITS),
+                 thetas=np.array(THETAS))
+        if args.measure_only:
+            return
+    lab = [labels(E[i], T[i], args.target_error * args.margin) for i in range(len(F))]
     cls = np.array([c for c, _ in lab], np.int32)
     th = np.array([t for _, t in lab], np.float32)
     n = len(F)
@@ -130,21 +155,45 @@ def main():
     test_idx, train_idx = perm[:nh], perm[nh:]
     pk = tfmm.Picker(0, seed=args.seed)
     loss = pk.train(F[train_idx], cls[train_idx], th[train_idx], steps=args.steps, lr=args.lr)
+    theta_max = float(th[train_idx].max())  # the head must not extrapolate past its labels
+    pk.set_theta_max(theta_max)
     np.savez(os.path.join(args.out, "weights.npz"), weights=pk.weights(),
-             train_idx=train_idx, test_idx=test_idx, target_error=args.target_error)
-    # how the picked configurations compare on held-out problems: the picked
-    # (ncrit, theta) is snapped to the grid to read its measured cost/error
+             train_idx=train_idx, test_idx=test_idx, target_error=args.target_error,
+             margin=args.margin, theta_max=theta_max)
+    # the picked configurations on the held-out problems, evaluated at the
+    # exact predicted (ncrit, theta) next to the fallback (64, 0.4)
     out = pk.predict(F)
-    nc, pt = tfmm.decode(out)
-    fb_a, fb_b = NCRITS.index(64), THETAS.index(0.4)
+    nc, pt = tfmm.decode(out, theta_max)
     report = []
+    ctx = tfmm.Context(0)
     for i in test_idx:
-        a = NCRITS.index(int(nc[i]))
-        b = int(np.argmin(np.abs(np.array(THETAS) - pt[i])))
-        report.append(dict(problem=meta[i], picked=(int(nc[i]), float(pt[i])),
-                           err=float(E[i][a, b]), time=float(T[i][a, b]),
-                           fallback_err=float(E[i][fb_a, fb_b]),
-                           fallback_time=float(T[i][fb_a, fb_b])))
+        dist, nb, seed = meta[i]
+        x, y, z, q = datasets.make(dist, int(nb), seed=int(seed))
+        rng = np.random.default_rng(int(seed))
+        targets = np.sort(rng.choice(int(nb), size=min(args.targets, int(nb)), replace=False))
+        ctx.set_bodies(x, y, z, q)
+        direct = ctx.direct(targets)
+        res = {}
+        for name, (a_nc, a_th) in (("picked", (int(nc[i]), float(pt[i]))), ("fallback", (64, 0.4))):
+            prm = tfmm.make_params(a_nc, a_th, tfmm.MIXED)
+            ctx.evaluate(prm)
+            best = 1e30
+            for _ in range(3):
+                t0 = time.perf_counter()
+                ctx.evaluate(prm)
+                ctx.sync()
+                best = min(best, time.perf_counter() - t0)
+            f = ctx.fields()[targets]
+            ep = np.sqrt(((f[:, 0] - direct[:, 0]) ** 2).sum() / (direct[:, 0] ** 2).sum())
+            ea = np.sqrt(((f[:, 1:] - direct[:, 1:]) ** 2).sum() / (direct[:, 1:] ** 2).sum())
+            res[name] = (a_nc, a_th, float(max(ep, ea)), best)
+        report.append(dict(problem=meta[i], picked=res["picked"][:2], err=res["picked"][2],
+                           time=res["picked"][3], fallback_err=res["fallback"][2],
+                           fallback_time=res["fallback"][3]))
+        log("held-out %-12s n=%-8d picked (%d, %.3f) err %.2e %.1f ms | fallback err %.2e "
+            "%.1f ms" % (dist, int(nb), res["picked"][0], res["picked"][1], res["picked"][2],
+                         1e3 * res["picked"][3], res["fallback"][2], 1e3 * res["fallback"][3]))
+    ctx.close()
     ok = np.mean([r["err"] <= args.target_error for r in report]) if report else float("nan")
     gain = (np.exp(np.mean([np.log(r["fallback_time"] / r["time"]) for r in report]))
             if report else float("nan"))

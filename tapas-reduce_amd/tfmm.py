@@ -67,7 +67,8 @@ EXPORTED = [
     "tfmm_traverse", "tfmm_downward", "tfmm_evaluate", "tfmm_get_fields", "tfmm_get_stats",
     "tfmm_get_sorted", "tfmm_get_cells", "tfmm_get_expansions", "tfmm_get_lists", "tfmm_direct",
     "tfmm_picker_nparam", "tfmm_picker_create", "tfmm_picker_destroy", "tfmm_picker_last_error",
-    "tfmm_picker_set_weights", "tfmm_picker_get_weights", "tfmm_picker_train",
+    "tfmm_picker_set_weights", "tfmm_picker_get_weights", "tfmm_picker_set_theta_max",
+    "tfmm_picker_train",
     "tfmm_picker_gradient", "tfmm_picker_predict", "tfmm_tree_features", "tfmm_pick_params",
 ]
 
@@ -112,6 +113,7 @@ def lib():
         L.tfmm_picker_last_error.argtypes = [vp]
         L.tfmm_picker_last_error.restype = ctypes.c_char_p
         L.tfmm_picker_set_weights.argtypes = [vp, vp, ctypes.c_int64]
+        L.tfmm_picker_set_theta_max.argtypes = [vp, ctypes.c_float]
         L.tfmm_picker_get_weights.argtypes = [vp, vp, ctypes.c_int64]
         L.tfmm_picker_train.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                         vp]
@@ -321,6 +323,20 @@ class Picker:
         w = np.ascontiguousarray(w, np.float32)
         self._check(lib().tfmm_picker_set_weights(self._h, _ptr(w), len(w)), "set_weights")
 
+    def set_theta_max(self, theta_max):
+        self._check(lib().tfmm_picker_set_theta_max(self._h, float(theta_max)), "set_theta_max")
+        self.theta_max = float(theta_max)
+
+    @classmethod
+    def load(cls, path, device=0):
+        """A picker with the weights (and theta cap) saved by picker_train.py"""
+        d = np.load(path, allow_pickle=False)
+        pk = cls(device)
+        pk.set_weights(d["weights"])
+        if "theta_max" in d.files:
+            pk.set_theta_max(float(d["theta_max"]))
+        return pk
+
     @staticmethod
     def _batch(X, cls, theta):
         X = np.ascontiguousarray(X, np.float32).reshape(-1, 16)
@@ -350,12 +366,13 @@ class Picker:
         return out
 
 
-def decode(out):
-    """(ncrit, theta) from raw picker outputs."""
+def decode(out, theta_max=0.6):
+    """(ncrit, theta) from raw picker outputs (theta capped at theta_max, as
+    tfmm_pick_params does)."""
     out = np.atleast_2d(out)
     cls = np.argmax(out[:, :5], axis=1)
     theta = 0.2 + 0.4 / (1.0 + np.exp(-out[:, 5].astype(np.float64)))
-    return np.array(NCRIT_CLASSES)[cls], theta
+    return np.array(NCRIT_CLASSES)[cls], np.minimum(theta, theta_max)
 
 
 def evaluate(x, y, z, q, ncrit=64, theta=0.4, precision=FP64, device=0):

@@ -71,3 +71,63 @@ def test_gpu_training_fits_a_separable_task():
     nc, th = tfmm.decode(pk.predict(X))
     acc = np.mean(nc == np.array(tfmm.NCRIT_CLASSES)[cls])
     assert acc > 0.9 and np.abs(th - theta).mean() < 0.03
+
+
+def test_trained_weights_file_is_consistent():
+    """The committed weights: right size, finite, theta cap inside the head's
+    range, and the held-out record written with them meets the target."""
+    import json
+    import os
+    here = os.path.join(os.path.dirname(__file__), "..", "tapas-reduce_amd", "picker")
+    d = np.load(os.path.join(here, "weights.npz"), allow_pickle=False)
+    assert d["weights"].shape == (mlp_oracle.NPARAM,) and np.isfinite(d["weights"]).all()
+    assert 0.2 <= float(d["theta_max"]) <= 0.6
+    s = json.load(open(os.path.join(here, "summary.json")))["summary"]
+    assert s["runs"] >= 2000 and s["heldout"] >= 10
+    assert s["heldout_meets_target"] == 1.0
+    assert s["heldout_geomean_speedup_vs_fallback"] > 1.0
+
+
+@pytest.mark.gpu
+def test_gpu_learned_parameters_beat_the_fallback():
+    """On problems outside the training set, the trained picker's (ncrit,
+    theta) meets the 1e-4 error target and is faster than the fallback
+    ncrit = 64, theta = 0.4 (best of 3 timed evaluations each)."""
+    import os
+    import time
+    import datasets
+    import tfmm
+    path = os.path.join(os.path.dirname(__file__), "..", "tapas-reduce_amd", "picker",
+                        "weights.npz")
+    pk = tfmm.Picker.load(path)
+    ratios = []
+    with tfmm.Context(0) as ctx:
+        for dist, n, seed in (("plummer", 1 << 20, 5001), ("two_plummer", 1 << 21, 5002)):
+            x, y, z, q = datasets.make(dist, n, seed=seed)
+            ctx.set_bodies(x, y, z, q)
+            rng = np.random.default_rng(seed)
+            targets = np.sort(rng.choice(n, size=1024, replace=False))
+            direct = ctx.direct(targets)
+            picked = ctx.pick_params(pk, tfmm.MIXED)
+            times = {}
+            for name, prm in (("picked", tfmm.make_params(int(picked.ncrit), float(picked.theta),
+                                                          tfmm.MIXED)),
+                              ("fallback", tfmm.make_params(64, 0.4, tfmm.MIXED))):
+                ctx.evaluate(prm)
+                best = 1e30
+                for _ in range(3):
+                    t0 = time.perf_counter()
+                    ctx.evaluate(prm)
+                    ctx.sync()
+                    best = min(best, time.perf_counter() - t0)
+                times[name] = best
+                f = ctx.fields()[targets]
+                ep = np.sqrt(((f[:, 0] - direct[:, 0]) ** 2).sum() / (direct[:, 0] ** 2).sum())
+                ea = np.sqrt(((f[:, 1:] - direct[:, 1:]) ** 2).sum() / (direct[:, 1:] ** 2).sum())
+                assert max(ep, ea) <= 1e-4, (dist, name, ep, ea)
+            print("%s n=%d picked (%d, %.3f): %.2f ms vs fallback %.2f ms" % (
+                dist, n, picked.ncrit, picked.theta, 1e3 * times["picked"],
+                1e3 * times["fallback"]))
+            ratios.append(times["fallback"] / times["picked"])
+    pk.close()
+    assert np.exp(np.mean(np.log(ratios))) > 1.05, ratios
