@@ -75,7 +75,9 @@ struct tfmm_ctx {
   // traversal
   DevBuf front_a, front_b, m2l_pairs, p2p_pairs, cnt, lkeys, lkeys_alt, lidx, lidx_alt;
   DevBuf m2l_off, m2l_src, p2p_off, p2p_src;
-  uint32_t m2l_cap = 0, p2p_cap = 0;
+  uint32_t m2l_cap = 0, p2p_cap = 0, front_cap = 0;
+  std::vector<uint32_t> dtt_hint;  // frontier sizes of the last traversal
+  int64_t dtt_hint_cells = -1;
   uint32_t n_m2l_exec = 0, n_p2p = 0;
   uint64_t n_m2l_all = 0, n_p2p_calls = 0, n_p2p_self = 0;
   uint32_t dtt_steps = 0;
@@ -96,7 +98,7 @@ struct tfmm_ctx {
   DevBuf acc, trg, out;
 
   // host staging (pinned)
-  uint32_t *h_u32 = nullptr;
+  uint32_t *h_u32 = nullptr;  // pinned staging, H_U32_WORDS words
   unsigned long long *h_u64 = nullptr;
 
   hipEvent_t ev[EV_COUNT] = {};
@@ -180,6 +182,8 @@ hipError_t reserve_cells(tfmm_ctx *c, size_t need, size_t used, hipStream_t s) {
   c->cell_cap = cap;
   return hipSuccess;
 }
+
+constexpr int H_U32_WORDS = 256;
 
 int bits_for(int64_t v) {
   int b = 1;
@@ -320,45 +324,81 @@ int sort_list(tfmm_ctx *c, DevBuf &pairs, uint32_t npairs, DevBuf &off, DevBuf &
   return TFMM_OK;
 }
 
+// The traversal runs 2 max_depth + 2 breadth-first steps (every step refines
+// one side of a pair by one level, so no pair survives longer), each reading
+// its input size on the device: one host round trip for the whole traversal.
+// The frontier buffers and the lists have capacities; if any count exceeds
+// its capacity the traversal is run again with the observed sizes.
 int do_dtt(tfmm_ctx *c) {
   hipStream_t s = c->stream;
   const Cells k = cells_of(c);
-  TRY(dev_reserve(c->cnt, 64));
+  const int max_depth = (int)c->lvl.size() - 2;
+  const int nsteps = 2 * max_depth + 2;
+  const int cwords = 16 + nsteps + 1;  // list counters, then frontier sizes 0..nsteps
+  TRY(dev_reserve(c->cnt, (size_t)cwords * 4));
   if (c->m2l_cap == 0) c->m2l_cap = (uint32_t)std::min<int64_t>(UINT32_MAX / 2, std::max<int64_t>(1 << 16, 200 * c->ncells));
   if (c->p2p_cap == 0) c->p2p_cap = (uint32_t)std::min<int64_t>(UINT32_MAX / 2, std::max<int64_t>(1 << 16, 40 * c->nleaves));
-  for (int attempt = 0; attempt < 3; attempt++) {
+  if (c->front_cap == 0) c->front_cap = (uint32_t)std::min<int64_t>(UINT32_MAX / 2, std::max<int64_t>(1 << 20, 256 * c->ncells));
+  if (cwords > H_U32_WORDS) return fail(c, TFMM_EINVAL, "tree too deep");
+  // A frontier that overflowed truncates every later step, so the counts
+  // seen after an overflow are lower bounds: capacities at least double.
+  for (int attempt = 0; attempt < 12; attempt++) {
     TRY(dev_reserve(c->m2l_pairs, (size_t)c->m2l_cap * 8));
     TRY(dev_reserve(c->p2p_pairs, (size_t)c->p2p_cap * 8));
-    TRY(dev_reserve(c->front_a, 64));
-    TRY(hipMemsetAsync(c->cnt.ptr, 0, 64, s));
-    c->h_u32[8] = 0;  // the root pair (0, 0), staged in pinned memory
+    TRY(dev_reserve(c->front_a, (size_t)c->front_cap * 8));
+    TRY(dev_reserve(c->front_b, (size_t)c->front_cap * 8));
+    uint32_t *cnt = dp<uint32_t>(c->cnt);
+    TRY(hipMemsetAsync(cnt, 0, (size_t)cwords * 4, s));
+    c->h_u32[8] = 0;  // the root pair (0, 0) and its count, staged in pinned memory
     c->h_u32[9] = 0;
+    c->h_u32[10] = 1;
     TRY(hipMemcpyAsync(c->front_a.ptr, c->h_u32 + 8, 8, hipMemcpyHostToDevice, s));
-    uint32_t nin = 1;
-    c->dtt_steps = 0;
-    while (nin > 0) {
-      const size_t out_need = (size_t)nin * 8 * 8;
-      TRY(dev_reserve(c->front_b, out_need));
-      TRY(hipMemsetAsync(c->cnt.ptr, 0, 4, s));
-      TRY(launch_dtt_step(dp<int2>(c->front_a), nullptr, nin, dp<int2>(c->front_b),
-                          (uint32_t)(out_need / 8), dp<int2>(c->m2l_pairs), c->m2l_cap,
-                          dp<int2>(c->p2p_pairs), c->p2p_cap, dp<uint32_t>(c->cnt), k,
-                          c->prm.theta, s));
-      TRY(hipMemcpyAsync(c->h_u32, c->cnt.ptr, 16, hipMemcpyDeviceToHost, s));
-      TRY(hipStreamSynchronize(s));
-      nin = c->h_u32[0];
+    TRY(hipMemcpyAsync(cnt + 16, c->h_u32 + 10, 4, hipMemcpyHostToDevice, s));
+    // grid of step st: the frontier size of the previous traversal of the same
+    // tree shape when there is one (checked below), else the capacity
+    const bool hint = (int)c->dtt_hint.size() == nsteps + 1 && c->dtt_hint_cells == c->ncells;
+    std::vector<uint32_t> grid(nsteps);
+    for (int st = 0; st < nsteps; st++) {
+      grid[st] = hint ? std::min<uint32_t>(c->front_cap, c->dtt_hint[st]) : c->front_cap;
+      TRY(launch_dtt_step(dp<int2>(c->front_a), cnt + 16 + st, grid[st], dp<int2>(c->front_b),
+                          c->front_cap, cnt + 16 + st + 1, dp<int2>(c->m2l_pairs), c->m2l_cap,
+                          dp<int2>(c->p2p_pairs), c->p2p_cap, cnt, k, c->prm.theta, s));
       std::swap(c->front_a, c->front_b);
-      c->dtt_steps++;
     }
+    TRY(hipMemcpyAsync(c->h_u32, cnt, (size_t)cwords * 4, hipMemcpyDeviceToHost, s));
+    TRY(hipStreamSynchronize(s));
+    uint32_t fmax = 0;
+    c->dtt_steps = 0;
+    for (int st = 0; st <= nsteps; st++) {
+      fmax = std::max(fmax, c->h_u32[16 + st]);
+      if (c->h_u32[16 + st]) c->dtt_steps = st + 1;
+    }
+    bool grid_short = false;
+    for (int st = 0; st < nsteps; st++)
+      grid_short |= grid[st] < c->front_cap && c->h_u32[16 + st] > grid[st];
+    if (grid_short) {  // a step saw more pairs than its hinted grid: run again unhinted
+      c->dtt_hint.clear();
+      attempt--;
+      continue;
+    }
+    if (c->h_u32[16 + nsteps] != 0) return fail(c, TFMM_EHIP, "traversal did not terminate");
     const uint32_t nm = c->h_u32[1], np = c->h_u32[2];
-    if (nm <= c->m2l_cap && np <= c->p2p_cap) {
+    if (nm <= c->m2l_cap && np <= c->p2p_cap && fmax <= c->front_cap) {
+      c->dtt_hint.assign(c->h_u32 + 16, c->h_u32 + 16 + nsteps + 1);
+      c->dtt_hint_cells = c->ncells;
       c->n_m2l_exec = nm;
       c->n_p2p = np;
       c->n_m2l_all = c->h_u32[3];
       return TFMM_OK;
     }
-    c->m2l_cap = std::max(c->m2l_cap, nm + nm / 8 + 1024);
-    c->p2p_cap = std::max(c->p2p_cap, np + np / 8 + 1024);
+    auto grow = [](uint32_t cap, uint32_t seen) {
+      const uint64_t want = std::max<uint64_t>(2 * (uint64_t)cap, (uint64_t)seen + seen / 4 + 1024);
+      return (uint32_t)std::min<uint64_t>(UINT32_MAX / 2, want);
+    };
+    const bool front_over = fmax > c->front_cap;
+    if (front_over || nm > c->m2l_cap) c->m2l_cap = grow(c->m2l_cap, nm);
+    if (front_over || np > c->p2p_cap) c->p2p_cap = grow(c->p2p_cap, np);
+    if (front_over) c->front_cap = grow(c->front_cap, fmax);
   }
   return fail(c, TFMM_EHIP, "traversal lists did not converge");
 }
@@ -575,7 +615,7 @@ int tfmm_create(int hip_device, tfmm_ctx **out) {
       return TFMM_EHIP;
     }
   }
-  if (hipHostMalloc((void **)&c->h_u32, 64, 0) != hipSuccess ||
+  if (hipHostMalloc((void **)&c->h_u32, H_U32_WORDS * 4, 0) != hipSuccess ||
       hipHostMalloc((void **)&c->h_u64, 64, 0) != hipSuccess) {
     tfmm_destroy(c);
     return TFMM_ENOMEM;

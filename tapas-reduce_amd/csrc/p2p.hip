@@ -172,10 +172,11 @@ __global__ __launch_bounds__(64) void k_p2p_fp32(Cells c, Csr list, Bodies b,
     for (int32_t base = lb; base < le; base += 64) {
       // metadata of source leaves base .. base + 63, one per lane
       const int nsl = min(64, le - base);
-      int32_t mbeg = 0, mnb = 0;
+      int32_t mbeg = 0, mnb = 0, msrc = -1;
       float mox = 0, moy = 0, moz = 0;
       if (lane < nsl) {
         const int32_t sj = list.src[base + lane];
+        msrc = sj;
         mbeg = c.beg[sj];
         mnb = c.nb[sj];
         mox = (float)(tcx - c.center[3 * (int64_t)sj]);
@@ -185,6 +186,7 @@ __global__ __launch_bounds__(64) void k_p2p_fp32(Cells c, Csr list, Bodies b,
       // chunk (k, off): bodies off .. off + 63 of source leaf k
       int k = 0, off = 0;
       int32_t kb = __builtin_amdgcn_readlane(mbeg, 0), kn = __builtin_amdgcn_readlane(mnb, 0);
+      int32_t ks = __builtin_amdgcn_readlane(msrc, 0);
       float kox = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mox), 0));
       float koy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(moy), 0));
       float koz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(moz), 0));
@@ -192,6 +194,7 @@ __global__ __launch_bounds__(64) void k_p2p_fp32(Cells c, Csr list, Bodies b,
       if (lane < kn) v = rel[kb + lane];
       while (true) {
         const int cn = min(P2P_CHUNK, kn - off);
+        const bool self = ks == ti;  // only the own leaf can hold r = 0 pairs
         // stage the current chunk (shifted into the target leaf's frame)
         __builtin_amdgcn_wave_barrier();
         sb[lane] = make_float4(v.x - kox, v.y - koy, v.z - koz, v.w);
@@ -207,6 +210,7 @@ __global__ __launch_bounds__(64) void k_p2p_fp32(Cells c, Csr list, Bodies b,
           } else {
             kb = __builtin_amdgcn_readlane(mbeg, k);
             kn = __builtin_amdgcn_readlane(mnb, k);
+            ks = __builtin_amdgcn_readlane(msrc, k);
             kox = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mox), k));
             koy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(moy), k));
             koz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(moz), k));
@@ -216,17 +220,35 @@ __global__ __launch_bounds__(64) void k_p2p_fp32(Cells c, Csr list, Bodies b,
         // the current chunk
         float p0 = 0, p1 = 0, p2 = 0, p3 = 0;
         if (tvalid) {
-          for (int jj = sub; jj < cn; jj += g) {
-            const float4 xj = sb[jj];
-            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            const float rinv = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;
-            const float qr = xj.w * rinv;
-            const float f = qr * rinv * rinv;
-            p0 += qr;
-            p1 -= dx * f;
-            p2 -= dy * f;
-            p3 -= dz * f;
+          if (self) {
+            for (int jj = sub; jj < cn; jj += g) {
+              const float4 xj = sb[jj];
+              const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+              const float r2 = dx * dx + dy * dy + dz * dz;
+              const float rinv = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;
+              const float qr = xj.w * rinv;
+              const float f = qr * rinv * rinv;
+              p0 += qr;
+              p1 -= dx * f;
+              p2 -= dy * f;
+              p3 -= dz * f;
+            }
+          } else {
+            // bodies of different leaves sit at different positions (equal
+            // positions give equal keys), so r > 0 unless they are closer than
+            // the FP32 resolution of the leaf-relative coordinates
+            for (int jj = sub; jj < cn; jj += g) {
+              const float4 xj = sb[jj];
+              const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+              const float r2 = dx * dx + dy * dy + dz * dz;
+              const float rinv = __builtin_amdgcn_rsqf(r2);
+              const float qr = xj.w * rinv;
+              const float f = qr * rinv * rinv;
+              p0 += qr;
+              p1 -= dx * f;
+              p2 -= dy * f;
+              p3 -= dz * f;
+            }
           }
         }
         kahan_add(s0, c0, p0);

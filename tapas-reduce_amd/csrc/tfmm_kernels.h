@@ -66,8 +66,8 @@ struct DttCounters {
   uint32_t *cnt;
 };
 hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin_max, int2 *out,
-                           uint32_t out_cap, int2 *m2l, uint32_t m2l_cap, int2 *p2p,
-                           uint32_t p2p_cap, uint32_t *cnt, Cells c, double theta,
+                           uint32_t out_cap, uint32_t *nout, int2 *m2l, uint32_t m2l_cap,
+                           int2 *p2p, uint32_t p2p_cap, uint32_t *cnt, Cells c, double theta,
                            hipStream_t s);
 hipError_t launch_pack_pairs(const int2 *pairs, uint32_t n, int sbits, uint64_t *keys,
                              uint32_t *idx, hipStream_t s);

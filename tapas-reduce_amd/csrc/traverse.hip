@@ -13,9 +13,11 @@
 //   else                      -> Ci x (children of Cj)
 // Here that recursion is expanded breadth first: one launch per frontier,
 // each pair either emits an M2L entry, a P2P entry or its 8 successors.
-// Counters (cnt, uint32 words): 0 next frontier, 1 executed M2L pairs, 2 P2P
-// pairs, 3 all M2L pairs; words 8-11 hold two uint64 sums: body-pair
-// interactions of the P2P pairs and those of the self pairs.  The
+// Counters: *nout the next frontier; cnt (uint32 words) 1 executed M2L pairs,
+// 2 P2P pairs, 3 all M2L pairs, words 8-11 two uint64 sums: body-pair
+// interactions of the P2P pairs and those of the self pairs.  The input size
+// is read on the device (*nin_dev, at most nin_max), so consecutive steps
+// need no host round trip.  The
 // MAC is evaluated in FP64 without contraction, in the reference's operation
 // order, so every decision is bit-identical to the reference's.
 #include "tfmm_device.h"
@@ -29,9 +31,9 @@ constexpr int DTT_WAVES = DTT_THREADS / 64;
 
 __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const uint32_t *nin_dev,
                                                   uint32_t nin_max, int2 *out, uint32_t out_cap,
-                                                  int2 *m2l, uint32_t m2l_cap, int2 *p2p,
-                                                  uint32_t p2p_cap, uint32_t *cnt, Cells c,
-                                                  double theta) {
+                                                  uint32_t *nout, int2 *m2l, uint32_t m2l_cap,
+                                                  int2 *p2p, uint32_t p2p_cap, uint32_t *cnt,
+                                                  Cells c, double theta) {
 #pragma clang fp contract(off)
   const uint32_t nin = nin_dev ? *nin_dev : nin_max;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -101,7 +103,7 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
     const int q = threadIdx.x;
     uint32_t tot = 0;
     for (int w = 0; w < DTT_WAVES; w++) tot += wc[w][q];
-    base[q] = tot ? atomicAdd(&cnt[q], tot) : 0u;
+    base[q] = tot ? atomicAdd(q == 0 ? nout : &cnt[q], tot) : 0u;
   } else if (threadIdx.x == 64) {
     unsigned long long tv = 0, ts = 0;
     for (int w = 0; w < DTT_WAVES; w++) {
@@ -166,12 +168,12 @@ __global__ void k_csr_offsets(const uint64_t *keys, uint32_t n, int sbits, int32
 }  // namespace
 
 hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin_max, int2 *out,
-                           uint32_t out_cap, int2 *m2l, uint32_t m2l_cap, int2 *p2p,
-                           uint32_t p2p_cap, uint32_t *cnt, Cells c, double theta,
+                           uint32_t out_cap, uint32_t *nout, int2 *m2l, uint32_t m2l_cap,
+                           int2 *p2p, uint32_t p2p_cap, uint32_t *cnt, Cells c, double theta,
                            hipStream_t s) {
   if (nin_max == 0) return hipSuccess;
   k_dtt_step<<<(nin_max + DTT_THREADS - 1) / DTT_THREADS, DTT_THREADS, 0, s>>>(
-      in, nin_dev, nin_max, out, out_cap, m2l, m2l_cap, p2p, p2p_cap, cnt, c, theta);
+      in, nin_dev, nin_max, out, out_cap, nout, m2l, m2l_cap, p2p, p2p_cap, cnt, c, theta);
   return hipGetLastError();
 }
 
