@@ -31,6 +31,7 @@ enum Ev {
   EV_M2L0,
   EV_M2LT,
   EV_M2L1,
+  EV_P2PREADY,
   EV_P2P0,
   EV_P2P1,
   EV_TRAV,
@@ -43,6 +44,7 @@ enum Ev {
 struct tfmm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // P2P runs here, overlapping the M2L work
   std::string err;
 
   // bodies (caller order)
@@ -527,15 +529,32 @@ int do_traverse(tfmm_ctx *c) {
   if (rc) return rc;
   rc = sort_list(c, c->p2p_pairs, c->n_p2p, c->p2p_off, c->p2p_src);
   if (rc) return rc;
-  rc = build_m2l_batches(c);
-  if (rc) return rc;
-  // M~ in the MFMA operand layout
-  TRY(dev_reserve(c->mq, (size_t)c->ncells * 112 * (mixed ? 4 : 8)));
-  TRY(launch_m2l_scale_m(k, (int32_t)c->ncells, dp<double>(c->M), c->w0, c->mq.ptr, mixed, s));
   if (mixed) {
     TRY(dev_reserve(c->relpos, (size_t)c->n * 16));
     TRY(launch_leaf_relpos(k, (int32_t)c->ncells, sorted_bodies(c), dp<float4>(c->relpos), s));
   }
+  // P2P on the side stream: it needs only the P2P list and the bodies, so it
+  // runs while the host builds the M2L batches (a few host round trips, during
+  // which the main stream would idle); joined before the downward pass.
+  TRY(dev_reserve(c->acc, (size_t)c->n * 32));
+  TRY(hipEventRecord(c->ev[EV_P2PREADY], s));
+  TRY(hipStreamWaitEvent(c->side, c->ev[EV_P2PREADY], 0));
+  TRY(hipEventRecord(c->ev[EV_P2P0], c->side));
+  Csr pl{dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src)};
+  if (mixed)
+    TRY(launch_p2p_fp32(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
+                        dp<double>(c->acc), c->side));
+  else
+    TRY(launch_p2p_fp64(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<double>(c->acc), c->side));
+  TRY(hipEventRecord(c->ev[EV_P2P1], c->side));
+  rc = build_m2l_batches(c);
+  if (rc) {
+    (void)hipStreamWaitEvent(s, c->ev[EV_P2P1], 0);
+    return rc;
+  }
+  // M~ in the MFMA operand layout
+  TRY(dev_reserve(c->mq, (size_t)c->ncells * 112 * (mixed ? 4 : 8)));
+  TRY(launch_m2l_scale_m(k, (int32_t)c->ncells, dp<double>(c->M), c->w0, c->mq.ptr, mixed, s));
   TRY(hipEventRecord(c->ev[EV_DTT1], s));
 
   const size_t lbytes = (size_t)c->ncells * NTERM * 16;
@@ -556,15 +575,7 @@ int do_traverse(tfmm_ctx *c) {
   }
   TRY(hipEventRecord(c->ev[EV_M2L1], s));
 
-  TRY(dev_reserve(c->acc, (size_t)c->n * 32));
-  TRY(hipEventRecord(c->ev[EV_P2P0], s));
-  Csr pl{dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src)};
-  if (mixed)
-    TRY(launch_p2p_fp32(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
-                        dp<double>(c->acc), s));
-  else
-    TRY(launch_p2p_fp64(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<double>(c->acc), s));
-  TRY(hipEventRecord(c->ev[EV_P2P1], s));
+  TRY(hipStreamWaitEvent(s, c->ev[EV_P2P1], 0));
   TRY(hipMemcpyAsync(c->h_u64, static_cast<char *>(c->cnt.ptr) + 32, 16, hipMemcpyDeviceToHost, s));
   return TFMM_OK;
 }
@@ -609,6 +620,10 @@ int tfmm_create(int hip_device, tfmm_ctx **out) {
     delete c;
     return TFMM_EHIP;
   }
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) {
+    tfmm_destroy(c);
+    return TFMM_EHIP;
+  }
   for (int i = 0; i < EV_COUNT; i++) {
     if (hipEventCreate(&c->ev[i]) != hipSuccess) {
       tfmm_destroy(c);
@@ -628,6 +643,7 @@ void tfmm_destroy(tfmm_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->side) (void)hipStreamSynchronize(c->side);
   DevBuf *bufs[] = {&c->own_x, &c->own_y, &c->own_z, &c->own_q, &c->keys, &c->keys_alt, &c->idx,
                     &c->idx_alt, &c->sort_tmp, &c->sx, &c->sy, &c->sz, &c->sq, &c->perm,
                     &c->region, &c->bbox_part, &c->c_key, &c->c_beg, &c->c_nb, &c->c_parent,
@@ -646,6 +662,7 @@ void tfmm_destroy(tfmm_ctx *c) {
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   if (c->h_u32) (void)hipHostFree(c->h_u32);
   if (c->h_u64) (void)hipHostFree(c->h_u64);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
