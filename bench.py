@@ -45,13 +45,14 @@ def parse():
     ap.add_argument("--ncrit", type=int, default=64)
     ap.add_argument("--theta", type=float, default=0.4)
     ap.add_argument("--targets", type=int, default=4096)
-    ap.add_argument("--cpu-sample", type=int, default=131072)
+    ap.add_argument("--cpu-sample", type=int, default=524288)
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
 
 
-def cpu_baseline(args):
-    """Reference pipeline (oracle/_ref/tapas_ref) on a bounded sample."""
+def cpu_baseline(args, ncrit, theta):
+    """Reference pipeline (oracle/_ref/tapas_ref) on a bounded sample, at the
+    headline's ncrit and theta."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     import datasets
@@ -61,17 +62,35 @@ def cpu_baseline(args):
     n = args.cpu_sample
     x, y, z, q = datasets.make(args.dist, n, seed=1)
     t0 = time.time()
-    r = oracle.run_ref(x, y, z, q, args.ncrit, args.theta, count=False, timeout=600)
+    r = oracle.run_ref(x, y, z, q, ncrit, theta, count=False, timeout=600)
     wall = time.time() - t0
     ph = r["times"]
     t = ph["tree"] + ph["upward"] + ph["traverse"] + ph["downward"]
     return {"value": n / t, "unit": "bodies/s", "cores": 1, "kind": "reference",
-            "sample": "%d-body %s (same generator and parameters), reference TapasFMM "
+            "sample": "%d-body %s (same generator, ncrit %d, theta %.4g), reference TapasFMM "
                       "pipeline, 1 MPI rank; phases tree %.2fs upward %.2fs traverse %.2fs "
                       "downward %.2fs (upward includes the reference's double upward pass "
-                      "on one rank); process wall %.1fs" % (n, args.dist, ph["tree"],
+                      "on one rank); process wall %.1fs" % (n, args.dist, ncrit, theta, ph["tree"],
                                                              ph["upward"], ph["traverse"],
                                                              ph["downward"], wall)}
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from a committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or
+    None when no summary covers this workload."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.txt"))):
+        fields = {}
+        with open(path) as f:
+            for line in f:
+                if ":" in line and not line.startswith(" "):
+                    k, v = line.split(":", 1)
+                    fields[k.strip()] = v.strip()
+        if fields.get("workload") == workload and fields.get("kernel") == kernel:
+            return {"bytes": float(fields["traffic_bytes_per_launch"]),
+                    "source": os.path.relpath(path, ROOT)}
+    return None
 
 
 def time_config(ctx, prm, steps, warmup):
@@ -167,7 +186,11 @@ def main():
                 "theta": args.theta, "rel_l2_pot": e64[0], "rel_l2_acc": e64[1],
                 "phases_ms": ph64, "speedup_of_headline": ms64 / ms_step}
 
-    cpu = None if args.no_cpu else cpu_baseline(args)
+    cpu = None if args.no_cpu else cpu_baseline(args, ncrit, theta)
+    workload = "%s N=%d P=10 ncrit=%d theta=%g %s%s" % (
+        args.dist, args.n, ncrit, round(theta, 4), args.precision, " (learned)" if picked else "")
+    tile_kernel = "k_m2l_tiles<%s>" % ("float" if precision == tfmm.MIXED else "double")
+    traffic = measured_traffic(workload, tile_kernel)
     out = {
         "metric": "FMM bodies/s at rel-L2 err <=1e-4, 4M Plummer; P2P % FP32 peak",
         "value": args.n / (ms_step * 1e-3),
@@ -181,9 +204,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if precision == tfmm.FP64 else "f32 P2P/M2L + f64 tree/P2M/M2M/L2L/L2P",
         "data": "synthetic",
-        "config": {"workload": "%s N=%d P=10 ncrit=%d theta=%g %s%s" % (
-            args.dist, args.n, ncrit, round(theta, 4), args.precision,
-            " (learned)" if picked else ""),
+        "config": {"workload": workload,
             "n_bodies": args.n, "distribution": args.dist, "expansion": 10,
             "ncrit": ncrit, "theta": theta, "precision": args.precision},
         "error": {"rel_l2_pot": err_pot, "rel_l2_acc": err_acc, "targets": len(targets),
@@ -194,10 +215,11 @@ def main():
                    "m2l_pairs": st["n_m2l_exec"], "m2l_groups": st["n_m2l_groups"],
                    "m2l_items": st["n_m2l_items"], "m2l_fallback": st["n_m2l_fallback"],
                    "p2p_leaf_pairs": st["n_p2p_leafpairs"], "p2p_body_pairs": pairs},
-        "roofline": {"kernel": "k_m2l_tiles<%s>" % ("float" if precision == tfmm.MIXED
-                                                       else "double"),
+        "roofline": {"kernel": tile_kernel,
                      "bound": "mfma", "achieved": m2l_tflops, "peak": peak, "unit": "TFLOP/s",
-                     "frac": m2l_tflops / peak, "traffic": None,
+                     "frac": m2l_tflops / peak,
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
                      "flops_per_pair": M2L_FLOPS_PER_PAIR, "pairs_per_launch": grouped,
                      "ms_per_launch": tiles_ms,
                      "p2p": {"tflops": p2p_tflops, "flops_per_pair": P2P_FLOPS_PER_PAIR,
