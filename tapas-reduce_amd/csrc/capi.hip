@@ -64,11 +64,7 @@ struct tfmm_ctx {
   int64_t ncells = 0, nleaves = 0;
   bool keys_in_alt = false;
   double h_region[6] = {0, 0, 0, 0, 0, 0};
-  // translation operators: valid for this root region and number of levels
-  void *tops_ptr = nullptr;
-  size_t tops_cap = 0;
-  int tops_levels = 0;
-  double tops_region[6] = {0, 0, 0, 0, 0, 0};
+  bool tops_ready = false;  // unit translation operators built
   double w0 = 0;  // root width
 
   // expansions
@@ -281,26 +277,15 @@ int do_upward(tfmm_ctx *c) {
   const Cells k = cells_of(c);
   TRY(launch_p2m(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->M), s));
   const int nl = (int)c->lvl.size() - 1;
-  // translation operators of every level with children (used again by L2L)
-  // depend only on the root region and the number of levels: rebuilt only
-  // when either changed since the last evaluation
-  const size_t opsz = translation_ops_doubles();
-  TRY(dev_reserve(c->tops, (size_t)std::max(nl - 1, 1) * opsz * 8));
-  TRY(hipStreamSynchronize(s));  // h_region of this tree
-  const bool same = c->tops_ptr == c->tops.ptr && c->tops_cap == c->tops.cap &&
-                    c->tops_levels >= nl &&
-                    memcmp(c->tops_region, c->h_region, sizeof(c->tops_region)) == 0;
-  if (!same) {
-    for (int d = 0; d + 1 < nl; d++)
-      TRY(launch_translation_ops(k, c->lvl[d + 1], dp<double>(c->tops) + (size_t)d * opsz, s));
-    c->tops_ptr = c->tops.ptr;
-    c->tops_cap = c->tops.cap;
-    c->tops_levels = nl;
-    memcpy(c->tops_region, c->h_region, sizeof(c->tops_region));
+  // the unit translation operators, shared by every level (built once)
+  if (!c->tops_ready) {
+    TRY(dev_reserve(c->tops, translation_ops_doubles() * 8));
+    TRY(launch_translation_ops(dp<double>(c->tops), s));
+    c->tops_ready = true;
   }
   for (int d = nl - 2; d >= 0; d--)
-    TRY(launch_m2m(k, c->lvl[d + 1], (c->lvl[d + 2] - c->lvl[d + 1]) / 8,
-                   dp<double>(c->tops) + (size_t)d * opsz, dp<double>(c->M), s));
+    TRY(launch_m2m(k, c->lvl[d + 1], (c->lvl[d + 2] - c->lvl[d + 1]) / 8, dp<double>(c->tops),
+                   dp<double>(c->M), s));
   return TFMM_OK;
 }
 
@@ -433,12 +418,7 @@ int build_m2l_batches(tfmm_ctx *c) {
   TRY(dev_reserve(c->g_y32, ngr * 112 * 4));
   TRY(dev_reserve(c->g_dl, ngr));
   TRY(dev_reserve(c->g_lt, ngr));
-  double w0 = 0;
-  TRY(hipMemcpyAsync(&c->h_u64[2], dp<double>(c->c_width), 8, hipMemcpyDeviceToHost, s));
-  TRY(hipStreamSynchronize(s));
-  memcpy(&w0, &c->h_u64[2], 8);
-  c->w0 = w0;
-  TRY(launch_m2l_group_harmonics(dp<uint32_t>(c->g_used), dp<uint32_t>(c->g_gid), nslots, w0,
+  TRY(launch_m2l_group_harmonics(dp<uint32_t>(c->g_used), dp<uint32_t>(c->g_gid), nslots,
                                  dp<double>(c->g_y64), dp<float>(c->g_y32), dp<int8_t>(c->g_dl),
                                  dp<int8_t>(c->g_lt), s));
   // split grouped / fallback pairs
@@ -554,7 +534,7 @@ int do_traverse(tfmm_ctx *c) {
   }
   // M~ in the MFMA operand layout
   TRY(dev_reserve(c->mq, (size_t)c->ncells * 112 * (mixed ? 4 : 8)));
-  TRY(launch_m2l_scale_m(k, (int32_t)c->ncells, dp<double>(c->M), c->w0, c->mq.ptr, mixed, s));
+  TRY(launch_m2l_scale_m(k, (int32_t)c->ncells, dp<double>(c->M), c->mq.ptr, mixed, s));
   TRY(hipEventRecord(c->ev[EV_DTT1], s));
 
   const size_t lbytes = (size_t)c->ncells * NTERM * 16;
@@ -565,7 +545,7 @@ int do_traverse(tfmm_ctx *c) {
   TRY(launch_m2l_tiles(dp<int32_t>(c->tile_off), ntiles, dp<uint2>(c->cols),
                        mixed ? (const void *)c->g_y32.ptr : (const void *)c->g_y64.ptr,
                        c->mq.ptr, dp<uint32_t>(c->codes), k,
-                       (int32_t)c->ncells, c->w0, dp<double>(c->L), mixed, s));
+                       (int32_t)c->ncells, dp<double>(c->L), mixed, s));
   TRY(hipEventRecord(c->ev[EV_M2LT], s));
   if (c->n_fallback) {
     Csr fl{dp<int32_t>(c->fb_off), dp<int32_t>(c->fb_src)};
@@ -584,10 +564,9 @@ int do_downward(tfmm_ctx *c) {
   hipStream_t s = c->stream;
   const Cells k = cells_of(c);
   const int nl = (int)c->lvl.size() - 1;
-  const size_t opsz = translation_ops_doubles();
   for (int d = 0; d + 1 < nl; d++)
-    TRY(launch_l2l(k, c->lvl[d + 1], (c->lvl[d + 2] - c->lvl[d + 1]) / 8,
-                   dp<double>(c->tops) + (size_t)d * opsz, dp<double>(c->L), s));
+    TRY(launch_l2l(k, c->lvl[d + 1], (c->lvl[d + 2] - c->lvl[d + 1]) / 8, dp<double>(c->tops),
+                   dp<double>(c->L), s));
   TRY(dev_reserve(c->trg, (size_t)c->n * 32));
   TRY(dev_reserve(c->out, (size_t)c->n * 32));
   TRY(dev_reserve(c->leaf_of, (size_t)c->n * 4));

@@ -15,11 +15,12 @@
 // group's pairs become the columns of a matrix product T * [M_s1 ... M_s16].
 // The harmonics of each group are formed once, in FP64.
 //
-// Scaling: with sigma_l = w_0 2^-l (w_0 the root width), the kernel works on
+// Scaling: with sigma the width of each cell, the kernel works on
 //   M~_n = M_n / sigma_s^n,  L~_j = L_j sigma_t^(j+1),  Y~_p = Y_p sigma_t^(p+1)
 // so that L~_j = sum_n 2^(n (l_t - l_s)) Y~_(j+n) M~_n with all quantities
 // O(1): FP32 neither under- nor overflows at P = 10 on deep trees, and the
-// power-of-two factors are exact.
+// power-of-two factors are exact.  Y~ depends only on the offset in units of
+// the cells' widths, so one group serves every tree of a batch.
 //
 // Work decomposition: targets are tiled by 128 consecutive cells (breadth
 // first, so one tile is one Morton range of one level); the pairs of a tile
@@ -184,22 +185,23 @@ __global__ __launch_bounds__(1024) void k_classify(const int2 *pairs, const int3
 }
 
 // Harmonics of every used group, FP64, scaled: Y~_p^q = Y_p^q sigma_t^(p+1)
-// 2^(p dl), dl = l_t - l_s.  Stored as NR reals (re/im interleaved, index
-// 2 tri(p, q)), in FP64 and FP32.
+// 2^(p dl), dl = l_t - l_s, i.e. the harmonics of the offset measured in
+// target widths.  Stored as NR reals (re/im interleaved, index 2 tri(p, q)),
+// in FP64 and FP32.
 __global__ void k_group_harmonics(const uint32_t *used, const uint32_t *gid_of_slot,
-                                  int32_t nslots, double w0, double *Y64, float *Y32,
-                                  int8_t *gdl, int8_t *glt) {
+                                  int32_t nslots, double *Y64, float *Y32, int8_t *gdl,
+                                  int8_t *glt) {
   const int32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= nslots || !used[slot]) return;
   const uint32_t g = gid_of_slot[slot];
   int lt, ls, o[3];
   slot_decode(slot, lt, ls, o);
   const int lf = lt > ls ? lt : ls;
-  const double half = ldexp(w0, -lf) * 0.5;
+  const double half = ldexp(0.5, lt - lf);  // half the finer width, target width = 1
   const double dx = o[0] * half, dy = o[1] * half, dz = o[2] * half;
   double rho, alpha, beta;
   cart2sph(dx, dy, dz, rho, alpha, beta);
-  const double sig = ldexp(w0, -lt);
+  const double sig = 1.0;
   const double x = cos(alpha), y = sin(alpha);
   const cd ei = mk(cos(beta), sin(beta));
   double *yo = Y64 + (int64_t)g * NR;
@@ -224,7 +226,7 @@ __global__ void k_group_harmonics(const uint32_t *used, const uint32_t *gid_of_s
 // M~ of every cell, in the quad-interleaved layout the MFMA B operand reads:
 // Mq[cell][q][t] = M~[cell][4 t + q], row r = 2 tri(n, m) + (0 re | 1 im).
 template <class T>
-__global__ void k_scale_m(Cells c, int32_t ncells, const double *M, double w0, T *Mq) {
+__global__ void k_scale_m(Cells c, int32_t ncells, const double *M, T *Mq) {
   const int32_t ci = blockIdx.x;
   if (ci >= ncells) return;
   const int r = threadIdx.x;  // 0..111
@@ -235,7 +237,8 @@ __global__ void k_scale_m(Cells c, int32_t ncells, const double *M, double w0, T
     const int tt = r >> 1;
     int n = 0;
     while (tri(n + 1, 0) <= tt) n++;
-    const double isig = 1.0 / ldexp(w0, -c.depth[ci]);
+    const double w = c.width[ci];
+    const double isig = w > 0 ? 1.0 / w : 0.0;  // zero width: only degree 0 is non-zero
     double sc = 1;
     for (int i = 0; i < n; i++) sc *= isig;
     v = M[2 * ((int64_t)ci * NTERM + tt) + (r & 1)] * sc;
@@ -569,7 +572,7 @@ __device__ inline int tile_of_block(int b, int ntiles) {
 template <class T>
 __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
     const int32_t *tile_off, const uint2 *cols, const T *Yg, const T *Mq,
-    const uint32_t *codes, Cells c, int32_t ncells, double w0, double *L) {
+    const uint32_t *codes, Cells c, int32_t ncells, double *L) {
   __shared__ T Lt[TILE][NR + 1];
   __shared__ T Lt0b[TILE][17];
   __shared__ T Tb[TILE_WAVES][NMULT * NR];
@@ -603,7 +606,7 @@ __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
     while (tri(jj + 1, 0) <= jk) jj++;
     const double v = (double)Lt[sl][row];
     if (v == 0) continue;  // nothing arrived (and a zero-width tree must not form 0 * inf)
-    const double isig = 1.0 / ldexp(w0, -c.depth[ci]);
+    const double isig = 1.0 / c.width[ci];
     double sc = isig;
     for (int i = 0; i < jj; i++) sc *= isig;
     L[2 * ((int64_t)ci * NTERM + jk) + (row & 1)] += v * sc;
@@ -649,19 +652,19 @@ hipError_t launch_m2l_classify(const int2 *pairs, const int32_t *slot_of, uint32
 }
 
 hipError_t launch_m2l_group_harmonics(const uint32_t *used, const uint32_t *gid_of_slot,
-                                      int32_t nslots, double w0, double *Y64, float *Y32,
-                                      int8_t *gdl, int8_t *glt, hipStream_t s) {
+                                      int32_t nslots, double *Y64, float *Y32, int8_t *gdl,
+                                      int8_t *glt, hipStream_t s) {
   if (nslots <= 0) return hipSuccess;
-  k_group_harmonics<<<(nslots + 127) / 128, 128, 0, s>>>(used, gid_of_slot, nslots, w0, Y64,
-                                                         Y32, gdl, glt);
+  k_group_harmonics<<<(nslots + 127) / 128, 128, 0, s>>>(used, gid_of_slot, nslots, Y64, Y32,
+                                                         gdl, glt);
   return hipGetLastError();
 }
 
-hipError_t launch_m2l_scale_m(Cells c, int32_t ncells, const double *M, double w0, void *Mq,
-                              bool fp32, hipStream_t s) {
+hipError_t launch_m2l_scale_m(Cells c, int32_t ncells, const double *M, void *Mq, bool fp32,
+                              hipStream_t s) {
   if (ncells <= 0) return hipSuccess;
-  if (fp32) k_scale_m<float><<<ncells, NR, 0, s>>>(c, ncells, M, w0, (float *)Mq);
-  else k_scale_m<double><<<ncells, NR, 0, s>>>(c, ncells, M, w0, (double *)Mq);
+  if (fp32) k_scale_m<float><<<ncells, NR, 0, s>>>(c, ncells, M, (float *)Mq);
+  else k_scale_m<double><<<ncells, NR, 0, s>>>(c, ncells, M, (double *)Mq);
   return hipGetLastError();
 }
 
@@ -717,17 +720,16 @@ hipError_t launch_m2l_tile_offsets(const int4 *items, uint32_t nitems, int32_t n
 
 hipError_t launch_m2l_tiles(const int32_t *tile_off, int32_t ntiles, const uint2 *cols,
                             const void *Yg, const void *Mq,
-                            const uint32_t *codes, Cells c, int32_t ncells, double w0, double *L,
-                            bool fp32, hipStream_t s) {
+                            const uint32_t *codes, Cells c, int32_t ncells, double *L, bool fp32,
+                            hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
   if (fp32)
     k_m2l_tiles<float><<<ntiles, 64 * TILE_WAVES, 0, s>>>(tile_off, cols, (const float *)Yg,
-                                                          (const float *)Mq, codes, c, ncells,
-                                                          w0, L);
+                                                          (const float *)Mq, codes, c, ncells, L);
   else
     k_m2l_tiles<double><<<ntiles, 64 * TILE_WAVES, 0, s>>>(tile_off, cols, (const double *)Yg,
                                                            (const double *)Mq,
-                                                           codes + NCODE, c, ncells, w0, L);
+                                                           codes + NCODE, c, ncells, L);
   return hipGetLastError();
 }
 

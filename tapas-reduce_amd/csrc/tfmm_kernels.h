@@ -50,8 +50,7 @@ hipError_t launch_children(Cells c, int32_t lbeg, int32_t lcnt, const uint32_t *
 hipError_t launch_p2m(Cells c, int32_t ncells, Bodies b, double *M, hipStream_t s);
 // translate.hip (FP64 M2M / L2L on MFMA)
 size_t translation_ops_doubles();
-hipError_t launch_translation_ops(Cells c, int32_t first_child_cell, double *ops,
-                                  hipStream_t s);
+hipError_t launch_translation_ops(double *ops, hipStream_t s);
 hipError_t launch_m2m(Cells c, int32_t first_child, int32_t npar, const double *ops, double *M,
                       hipStream_t s);
 hipError_t launch_l2l(Cells c, int32_t first_child, int32_t npar, const double *ops, double *L,
@@ -105,10 +104,10 @@ hipError_t launch_m2l_classify(const int2 *pairs, const int32_t *slot_of, uint32
                                uint32_t *gkeys, uint32_t *gvals, uint64_t *fkeys,
                                uint32_t *fvals, uint32_t *counters, hipStream_t s);
 hipError_t launch_m2l_group_harmonics(const uint32_t *used, const uint32_t *gid_of_slot,
-                                      int32_t nslots, double w0, double *Y64, float *Y32,
-                                      int8_t *gdl, int8_t *glt, hipStream_t s);
-hipError_t launch_m2l_scale_m(Cells c, int32_t ncells, const double *M, double w0, void *Mq,
-                              bool fp32, hipStream_t s);
+                                      int32_t nslots, double *Y64, float *Y32, int8_t *gdl,
+                                      int8_t *glt, hipStream_t s);
+hipError_t launch_m2l_scale_m(Cells c, int32_t ncells, const double *M, void *Mq, bool fp32,
+                              hipStream_t s);
 hipError_t launch_m2l_count_items(const uint32_t *keys, uint32_t n, uint32_t *flag,
                                   uint32_t *excl, uint32_t *segstart, uint32_t *iflag,
                                   uint32_t *iexcl, void *scan_tmp, uint32_t *h_pinned,
@@ -122,8 +121,8 @@ hipError_t launch_m2l_tile_offsets(const int4 *items, uint32_t nitems, int32_t n
                                    int32_t *off, hipStream_t s);
 hipError_t launch_m2l_tiles(const int32_t *tile_off, int32_t ntiles, const uint2 *cols,
                             const void *Yg, const void *Mq,
-                            const uint32_t *codes, Cells c, int32_t ncells, double w0, double *L,
-                            bool fp32, hipStream_t s);
+                            const uint32_t *codes, Cells c, int32_t ncells, double *L, bool fp32,
+                            hipStream_t s);
 
 // p2p.hip (mixed precision)
 hipError_t launch_leaf_relpos(Cells c, int32_t ncells, Bodies b, float4 *rel, hipStream_t s);

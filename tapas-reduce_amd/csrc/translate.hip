@@ -6,16 +6,20 @@
 //   L2L  LaplaceSphericalCPU_tapas.h:309-345  (dX = child - parent centre)
 //   called per cell by FMM_Upward / FMM_Downward, tapas_exafmm.cxx:87-140.
 // Both are real-linear maps of 55 complex (110 real) coefficients whose
-// entries depend only on dX, and within one level dX takes 8 values, one
-// per child octant (the cells of a level share their width).  So per level:
-//   M2M:  M_p (110)   += [T_0 .. T_7] (110 x 880) . [M_c0; ..; M_c7] (880)
-//   L2L:  L_c_o (110) += S_o (110 x 110) . L_p (110),   o = 0..7
-// The 8 children of a parent are adjacent cells, so the stacked children
-// coefficients of one parent are one contiguous column of 880 doubles.
-// The operators are built each evaluation by applying the reference's
-// formulas to unit coefficient vectors, with dX taken from the first parent
-// of the level (the centres of other parents differ from it by rounding
-// only), and the products run on v_mfma_f64_16x16x4 in FP64.
+// entries are regular harmonics of dX: the (j, n) entry of M2M scales as
+// |dX|^(j-n), that of L2L as |dX|^(n-j).  The 8 children of a parent sit at
+// dX = sigma_p u_o (sigma_p the parent's width, u_o = (+-1/4, +-1/4, +-1/4) for
+// octant o), so with operators T_o = T(u_o), S_o = S(u_o) built once:
+//   M2M:  M_p[j]   += sigma_p^j    sum_o sum_n T_o[j][n] sigma_p^-n M_c(o)[n]
+//   L2L:  L_c(o)[j] += sigma_p^-j  sum_n S_o[j][n] sigma_p^n L_p[n]
+// i.e. one product per level, [T_0 .. T_7] (110 x 880) against the stacked
+// children coefficients (the 8 children of a parent are adjacent cells, one
+// contiguous column of 880 doubles), with per-parent column and row factors.
+// Parents of any width (different levels, or different problems of a batch)
+// share the operators.  The operators are formed by applying the reference's
+// formulas to unit coefficient vectors; the products run on
+// v_mfma_f64_16x16x4 in FP64.  (dX of the reference comes from the cell
+// centres, which equal sigma_p u_o up to rounding.)
 #include "tfmm_device.h"
 #include "tfmm_kernels.h"
 
@@ -74,26 +78,38 @@ __device__ inline cd l2l_entry(int j, int k, const cd *Lp, const cd *Y) {
   return acc;
 }
 
-// Operators of one level.  Block (column, octant, kind): the image of unit
-// input vector `column` under M2M (kind 0) or L2L (kind 1) of the octant.
+__host__ __device__ constexpr int deg_of_coef(int t) {
+  int n = 0;
+  while ((n + 1) * (n + 2) / 2 <= t) n++;
+  return n;
+}
+__host__ __device__ constexpr int m2m_kend(int rb) {
+  const int last = (16 * rb + 15 < NV - 1 ? 16 * rb + 15 : NV - 1) >> 1;
+  const int d = deg_of_coef(last);
+  return (d + 1) * (d + 2) < NV ? (d + 1) * (d + 2) : NV;
+}
+__host__ __device__ constexpr int l2l_kbeg(int rb) {
+  const int d = deg_of_coef((16 * rb) >> 1);
+  return (d * (d + 1)) & ~3;
+}
+
+// The unit operators.  Block (column, octant, kind): the image of unit input
+// vector `column` under M2M (kind 0) or L2L (kind 1) of the octant.
 //   Tm: [110 rows][880 = octant * 110 + child column]
 //   Tl: [octant][110 rows][110 parent columns]
-__global__ __launch_bounds__(64) void k_build_ops(Cells c, int32_t first_child_cell, double *Tm,
-                                                  double *Tl) {
+__global__ __launch_bounds__(64) void k_build_ops(double *Tm, double *Tl) {
   const int col = blockIdx.x, oct = blockIdx.y, kind = blockIdx.z;
   const int lane = threadIdx.x;
   __shared__ cd Y[NTERM];
   __shared__ cd U[NTERM];
-  const int32_t p = c.parent[first_child_cell];
-  const int32_t ch = c.child[p] + oct;
   if (lane == 0) {
-    const double sg = kind == 0 ? 1.0 : -1.0;  // M2M: parent - child; L2L: child - parent
-    const double dx = sg * (c.center[3 * (int64_t)p] - c.center[3 * (int64_t)ch]);
-    const double dy = sg * (c.center[3 * (int64_t)p + 1] - c.center[3 * (int64_t)ch + 1]);
-    const double dz = sg * (c.center[3 * (int64_t)p + 2] - c.center[3 * (int64_t)ch + 2]);
+    // child centre - parent centre for a parent of width 1
+    const double u[3] = {(oct & 1) ? 0.25 : -0.25, (oct & 2) ? 0.25 : -0.25,
+                         (oct & 4) ? 0.25 : -0.25};
+    const double sg = kind == 0 ? -1.0 : 1.0;  // M2M: parent - child; L2L: child - parent
     double rho, ca, sa;
     cd ei;
-    sph_frame(dx, dy, dz, rho, ca, sa, ei);
+    sph_frame(sg * u[0], sg * u[1], sg * u[2], rho, ca, sa, ei);
     multipole_all<false>(rho, ca, sa, ei, [&](int n, int m, cd v, cd) { Y[tri(n, m)] = v; });
   }
   if (lane < NTERM) U[lane] = mk(lane == (col >> 1) && !(col & 1) ? 1.0 : 0.0,
@@ -113,6 +129,27 @@ __global__ __launch_bounds__(64) void k_build_ops(Cells c, int32_t first_child_c
   }
 }
 
+// per-parent factors sigma^(e * degree(k)) for k = 0..111 of the 16 parents
+// of a block (e = +1 or -1), into LDS.  A parent of zero width (all its
+// bodies at one point) keeps only the degree-0 terms: its children sit at
+// dX = 0, where the shifts reduce to the identity on coefficients that are
+// zero above degree 0 anyway.
+__device__ inline void parent_powers(double (*f)[NV + 2], const double *sig, int e) {
+  for (int t = threadIdx.x; t < 16 * (NV + 2); t += blockDim.x) {
+    const int colp = t / (NV + 2), kk = t % (NV + 2);
+    const int d = kk < NV ? deg_of_coef(kk >> 1) : 0;
+    const double sg = sig[colp];
+    double v = 1;
+    if (sg == 0) {
+      v = d == 0 ? 1.0 : 0.0;
+    } else {
+      const double s = e > 0 ? sg : 1.0 / sg;
+      for (int i = 0; i < d; i++) v *= s;
+    }
+    f[colp][kk] = v;
+  }
+}
+
 // MFMA f64 16x16x4 operands: lane l supplies A[l % 16][l / 16] and
 // B[l / 16][l % 16]; result i of lane l is D[l / 16 + 4 i][l % 16].
 //
@@ -120,21 +157,6 @@ __global__ __launch_bounds__(64) void k_build_ops(Cells c, int32_t first_child_c
 // rb only takes child columns below m2m_kend(rb) (child degree <= parent
 // degree), L2L row block rb only parent columns from l2l_kbeg(rb) on
 // (parent degree >= child degree; rounded down to the k-step of 4).
-__host__ __device__ constexpr int deg_of_coef(int t) {
-  int n = 0;
-  while ((n + 1) * (n + 2) / 2 <= t) n++;
-  return n;
-}
-__host__ __device__ constexpr int m2m_kend(int rb) {
-  const int last = (16 * rb + 15 < NV - 1 ? 16 * rb + 15 : NV - 1) >> 1;
-  const int d = deg_of_coef(last);
-  return (d + 1) * (d + 2) < NV ? (d + 1) * (d + 2) : NV;
-}
-__host__ __device__ constexpr int l2l_kbeg(int rb) {
-  const int d = deg_of_coef((16 * rb) >> 1);
-  return (d * (d + 1)) & ~3;
-}
-
 // M2M of one level: columns are the level's parents with children, in the
 // order of their children (the t-th such parent has children at
 // first_child + 8 t); block = 16 parents, wave rb = row block rb.
@@ -142,6 +164,18 @@ __global__ __launch_bounds__(7 * 64) void k_m2m_mfma(Cells c, int32_t first_chil
                                                      const double *Tm, double *M) {
   const int lane = threadIdx.x & 63, rb = threadIdx.x >> 6;
   const int col = lane & 15, kq = lane >> 4;
+  __shared__ double sig[16];
+  __shared__ double cf[16][NV + 2];  // sigma_p^-n of the B rows (child coefficients)
+  __shared__ double rf[16][NV + 2];  // sigma_p^j of the output rows
+  if (threadIdx.x < 16) {
+    const int32_t tt = (int32_t)blockIdx.x * 16 + threadIdx.x;
+    const int32_t cc = first_child + 8 * (tt < npar ? tt : 0);
+    sig[threadIdx.x] = c.width[c.parent[cc]];
+  }
+  __syncthreads();
+  parent_powers(cf, sig, -1);
+  parent_powers(rf, sig, +1);
+  __syncthreads();
   const int32_t t = (int32_t)blockIdx.x * 16 + col;
   const bool valid = t < npar;
   const int32_t c0 = first_child + 8 * (valid ? t : 0);
@@ -163,11 +197,12 @@ __global__ __launch_bounds__(7 * 64) void k_m2m_mfma(Cells c, int32_t first_chil
   for (int k0 = 0; k0 < kend; k0 += 4) {
     const int kc = k0 + kq;
     const bool in = kc < kend;
+    const double f = cf[col][kc];
     double a[8], b[8];
 #pragma unroll
     for (int o = 0; o < 8; o++) {
       a[o] = (in && arow_ok) ? A[o * NV + kc] : 0.0;
-      b[o] = in ? B[o * NV + kc] : 0.0;
+      b[o] = in ? B[o * NV + kc] * f : 0.0;
     }
 #pragma unroll
     for (int o = 0; o < 8; o++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[o], b[o], acc, 0, 0, 0);
@@ -177,7 +212,7 @@ __global__ __launch_bounds__(7 * 64) void k_m2m_mfma(Cells c, int32_t first_chil
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int r = rb * 16 + kq + 4 * i;
-    if (r < NV) M[(int64_t)p * NV + r] += acc[i];
+    if (r < NV) M[(int64_t)p * NV + r] += acc[i] * rf[col][r];
   }
 }
 
@@ -188,6 +223,18 @@ __global__ __launch_bounds__(8 * 64) void k_l2l_mfma(Cells c, int32_t first_chil
                                                      const double *Tl, double *L) {
   const int lane = threadIdx.x & 63, oct = threadIdx.x >> 6;
   const int col = lane & 15, kq = lane >> 4;
+  __shared__ double sig[16];
+  __shared__ double cf[16][NV + 2];  // sigma_p^n of the B rows (parent coefficients)
+  __shared__ double rf[16][NV + 2];  // sigma_p^-j of the output rows
+  if (threadIdx.x < 16) {
+    const int32_t tt = (int32_t)blockIdx.x * 16 + threadIdx.x;
+    const int32_t cc = first_child + 8 * (tt < npar ? tt : 0);
+    sig[threadIdx.x] = c.width[c.parent[cc]];
+  }
+  __syncthreads();
+  parent_powers(cf, sig, +1);
+  parent_powers(rf, sig, -1);
+  __syncthreads();
   const int32_t t = (int32_t)blockIdx.x * 16 + col;
   const bool valid = t < npar;
   const int32_t c0 = first_child + 8 * (valid ? t : 0);
@@ -200,7 +247,7 @@ __global__ __launch_bounds__(8 * 64) void k_l2l_mfma(Cells c, int32_t first_chil
   for (int k0 = 0; k0 < NV; k0 += 4) {
     const int kc = k0 + kq;
     const bool in = kc < NV;
-    const double b = in ? B[kc] : 0.0;
+    const double b = in ? B[kc] * cf[col][kc] : 0.0;
     double a[7];
 #pragma unroll
     for (int rb = 0; rb < 7; rb++) {
@@ -218,7 +265,7 @@ __global__ __launch_bounds__(8 * 64) void k_l2l_mfma(Cells c, int32_t first_chil
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const int r = rb * 16 + kq + 4 * i;
-      if (r < NV) L[cidx * NV + r] += acc[rb][i];
+      if (r < NV) L[cidx * NV + r] += acc[rb][i] * rf[col][r];
     }
   }
 }
@@ -227,11 +274,10 @@ __global__ __launch_bounds__(8 * 64) void k_l2l_mfma(Cells c, int32_t first_chil
 
 size_t translation_ops_doubles() { return (size_t)NV * KM + (size_t)8 * NV * NV; }
 
-// Operators of the level whose children start at cell first_child_cell;
-// ops: translation_ops_doubles() doubles (M2M operator, then L2L operators).
-hipError_t launch_translation_ops(Cells c, int32_t first_child_cell, double *ops,
-                                  hipStream_t s) {
-  k_build_ops<<<dim3(NV, 8, 2), 64, 0, s>>>(c, first_child_cell, ops, ops + (size_t)NV * KM);
+// The unit operators (built once per context); ops: translation_ops_doubles()
+// doubles (M2M operator, then the 8 L2L operators).
+hipError_t launch_translation_ops(double *ops, hipStream_t s) {
+  k_build_ops<<<dim3(NV, 8, 2), 64, 0, s>>>(ops, ops + (size_t)NV * KM);
   return hipGetLastError();
 }
 
