@@ -30,6 +30,10 @@
  *                        the -DCOUNT numP2P/numM2L counters
  *                        (tapas_exafmm.cxx:651-677, :39-50)
  *   tfmm_direct          CheckResult's direct sum (tapas_exafmm.cxx:288-383)
+ *   tfmm_set_batch*,     no single counterpart: what the reference does by
+ *   tfmm_batch_evaluate  running Partition + Map(...) once per problem
+ *                        (tapas_exafmm.cxx:560-627 in a loop), done as one
+ *                        forest in one set of launches
  */
 #ifndef TFMM_H_
 #define TFMM_H_
@@ -49,6 +53,8 @@ extern "C" {
 
 #define TFMM_P 10      /* expansion order (EXPANSION=10) */
 #define TFMM_NTERM 55  /* P*(P+1)/2 coefficients per cell */
+
+#define TFMM_MAX_PROBLEMS 65536 /* problems in one batch */
 
 #define TFMM_FP64 0  /* FP64 throughout */
 #define TFMM_MIXED 1 /* FP32 P2P with compensated sums, FP32 batched M2L; FP64 tree/L2L/L2P */
@@ -97,6 +103,9 @@ typedef struct {
   uint64_t n_m2l_groups, n_m2l_items, n_m2l_fallback;
   /* the batched M2L tile kernel alone (its one launch, inside ms_m2l) */
   double ms_m2l_tiles;
+  /* problems in the forest (1 unless the bodies were set as a batch);
+   * region_min/max above are those of problem 0 */
+  int64_t n_problems;
 } tfmm_stats_t;
 
 int tfmm_create(int hip_device, tfmm_ctx **out);
@@ -112,6 +121,26 @@ int tfmm_set_bodies(tfmm_ctx *ctx, const void *host_bodies, const tfmm_body_desc
  * arrays must stay valid until the next set_bodies call or destroy. */
 int tfmm_set_bodies_device(tfmm_ctx *ctx, int64_t n, const double *x, const double *y,
                            const double *z, const double *q);
+
+/* A batch of nprob independent problems evaluated as one forest: problem p
+ * has its own bounding box (with the single-problem rules), its own tree
+ * and no interaction with any other problem; every phase runs over all
+ * problems in the same launches.  Bodies are concatenated in problem order:
+ * fields, sorted keys, cells and lists are those of the concatenation, and
+ * the cells of problem p descend from root cell p.  tfmm_set_bodies*
+ * returns the context to a single problem. */
+/* copies: problem p = descs[p].n bodies at bodies[p] (AoS as described) */
+int tfmm_set_batch(tfmm_ctx *ctx, int nprob, const tfmm_body_desc *descs,
+                   const void *const *bodies);
+/* device-resident SoA arrays of offsets[nprob] bodies; problem p holds
+ * [offsets[p], offsets[p+1]) (host array, offsets[0] = 0, no empty problem) */
+int tfmm_set_batch_device(tfmm_ctx *ctx, int nprob, const int64_t *offsets, const double *x,
+                          const double *y, const double *z, const double *q);
+/* tfmm_set_batch + tfmm_evaluate */
+int tfmm_batch_evaluate(tfmm_ctx *ctx, int nprob, const tfmm_body_desc *descs,
+                        const void *const *bodies, const tfmm_params *params);
+/* number of problems; offsets (nprob + 1 entries) when not null */
+int tfmm_num_problems(const tfmm_ctx *ctx, int64_t *offsets);
 
 int tfmm_build_tree(tfmm_ctx *ctx, const tfmm_params *params);
 /* Change theta or the precision of the following phases without rebuilding
@@ -145,7 +174,8 @@ int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p
                    int32_t *p2p_src);
 
 /* FP64 direct sum on the current bodies for the given target indices (caller
- * order), normalised by the target charge; out: nt * 4 doubles (host). */
+ * order), normalised by the target charge; out: nt * 4 doubles (host).  In
+ * a batch each target sums over the bodies of its own problem. */
 int tfmm_direct(tfmm_ctx *ctx, const int64_t *targets, int64_t nt, double *out);
 
 /* ---------------------------------------------------------------------------

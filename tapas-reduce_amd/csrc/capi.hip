@@ -47,8 +47,12 @@ struct tfmm_ctx {
   hipStream_t side = nullptr;  // P2P runs here, overlapping the M2L work
   std::string err;
 
-  // bodies (caller order)
+  // bodies (caller order); a batch of nprob problems is one forest, problem p
+  // holding bodies [prob_off[p], prob_off[p+1])
   int64_t n = 0;
+  int nprob = 1;
+  std::vector<int64_t> prob_off;
+  DevBuf d_prob_off, bprob, pkey, pkey_alt, ppos, ppos_alt;
   const double *bx = nullptr, *by = nullptr, *bz = nullptr, *bq = nullptr;
   DevBuf own_x, own_y, own_z, own_q;
 
@@ -65,7 +69,6 @@ struct tfmm_ctx {
   bool keys_in_alt = false;
   double h_region[6] = {0, 0, 0, 0, 0, 0};
   bool tops_ready = false;  // unit translation operators built
-  double w0 = 0;  // root width
 
   // expansions
   DevBuf M, L;
@@ -214,18 +217,46 @@ int do_build_tree(tfmm_ctx *c) {
   TRY(dev_reserve(c->sz, n * 8));
   TRY(dev_reserve(c->sq, n * 8));
   TRY(dev_reserve(c->perm, n * 4));
-  TRY(dev_reserve(c->region, 6 * 8));
-  const int nparts = (int)std::min<int64_t>(1024, (n + 255) / 256);
-  TRY(dev_reserve(c->bbox_part, 2 * 8 * (size_t)nparts));
-
-  TRY(launch_bbox(c->bx, c->by, c->bz, n, dp<double>(c->bbox_part), nparts,
-                  dp<double>(c->region), s));
-  TRY(launch_keys(c->bx, c->by, c->bz, n, dp<double>(c->region), dp<uint64_t>(c->keys),
-                  dp<uint32_t>(c->idx), s));
+  const int np = c->nprob;
+  const bool forest = np > 1;
+  TRY(dev_reserve(c->region, 6 * 8 * (size_t)np));
+  const int64_t *off = forest ? dp<int64_t>(c->d_prob_off) : nullptr;
+  if (forest) {
+    TRY(dev_reserve(c->bprob, n * 4));
+    TRY(launch_bbox_forest(c->bx, c->by, c->bz, off, np, dp<double>(c->region), s));
+  } else {
+    const int nparts = (int)std::min<int64_t>(1024, (n + 255) / 256);
+    TRY(dev_reserve(c->bbox_part, 2 * 8 * (size_t)nparts));
+    TRY(launch_bbox(c->bx, c->by, c->bz, n, dp<double>(c->bbox_part), nparts,
+                    dp<double>(c->region), s));
+  }
+  TRY(launch_keys(c->bx, c->by, c->bz, n, dp<double>(c->region), off, np, dp<uint64_t>(c->keys),
+                  dp<uint32_t>(c->idx), forest ? dp<uint32_t>(c->bprob) : nullptr, s));
   bool alt = false;
   TRY(radix_sort_pairs(dp<uint64_t>(c->keys), dp<uint32_t>(c->idx), dp<uint64_t>(c->keys_alt),
                        dp<uint32_t>(c->idx_alt), n, DEPTHBITS, DEPTHBITS + 3 * MAXDEPTH,
                        c->sort_tmp.ptr, s, &alt));
+  const uint32_t *sprob = nullptr;
+  if (forest) {
+    // (problem, key) order: a stable sort by problem of the key-sorted bodies
+    TRY(dev_reserve(c->pkey, n * 4));
+    TRY(dev_reserve(c->pkey_alt, n * 4));
+    TRY(dev_reserve(c->ppos, n * 4));
+    TRY(dev_reserve(c->ppos_alt, n * 4));
+    const uint64_t *k1 = dp<uint64_t>(alt ? c->keys_alt : c->keys);
+    const uint32_t *o1 = dp<uint32_t>(alt ? c->idx_alt : c->idx);
+    TRY(launch_problem_keys(o1, dp<uint32_t>(c->bprob), n, dp<uint32_t>(c->pkey),
+                            dp<uint32_t>(c->ppos), s));
+    bool palt = false;
+    TRY(radix_sort_pairs32(dp<uint32_t>(c->pkey), dp<uint32_t>(c->ppos), dp<uint32_t>(c->pkey_alt),
+                           dp<uint32_t>(c->ppos_alt), n, 0, bits_for(np - 1), c->sort_tmp.ptr, s,
+                           &palt));
+    sprob = dp<uint32_t>(palt ? c->pkey_alt : c->pkey);
+    TRY(launch_problem_gather(dp<uint32_t>(palt ? c->ppos_alt : c->ppos), k1, o1, n,
+                              dp<uint64_t>(alt ? c->keys : c->keys_alt),
+                              dp<uint32_t>(alt ? c->idx : c->idx_alt), s));
+    alt = !alt;
+  }
   c->keys_in_alt = alt;
   const uint64_t *skeys = dp<uint64_t>(alt ? c->keys_alt : c->keys);
   const uint32_t *order = dp<uint32_t>(alt ? c->idx_alt : c->idx);
@@ -233,10 +264,10 @@ int do_build_tree(tfmm_ctx *c) {
                      dp<double>(c->sz), dp<double>(c->sq), dp<int32_t>(c->perm), s));
 
   // level-synchronous cell build
-  TRY(reserve_cells(c, 4096, 0, s));
-  TRY(launch_root(cells_of(c), n, dp<double>(c->region), s));
+  TRY(reserve_cells(c, std::max<size_t>(4096, 8 * (size_t)np), 0, s));
+  TRY(launch_roots(cells_of(c), n, off, np, dp<double>(c->region), s));
   c->lvl.assign(1, 0);
-  int32_t lbeg = 0, lcnt = 1;
+  int32_t lbeg = 0, lcnt = np;
   int64_t nleaves = 0;
   while (true) {
     TRY(dev_reserve(c->flag, (size_t)lcnt * 4));
@@ -258,7 +289,7 @@ int do_build_tree(tfmm_ctx *c) {
     if (need > INT32_MAX) return fail(c, TFMM_EINVAL, "too many cells");
     TRY(reserve_cells(c, (size_t)need, (size_t)next_beg, s));
     TRY(launch_children(cells_of(c), lbeg, lcnt, dp<uint32_t>(c->flag), dp<uint32_t>(c->slot),
-                        (int32_t)next_beg, skeys, dp<double>(c->region), s));
+                        (int32_t)next_beg, skeys, dp<double>(c->region), sprob, s));
     lbeg = (int32_t)next_beg;
     lcnt = (int32_t)(8 * splits);
   }
@@ -336,11 +367,7 @@ int do_dtt(tfmm_ctx *c) {
     TRY(dev_reserve(c->front_b, (size_t)c->front_cap * 8));
     uint32_t *cnt = dp<uint32_t>(c->cnt);
     TRY(hipMemsetAsync(cnt, 0, (size_t)cwords * 4, s));
-    c->h_u32[8] = 0;  // the root pair (0, 0) and its count, staged in pinned memory
-    c->h_u32[9] = 0;
-    c->h_u32[10] = 1;
-    TRY(hipMemcpyAsync(c->front_a.ptr, c->h_u32 + 8, 8, hipMemcpyHostToDevice, s));
-    TRY(hipMemcpyAsync(cnt + 16, c->h_u32 + 10, 4, hipMemcpyHostToDevice, s));
+    TRY(launch_root_pairs(dp<int2>(c->front_a), c->nprob, cnt + 16, s));
     // grid of step st: the frontier size of the previous traversal of the same
     // tree shape when there is one (checked below), else the capacity
     const bool hint = (int)c->dtt_hint.size() == nsteps + 1 && c->dtt_hint_cells == c->ncells;
@@ -656,28 +683,39 @@ int tfmm_sync(tfmm_ctx *c) {
   return TFMM_OK;
 }
 
-int tfmm_set_bodies(tfmm_ctx *c, const void *host, const tfmm_body_desc *d) {
-  if (!c) return TFMM_EINVAL;
+namespace {
+
+int check_desc(tfmm_ctx *c, const void *host, const tfmm_body_desc *d) {
   if (!d || !host) return fail(c, TFMM_EINVAL, "null bodies or descriptor");
   if (d->n < 1 || d->n > INT32_MAX) return fail(c, TFMM_EINVAL, "n must be in [1, 2^31)");
   if (d->stride < 8 || d->off_x + 24 > d->stride || d->off_q + 8 > d->stride)
     return fail(c, TFMM_EINVAL, "body descriptor offsets exceed the stride");
-  TRY(hipSetDevice(c->device));
-  const int64_t n = d->n;
-  std::vector<double> x(n), y(n), z(n), q(n);
+  return TFMM_OK;
+}
+
+// the caller's array-of-structs bodies into SoA host arrays, from index at
+int unpack_bodies(tfmm_ctx *c, const void *host, const tfmm_body_desc *d, int64_t at,
+                  std::vector<double> &x, std::vector<double> &y, std::vector<double> &z,
+                  std::vector<double> &q) {
   const char *base = reinterpret_cast<const char *>(host);
-  for (int64_t i = 0; i < n; i++) {
+  for (int64_t i = 0; i < d->n; i++) {
     const char *b = base + (size_t)i * d->stride;
     double X[3], Q;
     memcpy(X, b + d->off_x, 24);
     memcpy(&Q, b + d->off_q, 8);
     if (!std::isfinite(X[0]) || !std::isfinite(X[1]) || !std::isfinite(X[2]) || !std::isfinite(Q))
       return fail(c, TFMM_EINVAL, "non-finite body coordinate or charge");
-    x[i] = X[0];
-    y[i] = X[1];
-    z[i] = X[2];
-    q[i] = Q;
+    x[at + i] = X[0];
+    y[at + i] = X[1];
+    z[at + i] = X[2];
+    q[at + i] = Q;
   }
+  return TFMM_OK;
+}
+
+int upload_bodies(tfmm_ctx *c, int64_t n, const std::vector<double> &x,
+                  const std::vector<double> &y, const std::vector<double> &z,
+                  const std::vector<double> &q) {
   TRY(dev_reserve(c->own_x, n * 8));
   TRY(dev_reserve(c->own_y, n * 8));
   TRY(dev_reserve(c->own_z, n * 8));
@@ -696,6 +734,95 @@ int tfmm_set_bodies(tfmm_ctx *c, const void *host, const tfmm_body_desc *d) {
   return TFMM_OK;
 }
 
+// problem offsets of a batch onto the device
+int set_problems(tfmm_ctx *c, int nprob, const int64_t *off) {
+  c->nprob = nprob;
+  c->prob_off.assign(off, off + nprob + 1);
+  if (nprob > 1) {
+    TRY(dev_reserve(c->d_prob_off, (size_t)(nprob + 1) * 8));
+    TRY(hipMemcpyAsync(c->d_prob_off.ptr, c->prob_off.data(), (size_t)(nprob + 1) * 8,
+                       hipMemcpyHostToDevice, c->stream));
+    TRY(hipStreamSynchronize(c->stream));
+  }
+  return TFMM_OK;
+}
+
+}  // namespace
+
+int tfmm_set_bodies(tfmm_ctx *c, const void *host, const tfmm_body_desc *d) {
+  if (!c) return TFMM_EINVAL;
+  int rc = check_desc(c, host, d);
+  if (rc) return rc;
+  TRY(hipSetDevice(c->device));
+  const int64_t n = d->n;
+  std::vector<double> x(n), y(n), z(n), q(n);
+  rc = unpack_bodies(c, host, d, 0, x, y, z, q);
+  if (rc) return rc;
+  rc = upload_bodies(c, n, x, y, z, q);
+  if (rc) return rc;
+  const int64_t off[2] = {0, n};
+  return set_problems(c, 1, off);
+}
+
+int tfmm_set_batch(tfmm_ctx *c, int nprob, const tfmm_body_desc *descs,
+                   const void *const *bodies) {
+  if (!c) return TFMM_EINVAL;
+  if (nprob < 1 || nprob > TFMM_MAX_PROBLEMS)
+    return fail(c, TFMM_EINVAL, "nprob must be in [1, TFMM_MAX_PROBLEMS]");
+  if (!descs || !bodies) return fail(c, TFMM_EINVAL, "null descriptors or bodies");
+  std::vector<int64_t> off(nprob + 1, 0);
+  for (int p = 0; p < nprob; p++) {
+    int rc = check_desc(c, bodies[p], &descs[p]);
+    if (rc) return rc;
+    off[p + 1] = off[p] + descs[p].n;
+  }
+  if (off[nprob] > INT32_MAX) return fail(c, TFMM_EINVAL, "total body count must be < 2^31");
+  TRY(hipSetDevice(c->device));
+  const int64_t n = off[nprob];
+  std::vector<double> x(n), y(n), z(n), q(n);
+  for (int p = 0; p < nprob; p++) {
+    int rc = unpack_bodies(c, bodies[p], &descs[p], off[p], x, y, z, q);
+    if (rc) return rc;
+  }
+  int rc = upload_bodies(c, n, x, y, z, q);
+  if (rc) return rc;
+  return set_problems(c, nprob, off.data());
+}
+
+int tfmm_set_batch_device(tfmm_ctx *c, int nprob, const int64_t *offsets, const double *x,
+                          const double *y, const double *z, const double *q) {
+  if (!c) return TFMM_EINVAL;
+  if (nprob < 1 || nprob > TFMM_MAX_PROBLEMS)
+    return fail(c, TFMM_EINVAL, "nprob must be in [1, TFMM_MAX_PROBLEMS]");
+  if (!offsets) return fail(c, TFMM_EINVAL, "null offsets");
+  if (!x || !y || !z || !q) return fail(c, TFMM_EINVAL, "null body array");
+  if (offsets[0] != 0) return fail(c, TFMM_EINVAL, "offsets[0] must be 0");
+  for (int p = 0; p < nprob; p++)
+    if (offsets[p + 1] <= offsets[p]) return fail(c, TFMM_EINVAL, "every problem needs a body");
+  if (offsets[nprob] > INT32_MAX) return fail(c, TFMM_EINVAL, "total body count must be < 2^31");
+  TRY(hipSetDevice(c->device));
+  c->n = offsets[nprob];
+  c->bx = x;
+  c->by = y;
+  c->bz = z;
+  c->bq = q;
+  c->stage = 0;
+  return set_problems(c, nprob, offsets);
+}
+
+int tfmm_batch_evaluate(tfmm_ctx *c, int nprob, const tfmm_body_desc *descs,
+                        const void *const *bodies, const tfmm_params *p) {
+  int rc = tfmm_set_batch(c, nprob, descs, bodies);
+  if (rc) return rc;
+  return tfmm_evaluate(c, p);
+}
+
+int tfmm_num_problems(const tfmm_ctx *c, int64_t *offsets) {
+  if (!c) return TFMM_EINVAL;
+  if (offsets) memcpy(offsets, c->prob_off.data(), c->prob_off.size() * 8);
+  return c->nprob;
+}
+
 int tfmm_set_bodies_device(tfmm_ctx *c, int64_t n, const double *x, const double *y,
                            const double *z, const double *q) {
   if (!c) return TFMM_EINVAL;
@@ -707,7 +834,8 @@ int tfmm_set_bodies_device(tfmm_ctx *c, int64_t n, const double *x, const double
   c->bz = z;
   c->bq = q;
   c->stage = 0;
-  return TFMM_OK;
+  const int64_t off[2] = {0, n};
+  return set_problems(c, 1, off);
 }
 
 int tfmm_build_tree(tfmm_ctx *c, const tfmm_params *p) {
@@ -795,6 +923,7 @@ int tfmm_get_stats(tfmm_ctx *c, tfmm_stats_t *o) {
   o->nleaves = c->nleaves;
   o->nlevels = c->stage >= 1 ? (int32_t)c->lvl.size() - 1 : 0;
   o->max_depth = o->nlevels - 1;
+  o->n_problems = c->nprob;
   for (int d = 0; d < 3; d++) {
     o->region_min[d] = c->h_region[d];
     o->region_max[d] = c->h_region[3 + d];
@@ -913,17 +1042,43 @@ int tfmm_direct(tfmm_ctx *c, const int64_t *targets, int64_t nt, double *out) {
   for (int64_t i = 0; i < nt; i++)
     if (targets[i] < 0 || targets[i] >= c->n) return fail(c, TFMM_EINVAL, "target index out of range");
   TRY(hipSetDevice(c->device));
+  // targets grouped by problem; each group sums over its own problem's bodies
+  const int np = c->nprob;
+  std::vector<int64_t> rel(nt), where(nt), cnt(np + 1, 0);
+  auto prob_of = [&](int64_t i) {
+    return (int)(std::upper_bound(c->prob_off.begin(), c->prob_off.begin() + np, i) -
+                 c->prob_off.begin()) - 1;
+  };
+  for (int64_t i = 0; i < nt; i++) cnt[prob_of(targets[i]) + 1]++;
+  for (int p = 0; p < np; p++) cnt[p + 1] += cnt[p];
+  {
+    std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+    for (int64_t i = 0; i < nt; i++) {
+      const int p = prob_of(targets[i]);
+      const int64_t at = fill[p]++;
+      rel[at] = targets[i] - c->prob_off[p];
+      where[at] = i;
+    }
+  }
   DevBuf t, o;
+  std::vector<double> grouped((size_t)nt * 4);
   hipError_t e = dev_reserve(t, (size_t)nt * 8);
   if (e == hipSuccess) e = dev_reserve(o, (size_t)nt * 32);
-  if (e == hipSuccess) e = hipMemcpyAsync(t.ptr, targets, (size_t)nt * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(t.ptr, rel.data(), (size_t)nt * 8, hipMemcpyHostToDevice, c->stream);
+  for (int p = 0; p < np && e == hipSuccess; p++) {
+    const int64_t b = c->prob_off[p], m = c->prob_off[p + 1] - b;
+    if (cnt[p + 1] > cnt[p])
+      e = launch_direct_fp64(c->bx + b, c->by + b, c->bz + b, c->bq + b, m,
+                             dp<int64_t>(t) + cnt[p], cnt[p + 1] - cnt[p],
+                             dp<double>(o) + 4 * cnt[p], c->stream);
+  }
   if (e == hipSuccess)
-    e = launch_direct_fp64(c->bx, c->by, c->bz, c->bq, c->n, dp<int64_t>(t), nt, dp<double>(o), c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(out, o.ptr, (size_t)nt * 32, hipMemcpyDeviceToHost, c->stream);
+    e = hipMemcpyAsync(grouped.data(), o.ptr, (size_t)nt * 32, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   dev_release(t);
   dev_release(o);
   if (e != hipSuccess) return hipfail(c, e, "tfmm_direct");
+  for (int64_t a = 0; a < nt; a++) memcpy(out + 4 * where[a], &grouped[4 * a], 32);
   return TFMM_OK;
 }
 
@@ -931,6 +1086,7 @@ int tfmm_tree_features(tfmm_ctx *c, float *feat) {
   int rc = need_stage(c, 1, "tfmm_tree_features");
   if (rc) return rc;
   if (!feat) return fail(c, TFMM_EINVAL, "null feature array");
+  if (c->nprob > 1) return fail(c, TFMM_EINVAL, "tree features describe a single problem, not a batch");
   hipStream_t s = c->stream;
   TRY(dev_reserve(c->feat_acc, 16 * 8));
   TRY(hipMemsetAsync(c->feat_acc.ptr, 0, 16 * 8, s));
@@ -962,6 +1118,7 @@ int tfmm_pick_params(tfmm_ctx *c, tfmm_picker *picker, tfmm_params *io) {
   io->mutual = 0;
   if (!picker) return TFMM_OK;
   if (c->n < 1) return fail(c, TFMM_ESTATE, "no bodies set");
+  if (c->nprob > 1) return fail(c, TFMM_EINVAL, "the picker chooses parameters for a single problem, not a batch");
   if (c->stage < 1 || c->prm.ncrit != 64) {
     tfmm_params trial = *io;
     int rc = tfmm_build_tree(c, &trial);

@@ -35,16 +35,27 @@ struct Csr {
 hipError_t launch_bbox(const double *x, const double *y, const double *z, int64_t n,
                        double *partial, int nparts, double *region, hipStream_t s);
 hipError_t launch_keys(const double *x, const double *y, const double *z, int64_t n,
-                       const double *region, uint64_t *keys, uint32_t *idx, hipStream_t s);
+                       const double *regions, const int64_t *off, int nprob, uint64_t *keys,
+                       uint32_t *idx, uint32_t *bprob, hipStream_t s);
+// forest of nprob problems, problem p = bodies [off[p], off[p+1])
+hipError_t launch_bbox_forest(const double *x, const double *y, const double *z,
+                              const int64_t *off, int nprob, double *regions, hipStream_t s);
+hipError_t launch_problem_keys(const uint32_t *order, const uint32_t *bprob, int64_t n,
+                               uint32_t *pkey, uint32_t *pos, hipStream_t s);
+hipError_t launch_problem_gather(const uint32_t *pos, const uint64_t *keys1,
+                                 const uint32_t *order1, int64_t n, uint64_t *keys,
+                                 uint32_t *order, hipStream_t s);
+hipError_t launch_root_pairs(int2 *front, int nroots, uint32_t *count, hipStream_t s);
 hipError_t launch_permute(const uint32_t *order, const double *x, const double *y,
                           const double *z, const double *q, int64_t n, double *sx, double *sy,
                           double *sz, double *sq, int32_t *perm, hipStream_t s);
-hipError_t launch_root(Cells c, int64_t n, const double *region, hipStream_t s);
+hipError_t launch_roots(Cells c, int64_t n, const int64_t *off, int nprob,
+                        const double *regions, hipStream_t s);
 hipError_t launch_split_flags(Cells c, int32_t lbeg, int32_t lcnt, int ncrit, uint32_t *flag,
                               hipStream_t s);
 hipError_t launch_children(Cells c, int32_t lbeg, int32_t lcnt, const uint32_t *flag,
                            const uint32_t *slot, int32_t next_beg, const uint64_t *keys,
-                           const double *region, hipStream_t s);
+                           const double *regions, const uint32_t *sprob, hipStream_t s);
 
 // expansions.hip (FP64)
 hipError_t launch_p2m(Cells c, int32_t ncells, Bodies b, double *M, hipStream_t s);

@@ -79,11 +79,28 @@ __global__ __launch_bounds__(BB_THREADS) void k_bbox_final(const double *pmin,
   }
 }
 
+// Problem of body i of a forest: the last p with off[p] <= i.
+__device__ inline int problem_of(const int64_t *off, int nprob, int64_t i) {
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Keys against the region of the body's own problem (off == nullptr: one
+// problem); the problem index is kept per body for the forest sort.
 __global__ void k_keys(const double *x, const double *y, const double *z, int64_t n,
-                       const double *region, uint64_t *keys, uint32_t *idx) {
+                       const double *regions, const int64_t *off, int nprob, uint64_t *keys,
+                       uint32_t *idx, uint32_t *bprob) {
 #pragma clang fp contract(off)
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const int pb = off ? problem_of(off, nprob, i) : 0;
+  if (bprob) bprob[i] = (uint32_t)pb;
+  const double *region = regions + 6 * pb;
   const int nfin = 1 << MAXDEPTH;
   const double p[3] = {x[i], y[i], z[i]};
   int a[3];
@@ -97,6 +114,63 @@ __global__ void k_keys(const double *x, const double *y, const double *z, int64_
   }
   keys[i] = finest_key(a[0], a[1], a[2]);
   idx[i] = (uint32_t)i;
+}
+
+// Bounding box of every problem of a forest, one block per problem, with
+// the same scalar-cube rule as the single-problem box: problem p is its own
+// Partition, so its first body's x and y are the ones left out.
+__global__ __launch_bounds__(BB_THREADS) void k_bbox_forest(const double *x, const double *y,
+                                                            const double *z, const int64_t *off,
+                                                            double *regions) {
+  const int pb = blockIdx.x;
+  const int64_t b = off[pb], e = off[pb + 1];
+  double lo = INFINITY, hi = -INFINITY;
+  for (int64_t i = b + threadIdx.x; i < e; i += BB_THREADS) {
+    const double vz = z[i];
+    lo = fmin(lo, vz);
+    hi = fmax(hi, vz);
+    if (i != b) {
+      const double vx = x[i], vy = y[i];
+      lo = fmin(lo, fmin(vx, vy));
+      hi = fmax(hi, fmax(vx, vy));
+    }
+  }
+  __shared__ double slo[BB_THREADS], shi[BB_THREADS];
+  slo[threadIdx.x] = lo;
+  shi[threadIdx.x] = hi;
+  __syncthreads();
+  for (int o = BB_THREADS / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      slo[threadIdx.x] = fmin(slo[threadIdx.x], slo[threadIdx.x + o]);
+      shi[threadIdx.x] = fmax(shi[threadIdx.x], shi[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) {
+    regions[6 * pb + threadIdx.x] = slo[0];
+    regions[6 * pb + 3 + threadIdx.x] = shi[0];
+  }
+}
+
+// Forest order, step 1: after the bodies are sorted by key, the problem of
+// each sorted body becomes the key of a second, stable sort.
+__global__ void k_problem_keys(const uint32_t *order, const uint32_t *bprob, int64_t n,
+                               uint32_t *pkey, uint32_t *pos) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  pkey[i] = bprob[order[i]];
+  pos[i] = (uint32_t)i;
+}
+
+// Forest order, step 2: keys and body order by (problem, key).
+__global__ void k_problem_gather(const uint32_t *pos, const uint64_t *keys1,
+                                 const uint32_t *order1, int64_t n, uint64_t *keys,
+                                 uint32_t *order) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t j = pos[i];
+  keys[i] = keys1[j];
+  order[i] = order1[j];
 }
 
 // Gather the caller's bodies into sorted order (SoA FP64) and keep the
@@ -114,23 +188,33 @@ __global__ void k_permute(const uint32_t *order, const double *x, const double *
   perm[i] = (int32_t)o;
 }
 
-__global__ void k_root(Cells c, int64_t n, const double *region) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  c.key[0] = 0;
-  c.beg[0] = 0;
-  c.nb[0] = (int32_t)n;
-  c.parent[0] = -1;
-  c.child[0] = -1;
-  c.depth[0] = 0;
-  double mn[3] = {region[0], region[1], region[2]};
-  double mx[3] = {region[3], region[4], region[5]};
+// Root cells: one per problem (off == nullptr: the single root 0 of n bodies).
+__global__ void k_roots(Cells c, int64_t n, const int64_t *off, int nprob,
+                        const double *regions) {
+  const int pb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pb >= nprob) return;
+  const int64_t b = off ? off[pb] : 0, e = off ? off[pb + 1] : n;
+  c.key[pb] = 0;
+  c.beg[pb] = (int32_t)b;
+  c.nb[pb] = (int32_t)(e - b);
+  c.parent[pb] = -1;
+  c.child[pb] = -1;
+  c.depth[pb] = 0;
+  const double *region = regions + 6 * pb;
   double w = 0;
   for (int d = 0; d < 3; d++) {
-    c.center[3 * 0 + d] = (mx[d] + mn[d]) / 2;
-    const double wd = mx[d] - mn[d];
+    c.center[3 * (int64_t)pb + d] = (region[3 + d] + region[d]) / 2;
+    const double wd = region[3 + d] - region[d];
     w = (w < wd) ? wd : w;
   }
-  c.width[0] = w;
+  c.width[pb] = w;
+}
+
+// Traversal start: the pair (root, root) of every problem.
+__global__ void k_root_pairs(int2 *front, int nroots, uint32_t *count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nroots) front[i] = make_int2(i, i);
+  if (i == 0) *count = (uint32_t)nroots;
 }
 
 __global__ void k_split_flags(Cells c, int32_t lbeg, int32_t lcnt, int ncrit, uint32_t *flag) {
@@ -154,7 +238,7 @@ __device__ inline int32_t lower_bound_keys(const uint64_t *k, int32_t lo, int32_
 
 __global__ void k_children(Cells c, int32_t lbeg, int32_t lcnt, const uint32_t *flag,
                            const uint32_t *slot, int32_t next_beg, const uint64_t *keys,
-                           const double *region) {
+                           const double *regions, const uint32_t *sprob) {
   // one thread per (parent, child) -> 8 threads per parent
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)lcnt * 8) return;
@@ -175,6 +259,8 @@ __global__ void k_children(Cells c, int32_t lbeg, int32_t lcnt, const uint32_t *
   c.parent[ci] = p;
   c.child[ci] = -1;
   c.depth[ci] = c.depth[p] + 1;
+  // a split cell holds bodies, so its first body names its problem
+  const double *region = regions + (sprob ? 6 * (int64_t)sprob[pb] : 0);
   double rmin[3] = {region[0], region[1], region[2]};
   double rmax[3] = {region[3], region[4], region[5]};
   double mn[3], mx[3];
@@ -199,9 +285,39 @@ hipError_t launch_bbox(const double *x, const double *y, const double *z, int64_
 }
 
 hipError_t launch_keys(const double *x, const double *y, const double *z, int64_t n,
-                       const double *region, uint64_t *keys, uint32_t *idx, hipStream_t s) {
+                       const double *regions, const int64_t *off, int nprob, uint64_t *keys,
+                       uint32_t *idx, uint32_t *bprob, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  k_keys<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(x, y, z, n, region, keys, idx);
+  k_keys<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(x, y, z, n, regions, off, nprob, keys, idx,
+                                                      bprob);
+  return hipGetLastError();
+}
+
+hipError_t launch_bbox_forest(const double *x, const double *y, const double *z,
+                              const int64_t *off, int nprob, double *regions, hipStream_t s) {
+  if (nprob <= 0) return hipSuccess;
+  k_bbox_forest<<<nprob, BB_THREADS, 0, s>>>(x, y, z, off, regions);
+  return hipGetLastError();
+}
+
+hipError_t launch_problem_keys(const uint32_t *order, const uint32_t *bprob, int64_t n,
+                               uint32_t *pkey, uint32_t *pos, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_problem_keys<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(order, bprob, n, pkey, pos);
+  return hipGetLastError();
+}
+
+hipError_t launch_problem_gather(const uint32_t *pos, const uint64_t *keys1,
+                                 const uint32_t *order1, int64_t n, uint64_t *keys,
+                                 uint32_t *order, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_problem_gather<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(pos, keys1, order1, n, keys,
+                                                                order);
+  return hipGetLastError();
+}
+
+hipError_t launch_root_pairs(int2 *front, int nroots, uint32_t *count, hipStream_t s) {
+  k_root_pairs<<<(nroots + 255) / 256, 256, 0, s>>>(front, nroots, count);
   return hipGetLastError();
 }
 
@@ -214,8 +330,9 @@ hipError_t launch_permute(const uint32_t *order, const double *x, const double *
   return hipGetLastError();
 }
 
-hipError_t launch_root(Cells c, int64_t n, const double *region, hipStream_t s) {
-  k_root<<<1, 64, 0, s>>>(c, n, region);
+hipError_t launch_roots(Cells c, int64_t n, const int64_t *off, int nprob,
+                        const double *regions, hipStream_t s) {
+  k_roots<<<(nprob + 255) / 256, 256, 0, s>>>(c, n, off, nprob, regions);
   return hipGetLastError();
 }
 
@@ -228,11 +345,11 @@ hipError_t launch_split_flags(Cells c, int32_t lbeg, int32_t lcnt, int ncrit, ui
 
 hipError_t launch_children(Cells c, int32_t lbeg, int32_t lcnt, const uint32_t *flag,
                            const uint32_t *slot, int32_t next_beg, const uint64_t *keys,
-                           const double *region, hipStream_t s) {
+                           const double *regions, const uint32_t *sprob, hipStream_t s) {
   if (lcnt <= 0) return hipSuccess;
   const int64_t nt = (int64_t)lcnt * 8;
   k_children<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(c, lbeg, lcnt, flag, slot, next_beg,
-                                                           keys, region);
+                                                           keys, regions, sprob);
   return hipGetLastError();
 }
 

@@ -102,3 +102,30 @@ GENERATORS = {
 
 def make(dist, n, seed=0, **kw):
     return GENERATORS[dist](n, seed=seed, **kw)
+
+
+BATCH_KINDS = ("cube", "plummer", "ellipsoid")
+
+
+def batch(nprob, seed=0, nmin=1 << 10, nmax=1 << 16):
+    """Independent small problems: N log-uniform in [nmin, nmax], each drawn
+    from the uniform cube, a Plummer sphere (a = 1) or the 1:0.3:0.1
+    ellipsoid (uniform random choice), charges uniform in [-1, 1].
+    Returns a list of (x, y, z, q) and the list of kinds."""
+    rng = _rng(seed)
+    sizes = np.exp(rng.uniform(np.log(nmin), np.log(nmax), nprob)).astype(np.int64)
+    sizes = np.clip(sizes, nmin, nmax)
+    kinds = [BATCH_KINDS[k] for k in rng.integers(0, len(BATCH_KINDS), nprob)]
+    seeds = rng.integers(0, 2 ** 62, nprob)
+    out = []
+    for n, kind, s in zip(sizes, kinds, seeds):
+        if kind == "cube":
+            r = _rng(int(s))
+            pos = r.random((3, int(n)))
+            p = (pos[0].copy(), pos[1].copy(), pos[2].copy(), r.uniform(-1.0, 1.0, int(n)))
+        elif kind == "plummer":
+            p = plummer(int(n), seed=int(s), charges="uniform")
+        else:
+            p = ellipsoid(int(n), seed=int(s))
+        out.append(p)
+    return out, kinds

@@ -50,7 +50,8 @@ class Stats(ctypes.Structure):
                 ("ms_total", ctypes.c_double), ("ms_p2p", ctypes.c_double),
                 ("ms_m2l", ctypes.c_double), ("ms_dtt", ctypes.c_double),
                 ("n_m2l_groups", ctypes.c_uint64), ("n_m2l_items", ctypes.c_uint64),
-                ("n_m2l_fallback", ctypes.c_uint64), ("ms_m2l_tiles", ctypes.c_double)]
+                ("n_m2l_fallback", ctypes.c_uint64), ("ms_m2l_tiles", ctypes.c_double),
+                ("n_problems", ctypes.c_int64)]
 
     def as_dict(self):
         out = {}
@@ -62,7 +63,8 @@ class Stats(ctypes.Structure):
 
 EXPORTED = [
     "tfmm_create", "tfmm_destroy", "tfmm_last_error", "tfmm_stream", "tfmm_sync",
-    "tfmm_set_bodies", "tfmm_set_bodies_device", "tfmm_build_tree", "tfmm_set_params",
+    "tfmm_set_bodies", "tfmm_set_bodies_device", "tfmm_set_batch", "tfmm_set_batch_device",
+    "tfmm_batch_evaluate", "tfmm_num_problems", "tfmm_build_tree", "tfmm_set_params",
     "tfmm_upward",
     "tfmm_traverse", "tfmm_downward", "tfmm_evaluate", "tfmm_get_fields", "tfmm_get_stats",
     "tfmm_get_sorted", "tfmm_get_cells", "tfmm_get_expansions", "tfmm_get_lists", "tfmm_direct",
@@ -93,6 +95,12 @@ def lib():
         L.tfmm_sync.argtypes = [vp]
         L.tfmm_set_bodies.argtypes = [vp, vp, ctypes.POINTER(BodyDesc)]
         L.tfmm_set_bodies_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp]
+        L.tfmm_set_batch.argtypes = [vp, ctypes.c_int, ctypes.POINTER(BodyDesc),
+                                     ctypes.POINTER(vp)]
+        L.tfmm_set_batch_device.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp]
+        L.tfmm_batch_evaluate.argtypes = [vp, ctypes.c_int, ctypes.POINTER(BodyDesc),
+                                          ctypes.POINTER(vp), ctypes.POINTER(Params)]
+        L.tfmm_num_problems.argtypes = [vp, vp]
         L.tfmm_build_tree.argtypes = [vp, ctypes.POINTER(Params)]
         L.tfmm_evaluate.argtypes = [vp, ctypes.POINTER(Params)]
         L.tfmm_set_params.argtypes = [vp, ctypes.POINTER(Params)]
@@ -195,6 +203,39 @@ class Context:
         self._check(lib().tfmm_set_bodies_device(self._h, int(n), px, py, pz, pq),
                     "tfmm_set_bodies_device")
         self.n = int(n)
+
+    @staticmethod
+    def _aos(x, y, z, q):
+        body = np.zeros((len(x), 12), np.float64)  # the reference's 96-byte Body
+        body[:, 0] = x
+        body[:, 1] = y
+        body[:, 2] = z
+        body[:, 3] = q
+        return body
+
+    def set_batch(self, problems):
+        """Copy a batch of independent problems, a list of (x, y, z, q),
+        evaluated together as one forest (bodies concatenated in order)."""
+        bodies = [self._aos(*p) for p in problems]
+        descs = (BodyDesc * len(bodies))(*[BodyDesc(len(b), 96, 0, 24) for b in bodies])
+        ptrs = (ctypes.c_void_p * len(bodies))(*[b.ctypes.data for b in bodies])
+        self._check(lib().tfmm_set_batch(self._h, len(bodies), descs, ptrs), "tfmm_set_batch")
+        self.n = sum(len(b) for b in bodies)
+
+    def set_batch_device(self, offsets, px, py, pz, pq):
+        """Use device-resident FP64 SoA arrays holding len(offsets) - 1
+        problems, problem p = [offsets[p], offsets[p+1])."""
+        off = np.ascontiguousarray(offsets, np.int64)
+        self._check(lib().tfmm_set_batch_device(self._h, len(off) - 1, _ptr(off), px, py, pz,
+                                                pq), "tfmm_set_batch_device")
+        self.n = int(off[-1])
+
+    def problem_offsets(self):
+        n = lib().tfmm_num_problems(self._h, None)
+        self._check(min(n, 0), "tfmm_num_problems")
+        off = np.zeros(n + 1, np.int64)
+        lib().tfmm_num_problems(self._h, _ptr(off))
+        return off
 
     # -- phases ----------------------------------------------------------
     def build_tree(self, params):

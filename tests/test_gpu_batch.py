@@ -1,0 +1,94 @@
+"""Batched independent problems evaluated as one forest (through the C ABI):
+the batch gives, problem by problem, bit for bit what evaluating each problem
+alone gives -- keys, cells, interaction counts and fields -- in FP64 and in
+mixed precision, and the FP64 batch matches the CPU oracle."""
+import numpy as np
+import pytest
+
+import datasets
+import oracle
+import tfmm
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = tfmm.Context(0)
+    yield c
+    c.close()
+
+
+def problems():
+    probs, kinds = datasets.batch(9, seed=5, nmin=200, nmax=20000)
+    # edge cases: one body, a root leaf, a tight pair of clumps
+    probs.append((np.array([0.3]), np.array([0.2]), np.array([0.9]), np.array([1.0])))
+    x, y, z, q = datasets.cube(40, seed=2)
+    probs.append((x, y, z, q))
+    r = np.random.default_rng(3)
+    x = np.repeat([0.1, 0.7], 35) + r.normal(0, 1e-3, 70)
+    probs.append((x, r.random(70), r.random(70), r.uniform(-1, 1, 70)))
+    return probs
+
+
+@pytest.mark.parametrize("precision", [tfmm.FP64, tfmm.MIXED])
+def test_gpu_batch_matches_single_problems(ctx, precision):
+    probs = problems()
+    ncrit, theta = 32, 0.4
+    prm = tfmm.make_params(ncrit, theta, precision)
+    print("batch of %d problems, precision %d" % (len(probs), precision), flush=True)
+    ctx.set_batch(probs)
+    ctx.evaluate(prm)
+    fb = ctx.fields()
+    sb = ctx.stats()
+    keys_b, perm_b = ctx.sorted()
+    off = ctx.problem_offsets()
+    assert sb["n_problems"] == len(probs)
+    assert np.array_equal(off, np.concatenate([[0], np.cumsum([len(p[0]) for p in probs])]))
+    ncells = nm2l = np2p = 0
+    for i, p in enumerate(probs):
+        print("problem %d n=%d" % (i, len(p[0])), flush=True)
+        ctx.set_bodies(*p)
+        ctx.evaluate(prm)
+        f = ctx.fields()
+        s = ctx.stats()
+        k, perm = ctx.sorted()
+        assert np.array_equal(fb[off[i]:off[i + 1]], f), "fields of problem %d" % i
+        assert np.array_equal(keys_b[off[i]:off[i + 1]], k)
+        assert np.array_equal(np.sort(perm_b[off[i]:off[i + 1]] - off[i]), np.arange(len(k)))
+        ncells += s["ncells"]
+        nm2l += s["n_m2l"]
+        np2p += s["n_p2p_calls"]
+    assert sb["ncells"] == ncells and sb["n_m2l"] == nm2l and sb["n_p2p_calls"] == np2p
+
+
+def test_gpu_batch_against_oracle(ctx):
+    """FP64 batch within relative L2 1e-10 of the CPU oracle, per problem, and
+    the per-problem direct sum agrees with the oracle's."""
+    probs, _ = datasets.batch(4, seed=8, nmin=500, nmax=4000)
+    ctx.set_batch(probs)
+    ctx.evaluate(tfmm.make_params(64, 0.4, tfmm.FP64))
+    fb = ctx.fields()
+    off = ctx.problem_offsets()
+    t = np.array([0, 17, off[1] + 3, off[2], off[3] + 5, off[4] - 1], np.int64)
+    d = ctx.direct(t)
+    for i, (x, y, z, q) in enumerate(probs):
+        o = oracle.Run(x, y, z, q, 64, 0.4)
+        ep, ea = oracle.rel_l2(fb[off[i]:off[i + 1]], o.fields())
+        assert ep < 1e-10 and ea < 1e-10, (i, ep, ea)
+        sel = (t >= off[i]) & (t < off[i + 1])
+        dd = oracle.direct(x, y, z, q, t[sel] - off[i])
+        assert np.allclose(d[sel], dd, rtol=1e-11, atol=1e-11 * np.abs(dd).max())
+
+
+def test_gpu_batch_errors(ctx):
+    with pytest.raises(tfmm.TfmmError):
+        ctx.set_batch([])
+    x, y, z, q = datasets.cube(100, seed=1)
+    ctx.set_batch([(x, y, z, q), (x, y, z, q)])
+    pk = tfmm.Picker(0)
+    try:
+        with pytest.raises(tfmm.TfmmError):
+            ctx.pick_params(pk, tfmm.MIXED)  # the picker describes one problem
+    finally:
+        pk.close()
