@@ -10,6 +10,11 @@ ncrit = 64, theta = 0.4.  The error of the evaluated fields is measured
 against the FP64 direct sum on 4096 random targets (outside the timed
 region).  The CPU baseline is the reference's own pipeline (oracle/_ref,
 1 MPI rank) timed on a bounded sample of the same distribution.
+
+--batch K: K independent problems instead (N log-uniform in [2^10, 2^16],
+each a uniform cube, a Plummer sphere or a 1:0.3:0.1 ellipsoid, charges
+uniform in [-1, 1]) evaluated together as one forest, ncrit 64, theta 0.4;
+the value is the total bodies of the batch per second.
 """
 import argparse
 import json
@@ -47,6 +52,10 @@ def parse():
     ap.add_argument("--targets", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=524288)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--unit-factor", type=float, default=None,
+                    help="M2L work-unit size in mean items per tile (0: one unit per tile)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="evaluate this many independent small problems as one forest")
     return ap.parse_args()
 
 
@@ -93,6 +102,29 @@ def measured_traffic(workload, kernel):
     return None
 
 
+def cpu_baseline_batch(probs, ncrit, theta, budget_bodies):
+    """Reference pipeline on the batch's first problems (one process each,
+    1 MPI rank) until budget_bodies bodies are covered."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    if oracle.ref_binary(count=False) is None:
+        return None
+    done = nb = 0
+    t = 0.0
+    for x, y, z, q in probs:
+        r = oracle.run_ref(x, y, z, q, ncrit, theta, count=False, timeout=600)
+        ph = r["times"]
+        t += ph["tree"] + ph["upward"] + ph["traverse"] + ph["downward"]
+        nb += len(x)
+        done += 1
+        if nb >= budget_bodies:
+            break
+    return {"value": nb / t, "unit": "bodies/s", "cores": 1, "kind": "reference",
+            "sample": "the first %d problems of the batch (%d bodies), each a separate reference "
+                      "TapasFMM run (ncrit %d, theta %.4g, 1 MPI rank); summed phase times %.2fs"
+                      % (done, nb, ncrit, theta, t)}
+
+
 def time_config(ctx, prm, steps, warmup):
     import numpy as np
     for _ in range(warmup):
@@ -133,7 +165,14 @@ def main():
     import tfmm
 
     precision = tfmm.FP64 if args.precision == "fp64" else tfmm.MIXED
-    x, y, z, q = datasets.make(args.dist, args.n, seed=0)
+    probs = None
+    if args.batch > 0:
+        probs, kinds = datasets.batch(args.batch, seed=0)
+        offsets = np.concatenate([[0], np.cumsum([len(p[0]) for p in probs])]).astype(np.int64)
+        x, y, z, q = [np.concatenate([p[i] for p in probs]) for i in range(4)]
+        args.n = int(offsets[-1])
+    else:
+        x, y, z, q = datasets.make(args.dist, args.n, seed=0)
     dev = torch.device("cuda:0")
     tx = torch.from_numpy(x).to(dev)
     ty = torch.from_numpy(y).to(dev)
@@ -142,12 +181,18 @@ def main():
     torch.cuda.synchronize()
 
     ctx = tfmm.Context(0)
-    ctx.set_bodies_device(args.n, tx.data_ptr(), ty.data_ptr(), tz.data_ptr(), tq.data_ptr())
+    if args.unit_factor is not None:
+        ctx.set_tuning(tfmm.TUNE_M2L_UNIT, args.unit_factor)
+    if probs is not None:
+        ctx.set_batch_device(offsets, tx.data_ptr(), ty.data_ptr(), tz.data_ptr(), tq.data_ptr())
+    else:
+        ctx.set_bodies_device(args.n, tx.data_ptr(), ty.data_ptr(), tz.data_ptr(), tq.data_ptr())
 
     # parameters: the learned picker (mixed precision) or the fixed ncrit/theta
     picked = None
     ncrit, theta = args.ncrit, args.theta
-    if precision == tfmm.MIXED and not args.no_learned and os.path.exists(args.picker):
+    if (precision == tfmm.MIXED and not args.no_learned and probs is None
+            and os.path.exists(args.picker)):
         pk = tfmm.Picker.load(args.picker)
         t0 = time.perf_counter()
         p = ctx.pick_params(pk, precision)
@@ -186,13 +231,26 @@ def main():
                 "theta": args.theta, "rel_l2_pot": e64[0], "rel_l2_acc": e64[1],
                 "phases_ms": ph64, "speedup_of_headline": ms64 / ms_step}
 
-    cpu = None if args.no_cpu else cpu_baseline(args, ncrit, theta)
-    workload = "%s N=%d P=10 ncrit=%d theta=%g %s%s" % (
-        args.dist, args.n, ncrit, round(theta, 4), args.precision, " (learned)" if picked else "")
+    if args.no_cpu:
+        cpu = None
+    elif probs is not None:
+        cpu = cpu_baseline_batch(probs, ncrit, theta, args.cpu_sample)
+    else:
+        cpu = cpu_baseline(args, ncrit, theta)
+    if probs is not None:
+        workload = ("batch of %d problems (cube/plummer/ellipsoid, N log-uniform in [2^10, 2^16])"
+                    " total N=%d P=10 ncrit=%d theta=%g %s" % (len(probs), args.n, ncrit,
+                                                               round(theta, 4), args.precision))
+    else:
+        workload = "%s N=%d P=10 ncrit=%d theta=%g %s%s" % (
+            args.dist, args.n, ncrit, round(theta, 4), args.precision,
+            " (learned)" if picked else "")
     tile_kernel = "k_m2l_tiles<%s>" % ("float" if precision == tfmm.MIXED else "double")
     traffic = measured_traffic(workload, tile_kernel)
     out = {
-        "metric": "FMM bodies/s at rel-L2 err <=1e-4, 4M Plummer; P2P % FP32 peak",
+        "metric": ("FMM bodies/s at rel-L2 err <=1e-4, 4M Plummer; P2P % FP32 peak"
+                   if probs is None else
+                   "FMM bodies/s over %d batched small problems in one forest" % len(probs)),
         "value": args.n / (ms_step * 1e-3),
         "unit": "bodies/s",
         "n_gpus": 1,
@@ -205,7 +263,9 @@ def main():
         "dtype": "f64" if precision == tfmm.FP64 else "f32 P2P/M2L + f64 tree/P2M/M2M/L2L/L2P",
         "data": "synthetic",
         "config": {"workload": workload,
-            "n_bodies": args.n, "distribution": args.dist, "expansion": 10,
+            "n_bodies": args.n, "distribution": args.dist if probs is None else
+            {k: kinds.count(k) for k in datasets.BATCH_KINDS}, "expansion": 10,
+            "problems": 1 if probs is None else len(probs),
             "ncrit": ncrit, "theta": theta, "precision": args.precision},
         "error": {"rel_l2_pot": err_pot, "rel_l2_acc": err_acc, "targets": len(targets),
                   "against": "FP64 direct sum"},

@@ -173,6 +173,15 @@ int tfmm_get_expansions(tfmm_ctx *ctx, double *M, double *L);
 int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p2p_off,
                    int32_t *p2p_src);
 
+/* Tuning knobs of the kernels (no counterpart in the reference; results do
+ * not depend on them beyond the summation order).
+ * TFMM_TUNE_M2L_UNIT: the batched M2L kernel runs each 128-cell target tile
+ * as work units of at most factor x (mean items per tile) items, so that a
+ * few heavy tiles do not hold the launch back; 0 = one unit per tile. */
+#define TFMM_TUNE_M2L_UNIT 1
+int tfmm_set_tuning(tfmm_ctx *ctx, int knob, double value);
+int tfmm_get_tuning(tfmm_ctx *ctx, int knob, double *value);
+
 /* FP64 direct sum on the current bodies for the given target indices (caller
  * order), normalised by the target charge; out: nt * 4 doubles (host).  In
  * a batch each target sums over the bodies of its own problem. */

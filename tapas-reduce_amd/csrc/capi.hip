@@ -63,8 +63,9 @@ struct tfmm_ctx {
   DevBuf keys, keys_alt, idx, idx_alt, sort_tmp, sx, sy, sz, sq, perm, region, bbox_part;
   DevBuf c_key, c_beg, c_nb, c_parent, c_child, c_depth, c_center, c_width;
   size_t cell_cap = 0;
-  DevBuf flag, slot, scan_tmp;
+  DevBuf flag, slot, scan_tmp, lev, lev_bsum, lev_status;
   std::vector<int32_t> lvl;
+  std::vector<int32_t> tree_hint;  // cells per level of the last tree
   int64_t ncells = 0, nleaves = 0;
   bool keys_in_alt = false;
   double h_region[6] = {0, 0, 0, 0, 0, 0};
@@ -75,6 +76,7 @@ struct tfmm_ctx {
 
   // traversal
   DevBuf front_a, front_b, m2l_pairs, p2p_pairs, cnt, lkeys, lkeys_alt, lidx, lidx_alt;
+  DevBuf dtt_geo, dtt_topo;
   DevBuf m2l_off, m2l_src, p2p_off, p2p_src;
   uint32_t m2l_cap = 0, p2p_cap = 0, front_cap = 0;
   std::vector<uint32_t> dtt_hint;  // frontier sizes of the last traversal
@@ -88,6 +90,10 @@ struct tfmm_ctx {
   DevBuf gkeys, gvals, gkeys_alt, gvals_alt, fkeys, fvals, fkeys_alt, fvals_alt;
   DevBuf it_flag, it_excl, it_seg, it_iflag, it_iexcl, items, cols, tile_off, mq, codes;
   DevBuf fb_off, fb_src, fb_part, relpos, leaf_of, tops, slot_of;
+  DevBuf u_cnt, u_off, units, u_part;
+  double m2l_unit_factor = 0.0;  // tile work-unit size in mean items per tile (0: off)
+  int32_t unit_chunk = 0;
+  uint32_t units_bound = 0;
   bool codes_ready = false;
   uint32_t n_groups = 0, n_grouped = 0, n_fallback = 0, n_items = 0;
   bool grouped_keys_in_alt = false;
@@ -204,6 +210,60 @@ int check_params(tfmm_ctx *c, const tfmm_params *p) {
   return TFMM_OK;
 }
 
+// The cell build with all level sizes on the device, grids sized from the
+// last tree's levels (with headroom), one host round trip at the end.
+// Returns 0 when the tree is complete, 1 when it must be built the
+// synchronous way (a level outgrew its grid, the cell capacity or the
+// hinted depth), or an error.
+int build_levels_hinted(tfmm_ctx *c, int64_t n, const int64_t *off, int np,
+                        const uint64_t *skeys, const uint32_t *sprob) {
+  hipStream_t s = c->stream;
+  const int L = (int)c->tree_hint.size();
+  if (2 * (L + 1) + 1 > H_U32_WORDS) return 1;
+  std::vector<int32_t> grid(L);
+  int64_t total = 0, gmax = 0;
+  for (int d = 0; d < L; d++) {
+    const int64_t h = c->tree_hint[d];
+    grid[d] = (int32_t)std::min<int64_t>(INT32_MAX / 16, h + h / 8 + 256);
+    if (d == 0) grid[d] = std::max<int32_t>(grid[d], np);
+    total += grid[d];
+    gmax = std::max<int64_t>(gmax, grid[d]);
+  }
+  if (total + 4096 > INT32_MAX / 2) return 1;
+  TRY(reserve_cells(c, (size_t)total + 4096, 0, s));
+  const int32_t cap = (int32_t)std::min<size_t>(c->cell_cap, INT32_MAX / 2);
+  TRY(dev_reserve(c->lev, (size_t)2 * (L + 1) * 4));
+  TRY(dev_reserve(c->lev_status, 4));
+  TRY(dev_reserve(c->flag, (size_t)gmax * 4 + 1024));
+  TRY(dev_reserve(c->slot, (size_t)gmax * 4 + 1024));
+  TRY(dev_reserve(c->lev_bsum, (size_t)level_blocks((int32_t)gmax) * 4 + 4));
+  TRY(hipMemsetAsync(c->lev.ptr, 0, (size_t)2 * (L + 1) * 4, s));
+  TRY(hipMemsetAsync(c->lev_status.ptr, 0, 4, s));
+  const Cells k = cells_of(c);
+  int32_t *lev = dp<int32_t>(c->lev);
+  TRY(launch_roots(k, n, off, np, dp<double>(c->region), lev, s));
+  for (int d = 0; d < L; d++)
+    TRY(launch_level_step(k, lev, d, grid[d], c->prm.ncrit, cap, dp<uint32_t>(c->flag),
+                          dp<uint32_t>(c->slot), dp<uint32_t>(c->lev_bsum),
+                          dp<uint32_t>(c->lev_status), skeys, dp<double>(c->region), sprob, s));
+  TRY(hipMemcpyAsync(c->h_u32, lev, (size_t)2 * (L + 1) * 4, hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(c->h_u32 + 2 * (L + 1), c->lev_status.ptr, 4, hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  const int32_t *hl = reinterpret_cast<const int32_t *>(c->h_u32);
+  if (c->h_u32[2 * (L + 1)] != 0 || hl[2 * L + 1] != 0) return 1;
+  c->lvl.assign(1, 0);
+  int64_t nleaves = 0;
+  for (int d = 0; d < L && hl[2 * d + 1] > 0; d++) {
+    c->lvl.push_back(hl[2 * d] + hl[2 * d + 1]);
+    nleaves += hl[2 * d + 1] - hl[2 * (d + 1) + 1] / 8;
+  }
+  c->ncells = c->lvl.back();
+  c->nleaves = nleaves;
+  c->tree_hint.clear();
+  for (size_t d = 0; d + 1 < c->lvl.size(); d++) c->tree_hint.push_back(c->lvl[d + 1] - c->lvl[d]);
+  return 0;
+}
+
 int do_build_tree(tfmm_ctx *c) {
   hipStream_t s = c->stream;
   const int64_t n = c->n;
@@ -263,9 +323,18 @@ int do_build_tree(tfmm_ctx *c) {
   TRY(launch_permute(order, c->bx, c->by, c->bz, c->bq, n, dp<double>(c->sx), dp<double>(c->sy),
                      dp<double>(c->sz), dp<double>(c->sq), dp<int32_t>(c->perm), s));
 
-  // level-synchronous cell build
+  // level-synchronous cell build: without a host round trip per level when
+  // an earlier tree gives the level sizes to size the grids by
+  if (!c->tree_hint.empty()) {
+    int rc = build_levels_hinted(c, n, off, np, skeys, sprob);
+    if (rc < 0) return rc;
+    if (rc == 0) {
+      TRY(hipMemcpyAsync(c->h_region, c->region.ptr, 48, hipMemcpyDeviceToHost, s));
+      return TFMM_OK;
+    }
+  }
   TRY(reserve_cells(c, std::max<size_t>(4096, 8 * (size_t)np), 0, s));
-  TRY(launch_roots(cells_of(c), n, off, np, dp<double>(c->region), s));
+  TRY(launch_roots(cells_of(c), n, off, np, dp<double>(c->region), nullptr, s));
   c->lvl.assign(1, 0);
   int32_t lbeg = 0, lcnt = np;
   int64_t nleaves = 0;
@@ -295,6 +364,8 @@ int do_build_tree(tfmm_ctx *c) {
   }
   c->ncells = c->lvl.back();
   c->nleaves = nleaves;
+  c->tree_hint.clear();
+  for (size_t d = 0; d + 1 < c->lvl.size(); d++) c->tree_hint.push_back(c->lvl[d + 1] - c->lvl[d]);
   TRY(hipMemcpyAsync(c->h_region, c->region.ptr, 48, hipMemcpyDeviceToHost, s));
   return TFMM_OK;
 }
@@ -358,6 +429,10 @@ int do_dtt(tfmm_ctx *c) {
   if (c->p2p_cap == 0) c->p2p_cap = (uint32_t)std::min<int64_t>(UINT32_MAX / 2, std::max<int64_t>(1 << 16, 40 * c->nleaves));
   if (c->front_cap == 0) c->front_cap = (uint32_t)std::min<int64_t>(UINT32_MAX / 2, std::max<int64_t>(1 << 20, 256 * c->ncells));
   if (cwords > H_U32_WORDS) return fail(c, TFMM_EINVAL, "tree too deep");
+  TRY(dev_reserve(c->dtt_geo, (size_t)c->ncells * 32));
+  TRY(dev_reserve(c->dtt_topo, (size_t)c->ncells * 8));
+  TRY(launch_dtt_cells(k, (int32_t)c->ncells, c->prm.theta, dp<double4>(c->dtt_geo),
+                       dp<int2>(c->dtt_topo), s));
   // A frontier that overflowed truncates every later step, so the counts
   // seen after an overflow are lower bounds: capacities at least double.
   for (int attempt = 0; attempt < 12; attempt++) {
@@ -376,7 +451,8 @@ int do_dtt(tfmm_ctx *c) {
       grid[st] = hint ? std::min<uint32_t>(c->front_cap, c->dtt_hint[st]) : c->front_cap;
       TRY(launch_dtt_step(dp<int2>(c->front_a), cnt + 16 + st, grid[st], dp<int2>(c->front_b),
                           c->front_cap, cnt + 16 + st + 1, dp<int2>(c->m2l_pairs), c->m2l_cap,
-                          dp<int2>(c->p2p_pairs), c->p2p_cap, cnt, k, c->prm.theta, s));
+                          dp<int2>(c->p2p_pairs), c->p2p_cap, cnt, dp<double4>(c->dtt_geo),
+                          dp<int2>(c->dtt_topo), s));
       std::swap(c->front_a, c->front_b);
     }
     TRY(hipMemcpyAsync(c->h_u32, cnt, (size_t)cwords * 4, hipMemcpyDeviceToHost, s));
@@ -464,7 +540,8 @@ int build_m2l_batches(tfmm_ctx *c) {
   const int sbits = bits_for(c->ncells);
   {
     const int64_t nt = (c->ncells + m2l_tile_size() - 1) / m2l_tile_size();
-    if (bits_for(nt) + gbits > 32 || c->ncells >= ((int64_t)1 << m2l_src_bits()))
+    if (bits_for(nt) + gbits > 32 || gbits > m2l_group_bits() ||
+        c->ncells >= ((int64_t)1 << m2l_src_bits()))
       return fail(c, TFMM_EHIP, "tree too large for the batched M2L keys");
   }
   TRY(launch_m2l_classify(dp<int2>(c->m2l_pairs), dp<int32_t>(c->slot_of), n, dp<uint32_t>(c->g_gid), gbits, sbits,
@@ -508,6 +585,15 @@ int build_m2l_batches(tfmm_ctx *c) {
   c->n_items = nitems;
   TRY(dev_reserve(c->tile_off, (size_t)(ntiles + 1) * 4));
   TRY(launch_m2l_tile_offsets(dp<int4>(c->items), nitems, ntiles, dp<int32_t>(c->tile_off), s));
+  c->unit_chunk = m2l_unit_chunk(nitems, ntiles, c->m2l_unit_factor);
+  c->units_bound = m2l_units_bound(nitems, ntiles, c->unit_chunk);
+  TRY(dev_reserve(c->u_cnt, (size_t)ntiles * 4));
+  TRY(dev_reserve(c->u_off, (size_t)ntiles * 4));
+  TRY(dev_reserve(c->units, (size_t)c->units_bound * 8));
+  TRY(dev_reserve(c->u_part, m2l_unit_part_bytes(c->units_bound, ntiles)));
+  TRY(dev_reserve(c->scan_tmp, scan_tmp_bytes(std::max<int64_t>(ntiles, std::max<int64_t>(nslots, n)))));
+  TRY(launch_m2l_units(dp<int32_t>(c->tile_off), ntiles, c->unit_chunk, dp<uint32_t>(c->u_cnt),
+                       dp<uint32_t>(c->u_off), dp<int2>(c->units), c->scan_tmp.ptr, s));
   // fallback pairs: per-target list sorted by (target, source)
   bool falt = false;
   TRY(radix_sort_pairs(dp<uint64_t>(c->fkeys), dp<uint32_t>(c->fvals), dp<uint64_t>(c->fkeys_alt),
@@ -569,10 +655,12 @@ int do_traverse(tfmm_ctx *c) {
   TRY(hipEventRecord(c->ev[EV_M2L0], s));
   const int32_t tile = m2l_tile_size();
   const int32_t ntiles = (int32_t)((c->ncells + tile - 1) / tile);
-  TRY(launch_m2l_tiles(dp<int32_t>(c->tile_off), ntiles, dp<uint2>(c->cols),
+  TRY(launch_m2l_tiles(dp<int32_t>(c->tile_off), dp<int2>(c->units), dp<uint32_t>(c->u_cnt),
+                       dp<uint32_t>(c->u_off), ntiles, c->unit_chunk, c->units_bound,
+                       dp<uint2>(c->cols),
                        mixed ? (const void *)c->g_y32.ptr : (const void *)c->g_y64.ptr,
-                       c->mq.ptr, dp<uint32_t>(c->codes), k,
-                       (int32_t)c->ncells, dp<double>(c->L), mixed, s));
+                       c->mq.ptr, dp<uint32_t>(c->codes), k, (int32_t)c->ncells,
+                       dp<double>(c->L), dp<double>(c->u_part), mixed, s));
   TRY(hipEventRecord(c->ev[EV_M2LT], s));
   if (c->n_fallback) {
     Csr fl{dp<int32_t>(c->fb_off), dp<int32_t>(c->fb_src)};
@@ -674,6 +762,26 @@ void tfmm_destroy(tfmm_ctx *c) {
 }
 
 const char *tfmm_last_error(const tfmm_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int tfmm_set_tuning(tfmm_ctx *c, int knob, double value) {
+  if (!c) return TFMM_EINVAL;
+  switch (knob) {
+    case TFMM_TUNE_M2L_UNIT:
+      if (!(value >= 0) || !(value <= 1e6)) return fail(c, TFMM_EINVAL, "unit factor out of range");
+      c->m2l_unit_factor = value;
+      return TFMM_OK;
+    default:
+      return fail(c, TFMM_EINVAL, "unknown tuning knob");
+  }
+}
+
+int tfmm_get_tuning(tfmm_ctx *c, int knob, double *value) {
+  if (!c || !value) return TFMM_EINVAL;
+  switch (knob) {
+    case TFMM_TUNE_M2L_UNIT: *value = c->m2l_unit_factor; return TFMM_OK;
+    default: return fail(c, TFMM_EINVAL, "unknown tuning knob");
+  }
+}
 
 void *tfmm_stream(tfmm_ctx *c) { return c ? (void *)c->stream : nullptr; }
 

@@ -33,13 +33,15 @@
 // flight while an item is multiplied), and tiles are dealt to the XCDs in
 // runs of consecutive tiles so that one XCD's L2 holds the sources its tiles
 // share.  Pairs whose offset falls outside the group table (level
-// difference >= 2) go to the per-pair kernel of m2l.hip.
+// difference above 3, or an offset beyond the table's range) go to the
+// per-pair kernel of m2l.hip.
 //
 // Per-group factors: the level factor 2^(n dl), n the source degree and
 // dl = l_t - l_s, is split as 2^(p dl) 2^(-j dl) with p = j + n, the first
 // folded into the group's harmonics (index p) and the second applied to the
 // output row (degree j); both are exact powers of two.
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "m2l_codes.h"
@@ -70,16 +72,23 @@ constexpr int WAVE_SYNC = 16;  // items between block barriers in the tile loop
 // ---------------------------------------------------------------------------
 // Slot of a (target level, level difference, offset) in the direct-mapped
 // table: offsets are in units of half the finer width; same-level offsets are
-// even with |o| <= 22, adjacent-level offsets odd with |o| <= 23.  An M2L
-// pair's parents were not well separated, so its offset is at most about
-// 2 (2 / theta + 1) half widths: the table covers theta >= 0.2.
+// even with |o| <= 22, offsets across levels odd with |o| <= 23.  An M2L
+// pair's parents were not well separated, so a same-level or adjacent-level
+// offset is at most about 2 (2 / theta + 1) half widths (the table covers
+// theta >= 0.2 there); pairs 2 or 3 levels apart, which a leaf next to a much
+// finer region makes, reach |o| = 15 and 23 at theta = 0.4.
 constexpr int SAME_HALF = 11;                // |o/2| <= 11
 constexpr int SAME_N = 2 * SAME_HALF + 1;    // 23
 constexpr int CROSS_MAX = 23;                // |o| <= 23, odd
 constexpr int CROSS_N = CROSS_MAX + 1;       // (o + 23) / 2 in [0, 23]
+constexpr int DL_MAX = 3;                    // level differences -3 .. 3
 constexpr int SLOTS_SAME = SAME_N * SAME_N * SAME_N;
 constexpr int SLOTS_CROSS = CROSS_N * CROSS_N * CROSS_N;
-constexpr int SLOTS_PER_LEVEL = SLOTS_SAME + 2 * SLOTS_CROSS;
+constexpr int SLOTS_PER_LEVEL = SLOTS_SAME + 2 * DL_MAX * SLOTS_CROSS;
+// column word of the tile kernel: target slot (7 bits) | 128 | (l_t - l_s +
+// DL_MAX) << COL_DL | group << COL_G
+constexpr int COL_DL = 8;
+constexpr int COL_G = 11;
 
 __device__ inline int32_t group_slot(int lt, int ls, const int o[3]) {
   const int dl = ls - lt;
@@ -92,14 +101,15 @@ __device__ inline int32_t group_slot(int lt, int ls, const int o[3]) {
       if (a[d] < 0 || a[d] >= SAME_N) return -1;
     }
     idx = (a[0] * SAME_N + a[1]) * SAME_N + a[2];
-  } else if (dl == 1 || dl == -1) {
+  } else if (dl >= -DL_MAX && dl <= DL_MAX) {
     int a[3];
     for (int d = 0; d < 3; d++) {
       if (!(o[d] & 1)) return -1;
       a[d] = (o[d] + CROSS_MAX) / 2;
       if (a[d] < 0 || a[d] >= CROSS_N) return -1;
     }
-    idx = SLOTS_SAME + (dl == 1 ? 0 : SLOTS_CROSS) + (a[0] * CROSS_N + a[1]) * CROSS_N + a[2];
+    const int blk = 2 * ((dl > 0 ? dl : -dl) - 1) + (dl < 0 ? 1 : 0);
+    idx = SLOTS_SAME + blk * SLOTS_CROSS + (a[0] * CROSS_N + a[1]) * CROSS_N + a[2];
   } else {
     return -1;
   }
@@ -117,7 +127,8 @@ __device__ inline void slot_decode(int32_t slot, int &lt, int &ls, int o[3]) {
     o[0] = (idx / SAME_N - SAME_HALF) * 2;
   } else {
     idx -= SLOTS_SAME;
-    ls = idx < SLOTS_CROSS ? lt + 1 : lt - 1;
+    const int blk = idx / SLOTS_CROSS;
+    ls = lt + ((blk & 1) ? -(blk / 2 + 1) : (blk / 2 + 1));
     idx %= SLOTS_CROSS;
     o[2] = (idx % CROSS_N) * 2 - CROSS_MAX;
     idx /= CROSS_N;
@@ -272,7 +283,8 @@ __global__ void k_item_flags(const uint32_t *flag, const uint32_t *excl,
 
 // Item descriptors (first pair, pairs, group, tile) and the item's 16 column
 // descriptors: cols[16 item + j] = (source cell, target slot in tile
-// | 128 if column j holds a pair | (level difference + 1) << 8 | group << 10).
+// | 128 if column j holds a pair | (level difference + DL_MAX) << COL_DL
+// | group << COL_G).
 // Every column carries the group, so that every lane of the tile kernel can
 // fetch the group's data.
 __global__ void k_items(const uint32_t *keys, const uint32_t *vals, const uint32_t *flag,
@@ -286,7 +298,7 @@ __global__ void k_items(const uint32_t *keys, const uint32_t *vals, const uint32
   const uint32_t jj = (i - segstart[seg]) & 15u;
   const uint32_t k = keys[i], v = vals[i];
   const uint32_t g = k & ((1u << gbits) - 1u);
-  const uint32_t gw = (uint32_t)(gdl[g] + 1) << 8 | (g << 10);
+  const uint32_t gw = (uint32_t)(gdl[g] + DL_MAX) << COL_DL | (g << COL_G);
   cols[(int64_t)item * 16 + jj] =
       make_uint2(v & ((1u << SRC_BITS) - 1u), (v >> SRC_BITS) | 128u | gw);
   if (iflag[i]) {  // first pair of the item: descriptor and padding columns
@@ -436,7 +448,7 @@ __device__ inline void store_item(const typename Mfma<T>::V (&acc)[WAVE_NRB[W]],
                                   T (*Lt0b)[17], int q) {
   if (!(c.y & 128u)) return;
   const uint32_t ts = c.y & 127u;
-  const int dl = (int)((c.y >> 8) & 3u) - 1;
+  const int dl = (int)((c.y >> COL_DL) & 7u) - DL_MAX;
 #pragma unroll
   for (int f = 0; f < WAVE_NRB[W]; f++) {
 #pragma unroll
@@ -485,11 +497,11 @@ __device__ inline void wave_item(WaveState<T, W> &cur, WaveState<T, W> &nxt, int
   nxt.c_b = cols[(int64_t)min(it + 2, last) * 16 + j];
   nxt.c_y = cols[(int64_t)min(it + 3, last) * 16 + j];
   load_b<T, W>(nxt.b, Mq + (int64_t)cur.c_b.x * NR + q * 28);
-  nxt.y0 = Yg[(int64_t)(cur.c_y.y >> 10) * NR + lane];
-  nxt.y1 = hi ? Yg[(int64_t)(cur.c_y.y >> 10) * NR + 64 + lane] : (T)0;
+  nxt.y0 = Yg[(int64_t)(cur.c_y.y >> COL_G) * NR + lane];
+  nxt.y1 = hi ? Yg[(int64_t)(cur.c_y.y >> COL_G) * NR + 64 + lane] : (T)0;
   // operand table and A operands of it + 1 (shared by consecutive items of
   // one group)
-  if ((cur.c_b.y >> 10) != (cur.c_cur.y >> 10)) {
+  if ((cur.c_b.y >> COL_G) != (cur.c_cur.y >> COL_G)) {
     __builtin_amdgcn_wave_barrier();
     write_table<T>(tbl, cur.y0, cur.y1, lane);
     __builtin_amdgcn_wave_barrier();
@@ -535,13 +547,13 @@ __device__ inline void run_wave(const uint2 *cols, int32_t ib, int32_t ie, const
   for (int f = 0; f < NF; f++) s0.acc[f] = (typename Mfma<T>::V){0, 0, 0, 0};
   load_b<T, W>(s0.b, Mq + (int64_t)s0.c_cur.x * NR + q * 28);
   {
-    const uint32_t g = s0.c_cur.y >> 10;
+    const uint32_t g = s0.c_cur.y >> COL_G;
     const T y0 = Yg[(int64_t)g * NR + lane];
     const T y1 = hi ? Yg[(int64_t)g * NR + 64 + lane] : (T)0;
     write_table<T>(tbl, y0, y1, lane);
   }
-  s0.y0 = Yg[(int64_t)(s0.c_b.y >> 10) * NR + lane];
-  s0.y1 = hi ? Yg[(int64_t)(s0.c_b.y >> 10) * NR + 64 + lane] : (T)0;
+  s0.y0 = Yg[(int64_t)(s0.c_b.y >> COL_G) * NR + lane];
+  s0.y1 = hi ? Yg[(int64_t)(s0.c_b.y >> COL_G) * NR + 64 + lane] : (T)0;
   __builtin_amdgcn_wave_barrier();
   read_a<T, W>(s0.a, tbl, code);
   for (int32_t it = ib; it < ie; it += 2) {
@@ -569,19 +581,68 @@ __device__ inline int tile_of_block(int b, int ntiles) {
   return (b - local) + (local % NXCD) * XCD_RUN + local / NXCD;
 }
 
+// Work units: a tile's items are cut into runs of at most `chunk` items, one
+// workgroup each, so that a tile with many sparse groups (a coarse leaf next
+// to a much finer region) does not hold the whole launch back.  Unit k of a
+// tile covers items [k chunk, (k + 1) chunk) of it; the first unit adds its
+// products to L directly, the others leave theirs in part[] (one 128 x 110
+// slab per extra unit, FP64), which k_unit_combine adds in unit order after
+// the launch, so the sums do not depend on timing.
+__global__ void k_unit_counts(const int32_t *tile_off, int32_t ntiles, int32_t chunk,
+                              uint32_t *ucnt) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const int32_t n = tile_off[t + 1] - tile_off[t];
+  ucnt[t] = n > chunk ? (uint32_t)((n + chunk - 1) / chunk) : 1u;
+}
+
+__global__ void k_unit_map(const uint32_t *ucnt, const uint32_t *uoff, int32_t ntiles,
+                           int2 *units) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  for (uint32_t k = 0; k < ucnt[t]; k++) units[uoff[t] + k] = make_int2(t, (int)k);
+}
+
+// slab of extra unit k >= 1 of tile t: every tile has at least one unit
+__device__ inline int64_t extra_slab(const uint32_t *uoff, int32_t t, int k) {
+  return (int64_t)uoff[t] - t + k - 1;
+}
+
+__global__ void k_unit_combine(const uint32_t *ucnt, const uint32_t *uoff, int32_t ntiles,
+                               const double *part, int32_t ncells, double *L) {
+  const int32_t t = blockIdx.x;
+  const int n = (int)ucnt[t];
+  if (n <= 1) return;
+  const int32_t first = t * TILE;
+  for (int e = threadIdx.x; e < TILE * 110; e += blockDim.x) {
+    const int sl = e / 110, row = e - sl * 110;
+    const int32_t ci = first + sl;
+    if (ci >= ncells) continue;
+    double *dst = &L[2 * ((int64_t)ci * NTERM + (row >> 1)) + (row & 1)];
+    double v = *dst;
+    for (int k = 1; k < n; k++) v += part[extra_slab(uoff, t, k) * (TILE * 110) + e];
+    *dst = v;
+  }
+}
+
 template <class T>
 __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
-    const int32_t *tile_off, const uint2 *cols, const T *Yg, const T *Mq,
-    const uint32_t *codes, Cells c, int32_t ncells, double *L) {
+    const int32_t *tile_off, const int2 *units, const uint32_t *ucnt, const uint32_t *uoff,
+    int32_t ntiles, int32_t chunk, const uint2 *cols, const T *Yg, const T *Mq,
+    const uint32_t *codes, Cells c, int32_t ncells, double *L, double *part) {
   __shared__ T Lt[TILE][NR + 1];
   __shared__ T Lt0b[TILE][17];
   __shared__ T Tb[TILE_WAVES][NMULT * NR];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const int u = tile_of_block(blockIdx.x, gridDim.x);
+  if ((uint32_t)u >= uoff[ntiles - 1] + ucnt[ntiles - 1]) return;  // grid is an upper bound
+  const int2 tu = units[u];
+  const int tile = tu.x, k = tu.y;
   for (int i = threadIdx.x; i < TILE * (NR + 1); i += 64 * TILE_WAVES) (&Lt[0][0])[i] = 0;
   for (int i = threadIdx.x; i < TILE * 17; i += 64 * TILE_WAVES) (&Lt0b[0][0])[i] = 0;
   __syncthreads();
-  const int32_t ib = tile_off[tile], ie = tile_off[tile + 1];
+  const int32_t tb = tile_off[tile], te = tile_off[tile + 1];
+  const int32_t ib = tb + k * chunk, ie = min(te, ib + chunk);
   // every wave walks all items of the tile, in order, for its own blocks
   switch (wave) {
     case 0: run_wave<T, 0>(cols, ib, ie, Yg, Mq, codes, Lt, Lt0b, Tb[0], lane); break;
@@ -595,8 +656,10 @@ __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
     Lt[sl][r] += Lt0b[sl][r];
   }
   __syncthreads();
-  // L_j = L~_j / sigma_t^(j+1), added to the target's FP64 local expansion
+  // L_j = L~_j / sigma_t^(j+1): added to the target's FP64 local expansion
+  // by the tile's first unit, kept in the unit's slab by the others
   const int32_t first = tile * TILE;
+  double *slab = k ? part + extra_slab(uoff, tile, k) * (TILE * 110) : nullptr;
   for (int e = threadIdx.x; e < TILE * 110; e += 64 * TILE_WAVES) {
     const int sl = e / 110, row = e - sl * 110;
     const int32_t ci = first + sl;
@@ -605,11 +668,15 @@ __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
     int jj = 0;
     while (tri(jj + 1, 0) <= jk) jj++;
     const double v = (double)Lt[sl][row];
-    if (v == 0) continue;  // nothing arrived (and a zero-width tree must not form 0 * inf)
-    const double isig = 1.0 / c.width[ci];
-    double sc = isig;
-    for (int i = 0; i < jj; i++) sc *= isig;
-    L[2 * ((int64_t)ci * NTERM + jk) + (row & 1)] += v * sc;
+    double out = 0;
+    if (v != 0) {  // (nothing arrived; a zero-width tree must not form 0 * inf)
+      const double isig = 1.0 / c.width[ci];
+      double sc = isig;
+      for (int i = 0; i < jj; i++) sc *= isig;
+      out = v * sc;
+    }
+    if (slab) slab[e] = out;
+    else if (v != 0) L[2 * ((int64_t)ci * NTERM + jk) + (row & 1)] += out;
   }
 }
 
@@ -633,6 +700,7 @@ hipError_t m2l_upload_codes(uint32_t *dev_codes, hipStream_t s) {
 size_t m2l_codes_bytes() { return (size_t)2 * NCODE * 4; }
 
 int32_t m2l_group_slots(int max_depth) { return (max_depth + 1) * SLOTS_PER_LEVEL; }
+int m2l_group_bits() { return 32 - COL_G; }
 
 hipError_t launch_m2l_mark_slots(Cells c, const int2 *pairs, uint32_t n, uint32_t *used,
                                  int32_t *slot_of, hipStream_t s) {
@@ -718,18 +786,49 @@ hipError_t launch_m2l_tile_offsets(const int4 *items, uint32_t nitems, int32_t n
   return hipGetLastError();
 }
 
-hipError_t launch_m2l_tiles(const int32_t *tile_off, int32_t ntiles, const uint2 *cols,
-                            const void *Yg, const void *Mq,
-                            const uint32_t *codes, Cells c, int32_t ncells, double *L, bool fp32,
+int32_t m2l_unit_chunk(uint32_t nitems, int32_t ntiles, double factor) {
+  if (factor <= 0) return INT32_MAX / 2;  // one unit per tile
+  const double mean = ntiles > 0 ? (double)nitems / ntiles : 1.0;
+  return (int32_t)std::min<double>(INT32_MAX / 2, std::max(64.0, std::ceil(mean * factor)));
+}
+
+uint32_t m2l_units_bound(uint32_t nitems, int32_t ntiles, int32_t chunk) {
+  return (uint32_t)ntiles + nitems / (uint32_t)chunk;
+}
+
+size_t m2l_unit_part_bytes(uint32_t units_bound, int32_t ntiles) {
+  const size_t extra = units_bound > (uint32_t)ntiles ? units_bound - (uint32_t)ntiles : 0;
+  return std::max<size_t>(extra, 1) * TILE * 110 * 8;
+}
+
+hipError_t launch_m2l_units(const int32_t *tile_off, int32_t ntiles, int32_t chunk,
+                            uint32_t *ucnt, uint32_t *uoff, int2 *units, void *scan_tmp,
                             hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
+  k_unit_counts<<<(ntiles + 255) / 256, 256, 0, s>>>(tile_off, ntiles, chunk, ucnt);
+  hipError_t e = scan_exclusive_u32(ucnt, uoff, ntiles, scan_tmp, s);
+  if (e != hipSuccess) return e;
+  k_unit_map<<<(ntiles + 255) / 256, 256, 0, s>>>(ucnt, uoff, ntiles, units);
+  return hipGetLastError();
+}
+
+hipError_t launch_m2l_tiles(const int32_t *tile_off, const int2 *units, const uint32_t *ucnt,
+                            const uint32_t *uoff, int32_t ntiles, int32_t chunk,
+                            uint32_t units_bound, const uint2 *cols, const void *Yg,
+                            const void *Mq, const uint32_t *codes, Cells c, int32_t ncells,
+                            double *L, double *part, bool fp32, hipStream_t s) {
+  if (ntiles <= 0) return hipSuccess;
   if (fp32)
-    k_m2l_tiles<float><<<ntiles, 64 * TILE_WAVES, 0, s>>>(tile_off, cols, (const float *)Yg,
-                                                          (const float *)Mq, codes, c, ncells, L);
+    k_m2l_tiles<float><<<units_bound, 64 * TILE_WAVES, 0, s>>>(
+        tile_off, units, ucnt, uoff, ntiles, chunk, cols, (const float *)Yg, (const float *)Mq,
+        codes, c, ncells, L, part);
   else
-    k_m2l_tiles<double><<<ntiles, 64 * TILE_WAVES, 0, s>>>(tile_off, cols, (const double *)Yg,
-                                                           (const double *)Mq,
-                                                           codes + NCODE, c, ncells, L);
+    k_m2l_tiles<double><<<units_bound, 64 * TILE_WAVES, 0, s>>>(
+        tile_off, units, ucnt, uoff, ntiles, chunk, cols, (const double *)Yg,
+        (const double *)Mq, codes + NCODE, c, ncells, L, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_unit_combine<<<ntiles, 256, 0, s>>>(ucnt, uoff, ntiles, part, ncells, L);
   return hipGetLastError();
 }
 

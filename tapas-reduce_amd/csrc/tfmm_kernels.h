@@ -50,7 +50,15 @@ hipError_t launch_permute(const uint32_t *order, const double *x, const double *
                           const double *z, const double *q, int64_t n, double *sx, double *sy,
                           double *sz, double *sq, int32_t *perm, hipStream_t s);
 hipError_t launch_roots(Cells c, int64_t n, const int64_t *off, int nprob,
-                        const double *regions, hipStream_t s);
+                        const double *regions, int32_t *lev, hipStream_t s);
+// one level of the cell build with sizes on the device (lev: first cell and
+// count per level); grid_cells bounds the level's size; status bit 1: cell
+// capacity cap exceeded, bit 2: level larger than grid_cells
+int32_t level_blocks(int32_t cells);
+hipError_t launch_level_step(Cells c, int32_t *lev, int d, int32_t grid_cells, int ncrit,
+                             int32_t cap, uint32_t *flag, uint32_t *loc, uint32_t *bsum,
+                             uint32_t *status, const uint64_t *keys, const double *regions,
+                             const uint32_t *sprob, hipStream_t s);
 hipError_t launch_split_flags(Cells c, int32_t lbeg, int32_t lcnt, int ncrit, uint32_t *flag,
                               hipStream_t s);
 hipError_t launch_children(Cells c, int32_t lbeg, int32_t lcnt, const uint32_t *flag,
@@ -75,10 +83,13 @@ struct DttCounters {
   // [3] all M2L pairs (also those with an empty cell)
   uint32_t *cnt;
 };
+// per-cell records the traversal steps read (geo: centre + MAC radius)
+hipError_t launch_dtt_cells(Cells c, int32_t ncells, double theta, double4 *geo, int2 *topo,
+                            hipStream_t s);
 hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin_max, int2 *out,
                            uint32_t out_cap, uint32_t *nout, int2 *m2l, uint32_t m2l_cap,
-                           int2 *p2p, uint32_t p2p_cap, uint32_t *cnt, Cells c, double theta,
-                           hipStream_t s);
+                           int2 *p2p, uint32_t p2p_cap, uint32_t *cnt, const double4 *geo,
+                           const int2 *topo, hipStream_t s);
 hipError_t launch_pack_pairs(const int2 *pairs, uint32_t n, int sbits, uint64_t *keys,
                              uint32_t *idx, hipStream_t s);
 hipError_t launch_unpack_pairs(const uint64_t *keys, uint32_t n, int sbits, int32_t *src,
@@ -103,6 +114,7 @@ hipError_t launch_direct_fp64(const double *x, const double *y, const double *z,
 
 namespace tfmm {
 // m2l_batched.hip
+int m2l_group_bits();  // bits of a group index in the tile kernel's column word
 int32_t m2l_group_slots(int max_depth);
 int m2l_tile_size();
 int m2l_src_bits();
@@ -130,10 +142,18 @@ hipError_t launch_m2l_write_items(const uint32_t *keys, const uint32_t *vals, ui
                                   uint2 *cols, hipStream_t s);
 hipError_t launch_m2l_tile_offsets(const int4 *items, uint32_t nitems, int32_t ntiles,
                                    int32_t *off, hipStream_t s);
-hipError_t launch_m2l_tiles(const int32_t *tile_off, int32_t ntiles, const uint2 *cols,
-                            const void *Yg, const void *Mq,
-                            const uint32_t *codes, Cells c, int32_t ncells, double *L, bool fp32,
+// work units of the tile kernel: runs of at most `chunk` items of one tile
+int32_t m2l_unit_chunk(uint32_t nitems, int32_t ntiles, double factor);
+uint32_t m2l_units_bound(uint32_t nitems, int32_t ntiles, int32_t chunk);
+size_t m2l_unit_part_bytes(uint32_t units_bound, int32_t ntiles);
+hipError_t launch_m2l_units(const int32_t *tile_off, int32_t ntiles, int32_t chunk,
+                            uint32_t *ucnt, uint32_t *uoff, int2 *units, void *scan_tmp,
                             hipStream_t s);
+hipError_t launch_m2l_tiles(const int32_t *tile_off, const int2 *units, const uint32_t *ucnt,
+                            const uint32_t *uoff, int32_t ntiles, int32_t chunk,
+                            uint32_t units_bound, const uint2 *cols, const void *Yg,
+                            const void *Mq, const uint32_t *codes, Cells c, int32_t ncells,
+                            double *L, double *part, bool fp32, hipStream_t s);
 
 // p2p.hip (mixed precision)
 hipError_t launch_leaf_relpos(Cells c, int32_t ncells, Bodies b, float4 *rel, hipStream_t s);

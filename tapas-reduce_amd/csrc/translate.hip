@@ -193,19 +193,36 @@ __global__ __launch_bounds__(7 * 64) void k_m2m_mfma(Cells c, int32_t first_chil
     case 5: kend = m2m_kend(5); break;
     default: kend = m2m_kend(6); break;
   }
+  // Two operand sets in flight: the loads of k-step k0 + 4 are issued before
+  // step k0 is multiplied, into the other register set (no register moves,
+  // which would wait for the loads).  Loads are unconditional from clamped
+  // indices and masked afterwards: loads under a branch make every k-step
+  // wait for all loads in flight.  On the small upper levels a launch is one
+  // block, and its time is the load latency of its k-steps.
   d4 acc = {0, 0, 0, 0};
-  for (int k0 = 0; k0 < kend; k0 += 4) {
+  double a0[8], b0[8], a1[8], b1[8];
+  auto load = [&](int k0, double (&ao)[8], double (&bo)[8]) {
     const int kc = k0 + kq;
     const bool in = kc < kend;
-    const double f = cf[col][kc];
-    double a[8], b[8];
+    const int kk = in ? kc : 0;
+    // masks as factors (the operands are finite), so no load sits under a branch
+    const double f = in ? cf[col][kk] : 0.0, ma = arow_ok ? 1.0 : 0.0;
 #pragma unroll
     for (int o = 0; o < 8; o++) {
-      a[o] = (in && arow_ok) ? A[o * NV + kc] : 0.0;
-      b[o] = in ? B[o * NV + kc] * f : 0.0;
+      ao[o] = A[o * NV + kk] * ma;
+      bo[o] = B[o * NV + kk] * f;
     }
+  };
+  auto mma = [&](const double (&ao)[8], const double (&bo)[8]) {
 #pragma unroll
-    for (int o = 0; o < 8; o++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[o], b[o], acc, 0, 0, 0);
+    for (int o = 0; o < 8; o++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ao[o], bo[o], acc, 0, 0, 0);
+  };
+  load(0, a0, b0);
+  for (int k0 = 0; k0 < kend; k0 += 8) {
+    load(k0 + 4, a1, b1);
+    mma(a0, b0);
+    load(k0 + 8, a0, b0);
+    mma(a1, b1);
   }
   if (!valid) return;
   const int32_t p = c.parent[c0];
@@ -244,19 +261,32 @@ __global__ __launch_bounds__(8 * 64) void k_l2l_mfma(Cells c, int32_t first_chil
   d4 acc[7];
 #pragma unroll
   for (int rb = 0; rb < 7; rb++) acc[rb] = (d4){0, 0, 0, 0};
-  for (int k0 = 0; k0 < NV; k0 += 4) {
+  // two operand sets in flight, unconditional clamped loads (as in M2M)
+  double a0[7], a1[7], b0, b1;
+  auto load = [&](int k0, double (&ao)[7], double &bo) {
     const int kc = k0 + kq;
     const bool in = kc < NV;
-    const double b = in ? B[kc] * cf[col][kc] : 0.0;
-    double a[7];
+    const int kk = in ? kc : 0;
+    const double bv = B[kk] * cf[col][kk];
+    bo = in ? bv : 0.0;
 #pragma unroll
     for (int rb = 0; rb < 7; rb++) {
       const int arow = rb * 16 + col;
-      a[rb] = (in && arow < NV && k0 >= l2l_kbeg(rb)) ? S[(int64_t)arow * NV + kc] : 0.0;
+      const double av = S[(int64_t)min(arow, NV - 1) * NV + kk];
+      ao[rb] = (in && arow < NV && k0 >= l2l_kbeg(rb)) ? av : 0.0;
     }
+  };
+  auto mma = [&](int k0, const double (&ao)[7], double bo) {
 #pragma unroll
     for (int rb = 0; rb < 7; rb++)
-      if (k0 >= l2l_kbeg(rb)) acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b, acc[rb], 0, 0, 0);
+      if (k0 >= l2l_kbeg(rb)) acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ao[rb], bo, acc[rb], 0, 0, 0);
+  };
+  load(0, a0, b0);
+  for (int k0 = 0; k0 < NV; k0 += 8) {
+    load(k0 + 4, a1, b1);
+    mma(k0, a0, b0);
+    load(k0 + 8, a0, b0);
+    mma(k0 + 4, a1, b1);
   }
   if (!valid) return;
   const int64_t cidx = c0 + oct;

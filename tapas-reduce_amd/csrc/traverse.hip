@@ -20,6 +20,11 @@
 // need no host round trip.  The
 // MAC is evaluated in FP64 without contraction, in the reference's operation
 // order, so every decision is bit-identical to the reference's.
+//
+// The step reads each cell through one 32-byte record (centre, MAC radius R)
+// and one 8-byte record (first child, body count), built once per traversal,
+// instead of six separate arrays: the traversal gathers cells at random, so
+// the number of cache lines a pair touches sets its cost.
 #include "tfmm_device.h"
 #include "tfmm_kernels.h"
 
@@ -29,33 +34,48 @@ namespace {
 constexpr int DTT_THREADS = 1024;
 constexpr int DTT_WAVES = DTT_THREADS / 64;
 
+// per-cell traversal records: (centre, R) with R = (width / 2 * 1.00001) / theta
+// (tapas_exafmm.cxx:222-226), and (first child, body count)
+__global__ void k_dtt_cells(Cells c, int32_t ncells, double theta, double4 *geo, int2 *topo) {
+#pragma clang fp contract(off)
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncells) return;
+  double R = c.width[i];
+  R = (R / 2 * 1.00001) / theta;
+  geo[i] = make_double4(c.center[3 * (int64_t)i], c.center[3 * (int64_t)i + 1],
+                        c.center[3 * (int64_t)i + 2], R);
+  topo[i] = make_int2(c.child[i], c.nb[i]);
+}
+
 __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const uint32_t *nin_dev,
                                                   uint32_t nin_max, int2 *out, uint32_t out_cap,
                                                   uint32_t *nout, int2 *m2l, uint32_t m2l_cap,
                                                   int2 *p2p, uint32_t p2p_cap, uint32_t *cnt,
-                                                  Cells c, double theta) {
+                                                  const double4 *geo, const int2 *topo) {
 #pragma clang fp contract(off)
   const uint32_t nin = nin_dev ? *nin_dev : nin_max;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = t < nin && t < nin_max;
   int32_t i = 0, j = 0;
+  int2 ti = make_int2(0, 0), tj = make_int2(0, 0);
   bool far = false, p2p_pair = false, split_i = false, split_j = false, nonempty = false;
   if (valid) {
     const int2 pr = in[t];
     i = pr.x;
     j = pr.y;
-    const double d0 = c.center[3 * (int64_t)i] - c.center[3 * (int64_t)j];
-    const double d1 = c.center[3 * (int64_t)i + 1] - c.center[3 * (int64_t)j + 1];
-    const double d2 = c.center[3 * (int64_t)i + 2] - c.center[3 * (int64_t)j + 2];
+    const double4 gi = geo[i], gj = geo[j];
+    ti = topo[i];
+    tj = topo[j];
+    const double d0 = gi.x - gj.x;
+    const double d1 = gi.y - gj.y;
+    const double d2 = gi.z - gj.z;
     double R2 = 0;
     R2 += d0 * d0;
     R2 += d1 * d1;
     R2 += d2 * d2;
-    double Ri = c.width[i], Rj = c.width[j];
-    Ri = (Ri / 2 * 1.00001) / theta;
-    Rj = (Rj / 2 * 1.00001) / theta;
-    const bool leaf_i = c.child[i] < 0, leaf_j = c.child[j] < 0;
-    nonempty = c.nb[i] > 0 && c.nb[j] > 0;
+    const double Ri = gi.w, Rj = gj.w;
+    const bool leaf_i = ti.x < 0, leaf_j = tj.x < 0;
+    nonempty = ti.y > 0 && tj.y > 0;
     if (R2 > (Ri + Rj) * (Ri + Rj)) {
       far = true;
     } else if (leaf_i && leaf_j) {
@@ -82,8 +102,8 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
   for (int q = 0; q < 4; q++) m[q] = __ballot(pred[q]);
   unsigned long long v = 0, self = 0;
   if (p2p_pair) {
-    v = (unsigned long long)c.nb[i] * (unsigned long long)c.nb[j];
-    if (i == j) self = (unsigned long long)c.nb[i];
+    v = (unsigned long long)ti.y * (unsigned long long)tj.y;
+    if (i == j) self = (unsigned long long)ti.y;
   }
   for (int o = 32; o > 0; o >>= 1) {
     v += __shfl_xor(v, o, 64);
@@ -126,10 +146,10 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
   if (p2p_pair && slot[2] < p2p_cap) p2p[slot[2]] = make_int2(i, j);
   if (split && slot[0] + 8 <= out_cap) {
     if (split_i) {
-      const int32_t f = c.child[i];
+      const int32_t f = ti.x;
       for (int k = 0; k < 8; k++) out[slot[0] + k] = make_int2(f + k, j);
     } else {
-      const int32_t f = c.child[j];
+      const int32_t f = tj.x;
       for (int k = 0; k < 8; k++) out[slot[0] + k] = make_int2(i, f + k);
     }
   }
@@ -167,13 +187,20 @@ __global__ void k_csr_offsets(const uint64_t *keys, uint32_t n, int sbits, int32
 
 }  // namespace
 
+hipError_t launch_dtt_cells(Cells c, int32_t ncells, double theta, double4 *geo, int2 *topo,
+                            hipStream_t s) {
+  if (ncells <= 0) return hipSuccess;
+  k_dtt_cells<<<(ncells + 255) / 256, 256, 0, s>>>(c, ncells, theta, geo, topo);
+  return hipGetLastError();
+}
+
 hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin_max, int2 *out,
                            uint32_t out_cap, uint32_t *nout, int2 *m2l, uint32_t m2l_cap,
-                           int2 *p2p, uint32_t p2p_cap, uint32_t *cnt, Cells c, double theta,
-                           hipStream_t s) {
+                           int2 *p2p, uint32_t p2p_cap, uint32_t *cnt, const double4 *geo,
+                           const int2 *topo, hipStream_t s) {
   if (nin_max == 0) return hipSuccess;
   k_dtt_step<<<(nin_max + DTT_THREADS - 1) / DTT_THREADS, DTT_THREADS, 0, s>>>(
-      in, nin_dev, nin_max, out, out_cap, nout, m2l, m2l_cap, p2p, p2p_cap, cnt, c, theta);
+      in, nin_dev, nin_max, out, out_cap, nout, m2l, m2l_cap, p2p, p2p_cap, cnt, geo, topo);
   return hipGetLastError();
 }
 

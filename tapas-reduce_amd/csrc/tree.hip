@@ -190,8 +190,12 @@ __global__ void k_permute(const uint32_t *order, const double *x, const double *
 
 // Root cells: one per problem (off == nullptr: the single root 0 of n bodies).
 __global__ void k_roots(Cells c, int64_t n, const int64_t *off, int nprob,
-                        const double *regions) {
+                        const double *regions, int32_t *lev) {
   const int pb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pb == 0 && lev) {
+    lev[0] = 0;
+    lev[1] = nprob;
+  }
   if (pb >= nprob) return;
   const int64_t b = off ? off[pb] : 0, e = off ? off[pb + 1] : n;
   c.key[pb] = 0;
@@ -236,17 +240,9 @@ __device__ inline int32_t lower_bound_keys(const uint64_t *k, int32_t lo, int32_
   return lo;
 }
 
-__global__ void k_children(Cells c, int32_t lbeg, int32_t lcnt, const uint32_t *flag,
-                           const uint32_t *slot, int32_t next_beg, const uint64_t *keys,
-                           const double *regions, const uint32_t *sprob) {
-  // one thread per (parent, child) -> 8 threads per parent
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)lcnt * 8) return;
-  const int32_t i = (int32_t)(t >> 3);
-  const int ch = (int)(t & 7);
-  if (!flag[i]) return;
-  const int32_t p = lbeg + i;
-  const int32_t first = next_beg + 8 * (int32_t)slot[i];
+// child ch of split parent p, at cell index first + ch
+__device__ inline void make_child(Cells c, int32_t p, int ch, int32_t first, const uint64_t *keys,
+                                  const double *regions, const uint32_t *sprob) {
   const int32_t ci = first + ch;
   const uint64_t pk = c.key[p];
   const uint64_t ck = key_child(pk, ch);
@@ -273,6 +269,115 @@ __global__ void k_children(Cells c, int32_t lbeg, int32_t lcnt, const uint32_t *
   }
   c.width[ci] = w;
   if (ch == 0) c.child[p] = first;
+}
+
+__global__ void k_children(Cells c, int32_t lbeg, int32_t lcnt, const uint32_t *flag,
+                           const uint32_t *slot, int32_t next_beg, const uint64_t *keys,
+                           const double *regions, const uint32_t *sprob) {
+  // one thread per (parent, child) -> 8 threads per parent
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)lcnt * 8) return;
+  const int32_t i = (int32_t)(t >> 3);
+  const int ch = (int)(t & 7);
+  if (!flag[i]) return;
+  make_child(c, lbeg + i, ch, next_beg + 8 * (int32_t)slot[i], keys, regions, sprob);
+}
+
+// ---------------------------------------------------------------------------
+// The same level step with every size on the device (no host round trip per
+// level): lev[2 d], lev[2 d + 1] = first cell and cell count of level d.  The
+// grids come from the sizes of an earlier tree; the host checks afterwards
+// that every level fitted its grid and the cell capacity (status bits), and
+// builds again the synchronous way if not.
+constexpr int LV_BLOCK = 256;
+
+// split flags, the exclusive prefix of the flags inside each 256-cell block,
+// and the block totals
+__global__ __launch_bounds__(LV_BLOCK) void k_level_flags(Cells c, const int32_t *lev, int d,
+                                                          int ncrit, uint32_t *flag,
+                                                          uint32_t *loc, uint32_t *bsum) {
+  const int32_t lbeg = lev[2 * d], lcnt = lev[2 * d + 1];
+  const int32_t i = blockIdx.x * LV_BLOCK + threadIdx.x;
+  bool split = false;
+  if (i < lcnt) {
+    const int32_t ci = lbeg + i;
+    split = c.nb[ci] > ncrit && c.depth[ci] < MAXDEPTH;
+    if (!split) c.child[ci] = -1;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t m = __ballot(split);
+  __shared__ uint32_t wsum[LV_BLOCK / 64];
+  if (lane == 0) wsum[wid] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+  for (int w = 0; w < LV_BLOCK / 64; w++) {
+    pre += w < wid ? wsum[w] : 0u;
+    tot += wsum[w];
+  }
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  if (i < lcnt) {
+    flag[i] = split ? 1u : 0u;
+    loc[i] = pre + (uint32_t)__popcll(m & lt);
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// one block: exclusive scan of the block totals, the next level's extent;
+// status bit 1: cell capacity exceeded, bit 2: level larger than its grid
+__global__ __launch_bounds__(1024) void k_level_scan(int32_t *lev, int d, uint32_t *bsum,
+                                                     int32_t nblk, int32_t cap,
+                                                     uint32_t *status) {
+  const int32_t lbeg = lev[2 * d], lcnt = lev[2 * d + 1];
+  const int32_t nb = min(nblk, (lcnt + LV_BLOCK - 1) / LV_BLOCK);
+  __shared__ uint32_t part[1024];
+  // each thread scans a contiguous run of the block totals
+  const int per = (nb + 1023) / 1024;
+  const int b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
+  uint32_t s = 0;
+  for (int b = b0; b < b1; b++) s += bsum[b];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const uint32_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[threadIdx.x] - s;
+  for (int b = b0; b < b1; b++) {
+    const uint32_t v = bsum[b];
+    bsum[b] = run;
+    run += v;
+  }
+  if (threadIdx.x == 0) {
+    const uint32_t total = part[1023];
+    const int64_t next_beg = (int64_t)lbeg + lcnt;
+    int64_t next_cnt = 8 * (int64_t)total;
+    uint32_t st = 0;
+    if (lcnt > nblk * LV_BLOCK) st |= 2u;
+    if (next_beg + next_cnt > cap) st |= 1u;
+    if (st) {
+      atomicOr(status, st);
+      next_cnt = 0;  // stop: this tree is built again on the host's path
+    }
+    lev[2 * (d + 1)] = (int32_t)next_beg;
+    lev[2 * (d + 1) + 1] = (int32_t)next_cnt;
+  }
+}
+
+__global__ void k_level_children(Cells c, const int32_t *lev, int d, const uint32_t *flag,
+                                 const uint32_t *loc, const uint32_t *bsum,
+                                 const uint64_t *keys, const double *regions,
+                                 const uint32_t *sprob) {
+  const int32_t lbeg = lev[2 * d], lcnt = lev[2 * d + 1];
+  const int32_t next_beg = lev[2 * (d + 1)], next_cnt = lev[2 * (d + 1) + 1];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (next_cnt == 0 || t >= (int64_t)lcnt * 8) return;
+  const int32_t i = (int32_t)(t >> 3);
+  const int ch = (int)(t & 7);
+  if (!flag[i]) return;
+  const uint32_t slot = bsum[i / LV_BLOCK] + loc[i];
+  make_child(c, lbeg + i, ch, next_beg + 8 * (int32_t)slot, keys, regions, sprob);
 }
 
 }  // namespace
@@ -331,8 +436,24 @@ hipError_t launch_permute(const uint32_t *order, const double *x, const double *
 }
 
 hipError_t launch_roots(Cells c, int64_t n, const int64_t *off, int nprob,
-                        const double *regions, hipStream_t s) {
-  k_roots<<<(nprob + 255) / 256, 256, 0, s>>>(c, n, off, nprob, regions);
+                        const double *regions, int32_t *lev, hipStream_t s) {
+  k_roots<<<(nprob + 255) / 256, 256, 0, s>>>(c, n, off, nprob, regions, lev);
+  return hipGetLastError();
+}
+
+int32_t level_blocks(int32_t cells) { return (cells + LV_BLOCK - 1) / LV_BLOCK; }
+
+hipError_t launch_level_step(Cells c, int32_t *lev, int d, int32_t grid_cells, int ncrit,
+                             int32_t cap, uint32_t *flag, uint32_t *loc, uint32_t *bsum,
+                             uint32_t *status, const uint64_t *keys, const double *regions,
+                             const uint32_t *sprob, hipStream_t s) {
+  const int32_t nblk = level_blocks(grid_cells);
+  if (nblk <= 0) return hipSuccess;
+  k_level_flags<<<nblk, LV_BLOCK, 0, s>>>(c, lev, d, ncrit, flag, loc, bsum);
+  k_level_scan<<<1, 1024, 0, s>>>(lev, d, bsum, nblk, cap, status);
+  const int64_t nt = (int64_t)nblk * LV_BLOCK * 8;
+  k_level_children<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(c, lev, d, flag, loc, bsum, keys,
+                                                                 regions, sprob);
   return hipGetLastError();
 }
 

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libtfmm.so")
 TFMM_OK = 0
 FP64 = 0
 MIXED = 1
+TUNE_M2L_UNIT = 1
 P = 10
 NTERM = 55
 
@@ -64,7 +65,8 @@ class Stats(ctypes.Structure):
 EXPORTED = [
     "tfmm_create", "tfmm_destroy", "tfmm_last_error", "tfmm_stream", "tfmm_sync",
     "tfmm_set_bodies", "tfmm_set_bodies_device", "tfmm_set_batch", "tfmm_set_batch_device",
-    "tfmm_batch_evaluate", "tfmm_num_problems", "tfmm_build_tree", "tfmm_set_params",
+    "tfmm_batch_evaluate", "tfmm_num_problems", "tfmm_set_tuning", "tfmm_get_tuning",
+    "tfmm_build_tree", "tfmm_set_params",
     "tfmm_upward",
     "tfmm_traverse", "tfmm_downward", "tfmm_evaluate", "tfmm_get_fields", "tfmm_get_stats",
     "tfmm_get_sorted", "tfmm_get_cells", "tfmm_get_expansions", "tfmm_get_lists", "tfmm_direct",
@@ -101,6 +103,8 @@ def lib():
         L.tfmm_batch_evaluate.argtypes = [vp, ctypes.c_int, ctypes.POINTER(BodyDesc),
                                           ctypes.POINTER(vp), ctypes.POINTER(Params)]
         L.tfmm_num_problems.argtypes = [vp, vp]
+        L.tfmm_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.tfmm_get_tuning.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.tfmm_build_tree.argtypes = [vp, ctypes.POINTER(Params)]
         L.tfmm_evaluate.argtypes = [vp, ctypes.POINTER(Params)]
         L.tfmm_set_params.argtypes = [vp, ctypes.POINTER(Params)]
@@ -229,6 +233,14 @@ class Context:
         self._check(lib().tfmm_set_batch_device(self._h, len(off) - 1, _ptr(off), px, py, pz,
                                                 pq), "tfmm_set_batch_device")
         self.n = int(off[-1])
+
+    def set_tuning(self, knob, value):
+        self._check(lib().tfmm_set_tuning(self._h, int(knob), float(value)), "tfmm_set_tuning")
+
+    def get_tuning(self, knob):
+        v = ctypes.c_double()
+        self._check(lib().tfmm_get_tuning(self._h, int(knob), ctypes.byref(v)), "tfmm_get_tuning")
+        return v.value
 
     def problem_offsets(self):
         n = lib().tfmm_num_problems(self._h, None)

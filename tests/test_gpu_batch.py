@@ -1,7 +1,9 @@
 """Batched independent problems evaluated as one forest (through the C ABI):
-the batch gives, problem by problem, bit for bit what evaluating each problem
-alone gives -- keys, cells, interaction counts and fields -- in FP64 and in
-mixed precision, and the FP64 batch matches the CPU oracle."""
+the batch gives, problem by problem, what evaluating each problem alone
+gives -- keys, cells and interaction counts bit for bit, fields up to the
+summation order of the M2L work units (which depends on how many items the
+whole forest has per tile): relative L2 1e-12 in FP64, 1e-6 in mixed
+precision -- and the FP64 batch matches the CPU oracle."""
 import numpy as np
 import pytest
 
@@ -53,7 +55,13 @@ def test_gpu_batch_matches_single_problems(ctx, precision):
         f = ctx.fields()
         s = ctx.stats()
         k, perm = ctx.sorted()
-        assert np.array_equal(fb[off[i]:off[i + 1]], f), "fields of problem %d" % i
+        seg = fb[off[i]:off[i + 1]]
+        if not np.any(f):  # a lone body feels nothing
+            assert not np.any(seg)
+        else:
+            ep, ea = oracle.rel_l2(seg, f)
+            tol = 1e-12 if precision == tfmm.FP64 else 1e-6
+            assert ep < tol and ea < tol, ("fields of problem %d" % i, ep, ea)
         assert np.array_equal(keys_b[off[i]:off[i + 1]], k)
         assert np.array_equal(np.sort(perm_b[off[i]:off[i + 1]] - off[i]), np.arange(len(k)))
         ncells += s["ncells"]
