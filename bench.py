@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--targets", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=524288)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tile-kernel knobs instead of autotuning them")
     ap.add_argument("--unit-factor", type=float, default=None,
                     help="M2L work-unit size in mean items per tile (0: one unit per tile)")
     ap.add_argument("--batch", type=int, default=0,
@@ -204,6 +206,15 @@ def main():
         pk.close()
     prm = tfmm.make_params(ncrit, theta, precision)
 
+    # tile-kernel knobs: the library's autotuner on these bodies (untimed)
+    tuned = None
+    if not args.no_autotune and args.unit_factor is None:
+        t0 = time.perf_counter()
+        unit, run, tms = ctx.autotune(prm, reps=3)
+        tuned = {"m2l_unit_factor": unit, "m2l_xcd_run": run, "m2l_tiles_ms": tms,
+                 "seconds": time.perf_counter() - t0,
+                 "note": "16 candidate settings x 3 evaluations, outside the timed steps"}
+
     # accuracy reference: FP64 direct sum on random targets (untimed)
     rng = np.random.default_rng(123)
     targets = np.sort(rng.choice(args.n, size=min(args.targets, args.n), replace=False))
@@ -225,6 +236,8 @@ def main():
     fp64 = None
     if args.fp64_steps > 0 and precision == tfmm.MIXED:
         prm64 = tfmm.make_params(args.ncrit, args.theta, tfmm.FP64)
+        ctx.set_tuning(tfmm.TUNE_M2L_UNIT, 0)  # the plain form keeps the default knobs
+        ctx.set_tuning(tfmm.TUNE_M2L_XCD_RUN, 32)
         ms64, ph64, st64 = time_config(ctx, prm64, args.fp64_steps, 1)
         e64 = errors(ctx, targets, direct)
         fp64 = {"value": args.n / (ms64 * 1e-3), "ms_per_step": ms64, "ncrit": args.ncrit,
@@ -270,6 +283,7 @@ def main():
         "error": {"rel_l2_pot": err_pot, "rel_l2_acc": err_acc, "targets": len(targets),
                   "against": "FP64 direct sum"},
         "picker": picked,
+        "autotune": tuned,
         "phases_ms": phases,
         "counts": {"cells": st["ncells"], "leaves": st["nleaves"], "max_depth": st["max_depth"],
                    "m2l_pairs": st["n_m2l_exec"], "m2l_groups": st["n_m2l_groups"],

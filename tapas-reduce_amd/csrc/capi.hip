@@ -60,7 +60,7 @@ struct tfmm_ctx {
   int stage = 0;  // 0 none, 1 tree, 2 upward, 3 traverse, 4 downward
 
   // tree
-  DevBuf keys, keys_alt, idx, idx_alt, sort_tmp, sx, sy, sz, sq, perm, region, bbox_part;
+  DevBuf keys, keys_alt, idx, idx_alt, sort_tmp, sx, sy, sz, sq, perm, region, bbox_part, brec;
   DevBuf c_key, c_beg, c_nb, c_parent, c_child, c_depth, c_center, c_width;
   size_t cell_cap = 0;
   DevBuf flag, slot, scan_tmp, lev, lev_bsum, lev_status;
@@ -92,6 +92,7 @@ struct tfmm_ctx {
   DevBuf fb_off, fb_src, fb_part, relpos, leaf_of, tops, slot_of;
   DevBuf u_cnt, u_off, units, u_part;
   double m2l_unit_factor = 0.0;  // tile work-unit size in mean items per tile (0: off)
+  int m2l_xcd_run = 32;          // consecutive work units dealt to one XCD
   int32_t unit_chunk = 0;
   uint32_t units_bound = 0;
   bool codes_ready = false;
@@ -290,8 +291,10 @@ int do_build_tree(tfmm_ctx *c) {
     TRY(launch_bbox(c->bx, c->by, c->bz, n, dp<double>(c->bbox_part), nparts,
                     dp<double>(c->region), s));
   }
-  TRY(launch_keys(c->bx, c->by, c->bz, n, dp<double>(c->region), off, np, dp<uint64_t>(c->keys),
-                  dp<uint32_t>(c->idx), forest ? dp<uint32_t>(c->bprob) : nullptr, s));
+  TRY(dev_reserve(c->brec, n * 32));
+  TRY(launch_keys(c->bx, c->by, c->bz, c->bq, n, dp<double>(c->region), off, np,
+                  dp<uint64_t>(c->keys), dp<uint32_t>(c->idx),
+                  forest ? dp<uint32_t>(c->bprob) : nullptr, dp<double4>(c->brec), s));
   bool alt = false;
   TRY(radix_sort_pairs(dp<uint64_t>(c->keys), dp<uint32_t>(c->idx), dp<uint64_t>(c->keys_alt),
                        dp<uint32_t>(c->idx_alt), n, DEPTHBITS, DEPTHBITS + 3 * MAXDEPTH,
@@ -320,7 +323,7 @@ int do_build_tree(tfmm_ctx *c) {
   c->keys_in_alt = alt;
   const uint64_t *skeys = dp<uint64_t>(alt ? c->keys_alt : c->keys);
   const uint32_t *order = dp<uint32_t>(alt ? c->idx_alt : c->idx);
-  TRY(launch_permute(order, c->bx, c->by, c->bz, c->bq, n, dp<double>(c->sx), dp<double>(c->sy),
+  TRY(launch_permute(order, dp<double4>(c->brec), n, dp<double>(c->sx), dp<double>(c->sy),
                      dp<double>(c->sz), dp<double>(c->sq), dp<int32_t>(c->perm), s));
 
   // level-synchronous cell build: without a host round trip per level when
@@ -656,7 +659,8 @@ int do_traverse(tfmm_ctx *c) {
   const int32_t tile = m2l_tile_size();
   const int32_t ntiles = (int32_t)((c->ncells + tile - 1) / tile);
   TRY(launch_m2l_tiles(dp<int32_t>(c->tile_off), dp<int2>(c->units), dp<uint32_t>(c->u_cnt),
-                       dp<uint32_t>(c->u_off), ntiles, c->unit_chunk, c->units_bound,
+                       dp<uint32_t>(c->u_off), ntiles, c->unit_chunk, c->m2l_xcd_run,
+                       c->units_bound,
                        dp<uint2>(c->cols),
                        mixed ? (const void *)c->g_y32.ptr : (const void *)c->g_y64.ptr,
                        c->mq.ptr, dp<uint32_t>(c->codes), k, (int32_t)c->ncells,
@@ -770,15 +774,73 @@ int tfmm_set_tuning(tfmm_ctx *c, int knob, double value) {
       if (!(value >= 0) || !(value <= 1e6)) return fail(c, TFMM_EINVAL, "unit factor out of range");
       c->m2l_unit_factor = value;
       return TFMM_OK;
+    case TFMM_TUNE_M2L_XCD_RUN:
+      if (!(value >= 1) || !(value <= 4096) || value != std::floor(value))
+        return fail(c, TFMM_EINVAL, "XCD run must be an integer in [1, 4096]");
+      c->m2l_xcd_run = (int)value;
+      return TFMM_OK;
     default:
       return fail(c, TFMM_EINVAL, "unknown tuning knob");
   }
+}
+
+// Candidates of the knobs, tried jointly; the tile kernel's time (HIP events
+// around its launch) decides, median over the repetitions.
+int tfmm_autotune(tfmm_ctx *c, const tfmm_params *p, int reps, double *chosen) {
+  if (!c) return TFMM_EINVAL;
+  if (reps < 1 || reps > 64) return fail(c, TFMM_EINVAL, "reps must be in [1, 64]");
+  static const double kUnit[] = {0.0, 1.5, 2.0, 3.0};
+  static const int kRun[] = {8, 16, 32, 64};
+  const double unit0 = c->m2l_unit_factor;
+  const int run0 = c->m2l_xcd_run;
+  struct Cand { double unit; int run; std::vector<double> ms; };
+  std::vector<Cand> cand;
+  for (double u : kUnit)
+    for (int r : kRun) cand.push_back({u, r, {}});
+  int rc = tfmm_evaluate(c, p);  // sizes the buffers
+  for (int rep = 0; rep < reps && rc == TFMM_OK; rep++) {
+    for (Cand &k : cand) {
+      c->m2l_unit_factor = k.unit;
+      c->m2l_xcd_run = k.run;
+      rc = tfmm_evaluate(c, p);
+      if (rc) break;
+      tfmm_stats_t st;
+      rc = tfmm_get_stats(c, &st);
+      if (rc) break;
+      k.ms.push_back(st.ms_m2l_tiles);
+    }
+  }
+  if (rc) {
+    c->m2l_unit_factor = unit0;
+    c->m2l_xcd_run = run0;
+    return rc;
+  }
+  size_t best = 0;
+  double best_ms = 1e300;
+  for (size_t i = 0; i < cand.size(); i++) {
+    std::vector<double> v = cand[i].ms;
+    std::sort(v.begin(), v.end());
+    const double med = v[v.size() / 2];
+    if (med < best_ms) {
+      best_ms = med;
+      best = i;
+    }
+  }
+  c->m2l_unit_factor = cand[best].unit;
+  c->m2l_xcd_run = cand[best].run;
+  if (chosen) {
+    chosen[0] = cand[best].unit;
+    chosen[1] = cand[best].run;
+    chosen[2] = best_ms;
+  }
+  return TFMM_OK;
 }
 
 int tfmm_get_tuning(tfmm_ctx *c, int knob, double *value) {
   if (!c || !value) return TFMM_EINVAL;
   switch (knob) {
     case TFMM_TUNE_M2L_UNIT: *value = c->m2l_unit_factor; return TFMM_OK;
+    case TFMM_TUNE_M2L_XCD_RUN: *value = c->m2l_xcd_run; return TFMM_OK;
     default: return fail(c, TFMM_EINVAL, "unknown tuning knob");
   }
 }

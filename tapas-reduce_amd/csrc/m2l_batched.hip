@@ -572,22 +572,26 @@ __device__ inline void run_wave(const uint2 *cols, int32_t ib, int32_t ie, const
 // run of 256 tiles is dealt out as 8 blocks of 32 consecutive tiles, one per
 // XCD: the tiles an XCD runs at the same time read the same sources from
 // its L2.  The last partial run keeps the identity order (a bijection).
+// The run length is a tuning knob (32 by default).
 constexpr int NXCD = 8;
-constexpr int XCD_RUN = 32;
-__device__ inline int tile_of_block(int b, int ntiles) {
-  constexpr int S = NXCD * XCD_RUN;
-  if (b >= (ntiles / S) * S) return b;
+__device__ inline int tile_of_block(int b, int ntiles, int run) {
+  const int S = NXCD * run;
+  if (run <= 1 || b >= (ntiles / S) * S) return b;
   const int local = b % S;
-  return (b - local) + (local % NXCD) * XCD_RUN + local / NXCD;
+  return (b - local) + (local % NXCD) * run + local / NXCD;
 }
 
-// Work units: a tile's items are cut into runs of at most `chunk` items, one
+// Work units: a tile's items are cut into runs of about `chunk` items, one
 // workgroup each, so that a tile with many sparse groups (a coarse leaf next
 // to a much finer region) does not hold the whole launch back.  Unit k of a
-// tile covers items [k chunk, (k + 1) chunk) of it; the first unit adds its
-// products to L directly, the others leave theirs in part[] (one 128 x 110
-// slab per extra unit, FP64), which k_unit_combine adds in unit order after
-// the launch, so the sums do not depend on timing.
+// tile starts at the first segment (group) boundary at or after item
+// k chunk: a segment never straddles two units, and since a target occurs
+// at most once per segment, which of a target's terms a unit sums does not
+// depend on the order of the items inside a segment (which the traversal's
+// atomics leave open).  The first unit adds its products to L directly, the
+// others leave theirs in part[] (one 128 x 110 slab per extra unit, FP64),
+// which k_unit_combine adds in unit order after the launch, so the sums do
+// not depend on timing.
 __global__ void k_unit_counts(const int32_t *tile_off, int32_t ntiles, int32_t chunk,
                               uint32_t *ucnt) {
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -628,13 +632,13 @@ __global__ void k_unit_combine(const uint32_t *ucnt, const uint32_t *uoff, int32
 template <class T>
 __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
     const int32_t *tile_off, const int2 *units, const uint32_t *ucnt, const uint32_t *uoff,
-    int32_t ntiles, int32_t chunk, const uint2 *cols, const T *Yg, const T *Mq,
+    int32_t ntiles, int32_t chunk, int xcd_run, const uint2 *cols, const T *Yg, const T *Mq,
     const uint32_t *codes, Cells c, int32_t ncells, double *L, double *part) {
   __shared__ T Lt[TILE][NR + 1];
   __shared__ T Lt0b[TILE][17];
   __shared__ T Tb[TILE_WAVES][NMULT * NR];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int u = tile_of_block(blockIdx.x, gridDim.x);
+  const int u = tile_of_block(blockIdx.x, gridDim.x, xcd_run);
   if ((uint32_t)u >= uoff[ntiles - 1] + ucnt[ntiles - 1]) return;  // grid is an upper bound
   const int2 tu = units[u];
   const int tile = tu.x, k = tu.y;
@@ -642,7 +646,17 @@ __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
   for (int i = threadIdx.x; i < TILE * 17; i += 64 * TILE_WAVES) (&Lt0b[0][0])[i] = 0;
   __syncthreads();
   const int32_t tb = tile_off[tile], te = tile_off[tile + 1];
-  const int32_t ib = tb + k * chunk, ie = min(te, ib + chunk);
+  // first segment start at or after item j of the tile (a segment holds at
+  // most TILE / 16 items, so the walk is short)
+  auto seg_start = [&](int64_t j) -> int32_t {
+    if (j <= tb) return tb;
+    if (j >= te) return te;
+    int32_t i = (int32_t)j;
+    while (i < te && (cols[(int64_t)i * 16].y >> COL_G) == (cols[(int64_t)(i - 1) * 16].y >> COL_G)) i++;
+    return i;
+  };
+  const int32_t ib = seg_start((int64_t)tb + (int64_t)k * chunk);
+  const int32_t ie = seg_start((int64_t)tb + (int64_t)(k + 1) * chunk);
   // every wave walks all items of the tile, in order, for its own blocks
   switch (wave) {
     case 0: run_wave<T, 0>(cols, ib, ie, Yg, Mq, codes, Lt, Lt0b, Tb[0], lane); break;
@@ -813,18 +827,19 @@ hipError_t launch_m2l_units(const int32_t *tile_off, int32_t ntiles, int32_t chu
 }
 
 hipError_t launch_m2l_tiles(const int32_t *tile_off, const int2 *units, const uint32_t *ucnt,
-                            const uint32_t *uoff, int32_t ntiles, int32_t chunk,
+                            const uint32_t *uoff, int32_t ntiles, int32_t chunk, int xcd_run,
                             uint32_t units_bound, const uint2 *cols, const void *Yg,
                             const void *Mq, const uint32_t *codes, Cells c, int32_t ncells,
                             double *L, double *part, bool fp32, hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
   if (fp32)
     k_m2l_tiles<float><<<units_bound, 64 * TILE_WAVES, 0, s>>>(
-        tile_off, units, ucnt, uoff, ntiles, chunk, cols, (const float *)Yg, (const float *)Mq,
+        tile_off, units, ucnt, uoff, ntiles, chunk, xcd_run, cols, (const float *)Yg,
+        (const float *)Mq,
         codes, c, ncells, L, part);
   else
     k_m2l_tiles<double><<<units_bound, 64 * TILE_WAVES, 0, s>>>(
-        tile_off, units, ucnt, uoff, ntiles, chunk, cols, (const double *)Yg,
+        tile_off, units, ucnt, uoff, ntiles, chunk, xcd_run, cols, (const double *)Yg,
         (const double *)Mq, codes + NCODE, c, ncells, L, part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

@@ -34,9 +34,11 @@ struct Csr {
 // tree.hip
 hipError_t launch_bbox(const double *x, const double *y, const double *z, int64_t n,
                        double *partial, int nparts, double *region, hipStream_t s);
-hipError_t launch_keys(const double *x, const double *y, const double *z, int64_t n,
-                       const double *regions, const int64_t *off, int nprob, uint64_t *keys,
-                       uint32_t *idx, uint32_t *bprob, hipStream_t s);
+// keys, and the bodies packed as (x, y, z, q) records for the permutation
+hipError_t launch_keys(const double *x, const double *y, const double *z, const double *q,
+                       int64_t n, const double *regions, const int64_t *off, int nprob,
+                       uint64_t *keys, uint32_t *idx, uint32_t *bprob, double4 *rec,
+                       hipStream_t s);
 // forest of nprob problems, problem p = bodies [off[p], off[p+1])
 hipError_t launch_bbox_forest(const double *x, const double *y, const double *z,
                               const int64_t *off, int nprob, double *regions, hipStream_t s);
@@ -46,9 +48,8 @@ hipError_t launch_problem_gather(const uint32_t *pos, const uint64_t *keys1,
                                  const uint32_t *order1, int64_t n, uint64_t *keys,
                                  uint32_t *order, hipStream_t s);
 hipError_t launch_root_pairs(int2 *front, int nroots, uint32_t *count, hipStream_t s);
-hipError_t launch_permute(const uint32_t *order, const double *x, const double *y,
-                          const double *z, const double *q, int64_t n, double *sx, double *sy,
-                          double *sz, double *sq, int32_t *perm, hipStream_t s);
+hipError_t launch_permute(const uint32_t *order, const double4 *rec, int64_t n, double *sx,
+                          double *sy, double *sz, double *sq, int32_t *perm, hipStream_t s);
 hipError_t launch_roots(Cells c, int64_t n, const int64_t *off, int nprob,
                         const double *regions, int32_t *lev, hipStream_t s);
 // one level of the cell build with sizes on the device (lev: first cell and
@@ -150,7 +151,7 @@ hipError_t launch_m2l_units(const int32_t *tile_off, int32_t ntiles, int32_t chu
                             uint32_t *ucnt, uint32_t *uoff, int2 *units, void *scan_tmp,
                             hipStream_t s);
 hipError_t launch_m2l_tiles(const int32_t *tile_off, const int2 *units, const uint32_t *ucnt,
-                            const uint32_t *uoff, int32_t ntiles, int32_t chunk,
+                            const uint32_t *uoff, int32_t ntiles, int32_t chunk, int xcd_run,
                             uint32_t units_bound, const uint2 *cols, const void *Yg,
                             const void *Mq, const uint32_t *codes, Cells c, int32_t ncells,
                             double *L, double *part, bool fp32, hipStream_t s);

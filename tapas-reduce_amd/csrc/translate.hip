@@ -30,6 +30,15 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int NV = 2 * NTERM;  // 110 reals per cell
 constexpr int KM = 8 * NV;     // 880: stacked children
+constexpr int KS = (NV + 3) / 4;  // 28 k-steps of 4
+// The operators again in the order the MFMA A operand is read: for each
+// (row block, k-step, octant) of M2M and (octant, row block, k-step) of L2L,
+// 64 consecutive doubles, lane l holding A[16 rb + l % 16][4 ks + l / 16]
+// (zero outside the 110 x 110 blocks).  One k-step of one wave then reads
+// 512 contiguous bytes per operand instead of 16 scattered rows, so a block
+// streams each operator once through its CU instead of four times.
+constexpr int FRAG_M = 7 * KS * 8 * 64;
+constexpr int FRAG_L = 8 * 7 * KS * 64;
 
 __device__ inline int sgn(int n) { return (n & 1) ? -1 : 1; }
 __device__ inline int ipow2n(int n) { return (n >= 0) ? 1 : sgn(n); }
@@ -129,6 +138,20 @@ __global__ __launch_bounds__(64) void k_build_ops(double *Tm, double *Tl) {
   }
 }
 
+__global__ void k_frag_ops(const double *Tm, const double *Tl, double *Fm, double *Fl) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < FRAG_M) {
+    const int lane = e & 63, o = (e >> 6) & 7, ks = (e >> 9) % KS, rb = (e >> 9) / KS;
+    const int row = rb * 16 + (lane & 15), kc = 4 * ks + (lane >> 4);
+    Fm[e] = (row < NV && kc < NV) ? Tm[(int64_t)row * KM + o * NV + kc] : 0.0;
+  } else if (e < FRAG_M + FRAG_L) {
+    const int f = e - FRAG_M;
+    const int lane = f & 63, ks = (f >> 6) % KS, rb = ((f >> 6) / KS) % 7, oct = (f >> 6) / (KS * 7);
+    const int row = rb * 16 + (lane & 15), kc = 4 * ks + (lane >> 4);
+    Fl[f] = (row < NV && kc < NV) ? Tl[((int64_t)oct * NV + row) * NV + kc] : 0.0;
+  }
+}
+
 // per-parent factors sigma^(e * degree(k)) for k = 0..111 of the 16 parents
 // of a block (e = +1 or -1), into LDS.  A parent of zero width (all its
 // bodies at one point) keeps only the degree-0 terms: its children sit at
@@ -161,7 +184,7 @@ __device__ inline void parent_powers(double (*f)[NV + 2], const double *sig, int
 // order of their children (the t-th such parent has children at
 // first_child + 8 t); block = 16 parents, wave rb = row block rb.
 __global__ __launch_bounds__(7 * 64) void k_m2m_mfma(Cells c, int32_t first_child, int32_t npar,
-                                                     const double *Tm, double *M) {
+                                                     const double *Fm, double *M) {
   const int lane = threadIdx.x & 63, rb = threadIdx.x >> 6;
   const int col = lane & 15, kq = lane >> 4;
   __shared__ double sig[16];
@@ -180,9 +203,7 @@ __global__ __launch_bounds__(7 * 64) void k_m2m_mfma(Cells c, int32_t first_chil
   const bool valid = t < npar;
   const int32_t c0 = first_child + 8 * (valid ? t : 0);
   const double *B = M + (int64_t)c0 * NV;
-  const int arow = rb * 16 + col;
-  const double *A = Tm + (int64_t)min(arow, NV - 1) * KM;
-  const bool arow_ok = arow < NV;
+  const double *A = Fm + (int64_t)rb * KS * 8 * 64 + lane;
   int kend = NV;
   switch (rb) {  // constants per wave
     case 0: kend = m2m_kend(0); break;
@@ -205,11 +226,12 @@ __global__ __launch_bounds__(7 * 64) void k_m2m_mfma(Cells c, int32_t first_chil
     const int kc = k0 + kq;
     const bool in = kc < kend;
     const int kk = in ? kc : 0;
-    // masks as factors (the operands are finite), so no load sits under a branch
-    const double f = in ? cf[col][kk] : 0.0, ma = arow_ok ? 1.0 : 0.0;
+    const int ks = min(k0 >> 2, KS - 1);
+    // the mask as a factor (the operands are finite), so no load sits under a branch
+    const double f = in ? cf[col][kk] : 0.0;
 #pragma unroll
     for (int o = 0; o < 8; o++) {
-      ao[o] = A[o * NV + kk] * ma;
+      ao[o] = A[(ks * 8 + o) * 64];
       bo[o] = B[o * NV + kk] * f;
     }
   };
@@ -237,7 +259,7 @@ __global__ __launch_bounds__(7 * 64) void k_m2m_mfma(Cells c, int32_t first_chil
 // 7 row blocks of the child's coefficients at once (one B fragment per
 // k-step feeds 7 independent accumulators).
 __global__ __launch_bounds__(8 * 64) void k_l2l_mfma(Cells c, int32_t first_child, int32_t npar,
-                                                     const double *Tl, double *L) {
+                                                     const double *Fl, double *L) {
   const int lane = threadIdx.x & 63, oct = threadIdx.x >> 6;
   const int col = lane & 15, kq = lane >> 4;
   __shared__ double sig[16];
@@ -257,7 +279,7 @@ __global__ __launch_bounds__(8 * 64) void k_l2l_mfma(Cells c, int32_t first_chil
   const int32_t c0 = first_child + 8 * (valid ? t : 0);
   const int32_t p = c.parent[c0];
   const double *B = L + (int64_t)p * NV;
-  const double *S = Tl + (int64_t)oct * NV * NV;
+  const double *S = Fl + (int64_t)oct * 7 * KS * 64 + lane;
   d4 acc[7];
 #pragma unroll
   for (int rb = 0; rb < 7; rb++) acc[rb] = (d4){0, 0, 0, 0};
@@ -267,14 +289,10 @@ __global__ __launch_bounds__(8 * 64) void k_l2l_mfma(Cells c, int32_t first_chil
     const int kc = k0 + kq;
     const bool in = kc < NV;
     const int kk = in ? kc : 0;
-    const double bv = B[kk] * cf[col][kk];
-    bo = in ? bv : 0.0;
+    const int ks = min(k0 >> 2, KS - 1);
+    bo = B[kk] * (in ? cf[col][kk] : 0.0);
 #pragma unroll
-    for (int rb = 0; rb < 7; rb++) {
-      const int arow = rb * 16 + col;
-      const double av = S[(int64_t)min(arow, NV - 1) * NV + kk];
-      ao[rb] = (in && arow < NV && k0 >= l2l_kbeg(rb)) ? av : 0.0;
-    }
+    for (int rb = 0; rb < 7; rb++) ao[rb] = S[(rb * KS + ks) * 64];
   };
   auto mma = [&](int k0, const double (&ao)[7], double bo) {
 #pragma unroll
@@ -300,14 +318,135 @@ __global__ __launch_bounds__(8 * 64) void k_l2l_mfma(Cells c, int32_t first_chil
   }
 }
 
+// Levels with few parents: one block per (16 parents, row block) and one
+// wave per octant (M2M) or per quarter of the k-steps (L2L), every wave's
+// operands loaded at once; the partial products meet in LDS.  A single
+// wave walking all 28 k-steps of a row block waits out one load latency per
+// step, which on the upper levels (a handful of blocks) is the whole launch.
+__global__ __launch_bounds__(8 * 64) void k_m2m_small(Cells c, int32_t first_child, int32_t npar,
+                                                      const double *Fm, double *M) {
+  const int lane = threadIdx.x & 63, o = threadIdx.x >> 6, rb = blockIdx.y;
+  const int col = lane & 15, kq = lane >> 4;
+  __shared__ double sig[16];
+  __shared__ double cf[16][NV + 2];
+  __shared__ double rf[16][NV + 2];
+  __shared__ double red[8][64][4];
+  if (threadIdx.x < 16) {
+    const int32_t tt = (int32_t)blockIdx.x * 16 + threadIdx.x;
+    const int32_t cc = first_child + 8 * (tt < npar ? tt : 0);
+    sig[threadIdx.x] = c.width[c.parent[cc]];
+  }
+  __syncthreads();
+  parent_powers(cf, sig, -1);
+  parent_powers(rf, sig, +1);
+  __syncthreads();
+  const int32_t t = (int32_t)blockIdx.x * 16 + col;
+  const bool valid = t < npar;
+  const int32_t c0 = first_child + 8 * (valid ? t : 0);
+  const double *B = M + (int64_t)c0 * NV + o * NV;
+  const double *A = Fm + (int64_t)rb * KS * 8 * 64 + o * 64 + lane;
+  const int kend = m2m_kend(rb);
+  double a[KS], b[KS];
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++) {
+    const int kc = 4 * ks + kq;
+    const bool in = kc < kend;
+    const int kk = in ? kc : 0;
+    a[ks] = A[ks * 8 * 64];
+    b[ks] = B[kk] * (in ? cf[col][kk] : 0.0);
+  }
+  d4 acc = {0, 0, 0, 0};
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ks], b[ks], acc, 0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < 4; i++) red[o][lane][i] = acc[i];
+  __syncthreads();
+  if (o != 0 || !valid) return;
+  const int32_t p = c.parent[c0];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    double v = 0;
+#pragma unroll
+    for (int w = 0; w < 8; w++) v += red[w][lane][i];
+    const int r = rb * 16 + kq + 4 * i;
+    if (r < NV) M[(int64_t)p * NV + r] += v * rf[col][r];
+  }
+}
+
+constexpr int L2L_SPLIT = 4;                         // waves per (parents, octant, row block)
+constexpr int L2L_KPW = (KS + L2L_SPLIT - 1) / L2L_SPLIT;  // k-steps per wave
+
+__global__ __launch_bounds__(L2L_SPLIT * 64) void k_l2l_small(Cells c, int32_t first_child,
+                                                              int32_t npar, const double *Fl,
+                                                              double *L) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int oct = blockIdx.y, rb = blockIdx.z;
+  const int col = lane & 15, kq = lane >> 4;
+  __shared__ double sig[16];
+  __shared__ double cf[16][NV + 2];
+  __shared__ double rf[16][NV + 2];
+  __shared__ double red[L2L_SPLIT][64][4];
+  if (threadIdx.x < 16) {
+    const int32_t tt = (int32_t)blockIdx.x * 16 + threadIdx.x;
+    const int32_t cc = first_child + 8 * (tt < npar ? tt : 0);
+    sig[threadIdx.x] = c.width[c.parent[cc]];
+  }
+  __syncthreads();
+  parent_powers(cf, sig, +1);
+  parent_powers(rf, sig, -1);
+  __syncthreads();
+  const int32_t t = (int32_t)blockIdx.x * 16 + col;
+  const bool valid = t < npar;
+  const int32_t c0 = first_child + 8 * (valid ? t : 0);
+  const int32_t p = c.parent[c0];
+  const double *B = L + (int64_t)p * NV;
+  const double *S = Fl + ((int64_t)oct * 7 + rb) * KS * 64 + lane;
+  const int kbeg = l2l_kbeg(rb);
+  double a[L2L_KPW], b[L2L_KPW];
+#pragma unroll
+  for (int j = 0; j < L2L_KPW; j++) {
+    const int ks = w * L2L_KPW + j;
+    const int kc = 4 * ks + kq;
+    const bool in = ks < KS && kc < NV && 4 * ks >= kbeg;
+    const int kk = in ? kc : 0;
+    a[j] = S[(ks < KS ? ks : 0) * 64];
+    b[j] = B[kk] * (in ? cf[col][kk] : 0.0);
+  }
+  d4 acc = {0, 0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < L2L_KPW; j++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j], b[j], acc, 0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < 4; i++) red[w][lane][i] = acc[i];
+  __syncthreads();
+  if (w != 0 || !valid) return;
+  const int64_t cidx = c0 + oct;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    double v = 0;
+#pragma unroll
+    for (int q = 0; q < L2L_SPLIT; q++) v += red[q][lane][i];
+    const int r = rb * 16 + kq + 4 * i;
+    if (r < NV) L[cidx * NV + r] += v * rf[col][r];
+  }
+}
+
+// levels below this many 16-parent blocks take the split kernels (measured
+// crossover: the split L2L does 7 x 8 blocks per 16 parents)
+constexpr int SMALL_BLOCKS_M2M = 128;
+constexpr int SMALL_BLOCKS_L2L = 24;
+
 }  // namespace
 
-size_t translation_ops_doubles() { return (size_t)NV * KM + (size_t)8 * NV * NV; }
+size_t translation_ops_doubles() {
+  return (size_t)NV * KM + (size_t)8 * NV * NV + FRAG_M + FRAG_L;
+}
 
 // The unit operators (built once per context); ops: translation_ops_doubles()
-// doubles (M2M operator, then the 8 L2L operators).
+// doubles: the M2M operator, the 8 L2L operators, then both in fragment order.
 hipError_t launch_translation_ops(double *ops, hipStream_t s) {
-  k_build_ops<<<dim3(NV, 8, 2), 64, 0, s>>>(ops, ops + (size_t)NV * KM);
+  double *Tm = ops, *Tl = ops + (size_t)NV * KM, *Fm = Tl + (size_t)8 * NV * NV, *Fl = Fm + FRAG_M;
+  k_build_ops<<<dim3(NV, 8, 2), 64, 0, s>>>(Tm, Tl);
+  k_frag_ops<<<(FRAG_M + FRAG_L + 255) / 256, 256, 0, s>>>(Tm, Tl, Fm, Fl);
   return hipGetLastError();
 }
 
@@ -316,14 +455,24 @@ hipError_t launch_translation_ops(double *ops, hipStream_t s) {
 hipError_t launch_m2m(Cells c, int32_t first_child, int32_t npar, const double *ops, double *M,
                       hipStream_t s) {
   if (npar <= 0) return hipSuccess;
-  k_m2m_mfma<<<(npar + 15) / 16, 7 * 64, 0, s>>>(c, first_child, npar, ops, M);
+  const double *Fm = ops + (size_t)NV * KM + (size_t)8 * NV * NV;
+  const int nb = (npar + 15) / 16;
+  if (nb < SMALL_BLOCKS_M2M)
+    k_m2m_small<<<dim3(nb, 7), 8 * 64, 0, s>>>(c, first_child, npar, Fm, M);
+  else
+    k_m2m_mfma<<<nb, 7 * 64, 0, s>>>(c, first_child, npar, Fm, M);
   return hipGetLastError();
 }
 
 hipError_t launch_l2l(Cells c, int32_t first_child, int32_t npar, const double *ops, double *L,
                       hipStream_t s) {
   if (npar <= 0) return hipSuccess;
-  k_l2l_mfma<<<(npar + 15) / 16, 8 * 64, 0, s>>>(c, first_child, npar, ops + (size_t)NV * KM, L);
+  const double *Fl = ops + (size_t)NV * KM + (size_t)8 * NV * NV + FRAG_M;
+  const int nb = (npar + 15) / 16;
+  if (nb < SMALL_BLOCKS_L2L)
+    k_l2l_small<<<dim3(nb, 8, 7), L2L_SPLIT * 64, 0, s>>>(c, first_child, npar, Fl, L);
+  else
+    k_l2l_mfma<<<nb, 8 * 64, 0, s>>>(c, first_child, npar, Fl, L);
   return hipGetLastError();
 }
 

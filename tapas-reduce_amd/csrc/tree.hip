@@ -92,9 +92,12 @@ __device__ inline int problem_of(const int64_t *off, int nprob, int64_t i) {
 
 // Keys against the region of the body's own problem (off == nullptr: one
 // problem); the problem index is kept per body for the forest sort.
-__global__ void k_keys(const double *x, const double *y, const double *z, int64_t n,
-                       const double *regions, const int64_t *off, int nprob, uint64_t *keys,
-                       uint32_t *idx, uint32_t *bprob) {
+// Also packs each body into one 32-byte record (x, y, z, q): the permutation
+// gathers in random order, and one record per body is one cache line where
+// four arrays are four.
+__global__ void k_keys(const double *x, const double *y, const double *z, const double *q,
+                       int64_t n, const double *regions, const int64_t *off, int nprob,
+                       uint64_t *keys, uint32_t *idx, uint32_t *bprob, double4 *rec) {
 #pragma clang fp contract(off)
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -114,6 +117,7 @@ __global__ void k_keys(const double *x, const double *y, const double *z, int64_
   }
   keys[i] = finest_key(a[0], a[1], a[2]);
   idx[i] = (uint32_t)i;
+  rec[i] = make_double4(p[0], p[1], p[2], q[i]);
 }
 
 // Bounding box of every problem of a forest, one block per problem, with
@@ -175,16 +179,16 @@ __global__ void k_problem_gather(const uint32_t *pos, const uint64_t *keys1,
 
 // Gather the caller's bodies into sorted order (SoA FP64) and keep the
 // permutation.  perm[sorted] = original index.
-__global__ void k_permute(const uint32_t *order, const double *x, const double *y,
-                          const double *z, const double *q, int64_t n, double *sx, double *sy,
-                          double *sz, double *sq, int32_t *perm) {
+__global__ void k_permute(const uint32_t *order, const double4 *rec, int64_t n, double *sx,
+                          double *sy, double *sz, double *sq, int32_t *perm) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t o = order[i];
-  sx[i] = x[o];
-  sy[i] = y[o];
-  sz[i] = z[o];
-  sq[i] = q[o];
+  const double4 r = rec[o];
+  sx[i] = r.x;
+  sy[i] = r.y;
+  sz[i] = r.z;
+  sq[i] = r.w;
   perm[i] = (int32_t)o;
 }
 
@@ -389,12 +393,13 @@ hipError_t launch_bbox(const double *x, const double *y, const double *z, int64_
   return hipGetLastError();
 }
 
-hipError_t launch_keys(const double *x, const double *y, const double *z, int64_t n,
-                       const double *regions, const int64_t *off, int nprob, uint64_t *keys,
-                       uint32_t *idx, uint32_t *bprob, hipStream_t s) {
+hipError_t launch_keys(const double *x, const double *y, const double *z, const double *q,
+                       int64_t n, const double *regions, const int64_t *off, int nprob,
+                       uint64_t *keys, uint32_t *idx, uint32_t *bprob, double4 *rec,
+                       hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  k_keys<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(x, y, z, n, regions, off, nprob, keys, idx,
-                                                      bprob);
+  k_keys<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(x, y, z, q, n, regions, off, nprob, keys,
+                                                      idx, bprob, rec);
   return hipGetLastError();
 }
 
@@ -426,12 +431,10 @@ hipError_t launch_root_pairs(int2 *front, int nroots, uint32_t *count, hipStream
   return hipGetLastError();
 }
 
-hipError_t launch_permute(const uint32_t *order, const double *x, const double *y,
-                          const double *z, const double *q, int64_t n, double *sx, double *sy,
-                          double *sz, double *sq, int32_t *perm, hipStream_t s) {
+hipError_t launch_permute(const uint32_t *order, const double4 *rec, int64_t n, double *sx,
+                          double *sy, double *sz, double *sq, int32_t *perm, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  k_permute<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(order, x, y, z, q, n, sx, sy, sz, sq,
-                                                         perm);
+  k_permute<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(order, rec, n, sx, sy, sz, sq, perm);
   return hipGetLastError();
 }
 

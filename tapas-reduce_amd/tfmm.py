@@ -18,6 +18,7 @@ TFMM_OK = 0
 FP64 = 0
 MIXED = 1
 TUNE_M2L_UNIT = 1
+TUNE_M2L_XCD_RUN = 2
 P = 10
 NTERM = 55
 
@@ -66,6 +67,7 @@ EXPORTED = [
     "tfmm_create", "tfmm_destroy", "tfmm_last_error", "tfmm_stream", "tfmm_sync",
     "tfmm_set_bodies", "tfmm_set_bodies_device", "tfmm_set_batch", "tfmm_set_batch_device",
     "tfmm_batch_evaluate", "tfmm_num_problems", "tfmm_set_tuning", "tfmm_get_tuning",
+    "tfmm_autotune",
     "tfmm_build_tree", "tfmm_set_params",
     "tfmm_upward",
     "tfmm_traverse", "tfmm_downward", "tfmm_evaluate", "tfmm_get_fields", "tfmm_get_stats",
@@ -105,6 +107,7 @@ def lib():
         L.tfmm_num_problems.argtypes = [vp, vp]
         L.tfmm_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.tfmm_get_tuning.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.tfmm_autotune.argtypes = [vp, ctypes.POINTER(Params), ctypes.c_int, vp]
         L.tfmm_build_tree.argtypes = [vp, ctypes.POINTER(Params)]
         L.tfmm_evaluate.argtypes = [vp, ctypes.POINTER(Params)]
         L.tfmm_set_params.argtypes = [vp, ctypes.POINTER(Params)]
@@ -241,6 +244,14 @@ class Context:
         v = ctypes.c_double()
         self._check(lib().tfmm_get_tuning(self._h, int(knob), ctypes.byref(v)), "tfmm_get_tuning")
         return v.value
+
+    def autotune(self, params, reps=3):
+        """Pick the tile-kernel knobs for these bodies; returns
+        (unit factor, XCD run, ms of the M2L tile kernel)."""
+        out = np.zeros(3)
+        self._check(lib().tfmm_autotune(self._h, ctypes.byref(params), int(reps), _ptr(out)),
+                    "tfmm_autotune")
+        return float(out[0]), int(out[1]), float(out[2])
 
     def problem_offsets(self):
         n = lib().tfmm_num_problems(self._h, None)

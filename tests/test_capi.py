@@ -41,3 +41,11 @@ def test_params_struct_layout():
     import ctypes
     assert ctypes.sizeof(tfmm.Params) == 24
     assert ctypes.sizeof(tfmm.BodyDesc) == 32
+
+
+def test_tuning_knobs_declared():
+    """The tuning knobs and the autotuner are part of the C surface."""
+    txt = open(os.path.join(ROOT, "include", "tfmm.h")).read()
+    for name in ("TFMM_TUNE_M2L_UNIT 1", "TFMM_TUNE_M2L_XCD_RUN 2", "tfmm_autotune("):
+        assert name in txt
+    assert tfmm.TUNE_M2L_UNIT == 1 and tfmm.TUNE_M2L_XCD_RUN == 2

@@ -1,0 +1,63 @@
+"""Tuning knobs of the batched M2L kernel and the autotuner (through the C
+ABI): every setting gives the same fields up to summation order (relative
+L2 1e-12 in FP64, 1e-6 in mixed precision), the autotuner returns one of its
+candidates and leaves it set, and invalid settings are rejected."""
+import numpy as np
+import pytest
+
+import datasets
+import oracle
+import tfmm
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = tfmm.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("precision,tol", [(tfmm.FP64, 1e-12), (tfmm.MIXED, 1e-6)])
+def test_gpu_knobs_do_not_change_results(ctx, precision, tol):
+    x, y, z, q = datasets.plummer(120000, seed=9, charges="uniform")
+    print("plummer n=%d precision %d" % (len(x), precision), flush=True)
+    ctx.set_bodies(x, y, z, q)
+    prm = tfmm.make_params(32, 0.4, precision)
+    ctx.set_tuning(tfmm.TUNE_M2L_UNIT, 0)
+    ctx.set_tuning(tfmm.TUNE_M2L_XCD_RUN, 32)
+    ctx.evaluate(prm)
+    ref = ctx.fields()
+    for unit, run in ((0.5, 8), (1.0, 1), (1.5, 64), (3.0, 16)):
+        ctx.set_tuning(tfmm.TUNE_M2L_UNIT, unit)
+        ctx.set_tuning(tfmm.TUNE_M2L_XCD_RUN, run)
+        assert ctx.get_tuning(tfmm.TUNE_M2L_UNIT) == unit
+        assert ctx.get_tuning(tfmm.TUNE_M2L_XCD_RUN) == run
+        ctx.evaluate(prm)
+        f = ctx.fields()
+        ep, ea = oracle.rel_l2(f, ref)
+        assert ep < tol and ea < tol, (unit, run, ep, ea)
+        ctx.evaluate(prm)  # the same setting twice: bit for bit
+        assert np.array_equal(ctx.fields(), f)
+    ctx.set_tuning(tfmm.TUNE_M2L_UNIT, 0)
+    ctx.set_tuning(tfmm.TUNE_M2L_XCD_RUN, 32)
+
+
+def test_gpu_autotune(ctx):
+    x, y, z, q = datasets.cube(60000, seed=2)
+    ctx.set_bodies(x, y, z, q)
+    prm = tfmm.make_params(64, 0.4, tfmm.MIXED)
+    unit, run, ms = ctx.autotune(prm, reps=1)
+    assert unit in (0.0, 1.5, 2.0, 3.0) and run in (8, 16, 32, 64) and ms > 0
+    assert ctx.get_tuning(tfmm.TUNE_M2L_UNIT) == unit
+    assert ctx.get_tuning(tfmm.TUNE_M2L_XCD_RUN) == run
+    ctx.set_tuning(tfmm.TUNE_M2L_UNIT, 0)
+    ctx.set_tuning(tfmm.TUNE_M2L_XCD_RUN, 32)
+
+
+def test_gpu_invalid_knobs(ctx):
+    for knob, value in ((tfmm.TUNE_M2L_UNIT, -1.0), (tfmm.TUNE_M2L_XCD_RUN, 0),
+                        (tfmm.TUNE_M2L_XCD_RUN, 2.5), (99, 1.0)):
+        with pytest.raises(tfmm.TfmmError):
+            ctx.set_tuning(knob, value)
