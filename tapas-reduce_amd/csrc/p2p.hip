@@ -237,8 +237,7 @@ __global__ __launch_bounds__(64) void k_p2p_fp32(Cells c, Csr list, Bodies b,
             // bodies of different leaves sit at different positions (equal
             // positions give equal keys), so r > 0 unless they are closer than
             // the FP32 resolution of the leaf-relative coordinates
-            for (int jj = sub; jj < cn; jj += g) {
-              const float4 xj = sb[jj];
+            auto pair = [&](const float4 xj) {
               const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
               const float r2 = dx * dx + dy * dy + dz * dz;
               const float rinv = __builtin_amdgcn_rsqf(r2);
@@ -248,6 +247,24 @@ __global__ __launch_bounds__(64) void k_p2p_fp32(Cells c, Csr list, Bodies b,
               p1 -= dx * f;
               p2 -= dy * f;
               p3 -= dz * f;
+            };
+            if (g == 1) {
+              // one lane per target (leaves of >= 33 bodies, most of the
+              // pairs): the source loop is uniform across the wave, four
+              // sources per trip at immediate LDS offsets, so the counter,
+              // bound test and address step (three of the ~19 VALU slots of
+              // a pair) are paid once per four pairs; same order of the sums
+              int jj = 0;
+              for (; jj + 3 < cn; jj += 4) {
+                const float4 xa = sb[jj], xb = sb[jj + 1], xc = sb[jj + 2], xd = sb[jj + 3];
+                pair(xa);
+                pair(xb);
+                pair(xc);
+                pair(xd);
+              }
+              for (; jj < cn; jj++) pair(sb[jj]);
+            } else {
+              for (int jj = sub; jj < cn; jj += g) pair(sb[jj]);
             }
           }
         }
