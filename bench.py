@@ -210,10 +210,10 @@ def main():
     tuned = None
     if not args.no_autotune and args.unit_factor is None:
         t0 = time.perf_counter()
-        unit, run, tms = ctx.autotune(prm, reps=3)
-        tuned = {"m2l_unit_factor": unit, "m2l_xcd_run": run, "m2l_tiles_ms": tms,
-                 "seconds": time.perf_counter() - t0,
-                 "note": "16 candidate settings x 3 evaluations, outside the timed steps"}
+        unit, run, sched, tms = ctx.autotune(prm, reps=3)
+        tuned = {"m2l_unit_factor": unit, "m2l_xcd_run": run, "p2p_schedule": sched,
+                 "ms_per_evaluation": tms, "seconds": time.perf_counter() - t0,
+                 "note": "32 candidate settings x 3 evaluations, outside the timed steps"}
 
     # accuracy reference: FP64 direct sum on random targets (untimed)
     rng = np.random.default_rng(123)
@@ -238,6 +238,7 @@ def main():
         prm64 = tfmm.make_params(args.ncrit, args.theta, tfmm.FP64)
         ctx.set_tuning(tfmm.TUNE_M2L_UNIT, 0)  # the plain form keeps the default knobs
         ctx.set_tuning(tfmm.TUNE_M2L_XCD_RUN, 32)
+        ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, 0)
         ms64, ph64, st64 = time_config(ctx, prm64, args.fp64_steps, 1)
         e64 = errors(ctx, targets, direct)
         fp64 = {"value": args.n / (ms64 * 1e-3), "ms_per_step": ms64, "ncrit": args.ncrit,

@@ -182,13 +182,17 @@ int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p
 /* TFMM_TUNE_M2L_XCD_RUN: consecutive work units dealt to one of the 8 XCDs
  * (each has its own L2; neighbouring tiles share source expansions). */
 #define TFMM_TUNE_M2L_XCD_RUN 2
+/* TFMM_TUNE_P2P_SCHEDULE: 0 starts the P2P kernel (second stream) as soon
+ * as its list is sorted, beside the M2L batch building; 1 starts it with
+ * the M2L tile kernel, so FP32 VALU and MFMA work share the CUs. */
+#define TFMM_TUNE_P2P_SCHEDULE 3
 int tfmm_set_tuning(tfmm_ctx *ctx, int knob, double value);
 int tfmm_get_tuning(tfmm_ctx *ctx, int knob, double *value);
 /* Autotuner: evaluates the current bodies with params under every candidate
  * setting of the knobs (unit factor {0, 1.5, 2, 3} x XCD run {8, 16, 32,
- * 64}), reps times each, and keeps the setting whose batched M2L kernel was
- * fastest (median).  chosen (3 doubles, may be null): unit factor, XCD run,
- * milliseconds. */
+ * 64} x P2P schedule {0, 1}), reps times each, and keeps the setting whose
+ * evaluation was fastest (median).  chosen (4 doubles, may be null): unit
+ * factor, XCD run, P2P schedule, milliseconds per evaluation. */
 int tfmm_autotune(tfmm_ctx *ctx, const tfmm_params *params, int reps, double *chosen);
 
 /* FP64 direct sum on the current bodies for the given target indices (caller

@@ -93,6 +93,7 @@ struct tfmm_ctx {
   DevBuf u_cnt, u_off, units, u_part;
   double m2l_unit_factor = 0.0;  // tile work-unit size in mean items per tile (0: off)
   int m2l_xcd_run = 32;          // consecutive work units dealt to one XCD
+  int p2p_schedule = 0;          // 0: P2P beside the M2L build, 1: beside the tiles
   int32_t unit_chunk = 0;
   uint32_t units_bound = 0;
   bool codes_ready = false;
@@ -629,23 +630,35 @@ int do_traverse(tfmm_ctx *c) {
     TRY(dev_reserve(c->relpos, (size_t)c->n * 16));
     TRY(launch_leaf_relpos(k, (int32_t)c->ncells, sorted_bodies(c), dp<float4>(c->relpos), s));
   }
-  // P2P on the side stream: it needs only the P2P list and the bodies, so it
-  // runs while the host builds the M2L batches (a few host round trips, during
-  // which the main stream would idle); joined before the downward pass.
+  // P2P on the side stream: it needs only the P2P list and the bodies.
+  // Schedule 0 starts it as soon as its list is sorted, so it covers the host
+  // round trips of the M2L batch building (which then gets few CU slots);
+  // schedule 1 starts it together with the tile kernel, so the P2P (FP32
+  // VALU) and the tiles (MFMA, LDS) share the CUs.  Joined before the
+  // downward pass.
   TRY(dev_reserve(c->acc, (size_t)c->n * 32));
-  TRY(hipEventRecord(c->ev[EV_P2PREADY], s));
-  TRY(hipStreamWaitEvent(c->side, c->ev[EV_P2PREADY], 0));
-  TRY(hipEventRecord(c->ev[EV_P2P0], c->side));
-  Csr pl{dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src)};
-  if (mixed)
-    TRY(launch_p2p_fp32(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
-                        dp<double>(c->acc), c->side));
-  else
-    TRY(launch_p2p_fp64(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<double>(c->acc), c->side));
-  TRY(hipEventRecord(c->ev[EV_P2P1], c->side));
+  auto launch_p2p = [&]() -> int {
+    TRY(hipEventRecord(c->ev[EV_P2PREADY], s));
+    TRY(hipStreamWaitEvent(c->side, c->ev[EV_P2PREADY], 0));
+    TRY(hipEventRecord(c->ev[EV_P2P0], c->side));
+    Csr pl{dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src)};
+    if (mixed)
+      TRY(launch_p2p_fp32(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
+                          dp<double>(c->acc), c->side));
+    else
+      TRY(launch_p2p_fp64(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<double>(c->acc),
+                          c->side));
+    TRY(hipEventRecord(c->ev[EV_P2P1], c->side));
+    return TFMM_OK;
+  };
+  const bool p2p_early = c->p2p_schedule == 0;
+  if (p2p_early) {
+    rc = launch_p2p();
+    if (rc) return rc;
+  }
   rc = build_m2l_batches(c);
   if (rc) {
-    (void)hipStreamWaitEvent(s, c->ev[EV_P2P1], 0);
+    if (p2p_early) (void)hipStreamWaitEvent(s, c->ev[EV_P2P1], 0);
     return rc;
   }
   // M~ in the MFMA operand layout
@@ -656,6 +669,10 @@ int do_traverse(tfmm_ctx *c) {
   const size_t lbytes = (size_t)c->ncells * NTERM * 16;
   TRY(hipMemsetAsync(c->L.ptr, 0, lbytes, s));
   TRY(hipEventRecord(c->ev[EV_M2L0], s));
+  if (!p2p_early) {
+    rc = launch_p2p();
+    if (rc) return rc;
+  }
   const int32_t tile = m2l_tile_size();
   const int32_t ntiles = (int32_t)((c->ncells + tile - 1) / tile);
   TRY(launch_m2l_tiles(dp<int32_t>(c->tile_off), dp<int2>(c->units), dp<uint32_t>(c->u_cnt),
@@ -779,40 +796,47 @@ int tfmm_set_tuning(tfmm_ctx *c, int knob, double value) {
         return fail(c, TFMM_EINVAL, "XCD run must be an integer in [1, 4096]");
       c->m2l_xcd_run = (int)value;
       return TFMM_OK;
+    case TFMM_TUNE_P2P_SCHEDULE:
+      if (value != 0 && value != 1) return fail(c, TFMM_EINVAL, "P2P schedule must be 0 or 1");
+      c->p2p_schedule = (int)value;
+      return TFMM_OK;
     default:
       return fail(c, TFMM_EINVAL, "unknown tuning knob");
   }
 }
 
-// Candidates of the knobs, tried jointly; the tile kernel's time (HIP events
-// around its launch) decides, median over the repetitions.
+// Candidates of the knobs, tried jointly; the evaluation's time (HIP events
+// around it) decides, median over the repetitions.
 int tfmm_autotune(tfmm_ctx *c, const tfmm_params *p, int reps, double *chosen) {
   if (!c) return TFMM_EINVAL;
   if (reps < 1 || reps > 64) return fail(c, TFMM_EINVAL, "reps must be in [1, 64]");
   static const double kUnit[] = {0.0, 1.5, 2.0, 3.0};
   static const int kRun[] = {8, 16, 32, 64};
   const double unit0 = c->m2l_unit_factor;
-  const int run0 = c->m2l_xcd_run;
-  struct Cand { double unit; int run; std::vector<double> ms; };
+  const int run0 = c->m2l_xcd_run, sched0 = c->p2p_schedule;
+  struct Cand { double unit; int run, sched; std::vector<double> ms; };
   std::vector<Cand> cand;
-  for (double u : kUnit)
-    for (int r : kRun) cand.push_back({u, r, {}});
+  for (int sc = 0; sc < 2; sc++)
+    for (double u : kUnit)
+      for (int r : kRun) cand.push_back({u, r, sc, {}});
   int rc = tfmm_evaluate(c, p);  // sizes the buffers
   for (int rep = 0; rep < reps && rc == TFMM_OK; rep++) {
     for (Cand &k : cand) {
       c->m2l_unit_factor = k.unit;
       c->m2l_xcd_run = k.run;
+      c->p2p_schedule = k.sched;
       rc = tfmm_evaluate(c, p);
       if (rc) break;
       tfmm_stats_t st;
       rc = tfmm_get_stats(c, &st);
       if (rc) break;
-      k.ms.push_back(st.ms_m2l_tiles);
+      k.ms.push_back(st.ms_total);
     }
   }
   if (rc) {
     c->m2l_unit_factor = unit0;
     c->m2l_xcd_run = run0;
+    c->p2p_schedule = sched0;
     return rc;
   }
   size_t best = 0;
@@ -828,10 +852,12 @@ int tfmm_autotune(tfmm_ctx *c, const tfmm_params *p, int reps, double *chosen) {
   }
   c->m2l_unit_factor = cand[best].unit;
   c->m2l_xcd_run = cand[best].run;
+  c->p2p_schedule = cand[best].sched;
   if (chosen) {
     chosen[0] = cand[best].unit;
     chosen[1] = cand[best].run;
-    chosen[2] = best_ms;
+    chosen[2] = cand[best].sched;
+    chosen[3] = best_ms;
   }
   return TFMM_OK;
 }
@@ -841,6 +867,7 @@ int tfmm_get_tuning(tfmm_ctx *c, int knob, double *value) {
   switch (knob) {
     case TFMM_TUNE_M2L_UNIT: *value = c->m2l_unit_factor; return TFMM_OK;
     case TFMM_TUNE_M2L_XCD_RUN: *value = c->m2l_xcd_run; return TFMM_OK;
+    case TFMM_TUNE_P2P_SCHEDULE: *value = c->p2p_schedule; return TFMM_OK;
     default: return fail(c, TFMM_EINVAL, "unknown tuning knob");
   }
 }

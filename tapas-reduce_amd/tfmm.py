@@ -19,6 +19,7 @@ FP64 = 0
 MIXED = 1
 TUNE_M2L_UNIT = 1
 TUNE_M2L_XCD_RUN = 2
+TUNE_P2P_SCHEDULE = 3
 P = 10
 NTERM = 55
 
@@ -246,12 +247,12 @@ class Context:
         return v.value
 
     def autotune(self, params, reps=3):
-        """Pick the tile-kernel knobs for these bodies; returns
-        (unit factor, XCD run, ms of the M2L tile kernel)."""
-        out = np.zeros(3)
+        """Pick the kernel knobs for these bodies; returns (unit factor, XCD
+        run, P2P schedule, ms per evaluation)."""
+        out = np.zeros(4)
         self._check(lib().tfmm_autotune(self._h, ctypes.byref(params), int(reps), _ptr(out)),
                     "tfmm_autotune")
-        return float(out[0]), int(out[1]), float(out[2])
+        return float(out[0]), int(out[1]), int(out[2]), float(out[3])
 
     def problem_offsets(self):
         n = lib().tfmm_num_problems(self._h, None)

@@ -46,6 +46,7 @@ def test_params_struct_layout():
 def test_tuning_knobs_declared():
     """The tuning knobs and the autotuner are part of the C surface."""
     txt = open(os.path.join(ROOT, "include", "tfmm.h")).read()
-    for name in ("TFMM_TUNE_M2L_UNIT 1", "TFMM_TUNE_M2L_XCD_RUN 2", "tfmm_autotune("):
+    for name in ("TFMM_TUNE_M2L_UNIT 1", "TFMM_TUNE_M2L_XCD_RUN 2", "TFMM_TUNE_P2P_SCHEDULE 3",
+                 "tfmm_autotune("):
         assert name in txt
-    assert tfmm.TUNE_M2L_UNIT == 1 and tfmm.TUNE_M2L_XCD_RUN == 2
+    assert (tfmm.TUNE_M2L_UNIT, tfmm.TUNE_M2L_XCD_RUN, tfmm.TUNE_P2P_SCHEDULE) == (1, 2, 3)

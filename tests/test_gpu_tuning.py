@@ -29,9 +29,10 @@ def test_gpu_knobs_do_not_change_results(ctx, precision, tol):
     ctx.set_tuning(tfmm.TUNE_M2L_XCD_RUN, 32)
     ctx.evaluate(prm)
     ref = ctx.fields()
-    for unit, run in ((0.5, 8), (1.0, 1), (1.5, 64), (3.0, 16)):
+    for unit, run, sched in ((0.5, 8, 1), (1.0, 1, 0), (1.5, 64, 1), (3.0, 16, 0)):
         ctx.set_tuning(tfmm.TUNE_M2L_UNIT, unit)
         ctx.set_tuning(tfmm.TUNE_M2L_XCD_RUN, run)
+        ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, sched)
         assert ctx.get_tuning(tfmm.TUNE_M2L_UNIT) == unit
         assert ctx.get_tuning(tfmm.TUNE_M2L_XCD_RUN) == run
         ctx.evaluate(prm)
@@ -42,22 +43,27 @@ def test_gpu_knobs_do_not_change_results(ctx, precision, tol):
         assert np.array_equal(ctx.fields(), f)
     ctx.set_tuning(tfmm.TUNE_M2L_UNIT, 0)
     ctx.set_tuning(tfmm.TUNE_M2L_XCD_RUN, 32)
+    ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, 0)
 
 
 def test_gpu_autotune(ctx):
     x, y, z, q = datasets.cube(60000, seed=2)
     ctx.set_bodies(x, y, z, q)
     prm = tfmm.make_params(64, 0.4, tfmm.MIXED)
-    unit, run, ms = ctx.autotune(prm, reps=1)
-    assert unit in (0.0, 1.5, 2.0, 3.0) and run in (8, 16, 32, 64) and ms > 0
+    unit, run, sched, ms = ctx.autotune(prm, reps=1)
+    assert unit in (0.0, 1.5, 2.0, 3.0) and run in (8, 16, 32, 64) and sched in (0, 1)
+    assert ms > 0
     assert ctx.get_tuning(tfmm.TUNE_M2L_UNIT) == unit
     assert ctx.get_tuning(tfmm.TUNE_M2L_XCD_RUN) == run
+    assert ctx.get_tuning(tfmm.TUNE_P2P_SCHEDULE) == sched
     ctx.set_tuning(tfmm.TUNE_M2L_UNIT, 0)
     ctx.set_tuning(tfmm.TUNE_M2L_XCD_RUN, 32)
+    ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, 0)
 
 
 def test_gpu_invalid_knobs(ctx):
     for knob, value in ((tfmm.TUNE_M2L_UNIT, -1.0), (tfmm.TUNE_M2L_XCD_RUN, 0),
-                        (tfmm.TUNE_M2L_XCD_RUN, 2.5), (99, 1.0)):
+                        (tfmm.TUNE_M2L_XCD_RUN, 2.5), (tfmm.TUNE_P2P_SCHEDULE, 2),
+                        (99, 1.0)):
         with pytest.raises(tfmm.TfmmError):
             ctx.set_tuning(knob, value)
