@@ -56,6 +56,13 @@ def parse():
                     help="keep the default tile-kernel knobs instead of autotuning them")
     ap.add_argument("--unit-factor", type=float, default=None,
                     help="M2L work-unit size in mean items per tile (0: one unit per tile)")
+    ap.add_argument("--xcd-run", type=int, default=None,
+                    help="M2L work units dealt to one XCD in a row")
+    ap.add_argument("--p2p-schedule", type=int, default=None, choices=[0, 1],
+                    help="0: P2P beside the M2L batch building, 1: beside the M2L tiles")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy a 256 MiB device buffer three times (the reference kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     ap.add_argument("--batch", type=int, default=0,
                     help="evaluate this many independent small problems as one forest")
     return ap.parse_args()
@@ -100,7 +107,7 @@ def measured_traffic(workload, kernel):
                     fields[k.strip()] = v.strip()
         if fields.get("workload") == workload and fields.get("kernel") == kernel:
             return {"bytes": float(fields["traffic_bytes_per_launch"]),
-                    "source": os.path.relpath(path, ROOT)}
+                    "source": os.path.relpath(path, ROOT), "knobs": fields.get("knobs")}
     return None
 
 
@@ -154,12 +161,25 @@ def errors(ctx, targets, direct):
     return rel(f[:, 0], direct[:, 0]), rel(f[:, 1:], direct[:, 1:])
 
 
+def calibrate_copy():
+    import torch
+    x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")  # 256 MiB
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "device copy", "bytes_per_copy": x.numel() * 4, "copies": 3}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU; this build has no multi-GPU path (--gpus %d refused)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     import numpy as np
     import torch
 
@@ -183,8 +203,16 @@ def main():
     torch.cuda.synchronize()
 
     ctx = tfmm.Context(0)
+    fixed_knobs = {}
     if args.unit_factor is not None:
         ctx.set_tuning(tfmm.TUNE_M2L_UNIT, args.unit_factor)
+        fixed_knobs["m2l_unit_factor"] = args.unit_factor
+    if args.xcd_run is not None:
+        ctx.set_tuning(tfmm.TUNE_M2L_XCD_RUN, args.xcd_run)
+        fixed_knobs["m2l_xcd_run"] = args.xcd_run
+    if args.p2p_schedule is not None:
+        ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, args.p2p_schedule)
+        fixed_knobs["p2p_schedule"] = args.p2p_schedule
     if probs is not None:
         ctx.set_batch_device(offsets, tx.data_ptr(), ty.data_ptr(), tz.data_ptr(), tq.data_ptr())
     else:
@@ -207,13 +235,13 @@ def main():
     prm = tfmm.make_params(ncrit, theta, precision)
 
     # tile-kernel knobs: the library's autotuner on these bodies (untimed)
-    tuned = None
-    if not args.no_autotune and args.unit_factor is None:
+    tuned = {"fixed": fixed_knobs} if fixed_knobs else None
+    if not args.no_autotune and not fixed_knobs:
         t0 = time.perf_counter()
         unit, run, sched, tms = ctx.autotune(prm, reps=3)
         tuned = {"m2l_unit_factor": unit, "m2l_xcd_run": run, "p2p_schedule": sched,
                  "ms_per_evaluation": tms, "seconds": time.perf_counter() - t0,
-                 "note": "32 candidate settings x 3 evaluations, outside the timed steps"}
+                 "note": "16 candidate settings x 3 evaluations, outside the timed steps"}
 
     # accuracy reference: FP64 direct sum on random targets (untimed)
     rng = np.random.default_rng(123)
@@ -295,6 +323,7 @@ def main():
                      "frac": m2l_tflops / peak,
                      "traffic": traffic["bytes"] if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None,
+                     "traffic_knobs": traffic["knobs"] if traffic else None,
                      "flops_per_pair": M2L_FLOPS_PER_PAIR, "pairs_per_launch": grouped,
                      "ms_per_launch": tiles_ms,
                      "p2p": {"tflops": p2p_tflops, "flops_per_pair": P2P_FLOPS_PER_PAIR,

@@ -36,7 +36,7 @@ int main(int argc, char **argv) {
   std::uniform_real_distribution<double> u(-1, 1);
   double worst = 0;
   for (int t = 0; t < trials; t++) {
-    const int dl = (int)(t % 3) - 1;             // l_t - l_s
+    const int dl = (int)(t % 7) - 3;             // l_t - l_s in -3 .. 3
     const double sig_t = std::ldexp(1.0, -(t % 7)) * (1.0 + 0.5 * (u(rng) + 1));
     const double sig_s = std::ldexp(sig_t, dl);  // sigma_l = w0 2^-l
     double dX[3];

@@ -3,8 +3,8 @@
 tests/native/m2l_operator_check.cpp applies the kernel's operand codes
 (tapas-reduce_amd/csrc/m2l_codes.h), its table of harmonic multiples, the
 level factor split between harmonics and output rows, and the M~/L~ scalings
-in FP64, block by block and lane by lane as the kernel does, for same-level
-and adjacent-level pairs (and pairs on the z axis), and compares the result
+in FP64, block by block and lane by lane as the kernel does, for pairs up to
+three levels apart either way (and pairs on the z axis), and compares the result
 with the oracle's per-pair M2L (LaplaceSphericalCPU_tapas.h:208-273).
 """
 import os
@@ -31,7 +31,7 @@ def checker(tmp_path_factory):
 
 
 def test_block_sparse_m2l_matches_oracle(checker):
-    out = subprocess.run([checker, "90"], check=True, capture_output=True, text=True).stdout
+    out = subprocess.run([checker, "210"], check=True, capture_output=True, text=True).stdout
     rel = float(out.split()[1])
     # FP64 throughout: only rounding separates the two formulations
     assert rel < 1e-13, out
