@@ -60,6 +60,8 @@ def parse():
                     help="M2L work units dealt to one XCD in a row")
     ap.add_argument("--p2p-schedule", type=int, default=None, choices=[0, 1],
                     help="0: P2P beside the M2L batch building, 1: beside the M2L tiles")
+    ap.add_argument("--graph", action="store_true",
+                    help="launch the numeric phase of every evaluation as one hipGraph")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256 MiB device buffer three times (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -203,6 +205,8 @@ def main():
     torch.cuda.synchronize()
 
     ctx = tfmm.Context(0)
+    if args.graph:
+        ctx.set_tuning(tfmm.TUNE_GRAPH, 1)
     fixed_knobs = {}
     if args.unit_factor is not None:
         ctx.set_tuning(tfmm.TUNE_M2L_UNIT, args.unit_factor)
@@ -253,7 +257,7 @@ def main():
 
     pairs = st["n_p2p_calls"] - st["n_p2p_self"]
     p2p_ms = phases["ms_p2p"]
-    p2p_tflops = P2P_FLOPS_PER_PAIR * pairs / (p2p_ms * 1e-3) / 1e12
+    p2p_tflops = P2P_FLOPS_PER_PAIR * pairs / (p2p_ms * 1e-3) / 1e12 if p2p_ms > 0 else 0.0
     peak = PEAK_FP32_TFLOPS if precision == tfmm.MIXED else PEAK_FP64_TFLOPS
     # dominant kernel: the batched M2L tile kernel (one launch per step),
     # timed by HIP events on the context stream around that launch
@@ -313,6 +317,7 @@ def main():
                   "against": "FP64 direct sum"},
         "picker": picked,
         "autotune": tuned,
+        "graph": bool(args.graph),
         "phases_ms": phases,
         "counts": {"cells": st["ncells"], "leaves": st["nleaves"], "max_depth": st["max_depth"],
                    "m2l_pairs": st["n_m2l_exec"], "m2l_groups": st["n_m2l_groups"],

@@ -150,7 +150,9 @@ int tfmm_set_params(tfmm_ctx *ctx, const tfmm_params *params);
 int tfmm_upward(tfmm_ctx *ctx);
 int tfmm_traverse(tfmm_ctx *ctx);
 int tfmm_downward(tfmm_ctx *ctx);
-/* build_tree + upward + traverse + downward */
+/* build_tree + upward + traverse + downward: the traversal and the list
+ * building first, then the numeric kernels (as one hipGraph, see
+ * TFMM_TUNE_GRAPH); results as from the four calls. */
 int tfmm_evaluate(tfmm_ctx *ctx, const tfmm_params *params);
 
 /* Fields (pot, ax, ay, az) per body, 4 doubles each.  order 0: the caller's
@@ -182,17 +184,30 @@ int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p
 /* TFMM_TUNE_M2L_XCD_RUN: consecutive work units dealt to one of the 8 XCDs
  * (each has its own L2; neighbouring tiles share source expansions). */
 #define TFMM_TUNE_M2L_XCD_RUN 2
-/* TFMM_TUNE_P2P_SCHEDULE: 0 starts the P2P kernel (second stream) as soon
- * as its list is sorted, beside the M2L batch building; 1 starts it with
- * the M2L tile kernel, so FP32 VALU and MFMA work share the CUs. */
+/* TFMM_TUNE_P2P_SCHEDULE: where the P2P kernel (second stream) starts.  In
+ * tfmm_evaluate, 0 starts it with the numeric phase, beside the upward pass
+ * and the M2L kernels; 1 starts it with the M2L tile kernel.  In the
+ * separate tfmm_traverse, 0 starts it as soon as its list is sorted, beside
+ * the M2L batch building; 1 again with the tile kernel. */
 #define TFMM_TUNE_P2P_SCHEDULE 3
+/* TFMM_TUNE_GRAPH: 1 records tfmm_evaluate's numeric phase (P2M, M2M, P2P,
+ * M2L, L2L, L2P, whose sizes are fixed once the traversal and the M2L
+ * batches are built) into one hipGraph and launches it, updating the
+ * executable graph in place between evaluations; 0 (default) launches the
+ * same kernels one by one (measured equal in time: the GPU is never short of
+ * queued work).  With a HIP runtime that cannot record timing events inside
+ * a capture, the per-phase times inside the graph read 0.
+ * tfmm_upward / _traverse / _downward called separately always launch one
+ * by one. */
+#define TFMM_TUNE_GRAPH 4
 int tfmm_set_tuning(tfmm_ctx *ctx, int knob, double value);
 int tfmm_get_tuning(tfmm_ctx *ctx, int knob, double *value);
 /* Autotuner: evaluates the current bodies with params under every candidate
- * setting of the knobs (unit factor {0, 1.5, 2, 3} x XCD run {8, 16, 32,
- * 64} x P2P schedule {0, 1}), reps times each, and keeps the setting whose
- * evaluation was fastest (median).  chosen (4 doubles, may be null): unit
- * factor, XCD run, P2P schedule, milliseconds per evaluation. */
+ * setting of the tile-kernel knobs (unit factor {0, 1.5, 2, 3} x XCD run
+ * {8, 16, 32, 64}; the P2P schedule stays as set), reps times each, and
+ * keeps the setting whose evaluation was fastest (median).  chosen (4
+ * doubles, may be null): unit factor, XCD run, P2P schedule, milliseconds
+ * per evaluation. */
 int tfmm_autotune(tfmm_ctx *ctx, const tfmm_params *params, int reps, double *chosen);
 
 /* FP64 direct sum on the current bodies for the given target indices (caller

@@ -34,6 +34,7 @@ enum Ev {
   EV_P2PREADY,
   EV_P2P0,
   EV_P2P1,
+  EV_P2PJOIN,  // the P2P stream's end, for the join only (never timed)
   EV_TRAV,
   EV_DOWN,
   EV_COUNT
@@ -94,6 +95,11 @@ struct tfmm_ctx {
   double m2l_unit_factor = 0.0;  // tile work-unit size in mean items per tile (0: off)
   int m2l_xcd_run = 32;          // consecutive work units dealt to one XCD
   int p2p_schedule = 0;          // 0: P2P beside the M2L build, 1: beside the tiles
+  int use_graph = 0;             // tfmm_evaluate's numeric phase as one hipGraph
+  int ext_events = -1;           // external event records in a capture: -1 not probed
+  bool fused = false;            // last evaluation ran the fused order
+  bool capturing = false;        // inside the numeric phase's stream capture
+  hipGraphExec_t gexec = nullptr;
   int32_t unit_chunk = 0;
   uint32_t units_bound = 0;
   bool codes_ready = false;
@@ -712,6 +718,179 @@ int do_downward(tfmm_ctx *c) {
   return TFMM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// tfmm_evaluate's order.  Phase A holds everything whose sizes the host reads
+// back: the traversal, the list sorts and the M2L batch building, then every
+// buffer the numeric phase writes is reserved.  Phase B, the numeric work
+// (P2M, M2M, P2P, M2L, L2L, L2P), has all its sizes fixed by then: it
+// allocates nothing, reads nothing back and clears memory with a kernel, so
+// it can be recorded into one hipGraph and launched as one unit.  P2P forks
+// to the second stream at the start of phase B and overlaps the upward pass
+// and the M2L kernels.
+int fused_prepare(tfmm_ctx *c) {
+  hipStream_t s = c->stream;
+  const Cells k = cells_of(c);
+  const bool mixed = c->prm.precision == TFMM_MIXED;
+  TRY(hipEventRecord(c->ev[EV_DTT0], s));
+  int rc = do_dtt(c);
+  if (rc) return rc;
+  rc = sort_list(c, c->p2p_pairs, c->n_p2p, c->p2p_off, c->p2p_src);
+  if (rc) return rc;
+  if (mixed) {
+    TRY(dev_reserve(c->relpos, (size_t)c->n * 16));
+    TRY(launch_leaf_relpos(k, (int32_t)c->ncells, sorted_bodies(c), dp<float4>(c->relpos), s));
+  }
+  rc = build_m2l_batches(c);
+  if (rc) return rc;
+  TRY(hipMemcpyAsync(c->h_u64, static_cast<char *>(c->cnt.ptr) + 32, 16, hipMemcpyDeviceToHost, s));
+  const size_t ebytes = (size_t)c->ncells * NTERM * 16;
+  TRY(dev_reserve(c->M, ebytes));
+  TRY(dev_reserve(c->L, ebytes));
+  if (!c->tops_ready) {
+    TRY(dev_reserve(c->tops, translation_ops_doubles() * 8));
+    TRY(launch_translation_ops(dp<double>(c->tops), s));
+    c->tops_ready = true;
+  }
+  TRY(dev_reserve(c->mq, (size_t)c->ncells * 112 * (mixed ? 4 : 8)));
+  TRY(dev_reserve(c->acc, (size_t)c->n * 32));
+  TRY(dev_reserve(c->trg, (size_t)c->n * 32));
+  TRY(dev_reserve(c->out, (size_t)c->n * 32));
+  TRY(dev_reserve(c->leaf_of, (size_t)c->n * 4));
+  if (c->n_fallback)
+    TRY(dev_reserve(c->fb_part, m2l_pairs_scratch_bytes((int32_t)c->n_fallback)));
+  TRY(hipEventRecord(c->ev[EV_DTT1], s));
+  return TFMM_OK;
+}
+
+// A timing event: inside a capture it must become an event node of the graph
+// (an external record), or the host could not read its time afterwards.  A
+// HIP runtime without external records in captures (found by a probe) gets
+// none: the phase times inside the graph then read 0.
+hipError_t rec(tfmm_ctx *c, int ev, hipStream_t st) {
+  if (!c->capturing) return hipEventRecord(c->ev[ev], st);
+  if (c->ext_events != 1) return hipSuccess;
+  return hipEventRecordWithFlags(c->ev[ev], st, hipEventRecordExternal);
+}
+
+// Whether this HIP runtime records external events inside a stream capture
+// (an empty capture with one such record).
+void probe_external_events(tfmm_ctx *c) {
+  hipStream_t s = c->stream;
+  hipGraph_t g = nullptr;
+  bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) == hipSuccess;
+  if (ok) {
+    const bool rec_ok =
+        hipEventRecordWithFlags(c->ev[EV_DOWN], s, hipEventRecordExternal) == hipSuccess;
+    ok = hipStreamEndCapture(s, &g) == hipSuccess && rec_ok;
+  }
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
+  c->ext_events = ok ? 1 : 0;
+}
+
+int fused_numeric(tfmm_ctx *c) {
+  hipStream_t s = c->stream;
+  const Cells k = cells_of(c);
+  const bool mixed = c->prm.precision == TFMM_MIXED;
+  // P2P on the second stream, joined before the downward pass; forked at the
+  // start of the phase (schedule 0) or at the tile kernel (schedule 1)
+  auto fork_p2p = [&]() -> int {
+    TRY(hipEventRecord(c->ev[EV_P2PREADY], s));
+    TRY(hipStreamWaitEvent(c->side, c->ev[EV_P2PREADY], 0));
+    TRY(rec(c, EV_P2P0, c->side));
+    Csr pl{dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src)};
+    if (mixed)
+      TRY(launch_p2p_fp32(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
+                          dp<double>(c->acc), c->side));
+    else
+      TRY(launch_p2p_fp64(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<double>(c->acc),
+                          c->side));
+    TRY(rec(c, EV_P2P1, c->side));
+    TRY(hipEventRecord(c->ev[EV_P2PJOIN], c->side));
+    return TFMM_OK;
+  };
+  int rc;
+  if (c->p2p_schedule == 0 && (rc = fork_p2p())) return rc;
+  // upward
+  const size_t ebytes = (size_t)c->ncells * NTERM * 16;
+  TRY(launch_fill_zero(c->M.ptr, ebytes, s));
+  TRY(launch_p2m(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->M), s));
+  const int nl = (int)c->lvl.size() - 1;
+  for (int d = nl - 2; d >= 0; d--)
+    TRY(launch_m2m(k, c->lvl[d + 1], (c->lvl[d + 2] - c->lvl[d + 1]) / 8, dp<double>(c->tops),
+                   dp<double>(c->M), s));
+  TRY(rec(c, EV_UP, s));
+  // M2L
+  TRY(launch_m2l_scale_m(k, (int32_t)c->ncells, dp<double>(c->M), c->mq.ptr, mixed, s));
+  TRY(launch_fill_zero(c->L.ptr, ebytes, s));
+  TRY(rec(c, EV_M2L0, s));
+  if (c->p2p_schedule == 1 && (rc = fork_p2p())) return rc;
+  const int32_t tile = m2l_tile_size();
+  const int32_t ntiles = (int32_t)((c->ncells + tile - 1) / tile);
+  TRY(launch_m2l_tiles(dp<int32_t>(c->tile_off), dp<int2>(c->units), dp<uint32_t>(c->u_cnt),
+                       dp<uint32_t>(c->u_off), ntiles, c->unit_chunk, c->m2l_xcd_run,
+                       c->units_bound, dp<uint2>(c->cols),
+                       mixed ? (const void *)c->g_y32.ptr : (const void *)c->g_y64.ptr,
+                       c->mq.ptr, dp<uint32_t>(c->codes), k, (int32_t)c->ncells,
+                       dp<double>(c->L), dp<double>(c->u_part), mixed, s));
+  TRY(rec(c, EV_M2LT, s));
+  if (c->n_fallback) {
+    Csr fl{dp<int32_t>(c->fb_off), dp<int32_t>(c->fb_src)};
+    TRY(launch_m2l_fp64(k, (int32_t)c->ncells, fl, (int32_t)c->n_fallback, dp<double>(c->M),
+                        dp<double>(c->L), c->fb_part.ptr, s));
+  }
+  TRY(rec(c, EV_M2L1, s));
+  TRY(hipStreamWaitEvent(s, c->ev[EV_P2PJOIN], 0));
+  TRY(rec(c, EV_TRAV, s));
+  // downward
+  for (int d = 0; d + 1 < nl; d++)
+    TRY(launch_l2l(k, c->lvl[d + 1], (c->lvl[d + 2] - c->lvl[d + 1]) / 8, dp<double>(c->tops),
+                   dp<double>(c->L), s));
+  TRY(launch_l2p(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->L), dp<double>(c->acc),
+                 dp<int32_t>(c->perm), dp<int32_t>(c->leaf_of), dp<double>(c->trg),
+                 dp<double>(c->out), s));
+  TRY(rec(c, EV_DOWN, s));
+  return TFMM_OK;
+}
+
+// Phase B recorded into a graph and launched; the executable graph of the
+// previous evaluation is updated in place when only parameters changed.
+int fused_numeric_graph(tfmm_ctx *c) {
+  hipStream_t s = c->stream;
+  if (c->ext_events < 0) probe_external_events(c);
+  TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+  c->capturing = true;
+  const int rc = fused_numeric(c);
+  c->capturing = false;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return hipfail(c, e, "hipStreamEndCapture");
+  if (c->gexec) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult res;
+    if (hipGraphExecUpdate(c->gexec, g, &bad, &res) != hipSuccess ||
+        res != hipGraphExecUpdateSuccess) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(c->gexec);
+      c->gexec = nullptr;
+    }
+  }
+  hipError_t e2 = hipSuccess;
+  if (!c->gexec) e2 = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e2 != hipSuccess) {
+    c->gexec = nullptr;
+    return hipfail(c, e2, "hipGraphInstantiate");
+  }
+  TRY(hipGraphLaunch(c->gexec, s));
+  if (c->ext_events != 1) TRY(hipEventRecord(c->ev[EV_DOWN], s));  // the evaluation's end
+  return TFMM_OK;
+}
+
 int need_stage(tfmm_ctx *c, int st, const char *what) {
   if (!c) return TFMM_EINVAL;
   if (c->stage < st) return fail(c, TFMM_ESTATE, std::string(what) + " called out of order");
@@ -771,8 +950,12 @@ void tfmm_destroy(tfmm_ctx *c) {
                     &c->fkeys, &c->fvals, &c->fkeys_alt, &c->fvals_alt, &c->it_flag,
                     &c->it_excl, &c->it_seg, &c->it_iflag, &c->it_iexcl, &c->items,
                     &c->tile_off, &c->cols, &c->mq, &c->codes, &c->fb_off, &c->fb_src, &c->fb_part, &c->relpos, &c->leaf_of, &c->tops, &c->slot_of,
-                    &c->feat_acc, &c->feat_x, &c->feat_out};
+                    &c->feat_acc, &c->feat_x, &c->feat_out, &c->d_prob_off, &c->bprob,
+                    &c->pkey, &c->pkey_alt, &c->ppos, &c->ppos_alt, &c->brec, &c->lev,
+                    &c->lev_bsum, &c->lev_status, &c->dtt_geo, &c->dtt_topo, &c->u_cnt,
+                    &c->u_off, &c->units, &c->u_part};
   for (DevBuf *b : bufs) dev_release(*b);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   for (int i = 0; i < EV_COUNT; i++)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   if (c->h_u32) (void)hipHostFree(c->h_u32);
@@ -800,13 +983,20 @@ int tfmm_set_tuning(tfmm_ctx *c, int knob, double value) {
       if (value != 0 && value != 1) return fail(c, TFMM_EINVAL, "P2P schedule must be 0 or 1");
       c->p2p_schedule = (int)value;
       return TFMM_OK;
+    case TFMM_TUNE_GRAPH:
+      if (value != 0 && value != 1) return fail(c, TFMM_EINVAL, "graph must be 0 or 1");
+      c->use_graph = (int)value;
+      return TFMM_OK;
     default:
       return fail(c, TFMM_EINVAL, "unknown tuning knob");
   }
 }
 
-// Candidates of the knobs, tried jointly; the evaluation's time (HIP events
-// around it) decides, median over the repetitions.
+// Candidates of the tile-kernel knobs, tried jointly; the evaluation's time
+// (HIP events around it) decides, median over the repetitions.  The P2P
+// schedule stays as set: running the P2P beside the tiles measured within
+// noise of running it beside the batch building, and it would fold the
+// P2P's time into the tile kernel's.
 int tfmm_autotune(tfmm_ctx *c, const tfmm_params *p, int reps, double *chosen) {
   if (!c) return TFMM_EINVAL;
   if (reps < 1 || reps > 64) return fail(c, TFMM_EINVAL, "reps must be in [1, 64]");
@@ -816,9 +1006,8 @@ int tfmm_autotune(tfmm_ctx *c, const tfmm_params *p, int reps, double *chosen) {
   const int run0 = c->m2l_xcd_run, sched0 = c->p2p_schedule;
   struct Cand { double unit; int run, sched; std::vector<double> ms; };
   std::vector<Cand> cand;
-  for (int sc = 0; sc < 2; sc++)
-    for (double u : kUnit)
-      for (int r : kRun) cand.push_back({u, r, sc, {}});
+  for (double u : kUnit)
+    for (int r : kRun) cand.push_back({u, r, sched0, {}});
   int rc = tfmm_evaluate(c, p);  // sizes the buffers
   for (int rep = 0; rep < reps && rc == TFMM_OK; rep++) {
     for (Cand &k : cand) {
@@ -868,6 +1057,7 @@ int tfmm_get_tuning(tfmm_ctx *c, int knob, double *value) {
     case TFMM_TUNE_M2L_UNIT: *value = c->m2l_unit_factor; return TFMM_OK;
     case TFMM_TUNE_M2L_XCD_RUN: *value = c->m2l_xcd_run; return TFMM_OK;
     case TFMM_TUNE_P2P_SCHEDULE: *value = c->p2p_schedule; return TFMM_OK;
+    case TFMM_TUNE_GRAPH: *value = c->use_graph; return TFMM_OK;
     default: return fail(c, TFMM_EINVAL, "unknown tuning knob");
   }
 }
@@ -1065,6 +1255,7 @@ int tfmm_set_params(tfmm_ctx *c, const tfmm_params *p) {
 int tfmm_upward(tfmm_ctx *c) {
   int rc = need_stage(c, 1, "tfmm_upward");
   if (rc) return rc;
+  c->fused = false;
   rc = do_upward(c);
   if (rc) return rc;
   TRY(hipEventRecord(c->ev[EV_UP], c->stream));
@@ -1094,10 +1285,14 @@ int tfmm_downward(tfmm_ctx *c) {
 
 int tfmm_evaluate(tfmm_ctx *c, const tfmm_params *p) {
   int rc = tfmm_build_tree(c, p);
-  if (!rc) rc = tfmm_upward(c);
-  if (!rc) rc = tfmm_traverse(c);
-  if (!rc) rc = tfmm_downward(c);
-  return rc;
+  if (rc) return rc;
+  c->fused = true;
+  rc = fused_prepare(c);
+  if (rc) return rc;
+  rc = c->use_graph ? fused_numeric_graph(c) : fused_numeric(c);
+  if (rc) return rc;
+  c->stage = 4;
+  return TFMM_OK;
 }
 
 int tfmm_get_fields(tfmm_ctx *c, double *dst, int order, int on_device) {
@@ -1137,13 +1332,22 @@ int tfmm_get_stats(tfmm_ctx *c, tfmm_stats_t *o) {
   }
   auto el = [&](int a, int b) {
     float ms = 0;
-    if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) return 0.0;
+    if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) {
+      (void)hipGetLastError();  // not timed: leave no error for the next call to find
+      return 0.0;
+    }
     return (double)ms;
   };
   if (c->stage >= 1) o->ms_tree = el(EV_BEGIN, EV_TREE);
-  if (c->stage >= 2) o->ms_upward = el(EV_TREE, EV_UP);
+  if (c->fused && c->stage >= 4) {
+    // fused order: traversal and list building, then upward, M2L/P2P, downward
+    o->ms_upward = el(EV_DTT1, EV_UP);
+    o->ms_traverse = el(EV_DTT0, EV_DTT1) + el(EV_UP, EV_TRAV);
+  } else {
+    if (c->stage >= 2) o->ms_upward = el(EV_TREE, EV_UP);
+    if (c->stage >= 3) o->ms_traverse = el(EV_UP, EV_TRAV);
+  }
   if (c->stage >= 3) {
-    o->ms_traverse = el(EV_UP, EV_TRAV);
     o->ms_dtt = el(EV_DTT0, EV_DTT1);
     o->ms_m2l = el(EV_M2L0, EV_M2L1);
     o->ms_m2l_tiles = el(EV_M2L0, EV_M2LT);

@@ -162,7 +162,25 @@ __global__ __launch_bounds__(256) void k_l2p(Cells c, Bodies b, const int32_t *l
   out[4 * o + 3] = t3;
 }
 
+// Zero fill (in place of hipMemsetAsync, which must not be recorded into a
+// captured graph): 16-byte stores, the tail by single bytes.
+__global__ void k_fill_zero(uint4 *p, size_t n16, unsigned char *tail, int ntail) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n16) p[i] = make_uint4(0, 0, 0, 0);
+  if (i < (size_t)ntail) tail[i] = 0;
+}
+
 }  // namespace
+
+hipError_t launch_fill_zero(void *ptr, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return hipSuccess;
+  const size_t n16 = bytes / 16;
+  const int ntail = (int)(bytes - n16 * 16);
+  const size_t nt = n16 > (size_t)ntail ? n16 : (size_t)ntail;
+  k_fill_zero<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(
+      static_cast<uint4 *>(ptr), n16, static_cast<unsigned char *>(ptr) + n16 * 16, ntail);
+  return hipGetLastError();
+}
 
 hipError_t launch_p2m(Cells c, int32_t ncells, Bodies b, double *M, hipStream_t s) {
   if (ncells <= 0) return hipSuccess;

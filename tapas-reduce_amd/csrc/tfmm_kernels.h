@@ -67,6 +67,8 @@ hipError_t launch_children(Cells c, int32_t lbeg, int32_t lcnt, const uint32_t *
                            const double *regions, const uint32_t *sprob, hipStream_t s);
 
 // expansions.hip (FP64)
+// zero fill that can be captured into a hipGraph (ptr 16-byte aligned)
+hipError_t launch_fill_zero(void *ptr, size_t bytes, hipStream_t s);
 hipError_t launch_p2m(Cells c, int32_t ncells, Bodies b, double *M, hipStream_t s);
 // translate.hip (FP64 M2M / L2L on MFMA)
 size_t translation_ops_doubles();

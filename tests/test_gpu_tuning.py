@@ -62,8 +62,53 @@ def test_gpu_autotune(ctx):
 
 
 def test_gpu_invalid_knobs(ctx):
-    for knob, value in ((tfmm.TUNE_M2L_UNIT, -1.0), (tfmm.TUNE_M2L_XCD_RUN, 0),
+    for knob, value in ((tfmm.TUNE_M2L_UNIT, -1.0), (tfmm.TUNE_M2L_XCD_RUN, 0), (tfmm.TUNE_GRAPH, 2),
                         (tfmm.TUNE_M2L_XCD_RUN, 2.5), (tfmm.TUNE_P2P_SCHEDULE, 2),
                         (99, 1.0)):
         with pytest.raises(tfmm.TfmmError):
             ctx.set_tuning(knob, value)
+
+
+@pytest.mark.parametrize("precision", [tfmm.FP64, tfmm.MIXED])
+def test_gpu_graph_replays_match_eager(ctx, precision):
+    """tfmm_evaluate's numeric phase as a hipGraph: every replay (the graph
+    updated in place, or instantiated again when the sizes change) gives bit
+    for bit the fields of the same kernels launched one by one, with eager
+    work and synchronisations between the replays."""
+    prm = tfmm.make_params(32, 0.4, precision)
+    cases = [datasets.plummer(50000, seed=3, charges="uniform"), datasets.cube(30000, seed=4),
+             datasets.plummer(50000, seed=3, charges="uniform")]
+    for i, (x, y, z, q) in enumerate(cases):
+        print("case %d n=%d precision %d" % (i, len(x), precision), flush=True)
+        ctx.set_bodies(x, y, z, q)
+        ctx.set_tuning(tfmm.TUNE_GRAPH, 0)
+        ctx.evaluate(prm)
+        eager = ctx.fields()
+        st_e = ctx.stats()
+        ctx.set_tuning(tfmm.TUNE_GRAPH, 1)
+        for rep in range(3):
+            ctx.evaluate(prm)
+            f = ctx.fields()
+            assert np.array_equal(f, eager), (i, rep)
+            st = ctx.stats()
+            assert st["n_m2l"] == st_e["n_m2l"] and st["n_p2p_calls"] == st_e["n_p2p_calls"]
+            assert st["ms_total"] > 0
+            t = np.array([0, len(x) // 2, len(x) - 1])
+            d = ctx.direct(t)  # eager work between the replays
+            assert np.all(np.isfinite(d))
+    ctx.set_tuning(tfmm.TUNE_GRAPH, 0)
+
+
+def test_gpu_evaluate_matches_phase_calls(ctx):
+    """tfmm_evaluate's order (traversal first, numeric phase after) gives the
+    fields of the four separate phase calls bit for bit."""
+    x, y, z, q = datasets.plummer(40000, seed=6, charges="uniform")
+    ctx.set_bodies(x, y, z, q)
+    prm = tfmm.make_params(64, 0.4, tfmm.MIXED)
+    ctx.evaluate(prm)
+    f1 = ctx.fields()
+    ctx.build_tree(prm)
+    ctx.upward()
+    ctx.traverse()
+    ctx.downward()
+    assert np.array_equal(ctx.fields(), f1)
