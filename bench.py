@@ -47,7 +47,9 @@ def parse():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=3,
                     help="steps of the plain FP64 form timed beside the headline (0: skip)")
-    ap.add_argument("--ncrit", type=int, default=64)
+    ap.add_argument("--ncrit", type=int, default=None,
+                    help="leaf size (default 64; for --batch, the fastest of 64/128/256 at "
+                         "theta 0.4, timed before the steps)")
     ap.add_argument("--theta", type=float, default=0.4)
     ap.add_argument("--targets", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=524288)
@@ -224,7 +226,23 @@ def main():
 
     # parameters: the learned picker (mixed precision) or the fixed ncrit/theta
     picked = None
-    ncrit, theta = args.ncrit, args.theta
+    ncrit, theta = (args.ncrit or 64), args.theta
+    if probs is not None and args.ncrit is None:
+        # a batch fixes theta only: the leaf size is the fastest candidate on
+        # these bodies (the forest has no single-problem statistics for the picker)
+        sweep = {}
+        for nc in (64, 128, 256):
+            p_nc = tfmm.make_params(nc, theta, precision)
+            ctx.evaluate(p_nc)
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(2):
+                ctx.evaluate(p_nc)
+            ctx.sync()
+            sweep[nc] = (time.perf_counter() - t0) / 2 * 1e3
+        ncrit = min(sweep, key=sweep.get)
+        picked = {"ncrit": ncrit, "theta": theta, "sweep_ms": sweep,
+                  "note": "leaf size timed at 64/128/256 (theta fixed), outside the steps"}
     if (precision == tfmm.MIXED and not args.no_learned and probs is None
             and os.path.exists(args.picker)):
         pk = tfmm.Picker.load(args.picker)
@@ -267,13 +285,13 @@ def main():
 
     fp64 = None
     if args.fp64_steps > 0 and precision == tfmm.MIXED:
-        prm64 = tfmm.make_params(args.ncrit, args.theta, tfmm.FP64)
+        prm64 = tfmm.make_params(args.ncrit or 64, args.theta, tfmm.FP64)
         ctx.set_tuning(tfmm.TUNE_M2L_UNIT, 0)  # the plain form keeps the default knobs
         ctx.set_tuning(tfmm.TUNE_M2L_XCD_RUN, 32)
         ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, 0)
         ms64, ph64, st64 = time_config(ctx, prm64, args.fp64_steps, 1)
         e64 = errors(ctx, targets, direct)
-        fp64 = {"value": args.n / (ms64 * 1e-3), "ms_per_step": ms64, "ncrit": args.ncrit,
+        fp64 = {"value": args.n / (ms64 * 1e-3), "ms_per_step": ms64, "ncrit": args.ncrit or 64,
                 "theta": args.theta, "rel_l2_pot": e64[0], "rel_l2_acc": e64[1],
                 "phases_ms": ph64, "speedup_of_headline": ms64 / ms_step}
 
