@@ -200,6 +200,13 @@ int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p
  * tfmm_upward / _traverse / _downward called separately always launch one
  * by one. */
 #define TFMM_TUNE_GRAPH 4
+/* TFMM_TUNE_P2P_WAVES: 0 (default) runs the FP32 P2P as one workgroup per
+ * cell; k in 1..8 runs it as k resident waves per SIMD that take target
+ * leaves from a counter, so it holds a bounded share of the CUs and the
+ * kernels beside it keep slots.  Schedule 2 of TFMM_TUNE_P2P_SCHEDULE
+ * (tfmm_evaluate without a graph only) starts the P2P as soon as its list
+ * is sorted, beside the M2L batch building. */
+#define TFMM_TUNE_P2P_WAVES 5
 int tfmm_set_tuning(tfmm_ctx *ctx, int knob, double value);
 int tfmm_get_tuning(tfmm_ctx *ctx, int knob, double *value);
 /* Autotuner: evaluates the current bodies with params under every candidate

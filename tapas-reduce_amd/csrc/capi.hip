@@ -96,9 +96,13 @@ struct tfmm_ctx {
   int m2l_xcd_run = 32;          // consecutive work units dealt to one XCD
   int p2p_schedule = 0;          // 0: P2P beside the M2L build, 1: beside the tiles
   int use_graph = 0;             // tfmm_evaluate's numeric phase as one hipGraph
+  int p2p_waves = 0;             // resident P2P waves per SIMD (0: one workgroup per cell)
+  int n_cu = 0;                  // compute units of the device
+  DevBuf p2p_next;               // the resident P2P's work counter
   int ext_events = -1;           // external event records in a capture: -1 not probed
   bool fused = false;            // last evaluation ran the fused order
   bool capturing = false;        // inside the numeric phase's stream capture
+  bool p2p_forked = false;       // this evaluation's P2P already started in phase A
   hipGraphExec_t gexec = nullptr;
   int32_t unit_chunk = 0;
   uint32_t units_bound = 0;
@@ -623,6 +627,9 @@ int build_m2l_batches(tfmm_ctx *c) {
   return TFMM_OK;
 }
 
+// resident P2P workgroups (one wave each) for the knob's waves per SIMD
+int p2p_resident(tfmm_ctx *c) { return c->p2p_waves > 0 ? c->p2p_waves * 4 * c->n_cu : 0; }
+
 int do_traverse(tfmm_ctx *c) {
   hipStream_t s = c->stream;
   const Cells k = cells_of(c);
@@ -650,7 +657,8 @@ int do_traverse(tfmm_ctx *c) {
     Csr pl{dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src)};
     if (mixed)
       TRY(launch_p2p_fp32(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
-                          dp<double>(c->acc), c->side));
+                          dp<double>(c->acc), p2p_resident(c), dp<uint32_t>(c->p2p_next),
+                          c->side));
     else
       TRY(launch_p2p_fp64(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<double>(c->acc),
                           c->side));
@@ -727,41 +735,6 @@ int do_downward(tfmm_ctx *c) {
 // it can be recorded into one hipGraph and launched as one unit.  P2P forks
 // to the second stream at the start of phase B and overlaps the upward pass
 // and the M2L kernels.
-int fused_prepare(tfmm_ctx *c) {
-  hipStream_t s = c->stream;
-  const Cells k = cells_of(c);
-  const bool mixed = c->prm.precision == TFMM_MIXED;
-  TRY(hipEventRecord(c->ev[EV_DTT0], s));
-  int rc = do_dtt(c);
-  if (rc) return rc;
-  rc = sort_list(c, c->p2p_pairs, c->n_p2p, c->p2p_off, c->p2p_src);
-  if (rc) return rc;
-  if (mixed) {
-    TRY(dev_reserve(c->relpos, (size_t)c->n * 16));
-    TRY(launch_leaf_relpos(k, (int32_t)c->ncells, sorted_bodies(c), dp<float4>(c->relpos), s));
-  }
-  rc = build_m2l_batches(c);
-  if (rc) return rc;
-  TRY(hipMemcpyAsync(c->h_u64, static_cast<char *>(c->cnt.ptr) + 32, 16, hipMemcpyDeviceToHost, s));
-  const size_t ebytes = (size_t)c->ncells * NTERM * 16;
-  TRY(dev_reserve(c->M, ebytes));
-  TRY(dev_reserve(c->L, ebytes));
-  if (!c->tops_ready) {
-    TRY(dev_reserve(c->tops, translation_ops_doubles() * 8));
-    TRY(launch_translation_ops(dp<double>(c->tops), s));
-    c->tops_ready = true;
-  }
-  TRY(dev_reserve(c->mq, (size_t)c->ncells * 112 * (mixed ? 4 : 8)));
-  TRY(dev_reserve(c->acc, (size_t)c->n * 32));
-  TRY(dev_reserve(c->trg, (size_t)c->n * 32));
-  TRY(dev_reserve(c->out, (size_t)c->n * 32));
-  TRY(dev_reserve(c->leaf_of, (size_t)c->n * 4));
-  if (c->n_fallback)
-    TRY(dev_reserve(c->fb_part, m2l_pairs_scratch_bytes((int32_t)c->n_fallback)));
-  TRY(hipEventRecord(c->ev[EV_DTT1], s));
-  return TFMM_OK;
-}
-
 // A timing event: inside a capture it must become an event node of the graph
 // (an external record), or the host could not read its time afterwards.  A
 // HIP runtime without external records in captures (found by a probe) gets
@@ -788,29 +761,82 @@ void probe_external_events(tfmm_ctx *c) {
   c->ext_events = ok ? 1 : 0;
 }
 
+// The P2P on the second stream (forked from and later joined to the context
+// stream by events).
+int fork_p2p(tfmm_ctx *c) {
+  hipStream_t s = c->stream;
+  const Cells k = cells_of(c);
+  const bool mixed = c->prm.precision == TFMM_MIXED;
+  TRY(hipEventRecord(c->ev[EV_P2PREADY], s));
+  TRY(hipStreamWaitEvent(c->side, c->ev[EV_P2PREADY], 0));
+  TRY(rec(c, EV_P2P0, c->side));
+  Csr pl{dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src)};
+  if (mixed)
+    TRY(launch_p2p_fp32(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
+                        dp<double>(c->acc), p2p_resident(c), dp<uint32_t>(c->p2p_next), c->side));
+  else
+    TRY(launch_p2p_fp64(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<double>(c->acc),
+                        c->side));
+  TRY(rec(c, EV_P2P1, c->side));
+  TRY(hipEventRecord(c->ev[EV_P2PJOIN], c->side));
+  return TFMM_OK;
+}
+
+int fused_prepare(tfmm_ctx *c) {
+  hipStream_t s = c->stream;
+  const Cells k = cells_of(c);
+  const bool mixed = c->prm.precision == TFMM_MIXED;
+  TRY(hipEventRecord(c->ev[EV_DTT0], s));
+  int rc = do_dtt(c);
+  if (rc) return rc;
+  rc = sort_list(c, c->p2p_pairs, c->n_p2p, c->p2p_off, c->p2p_src);
+  if (rc) return rc;
+  if (mixed) {
+    TRY(dev_reserve(c->relpos, (size_t)c->n * 16));
+    TRY(launch_leaf_relpos(k, (int32_t)c->ncells, sorted_bodies(c), dp<float4>(c->relpos), s));
+  }
+  TRY(dev_reserve(c->acc, (size_t)c->n * 32));
+  c->p2p_forked = false;
+  if (c->p2p_schedule == 2 && !c->use_graph) {
+    // the P2P list is complete: start the P2P now, beside the M2L batch
+    // building (with resident waves it leaves that work CU slots)
+    rc = fork_p2p(c);
+    if (rc) return rc;
+    c->p2p_forked = true;
+  }
+  rc = build_m2l_batches(c);
+  if (rc) {
+    if (c->p2p_forked) (void)hipStreamWaitEvent(s, c->ev[EV_P2PJOIN], 0);
+    return rc;
+  }
+  TRY(hipMemcpyAsync(c->h_u64, static_cast<char *>(c->cnt.ptr) + 32, 16, hipMemcpyDeviceToHost, s));
+  const size_t ebytes = (size_t)c->ncells * NTERM * 16;
+  TRY(dev_reserve(c->M, ebytes));
+  TRY(dev_reserve(c->L, ebytes));
+  if (!c->tops_ready) {
+    TRY(dev_reserve(c->tops, translation_ops_doubles() * 8));
+    TRY(launch_translation_ops(dp<double>(c->tops), s));
+    c->tops_ready = true;
+  }
+  TRY(dev_reserve(c->mq, (size_t)c->ncells * 112 * (mixed ? 4 : 8)));
+  TRY(dev_reserve(c->trg, (size_t)c->n * 32));
+  TRY(dev_reserve(c->out, (size_t)c->n * 32));
+  TRY(dev_reserve(c->leaf_of, (size_t)c->n * 4));
+  if (c->n_fallback)
+    TRY(dev_reserve(c->fb_part, m2l_pairs_scratch_bytes((int32_t)c->n_fallback)));
+  TRY(hipEventRecord(c->ev[EV_DTT1], s));
+  return TFMM_OK;
+}
+
 int fused_numeric(tfmm_ctx *c) {
   hipStream_t s = c->stream;
   const Cells k = cells_of(c);
   const bool mixed = c->prm.precision == TFMM_MIXED;
   // P2P on the second stream, joined before the downward pass; forked at the
-  // start of the phase (schedule 0) or at the tile kernel (schedule 1)
-  auto fork_p2p = [&]() -> int {
-    TRY(hipEventRecord(c->ev[EV_P2PREADY], s));
-    TRY(hipStreamWaitEvent(c->side, c->ev[EV_P2PREADY], 0));
-    TRY(rec(c, EV_P2P0, c->side));
-    Csr pl{dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src)};
-    if (mixed)
-      TRY(launch_p2p_fp32(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
-                          dp<double>(c->acc), c->side));
-    else
-      TRY(launch_p2p_fp64(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<double>(c->acc),
-                          c->side));
-    TRY(rec(c, EV_P2P1, c->side));
-    TRY(hipEventRecord(c->ev[EV_P2PJOIN], c->side));
-    return TFMM_OK;
-  };
+  // start of the phase (schedule 0), at the tile kernel (1), or already in
+  // phase A (2)
   int rc;
-  if (c->p2p_schedule == 0 && (rc = fork_p2p())) return rc;
+  if (!c->p2p_forked && c->p2p_schedule != 1 && (rc = fork_p2p(c))) return rc;
   // upward
   const size_t ebytes = (size_t)c->ncells * NTERM * 16;
   TRY(launch_fill_zero(c->M.ptr, ebytes, s));
@@ -824,7 +850,7 @@ int fused_numeric(tfmm_ctx *c) {
   TRY(launch_m2l_scale_m(k, (int32_t)c->ncells, dp<double>(c->M), c->mq.ptr, mixed, s));
   TRY(launch_fill_zero(c->L.ptr, ebytes, s));
   TRY(rec(c, EV_M2L0, s));
-  if (c->p2p_schedule == 1 && (rc = fork_p2p())) return rc;
+  if (!c->p2p_forked && c->p2p_schedule == 1 && (rc = fork_p2p(c))) return rc;
   const int32_t tile = m2l_tile_size();
   const int32_t ntiles = (int32_t)((c->ncells + tile - 1) / tile);
   TRY(launch_m2l_tiles(dp<int32_t>(c->tile_off), dp<int2>(c->units), dp<uint32_t>(c->u_cnt),
@@ -914,6 +940,12 @@ int tfmm_create(int hip_device, tfmm_ctx **out) {
     delete c;
     return TFMM_EHIP;
   }
+  if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, hip_device) !=
+          hipSuccess ||
+      dev_reserve(c->p2p_next, 256) != hipSuccess) {
+    delete c;
+    return TFMM_EHIP;
+  }
   if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) {
     tfmm_destroy(c);
     return TFMM_EHIP;
@@ -953,7 +985,7 @@ void tfmm_destroy(tfmm_ctx *c) {
                     &c->feat_acc, &c->feat_x, &c->feat_out, &c->d_prob_off, &c->bprob,
                     &c->pkey, &c->pkey_alt, &c->ppos, &c->ppos_alt, &c->brec, &c->lev,
                     &c->lev_bsum, &c->lev_status, &c->dtt_geo, &c->dtt_topo, &c->u_cnt,
-                    &c->u_off, &c->units, &c->u_part};
+                    &c->u_off, &c->units, &c->u_part, &c->p2p_next};
   for (DevBuf *b : bufs) dev_release(*b);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   for (int i = 0; i < EV_COUNT; i++)
@@ -980,12 +1012,18 @@ int tfmm_set_tuning(tfmm_ctx *c, int knob, double value) {
       c->m2l_xcd_run = (int)value;
       return TFMM_OK;
     case TFMM_TUNE_P2P_SCHEDULE:
-      if (value != 0 && value != 1) return fail(c, TFMM_EINVAL, "P2P schedule must be 0 or 1");
+      if (value != 0 && value != 1 && value != 2)
+        return fail(c, TFMM_EINVAL, "P2P schedule must be 0, 1 or 2");
       c->p2p_schedule = (int)value;
       return TFMM_OK;
     case TFMM_TUNE_GRAPH:
       if (value != 0 && value != 1) return fail(c, TFMM_EINVAL, "graph must be 0 or 1");
       c->use_graph = (int)value;
+      return TFMM_OK;
+    case TFMM_TUNE_P2P_WAVES:
+      if (!(value >= 0) || !(value <= 8) || value != std::floor(value))
+        return fail(c, TFMM_EINVAL, "P2P waves per SIMD must be an integer in [0, 8]");
+      c->p2p_waves = (int)value;
       return TFMM_OK;
     default:
       return fail(c, TFMM_EINVAL, "unknown tuning knob");
@@ -1058,6 +1096,7 @@ int tfmm_get_tuning(tfmm_ctx *c, int knob, double *value) {
     case TFMM_TUNE_M2L_XCD_RUN: *value = c->m2l_xcd_run; return TFMM_OK;
     case TFMM_TUNE_P2P_SCHEDULE: *value = c->p2p_schedule; return TFMM_OK;
     case TFMM_TUNE_GRAPH: *value = c->use_graph; return TFMM_OK;
+    case TFMM_TUNE_P2P_WAVES: *value = c->p2p_waves; return TFMM_OK;
     default: return fail(c, TFMM_EINVAL, "unknown tuning knob");
   }
 }

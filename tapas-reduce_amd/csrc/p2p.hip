@@ -147,16 +147,14 @@ __device__ inline void kahan_add(float &s, float &c, float v) {
 // into registers while the current chunk is computed from LDS.  The block is
 // one wave, whose LDS operations complete in issue order, so one staging
 // buffer suffices.
-__global__ __launch_bounds__(64) void k_p2p_fp32(Cells c, Csr list, Bodies b,
-                                                const float4 *rel, double *acc) {
-  const int32_t ti = blockIdx.x;
+// The work of one target leaf (one wave); returns at once for other cells.
+__device__ inline void p2p_fp32_leaf(int32_t ti, Cells c, Csr list, Bodies b, const float4 *rel,
+                                     double *acc, float4 *sb, double (*red)[64]) {
   if (c.child[ti] >= 0) return;
   const int32_t nt = c.nb[ti];
   if (nt == 0) return;
   const int32_t tb = c.beg[ti];
   const int lane = threadIdx.x;
-  __shared__ float4 sb[P2P_CHUNK];
-  __shared__ double red[4][64];
   const int32_t lb = list.off[ti], le = list.off[ti + 1];
   const double tcx = c.center[3 * (int64_t)ti], tcy = c.center[3 * (int64_t)ti + 1],
                tcz = c.center[3 * (int64_t)ti + 2];
@@ -301,6 +299,34 @@ __global__ __launch_bounds__(64) void k_p2p_fp32(Cells c, Csr list, Bodies b,
   }
 }
 
+
+__global__ __launch_bounds__(64) void k_p2p_fp32(Cells c, Csr list, Bodies b,
+                                                const float4 *rel, double *acc) {
+  __shared__ float4 sb[P2P_CHUNK];
+  __shared__ double red[4][64];
+  p2p_fp32_leaf(blockIdx.x, c, list, b, rel, acc, sb, red);
+}
+
+// A fixed number of resident waves that take target cells from a counter
+// (zeroed before the launch) until none are left: the P2P then holds a
+// bounded share of the CUs, and kernels on the other stream still find
+// slots.  Each target is computed by one wave exactly as above, so the order
+// in which the waves take them does not change any result.
+__global__ __launch_bounds__(64) void k_p2p_fp32_resident(Cells c, int32_t ncells, Csr list,
+                                                         Bodies b, const float4 *rel,
+                                                         double *acc, uint32_t *next) {
+  __shared__ float4 sb[P2P_CHUNK];
+  __shared__ double red[4][64];
+  __shared__ uint32_t take;
+  while (true) {
+    __syncthreads();
+    if (threadIdx.x == 0) take = atomicAdd(next, 1u);
+    __syncthreads();
+    const uint32_t ti = take;
+    if (ti >= (uint32_t)ncells) return;
+    p2p_fp32_leaf((int32_t)ti, c, list, b, rel, acc, sb, red);
+  }
+}
 }  // namespace
 
 hipError_t launch_leaf_relpos(Cells c, int32_t ncells, Bodies b, float4 *rel, hipStream_t s) {
@@ -310,9 +336,16 @@ hipError_t launch_leaf_relpos(Cells c, int32_t ncells, Bodies b, float4 *rel, hi
 }
 
 hipError_t launch_p2p_fp32(Cells c, int32_t ncells, Csr list, Bodies b, const float4 *rel,
-                           double *acc, hipStream_t s) {
+                           double *acc, int resident, uint32_t *next, hipStream_t s) {
   if (ncells <= 0) return hipSuccess;
-  k_p2p_fp32<<<ncells, 64, 0, s>>>(c, list, b, rel, acc);
+  if (resident > 0 && next) {
+    hipError_t e = launch_fill_zero(next, 4, s);
+    if (e != hipSuccess) return e;
+    const int grid = resident < ncells ? resident : ncells;
+    k_p2p_fp32_resident<<<grid, 64, 0, s>>>(c, ncells, list, b, rel, acc, next);
+  } else {
+    k_p2p_fp32<<<ncells, 64, 0, s>>>(c, list, b, rel, acc);
+  }
   return hipGetLastError();
 }
 

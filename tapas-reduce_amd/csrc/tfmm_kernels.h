@@ -160,8 +160,10 @@ hipError_t launch_m2l_tiles(const int32_t *tile_off, const int2 *units, const ui
 
 // p2p.hip (mixed precision)
 hipError_t launch_leaf_relpos(Cells c, int32_t ncells, Bodies b, float4 *rel, hipStream_t s);
+// resident > 0: that many resident waves take target cells from the counter
+// next (4 bytes, zeroed here); 0: one workgroup per cell
 hipError_t launch_p2p_fp32(Cells c, int32_t ncells, Csr list, Bodies b, const float4 *rel,
-                           double *acc, hipStream_t s);
+                           double *acc, int resident, uint32_t *next, hipStream_t s);
 }  // namespace tfmm
 
 namespace tfmm {

@@ -21,6 +21,7 @@ TUNE_M2L_UNIT = 1
 TUNE_M2L_XCD_RUN = 2
 TUNE_P2P_SCHEDULE = 3
 TUNE_GRAPH = 4
+TUNE_P2P_WAVES = 5
 P = 10
 NTERM = 55
 

@@ -62,8 +62,8 @@ def test_gpu_autotune(ctx):
 
 
 def test_gpu_invalid_knobs(ctx):
-    for knob, value in ((tfmm.TUNE_M2L_UNIT, -1.0), (tfmm.TUNE_M2L_XCD_RUN, 0), (tfmm.TUNE_GRAPH, 2),
-                        (tfmm.TUNE_M2L_XCD_RUN, 2.5), (tfmm.TUNE_P2P_SCHEDULE, 2),
+    for knob, value in ((tfmm.TUNE_M2L_UNIT, -1.0), (tfmm.TUNE_M2L_XCD_RUN, 0), (tfmm.TUNE_GRAPH, 2), (tfmm.TUNE_P2P_WAVES, 9), (tfmm.TUNE_P2P_SCHEDULE, 3),
+                        (tfmm.TUNE_M2L_XCD_RUN, 2.5),
                         (99, 1.0)):
         with pytest.raises(tfmm.TfmmError):
             ctx.set_tuning(knob, value)
@@ -112,3 +112,25 @@ def test_gpu_evaluate_matches_phase_calls(ctx):
     ctx.traverse()
     ctx.downward()
     assert np.array_equal(ctx.fields(), f1)
+
+
+@pytest.mark.parametrize("schedule", [0, 1, 2])
+def test_gpu_resident_p2p_and_schedules(ctx, schedule):
+    """The FP32 P2P as resident waves taking target leaves from a counter, and
+    every P2P schedule, give bit for bit the fields of the default launch."""
+    x, y, z, q = datasets.plummer(60000, seed=12, charges="uniform")
+    ctx.set_bodies(x, y, z, q)
+    prm = tfmm.make_params(32, 0.4, tfmm.MIXED)
+    ctx.set_tuning(tfmm.TUNE_P2P_WAVES, 0)
+    ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, 0)
+    ctx.evaluate(prm)
+    ref = ctx.fields()
+    ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, schedule)
+    for waves in (0, 1, 3):
+        print("schedule %d waves %d" % (schedule, waves), flush=True)
+        ctx.set_tuning(tfmm.TUNE_P2P_WAVES, waves)
+        for _ in range(2):
+            ctx.evaluate(prm)
+            assert np.array_equal(ctx.fields(), ref), (schedule, waves)
+    ctx.set_tuning(tfmm.TUNE_P2P_WAVES, 0)
+    ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, 0)
