@@ -112,13 +112,43 @@ __global__ void k_body_leaf(Cells c, int32_t ncells, int32_t *leaf_of) {
 
 // L2P: trg = p2p / q + local field of the leaf's expansion; one thread per
 // body, written in sorted order (trg) and in input order (out[perm]).
+// The block's bodies are consecutive in sorted order, so they belong to a few
+// runs of one leaf each: the expansions of the first L2P_RUNS runs are
+// staged in LDS and read from there (bodies of later runs read global
+// memory).
+constexpr int L2P_RUNS = 16;
+
 __global__ __launch_bounds__(256) void k_l2p(Cells c, Bodies b, const int32_t *leaf_of,
                                              const double *L, const double *p2p,
                                              const int32_t *perm, double *trg, double *out) {
+  __shared__ cd Lrun[L2P_RUNS][NTERM];
+  __shared__ int32_t run_leaf[L2P_RUNS];
+  __shared__ uint32_t wave_runs[4];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= b.n) return;
-  const int32_t ci = leaf_of[i];
-  const cd *Ls = reinterpret_cast<const cd *>(L) + (int64_t)ci * NTERM;
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
+  const bool valid = i < b.n;
+  const int32_t ci = valid ? leaf_of[i] : -1;
+  // run index of this body: run starts before it in the block
+  const bool start = valid && (i == i0 || leaf_of[i - 1] != ci);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t m = __ballot(start);
+  if (lane == 0) wave_runs[wid] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t before = 0;
+  for (int w = 0; w < wid; w++) before += wave_runs[w];
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int run = (int)(before + (uint32_t)__popcll(m & upto)) - 1;
+  if (start && run < L2P_RUNS) run_leaf[run] = ci;
+  __syncthreads();
+  const int nruns = (int)(wave_runs[0] + wave_runs[1] + wave_runs[2] + wave_runs[3]);
+  const int staged = nruns < L2P_RUNS ? nruns : L2P_RUNS;
+  for (int e = threadIdx.x; e < staged * NTERM; e += blockDim.x) {
+    const int r = e / NTERM, t = e - r * NTERM;
+    Lrun[r][t] = reinterpret_cast<const cd *>(L)[(int64_t)run_leaf[r] * NTERM + t];
+  }
+  __syncthreads();
+  if (!valid) return;
+  const cd *Ls = run < L2P_RUNS ? Lrun[run] : reinterpret_cast<const cd *>(L) + (int64_t)ci * NTERM;
   const double dx = b.x[i] - c.center[3 * (int64_t)ci], dy = b.y[i] - c.center[3 * (int64_t)ci + 1],
                dz = b.z[i] - c.center[3 * (int64_t)ci + 2];
   const double q = b.q[i];
