@@ -275,6 +275,31 @@ def read_ref_output(path):
                 M=M, L=Lx)
 
 
+MPIEXEC = "/opt/conda/bin/mpiexec"
+
+
+def run_ref_ranks(x, y, z, q, ncrit, theta, ranks, timeout=None):
+    """Time the compiled reference with `ranks` MPI ranks (each starting from
+    a slice of the bodies); returns the four phase times (maxima over the
+    ranks), or None when no MPI launcher is available."""
+    exe = ref_binary(count=False)
+    if exe is None or not os.path.exists(MPIEXEC):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        fout = os.path.join(td, "out.bin")
+        write_input(fin, x, y, z, q, ncrit, theta)
+        env = dict(os.environ)
+        env["OMP_NUM_THREADS"] = "1"
+        subprocess.run([MPIEXEC, "-n", str(int(ranks)), exe, fin, fout, "0"], check=True,
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, env=env, timeout=timeout)
+        with open(fout, "rb") as f:
+            buf = f.read()
+        n, p = struct.unpack_from("<qi", buf, 0)
+        t = struct.unpack_from("<4d", buf, 12)
+        return {"n": n, "ranks": p, "times": dict(zip(("tree", "upward", "traverse", "downward"), t))}
+
+
 def run_ref(x, y, z, q, ncrit=64, theta=0.4, count=True, dump_expansions=False, timeout=None):
     """Run the compiled reference on the bodies; returns its dump as a dict."""
     exe = ref_binary(count)

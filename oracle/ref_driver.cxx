@@ -10,7 +10,13 @@
 //
 // Input file (little endian):  int64 n, int32 ncrit, double theta,
 //                              double x[n], y[n], z[n], q[n]
-// Output file:                 see write_out() below.
+// Output file:                 see the end of main() below.
+//
+// Under mpiexec with P > 1 ranks it only times the pipeline: rank r takes
+// bodies [r n / P, (r + 1) n / P) (the reference's Partition then samples
+// and redistributes them, TP/hot.h:1240-1284), every phase is bracketed by
+// barriers, and rank 0 writes int64 n, int32 P and the four phase times
+// (the maxima over the ranks) instead of the dump.
 #define main tapas_reference_app_main
 #include "tapas_exafmm.cxx"
 #undef main
@@ -51,17 +57,54 @@ int main(int argc, char **argv) {
   if (fread(x.data(), 8, n, fi) != (size_t)n || fread(y.data(), 8, n, fi) != (size_t)n ||
       fread(z.data(), 8, n, fi) != (size_t)n || fread(q.data(), 8, n, fi) != (size_t)n) return 2;
   fclose(fi);
+  int rank = 0, nranks = 1;
+  MPI_Comm_rank(MPI_COMM_WORLD, &rank);
+  MPI_Comm_size(MPI_COMM_WORLD, &nranks);
+  const int64_t lo = n * rank / nranks, hi = n * (rank + 1) / nranks;
 
-  Bodies bodies(n);
-  for (int64_t i = 0; i < n; i++) {
-    Body &b = bodies[i];
+  Bodies bodies(hi - lo);
+  for (int64_t i = lo; i < hi; i++) {
+    Body &b = bodies[i - lo];
     b.X[0] = x[i]; b.X[1] = y[i]; b.X[2] = z[i];
     b.SRC = q[i];
     b.IBODY = (int)i;
-    b.IRANK = 0;
+    b.IRANK = rank;
     b.ICELL = 0;
     b.WEIGHT = 1;
     b.TRG = 0;
+  }
+
+  if (nranks > 1) {  // timing only
+    double t[5];
+    MPI_Barrier(MPI_COMM_WORLD);
+    t[0] = GetTime();
+    TapasFMM::Cell *root = TapasFMM::Partition(bodies.data(), bodies.size(), ncrit);
+    MPI_Barrier(MPI_COMM_WORLD);
+    t[1] = GetTime();
+    root->SetOptMutual(0);
+    TapasFMM::Map(FMM_Upward(), *root, (real_t)theta);
+    MPI_Barrier(MPI_COMM_WORLD);
+    t[2] = GetTime();
+    TapasFMM::Map(FMM_DTT(), tapas::Product(*root, *root), 0, 1000, (real_t)theta);
+    MPI_Barrier(MPI_COMM_WORLD);
+    t[3] = GetTime();
+    TapasFMM::Map(FMM_Downward(), *root);
+    MPI_Barrier(MPI_COMM_WORLD);
+    t[4] = GetTime();
+    double ph[4] = {t[1] - t[0], t[2] - t[1], t[3] - t[2], t[4] - t[3]}, mx[4];
+    MPI_Reduce(ph, mx, 4, MPI_DOUBLE, MPI_MAX, 0, MPI_COMM_WORLD);
+    if (rank == 0) {
+      FILE *fo = fopen(argv[2], "wb");
+      if (!fo) { perror("output"); return 2; }
+      put(fo, (int64_t)n);
+      put(fo, (int32_t)nranks);
+      for (int d = 0; d < 4; d++) put(fo, mx[d]);
+      fclose(fo);
+      printf("ref: n=%lld ranks=%d tree=%.3f up=%.3f dtt=%.3f down=%.3f\n", (long long)n,
+             nranks, mx[0], mx[1], mx[2], mx[3]);
+    }
+    MPI_Finalize();
+    return 0;
   }
 
   double t0 = GetTime();
