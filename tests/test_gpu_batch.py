@@ -100,3 +100,64 @@ def test_gpu_batch_errors(ctx):
             ctx.pick_params(pk, tfmm.MIXED)  # the picker describes one problem
     finally:
         pk.close()
+
+
+def test_gpu_batch_degenerate_problems(ctx):
+    """A forest holding a problem whose bodies all coincide (a zero-width
+    root split down to depth 19) and a single-body problem beside ordinary
+    ones: those fields are zero, the others match their single-problem runs."""
+    r = np.random.default_rng(9)
+    same = (np.full(100, 0.25), np.full(100, -0.5), np.full(100, 2.0), r.uniform(-1, 1, 100))
+    a = datasets.cube(3000, seed=21)
+    b = datasets.plummer(5000, seed=22, charges="uniform")
+    lone = (np.array([1.0]), np.array([2.0]), np.array([3.0]), np.array([0.5]))
+    probs = [a, same, b, lone]
+    prm = tfmm.make_params(64, 0.4, tfmm.MIXED)
+    ctx.set_batch(probs)
+    ctx.evaluate(prm)
+    fb = ctx.fields()
+    off = ctx.problem_offsets()
+    assert np.all(np.isfinite(fb))
+    assert not np.any(fb[off[1]:off[2]]) and not np.any(fb[off[3]:off[4]])
+    for i in (0, 2):
+        ctx.set_bodies(*probs[i])
+        ctx.evaluate(prm)
+        ep, ea = oracle.rel_l2(fb[off[i]:off[i + 1]], ctx.fields())
+        assert ep < 1e-6 and ea < 1e-6, (i, ep, ea)
+
+
+def test_gpu_batch_device_matches_host_copy(ctx):
+    """tfmm_set_batch_device (device SoA arrays + offsets) gives the fields of
+    tfmm_set_batch (host AoS copies) bit for bit; bad offsets are rejected."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")  # the runtime the library already uses
+    probs, _ = datasets.batch(6, seed=31, nmin=300, nmax=9000)
+    prm = tfmm.make_params(32, 0.4, tfmm.FP64)
+    ctx.set_batch(probs)
+    ctx.evaluate(prm)
+    ref = ctx.fields()
+    off = np.concatenate([[0], np.cumsum([len(p[0]) for p in probs])]).astype(np.int64)
+    ptrs = []
+    try:
+        for k in range(4):
+            host = np.ascontiguousarray(np.concatenate([p[k] for p in probs]))
+            d = ctypes.c_void_p()
+            assert hip.hipMalloc(ctypes.byref(d), ctypes.c_size_t(host.nbytes)) == 0
+            ptrs.append(d)
+            assert hip.hipMemcpy(d, host.ctypes.data_as(ctypes.c_void_p),
+                                 ctypes.c_size_t(host.nbytes), 1) == 0  # host to device
+        ctx.set_batch_device(off, *[d.value for d in ptrs])
+        ctx.evaluate(prm)
+        assert np.array_equal(ctx.fields(), ref)
+        bad = off.copy()
+        bad[2] = bad[1]  # an empty problem
+        with pytest.raises(tfmm.TfmmError):
+            ctx.set_batch_device(bad, *[d.value for d in ptrs])
+        bad = off + 1  # offsets[0] != 0
+        with pytest.raises(tfmm.TfmmError):
+            ctx.set_batch_device(bad, *[d.value for d in ptrs])
+        ctx.set_batch(probs)  # the context no longer borrows the arrays
+    finally:
+        ctx.sync()
+        for d in ptrs:
+            hip.hipFree(d)

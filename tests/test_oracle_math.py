@@ -317,3 +317,21 @@ def test_root_leaf_is_direct_sum():
     assert o.ncells == 1
     ref = oracle.direct(x, y, z, q, np.arange(50))
     assert np.allclose(o.fields(), ref, rtol=1e-13, atol=1e-13)
+
+
+def test_batch_generator_follows_its_definition():
+    """The batched workload: sizes log-uniform within [2^10, 2^16], the three
+    kinds, charges in [-1, 1], ellipsoid semi-axes 1 : 0.3 : 0.1."""
+    import datasets
+    probs, kinds = datasets.batch(64, seed=3)
+    sizes = np.array([len(p[0]) for p in probs])
+    assert sizes.min() >= 1 << 10 and sizes.max() <= 1 << 16
+    assert set(kinds) <= {"cube", "plummer", "ellipsoid"} and len(set(kinds)) == 3
+    for (x, y, z, q), k in zip(probs, kinds):
+        assert np.all(np.abs(q) <= 1)
+        if k == "ellipsoid":
+            assert np.all((x / 1.0) ** 2 + (y / 0.3) ** 2 + (z / 0.1) ** 2 <= 1 + 1e-12)
+        if k == "cube":
+            assert x.min() >= 0 and x.max() < 1
+    # log-uniform: the median size is near the geometric mean 2^13
+    assert 2 ** 11.5 < np.median(sizes) < 2 ** 14.5
