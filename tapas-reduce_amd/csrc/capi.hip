@@ -103,6 +103,7 @@ struct tfmm_ctx {
   bool fused = false;            // last evaluation ran the fused order
   bool capturing = false;        // inside the numeric phase's stream capture
   bool p2p_forked = false;       // this evaluation's P2P already started in phase A
+  bool inner_timed = true;       // the last evaluation recorded its per-phase events
   hipGraphExec_t gexec = nullptr;
   int32_t unit_chunk = 0;
   uint32_t units_bound = 0;
@@ -1295,6 +1296,7 @@ int tfmm_upward(tfmm_ctx *c) {
   int rc = need_stage(c, 1, "tfmm_upward");
   if (rc) return rc;
   c->fused = false;
+  c->inner_timed = true;
   rc = do_upward(c);
   if (rc) return rc;
   TRY(hipEventRecord(c->ev[EV_UP], c->stream));
@@ -1330,6 +1332,7 @@ int tfmm_evaluate(tfmm_ctx *c, const tfmm_params *p) {
   if (rc) return rc;
   rc = c->use_graph ? fused_numeric_graph(c) : fused_numeric(c);
   if (rc) return rc;
+  c->inner_timed = !c->use_graph || c->ext_events == 1;
   c->stage = 4;
   return TFMM_OK;
 }
@@ -1378,6 +1381,12 @@ int tfmm_get_stats(tfmm_ctx *c, tfmm_stats_t *o) {
     return (double)ms;
   };
   if (c->stage >= 1) o->ms_tree = el(EV_BEGIN, EV_TREE);
+  if (!c->inner_timed) {
+    // a graph without per-phase events: the totals only
+    if (c->stage >= 3) o->ms_dtt = el(EV_DTT0, EV_DTT1);
+    if (c->stage >= 4) o->ms_total = el(EV_BEGIN, EV_DOWN);
+    return TFMM_OK;
+  }
   if (c->fused && c->stage >= 4) {
     // fused order: traversal and list building, then upward, M2L/P2P, downward
     o->ms_upward = el(EV_DTT1, EV_UP);
