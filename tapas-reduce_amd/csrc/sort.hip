@@ -118,10 +118,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const K *keys, int64_
 
 template <class K>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
-    const K *keys_in, const uint32_t *vals_in, K *keys_out, uint32_t *vals_out, int64_t n,
-    int shift, int nbits, const uint32_t *offsets, int nblocks) {
+    const K *__restrict__ keys_in, const uint32_t *__restrict__ vals_in, K *__restrict__ keys_out,
+    uint32_t *__restrict__ vals_out, int64_t n, int shift, int nbits,
+    const uint32_t *__restrict__ offsets, int nblocks) {
   __shared__ uint32_t whist[RS_WAVES][RS_BINS];
   __shared__ uint32_t dstart[RS_BINS];
+  __shared__ uint32_t gshift[RS_BINS];  // global position of the tile's run of digit d, less dstart[d]
   __shared__ K sk[RS_TILE];
   __shared__ uint32_t sv[RS_TILE];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -176,7 +178,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
   uint32_t start = block_excl_scan(mine, &unused);
 #pragma unroll
   for (int u = 0; u < DPT; u++) {
-    dstart[threadIdx.x * DPT + u] = start;
+    const int d = threadIdx.x * DPT + u;
+    dstart[d] = start;
+    // the digit's global offsets, read once here so the write loop below
+    // waits on no global load
+    if (d <= (int)mask) gshift[d] = offsets[(int64_t)d * nblocks + blockIdx.x] - start;
     start += cnt[u];
   }
   __syncthreads();
@@ -198,8 +204,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
   for (int i = threadIdx.x; i < tn; i += RS_THREADS) {
     const K key = sk[i];
     const uint32_t d = digit_of(key, shift, mask);
-    const uint64_t pos =
-        (uint64_t)offsets[(int64_t)d * nblocks + blockIdx.x] + (uint32_t)i - dstart[d];
+    const uint32_t pos = gshift[d] + (uint32_t)i;
     keys_out[pos] = key;
     vals_out[pos] = sv[i];
   }
