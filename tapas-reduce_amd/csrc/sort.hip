@@ -100,7 +100,8 @@ __device__ inline uint32_t digit_of(K k, int shift, uint32_t mask) {
 }
 
 template <class K>
-__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const K *keys, int64_t n, int shift,
+__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const K *__restrict__ keys, int64_t n,
+                                                           int shift,
                                                            uint32_t mask, uint32_t *counts,
                                                            int nblocks) {
   __shared__ uint32_t hist[RS_BINS];
@@ -121,11 +122,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const K *__restrict__ keys_in, const uint32_t *__restrict__ vals_in, K *__restrict__ keys_out,
     uint32_t *__restrict__ vals_out, int64_t n, int shift, int nbits,
     const uint32_t *__restrict__ offsets, int nblocks) {
-  __shared__ uint32_t whist[RS_WAVES][RS_BINS];
+  // The wave histograms are dead once every key knows its place in the tile,
+  // so the tile's staging arrays reuse their LDS (more blocks per CU).
+  constexpr size_t HIST_BYTES = sizeof(uint32_t) * RS_WAVES * RS_BINS;
+  constexpr size_t STAGE_BYTES = (sizeof(K) + sizeof(uint32_t)) * RS_TILE;
+  __shared__ __align__(16) unsigned char pool[HIST_BYTES > STAGE_BYTES ? HIST_BYTES : STAGE_BYTES];
+  uint32_t (*whist)[RS_BINS] = reinterpret_cast<uint32_t (*)[RS_BINS]>(pool);
+  K *sk = reinterpret_cast<K *>(pool);
+  uint32_t *sv = reinterpret_cast<uint32_t *>(pool + sizeof(K) * RS_TILE);
   __shared__ uint32_t dstart[RS_BINS];
   __shared__ uint32_t gshift[RS_BINS];  // global position of the tile's run of digit d, less dstart[d]
-  __shared__ K sk[RS_TILE];
-  __shared__ uint32_t sv[RS_TILE];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const uint32_t mask = (1u << nbits) - 1u;
   for (int i = lane; i < RS_BINS; i += 64) whist[wid][i] = 0;
@@ -186,16 +192,21 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     start += cnt[u];
   }
   __syncthreads();
-  // stage the tile in LDS in digit order (stable), then write each digit's
-  // run to consecutive global positions
+  // each key's place in the tile (digit order, stable); then, with the wave
+  // histograms no longer read, stage the tile in LDS in that order and write
+  // each digit's run to consecutive global positions
+#pragma unroll
+  for (int r = 0; r < RS_ROWS; r++) {
+    const uint32_t d = digit_of(k[r], shift, mask);
+    rank[r] += dstart[d] + whist[wid][d];
+  }
+  __syncthreads();
 #pragma unroll
   for (int r = 0; r < RS_ROWS; r++) {
     const int64_t j = wbase + r * 64 + lane;
     if (j < n) {
-      const uint32_t d = digit_of(k[r], shift, mask);
-      const uint32_t lpos = dstart[d] + whist[wid][d] + rank[r];
-      sk[lpos] = k[r];
-      sv[lpos] = v[r];
+      sk[rank[r]] = k[r];
+      sv[rank[r]] = v[r];
     }
   }
   __syncthreads();
