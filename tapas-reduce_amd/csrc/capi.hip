@@ -409,22 +409,23 @@ int do_upward(tfmm_ctx *c) {
 int sort_list(tfmm_ctx *c, DevBuf &pairs, uint32_t npairs, DevBuf &off, DevBuf &src) {
   hipStream_t s = c->stream;
   const int sbits = bits_for(c->ncells);
-  TRY(dev_reserve(c->lkeys, (size_t)npairs * 8 + 8));
-  TRY(dev_reserve(c->lkeys_alt, (size_t)npairs * 8 + 8));
+  // (target, source) order: targets by a stable radix sort, then each
+  // target's sources sorted in place of the sort's low passes
+  TRY(dev_reserve(c->lkeys, (size_t)npairs * 4 + 4));
+  TRY(dev_reserve(c->lkeys_alt, (size_t)npairs * 4 + 4));
   TRY(dev_reserve(c->lidx, (size_t)npairs * 4 + 4));
   TRY(dev_reserve(c->lidx_alt, (size_t)npairs * 4 + 4));
   TRY(dev_reserve(c->sort_tmp, radix_tmp_bytes(npairs)));
   TRY(dev_reserve(off, (size_t)(c->ncells + 1) * 4));
   TRY(dev_reserve(src, (size_t)npairs * 4 + 4));
-  TRY(launch_pack_pairs(dp<int2>(pairs), npairs, sbits, dp<uint64_t>(c->lkeys),
-                        dp<uint32_t>(c->lidx), s));
+  TRY(launch_split_pairs(dp<int2>(pairs), npairs, dp<uint32_t>(c->lkeys), dp<uint32_t>(c->lidx),
+                         s));
   bool alt = false;
-  TRY(radix_sort_pairs(dp<uint64_t>(c->lkeys), dp<uint32_t>(c->lidx), dp<uint64_t>(c->lkeys_alt),
-                       dp<uint32_t>(c->lidx_alt), npairs, 0, 2 * sbits, c->sort_tmp.ptr, s,
-                       &alt));
-  const uint64_t *sk = dp<uint64_t>(alt ? c->lkeys_alt : c->lkeys);
-  TRY(launch_csr_offsets(sk, npairs, sbits, (int32_t)c->ncells, dp<int32_t>(off), s));
-  TRY(launch_unpack_pairs(sk, npairs, sbits, dp<int32_t>(src), s));
+  TRY(radix_sort_pairs32(dp<uint32_t>(c->lkeys), dp<uint32_t>(c->lidx), dp<uint32_t>(c->lkeys_alt),
+                         dp<uint32_t>(c->lidx_alt), npairs, 0, sbits, c->sort_tmp.ptr, s, &alt));
+  TRY(launch_csr_sorted_sources(dp<uint32_t>(alt ? c->lkeys_alt : c->lkeys),
+                                dp<uint32_t>(alt ? c->lidx_alt : c->lidx), npairs,
+                                (int32_t)c->ncells, dp<int32_t>(off), dp<int32_t>(src), s));
   return TFMM_OK;
 }
 

@@ -93,6 +93,12 @@ hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin
                            uint32_t out_cap, uint32_t *nout, int2 *m2l, uint32_t m2l_cap,
                            int2 *p2p, uint32_t p2p_cap, uint32_t *cnt, const double4 *geo,
                            const int2 *topo, hipStream_t s);
+// leaf-pair list: targets and sources of the pairs; then, from the
+// target-sorted list, the CSR offsets and every target's sources ascending
+hipError_t launch_split_pairs(const int2 *pairs, uint32_t n, uint32_t *tgt, uint32_t *src,
+                              hipStream_t s);
+hipError_t launch_csr_sorted_sources(const uint32_t *tgt, const uint32_t *in, uint32_t n,
+                                     int32_t ncells, int32_t *off, int32_t *out, hipStream_t s);
 hipError_t launch_pack_pairs(const int2 *pairs, uint32_t n, int sbits, uint64_t *keys,
                              uint32_t *idx, hipStream_t s);
 hipError_t launch_unpack_pairs(const uint64_t *keys, uint32_t n, int sbits, int32_t *src,

@@ -185,7 +185,80 @@ __global__ void k_csr_offsets(const uint64_t *keys, uint32_t n, int sbits, int32
   off[t] = (int32_t)lo;
 }
 
+// Leaf-pair list in CSR form, in two steps: a stable radix sort of the
+// pairs by target only, then every target's sources sorted by one wave
+// (rank = number of smaller sources, ties by position).  The result is the
+// (target, source) order of a sort of the whole pair, with half the radix
+// passes.
+__global__ void k_split_pairs(const int2 *pairs, uint32_t n, uint32_t *tgt, uint32_t *src) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int2 p = pairs[t];
+  tgt[t] = (uint32_t)p.x;
+  src[t] = (uint32_t)p.y;
+}
+
+// off[t] = first entry whose target is >= t (t = 0..ncells)
+__global__ void k_csr_offsets32(const uint32_t *tgt, uint32_t n, int32_t ncells, int32_t *off) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ncells) return;
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    if (tgt[mid] < (uint32_t)t) lo = mid + 1;
+    else hi = mid;
+  }
+  off[t] = (int32_t)lo;
+}
+
+constexpr int SEG_CAP = 2048;  // sources of one target staged in LDS
+
+template <class P>
+__device__ inline void rank_sources(P v, int32_t n, int lane, int32_t *out) {
+  for (int32_t i = lane; i < n; i += 64) {
+    const uint32_t x = v[i];
+    int32_t r = 0;
+    for (int32_t k = 0; k < n; k++) {
+      const uint32_t y = v[k];
+      r += (y < x || (y == x && k < i)) ? 1 : 0;
+    }
+    out[r] = (int32_t)x;
+  }
+}
+
+// one wave per target: its sources in ascending order
+__global__ __launch_bounds__(64) void k_seg_sort(const int32_t *off, const uint32_t *in,
+                                                 int32_t *out) {
+  __shared__ uint32_t v[SEG_CAP];
+  const int lane = threadIdx.x;
+  const int32_t b = off[blockIdx.x], n = off[blockIdx.x + 1] - b;
+  if (n <= 0) return;
+  if (n <= SEG_CAP) {
+    for (int32_t i = lane; i < n; i += 64) v[i] = in[b + i];
+    __syncthreads();
+    rank_sources(static_cast<const uint32_t *>(v), n, lane, out + b);
+  } else {
+    rank_sources(in + b, n, lane, out + b);  // long lists rank from global memory
+  }
+}
+
 }  // namespace
+
+hipError_t launch_split_pairs(const int2 *pairs, uint32_t n, uint32_t *tgt, uint32_t *src,
+                              hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_split_pairs<<<(n + 255) / 256, 256, 0, s>>>(pairs, n, tgt, src);
+  return hipGetLastError();
+}
+
+// off (ncells + 1) from the target-sorted list; out: each target's sources
+// (in, in the target-sorted order) in ascending order
+hipError_t launch_csr_sorted_sources(const uint32_t *tgt, const uint32_t *in, uint32_t n,
+                                     int32_t ncells, int32_t *off, int32_t *out, hipStream_t s) {
+  k_csr_offsets32<<<(ncells + 1 + 255) / 256, 256, 0, s>>>(tgt, n, ncells, off);
+  if (n > 0 && ncells > 0) k_seg_sort<<<ncells, 64, 0, s>>>(off, in, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_dtt_cells(Cells c, int32_t ncells, double theta, double4 *geo, int2 *topo,
                             hipStream_t s) {

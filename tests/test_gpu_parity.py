@@ -112,8 +112,36 @@ def test_gpu_lists_match_oracle(ctx):
     assert gpu_pairs(mo, ms) == orc_pairs(om, True)
     assert gpu_pairs(po, ps) == orc_pairs(op, False)
     # sorted by target, then source: deterministic order
-    tgt = np.repeat(np.arange(len(mo) - 1), np.diff(mo))
-    assert np.all(np.diff(tgt.astype(np.int64) * (1 << 32) + ms) > 0)
+    for off, src in ((mo, ms), (po, ps)):
+        tgt = np.repeat(np.arange(len(off) - 1), np.diff(off))
+        assert np.all(np.diff(tgt.astype(np.int64) * (1 << 32) + src) > 0)
+
+
+def test_gpu_long_p2p_list_matches_oracle(ctx):
+    """A large leaf beside a dense cluster of small leaves gets more P2P
+    sources (2,306) than one wave's sort stages in LDS (2,048), so its list is
+    ranked from global memory: the lists still equal the oracle's, in
+    (target, source) order, and the fields match."""
+    rng = np.random.default_rng(7)
+    m = 8000
+    cl = 0.47 + 0.03 * rng.random((m, 3))
+    x = np.concatenate([cl[:, 0], [0.9, 0.0]])
+    y = np.concatenate([cl[:, 1], [0.9, 0.0]])
+    z = np.concatenate([cl[:, 2], [0.9, 0.0]])
+    q = rng.uniform(-1, 1, len(x))
+    f, _ = evaluate(ctx, x, y, z, q, 4, 0.5)
+    c = ctx.cells()
+    (mo, ms), (po, ps) = ctx.lists()
+    assert np.diff(po).max() > 2048
+    o = oracle.Run(x, y, z, q, 4, 0.5, phases=3, record_lists=True)
+    oc = o.cells()
+    om, op = o.lists()
+    tgt = np.repeat(np.arange(len(po) - 1), np.diff(po))
+    assert set(zip(c["key"][tgt].tolist(), c["key"][ps].tolist())) == \
+        set(zip(oc["key"][op[:, 0]].tolist(), oc["key"][op[:, 1]].tolist()))
+    assert np.all(np.diff(tgt.astype(np.int64) * (1 << 32) + ps) > 0)
+    ep, ea = oracle.rel_l2(f, o.fields())
+    assert ep < TOL and ea < TOL, (ep, ea)
 
 
 def test_gpu_c1_matches_reference(ctx):
