@@ -89,7 +89,7 @@ struct tfmm_ctx {
   // batched M2L (offset groups)
   DevBuf g_used, g_gid, g_y64, g_y32, g_dl, g_lt, g_cnt;
   DevBuf gkeys, gvals, gkeys_alt, gvals_alt, fkeys, fvals, fkeys_alt, fvals_alt;
-  DevBuf it_flag, it_excl, it_seg, it_iflag, it_iexcl, items, cols, tile_off, mq, codes;
+  DevBuf it_flag, it_iflag, it_iexcl, items, cols, tile_off, mq, codes;  // it_flag: position in segment
   DevBuf fb_off, fb_src, fb_part, relpos, leaf_of, tops, slot_of;
   DevBuf u_cnt, u_off, units, u_part;
   double m2l_unit_factor = 0.0;  // tile work-unit size in mean items per tile (0: off)
@@ -581,21 +581,16 @@ int build_m2l_batches(tfmm_ctx *c) {
   // work items: runs of <= 16 pairs of one group within one tile
   const size_t ng1 = (size_t)std::max<uint32_t>(ngp, 1);
   TRY(dev_reserve(c->it_flag, ng1 * 4));
-  TRY(dev_reserve(c->it_excl, ng1 * 4));
-  TRY(dev_reserve(c->it_seg, ng1 * 4));
   TRY(dev_reserve(c->it_iflag, ng1 * 4));
   TRY(dev_reserve(c->it_iexcl, ng1 * 4));
-  uint32_t nitems = 0, nseg = 0;
-  TRY(launch_m2l_count_items(gk, ngp, dp<uint32_t>(c->it_flag), dp<uint32_t>(c->it_excl),
-                             dp<uint32_t>(c->it_seg), dp<uint32_t>(c->it_iflag),
-                             dp<uint32_t>(c->it_iexcl), c->scan_tmp.ptr, c->h_u32, &nseg,
-                             &nitems, s));
+  uint32_t nitems = 0;
+  TRY(launch_m2l_count_items(gk, ngp, dp<uint32_t>(c->it_flag), dp<uint32_t>(c->it_iflag),
+                             dp<uint32_t>(c->it_iexcl), c->scan_tmp.ptr, c->h_u32, &nitems, s));
   const size_t ni1 = std::max<uint32_t>(nitems, 1);
   TRY(dev_reserve(c->items, ni1 * 16));
   TRY(dev_reserve(c->cols, ni1 * 16 * 8));
   const uint32_t *gvs = dp<uint32_t>(alt ? c->gvals_alt : c->gvals);
   TRY(launch_m2l_write_items(gk, gvs, ngp, gbits, dp<uint32_t>(c->it_flag),
-                             dp<uint32_t>(c->it_excl), dp<uint32_t>(c->it_seg), nseg,
                              dp<uint32_t>(c->it_iflag), dp<uint32_t>(c->it_iexcl),
                              dp<int8_t>(c->g_dl), dp<int4>(c->items), dp<uint2>(c->cols), s));
   c->n_items = nitems;
@@ -982,7 +977,7 @@ void tfmm_destroy(tfmm_ctx *c) {
                     &c->trg, &c->out, &c->g_used, &c->g_gid, &c->g_y64, &c->g_y32, &c->g_dl,
                     &c->g_lt, &c->g_cnt, &c->gkeys, &c->gvals, &c->gkeys_alt, &c->gvals_alt,
                     &c->fkeys, &c->fvals, &c->fkeys_alt, &c->fvals_alt, &c->it_flag,
-                    &c->it_excl, &c->it_seg, &c->it_iflag, &c->it_iexcl, &c->items,
+                    &c->it_iflag, &c->it_iexcl, &c->items,
                     &c->tile_off, &c->cols, &c->mq, &c->codes, &c->fb_off, &c->fb_src, &c->fb_part, &c->relpos, &c->leaf_of, &c->tops, &c->slot_of,
                     &c->feat_acc, &c->feat_x, &c->feat_out, &c->d_prob_off, &c->bprob,
                     &c->pkey, &c->pkey_alt, &c->ppos, &c->ppos_alt, &c->brec, &c->lev,

@@ -260,25 +260,54 @@ __global__ void k_scale_m(Cells c, int32_t ncells, const double *M, T *Mq) {
 // ---------------------------------------------------------------------------
 // work items
 // ---------------------------------------------------------------------------
-__global__ void k_seg_flags(const uint32_t *keys, uint32_t n, uint32_t *flag) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
-}
+// Position of every grouped pair in its (tile, group) segment, from the
+// sorted keys alone.  A target occurs at most once per segment, so a segment
+// holds at most TILE = 128 pairs: the block reads the 128 keys before its
+// own and 64 after, marks the segment starts (ballot per row of 64 keys),
+// and every pair finds its segment's start among the marks of its row and
+// the two rows before, and the next start among those of its row and the
+// row after.  Items are runs of 16 pairs of one segment: iflag marks a pair
+// that starts one; ipos = position in the segment | (pairs of the item) << 8
+// for an item start.
+constexpr int IP_WAVES = 16;
+constexpr int IP_ROWS = IP_WAVES + 3;  // 2 rows before the block's, 1 after
 
-__global__ void k_seg_starts(const uint32_t *flag, const uint32_t *excl, uint32_t n,
-                             uint32_t *segstart) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !flag[i]) return;
-  segstart[excl[i]] = i;
-}
-
-__global__ void k_item_flags(const uint32_t *flag, const uint32_t *excl,
-                             const uint32_t *segstart, uint32_t n, uint32_t *iflag) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t seg = excl[i] + flag[i] - 1;
-  iflag[i] = ((i - segstart[seg]) % 16u == 0u) ? 1u : 0u;
+__global__ __launch_bounds__(64 * IP_WAVES) void k_item_pos(const uint32_t *keys, uint32_t n,
+                                                            uint32_t *ipos, uint32_t *iflag) {
+  __shared__ uint32_t sk[IP_ROWS * 64];
+  __shared__ uint64_t mark[IP_ROWS];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int64_t w0 = (int64_t)blockIdx.x * (64 * IP_WAVES) - 128;  // key of window slot 0
+  for (int t = threadIdx.x; t < IP_ROWS * 64; t += 64 * IP_WAVES) {
+    const int64_t j = w0 + t;
+    // outside [0, n): a value no key takes (the tile field of a key is at
+    // most 2^tbits - 2), so the last segment ends at n
+    sk[t] = (j >= 0 && j < (int64_t)n) ? keys[j] : 0xffffffffu;
+  }
+  __syncthreads();
+  for (int r = wid; r < IP_ROWS; r += IP_WAVES) {
+    const int t = 64 * r + lane;
+    const uint64_t m = __ballot(t == 0 || sk[t] != sk[t - 1]);
+    if (lane == 0) mark[r] = m;
+  }
+  __syncthreads();
+  const int R = wid + 2;
+  const int64_t i = w0 + 64 * R + lane;
+  if (i >= (int64_t)n) return;
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // bits <= lane
+  int start;  // window slot of the segment's first pair
+  if (mark[R] & upto) start = 64 * R + 63 - __clzll(mark[R] & upto);
+  else if (mark[R - 1]) start = 64 * (R - 1) + 63 - __clzll(mark[R - 1]);
+  else start = 64 * (R - 2) + 63 - __clzll(mark[R - 2]);  // mark[R - 2] has slot 0 at least
+  const int d = min(64 * R + lane - start, TILE - 1);
+  int next;  // window slot of the next segment's first pair
+  if (mark[R] & ~upto) next = 64 * R + __ffsll((unsigned long long)(mark[R] & ~upto)) - 1;
+  else if (mark[R + 1]) next = 64 * (R + 1) + __ffsll((unsigned long long)mark[R + 1]) - 1;
+  else next = 64 * (R + 2);
+  const int cnt = min(16, next - (64 * R + lane));
+  const bool first = (d & 15) == 0;
+  ipos[i] = (uint32_t)d | (first ? (uint32_t)cnt << 8 : 0u);
+  iflag[i] = first ? 1u : 0u;
 }
 
 // Item descriptors (first pair, pairs, group, tile) and the item's 16 column
@@ -287,23 +316,20 @@ __global__ void k_item_flags(const uint32_t *flag, const uint32_t *excl,
 // | group << COL_G).
 // Every column carries the group, so that every lane of the tile kernel can
 // fetch the group's data.
-__global__ void k_items(const uint32_t *keys, const uint32_t *vals, const uint32_t *flag,
-                        const uint32_t *excl, const uint32_t *segstart, uint32_t nseg,
+__global__ void k_items(const uint32_t *keys, const uint32_t *vals, const uint32_t *ipos,
                         const uint32_t *iflag, const uint32_t *iexcl, uint32_t n, int gbits,
                         const int8_t *gdl, int4 *items, uint2 *cols) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t seg = excl[i] + flag[i] - 1;
   const uint32_t item = iexcl[i] + iflag[i] - 1;
-  const uint32_t jj = (i - segstart[seg]) & 15u;
+  const uint32_t ip = ipos[i], jj = ip & 15u;
   const uint32_t k = keys[i], v = vals[i];
   const uint32_t g = k & ((1u << gbits) - 1u);
   const uint32_t gw = (uint32_t)(gdl[g] + DL_MAX) << COL_DL | (g << COL_G);
   cols[(int64_t)item * 16 + jj] =
       make_uint2(v & ((1u << SRC_BITS) - 1u), (v >> SRC_BITS) | 128u | gw);
   if (iflag[i]) {  // first pair of the item: descriptor and padding columns
-    const uint32_t segend = (seg + 1 < nseg) ? segstart[seg + 1] : n;
-    const uint32_t cnt = min(16u, segend - i);
+    const uint32_t cnt = ip >> 8;  // pairs of the segment from i on, at most 16
     int4 it;
     it.x = (int)i;                             // first pair
     it.y = (int)cnt;                           // pairs in the item
@@ -750,27 +776,14 @@ hipError_t launch_m2l_scale_m(Cells c, int32_t ncells, const double *M, void *Mq
   return hipGetLastError();
 }
 
-hipError_t launch_m2l_count_items(const uint32_t *keys, uint32_t n, uint32_t *flag,
-                                  uint32_t *excl, uint32_t *segstart, uint32_t *iflag,
-                                  uint32_t *iexcl, void *scan_tmp, uint32_t *h_pinned,
-                                  uint32_t *nseg_out, uint32_t *nitems_out, hipStream_t s) {
+hipError_t launch_m2l_count_items(const uint32_t *keys, uint32_t n, uint32_t *ipos,
+                                  uint32_t *iflag, uint32_t *iexcl, void *scan_tmp,
+                                  uint32_t *h_pinned, uint32_t *nitems_out, hipStream_t s) {
   *nitems_out = 0;
-  *nseg_out = 0;
   if (n == 0) return hipSuccess;
-  const unsigned nb = (n + 255) / 256;
-  k_seg_flags<<<nb, 256, 0, s>>>(keys, n, flag);
-  hipError_t e = scan_exclusive_u32(flag, excl, n, scan_tmp, s);
-  if (e != hipSuccess) return e;
-  e = hipMemcpyAsync(h_pinned, excl + (n - 1), 4, hipMemcpyDeviceToHost, s);
-  if (e != hipSuccess) return e;
-  e = hipMemcpyAsync(h_pinned + 1, flag + (n - 1), 4, hipMemcpyDeviceToHost, s);
-  if (e != hipSuccess) return e;
-  e = hipStreamSynchronize(s);
-  if (e != hipSuccess) return e;
-  const uint32_t nseg = h_pinned[0] + h_pinned[1];
-  k_seg_starts<<<nb, 256, 0, s>>>(flag, excl, n, segstart);
-  k_item_flags<<<nb, 256, 0, s>>>(flag, excl, segstart, n, iflag);
-  e = scan_exclusive_u32(iflag, iexcl, n, scan_tmp, s);
+  const unsigned per = 64 * IP_WAVES;
+  k_item_pos<<<(n + per - 1) / per, per, 0, s>>>(keys, n, ipos, iflag);
+  hipError_t e = scan_exclusive_u32(iflag, iexcl, n, scan_tmp, s);
   if (e != hipSuccess) return e;
   e = hipMemcpyAsync(h_pinned, iexcl + (n - 1), 4, hipMemcpyDeviceToHost, s);
   if (e != hipSuccess) return e;
@@ -778,19 +791,17 @@ hipError_t launch_m2l_count_items(const uint32_t *keys, uint32_t n, uint32_t *fl
   if (e != hipSuccess) return e;
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) return e;
-  *nseg_out = nseg;
   *nitems_out = h_pinned[0] + h_pinned[1];
   return hipGetLastError();
 }
 
 hipError_t launch_m2l_write_items(const uint32_t *keys, const uint32_t *vals, uint32_t n,
-                                  int gbits, const uint32_t *flag, const uint32_t *excl,
-                                  const uint32_t *segstart, uint32_t nseg, const uint32_t *iflag,
+                                  int gbits, const uint32_t *ipos, const uint32_t *iflag,
                                   const uint32_t *iexcl, const int8_t *gdl, int4 *items,
                                   uint2 *cols, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  k_items<<<(n + 255) / 256, 256, 0, s>>>(keys, vals, flag, excl, segstart, nseg, iflag, iexcl,
-                                          n, gbits, gdl, items, cols);
+  k_items<<<(n + 255) / 256, 256, 0, s>>>(keys, vals, ipos, iflag, iexcl, n, gbits, gdl, items,
+                                          cols);
   return hipGetLastError();
 }
 
