@@ -94,7 +94,7 @@ struct tfmm_ctx {
   DevBuf u_cnt, u_off, units, u_part;
   double m2l_unit_factor = 0.0;  // tile work-unit size in mean items per tile (0: off)
   int m2l_xcd_run = 32;          // consecutive work units dealt to one XCD
-  int p2p_schedule = 0;          // 0: P2P beside the M2L build, 1: beside the tiles
+  int p2p_schedule = 0;          // where the P2P starts (TFMM_TUNE_P2P_SCHEDULE)
   int use_graph = 0;             // tfmm_evaluate's numeric phase as one hipGraph
   int p2p_waves = 0;             // resident P2P waves per SIMD (0: one workgroup per cell)
   int n_cu = 0;                  // compute units of the device
@@ -517,20 +517,22 @@ int build_m2l_batches(tfmm_ctx *c) {
   const uint32_t n = c->n_m2l_exec;
   const int max_depth = (int)c->lvl.size() - 2;
   const int32_t nslots = m2l_group_slots(max_depth);
-  TRY(dev_reserve(c->g_used, (size_t)nslots * 4));
+  // g_used: nslots flags, then the count of pairs outside the table
+  TRY(dev_reserve(c->g_used, (size_t)(nslots + 1) * 4));
   TRY(dev_reserve(c->g_gid, (size_t)nslots * 4));
   TRY(dev_reserve(c->scan_tmp, scan_tmp_bytes(std::max<int64_t>(nslots, n))));
-  TRY(hipMemsetAsync(c->g_used.ptr, 0, (size_t)nslots * 4, s));
+  TRY(hipMemsetAsync(c->g_used.ptr, 0, (size_t)(nslots + 1) * 4, s));
   TRY(dev_reserve(c->slot_of, (size_t)std::max<uint32_t>(n, 1) * 4));
   TRY(launch_m2l_mark_slots(k, dp<int2>(c->m2l_pairs), n, dp<uint32_t>(c->g_used),
-                            dp<int32_t>(c->slot_of), s));
+                            dp<int32_t>(c->slot_of), dp<uint32_t>(c->g_used) + nslots, s));
   TRY(scan_exclusive_u32(dp<uint32_t>(c->g_used), dp<uint32_t>(c->g_gid), nslots,
                          c->scan_tmp.ptr, s));
   TRY(hipMemcpyAsync(c->h_u32, dp<uint32_t>(c->g_gid) + nslots - 1, 4, hipMemcpyDeviceToHost, s));
-  TRY(hipMemcpyAsync(c->h_u32 + 1, dp<uint32_t>(c->g_used) + nslots - 1, 4,
+  TRY(hipMemcpyAsync(c->h_u32 + 1, dp<uint32_t>(c->g_used) + nslots - 1, 8,
                      hipMemcpyDeviceToHost, s));
   TRY(hipStreamSynchronize(s));
   const uint32_t ng = c->h_u32[0] + c->h_u32[1];
+  const uint32_t n_outside = c->h_u32[2];
   c->n_groups = ng;
   const size_t ngr = std::max<uint32_t>(ng, 1);
   TRY(dev_reserve(c->g_y64, ngr * 112 * 8));
@@ -560,12 +562,21 @@ int build_m2l_batches(tfmm_ctx *c) {
         c->ncells >= ((int64_t)1 << m2l_src_bits()))
       return fail(c, TFMM_EHIP, "tree too large for the batched M2L keys");
   }
-  TRY(launch_m2l_classify(dp<int2>(c->m2l_pairs), dp<int32_t>(c->slot_of), n, dp<uint32_t>(c->g_gid), gbits, sbits,
-                          dp<uint32_t>(c->gkeys), dp<uint32_t>(c->gvals), dp<uint64_t>(c->fkeys),
-                          dp<uint32_t>(c->fvals), dp<uint32_t>(c->g_cnt), s));
-  TRY(hipMemcpyAsync(c->h_u32, c->g_cnt.ptr, 8, hipMemcpyDeviceToHost, s));
-  TRY(hipStreamSynchronize(s));
-  const uint32_t ngp = c->h_u32[0], nfb = c->h_u32[1];
+  uint32_t ngp = n, nfb = 0;
+  if (n_outside == 0) {  // the usual case: no compaction, no read-back
+    TRY(launch_m2l_classify_all(dp<int2>(c->m2l_pairs), dp<int32_t>(c->slot_of), n,
+                                dp<uint32_t>(c->g_gid), gbits, dp<uint32_t>(c->gkeys),
+                                dp<uint32_t>(c->gvals), s));
+  } else {
+    TRY(launch_m2l_classify(dp<int2>(c->m2l_pairs), dp<int32_t>(c->slot_of), n, dp<uint32_t>(c->g_gid), gbits, sbits,
+                            dp<uint32_t>(c->gkeys), dp<uint32_t>(c->gvals), dp<uint64_t>(c->fkeys),
+                            dp<uint32_t>(c->fvals), dp<uint32_t>(c->g_cnt), s));
+    TRY(hipMemcpyAsync(c->h_u32, c->g_cnt.ptr, 8, hipMemcpyDeviceToHost, s));
+    TRY(hipStreamSynchronize(s));
+    ngp = c->h_u32[0];
+    nfb = c->h_u32[1];
+    if (nfb != n_outside || ngp + nfb != n) return fail(c, TFMM_EHIP, "M2L pair split is inconsistent");
+  }
   c->n_grouped = ngp;
   c->n_fallback = nfb;
   const int32_t tile = m2l_tile_size();

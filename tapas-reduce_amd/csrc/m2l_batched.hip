@@ -150,15 +150,21 @@ __device__ inline int32_t pair_slot(const Cells &c, int32_t t, int32_t s) {
   return group_slot(lt, ls, o);
 }
 
-// group-table slot of every pair (-1: outside the table), and the used slots
+// group-table slot of every pair (-1: outside the table), the used slots,
+// and the number of pairs outside the table (*nout, one atomic per wave
+// that has any)
 __global__ void k_mark_slots(Cells c, const int2 *pairs, uint32_t n, uint32_t *used,
-                             int32_t *slot_of) {
+                             int32_t *slot_of, uint32_t *nout) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int2 p = pairs[i];
-  const int32_t s = pair_slot(c, p.x, p.y);
-  slot_of[i] = s;
-  if (s >= 0) used[s] = 1u;
+  int32_t s = 0;
+  if (i < n) {
+    const int2 p = pairs[i];
+    s = pair_slot(c, p.x, p.y);
+    slot_of[i] = s;
+    if (s >= 0) used[s] = 1u;
+  }
+  const uint64_t m = __ballot(s < 0);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(nout, (uint32_t)__popcll(m));
 }
 
 // Split the executed M2L pairs into grouped ones (sort key: tile << gbits |
@@ -193,6 +199,18 @@ __global__ __launch_bounds__(1024) void k_classify(const int2 *pairs, const int3
     fkeys[af] = ((uint64_t)(uint32_t)p.x << sbits) | (uint64_t)(uint32_t)p.y;
     fvals[af] = 0;
   }
+}
+
+// k_classify when every pair is in the table: keys and values in pair order
+__global__ void k_classify_all(const int2 *pairs, const int32_t *slot_of, uint32_t n,
+                               const uint32_t *gid_of_slot, int gbits, uint32_t *gkeys,
+                               uint32_t *gvals) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int2 p = pairs[i];
+  const uint64_t tile = (uint32_t)p.x / TILE, in_tile = (uint32_t)p.x % TILE;
+  gkeys[i] = (uint32_t)(tile << gbits) | gid_of_slot[slot_of[i]];
+  gvals[i] = (uint32_t)p.y | (uint32_t)(in_tile << SRC_BITS);
 }
 
 // Harmonics of every used group, FP64, scaled: Y~_p^q = Y_p^q sigma_t^(p+1)
@@ -743,9 +761,18 @@ int32_t m2l_group_slots(int max_depth) { return (max_depth + 1) * SLOTS_PER_LEVE
 int m2l_group_bits() { return 32 - COL_G; }
 
 hipError_t launch_m2l_mark_slots(Cells c, const int2 *pairs, uint32_t n, uint32_t *used,
-                                 int32_t *slot_of, hipStream_t s) {
+                                 int32_t *slot_of, uint32_t *nout, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  k_mark_slots<<<(n + 255) / 256, 256, 0, s>>>(c, pairs, n, used, slot_of);
+  k_mark_slots<<<(n + 255) / 256, 256, 0, s>>>(c, pairs, n, used, slot_of, nout);
+  return hipGetLastError();
+}
+
+hipError_t launch_m2l_classify_all(const int2 *pairs, const int32_t *slot_of, uint32_t n,
+                                   const uint32_t *gid_of_slot, int gbits, uint32_t *gkeys,
+                                   uint32_t *gvals, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_classify_all<<<(n + 255) / 256, 256, 0, s>>>(pairs, slot_of, n, gid_of_slot, gbits, gkeys,
+                                                 gvals);
   return hipGetLastError();
 }
 

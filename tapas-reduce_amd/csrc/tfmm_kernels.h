@@ -130,7 +130,11 @@ int m2l_src_bits();
 hipError_t m2l_upload_codes(uint32_t *dev_codes, hipStream_t s);
 size_t m2l_codes_bytes();
 hipError_t launch_m2l_mark_slots(Cells c, const int2 *pairs, uint32_t n, uint32_t *used,
-                                 int32_t *slot_of, hipStream_t s);
+                                 int32_t *slot_of, uint32_t *nout, hipStream_t s);
+// every pair in the group table: grouped keys and values in pair order
+hipError_t launch_m2l_classify_all(const int2 *pairs, const int32_t *slot_of, uint32_t n,
+                                   const uint32_t *gid_of_slot, int gbits, uint32_t *gkeys,
+                                   uint32_t *gvals, hipStream_t s);
 hipError_t launch_m2l_classify(const int2 *pairs, const int32_t *slot_of, uint32_t n,
                                const uint32_t *gid_of_slot, int gbits, int sbits,
                                uint32_t *gkeys, uint32_t *gvals, uint64_t *fkeys,
