@@ -99,8 +99,6 @@ hipError_t launch_split_pairs(const int2 *pairs, uint32_t n, uint32_t *tgt, uint
                               hipStream_t s);
 hipError_t launch_csr_sorted_sources(const uint32_t *tgt, const uint32_t *in, uint32_t n,
                                      int32_t ncells, int32_t *off, int32_t *out, hipStream_t s);
-hipError_t launch_pack_pairs(const int2 *pairs, uint32_t n, int sbits, uint64_t *keys,
-                             uint32_t *idx, hipStream_t s);
 hipError_t launch_unpack_pairs(const uint64_t *keys, uint32_t n, int sbits, int32_t *src,
                                hipStream_t s);
 hipError_t launch_csr_offsets(const uint64_t *sorted_keys, uint32_t n, int sbits,

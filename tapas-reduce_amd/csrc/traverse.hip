@@ -155,15 +155,6 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
   }
 }
 
-__global__ void k_pack_pairs(const int2 *pairs, uint32_t n, int sbits, uint64_t *keys,
-                             uint32_t *idx) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int2 p = pairs[t];
-  keys[t] = ((uint64_t)(uint32_t)p.x << sbits) | (uint64_t)(uint32_t)p.y;
-  if (idx) idx[t] = t;
-}
-
 __global__ void k_unpack_pairs(const uint64_t *keys, uint32_t n, int sbits, int32_t *src) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -274,13 +265,6 @@ hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin
   if (nin_max == 0) return hipSuccess;
   k_dtt_step<<<(nin_max + DTT_THREADS - 1) / DTT_THREADS, DTT_THREADS, 0, s>>>(
       in, nin_dev, nin_max, out, out_cap, nout, m2l, m2l_cap, p2p, p2p_cap, cnt, geo, topo);
-  return hipGetLastError();
-}
-
-hipError_t launch_pack_pairs(const int2 *pairs, uint32_t n, int sbits, uint64_t *keys,
-                             uint32_t *idx, hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  k_pack_pairs<<<(n + 255) / 256, 256, 0, s>>>(pairs, n, sbits, keys, idx);
   return hipGetLastError();
 }
 
