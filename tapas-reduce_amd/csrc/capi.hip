@@ -451,8 +451,14 @@ int do_dtt(tfmm_ctx *c) {
                        dp<int2>(c->dtt_topo), s));
   // A frontier that overflowed truncates every later step, so the counts
   // seen after an overflow are lower bounds: capacities at least double.
+  // M2L group-table flags (one per slot), then the count of M2L pairs
+  // outside the table, filled by the steps beside the lists
+  const int32_t nslots = m2l_group_slots(max_depth);
+  TRY(dev_reserve(c->g_used, (size_t)(nslots + 1) * 4));
   for (int attempt = 0; attempt < 12; attempt++) {
     TRY(dev_reserve(c->m2l_pairs, (size_t)c->m2l_cap * 8));
+    TRY(dev_reserve(c->slot_of, (size_t)c->m2l_cap * 4));
+    TRY(hipMemsetAsync(c->g_used.ptr, 0, (size_t)(nslots + 1) * 4, s));
     TRY(dev_reserve(c->p2p_pairs, (size_t)c->p2p_cap * 8));
     TRY(dev_reserve(c->front_a, (size_t)c->front_cap * 8));
     TRY(dev_reserve(c->front_b, (size_t)c->front_cap * 8));
@@ -468,7 +474,8 @@ int do_dtt(tfmm_ctx *c) {
       TRY(launch_dtt_step(dp<int2>(c->front_a), cnt + 16 + st, grid[st], dp<int2>(c->front_b),
                           c->front_cap, cnt + 16 + st + 1, dp<int2>(c->m2l_pairs), c->m2l_cap,
                           dp<int2>(c->p2p_pairs), c->p2p_cap, cnt, dp<double4>(c->dtt_geo),
-                          dp<int2>(c->dtt_topo), s));
+                          dp<int2>(c->dtt_topo), k.depth, k.width, dp<int32_t>(c->slot_of),
+                          dp<uint32_t>(c->g_used), dp<uint32_t>(c->g_used) + nslots, s));
       std::swap(c->front_a, c->front_b);
     }
     TRY(hipMemcpyAsync(c->h_u32, cnt, (size_t)cwords * 4, hipMemcpyDeviceToHost, s));
@@ -517,14 +524,10 @@ int build_m2l_batches(tfmm_ctx *c) {
   const uint32_t n = c->n_m2l_exec;
   const int max_depth = (int)c->lvl.size() - 2;
   const int32_t nslots = m2l_group_slots(max_depth);
-  // g_used: nslots flags, then the count of pairs outside the table
-  TRY(dev_reserve(c->g_used, (size_t)(nslots + 1) * 4));
+  // g_used (nslots flags, then the count of pairs outside the table) and
+  // slot_of (every pair's slot) come from the traversal
   TRY(dev_reserve(c->g_gid, (size_t)nslots * 4));
   TRY(dev_reserve(c->scan_tmp, scan_tmp_bytes(std::max<int64_t>(nslots, n))));
-  TRY(hipMemsetAsync(c->g_used.ptr, 0, (size_t)(nslots + 1) * 4, s));
-  TRY(dev_reserve(c->slot_of, (size_t)std::max<uint32_t>(n, 1) * 4));
-  TRY(launch_m2l_mark_slots(k, dp<int2>(c->m2l_pairs), n, dp<uint32_t>(c->g_used),
-                            dp<int32_t>(c->slot_of), dp<uint32_t>(c->g_used) + nslots, s));
   TRY(scan_exclusive_u32(dp<uint32_t>(c->g_used), dp<uint32_t>(c->g_gid), nslots,
                          c->scan_tmp.ptr, s));
   TRY(hipMemcpyAsync(c->h_u32, dp<uint32_t>(c->g_gid) + nslots - 1, 4, hipMemcpyDeviceToHost, s));

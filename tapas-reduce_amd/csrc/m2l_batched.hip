@@ -45,6 +45,7 @@
 #include <vector>
 
 #include "m2l_codes.h"
+#include "m2l_groups.h"
 #include "tfmm_device.h"
 #include "tfmm_kernels.h"
 #include "tfmm_internal.h"
@@ -70,51 +71,19 @@ constexpr int WAVE_SYNC = 16;  // items between block barriers in the tile loop
 // ---------------------------------------------------------------------------
 // group table
 // ---------------------------------------------------------------------------
-// Slot of a (target level, level difference, offset) in the direct-mapped
-// table: offsets are in units of half the finer width; same-level offsets are
-// even with |o| <= 22, offsets across levels odd with |o| <= 23.  An M2L
-// pair's parents were not well separated, so a same-level or adjacent-level
-// offset is at most about 2 (2 / theta + 1) half widths (the table covers
-// theta >= 0.2 there); pairs 2 or 3 levels apart, which a leaf next to a much
-// finer region makes, reach |o| = 15 and 23 at theta = 0.4.
-constexpr int SAME_HALF = 11;                // |o/2| <= 11
-constexpr int SAME_N = 2 * SAME_HALF + 1;    // 23
-constexpr int CROSS_MAX = 23;                // |o| <= 23, odd
-constexpr int CROSS_N = CROSS_MAX + 1;       // (o + 23) / 2 in [0, 23]
-constexpr int DL_MAX = 3;                    // level differences -3 .. 3
-constexpr int SLOTS_SAME = SAME_N * SAME_N * SAME_N;
-constexpr int SLOTS_CROSS = CROSS_N * CROSS_N * CROSS_N;
-constexpr int SLOTS_PER_LEVEL = SLOTS_SAME + 2 * DL_MAX * SLOTS_CROSS;
+// group table (m2l_groups.h)
+using m2lg::CROSS_MAX;
+using m2lg::CROSS_N;
+using m2lg::DL_MAX;
+using m2lg::SAME_HALF;
+using m2lg::SAME_N;
+using m2lg::SLOTS_CROSS;
+using m2lg::SLOTS_PER_LEVEL;
+using m2lg::SLOTS_SAME;
 // column word of the tile kernel: target slot (7 bits) | 128 | (l_t - l_s +
 // DL_MAX) << COL_DL | group << COL_G
 constexpr int COL_DL = 8;
 constexpr int COL_G = 11;
-
-__device__ inline int32_t group_slot(int lt, int ls, const int o[3]) {
-  const int dl = ls - lt;
-  int idx;
-  if (dl == 0) {
-    int a[3];
-    for (int d = 0; d < 3; d++) {
-      if (o[d] & 1) return -1;
-      a[d] = o[d] / 2 + SAME_HALF;
-      if (a[d] < 0 || a[d] >= SAME_N) return -1;
-    }
-    idx = (a[0] * SAME_N + a[1]) * SAME_N + a[2];
-  } else if (dl >= -DL_MAX && dl <= DL_MAX) {
-    int a[3];
-    for (int d = 0; d < 3; d++) {
-      if (!(o[d] & 1)) return -1;
-      a[d] = (o[d] + CROSS_MAX) / 2;
-      if (a[d] < 0 || a[d] >= CROSS_N) return -1;
-    }
-    const int blk = 2 * ((dl > 0 ? dl : -dl) - 1) + (dl < 0 ? 1 : 0);
-    idx = SLOTS_SAME + blk * SLOTS_CROSS + (a[0] * CROSS_N + a[1]) * CROSS_N + a[2];
-  } else {
-    return -1;
-  }
-  return lt * SLOTS_PER_LEVEL + idx;
-}
 
 __device__ inline void slot_decode(int32_t slot, int &lt, int &ls, int o[3]) {
   lt = slot / SLOTS_PER_LEVEL;
@@ -137,35 +106,6 @@ __device__ inline void slot_decode(int32_t slot, int &lt, int &ls, int o[3]) {
   }
 }
 
-// offset of an M2L pair in half widths of the finer cell; slot or -1
-__device__ inline int32_t pair_slot(const Cells &c, int32_t t, int32_t s) {
-  const int lt = c.depth[t], ls = c.depth[s];
-  const double half = 0.5 * fmin(c.width[t], c.width[s]);
-  int o[3];
-  for (int d = 0; d < 3; d++) {
-    const double v = (c.center[3 * (int64_t)t + d] - c.center[3 * (int64_t)s + d]) / half;
-    if (!(fabs(v) < 64.0)) return -1;
-    o[d] = (int)rint(v);
-  }
-  return group_slot(lt, ls, o);
-}
-
-// group-table slot of every pair (-1: outside the table), the used slots,
-// and the number of pairs outside the table (*nout, one atomic per wave
-// that has any)
-__global__ void k_mark_slots(Cells c, const int2 *pairs, uint32_t n, uint32_t *used,
-                             int32_t *slot_of, uint32_t *nout) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  int32_t s = 0;
-  if (i < n) {
-    const int2 p = pairs[i];
-    s = pair_slot(c, p.x, p.y);
-    slot_of[i] = s;
-    if (s >= 0) used[s] = 1u;
-  }
-  const uint64_t m = __ballot(s < 0);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(nout, (uint32_t)__popcll(m));
-}
 
 // Split the executed M2L pairs into grouped ones (sort key: tile << gbits |
 // group, value: source cell | slot in tile << SRC_BITS) and the rest (key:
@@ -759,13 +699,6 @@ size_t m2l_codes_bytes() { return (size_t)2 * NCODE * 4; }
 
 int32_t m2l_group_slots(int max_depth) { return (max_depth + 1) * SLOTS_PER_LEVEL; }
 int m2l_group_bits() { return 32 - COL_G; }
-
-hipError_t launch_m2l_mark_slots(Cells c, const int2 *pairs, uint32_t n, uint32_t *used,
-                                 int32_t *slot_of, uint32_t *nout, hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  k_mark_slots<<<(n + 255) / 256, 256, 0, s>>>(c, pairs, n, used, slot_of, nout);
-  return hipGetLastError();
-}
 
 hipError_t launch_m2l_classify_all(const int2 *pairs, const int32_t *slot_of, uint32_t n,
                                    const uint32_t *gid_of_slot, int gbits, uint32_t *gkeys,

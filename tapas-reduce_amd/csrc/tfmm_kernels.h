@@ -89,10 +89,15 @@ struct DttCounters {
 // per-cell records the traversal steps read (geo: centre + MAC radius)
 hipError_t launch_dtt_cells(Cells c, int32_t ncells, double theta, double4 *geo, int2 *topo,
                             hipStream_t s);
+// One breadth-first traversal step.  Every M2L pair stored also gets its
+// group-table slot (m2l_slot, -1 outside the table), marks used[slot] and
+// counts the pairs outside the table in *n_outside.
 hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin_max, int2 *out,
                            uint32_t out_cap, uint32_t *nout, int2 *m2l, uint32_t m2l_cap,
                            int2 *p2p, uint32_t p2p_cap, uint32_t *cnt, const double4 *geo,
-                           const int2 *topo, hipStream_t s);
+                           const int2 *topo, const int32_t *depth, const double *width,
+                           int32_t *m2l_slot, uint32_t *used, uint32_t *n_outside,
+                           hipStream_t s);
 // leaf-pair list: targets and sources of the pairs; then, from the
 // target-sorted list, the CSR offsets and every target's sources ascending
 hipError_t launch_split_pairs(const int2 *pairs, uint32_t n, uint32_t *tgt, uint32_t *src,
@@ -127,8 +132,6 @@ int m2l_tile_size();
 int m2l_src_bits();
 hipError_t m2l_upload_codes(uint32_t *dev_codes, hipStream_t s);
 size_t m2l_codes_bytes();
-hipError_t launch_m2l_mark_slots(Cells c, const int2 *pairs, uint32_t n, uint32_t *used,
-                                 int32_t *slot_of, uint32_t *nout, hipStream_t s);
 // every pair in the group table: grouped keys and values in pair order
 hipError_t launch_m2l_classify_all(const int2 *pairs, const int32_t *slot_of, uint32_t n,
                                    const uint32_t *gid_of_slot, int gbits, uint32_t *gkeys,

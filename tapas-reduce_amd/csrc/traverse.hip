@@ -25,6 +25,7 @@
 // and one 8-byte record (first child, body count), built once per traversal,
 // instead of six separate arrays: the traversal gathers cells at random, so
 // the number of cache lines a pair touches sets its cost.
+#include "m2l_groups.h"
 #include "tfmm_device.h"
 #include "tfmm_kernels.h"
 
@@ -51,7 +52,10 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
                                                   uint32_t nin_max, int2 *out, uint32_t out_cap,
                                                   uint32_t *nout, int2 *m2l, uint32_t m2l_cap,
                                                   int2 *p2p, uint32_t p2p_cap, uint32_t *cnt,
-                                                  const double4 *geo, const int2 *topo) {
+                                                  const double4 *geo, const int2 *topo,
+                                                  const int32_t *depth, const double *width,
+                                                  int32_t *m2l_slot, uint32_t *used,
+                                                  uint32_t *n_outside) {
 #pragma clang fp contract(off)
   const uint32_t nin = nin_dev ? *nin_dev : nin_max;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -59,11 +63,13 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
   int32_t i = 0, j = 0;
   int2 ti = make_int2(0, 0), tj = make_int2(0, 0);
   bool far = false, p2p_pair = false, split_i = false, split_j = false, nonempty = false;
+  double4 gi = make_double4(0, 0, 0, 0), gj = gi;
   if (valid) {
     const int2 pr = in[t];
     i = pr.x;
     j = pr.y;
-    const double4 gi = geo[i], gj = geo[j];
+    gi = geo[i];
+    gj = geo[j];
     ti = topo[i];
     tj = topo[j];
     const double d0 = gi.x - gj.x;
@@ -142,7 +148,21 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
     for (int w = 0; w < wid; w++) pre += wc[w][q];
     slot[q] = pre + (uint32_t)__popcll(m[q] & lt) * (q == 0 ? 8u : 1u);
   }
-  if (far && nonempty && slot[1] < m2l_cap) m2l[slot[1]] = make_int2(i, j);
+  bool outside = false;
+  if (far && nonempty && slot[1] < m2l_cap) {
+    m2l[slot[1]] = make_int2(i, j);
+    // the pair's M2L offset group (same arithmetic on the same centres as
+    // a separate pass over the list would do)
+    const double ci[3] = {gi.x, gi.y, gi.z}, cj[3] = {gj.x, gj.y, gj.z};
+    const int32_t g = m2lg::pair_slot(depth[i], depth[j], width[i], width[j], ci, cj);
+    m2l_slot[slot[1]] = g;
+    if (g >= 0) used[g] = 1u;
+    outside = g < 0;
+  }
+  {
+    const uint64_t mo = __ballot(outside);
+    if (lane == 0 && mo) atomicAdd(n_outside, (uint32_t)__popcll(mo));
+  }
   if (p2p_pair && slot[2] < p2p_cap) p2p[slot[2]] = make_int2(i, j);
   if (split && slot[0] + 8 <= out_cap) {
     if (split_i) {
@@ -261,10 +281,13 @@ hipError_t launch_dtt_cells(Cells c, int32_t ncells, double theta, double4 *geo,
 hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin_max, int2 *out,
                            uint32_t out_cap, uint32_t *nout, int2 *m2l, uint32_t m2l_cap,
                            int2 *p2p, uint32_t p2p_cap, uint32_t *cnt, const double4 *geo,
-                           const int2 *topo, hipStream_t s) {
+                           const int2 *topo, const int32_t *depth, const double *width,
+                           int32_t *m2l_slot, uint32_t *used, uint32_t *n_outside,
+                           hipStream_t s) {
   if (nin_max == 0) return hipSuccess;
   k_dtt_step<<<(nin_max + DTT_THREADS - 1) / DTT_THREADS, DTT_THREADS, 0, s>>>(
-      in, nin_dev, nin_max, out, out_cap, nout, m2l, m2l_cap, p2p, p2p_cap, cnt, geo, topo);
+      in, nin_dev, nin_max, out, out_cap, nout, m2l, m2l_cap, p2p, p2p_cap, cnt, geo, topo, depth,
+      width, m2l_slot, used, n_outside);
   return hipGetLastError();
 }
 
