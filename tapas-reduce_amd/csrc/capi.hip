@@ -89,7 +89,8 @@ struct tfmm_ctx {
   // batched M2L (offset groups)
   DevBuf g_used, g_gid, g_y64, g_y32, g_dl, g_lt, g_cnt;
   DevBuf gkeys, gvals, gkeys_alt, gvals_alt, fkeys, fvals, fkeys_alt, fvals_alt;
-  DevBuf it_flag, it_iflag, it_iexcl, items, cols, tile_off, mq, codes;  // it_flag: position in segment
+  // it_flag: position in segment per grouped pair; it_iflag/it_iexcl: item starts per block
+  DevBuf it_flag, it_iflag, it_iexcl, items, cols, tile_off, mq, codes;
   DevBuf fb_off, fb_src, fb_part, relpos, leaf_of, tops, slot_of;
   DevBuf u_cnt, u_off, units, u_part;
   double m2l_unit_factor = 0.0;  // tile work-unit size in mean items per tile (0: off)
@@ -595,8 +596,9 @@ int build_m2l_batches(tfmm_ctx *c) {
   // work items: runs of <= 16 pairs of one group within one tile
   const size_t ng1 = (size_t)std::max<uint32_t>(ngp, 1);
   TRY(dev_reserve(c->it_flag, ng1 * 4));
-  TRY(dev_reserve(c->it_iflag, ng1 * 4));
-  TRY(dev_reserve(c->it_iexcl, ng1 * 4));
+  const size_t nblk = (ng1 + 1023) / 1024;
+  TRY(dev_reserve(c->it_iflag, nblk * 4));
+  TRY(dev_reserve(c->it_iexcl, nblk * 4));
   uint32_t nitems = 0;
   TRY(launch_m2l_count_items(gk, ngp, dp<uint32_t>(c->it_flag), dp<uint32_t>(c->it_iflag),
                              dp<uint32_t>(c->it_iexcl), c->scan_tmp.ptr, c->h_u32, &nitems, s));
@@ -605,8 +607,8 @@ int build_m2l_batches(tfmm_ctx *c) {
   TRY(dev_reserve(c->cols, ni1 * 16 * 8));
   const uint32_t *gvs = dp<uint32_t>(alt ? c->gvals_alt : c->gvals);
   TRY(launch_m2l_write_items(gk, gvs, ngp, gbits, dp<uint32_t>(c->it_flag),
-                             dp<uint32_t>(c->it_iflag), dp<uint32_t>(c->it_iexcl),
-                             dp<int8_t>(c->g_dl), dp<int4>(c->items), dp<uint2>(c->cols), s));
+                             dp<uint32_t>(c->it_iexcl), dp<int8_t>(c->g_dl), dp<int4>(c->items),
+                             dp<uint2>(c->cols), s));
   c->n_items = nitems;
   TRY(dev_reserve(c->tile_off, (size_t)(ntiles + 1) * 4));
   TRY(launch_m2l_tile_offsets(dp<int4>(c->items), nitems, ntiles, dp<int32_t>(c->tile_off), s));

@@ -224,16 +224,18 @@ __global__ void k_scale_m(Cells c, int32_t ncells, const double *M, T *Mq) {
 // own and 64 after, marks the segment starts (ballot per row of 64 keys),
 // and every pair finds its segment's start among the marks of its row and
 // the two rows before, and the next start among those of its row and the
-// row after.  Items are runs of 16 pairs of one segment: iflag marks a pair
-// that starts one; ipos = position in the segment | (pairs of the item) << 8
-// for an item start.
+// row after.  Items are runs of 16 pairs of one segment: a pair starts one
+// when its position in the segment is a multiple of 16; ipos = position in
+// the segment | (pairs of the item) << 8 for an item start, and bcount[b]
+// the item starts among block b's pairs.
 constexpr int IP_WAVES = 16;
 constexpr int IP_ROWS = IP_WAVES + 3;  // 2 rows before the block's, 1 after
 
 __global__ __launch_bounds__(64 * IP_WAVES) void k_item_pos(const uint32_t *keys, uint32_t n,
-                                                            uint32_t *ipos, uint32_t *iflag) {
+                                                            uint32_t *ipos, uint32_t *bcount) {
   __shared__ uint32_t sk[IP_ROWS * 64];
   __shared__ uint64_t mark[IP_ROWS];
+  __shared__ uint32_t wcount[IP_WAVES];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int64_t w0 = (int64_t)blockIdx.x * (64 * IP_WAVES) - 128;  // key of window slot 0
   for (int t = threadIdx.x; t < IP_ROWS * 64; t += 64 * IP_WAVES) {
@@ -251,21 +253,30 @@ __global__ __launch_bounds__(64 * IP_WAVES) void k_item_pos(const uint32_t *keys
   __syncthreads();
   const int R = wid + 2;
   const int64_t i = w0 + 64 * R + lane;
-  if (i >= (int64_t)n) return;
-  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // bits <= lane
-  int start;  // window slot of the segment's first pair
-  if (mark[R] & upto) start = 64 * R + 63 - __clzll(mark[R] & upto);
-  else if (mark[R - 1]) start = 64 * (R - 1) + 63 - __clzll(mark[R - 1]);
-  else start = 64 * (R - 2) + 63 - __clzll(mark[R - 2]);  // mark[R - 2] has slot 0 at least
-  const int d = min(64 * R + lane - start, TILE - 1);
-  int next;  // window slot of the next segment's first pair
-  if (mark[R] & ~upto) next = 64 * R + __ffsll((unsigned long long)(mark[R] & ~upto)) - 1;
-  else if (mark[R + 1]) next = 64 * (R + 1) + __ffsll((unsigned long long)mark[R + 1]) - 1;
-  else next = 64 * (R + 2);
-  const int cnt = min(16, next - (64 * R + lane));
-  const bool first = (d & 15) == 0;
-  ipos[i] = (uint32_t)d | (first ? (uint32_t)cnt << 8 : 0u);
-  iflag[i] = first ? 1u : 0u;
+  bool first = false;
+  if (i < (int64_t)n) {
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // bits <= lane
+    int start;  // window slot of the segment's first pair
+    if (mark[R] & upto) start = 64 * R + 63 - __clzll(mark[R] & upto);
+    else if (mark[R - 1]) start = 64 * (R - 1) + 63 - __clzll(mark[R - 1]);
+    else start = 64 * (R - 2) + 63 - __clzll(mark[R - 2]);  // mark[R - 2] has slot 0 at least
+    const int d = min(64 * R + lane - start, TILE - 1);
+    int next;  // window slot of the next segment's first pair
+    if (mark[R] & ~upto) next = 64 * R + __ffsll((unsigned long long)(mark[R] & ~upto)) - 1;
+    else if (mark[R + 1]) next = 64 * (R + 1) + __ffsll((unsigned long long)mark[R + 1]) - 1;
+    else next = 64 * (R + 2);
+    const int cnt = min(16, next - (64 * R + lane));
+    first = (d & 15) == 0;
+    ipos[i] = (uint32_t)d | (first ? (uint32_t)cnt << 8 : 0u);
+  }
+  const uint64_t fm = __ballot(first);
+  if (lane == 0) wcount[wid] = (uint32_t)__popcll(fm);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < IP_WAVES; w++) t += wcount[w];
+    bcount[blockIdx.x] = t;
+  }
 }
 
 // Item descriptors (first pair, pairs, group, tile) and the item's 16 column
@@ -274,19 +285,35 @@ __global__ __launch_bounds__(64 * IP_WAVES) void k_item_pos(const uint32_t *keys
 // | group << COL_G).
 // Every column carries the group, so that every lane of the tile kernel can
 // fetch the group's data.
-__global__ void k_items(const uint32_t *keys, const uint32_t *vals, const uint32_t *ipos,
-                        const uint32_t *iflag, const uint32_t *iexcl, uint32_t n, int gbits,
-                        const int8_t *gdl, int4 *items, uint2 *cols) {
+// Blocks as in k_item_pos; bstart: exclusive prefix of its block counts.
+__global__ __launch_bounds__(64 * IP_WAVES) void k_items(const uint32_t *keys,
+                                                         const uint32_t *vals,
+                                                         const uint32_t *ipos,
+                                                         const uint32_t *bstart, uint32_t n,
+                                                         int gbits, const int8_t *gdl,
+                                                         int4 *items, uint2 *cols) {
+  __shared__ uint32_t wcount[IP_WAVES];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t item = iexcl[i] + iflag[i] - 1;
-  const uint32_t ip = ipos[i], jj = ip & 15u;
+  const bool valid = i < n;
+  const uint32_t ip = valid ? ipos[i] : 1u;
+  const bool first = valid && (ip & 15u) == 0;
+  // item of pair i = item starts at or before i, less one
+  const uint64_t fm = __ballot(first);
+  if (lane == 0) wcount[wid] = (uint32_t)__popcll(fm);
+  __syncthreads();
+  if (!valid) return;
+  uint32_t before = bstart[blockIdx.x];
+  for (int w = 0; w < wid; w++) before += wcount[w];
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const uint32_t item = before + (uint32_t)__popcll(fm & upto) - 1;
+  const uint32_t jj = ip & 15u;
   const uint32_t k = keys[i], v = vals[i];
   const uint32_t g = k & ((1u << gbits) - 1u);
   const uint32_t gw = (uint32_t)(gdl[g] + DL_MAX) << COL_DL | (g << COL_G);
   cols[(int64_t)item * 16 + jj] =
       make_uint2(v & ((1u << SRC_BITS) - 1u), (v >> SRC_BITS) | 128u | gw);
-  if (iflag[i]) {  // first pair of the item: descriptor and padding columns
+  if (first) {  // first pair of the item: descriptor and padding columns
     const uint32_t cnt = ip >> 8;  // pairs of the segment from i on, at most 16
     int4 it;
     it.x = (int)i;                             // first pair
@@ -737,17 +764,17 @@ hipError_t launch_m2l_scale_m(Cells c, int32_t ncells, const double *M, void *Mq
 }
 
 hipError_t launch_m2l_count_items(const uint32_t *keys, uint32_t n, uint32_t *ipos,
-                                  uint32_t *iflag, uint32_t *iexcl, void *scan_tmp,
+                                  uint32_t *bcount, uint32_t *bstart, void *scan_tmp,
                                   uint32_t *h_pinned, uint32_t *nitems_out, hipStream_t s) {
   *nitems_out = 0;
   if (n == 0) return hipSuccess;
-  const unsigned per = 64 * IP_WAVES;
-  k_item_pos<<<(n + per - 1) / per, per, 0, s>>>(keys, n, ipos, iflag);
-  hipError_t e = scan_exclusive_u32(iflag, iexcl, n, scan_tmp, s);
+  const unsigned per = 64 * IP_WAVES, nb = (n + per - 1) / per;
+  k_item_pos<<<nb, per, 0, s>>>(keys, n, ipos, bcount);
+  hipError_t e = scan_exclusive_u32(bcount, bstart, nb, scan_tmp, s);
   if (e != hipSuccess) return e;
-  e = hipMemcpyAsync(h_pinned, iexcl + (n - 1), 4, hipMemcpyDeviceToHost, s);
+  e = hipMemcpyAsync(h_pinned, bstart + (nb - 1), 4, hipMemcpyDeviceToHost, s);
   if (e != hipSuccess) return e;
-  e = hipMemcpyAsync(h_pinned + 1, iflag + (n - 1), 4, hipMemcpyDeviceToHost, s);
+  e = hipMemcpyAsync(h_pinned + 1, bcount + (nb - 1), 4, hipMemcpyDeviceToHost, s);
   if (e != hipSuccess) return e;
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) return e;
@@ -756,12 +783,12 @@ hipError_t launch_m2l_count_items(const uint32_t *keys, uint32_t n, uint32_t *ip
 }
 
 hipError_t launch_m2l_write_items(const uint32_t *keys, const uint32_t *vals, uint32_t n,
-                                  int gbits, const uint32_t *ipos, const uint32_t *iflag,
-                                  const uint32_t *iexcl, const int8_t *gdl, int4 *items,
-                                  uint2 *cols, hipStream_t s) {
+                                  int gbits, const uint32_t *ipos, const uint32_t *bstart,
+                                  const int8_t *gdl, int4 *items, uint2 *cols, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  k_items<<<(n + 255) / 256, 256, 0, s>>>(keys, vals, ipos, iflag, iexcl, n, gbits, gdl, items,
-                                          cols);
+  const unsigned per = 64 * IP_WAVES;
+  k_items<<<(n + per - 1) / per, per, 0, s>>>(keys, vals, ipos, bstart, n, gbits, gdl, items,
+                                             cols);
   return hipGetLastError();
 }
 

@@ -146,14 +146,14 @@ hipError_t launch_m2l_group_harmonics(const uint32_t *used, const uint32_t *gid_
 hipError_t launch_m2l_scale_m(Cells c, int32_t ncells, const double *M, void *Mq, bool fp32,
                               hipStream_t s);
 // items of the (tile, group)-sorted grouped pairs: every pair's position in
-// its segment and the item starts (runs of 16), their count read back
+// its segment and the item starts (runs of 16) per block of 1024 pairs,
+// their count read back; bcount/bstart: one word per block
 hipError_t launch_m2l_count_items(const uint32_t *keys, uint32_t n, uint32_t *ipos,
-                                  uint32_t *iflag, uint32_t *iexcl, void *scan_tmp,
+                                  uint32_t *bcount, uint32_t *bstart, void *scan_tmp,
                                   uint32_t *h_pinned, uint32_t *nitems_out, hipStream_t s);
 hipError_t launch_m2l_write_items(const uint32_t *keys, const uint32_t *vals, uint32_t n,
-                                  int gbits, const uint32_t *ipos, const uint32_t *iflag,
-                                  const uint32_t *iexcl, const int8_t *gdl, int4 *items,
-                                  uint2 *cols, hipStream_t s);
+                                  int gbits, const uint32_t *ipos, const uint32_t *bstart,
+                                  const int8_t *gdl, int4 *items, uint2 *cols, hipStream_t s);
 hipError_t launch_m2l_tile_offsets(const int4 *items, uint32_t nitems, int32_t ntiles,
                                    int32_t *off, hipStream_t s);
 // work units of the tile kernel: runs of at most `chunk` items of one tile
