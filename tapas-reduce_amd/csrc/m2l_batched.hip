@@ -66,7 +66,7 @@ constexpr int NCODE = NBLK * 4 * 64;
 constexpr int SRC_BITS = 25;  // source cell bits of a grouped pair's value
 constexpr int TILE = 128;
 constexpr int TILE_WAVES = 4;
-constexpr int WAVE_SYNC = 16;  // items between block barriers in the tile loop
+constexpr int WAVE_SYNC = 64;  // items between block barriers in the tile loop
 
 // ---------------------------------------------------------------------------
 // group table

@@ -33,6 +33,7 @@ namespace tfmm {
 namespace {
 
 constexpr int DTT_THREADS = 1024;
+constexpr int DTT_PPT = 4;  // frontier pairs per thread
 constexpr int DTT_WAVES = DTT_THREADS / 64;
 
 // per-cell traversal records: (centre, R) with R = (width / 2 * 1.00001) / theta
@@ -48,6 +49,9 @@ __global__ void k_dtt_cells(Cells c, int32_t ncells, double theta, double4 *geo,
   topo[i] = make_int2(c.child[i], c.nb[i]);
 }
 
+// Kind of a frontier pair after the MAC: what the step emits for it.
+enum : int { DK_NONE = 0, DK_M2L, DK_FAR_EMPTY, DK_P2P, DK_SPLIT_I, DK_SPLIT_J };
+
 __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const uint32_t *nin_dev,
                                                   uint32_t nin_max, int2 *out, uint32_t out_cap,
                                                   uint32_t *nout, int2 *m2l, uint32_t m2l_cap,
@@ -58,78 +62,100 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
                                                   uint32_t *n_outside) {
 #pragma clang fp contract(off)
   const uint32_t nin = nin_dev ? *nin_dev : nin_max;
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = t < nin && t < nin_max;
-  int32_t i = 0, j = 0;
-  int2 ti = make_int2(0, 0), tj = make_int2(0, 0);
-  bool far = false, p2p_pair = false, split_i = false, split_j = false, nonempty = false;
-  double4 gi = make_double4(0, 0, 0, 0), gj = gi;
-  if (valid) {
-    const int2 pr = in[t];
-    i = pr.x;
-    j = pr.y;
-    gi = geo[i];
-    gj = geo[j];
-    ti = topo[i];
-    tj = topo[j];
-    const double d0 = gi.x - gj.x;
-    const double d1 = gi.y - gj.y;
-    const double d2 = gi.z - gj.z;
-    double R2 = 0;
-    R2 += d0 * d0;
-    R2 += d1 * d1;
-    R2 += d2 * d2;
-    const double Ri = gi.w, Rj = gj.w;
-    const bool leaf_i = ti.x < 0, leaf_j = tj.x < 0;
-    nonempty = ti.y > 0 && tj.y > 0;
-    if (R2 > (Ri + Rj) * (Ri + Rj)) {
-      far = true;
-    } else if (leaf_i && leaf_j) {
-      p2p_pair = nonempty;
-    } else if (leaf_j) {
-      split_i = true;
-    } else if (leaf_i) {
-      split_j = true;
-    } else if (Ri >= Rj) {
-      split_i = true;
-    } else {
-      split_j = true;
-    }
-  }
-  // Appends of the block to the four lists (0 next frontier, 8 entries per
-  // pair; 1 executed M2L; 2 P2P; 3 all M2L, counted only) and the body-pair
-  // sums: one global atomic per list per block, two block barriers.
-  const bool split = split_i || split_j;
-  const bool pred[4] = {split, far && nonempty, p2p_pair, far};
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  uint64_t m[4];
-#pragma unroll
-  for (int q = 0; q < 4; q++) m[q] = __ballot(pred[q]);
+  // Each thread takes DTT_PPT pairs (rows r: pair blockIdx.x * DTT_THREADS *
+  // DTT_PPT + r * DTT_THREADS + threadIdx.x).  Pass 1 classifies them and
+  // counts the block's appends to the four lists (0 next frontier, 8 entries
+  // per pair; 1 executed M2L; 2 P2P; 3 all M2L, counted only) per row and
+  // wave; one global atomic per list per block reserves the block's ranges
+  // (same-address atomics are what bound this kernel); pass 2 writes.
+  __shared__ uint32_t wc[DTT_PPT][DTT_WAVES][4];
+  __shared__ unsigned long long ws[DTT_WAVES][2];
+  int32_t pi[DTT_PPT], pj[DTT_PPT], kind[DTT_PPT], fch[DTT_PPT], grp[DTT_PPT];
   unsigned long long v = 0, self = 0;
-  if (p2p_pair) {
-    v = (unsigned long long)ti.y * (unsigned long long)tj.y;
-    if (i == j) self = (unsigned long long)ti.y;
+#pragma unroll
+  for (int r = 0; r < DTT_PPT; r++) {
+    const uint32_t t = (blockIdx.x * DTT_PPT + r) * DTT_THREADS + threadIdx.x;
+    const bool valid = t < nin && t < nin_max;
+    int32_t i = 0, j = 0, k = DK_NONE, f = 0, g = -1;
+    if (valid) {
+      const int2 pr = in[t];
+      i = pr.x;
+      j = pr.y;
+      const double4 gi = geo[i], gj = geo[j];
+      const int2 ti = topo[i], tj = topo[j];
+      const double d0 = gi.x - gj.x;
+      const double d1 = gi.y - gj.y;
+      const double d2 = gi.z - gj.z;
+      double R2 = 0;
+      R2 += d0 * d0;
+      R2 += d1 * d1;
+      R2 += d2 * d2;
+      const double Ri = gi.w, Rj = gj.w;
+      const bool leaf_i = ti.x < 0, leaf_j = tj.x < 0;
+      const bool nonempty = ti.y > 0 && tj.y > 0;
+      if (R2 > (Ri + Rj) * (Ri + Rj)) {
+        k = nonempty ? DK_M2L : DK_FAR_EMPTY;
+        if (nonempty) {
+          // the pair's M2L offset group (same arithmetic on the same centres
+          // as a separate pass over the list would do)
+          const double ci[3] = {gi.x, gi.y, gi.z}, cj[3] = {gj.x, gj.y, gj.z};
+          g = m2lg::pair_slot(depth[i], depth[j], width[i], width[j], ci, cj);
+        }
+      } else if (leaf_i && leaf_j) {
+        if (nonempty) {
+          k = DK_P2P;
+          v += (unsigned long long)ti.y * (unsigned long long)tj.y;
+          if (i == j) self += (unsigned long long)ti.y;
+        }
+      } else if (leaf_j || (!leaf_i && Ri >= Rj)) {
+        k = DK_SPLIT_I;
+        f = ti.x;
+      } else {
+        k = DK_SPLIT_J;
+        f = tj.x;
+      }
+    }
+    pi[r] = i;
+    pj[r] = j;
+    kind[r] = k;
+    fch[r] = f;
+    grp[r] = g;
+    const uint64_t m0 = __ballot(k == DK_SPLIT_I || k == DK_SPLIT_J);
+    const uint64_t m1 = __ballot(k == DK_M2L);
+    const uint64_t m2 = __ballot(k == DK_P2P);
+    const uint64_t m3 = __ballot(k == DK_M2L || k == DK_FAR_EMPTY);
+    if (lane == 0) {
+      wc[r][wid][0] = (uint32_t)__popcll(m0) * 8u;
+      wc[r][wid][1] = (uint32_t)__popcll(m1);
+      wc[r][wid][2] = (uint32_t)__popcll(m2);
+      wc[r][wid][3] = (uint32_t)__popcll(m3);
+    }
   }
   for (int o = 32; o > 0; o >>= 1) {
     v += __shfl_xor(v, o, 64);
     self += __shfl_xor(self, o, 64);
   }
-  __shared__ uint32_t wc[DTT_WAVES][4];
-  __shared__ unsigned long long ws[DTT_WAVES][2];
-  __shared__ uint32_t base[4];
   if (lane == 0) {
-#pragma unroll
-    for (int q = 0; q < 4; q++) wc[wid][q] = (uint32_t)__popcll(m[q]) * (q == 0 ? 8u : 1u);
     ws[wid][0] = v;
     ws[wid][1] = self;
   }
   __syncthreads();
+  // one thread per list: the block's total, its global range, and the
+  // (row, wave) counts turned into absolute starts
   if (threadIdx.x < 4) {
     const int q = threadIdx.x;
     uint32_t tot = 0;
-    for (int w = 0; w < DTT_WAVES; w++) tot += wc[w][q];
-    base[q] = tot ? atomicAdd(q == 0 ? nout : &cnt[q], tot) : 0u;
+    for (int r = 0; r < DTT_PPT; r++)
+      for (int w = 0; w < DTT_WAVES; w++) tot += wc[r][w][q];
+    uint32_t run = tot ? atomicAdd(q == 0 ? nout : &cnt[q], tot) : 0u;
+    for (int r = 0; r < DTT_PPT; r++)
+      for (int w = 0; w < DTT_WAVES; w++) {
+        const uint32_t c = wc[r][w][q];
+        wc[r][w][q] = run;
+        run += c;
+      }
   } else if (threadIdx.x == 64) {
     unsigned long long tv = 0, ts = 0;
     for (int w = 0; w < DTT_WAVES; w++) {
@@ -141,38 +167,40 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
     if (ts) atomicAdd(&pc[1], ts);
   }
   __syncthreads();
-  uint32_t slot[3];
+  uint32_t outside = 0;
 #pragma unroll
-  for (int q = 0; q < 3; q++) {
-    uint32_t pre = base[q];
-    for (int w = 0; w < wid; w++) pre += wc[w][q];
-    slot[q] = pre + (uint32_t)__popcll(m[q] & lt) * (q == 0 ? 8u : 1u);
-  }
-  bool outside = false;
-  if (far && nonempty && slot[1] < m2l_cap) {
-    m2l[slot[1]] = make_int2(i, j);
-    // the pair's M2L offset group (same arithmetic on the same centres as
-    // a separate pass over the list would do)
-    const double ci[3] = {gi.x, gi.y, gi.z}, cj[3] = {gj.x, gj.y, gj.z};
-    const int32_t g = m2lg::pair_slot(depth[i], depth[j], width[i], width[j], ci, cj);
-    m2l_slot[slot[1]] = g;
-    if (g >= 0) used[g] = 1u;
-    outside = g < 0;
-  }
-  {
-    const uint64_t mo = __ballot(outside);
-    if (lane == 0 && mo) atomicAdd(n_outside, (uint32_t)__popcll(mo));
-  }
-  if (p2p_pair && slot[2] < p2p_cap) p2p[slot[2]] = make_int2(i, j);
-  if (split && slot[0] + 8 <= out_cap) {
-    if (split_i) {
-      const int32_t f = ti.x;
-      for (int k = 0; k < 8; k++) out[slot[0] + k] = make_int2(f + k, j);
-    } else {
-      const int32_t f = tj.x;
-      for (int k = 0; k < 8; k++) out[slot[0] + k] = make_int2(i, f + k);
+  for (int r = 0; r < DTT_PPT; r++) {
+    const int k = kind[r];
+    const int32_t i = pi[r], j = pj[r];
+    const bool split = k == DK_SPLIT_I || k == DK_SPLIT_J;
+    const uint64_t m0 = __ballot(split);
+    const uint64_t m1 = __ballot(k == DK_M2L);
+    const uint64_t m2 = __ballot(k == DK_P2P);
+    if (k == DK_M2L) {
+      const uint32_t sl = wc[r][wid][1] + (uint32_t)__popcll(m1 & lt);
+      if (sl < m2l_cap) {
+        m2l[sl] = make_int2(i, j);
+        m2l_slot[sl] = grp[r];
+        if (grp[r] >= 0) used[grp[r]] = 1u;
+        else outside++;
+      }
+    } else if (k == DK_P2P) {
+      const uint32_t sl = wc[r][wid][2] + (uint32_t)__popcll(m2 & lt);
+      if (sl < p2p_cap) p2p[sl] = make_int2(i, j);
+    } else if (split) {
+      const uint32_t sl = wc[r][wid][0] + (uint32_t)__popcll(m0 & lt) * 8u;
+      if (sl + 8 <= out_cap) {
+        const int32_t f = fch[r];
+        if (k == DK_SPLIT_I) {
+          for (int c = 0; c < 8; c++) out[sl + c] = make_int2(f + c, j);
+        } else {
+          for (int c = 0; c < 8; c++) out[sl + c] = make_int2(i, f + c);
+        }
+      }
     }
   }
+  for (int o = 32; o > 0; o >>= 1) outside += __shfl_xor(outside, o, 64);
+  if (lane == 0 && outside) atomicAdd(n_outside, outside);
 }
 
 __global__ void k_unpack_pairs(const uint64_t *keys, uint32_t n, int sbits, int32_t *src) {
@@ -285,7 +313,8 @@ hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin
                            int32_t *m2l_slot, uint32_t *used, uint32_t *n_outside,
                            hipStream_t s) {
   if (nin_max == 0) return hipSuccess;
-  k_dtt_step<<<(nin_max + DTT_THREADS - 1) / DTT_THREADS, DTT_THREADS, 0, s>>>(
+  const uint32_t per = DTT_THREADS * DTT_PPT;
+  k_dtt_step<<<(nin_max + per - 1) / per, DTT_THREADS, 0, s>>>(
       in, nin_dev, nin_max, out, out_cap, nout, m2l, m2l_cap, p2p, p2p_cap, cnt, geo, topo, depth,
       width, m2l_slot, used, n_outside);
   return hipGetLastError();
