@@ -265,14 +265,26 @@ __device__ inline void rank_sources(P v, int32_t n, int lane, int32_t *out) {
   }
 }
 
-// one wave per target: its sources in ascending order
+// one wave per target: its sources in ascending order (up to 64 sorted in
+// registers, longer lists ranked)
 __global__ __launch_bounds__(64) void k_seg_sort(const int32_t *off, const uint32_t *in,
                                                  int32_t *out) {
   __shared__ uint32_t v[SEG_CAP];
   const int lane = threadIdx.x;
   const int32_t b = off[blockIdx.x], n = off[blockIdx.x + 1] - b;
   if (n <= 0) return;
-  if (n <= SEG_CAP) {
+  if (n <= 64) {
+    // one source per lane: bitonic sort across the wave, ascending (lanes
+    // past n hold a value above every cell index)
+    uint32_t x = lane < n ? in[b + lane] : 0xffffffffu;
+    for (int k = 2; k <= 64; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const uint32_t y = __shfl_xor(x, j, 64);
+        const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+        x = keep_min ? min(x, y) : max(x, y);
+      }
+    if (lane < n) out[b + lane] = (int32_t)x;
+  } else if (n <= SEG_CAP) {
     for (int32_t i = lane; i < n; i += 64) v[i] = in[b + i];
     __syncthreads();
     rank_sources(static_cast<const uint32_t *>(v), n, lane, out + b);
