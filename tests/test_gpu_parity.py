@@ -264,3 +264,58 @@ def test_gpu_m2l_groups_cover_pairs(ctx):
     print("plummer: %d of %d M2L pairs outside the offset groups" % (st["n_m2l_fallback"],
                                                                      st["n_m2l_exec"]))
     assert st["n_m2l_fallback"] < 0.05 * st["n_m2l_exec"]
+
+
+@pytest.mark.parametrize("ncrit", [24, 48, 100, 256])
+def test_gpu_fp64_leaf_sizes_match_oracle(ctx, ncrit):
+    """FP64 fields equal the oracle's within 1e-10 on a Plummer sphere whose
+    leaves span every lane layout of the P2P kernel (several lanes per target,
+    one lane per target, several passes per leaf)."""
+    x, y, z, q = datasets.plummer(30000, seed=5, charges="uniform")
+    f, st = evaluate(ctx, x, y, z, q, ncrit, 0.4)
+    o = oracle.Run(x, y, z, q, ncrit, 0.4)
+    c = o.counts()
+    assert st["n_m2l"] == c["m2l"] and st["n_p2p_leafpairs"] == c["p2p_leafpairs"]
+    ep, ea = oracle.rel_l2(f, o.fields())
+    print("ncrit %d: rel-L2 pot %.2e acc %.2e" % (ncrit, ep, ea))
+    assert ep < TOL and ea < TOL
+
+
+@pytest.mark.parametrize("ncrit", [24, 48, 100, 256])
+def test_gpu_mixed_leaf_sizes_against_fp64(ctx, ncrit):
+    """Mixed precision within 1e-5 of the FP64 FMM for the same lane layouts."""
+    x, y, z, q = datasets.plummer(60000, seed=6)
+    f64, st64 = evaluate(ctx, x, y, z, q, ncrit, 0.4, tfmm.FP64)
+    f32, st32 = evaluate(ctx, x, y, z, q, ncrit, 0.4, tfmm.MIXED)
+    assert st32["n_p2p_calls"] == st64["n_p2p_calls"]
+    emix = oracle.rel_l2(f32, f64)
+    print("ncrit %d: mixed vs fp64 %.2e %.2e" % (ncrit, *emix))
+    assert max(emix) < 1e-5
+
+
+def test_gpu_mixed_coincident_across_leaf_boundary(ctx):
+    """Two bodies 2e-12 apart on either side of the plane x = 0.5 (a cell
+    boundary at every depth of the [0, 1] cube) have equal leaf-relative FP32
+    coordinates, so their FP32 distance is exactly zero.  The mixed P2P must
+    drop that pair (as it drops a body's pair with itself) and stay finite; all
+    other bodies keep the mixed-precision accuracy against the FP64 FMM."""
+    rng = np.random.default_rng(8)
+    n = 20000
+    x, y, z = rng.random((3, n))
+    q = rng.uniform(-1, 1, n)
+    x[0], y[0], z[0] = 0.5, 0.5, 0.5  # body 0 (its x/y enter the bbox specially) inside
+    x[1], y[1], z[1] = 0.0, 0.0, 0.0  # region [0, 1]^3 exactly
+    x[2], y[2], z[2] = 1.0, 1.0, 1.0
+    a, b = 3, 4
+    x[a], x[b] = 0.5 - 1e-12, 0.5 + 1e-12
+    y[a] = y[b] = 0.3
+    z[a] = z[b] = 0.7
+    f64, _ = evaluate(ctx, x, y, z, q, 16, 0.4, tfmm.FP64)
+    f32, _ = evaluate(ctx, x, y, z, q, 16, 0.4, tfmm.MIXED)
+    assert np.isfinite(f64).all()
+    assert np.isfinite(f32).all(), np.argwhere(~np.isfinite(f32))
+    keep = np.ones(n, bool)
+    keep[[a, b]] = False
+    emix = oracle.rel_l2(f32[keep], f64[keep])
+    print("mixed vs fp64 without the close pair: %.2e %.2e" % emix)
+    assert max(emix) < 1e-5
