@@ -222,6 +222,20 @@ int tfmm_autotune(tfmm_ctx *ctx, const tfmm_params *params, int reps, double *ch
  * a batch each target sums over the bodies of its own problem. */
 int tfmm_direct(tfmm_ctx *ctx, const int64_t *targets, int64_t nt, double *out);
 
+/* tfmm_direct without any device allocation.  No counterpart in the
+ * reference: this entry point is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The target indices and the fields are staged in a
+ * static device array of the library sized for TFMM_DIRECT_NOALLOC_MAX
+ * targets (40 bytes each, 2.5 MiB; every GPU the library runs on holds its
+ * copy until the process ends); more targets return TFMM_EINVAL.  Same
+ * results as tfmm_direct, bit for bit.  All calls on one GPU share that
+ * GPU's array, which keeps the last call's contents: calls that overlap on
+ * one GPU (from several contexts or host threads) race.  Targets of one
+ * problem that are adjacent in `targets` are summed by one launch. */
+#define TFMM_DIRECT_NOALLOC_MAX 65536
+int tfmm_direct_noalloc(tfmm_ctx *ctx, const int64_t *targets, int64_t nt, double *out);
+
 /* ---------------------------------------------------------------------------
  * Learned parameter picker.  No counterpart in the reference (its ncrit and
  * theta are fixed command-line values, EX/include/args.h:101-158): an FP32

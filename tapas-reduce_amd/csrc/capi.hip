@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -117,6 +118,9 @@ struct tfmm_ctx {
 
   // fields
   DevBuf acc, trg, out;
+
+  // this device's copy of the static scratch of tfmm_direct_noalloc
+  void *noalloc = nullptr;
 
   // host staging (pinned)
   uint32_t *h_u32 = nullptr;  // pinned staging, H_U32_WORDS words
@@ -940,6 +944,36 @@ int need_stage(tfmm_ctx *c, int st, const char *what) {
 
 extern "C" {
 
+namespace {
+
+// Scratch of tfmm_direct_noalloc: the target indices (8 bytes each), then the
+// fields (32 bytes each) of up to TFMM_DIRECT_NOALLOC_MAX targets.  Host code
+// uses it only through the per-device address hipGetSymbolAddress gives.
+constexpr size_t NOALLOC_IDX_BYTES = (size_t)TFMM_DIRECT_NOALLOC_MAX * 8;
+constexpr size_t NOALLOC_BYTES = (size_t)TFMM_DIRECT_NOALLOC_MAX * 40;
+__device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[MAX_DEVICES] = {};
+
+// the scratch's address on the current device (looked up once per device)
+hipError_t scratch_address(int device, void **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+  if (!g_scratch[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch[device] = p;
+  }
+  *out = g_scratch[device];
+  return hipSuccess;
+}
+
+}  // namespace
+
 int tfmm_create(int hip_device, tfmm_ctx **out) {
   if (!out) return TFMM_EINVAL;
   *out = nullptr;
@@ -973,6 +1007,10 @@ int tfmm_create(int hip_device, tfmm_ctx **out) {
       hipHostMalloc((void **)&c->h_u64, 64, 0) != hipSuccess) {
     tfmm_destroy(c);
     return TFMM_ENOMEM;
+  }
+  if (scratch_address(hip_device, &c->noalloc) != hipSuccess) {
+    tfmm_destroy(c);
+    return TFMM_EHIP;
   }
   *out = c;
   return TFMM_OK;
@@ -1530,7 +1568,7 @@ int tfmm_direct(tfmm_ctx *c, const int64_t *targets, int64_t nt, double *out) {
     const int64_t b = c->prob_off[p], m = c->prob_off[p + 1] - b;
     if (cnt[p + 1] > cnt[p])
       e = launch_direct_fp64(c->bx + b, c->by + b, c->bz + b, c->bq + b, m,
-                             dp<int64_t>(t) + cnt[p], cnt[p + 1] - cnt[p],
+                             dp<int64_t>(t) + cnt[p], 0, cnt[p + 1] - cnt[p],
                              dp<double>(o) + 4 * cnt[p], c->stream);
   }
   if (e == hipSuccess)
@@ -1540,6 +1578,41 @@ int tfmm_direct(tfmm_ctx *c, const int64_t *targets, int64_t nt, double *out) {
   dev_release(o);
   if (e != hipSuccess) return hipfail(c, e, "tfmm_direct");
   for (int64_t a = 0; a < nt; a++) memcpy(out + 4 * where[a], &grouped[4 * a], 32);
+  return TFMM_OK;
+}
+
+int tfmm_direct_noalloc(tfmm_ctx *c, const int64_t *targets, int64_t nt, double *out) {
+  if (!c) return TFMM_EINVAL;
+  if (c->n < 1) return fail(c, TFMM_ESTATE, "no bodies set");
+  if (nt < 0 || (nt > 0 && (!targets || !out))) return fail(c, TFMM_EINVAL, "bad targets");
+  if (nt > TFMM_DIRECT_NOALLOC_MAX)
+    return fail(c, TFMM_EINVAL, "tfmm_direct_noalloc: more targets than TFMM_DIRECT_NOALLOC_MAX");
+  if (nt == 0) return TFMM_OK;
+  for (int64_t i = 0; i < nt; i++)
+    if (targets[i] < 0 || targets[i] >= c->n) return fail(c, TFMM_EINVAL, "target index out of range");
+  if (!c->noalloc) return fail(c, TFMM_ESTATE, "tfmm_direct_noalloc: no scratch address");
+  TRY(hipSetDevice(c->device));
+  int64_t *d_idx = static_cast<int64_t *>(c->noalloc);
+  double *d_out = reinterpret_cast<double *>(static_cast<unsigned char *>(c->noalloc) +
+                                             NOALLOC_IDX_BYTES);
+  hipError_t e = hipMemcpyAsync(d_idx, targets, (size_t)nt * 8, hipMemcpyHostToDevice, c->stream);
+  // one launch per run of adjacent targets of one problem; the kernel turns
+  // the caller's (forest) indices into the problem's own by its base
+  const int np = c->nprob;
+  for (int64_t i = 0; i < nt && e == hipSuccess;) {
+    const int p = (int)(std::upper_bound(c->prob_off.begin(), c->prob_off.begin() + np,
+                                         targets[i]) - c->prob_off.begin()) - 1;
+    const int64_t b = c->prob_off[p], end = c->prob_off[p + 1];
+    int64_t j = i + 1;
+    while (j < nt && targets[j] >= b && targets[j] < end) j++;
+    e = launch_direct_fp64(c->bx + b, c->by + b, c->bz + b, c->bq + b, end - b, d_idx + i, b,
+                           j - i, d_out + 4 * i, c->stream);
+    i = j;
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, d_out, (size_t)nt * 32, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hipfail(c, e, "tfmm_direct_noalloc");
   return TFMM_OK;
 }
 

@@ -14,14 +14,14 @@ constexpr int DT = 256;
 
 __global__ __launch_bounds__(DT) void k_direct(const double *x, const double *y,
                                                const double *z, const double *q, int64_t n,
-                                               const int64_t *targets, int64_t nt,
-                                               double *out) {
+                                               const int64_t *targets, int64_t tbase,
+                                               int64_t nt, double *out) {
   __shared__ double sx[DT], sy[DT], sz[DT], sq[DT];
   const int64_t t = (int64_t)blockIdx.x * DT + threadIdx.x;
   const bool valid = t < nt;
   double xi = 0, yi = 0, zi = 0, qi = 1;
   if (valid) {
-    const int64_t i = targets[t];
+    const int64_t i = targets[t] - tbase;
     xi = x[i];
     yi = y[i];
     zi = z[i];
@@ -64,10 +64,11 @@ __global__ __launch_bounds__(DT) void k_direct(const double *x, const double *y,
 }  // namespace
 
 hipError_t launch_direct_fp64(const double *x, const double *y, const double *z,
-                              const double *q, int64_t n, const int64_t *targets, int64_t nt,
-                              double *out, hipStream_t s) {
+                              const double *q, int64_t n, const int64_t *targets, int64_t tbase,
+                              int64_t nt, double *out, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  k_direct<<<(unsigned)((nt + DT - 1) / DT), DT, 0, s>>>(x, y, z, q, n, targets, nt, out);
+  k_direct<<<(unsigned)((nt + DT - 1) / DT), DT, 0, s>>>(x, y, z, q, n, targets, tbase, nt,
+                                                         out);
   return hipGetLastError();
 }
 

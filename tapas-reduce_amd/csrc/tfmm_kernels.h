@@ -118,9 +118,10 @@ hipError_t launch_p2p_fp64(Cells c, int32_t ncells, Csr list, Bodies b, double *
                            hipStream_t s);
 
 // direct.hip
+// target t sums over the n bodies at x, y, z, q: index targets[t] - tbase
 hipError_t launch_direct_fp64(const double *x, const double *y, const double *z,
-                              const double *q, int64_t n, const int64_t *targets, int64_t nt,
-                              double *out, hipStream_t s);
+                              const double *q, int64_t n, const int64_t *targets, int64_t tbase,
+                              int64_t nt, double *out, hipStream_t s);
 
 }  // namespace tfmm
 

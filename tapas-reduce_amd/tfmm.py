@@ -22,6 +22,7 @@ TUNE_M2L_XCD_RUN = 2
 TUNE_P2P_SCHEDULE = 3
 TUNE_GRAPH = 4
 TUNE_P2P_WAVES = 5
+DIRECT_NOALLOC_MAX = 65536
 P = 10
 NTERM = 55
 
@@ -75,6 +76,7 @@ EXPORTED = [
     "tfmm_upward",
     "tfmm_traverse", "tfmm_downward", "tfmm_evaluate", "tfmm_get_fields", "tfmm_get_stats",
     "tfmm_get_sorted", "tfmm_get_cells", "tfmm_get_expansions", "tfmm_get_lists", "tfmm_direct",
+    "tfmm_direct_noalloc",
     "tfmm_picker_nparam", "tfmm_picker_create", "tfmm_picker_destroy", "tfmm_picker_last_error",
     "tfmm_picker_set_weights", "tfmm_picker_get_weights", "tfmm_picker_set_theta_max",
     "tfmm_picker_train",
@@ -123,6 +125,7 @@ def lib():
         L.tfmm_get_expansions.argtypes = [vp, vp, vp]
         L.tfmm_get_lists.argtypes = [vp, vp, vp, vp, vp]
         L.tfmm_direct.argtypes = [vp, vp, ctypes.c_int64, vp]
+        L.tfmm_direct_noalloc.argtypes = [vp, vp, ctypes.c_int64, vp]
         L.tfmm_picker_nparam.restype = ctypes.c_int64
         L.tfmm_picker_nparam.argtypes = []
         L.tfmm_picker_create.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(vp)]
@@ -347,6 +350,15 @@ class Context:
         t = np.ascontiguousarray(targets, np.int64)
         out = np.zeros((len(t), 4), np.float64)
         self._check(lib().tfmm_direct(self._h, _ptr(t), len(t), _ptr(out)), "tfmm_direct")
+        return out
+
+    def direct_noalloc(self, targets):
+        """tfmm_direct staged in the library's static device array (no
+        device allocation; at most DIRECT_NOALLOC_MAX targets)."""
+        t = np.ascontiguousarray(targets, np.int64)
+        out = np.zeros((len(t), 4), np.float64)
+        self._check(lib().tfmm_direct_noalloc(self._h, _ptr(t), len(t), _ptr(out)),
+                    "tfmm_direct_noalloc")
         return out
 
 
