@@ -149,6 +149,17 @@ int tfmm_build_tree(tfmm_ctx *ctx, const tfmm_params *params);
 int tfmm_set_params(tfmm_ctx *ctx, const tfmm_params *params);
 int tfmm_upward(tfmm_ctx *ctx);
 int tfmm_traverse(tfmm_ctx *ctx);
+/* tfmm_traverse with interaction lists recorded by the caller instead of the
+ * GPU traversal: the reference's own FMM_DTT functor run on host cells, its
+ * M2L(Ci, Cj, ...) calls and Map(P2P(), Product(Ci.bodies(), Cj.bodies()), ...)
+ * products recorded, as the reference's GPUMapper records P2P products
+ * (TP/hot/mapper.h:452-690), then executed by the library's M2L and P2P
+ * kernels.  m2l: n_m2l (target, source) cell pairs, 2 int32 each, in the
+ * breadth-first cell order of tfmm_get_cells, empty cells included (counted
+ * like the reference's numM2L, not executed); p2p: n_p2p leaf pairs (pairs with
+ * an empty leaf are skipped).  One problem only (not a batch). */
+int tfmm_traverse_lists(tfmm_ctx *ctx, const int32_t *m2l, int64_t n_m2l, const int32_t *p2p,
+                        int64_t n_p2p);
 int tfmm_downward(tfmm_ctx *ctx);
 /* build_tree + upward + traverse + downward: the traversal and the list
  * building first, then the numeric kernels (as one hipGraph, see

@@ -647,14 +647,14 @@ int build_m2l_batches(tfmm_ctx *c) {
 // resident P2P workgroups (one wave each) for the knob's waves per SIMD
 int p2p_resident(tfmm_ctx *c) { return c->p2p_waves > 0 ? c->p2p_waves * 4 * c->n_cu : 0; }
 
-int do_traverse(tfmm_ctx *c) {
+// The M2L and P2P pairs of the traversal (c->m2l_pairs with their group
+// slots, c->p2p_pairs, the counters) into sorted lists, M2L batches and the
+// M2L and P2P kernels.
+int run_lists(tfmm_ctx *c) {
   hipStream_t s = c->stream;
   const Cells k = cells_of(c);
   const bool mixed = c->prm.precision == TFMM_MIXED;
-  TRY(hipEventRecord(c->ev[EV_DTT0], s));
-  int rc = do_dtt(c);
-  if (rc) return rc;
-  rc = sort_list(c, c->p2p_pairs, c->n_p2p, c->p2p_off, c->p2p_src);
+  int rc = sort_list(c, c->p2p_pairs, c->n_p2p, c->p2p_off, c->p2p_src);
   if (rc) return rc;
   if (mixed) {
     TRY(dev_reserve(c->relpos, (size_t)c->n * 16));
@@ -725,6 +725,84 @@ int do_traverse(tfmm_ctx *c) {
   TRY(hipStreamWaitEvent(s, c->ev[EV_P2P1], 0));
   TRY(hipMemcpyAsync(c->h_u64, static_cast<char *>(c->cnt.ptr) + 32, 16, hipMemcpyDeviceToHost, s));
   return TFMM_OK;
+}
+
+int do_traverse(tfmm_ctx *c) {
+  TRY(hipEventRecord(c->ev[EV_DTT0], c->stream));
+  int rc = do_dtt(c);
+  if (rc) return rc;
+  return run_lists(c);
+}
+
+// Interaction lists recorded outside the library (the reference's FMM_DTT
+// functor run on host cells, TP/hot/mapper.h:452-690 records its P2P products
+// the same way) in place of the GPU traversal: M2L pairs (target, source) as
+// the functor called M2L, empty cells included and counted like the
+// reference's numM2L; P2P pairs of leaves.  Pairs with an empty cell are
+// counted but not executed, as in the traversal.
+int traverse_given(tfmm_ctx *c, const int32_t *m2l, int64_t nm, const int32_t *p2p, int64_t np) {
+  hipStream_t s = c->stream;
+  const Cells k = cells_of(c);
+  if (nm < 0 || np < 0 || (nm && !m2l) || (np && !p2p))
+    return fail(c, TFMM_EINVAL, "tfmm_traverse_lists: bad list");
+  if (nm > (int64_t)(UINT32_MAX / 2) || np > (int64_t)(UINT32_MAX / 2))
+    return fail(c, TFMM_EINVAL, "tfmm_traverse_lists: list too long");
+  const int64_t nc = c->ncells;
+  std::vector<int32_t> nb(nc), child(nc);
+  TRY(hipMemcpyAsync(nb.data(), k.nb, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(child.data(), k.child, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  std::vector<int2> me, pe;
+  me.reserve((size_t)nm);
+  pe.reserve((size_t)np);
+  for (int64_t e = 0; e < nm; e++) {
+    const int32_t i = m2l[2 * e], j = m2l[2 * e + 1];
+    if (i < 0 || i >= nc || j < 0 || j >= nc)
+      return fail(c, TFMM_EINVAL, "tfmm_traverse_lists: M2L cell index out of range");
+    if (nb[i] > 0 && nb[j] > 0) me.push_back(make_int2(i, j));
+  }
+  unsigned long long pairs = 0, self = 0;
+  for (int64_t e = 0; e < np; e++) {
+    const int32_t i = p2p[2 * e], j = p2p[2 * e + 1];
+    if (i < 0 || i >= nc || j < 0 || j >= nc)
+      return fail(c, TFMM_EINVAL, "tfmm_traverse_lists: P2P cell index out of range");
+    if (child[i] >= 0 || child[j] >= 0)
+      return fail(c, TFMM_EINVAL, "tfmm_traverse_lists: P2P pair of a cell that is not a leaf");
+    if (nb[i] == 0 || nb[j] == 0) continue;
+    pe.push_back(make_int2(i, j));
+    pairs += (unsigned long long)nb[i] * (unsigned long long)nb[j];
+    if (i == j) self += (unsigned long long)nb[i];
+  }
+  TRY(hipEventRecord(c->ev[EV_DTT0], s));
+  const int max_depth = (int)c->lvl.size() - 2;
+  const int32_t nslots = m2l_group_slots(max_depth);
+  const size_t nme = std::max<size_t>(me.size(), 1), npe = std::max<size_t>(pe.size(), 1);
+  TRY(dev_reserve(c->cnt, 16 * 4));
+  TRY(dev_reserve(c->g_used, (size_t)(nslots + 1) * 4));
+  TRY(dev_reserve(c->m2l_pairs, nme * 8));
+  TRY(dev_reserve(c->slot_of, nme * 4));
+  TRY(dev_reserve(c->p2p_pairs, npe * 8));
+  TRY(dev_reserve(c->dtt_geo, (size_t)nc * 32));
+  TRY(dev_reserve(c->dtt_topo, (size_t)nc * 8));
+  TRY(hipMemsetAsync(c->g_used.ptr, 0, (size_t)(nslots + 1) * 4, s));
+  c->h_u64[0] = pairs;
+  c->h_u64[1] = self;
+  TRY(hipMemcpyAsync(static_cast<char *>(c->cnt.ptr) + 32, c->h_u64, 16, hipMemcpyHostToDevice, s));
+  if (!me.empty())
+    TRY(hipMemcpyAsync(c->m2l_pairs.ptr, me.data(), me.size() * 8, hipMemcpyHostToDevice, s));
+  if (!pe.empty())
+    TRY(hipMemcpyAsync(c->p2p_pairs.ptr, pe.data(), pe.size() * 8, hipMemcpyHostToDevice, s));
+  TRY(launch_dtt_cells(k, (int32_t)nc, c->prm.theta, dp<double4>(c->dtt_geo),
+                       dp<int2>(c->dtt_topo), s));
+  TRY(launch_pair_slots(dp<int2>(c->m2l_pairs), (uint32_t)me.size(), dp<double4>(c->dtt_geo),
+                        k.depth, k.width, dp<int32_t>(c->slot_of), dp<uint32_t>(c->g_used),
+                        dp<uint32_t>(c->g_used) + nslots, s));
+  // the uploads read host vectors that end with this call
+  TRY(hipStreamSynchronize(s));
+  c->n_m2l_exec = (uint32_t)me.size();
+  c->n_m2l_all = (uint64_t)nm;
+  c->n_p2p = (uint32_t)pe.size();
+  return run_lists(c);
 }
 
 int do_downward(tfmm_ctx *c) {
@@ -1358,6 +1436,18 @@ int tfmm_traverse(tfmm_ctx *c) {
   int rc = need_stage(c, 2, "tfmm_traverse");
   if (rc) return rc;
   rc = do_traverse(c);
+  if (rc) return rc;
+  TRY(hipEventRecord(c->ev[EV_TRAV], c->stream));
+  c->stage = 3;
+  return TFMM_OK;
+}
+
+int tfmm_traverse_lists(tfmm_ctx *c, const int32_t *m2l, int64_t n_m2l, const int32_t *p2p,
+                        int64_t n_p2p) {
+  int rc = need_stage(c, 2, "tfmm_traverse_lists");
+  if (rc) return rc;
+  if (c->nprob > 1) return fail(c, TFMM_EINVAL, "tfmm_traverse_lists: one problem, not a batch");
+  rc = traverse_given(c, m2l, n_m2l, p2p, n_p2p);
   if (rc) return rc;
   TRY(hipEventRecord(c->ev[EV_TRAV], c->stream));
   c->stage = 3;

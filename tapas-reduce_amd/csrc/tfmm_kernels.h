@@ -98,6 +98,10 @@ hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin
                            const int2 *topo, const int32_t *depth, const double *width,
                            int32_t *m2l_slot, uint32_t *used, uint32_t *n_outside,
                            hipStream_t s);
+// group-table slots of M2L pairs given from outside the traversal
+hipError_t launch_pair_slots(const int2 *pairs, uint32_t n, const double4 *geo,
+                             const int32_t *depth, const double *width, int32_t *slot,
+                             uint32_t *used, uint32_t *n_outside, hipStream_t s);
 // leaf-pair list: targets and sources of the pairs; then, from the
 // target-sorted list, the CSR offsets and every target's sources ascending
 hipError_t launch_split_pairs(const int2 *pairs, uint32_t n, uint32_t *tgt, uint32_t *src,

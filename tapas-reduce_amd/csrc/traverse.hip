@@ -203,6 +203,23 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
   if (lane == 0 && outside) atomicAdd(n_outside, outside);
 }
 
+// Group-table slots of M2L pairs that come from outside the traversal (the
+// host recursion of the reference's own FMM_DTT functor): the same slot, used
+// flag and outside count a traversal step gives the pairs it emits.
+__global__ void k_pair_slots(const int2 *pairs, uint32_t n, const double4 *geo,
+                             const int32_t *depth, const double *width, int32_t *slot,
+                             uint32_t *used, uint32_t *n_outside) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int2 pr = pairs[t];
+  const double4 gi = geo[pr.x], gj = geo[pr.y];
+  const double ci[3] = {gi.x, gi.y, gi.z}, cj[3] = {gj.x, gj.y, gj.z};
+  const int32_t g = m2lg::pair_slot(depth[pr.x], depth[pr.y], width[pr.x], width[pr.y], ci, cj);
+  slot[t] = g;
+  if (g >= 0) used[g] = 1u;
+  else atomicAdd(n_outside, 1u);
+}
+
 __global__ void k_unpack_pairs(const uint64_t *keys, uint32_t n, int sbits, int32_t *src) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -329,6 +346,14 @@ hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin
   k_dtt_step<<<(nin_max + per - 1) / per, DTT_THREADS, 0, s>>>(
       in, nin_dev, nin_max, out, out_cap, nout, m2l, m2l_cap, p2p, p2p_cap, cnt, geo, topo, depth,
       width, m2l_slot, used, n_outside);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_slots(const int2 *pairs, uint32_t n, const double4 *geo,
+                             const int32_t *depth, const double *width, int32_t *slot,
+                             uint32_t *used, uint32_t *n_outside, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_pair_slots<<<(n + 255) / 256, 256, 0, s>>>(pairs, n, geo, depth, width, slot, used, n_outside);
   return hipGetLastError();
 }
 

@@ -1,6 +1,8 @@
-// tapas_fmm.h -- the Tapas FMM user surface (Partition / Map / Product, the
-// Cell and Body layout and the FMM functor names) on top of the MI355X
-// library's C ABI (include/tfmm.h).  Header-only C++11; link libtfmm.so.
+// tapas_fmm.h -- the Tapas FMM user surface on top of the MI355X library's C
+// ABI (include/tfmm.h): Partition, the Map overloads over cells, subcells,
+// bodies and their products, tapas::Product, the Cell and Body layout, and the
+// FMM kernel functions the application's functors call.  Header-only C++11;
+// link libtfmm.so.
 //
 // Reference surface it mirrors (TP = cpp/include/tapas,
 // EX = sample/exafmm-dev-13274dd4ac68):
@@ -8,23 +10,39 @@
 //                                   SRC at 24, IBODY at 32, TRG at 56)
 //   BodyAttr = kvec4, CellAttr      EX/examples/tapas_exafmm.h:10-17
 //   TapasFMM::Partition             TP/hot.h:1349-1352
-//   TapasFMM::Map overloads         TP/hot.h:1354-1396
+//   TapasFMM::Map overloads         TP/hot.h:1354-1396 (products, subcells,
+//                                   bodies, one cell)
 //   tapas::Product                  TP/iterator.h:407-434
-//   FMM_Upward / FMM_DTT /          EX/examples/tapas_exafmm.cxx:87-140, 209-286
-//   FMM_Downward
+//   Cell::Distance(c, tapas::Center) TP/hot.h:427-429, TP/geometry.h:80-83
+//   kernel calls of the functors    EX/examples/LaplaceSphericalCPU_tapas.h
+//                                   (P2M :138, M2M :165, M2L :208, L2P :275,
+//                                   L2L :309), LaplaceP2PCPU_tapas.h:48-74 (P2P)
 //   root->local_body(i) /           TP/hot.h:949-1050, used by CopyBackResult
 //   local_body_attr(i) / local_nb() (tapas_exafmm.cxx:388-398)
 //
-// The reference's mapper runs the user functor on host cells (or records P2P
-// products for its CUDA vectormap, TP/hot/mapper.h:452-690).  Here the
-// mapper recognises the three FMM functor types and runs the corresponding
-// phase of the GPU library instead: the functor objects are tags, their
-// bodies are the HIP kernels.  Errors of the library are thrown as
-// tapas_gpu::Error (the reference aborts).
+// Two ways to drive the GPU:
+//  * The application's own functors (FMM_Upward, FMM_DTT, FMM_Downward as
+//    written in EX/examples/tapas_exafmm.cxx:87-286, any struct with the
+//    same operator() signatures).  A Map from the root runs the functor on
+//    host cells, as the reference's mapper does; the kernel functions below
+//    only record what is asked of them, and when the root-level Map returns
+//    the recorded work is launched on the GPU (the reference's GPUMapper
+//    records P2P products and launches them the same way,
+//    TP/hot/mapper.h:452-690).  The recorded kernels also tell the mapper
+//    which pass the functor is (P2M/M2M: upward, L2L/L2P: downward), as the
+//    reference inspects a functor's direction by running it on proxy cells
+//    (TP/hot/exact_let.h:322-371).  The traversal's M2L and P2P lists are
+//    executed exactly as recorded (tfmm_traverse_lists), so a user-written
+//    traversal decides the interaction set.
+//  * The tags tapas_gpu::FMM_Upward / FMM_DTT / FMM_Downward, which run the
+//    library's own GPU traversal and passes directly.
+// Host-side values (Cell::attr(), body attributes) are copies downloaded from
+// the device after a pass; writing them does not change the GPU's data.
+// Library errors and unsupported calls throw tapas_gpu::Error (the reference
+// aborts).
 #ifndef TAPAS_GPU_TAPAS_FMM_H_
 #define TAPAS_GPU_TAPAS_FMM_H_
 
-#include <array>
 #include <complex>
 #include <cstddef>
 #include <cstdint>
@@ -32,6 +50,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/tfmm.h"
@@ -42,25 +61,75 @@ typedef double real_t;
 typedef std::complex<real_t> complex_t;
 typedef int64_t index_t;
 
+// Fixed-size vector with the operations the FMM application uses on
+// positions, body attributes and expansion coefficients (EX/include/vec.h).
+template <int N, class T>
+struct vec {
+  T a[N];
+  vec() {
+    for (int i = 0; i < N; i++) a[i] = T(0);
+  }
+  vec(const T &s) {  // NOLINT: broadcast, as the reference's vec(const T&)
+    for (int i = 0; i < N; i++) a[i] = s;
+  }
+  vec &operator=(const T &s) {
+    for (int i = 0; i < N; i++) a[i] = s;
+    return *this;
+  }
+  T &operator[](int i) { return a[i]; }
+  const T &operator[](int i) const { return a[i]; }
+  vec &operator+=(const vec &o) {
+    for (int i = 0; i < N; i++) a[i] += o.a[i];
+    return *this;
+  }
+  vec &operator-=(const vec &o) {
+    for (int i = 0; i < N; i++) a[i] -= o.a[i];
+    return *this;
+  }
+  vec &operator*=(const T &s) {
+    for (int i = 0; i < N; i++) a[i] *= s;
+    return *this;
+  }
+  vec operator+(const vec &o) const { return vec(*this) += o; }
+  vec operator-(const vec &o) const { return vec(*this) -= o; }
+  vec operator*(const T &s) const { return vec(*this) *= s; }
+  bool operator==(const vec &o) const {
+    for (int i = 0; i < N; i++)
+      if (!(a[i] == o.a[i])) return false;
+    return true;
+  }
+  bool operator!=(const vec &o) const { return !(*this == o); }
+};
+template <int N, class T>
+inline T norm(const vec<N, T> &v) {  // squared length, as EX/include/vec.h's norm
+  T s = 0;
+  for (int i = 0; i < N; i++) s += v[i] * v[i];
+  return s;
+}
+
+typedef vec<3, real_t> vec3;
+typedef vec<4, real_t> kvec4;
+typedef vec<TFMM_NTERM, complex_t> vecP;
+
 // The reference's Body (types.h:62-74, built without -mavx).
 struct Body {
-  real_t X[3];
+  vec3 X;
   real_t SRC;
   int IBODY;
   int IRANK;
   uint64_t ICELL;
   real_t WEIGHT;
-  real_t TRG[4];
+  kvec4 TRG;
 } __attribute__((aligned(16)));
 static_assert(sizeof(Body) == 96, "Body must keep the reference's 96-byte layout");
 static_assert(offsetof(Body, SRC) == 24 && offsetof(Body, TRG) == 56, "Body field offsets");
 
-typedef std::array<real_t, 4> BodyAttr;  // kvec4: (potential, ax, ay, az)
+typedef kvec4 BodyAttr;  // (potential, ax, ay, az)
 
 struct CellAttr {
   real_t R;
-  complex_t M[TFMM_NTERM];
-  complex_t L[TFMM_NTERM];
+  vecP M;
+  vecP L;
 };
 
 class Error : public std::runtime_error {
@@ -68,14 +137,39 @@ class Error : public std::runtime_error {
   explicit Error(const std::string &m) : std::runtime_error(m) {}
 };
 
-// FMM functors: the user-visible names of the reference application.
+// Tags that run the library's own GPU passes (see the file comment).
 struct FMM_Upward {};
 struct FMM_Downward {};
 struct FMM_DTT {};
 
 enum class Precision { FP64 = TFMM_FP64, Mixed = TFMM_MIXED };
 
+// tapas::Center: Distance between cell centres
+struct CenterTag {};
+
 namespace detail {
+
+// What a root-level Map recorded while the application's functor ran on
+// host cells.
+struct Recorder {
+  bool active = false;
+  std::vector<int32_t> m2l, p2p;        // (target, source) pairs
+  std::vector<uint8_t> p2m, m2m, l2l, l2p;  // per cell
+  void start(size_t nc) {
+    active = true;
+    m2l.clear();
+    p2p.clear();
+    p2m.assign(nc, 0);
+    m2m.assign(nc, 0);
+    l2l.assign(nc, 0);
+    l2p.assign(nc, 0);
+  }
+  static bool any(const std::vector<uint8_t> &v) {
+    for (uint8_t x : v)
+      if (x) return true;
+    return false;
+  }
+};
 
 struct Tree {
   tfmm_ctx *ctx = nullptr;
@@ -88,9 +182,10 @@ struct Tree {
   std::vector<double> center, width;
   std::vector<int32_t> first_child;  // -1 for leaves
   std::vector<int32_t> parent;
-  std::vector<CellAttr> cell_attr;   // filled on demand
+  std::vector<CellAttr> cell_attr;   // host copy, filled on demand
   bool attr_valid = false;
   int stage = 0;
+  Recorder rec;
 
   ~Tree() {
     if (ctx) tfmm_destroy(ctx);
@@ -99,9 +194,29 @@ struct Tree {
     if (rc != TFMM_OK)
       throw Error(std::string(what) + ": " + (ctx ? tfmm_last_error(ctx) : "no context"));
   }
+  size_t ncells() const { return key.size(); }
+  void load_attrs() {
+    const size_t nc = ncells();
+    cell_attr.assign(nc, CellAttr());
+    if (stage >= 2) {
+      std::vector<double> M(nc * TFMM_NTERM * 2), L(nc * TFMM_NTERM * 2);
+      check(tfmm_get_expansions(ctx, M.data(), stage >= 3 ? L.data() : nullptr),
+            "tfmm_get_expansions");
+      for (size_t c = 0; c < nc; c++)
+        for (int i = 0; i < TFMM_NTERM; i++) {
+          const size_t o = 2 * (c * TFMM_NTERM + i);
+          cell_attr[c].M[i] = complex_t(M[o], M[o + 1]);
+          cell_attr[c].L[i] = complex_t(L[o], L[o + 1]);
+        }
+    }
+    attr_valid = true;
+  }
 };
 
 }  // namespace detail
+
+class SubCellRange;
+class BodyRange;
 
 class Cell {
  public:
@@ -112,20 +227,37 @@ class Cell {
   int nsubcells() const { return IsLeaf() ? 0 : 8; }
   uint64_t key() const { return t_->key[idx_]; }
   int depth() const { return (int)(key() & 31u); }
-  std::array<real_t, 3> center() const {
-    return {{t_->center[3 * idx_], t_->center[3 * idx_ + 1], t_->center[3 * idx_ + 2]}};
+  vec3 center() const {
+    vec3 c;
+    for (int d = 0; d < 3; d++) c[d] = t_->center[3 * idx_ + d];
+    return c;
   }
-  // max over the dimensions of the region width (the MAC radius uses it)
+  // region width along dimension d (cells are cubes: the bounding region is a
+  // scalar cube, TP/hot/buildtree.h:77-86)
   real_t width(int) const { return t_->width[idx_]; }
+  // squared distance of the centres, ((dx^2 + dy^2) + dz^2) as
+  // TP/geometry.h:80-83 sums it
+  real_t Distance(const Cell &o, CenterTag) const {
+    real_t r2 = 0;
+    for (int d = 0; d < 3; d++) {
+      const real_t dd = t_->center[3 * idx_ + d] - o.t_->center[3 * o.idx_ + d];
+      r2 += dd * dd;
+    }
+    return r2;
+  }
   Cell subcell(int i) const {
-    if (IsLeaf()) throw Error("Cell::subcell called for a leaf cell");
+    if (IsLeaf()) throw Error("Cell::subcell(int) is called for a leaf cell");
+    if (i < 0 || i >= 8) throw Error("Cell::subcell(int): index out of range");
     return Cell(t_, t_->first_child[idx_] + i);
   }
   Cell parent() const {
-    if (IsRoot()) throw Error("Cell::parent called for the root");
+    if (IsRoot()) throw Error("Cell::parent() is called for the root");
     return Cell(t_, t_->parent[idx_]);
   }
+  inline SubCellRange subcells() const;
+  inline BodyRange bodies() const;
   bool operator==(const Cell &o) const { return t_ == o.t_ && idx_ == o.idx_; }
+  bool operator!=(const Cell &o) const { return !(*this == o); }
 
   // body i of a leaf (sorted order)
   Body &body(index_t i) { return t_->bodies[t_->beg[idx_] + i]; }
@@ -134,25 +266,10 @@ class Cell {
   Body &local_body(index_t i) { return t_->bodies[i]; }
   BodyAttr &local_body_attr(index_t i) { return t_->attrs[i]; }
 
-  // multipole / local expansion (downloaded from the device on first use)
-  const CellAttr &attr() const {
-    if (!t_->attr_valid) {
-      const size_t nc = t_->key.size();
-      std::vector<double> M(nc * TFMM_NTERM * 2), L(nc * TFMM_NTERM * 2);
-      const bool have_l = t_->stage >= 3;
-      t_->check(tfmm_get_expansions(t_->ctx, M.data(), have_l ? L.data() : nullptr),
-                "tfmm_get_expansions");
-      t_->cell_attr.assign(nc, CellAttr());
-      for (size_t c = 0; c < nc; c++) {
-        for (int i = 0; i < TFMM_NTERM; i++) {
-          const size_t o = 2 * (c * TFMM_NTERM + i);
-          t_->cell_attr[c].M[i] = complex_t(M[o], M[o + 1]);
-          t_->cell_attr[c].L[i] = complex_t(L[o], L[o + 1]);
-        }
-        t_->cell_attr[c].R = 0;
-      }
-      t_->attr_valid = true;
-    }
+  // multipole / local expansion: a host copy, downloaded from the device
+  // after a pass (zero before the upward pass)
+  CellAttr &attr() const {
+    if (!t_->attr_valid) t_->load_attrs();
     return t_->cell_attr[idx_];
   }
 
@@ -166,15 +283,124 @@ class Cell {
   index_t idx_;
 };
 
-// tapas::Product of two cells (the only product the FMM driver maps over)
-struct CellProduct {
-  Cell a, b;
+// Cell::subcells(): the 8 children of an internal cell
+class SubCellRange {
+ public:
+  explicit SubCellRange(const Cell &c) : cell_(c) {}
+  int size() const { return cell_.nsubcells(); }
+  Cell operator[](int i) const { return cell_.subcell(i); }
+  const Cell &cell() const { return cell_; }
+
+ private:
+  Cell cell_;
 };
-inline CellProduct Product(Cell &a, Cell &b) { return CellProduct{a, b}; }
+
+// Cell::bodies(): the bodies of a leaf with their attributes
+class BodyRange {
+ public:
+  explicit BodyRange(const Cell &c) : cell_(c) {}
+  index_t size() const { return cell_.nb(); }
+  Body &operator[](index_t i) { return cell_.body(i); }
+  BodyAttr &attr(index_t i) { return cell_.body_attr(i); }
+  const Cell &cell() const { return cell_; }
+
+ private:
+  Cell cell_;
+};
+
+inline SubCellRange Cell::subcells() const { return SubCellRange(*this); }
+inline BodyRange Cell::bodies() const {
+  if (!IsLeaf()) throw Error("Cell::bodies() is called for a cell that is not a leaf");
+  return BodyRange(*this);
+}
+
+// tapas::Product of two ranges (TP/iterator.h:407-434)
+template <class A, class B>
+struct ProductRange {
+  A t1;
+  B t2;
+};
+inline ProductRange<Cell, Cell> Product(const Cell &a, const Cell &b) { return {a, b}; }
+inline ProductRange<SubCellRange, Cell> Product(const SubCellRange &a, const Cell &b) {
+  return {a, b};
+}
+inline ProductRange<Cell, SubCellRange> Product(const Cell &a, const SubCellRange &b) {
+  return {a, b};
+}
+inline ProductRange<SubCellRange, SubCellRange> Product(const SubCellRange &a,
+                                                        const SubCellRange &b) {
+  return {a, b};
+}
+inline ProductRange<BodyRange, BodyRange> Product(const BodyRange &a, const BodyRange &b) {
+  return {a, b};
+}
+// the CellProduct of the tag interface
+typedef ProductRange<Cell, Cell> CellProduct;
+
+// ---------------------------------------------------------------------------
+// Kernel functions called by the application's functors (the names and
+// signatures of LaplaceSphericalCPU_tapas.h / LaplaceP2PCPU_tapas.h).  Inside
+// a root-level Map they record the call; the GPU runs the recorded work.
+// ---------------------------------------------------------------------------
+namespace detail {
+inline Recorder &recorder(const Cell &c, const char *what) {
+  Recorder &r = c.tree()->rec;
+  if (!r.active)
+    throw Error(std::string(what) + " is called outside a TapasFMM::Map from the root");
+  return r;
+}
+inline void no_periodic(const vec3 &xp) {
+  if (xp[0] != 0 || xp[1] != 0 || xp[2] != 0)
+    throw Error("periodic images are not supported (Xperiodic must be 0)");
+}
+}  // namespace detail
+
+inline void P2M(Cell &c) {
+  if (!c.IsLeaf()) throw Error("P2M is called for a cell that is not a leaf");
+  detail::recorder(c, "P2M").p2m[c.index()] = 1;
+}
+inline void M2M(Cell &c) {
+  if (c.IsLeaf()) throw Error("M2M is called for a leaf");
+  detail::recorder(c, "M2M").m2m[c.index()] = 1;
+}
+inline void L2L(Cell &c) {
+  if (c.IsRoot()) throw Error("L2L is called for the root");
+  detail::recorder(c, "L2L").l2l[c.index()] = 1;
+}
+inline void M2L(Cell &Ci, Cell &Cj, vec3 Xperiodic, bool mutual) {
+  detail::no_periodic(Xperiodic);
+  if (mutual) throw Error("mutual interaction is not supported (mutual must be 0)");
+  detail::Recorder &r = detail::recorder(Ci, "M2L");
+  r.m2l.push_back((int32_t)Ci.index());
+  r.m2l.push_back((int32_t)Cj.index());
+}
+// L2P of one body (the reference passes the leaf as a pointer); mapped over a
+// leaf's bodies it records the leaf's L2P once
+inline void L2P(Body &, BodyAttr &, Cell *c) {
+  if (!c) throw Error("L2P needs its leaf");
+  detail::recorder(*c, "L2P").l2p[c->index()] = 1;
+}
+struct P2P {
+  template <class B, class A>
+  void operator()(B &, A &, B &, A &, vec3, int) const {
+    throw Error("P2P runs on the GPU: map it over Product(Ci.bodies(), Cj.bodies())");
+  }
+};
+
+namespace detail {
+template <class F>
+struct is_l2p : std::false_type {};
+template <>
+struct is_l2p<void (*)(Body &, BodyAttr &, Cell *)> : std::true_type {};
+template <class F>
+struct is_p2p : std::is_same<typename std::decay<F>::type, P2P> {};
+}  // namespace detail
 
 struct TapasFMM {
   typedef tapas_gpu::Cell Cell;
   typedef tapas_gpu::Body Body;
+  typedef tapas_gpu::BodyAttr BodyAttr;
+  typedef tapas_gpu::CellAttr CellAttr;
 
   // Copies the bodies and builds the octree on the GPU.  theta and the
   // precision are taken at Map(FMM_DTT, ...); the tree does not depend on them.
@@ -223,39 +449,152 @@ struct TapasFMM {
     t->check(tfmm_get_sorted(t->ctx, keys.data(), perm.data()), "tfmm_get_sorted");
     t->bodies.resize(nb);
     for (index_t i = 0; i < nb; i++) t->bodies[i] = b[perm[i]];
-    t->attrs.assign(nb, BodyAttr{{0, 0, 0, 0}});
+    t->attrs.assign(nb, BodyAttr(0.0));
     t->stage = 1;
     return new Cell(t, 0);
   }
 
+  // ---- the library's own passes (tags) ----------------------------------
+
   // Map(FMM_Upward(), *root, theta): P2M at the leaves, M2M upward
   static void Map(FMM_Upward, Cell &root, real_t /*theta*/) {
-    auto t = root.tree();
     if (!root.IsRoot()) throw Error("the GPU mapper runs FMM_Upward from the root");
-    t->check(tfmm_upward(t->ctx), "tfmm_upward");
-    t->attr_valid = false;
-    t->stage = 2;
+    upward(root.tree());
   }
-
-  // Map(FMM_DTT(), Product(*root, *root), mutual, nspawn, theta): the dual
-  // tree traversal with its M2L and P2P kernels
+  // Map(FMM_DTT(), Product(*root, *root), mutual, nspawn, theta): the GPU
+  // dual tree traversal with its M2L and P2P kernels
   static void Map(FMM_DTT, CellProduct prod, int mutual, int /*nspawn*/, real_t theta) {
-    if (!(prod.a.IsRoot() && prod.b.IsRoot()))
+    if (!(prod.t1.IsRoot() && prod.t2.IsRoot()))
       throw Error("the GPU mapper runs FMM_DTT from Product(root, root)");
     if (mutual) throw Error("mutual interaction is not supported (mutual must be 0)");
-    auto t = prod.a.tree();
-    t->prm.theta = theta;
-    t->check(tfmm_set_params(t->ctx, &t->prm), "tfmm_set_params");
+    auto t = prod.t1.tree();
+    set_theta(t, theta);
     t->check(tfmm_traverse(t->ctx), "tfmm_traverse");
     t->attr_valid = false;
     t->stage = 3;
   }
-
   // Map(FMM_Downward(), *root): L2L downward, L2P at the leaves; the fields
   // become visible through local_body_attr(i)
   static void Map(FMM_Downward, Cell &root) {
-    auto t = root.tree();
     if (!root.IsRoot()) throw Error("the GPU mapper runs FMM_Downward from the root");
+    downward(root.tree());
+  }
+
+  // ---- the application's functors (TP/hot.h:1354-1396) ------------------
+
+  // One cell.  From the root (outside any other Map) the functor runs on the
+  // host cells with its kernel calls recorded, and the recorded pass runs on
+  // the GPU when it returns; inside a Map the functor is simply called.
+  template <class F, class... Args>
+  static void Map(F f, Cell &c, Args... args) {
+    auto t = c.tree();
+    if (t->rec.active) {
+      f(c, args...);
+      return;
+    }
+    if (!c.IsRoot()) throw Error("the GPU mapper starts a pass from the root");
+    t->rec.start(t->ncells());
+    try {
+      f(c, args...);
+    } catch (...) {
+      t->rec.active = false;
+      throw;
+    }
+    t->rec.active = false;
+    run_recorded_pass(t);
+  }
+
+  // Subcells of a cell (inside a Map)
+  template <class F, class... Args>
+  static void Map(F f, SubCellRange r, Args... args) {
+    for (int i = 0; i < r.size(); i++) {
+      Cell ci = r[i];
+      Map(f, ci, args...);
+    }
+  }
+
+  // Bodies of a leaf: L2P is recorded for the leaf; any other body functor
+  // runs on the host copies of the bodies
+  template <class F, class... Args>
+  static void Map(F f, BodyRange r, Args... args) {
+    map_bodies(f, r, typename detail::is_l2p<F>::type(), args...);
+  }
+
+  // Cell products: from Product(root, root) outside any Map, the functor is
+  // a traversal whose M2L calls and P2P products are recorded, then run by
+  // the GPU kernels; inside a Map every pair of the product is visited.
+  template <class F, class... Args>
+  static void Map(F f, ProductRange<Cell, Cell> p, Args... args) {
+    auto t = p.t1.tree();
+    if (t->rec.active) {
+      f(p.t1, p.t2, args...);
+      return;
+    }
+    if (!(p.t1.IsRoot() && p.t2.IsRoot()))
+      throw Error("the GPU mapper starts a traversal from Product(root, root)");
+    t->rec.start(t->ncells());
+    try {
+      f(p.t1, p.t2, args...);
+    } catch (...) {
+      t->rec.active = false;
+      throw;
+    }
+    t->rec.active = false;
+    run_recorded_pass(t);
+  }
+  template <class F, class... Args>
+  static void Map(F f, ProductRange<SubCellRange, Cell> p, Args... args) {
+    for (int i = 0; i < p.t1.size(); i++) {
+      Cell ci = p.t1[i], cj = p.t2;
+      f(ci, cj, args...);
+    }
+  }
+  template <class F, class... Args>
+  static void Map(F f, ProductRange<Cell, SubCellRange> p, Args... args) {
+    for (int j = 0; j < p.t2.size(); j++) {
+      Cell ci = p.t1, cj = p.t2[j];
+      f(ci, cj, args...);
+    }
+  }
+  template <class F, class... Args>
+  static void Map(F f, ProductRange<SubCellRange, SubCellRange> p, Args... args) {
+    for (int i = 0; i < p.t1.size(); i++)
+      for (int j = 0; j < p.t2.size(); j++) {
+        Cell ci = p.t1[i], cj = p.t2[j];
+        f(ci, cj, args...);
+      }
+  }
+
+  // Body products (the P2P of two leaves): recorded for the GPU P2P kernel;
+  // an empty product is skipped, as TP/hot.h:1367 skips it
+  template <class... Args>
+  static void Map(P2P, ProductRange<BodyRange, BodyRange> p, vec3 Xperiodic, int mutual,
+                  Args...) {
+    detail::no_periodic(Xperiodic);
+    if (mutual) throw Error("mutual interaction is not supported (mutual must be 0)");
+    if (p.t1.size() == 0 || p.t2.size() == 0) return;
+    detail::Recorder &r = detail::recorder(p.t1.cell(), "Map(P2P, Product(bodies, bodies))");
+    r.p2p.push_back((int32_t)p.t1.cell().index());
+    r.p2p.push_back((int32_t)p.t2.cell().index());
+  }
+
+  static tfmm_stats_t Stats(Cell &root) {
+    tfmm_stats_t st;
+    root.tree()->check(tfmm_get_stats(root.tree()->ctx, &st), "tfmm_get_stats");
+    return st;
+  }
+
+ private:
+  static void set_theta(const std::shared_ptr<detail::Tree> &t, real_t theta) {
+    t->prm.theta = theta;
+    t->check(tfmm_set_params(t->ctx, &t->prm), "tfmm_set_params");
+  }
+  static void upward(const std::shared_ptr<detail::Tree> &t) {
+    t->check(tfmm_upward(t->ctx), "tfmm_upward");
+    t->attr_valid = false;
+    t->stage = 2;
+  }
+  static void downward(const std::shared_ptr<detail::Tree> &t) {
     t->check(tfmm_downward(t->ctx), "tfmm_downward");
     std::vector<double> f(4 * (size_t)t->n);
     t->check(tfmm_get_fields(t->ctx, f.data(), 1, 0), "tfmm_get_fields");
@@ -265,13 +604,60 @@ struct TapasFMM {
     t->stage = 4;
   }
 
-  static tfmm_stats_t Stats(Cell &root) {
-    tfmm_stats_t st;
-    root.tree()->check(tfmm_get_stats(root.tree()->ctx, &st), "tfmm_get_stats");
-    return st;
+  template <class F, class... Args>
+  static void map_bodies(F, BodyRange r, std::true_type, Args...) {
+    if (r.size() == 0) return;
+    detail::recorder(r.cell(), "Map(L2P, bodies)").l2p[r.cell().index()] = 1;
+  }
+  template <class F, class... Args>
+  static void map_bodies(F f, BodyRange r, std::false_type, Args... args) {
+    for (index_t i = 0; i < r.size(); i++) f(r[i], r.attr(i), args...);
+  }
+
+  // The pass the recorded kernels make up, launched on the GPU.  A pass must
+  // be whole (the GPU kernels treat every cell): the upward pass has P2M at
+  // every leaf and M2M at every internal cell, the downward pass L2L at every
+  // cell but the root and L2P at every non-empty leaf.
+  static void run_recorded_pass(const std::shared_ptr<detail::Tree> &t) {
+    detail::Recorder &r = t->rec;
+    const bool up = r.any(r.p2m) || r.any(r.m2m);
+    const bool down = r.any(r.l2l) || r.any(r.l2p);
+    const bool trav = !r.m2l.empty() || !r.p2p.empty();
+    if ((int)up + (int)down + (int)trav > 1)
+      throw Error("a functor mixes kernels of different passes (upward, traversal, downward)");
+    const size_t nc = t->ncells();
+    if (up) {
+      for (size_t c = 0; c < nc; c++) {
+        const bool leaf = t->first_child[c] < 0;
+        if (leaf ? !r.p2m[c] : !r.m2m[c])
+          throw Error("the GPU mapper runs whole upward passes: a cell got no P2M/M2M");
+      }
+      upward(t);
+    } else if (down) {
+      if (t->stage < 3) throw Error("downward pass before the traversal");
+      for (size_t c = 1; c < nc; c++) {
+        const bool leaf = t->first_child[c] < 0;
+        if (!r.l2l[c] || (leaf && t->nb[c] > 0 && !r.l2p[c]))
+          throw Error("the GPU mapper runs whole downward passes: a cell got no L2L/L2P");
+      }
+      downward(t);
+    } else if (trav) {
+      if (t->stage < 2) throw Error("traversal before the upward pass");
+      t->check(tfmm_traverse_lists(t->ctx, r.m2l.data(), (int64_t)(r.m2l.size() / 2),
+                                   r.p2p.data(), (int64_t)(r.p2p.size() / 2)),
+               "tfmm_traverse_lists");
+      t->attr_valid = false;
+      t->stage = 3;
+    }
   }
 };
 
 }  // namespace tapas_gpu
+
+// The reference's tapas::Product and tapas::Center spellings.
+namespace tapas {
+using tapas_gpu::Product;
+constexpr tapas_gpu::CenterTag Center{};
+}  // namespace tapas
 
 #endif  // TAPAS_GPU_TAPAS_FMM_H_

@@ -76,7 +76,7 @@ EXPORTED = [
     "tfmm_upward",
     "tfmm_traverse", "tfmm_downward", "tfmm_evaluate", "tfmm_get_fields", "tfmm_get_stats",
     "tfmm_get_sorted", "tfmm_get_cells", "tfmm_get_expansions", "tfmm_get_lists", "tfmm_direct",
-    "tfmm_direct_noalloc",
+    "tfmm_direct_noalloc", "tfmm_traverse_lists",
     "tfmm_picker_nparam", "tfmm_picker_create", "tfmm_picker_destroy", "tfmm_picker_last_error",
     "tfmm_picker_set_weights", "tfmm_picker_get_weights", "tfmm_picker_set_theta_max",
     "tfmm_picker_train",
@@ -118,6 +118,7 @@ def lib():
         L.tfmm_set_params.argtypes = [vp, ctypes.POINTER(Params)]
         for f in ("tfmm_upward", "tfmm_traverse", "tfmm_downward"):
             getattr(L, f).argtypes = [vp]
+        L.tfmm_traverse_lists.argtypes = [vp, vp, ctypes.c_int64, vp, ctypes.c_int64]
         L.tfmm_get_fields.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
         L.tfmm_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.tfmm_get_sorted.argtypes = [vp, vp, vp]
@@ -278,6 +279,14 @@ class Context:
 
     def traverse(self):
         self._check(lib().tfmm_traverse(self._h), "tfmm_traverse")
+
+    def traverse_lists(self, m2l, p2p):
+        """Run the M2L and P2P kernels over given (target, source) cell pairs
+        instead of the GPU traversal."""
+        m = np.ascontiguousarray(m2l, np.int32).reshape(-1, 2)
+        p = np.ascontiguousarray(p2p, np.int32).reshape(-1, 2)
+        self._check(lib().tfmm_traverse_lists(self._h, _ptr(m), len(m), _ptr(p), len(p)),
+                    "tfmm_traverse_lists")
 
     def downward(self):
         self._check(lib().tfmm_downward(self._h), "tfmm_downward")
