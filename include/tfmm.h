@@ -65,8 +65,10 @@ typedef struct tfmm_ctx tfmm_ctx;
  * EX/include/args.h:101-158). */
 typedef struct {
   int32_t ncrit;     /* a cell is a leaf iff it holds <= ncrit bodies */
-  int32_t mutual;    /* must be 0: mutual interaction is not supported (as on the
-                        reference's CUDA path, tapas_exafmm.cxx:507-515) */
+  int32_t mutual;    /* 0 or 1: 1 visits every unordered cell pair once and applies
+                        its M2L and P2P to both cells (the reference's default
+                        --mutual 1, EX/include/args.h:102; its CUDA path refused
+                        it, tapas_exafmm.cxx:507-515) */
   int32_t precision; /* TFMM_FP64 or TFMM_MIXED */
   int32_t reserved;
   double theta;      /* multipole acceptance parameter, 0 < theta */
@@ -89,10 +91,13 @@ typedef struct {
   int32_t max_depth;
   double region_min[3];
   double region_max[3];
-  uint64_t n_m2l;            /* all M2L pairs the traversal produced (numM2L) */
-  uint64_t n_m2l_exec;       /* M2L pairs executed (both cells non-empty) */
-  uint64_t n_p2p_leafpairs;  /* P2P leaf pairs (both non-empty) */
-  uint64_t n_p2p_calls;      /* body pairs visited (numP2P: includes r == 0 self pairs) */
+  uint64_t n_m2l;            /* M2L calls of the traversal (numM2L; a mutual pair once) */
+  uint64_t n_m2l_exec;       /* M2L list entries executed (both cells non-empty; a
+                                mutual pair is two entries) */
+  uint64_t n_p2p_leafpairs;  /* P2P leaf-pair calls (both non-empty; a mutual pair once) */
+  uint64_t n_p2p_calls;      /* body pairs the reference's P2P functor visits (numP2P:
+                                includes r == 0 self pairs; under mutual the pairs
+                                j <= i of a leaf with itself) */
   uint64_t n_p2p_self;       /* r == 0 self pairs among them */
   /* milliseconds of the last evaluation, measured with HIP events on the
    * context stream; kernel entries are summed over their launches */
@@ -157,7 +162,10 @@ int tfmm_traverse(tfmm_ctx *ctx);
  * kernels.  m2l: n_m2l (target, source) cell pairs, 2 int32 each, in the
  * breadth-first cell order of tfmm_get_cells, empty cells included (counted
  * like the reference's numM2L, not executed); p2p: n_p2p leaf pairs (pairs with
- * an empty leaf are skipped).  One problem only (not a batch). */
+ * an empty leaf are skipped).  With the mutual parameter set (tfmm_set_params)
+ * the lists hold every unordered pair in both directions (a leaf with itself
+ * once) and the counters count each pair once, as the reference's functors
+ * are called.  One problem only (not a batch). */
 int tfmm_traverse_lists(tfmm_ctx *ctx, const int32_t *m2l, int64_t n_m2l, const int32_t *p2p,
                         int64_t n_p2p);
 int tfmm_downward(tfmm_ctx *ctx);

@@ -296,12 +296,8 @@ static void m2m(cx *Mp, const cx *Mc, const double dX[3]) {
   }
 }
 
-/* target L += M2L(source M); dX = target centre - source centre */
-static void m2l(cx *Li, const cx *Mj, const double dX[3]) {
-  cx Ynmi[P * P];
-  double rho, alpha, beta;
-  cart2sph(dX, &rho, &alpha, &beta);
-  eval_local(rho, alpha, beta, Ynmi);
+/* target L += M2L(source M) with the singular harmonics Ynmi of the offset */
+static void m2l_y(cx *Li, const cx *Mj, const cx *Ynmi) {
   for (int j = 0; j < P; j++) {
     double Cnm = oddeven(j);
     for (int k = 0; k <= j; k++) {
@@ -323,6 +319,28 @@ static void m2l(cx *Li, const cx *Mj, const double dX[3]) {
       Li[jks] = cx_add(Li[jks], L);
     }
   }
+}
+
+/* target L += M2L(source M); dX = target centre - source centre */
+static void m2l(cx *Li, const cx *Mj, const double dX[3]) {
+  cx Ynmi[P * P];
+  double rho, alpha, beta;
+  cart2sph(dX, &rho, &alpha, &beta);
+  eval_local(rho, alpha, beta, Ynmi);
+  m2l_y(Li, Mj, Ynmi);
+}
+
+/* Mutual M2L (LaplaceSphericalCPU_tapas.h:208-273 with mutual = 1): L_i +=
+ * M2L(M_j) as above, and L_j += M2L(M_i) with the harmonics evaluated at
+ * (rho, alpha + pi, beta), the reference's form of the reversed offset (:225) */
+static void m2l_mutual(cx *Li, cx *Lj, const cx *Mi, const cx *Mj, const double dX[3]) {
+  cx Ynmi[P * P], Ynmj[P * P];
+  double rho, alpha, beta;
+  cart2sph(dX, &rho, &alpha, &beta);
+  eval_local(rho, alpha, beta, Ynmi);
+  eval_local(rho, alpha + 3.14159265358979323846 /* M_PI */, beta, Ynmj);
+  m2l_y(Li, Mj, Ynmi);
+  m2l_y(Lj, Mi, Ynmj);
 }
 
 /* child L += shifted parent L; dX = child centre - parent centre */
@@ -432,6 +450,9 @@ struct orc_ctx {
   int max_depth;
   /* counters */
   uint64_t n_m2l, n_p2p_leafpairs, n_p2p_bodypairs, n_p2p_selfpairs;
+  uint64_t n_m2l_rec, n_p2p_rec; /* entries of the recorded (directed) lists */
+  /* mutual interaction (the reference's --mutual 1) */
+  int mutual;
   /* optional interaction lists (cell indices) */
   int record;
   int64_t *m2l_list, n_m2l_cap;
@@ -521,26 +542,40 @@ static void push_pair(int64_t **list, int64_t *cap, uint64_t cnt, int64_t a, int
   (*list)[2 * cnt + 1] = b;
 }
 
-static void p2p_leaves(orc_ctx *c, const ocell *ci, const ocell *cj) {
-  for (int64_t i = ci->bid; i < ci->bid + ci->nb; i++) {
-    for (int64_t j = cj->bid; j < cj->bid + cj->nb; j++) {
-      double dX[3] = {c->x[i] - c->x[j], c->y[i] - c->y[j], c->z[i] - c->z[j]};
-      double R2 = 0;
-      for (int d = 0; d < 3; d++) R2 += dX[d] * dX[d];
-      R2 += 0.0; /* softening EPS2 = 0 */
-      if (R2 != 0) {
-        double invR2 = 1.0 / R2;
-        double invR = c->q[i] * c->q[j] * sqrt(invR2);
-        for (int d = 0; d < 3; d++) dX[d] *= invR2 * invR;
-        c->trg[4 * i + 0] += invR;
-        c->trg[4 * i + 1] -= dX[0];
-        c->trg[4 * i + 2] -= dX[1];
-        c->trg[4 * i + 3] -= dX[2];
-        c->n_p2p_bodypairs++;
-      } else {
-        c->n_p2p_selfpairs++;
-      }
+/* P2P functor on one body pair (LaplaceP2PCPU_tapas.h:48-74): target i, and
+ * with mutual the source j too when the positions differ */
+static void p2p_pair(orc_ctx *c, int64_t i, int64_t j, int mutual) {
+  double dX[3] = {c->x[i] - c->x[j], c->y[i] - c->y[j], c->z[i] - c->z[j]};
+  double R2 = 0;
+  for (int d = 0; d < 3; d++) R2 += dX[d] * dX[d];
+  R2 += 0.0; /* softening EPS2 = 0 */
+  if (R2 != 0) {
+    double invR2 = 1.0 / R2;
+    double invR = c->q[i] * c->q[j] * sqrt(invR2);
+    for (int d = 0; d < 3; d++) dX[d] *= invR2 * invR;
+    c->trg[4 * i + 0] += invR;
+    c->trg[4 * i + 1] -= dX[0];
+    c->trg[4 * i + 2] -= dX[1];
+    c->trg[4 * i + 3] -= dX[2];
+    if (mutual && (c->x[i] != c->x[j] || c->y[i] != c->y[j] || c->z[i] != c->z[j])) {
+      c->trg[4 * j + 0] += invR;
+      c->trg[4 * j + 1] += dX[0];
+      c->trg[4 * j + 2] += dX[1];
+      c->trg[4 * j + 3] += dX[2];
     }
+    c->n_p2p_bodypairs++;
+  } else {
+    c->n_p2p_selfpairs++;
+  }
+}
+
+/* Map(P2P, Product(Ci.bodies(), Cj.bodies())): every pair, or with mutual
+ * on one leaf the pairs j <= i only (TP/hot/mapper.h:126-169) */
+static void p2p_leaves(orc_ctx *c, const ocell *ci, const ocell *cj) {
+  const int same = c->mutual && ci == cj;
+  for (int64_t i = ci->bid; i < ci->bid + ci->nb; i++) {
+    const int64_t jend = same ? i + 1 : cj->bid + cj->nb;
+    for (int64_t j = cj->bid; j < jend; j++) p2p_pair(c, i, j, c->mutual);
   }
 }
 
@@ -563,12 +598,19 @@ static void dtt(orc_ctx *c, int64_t ii, int64_t jj) {
   Rj = (Rj / 2 * 1.00001) / c->theta;
   if (R2 > (Ri + Rj) * (Ri + Rj)) {
     double dX[3] = {d0, d1, d2};
-    m2l(c->L + ii * NTERM, c->M + jj * NTERM, dX);
-    if (c->record) push_pair(&c->m2l_list, &c->n_m2l_cap, c->n_m2l, ii, jj);
+    if (c->mutual)
+      m2l_mutual(c->L + ii * NTERM, c->L + jj * NTERM, c->M + ii * NTERM, c->M + jj * NTERM, dX);
+    else
+      m2l(c->L + ii * NTERM, c->M + jj * NTERM, dX);
+    /* lists hold directed (target, source) pairs: a mutual call is two */
+    if (c->record) push_pair(&c->m2l_list, &c->n_m2l_cap, c->n_m2l_rec++, ii, jj);
+    if (c->record && c->mutual) push_pair(&c->m2l_list, &c->n_m2l_cap, c->n_m2l_rec++, jj, ii);
     c->n_m2l++;
   } else if (Ci->leaf && Cj->leaf) {
     if (Ci->nb > 0 && Cj->nb > 0) {
-      if (c->record) push_pair(&c->p2p_list, &c->n_p2p_cap, c->n_p2p_leafpairs, ii, jj);
+      if (c->record) push_pair(&c->p2p_list, &c->n_p2p_cap, c->n_p2p_rec++, ii, jj);
+      if (c->record && c->mutual && ii != jj)
+        push_pair(&c->p2p_list, &c->n_p2p_cap, c->n_p2p_rec++, jj, ii);
       c->n_p2p_leafpairs++;
       p2p_leaves(c, Ci, Cj);
     }
@@ -576,6 +618,11 @@ static void dtt(orc_ctx *c, int64_t ii, int64_t jj) {
     for (int i = 0; i < 8; i++) dtt(c, c->cells[ii].child[i], jj);
   } else if (Ci->leaf) {
     for (int j = 0; j < 8; j++) dtt(c, ii, c->cells[jj].child[j]);
+  } else if (c->mutual && ii == jj) {
+    /* both sides split, each unordered pair of children once
+     * (tapas_exafmm.cxx:270-271, TP/hot/mapper.h:29-121 with i <= j) */
+    for (int i = 0; i < 8; i++)
+      for (int j = i; j < 8; j++) dtt(c, c->cells[ii].child[i], c->cells[jj].child[j]);
   } else if (Ri >= Rj) {
     for (int i = 0; i < 8; i++) dtt(c, c->cells[ii].child[i], jj);
   } else {
@@ -603,8 +650,15 @@ static void downward(orc_ctx *c, int64_t ci) {
 
 orc_ctx *orc_run(const double *x, const double *y, const double *z, const double *q, int64_t n,
                  int ncrit, double theta, int phases, int record_lists) {
+  return orc_run_mutual(x, y, z, q, n, ncrit, theta, phases, record_lists, 0);
+}
+
+orc_ctx *orc_run_mutual(const double *x, const double *y, const double *z, const double *q,
+                        int64_t n, int ncrit, double theta, int phases, int record_lists,
+                        int mutual) {
   if (n <= 0 || ncrit <= 0 || !(theta > 0)) return NULL;
   orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+  c->mutual = mutual ? 1 : 0;
   c->n = n;
   c->ncrit = ncrit;
   c->theta = theta;
@@ -694,8 +748,14 @@ void orc_counts(const orc_ctx *c, uint64_t *out4) {
 
 /* recorded lists as (target cell index, source cell index) pairs */
 void orc_lists(const orc_ctx *c, int64_t *m2l, int64_t *p2p) {
-  if (m2l && c->m2l_list) memcpy(m2l, c->m2l_list, sizeof(int64_t) * 2 * c->n_m2l);
-  if (p2p && c->p2p_list) memcpy(p2p, c->p2p_list, sizeof(int64_t) * 2 * c->n_p2p_leafpairs);
+  if (m2l && c->m2l_list) memcpy(m2l, c->m2l_list, sizeof(int64_t) * 2 * c->n_m2l_rec);
+  if (p2p && c->p2p_list) memcpy(p2p, c->p2p_list, sizeof(int64_t) * 2 * c->n_p2p_rec);
+}
+
+/* entries of the recorded lists (directed pairs: a mutual call is two) */
+void orc_list_sizes(const orc_ctx *c, int64_t *out2) {
+  out2[0] = (int64_t)c->n_m2l_rec;
+  out2[1] = (int64_t)c->n_p2p_rec;
 }
 
 /* FP64 direct sum on a target subset, normalised by the target charge as the

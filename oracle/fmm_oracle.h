@@ -42,6 +42,10 @@ void orc_l2p(double *trg, const double *bx, double q, const double *L, const dou
 /* full evaluation; phases: 0 tree, 1 +upward, 2 +traversal, 3 +downward */
 orc_ctx *orc_run(const double *x, const double *y, const double *z, const double *q, int64_t n,
                  int ncrit, double theta, int phases, int record_lists);
+/* the same with the reference's mutual interaction (mutual = 1) */
+orc_ctx *orc_run_mutual(const double *x, const double *y, const double *z, const double *q,
+                        int64_t n, int ncrit, double theta, int phases, int record_lists,
+                        int mutual);
 void orc_free(orc_ctx *c);
 void orc_region(const orc_ctx *c, double *rmin, double *rmax);
 int64_t orc_ncells(const orc_ctx *c);
@@ -53,6 +57,7 @@ void orc_expansions(const orc_ctx *c, double *M, double *L);
 void orc_fields(const orc_ctx *c, double *trg4);
 void orc_counts(const orc_ctx *c, uint64_t *out4);
 void orc_lists(const orc_ctx *c, int64_t *m2l, int64_t *p2p);
+void orc_list_sizes(const orc_ctx *c, int64_t *out2);
 
 void orc_direct(const double *x, const double *y, const double *z, const double *q, int64_t n,
                 const int64_t *targets, int64_t nt, double *trg4);

@@ -38,8 +38,12 @@ def lib():
         L.orc_run.restype = ctypes.c_void_p
         L.orc_run.argtypes = [_d, _d, _d, _d, ctypes.c_int64, ctypes.c_int, ctypes.c_double,
                               ctypes.c_int, ctypes.c_int]
+        L.orc_run_mutual.restype = ctypes.c_void_p
+        L.orc_run_mutual.argtypes = [_d, _d, _d, _d, ctypes.c_int64, ctypes.c_int,
+                                     ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         for name in ("orc_free", "orc_region", "orc_ncells", "orc_max_depth", "orc_sorted",
-                     "orc_cells", "orc_expansions", "orc_fields", "orc_counts", "orc_lists"):
+                     "orc_cells", "orc_expansions", "orc_fields", "orc_counts", "orc_lists",
+                     "orc_list_sizes"):
             getattr(L, name).argtypes = None
         L.orc_ncells.restype = ctypes.c_int64
         L.orc_max_depth.restype = ctypes.c_int
@@ -129,15 +133,16 @@ def l2p(trg, bx, q, L, center):
 
 class Run:
     """One oracle evaluation.  phases: 0 tree, 1 +upward, 2 +traversal,
-    3 +downward (full FMM)."""
+    3 +downward (full FMM); mutual as the reference's --mutual."""
 
-    def __init__(self, x, y, z, q, ncrit=64, theta=0.4, phases=3, record_lists=False):
+    def __init__(self, x, y, z, q, ncrit=64, theta=0.4, phases=3, record_lists=False, mutual=0):
         L = lib()
         self.x, self.y, self.z, self.q = [np.ascontiguousarray(a, np.float64) for a in (x, y, z, q)]
         self.n = len(self.x)
-        h = L.orc_run(self.x.ctypes.data_as(_d), self.y.ctypes.data_as(_d),
-                      self.z.ctypes.data_as(_d), self.q.ctypes.data_as(_d), self.n, int(ncrit),
-                      float(theta), int(phases), int(bool(record_lists)))
+        h = L.orc_run_mutual(self.x.ctypes.data_as(_d), self.y.ctypes.data_as(_d),
+                             self.z.ctypes.data_as(_d), self.q.ctypes.data_as(_d), self.n,
+                             int(ncrit), float(theta), int(phases), int(bool(record_lists)),
+                             int(bool(mutual)))
         if not h:
             raise ValueError("oracle rejected the input")
         self._h = ctypes.c_void_p(h)
@@ -196,9 +201,11 @@ class Run:
                     p2p_selfpairs=int(c[3]))
 
     def lists(self):
-        c = self.counts()
-        m = np.zeros((c["m2l"], 2), np.int64)
-        p = np.zeros((c["p2p_leafpairs"], 2), np.int64)
+        """Recorded (target, source) cell pairs; a mutual call gives two."""
+        sz = np.zeros(2, np.int64)
+        lib().orc_list_sizes(self._h, _p(sz))
+        m = np.zeros((int(sz[0]), 2), np.int64)
+        p = np.zeros((int(sz[1]), 2), np.int64)
         lib().orc_lists(self._h, _p(m), _p(p))
         return m, p
 
@@ -300,8 +307,10 @@ def run_ref_ranks(x, y, z, q, ncrit, theta, ranks, timeout=None):
         return {"n": n, "ranks": p, "times": dict(zip(("tree", "upward", "traverse", "downward"), t))}
 
 
-def run_ref(x, y, z, q, ncrit=64, theta=0.4, count=True, dump_expansions=False, timeout=None):
-    """Run the compiled reference on the bodies; returns its dump as a dict."""
+def run_ref(x, y, z, q, ncrit=64, theta=0.4, count=True, dump_expansions=False, timeout=None,
+            mutual=0):
+    """Run the compiled reference on the bodies (with the given mutual flag);
+    returns its dump as a dict."""
     exe = ref_binary(count)
     if exe is None:
         raise FileNotFoundError("oracle/_ref not built")
@@ -311,6 +320,7 @@ def run_ref(x, y, z, q, ncrit=64, theta=0.4, count=True, dump_expansions=False, 
         write_input(fin, x, y, z, q, ncrit, theta)
         env = dict(os.environ)
         env.setdefault("OMP_NUM_THREADS", "1")
-        subprocess.run([exe, fin, fout, "1" if dump_expansions else "0"], check=True,
+        subprocess.run([exe, fin, fout, "1" if dump_expansions else "0", str(int(mutual))],
+                       check=True,
                        stdout=subprocess.PIPE, stderr=subprocess.PIPE, env=env, timeout=timeout)
         return read_ref_output(fout)

@@ -6,7 +6,9 @@
 // The reference application file is compiled unchanged; only its main() is
 // renamed so that this driver can call the same functors
 // (FMM_Upward / FMM_DTT / FMM_Downward, EX/examples/tapas_exafmm.cxx:87-286)
-// in the same sequence as its main (tapas_exafmm.cxx:557-641), with mutual=0.
+// in the same sequence as its main (tapas_exafmm.cxx:557-641), with the
+// mutual flag of the command line (default 0; the reference's main sets it
+// with root->SetOptMutual(args.mutual), tapas_exafmm.cxx:570).
 //
 // Input file (little endian):  int64 n, int32 ncrit, double theta,
 //                              double x[n], y[n], z[n], q[n]
@@ -45,10 +47,11 @@ struct CellRec {
 int main(int argc, char **argv) {
   MPI_Init(&argc, &argv);
   if (argc < 3) {
-    fprintf(stderr, "usage: %s in.bin out.bin [dump_expansions=0|1]\n", argv[0]);
+    fprintf(stderr, "usage: %s in.bin out.bin [dump_expansions=0|1 [mutual=0|1]]\n", argv[0]);
     return 2;
   }
   const bool dump_exp = argc > 3 && atoi(argv[3]) != 0;
+  const int mutual = argc > 4 ? atoi(argv[4]) : 0;
   FILE *fi = fopen(argv[1], "rb");
   if (!fi) { perror("input"); return 2; }
   int64_t n = 0; int32_t ncrit = 0; double theta = 0;
@@ -110,11 +113,11 @@ int main(int argc, char **argv) {
   double t0 = GetTime();
   TapasFMM::Cell *root = TapasFMM::Partition(bodies.data(), bodies.size(), ncrit);
   double t1 = GetTime();
-  root->SetOptMutual(0);
+  root->SetOptMutual(mutual);
   TapasFMM::Map(FMM_Upward(), *root, (real_t)theta);
   double t2 = GetTime();
   ResetCount();
-  TapasFMM::Map(FMM_DTT(), tapas::Product(*root, *root), 0, 1000, (real_t)theta);
+  TapasFMM::Map(FMM_DTT(), tapas::Product(*root, *root), mutual, 1000, (real_t)theta);
   uint64_t c_m2l = 0, c_p2p = 0;
 #ifdef COUNT
   c_m2l = numM2L; c_p2p = numP2P;

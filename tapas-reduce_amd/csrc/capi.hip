@@ -85,6 +85,7 @@ struct tfmm_ctx {
   int64_t dtt_hint_cells = -1;
   uint32_t n_m2l_exec = 0, n_p2p = 0;
   uint64_t n_m2l_all = 0, n_p2p_calls = 0, n_p2p_self = 0;
+  uint64_t n_p2p_calls_leaf = 0;  // P2P leaf-pair calls (a mutual pair once)
   uint32_t dtt_steps = 0;
 
   // batched M2L (offset groups)
@@ -221,8 +222,7 @@ int check_params(tfmm_ctx *c, const tfmm_params *p) {
   if (p->ncrit < 1) return fail(c, TFMM_EINVAL, "ncrit must be >= 1");
   if (!(p->theta > 0) || !(p->theta < 1e300))
     return fail(c, TFMM_EINVAL, "theta must be a positive finite number");
-  if (p->mutual != 0)
-    return fail(c, TFMM_EINVAL, "mutual interaction is not supported (mutual must be 0)");
+  if (p->mutual != 0 && p->mutual != 1) return fail(c, TFMM_EINVAL, "mutual must be 0 or 1");
   if (p->precision != TFMM_FP64 && p->precision != TFMM_MIXED)
     return fail(c, TFMM_EINVAL, "precision must be TFMM_FP64 or TFMM_MIXED");
   return TFMM_OK;
@@ -480,7 +480,8 @@ int do_dtt(tfmm_ctx *c) {
                           c->front_cap, cnt + 16 + st + 1, dp<int2>(c->m2l_pairs), c->m2l_cap,
                           dp<int2>(c->p2p_pairs), c->p2p_cap, cnt, dp<double4>(c->dtt_geo),
                           dp<int2>(c->dtt_topo), k.depth, k.width, dp<int32_t>(c->slot_of),
-                          dp<uint32_t>(c->g_used), dp<uint32_t>(c->g_used) + nslots, s));
+                          dp<uint32_t>(c->g_used), dp<uint32_t>(c->g_used) + nslots,
+                          c->prm.mutual, s));
       std::swap(c->front_a, c->front_b);
     }
     TRY(hipMemcpyAsync(c->h_u32, cnt, (size_t)cwords * 4, hipMemcpyDeviceToHost, s));
@@ -507,6 +508,7 @@ int do_dtt(tfmm_ctx *c) {
       c->n_m2l_exec = nm;
       c->n_p2p = np;
       c->n_m2l_all = c->h_u32[3];
+      c->n_p2p_calls_leaf = c->h_u32[4];
       return TFMM_OK;
     }
     auto grow = [](uint32_t cap, uint32_t seen) {
@@ -752,13 +754,20 @@ int traverse_given(tfmm_ctx *c, const int32_t *m2l, int64_t nm, const int32_t *p
   TRY(hipMemcpyAsync(nb.data(), k.nb, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
   TRY(hipMemcpyAsync(child.data(), k.child, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
   TRY(hipStreamSynchronize(s));
+  // Counters as the reference counts calls: with mutual the lists hold each
+  // unordered pair in both directions, one call of the reference's functors
+  // (the entry with i <= j counts; a leaf with itself visits the j <= i body
+  // pairs).
+  const bool mutual = c->prm.mutual != 0;
   std::vector<int2> me, pe;
   me.reserve((size_t)nm);
   pe.reserve((size_t)np);
+  uint64_t m2l_calls = 0, p2p_calls = 0;
   for (int64_t e = 0; e < nm; e++) {
     const int32_t i = m2l[2 * e], j = m2l[2 * e + 1];
     if (i < 0 || i >= nc || j < 0 || j >= nc)
       return fail(c, TFMM_EINVAL, "tfmm_traverse_lists: M2L cell index out of range");
+    if (!mutual || i <= j) m2l_calls++;
     if (nb[i] > 0 && nb[j] > 0) me.push_back(make_int2(i, j));
   }
   unsigned long long pairs = 0, self = 0;
@@ -770,7 +779,12 @@ int traverse_given(tfmm_ctx *c, const int32_t *m2l, int64_t nm, const int32_t *p
       return fail(c, TFMM_EINVAL, "tfmm_traverse_lists: P2P pair of a cell that is not a leaf");
     if (nb[i] == 0 || nb[j] == 0) continue;
     pe.push_back(make_int2(i, j));
-    pairs += (unsigned long long)nb[i] * (unsigned long long)nb[j];
+    if (mutual && i > j) continue;
+    p2p_calls++;
+    if (mutual && i == j)
+      pairs += (unsigned long long)nb[i] * (unsigned long long)(nb[i] + 1) / 2;
+    else
+      pairs += (unsigned long long)nb[i] * (unsigned long long)nb[j];
     if (i == j) self += (unsigned long long)nb[i];
   }
   TRY(hipEventRecord(c->ev[EV_DTT0], s));
@@ -800,8 +814,9 @@ int traverse_given(tfmm_ctx *c, const int32_t *m2l, int64_t nm, const int32_t *p
   // the uploads read host vectors that end with this call
   TRY(hipStreamSynchronize(s));
   c->n_m2l_exec = (uint32_t)me.size();
-  c->n_m2l_all = (uint64_t)nm;
+  c->n_m2l_all = m2l_calls;
   c->n_p2p = (uint32_t)pe.size();
+  c->n_p2p_calls_leaf = p2p_calls;
   return run_lists(c);
 }
 
@@ -1505,7 +1520,7 @@ int tfmm_get_stats(tfmm_ctx *c, tfmm_stats_t *o) {
   if (c->stage >= 3) {
     o->n_m2l = c->n_m2l_all;
     o->n_m2l_exec = c->n_m2l_exec;
-    o->n_p2p_leafpairs = c->n_p2p;
+    o->n_p2p_leafpairs = c->n_p2p_calls_leaf;
     o->n_p2p_calls = c->h_u64[0];
     o->n_p2p_self = c->h_u64[1];
     o->n_m2l_groups = c->n_groups;

@@ -96,7 +96,7 @@ hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin
                            uint32_t out_cap, uint32_t *nout, int2 *m2l, uint32_t m2l_cap,
                            int2 *p2p, uint32_t p2p_cap, uint32_t *cnt, const double4 *geo,
                            const int2 *topo, const int32_t *depth, const double *width,
-                           int32_t *m2l_slot, uint32_t *used, uint32_t *n_outside,
+                           int32_t *m2l_slot, uint32_t *used, uint32_t *n_outside, int mutual,
                            hipStream_t s);
 // group-table slots of M2L pairs given from outside the traversal
 hipError_t launch_pair_slots(const int2 *pairs, uint32_t n, const double4 *geo,

@@ -1,25 +1,33 @@
 // traverse.hip -- dual tree traversal on the GPU.
 //
 // The reference recursion FMM_DTT / tapas_splitCell
-// (sample/exafmm-dev-13274dd4ac68/examples/tapas_exafmm.cxx:209-286, with
-// mutual = 0) visits a deterministic set of (target, source) cell pairs:
+// (sample/exafmm-dev-13274dd4ac68/examples/tapas_exafmm.cxx:209-286) visits a
+// deterministic set of (target, source) cell pairs:
 //   R2 = |centre_i - centre_j|^2  (TP/geometry.h:80-83, TP/vec.h:52-58)
 //   R  = (max_d width_d / 2 * 1.00001) / theta
 //   R2 > (Ri + Rj)^2          -> M2L(Ci, Cj)
 //   both leaves               -> P2P(Ci, Cj)   (skipped if either is empty)
 //   Cj leaf                   -> (children of Ci) x Cj
 //   Ci leaf                   -> Ci x (children of Cj)
+//   mutual and Ci == Cj       -> (children a) x (children b), a <= b
 //   Ri >= Rj                  -> (children of Ci) x Cj
 //   else                      -> Ci x (children of Cj)
-// Here that recursion is expanded breadth first: one launch per frontier,
-// each pair either emits an M2L entry, a P2P entry or its 8 successors.
-// Counters: *nout the next frontier; cnt (uint32 words) 1 executed M2L pairs,
-// 2 P2P pairs, 3 all M2L pairs, words 8-11 two uint64 sums: body-pair
-// interactions of the P2P pairs and those of the self pairs.  The input size
-// is read on the device (*nin_dev, at most nin_max), so consecutive steps
-// need no host round trip.  The
-// MAC is evaluated in FP64 without contraction, in the reference's operation
-// order, so every decision is bit-identical to the reference's.
+// With mutual (the reference's --mutual 1) every pair is an unordered pair
+// whose M2L and P2P act on both cells (LaplaceSphericalCPU_tapas.h:208-273,
+// LaplaceP2PCPU_tapas.h:66-71; the a <= b product is TP/hot/mapper.h:29-121):
+// such a pair is emitted into the lists in both directions, (i <- j) and
+// (j <- i), so the target-major M2L and P2P kernels serve it unchanged.
+// Here the recursion is expanded breadth first: one launch per frontier,
+// each pair either emits list entries or its 8 (36) successors.
+// Counters: *nout the next frontier; cnt (uint32 words) 1 M2L list entries
+// executed, 2 P2P list entries, 3 M2L calls (numM2L: a mutual pair once,
+// empty cells included), 4 P2P leaf-pair calls, words 8-11 two uint64 sums:
+// body pairs the reference's P2P functor visits (numP2P: i x j, or the
+// j <= i pairs of a leaf with itself under mutual) and the r = 0 pairs of a
+// body with itself among them.  The input size is read on the device
+// (*nin_dev, at most nin_max), so consecutive steps need no host round trip.
+// The MAC is evaluated in FP64 without contraction, in the reference's
+// operation order, so every decision is bit-identical to the reference's.
 //
 // The step reads each cell through one 32-byte record (centre, MAC radius R)
 // and one 8-byte record (first child, body count), built once per traversal,
@@ -50,7 +58,7 @@ __global__ void k_dtt_cells(Cells c, int32_t ncells, double theta, double4 *geo,
 }
 
 // Kind of a frontier pair after the MAC: what the step emits for it.
-enum : int { DK_NONE = 0, DK_M2L, DK_FAR_EMPTY, DK_P2P, DK_SPLIT_I, DK_SPLIT_J };
+enum : int { DK_NONE = 0, DK_M2L, DK_FAR_EMPTY, DK_P2P, DK_SPLIT_I, DK_SPLIT_J, DK_SPLIT_BOTH };
 
 __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const uint32_t *nin_dev,
                                                   uint32_t nin_max, int2 *out, uint32_t out_cap,
@@ -59,7 +67,7 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
                                                   const double4 *geo, const int2 *topo,
                                                   const int32_t *depth, const double *width,
                                                   int32_t *m2l_slot, uint32_t *used,
-                                                  uint32_t *n_outside) {
+                                                  uint32_t *n_outside, int mutual) {
 #pragma clang fp contract(off)
   const uint32_t nin = nin_dev ? *nin_dev : nin_max;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -70,15 +78,17 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
   // per pair; 1 executed M2L; 2 P2P; 3 all M2L, counted only) per row and
   // wave; one global atomic per list per block reserves the block's ranges
   // (same-address atomics are what bound this kernel); pass 2 writes.
-  __shared__ uint32_t wc[DTT_PPT][DTT_WAVES][4];
+  __shared__ uint32_t wc[DTT_PPT][DTT_WAVES][5];
   __shared__ unsigned long long ws[DTT_WAVES][2];
-  int32_t pi[DTT_PPT], pj[DTT_PPT], kind[DTT_PPT], fch[DTT_PPT], grp[DTT_PPT];
+  int32_t pi[DTT_PPT], pj[DTT_PPT], kind[DTT_PPT], fch[DTT_PPT], grp[DTT_PPT], grp2[DTT_PPT];
+  bool dbl[DTT_PPT];
   unsigned long long v = 0, self = 0;
 #pragma unroll
   for (int r = 0; r < DTT_PPT; r++) {
     const uint32_t t = (blockIdx.x * DTT_PPT + r) * DTT_THREADS + threadIdx.x;
     const bool valid = t < nin && t < nin_max;
-    int32_t i = 0, j = 0, k = DK_NONE, f = 0, g = -1;
+    int32_t i = 0, j = 0, k = DK_NONE, f = 0, g = -1, g2 = -1;
+    bool two = false;  // the pair's entries go into the lists in both directions
     if (valid) {
       const int2 pr = in[t];
       i = pr.x;
@@ -97,19 +107,35 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
       const bool nonempty = ti.y > 0 && tj.y > 0;
       if (R2 > (Ri + Rj) * (Ri + Rj)) {
         k = nonempty ? DK_M2L : DK_FAR_EMPTY;
+        two = mutual != 0;
         if (nonempty) {
           // the pair's M2L offset group (same arithmetic on the same centres
           // as a separate pass over the list would do)
           const double ci[3] = {gi.x, gi.y, gi.z}, cj[3] = {gj.x, gj.y, gj.z};
           g = m2lg::pair_slot(depth[i], depth[j], width[i], width[j], ci, cj);
+          if (two) g2 = m2lg::pair_slot(depth[j], depth[i], width[j], width[i], cj, ci);
         }
       } else if (leaf_i && leaf_j) {
         if (nonempty) {
           k = DK_P2P;
-          v += (unsigned long long)ti.y * (unsigned long long)tj.y;
+          two = mutual != 0 && i != j;
+          if (mutual != 0 && i == j) {
+            v += (unsigned long long)ti.y * (unsigned long long)(ti.y + 1) / 2;
+          } else {
+            v += (unsigned long long)ti.y * (unsigned long long)tj.y;
+          }
           if (i == j) self += (unsigned long long)ti.y;
         }
-      } else if (leaf_j || (!leaf_i && Ri >= Rj)) {
+      } else if (leaf_j) {
+        k = DK_SPLIT_I;
+        f = ti.x;
+      } else if (leaf_i) {
+        k = DK_SPLIT_J;
+        f = tj.x;
+      } else if (mutual != 0 && i == j) {
+        k = DK_SPLIT_BOTH;
+        f = ti.x;
+      } else if (Ri >= Rj) {
         k = DK_SPLIT_I;
         f = ti.x;
       } else {
@@ -122,15 +148,19 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
     kind[r] = k;
     fch[r] = f;
     grp[r] = g;
+    grp2[r] = g2;
+    dbl[r] = two;
     const uint64_t m0 = __ballot(k == DK_SPLIT_I || k == DK_SPLIT_J);
-    const uint64_t m1 = __ballot(k == DK_M2L);
-    const uint64_t m2 = __ballot(k == DK_P2P);
+    const uint64_t m0b = __ballot(k == DK_SPLIT_BOTH);
+    const uint64_t m1 = __ballot(k == DK_M2L), m1d = __ballot(k == DK_M2L && two);
+    const uint64_t m2 = __ballot(k == DK_P2P), m2d = __ballot(k == DK_P2P && two);
     const uint64_t m3 = __ballot(k == DK_M2L || k == DK_FAR_EMPTY);
     if (lane == 0) {
-      wc[r][wid][0] = (uint32_t)__popcll(m0) * 8u;
-      wc[r][wid][1] = (uint32_t)__popcll(m1);
-      wc[r][wid][2] = (uint32_t)__popcll(m2);
+      wc[r][wid][0] = (uint32_t)__popcll(m0) * 8u + (uint32_t)__popcll(m0b) * 36u;
+      wc[r][wid][1] = (uint32_t)(__popcll(m1) + __popcll(m1d));
+      wc[r][wid][2] = (uint32_t)(__popcll(m2) + __popcll(m2d));
       wc[r][wid][3] = (uint32_t)__popcll(m3);
+      wc[r][wid][4] = (uint32_t)__popcll(m2);
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -144,7 +174,7 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
   __syncthreads();
   // one thread per list: the block's total, its global range, and the
   // (row, wave) counts turned into absolute starts
-  if (threadIdx.x < 4) {
+  if (threadIdx.x < 5) {
     const int q = threadIdx.x;
     uint32_t tot = 0;
     for (int r = 0; r < DTT_PPT; r++)
@@ -172,25 +202,39 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
   for (int r = 0; r < DTT_PPT; r++) {
     const int k = kind[r];
     const int32_t i = pi[r], j = pj[r];
+    const bool two = dbl[r];
     const bool split = k == DK_SPLIT_I || k == DK_SPLIT_J;
-    const uint64_t m0 = __ballot(split);
-    const uint64_t m1 = __ballot(k == DK_M2L);
-    const uint64_t m2 = __ballot(k == DK_P2P);
+    const uint64_t m0 = __ballot(split), m0b = __ballot(k == DK_SPLIT_BOTH);
+    const uint64_t m1 = __ballot(k == DK_M2L), m1d = __ballot(k == DK_M2L && two);
+    const uint64_t m2 = __ballot(k == DK_P2P), m2d = __ballot(k == DK_P2P && two);
     if (k == DK_M2L) {
-      const uint32_t sl = wc[r][wid][1] + (uint32_t)__popcll(m1 & lt);
-      if (sl < m2l_cap) {
-        m2l[sl] = make_int2(i, j);
-        m2l_slot[sl] = grp[r];
-        if (grp[r] >= 0) used[grp[r]] = 1u;
-        else outside++;
+      const uint32_t sl =
+          wc[r][wid][1] + (uint32_t)(__popcll(m1 & lt) + __popcll(m1d & lt));
+      for (int e = 0; e < (two ? 2 : 1); e++) {
+        const int32_t ge = e ? grp2[r] : grp[r];
+        if (sl + e < m2l_cap) {
+          m2l[sl + e] = e ? make_int2(j, i) : make_int2(i, j);
+          m2l_slot[sl + e] = ge;
+          if (ge >= 0) used[ge] = 1u;
+          else outside++;
+        }
       }
     } else if (k == DK_P2P) {
-      const uint32_t sl = wc[r][wid][2] + (uint32_t)__popcll(m2 & lt);
+      const uint32_t sl =
+          wc[r][wid][2] + (uint32_t)(__popcll(m2 & lt) + __popcll(m2d & lt));
       if (sl < p2p_cap) p2p[sl] = make_int2(i, j);
-    } else if (split) {
-      const uint32_t sl = wc[r][wid][0] + (uint32_t)__popcll(m0 & lt) * 8u;
-      if (sl + 8 <= out_cap) {
-        const int32_t f = fch[r];
+      if (two && sl + 1 < p2p_cap) p2p[sl + 1] = make_int2(j, i);
+    } else if (split || k == DK_SPLIT_BOTH) {
+      const uint32_t sl =
+          wc[r][wid][0] + (uint32_t)__popcll(m0 & lt) * 8u + (uint32_t)__popcll(m0b & lt) * 36u;
+      const int32_t f = fch[r];
+      if (k == DK_SPLIT_BOTH) {
+        if (sl + 36 <= out_cap) {
+          uint32_t o = sl;
+          for (int a = 0; a < 8; a++)
+            for (int b = a; b < 8; b++) out[o++] = make_int2(f + a, f + b);
+        }
+      } else if (sl + 8 <= out_cap) {
         if (k == DK_SPLIT_I) {
           for (int c = 0; c < 8; c++) out[sl + c] = make_int2(f + c, j);
         } else {
@@ -339,13 +383,13 @@ hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin
                            uint32_t out_cap, uint32_t *nout, int2 *m2l, uint32_t m2l_cap,
                            int2 *p2p, uint32_t p2p_cap, uint32_t *cnt, const double4 *geo,
                            const int2 *topo, const int32_t *depth, const double *width,
-                           int32_t *m2l_slot, uint32_t *used, uint32_t *n_outside,
+                           int32_t *m2l_slot, uint32_t *used, uint32_t *n_outside, int mutual,
                            hipStream_t s) {
   if (nin_max == 0) return hipSuccess;
   const uint32_t per = DTT_THREADS * DTT_PPT;
   k_dtt_step<<<(nin_max + per - 1) / per, DTT_THREADS, 0, s>>>(
       in, nin_dev, nin_max, out, out_cap, nout, m2l, m2l_cap, p2p, p2p_cap, cnt, geo, topo, depth,
-      width, m2l_slot, used, n_outside);
+      width, m2l_slot, used, n_outside, mutual);
   return hipGetLastError();
 }
 

@@ -6,7 +6,7 @@
 // 100-target direct-sum check.
 //
 // Options (as the reference's EX/include/args.h): -n bodies, -c ncrit,
-// -t theta, -m mutual (must be 0), -d cube|plummer|sphere, -r repeat,
+// -t theta, -m mutual (0 or 1), -d cube|plummer|sphere, -r repeat,
 // -k check (0/1), plus -p fp64|mixed.  Bodies follow the reference's
 // dataset semantics (EX/include/dataset.h:131-271): cube U[-pi,pi)^3 or the
 // Plummer sphere scaled by 3pi/16, charges drand48() - 0.5, srand48(0).
@@ -86,9 +86,8 @@ int main(int argc, char **argv) {
       default: fprintf(stderr, "unknown option\n"); return 2;
     }
   }
-  if (mutual) {
-    // the reference's CUDA path rejects mutual interaction the same way
-    fprintf(stderr, "TapasFMM: [Error] Mutual is not supported for the GPU implementation.\n");
+  if (mutual != 0 && mutual != 1) {
+    fprintf(stderr, "TapasFMM: [Error] mutual must be 0 or 1.\n");
     return 1;
   }
   const Precision precision = prec == "mixed" ? Precision::Mixed : Precision::FP64;
@@ -96,6 +95,7 @@ int main(int argc, char **argv) {
   for (int t = 0; t < repeat; t++) {
     const double t0 = now();
     Cell *root = TapasFMM::Partition(bodies.data(), (index_t)bodies.size(), ncrit, 0, precision);
+    root->SetOptMutual(mutual);
     const double t1 = now();
     TapasFMM::Map(FMM_Upward(), *root, theta);
     const double t2 = now();

@@ -185,6 +185,7 @@ struct Tree {
   std::vector<CellAttr> cell_attr;   // host copy, filled on demand
   bool attr_valid = false;
   int stage = 0;
+  bool opt_mutual = false;  // root->SetOptMutual (the reference's --mutual)
   Recorder rec;
 
   ~Tree() {
@@ -258,6 +259,14 @@ class Cell {
   inline BodyRange bodies() const;
   bool operator==(const Cell &o) const { return t_ == o.t_ && idx_ == o.idx_; }
   bool operator!=(const Cell &o) const { return !(*this == o); }
+  // mutual interaction of the whole tree (TP/hot.h:407-409): a product of a
+  // cell's subcells with themselves then visits the pairs a <= b only
+  bool GetOptMutual() const { return t_->opt_mutual; }
+  bool SetOptMutual(bool b) {
+    const bool old = t_->opt_mutual;
+    t_->opt_mutual = b;
+    return old;
+  }
 
   // body i of a leaf (sorted order)
   Body &body(index_t i) { return t_->bodies[t_->beg[idx_] + i]; }
@@ -367,12 +376,17 @@ inline void L2L(Cell &c) {
   if (c.IsRoot()) throw Error("L2L is called for the root");
   detail::recorder(c, "L2L").l2l[c.index()] = 1;
 }
+// with mutual the call also gives Cj the M2L of Ci's multipole
+// (LaplaceSphericalCPU_tapas.h:225): recorded as the reversed pair too
 inline void M2L(Cell &Ci, Cell &Cj, vec3 Xperiodic, bool mutual) {
   detail::no_periodic(Xperiodic);
-  if (mutual) throw Error("mutual interaction is not supported (mutual must be 0)");
   detail::Recorder &r = detail::recorder(Ci, "M2L");
   r.m2l.push_back((int32_t)Ci.index());
   r.m2l.push_back((int32_t)Cj.index());
+  if (mutual) {
+    r.m2l.push_back((int32_t)Cj.index());
+    r.m2l.push_back((int32_t)Ci.index());
+  }
 }
 // L2P of one body (the reference passes the leaf as a pointer); mapped over a
 // leaf's bodies it records the leaf's L2P once
@@ -466,8 +480,8 @@ struct TapasFMM {
   static void Map(FMM_DTT, CellProduct prod, int mutual, int /*nspawn*/, real_t theta) {
     if (!(prod.t1.IsRoot() && prod.t2.IsRoot()))
       throw Error("the GPU mapper runs FMM_DTT from Product(root, root)");
-    if (mutual) throw Error("mutual interaction is not supported (mutual must be 0)");
     auto t = prod.t1.tree();
+    t->prm.mutual = mutual ? 1 : 0;
     set_theta(t, theta);
     t->check(tfmm_traverse(t->ctx), "tfmm_traverse");
     t->attr_valid = false;
@@ -556,26 +570,36 @@ struct TapasFMM {
       f(ci, cj, args...);
     }
   }
+  // with the tree's mutual option, the subcells of one cell with themselves
+  // give the pairs a <= b only (TP/hot/mapper.h:29-121)
   template <class F, class... Args>
   static void Map(F f, ProductRange<SubCellRange, SubCellRange> p, Args... args) {
+    const bool tri = p.t1.cell().GetOptMutual() && p.t1.cell() == p.t2.cell();
     for (int i = 0; i < p.t1.size(); i++)
-      for (int j = 0; j < p.t2.size(); j++) {
+      for (int j = tri ? i : 0; j < p.t2.size(); j++) {
         Cell ci = p.t1[i], cj = p.t2[j];
         f(ci, cj, args...);
       }
   }
 
   // Body products (the P2P of two leaves): recorded for the GPU P2P kernel;
-  // an empty product is skipped, as TP/hot.h:1367 skips it
+  // an empty product is skipped, as TP/hot.h:1367 skips it.  With mutual the
+  // source leaf gets the pair's interaction too (LaplaceP2PCPU_tapas.h:66-71):
+  // recorded as the reversed leaf pair (a leaf with itself needs no second
+  // entry: the kernel gives every body of it all its pairs).
   template <class... Args>
   static void Map(P2P, ProductRange<BodyRange, BodyRange> p, vec3 Xperiodic, int mutual,
                   Args...) {
     detail::no_periodic(Xperiodic);
-    if (mutual) throw Error("mutual interaction is not supported (mutual must be 0)");
     if (p.t1.size() == 0 || p.t2.size() == 0) return;
     detail::Recorder &r = detail::recorder(p.t1.cell(), "Map(P2P, Product(bodies, bodies))");
-    r.p2p.push_back((int32_t)p.t1.cell().index());
-    r.p2p.push_back((int32_t)p.t2.cell().index());
+    const int32_t i = (int32_t)p.t1.cell().index(), j = (int32_t)p.t2.cell().index();
+    r.p2p.push_back(i);
+    r.p2p.push_back(j);
+    if (mutual && i != j) {
+      r.p2p.push_back(j);
+      r.p2p.push_back(i);
+    }
   }
 
   static tfmm_stats_t Stats(Cell &root) {
@@ -643,6 +667,9 @@ struct TapasFMM {
       downward(t);
     } else if (trav) {
       if (t->stage < 2) throw Error("traversal before the upward pass");
+      // the counters of a mutual traversal count each recorded pair once
+      t->prm.mutual = t->opt_mutual ? 1 : 0;
+      t->check(tfmm_set_params(t->ctx, &t->prm), "tfmm_set_params");
       t->check(tfmm_traverse_lists(t->ctx, r.m2l.data(), (int64_t)(r.m2l.size() / 2),
                                    r.p2p.data(), (int64_t)(r.p2p.size() / 2)),
                "tfmm_traverse_lists");

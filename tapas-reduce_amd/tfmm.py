@@ -334,15 +334,18 @@ class Context:
         return M, Lx
 
     def lists(self):
-        st = self.stats()
-        nc = st["ncells"]
+        """Executed M2L and P2P list entries, CSR over target cells (a mutual
+        pair is an entry in each direction)."""
+        nc = self.stats()["ncells"]
         mo = np.zeros(nc + 1, np.int32)
-        ms = np.zeros(max(1, st["n_m2l_exec"]), np.int32)
         po = np.zeros(nc + 1, np.int32)
-        ps = np.zeros(max(1, st["n_p2p_leafpairs"]), np.int32)
+        self._check(lib().tfmm_get_lists(self._h, _ptr(mo), None, _ptr(po), None),
+                    "tfmm_get_lists")
+        ms = np.zeros(max(1, int(mo[-1])), np.int32)
+        ps = np.zeros(max(1, int(po[-1])), np.int32)
         self._check(lib().tfmm_get_lists(self._h, _ptr(mo), _ptr(ms), _ptr(po), _ptr(ps)),
                     "tfmm_get_lists")
-        return (mo, ms[:st["n_m2l_exec"]]), (po, ps[:st["n_p2p_leafpairs"]])
+        return (mo, ms[:int(mo[-1])]), (po, ps[:int(po[-1])])
 
     def tree_features(self):
         f = np.zeros(16, np.float32)

@@ -34,7 +34,8 @@ def digest(*arrays):
 
 
 def cases():
-    """(name, bodies, ncrit, theta, full) for every fixture."""
+    """(name, bodies, ncrit, theta, full[, mutual, expansions]) for every
+    fixture."""
     out = []
     out.append(("c0_cube16k", datasets.cube(16384, seed=0), 64, 0.4, True))
     # (a root that is a leaf, n <= ncrit, makes the reference abort in its
@@ -59,26 +60,45 @@ def cases():
     y[0] = 1.4
     out.append(("bbox_quirk500", (x, y, z, q), 16, 0.4, True))
     out.append(("c1_cube1m", datasets.cube(1 << 20, seed=0), 64, 0.4, False))
+    # mutual interaction (the reference CLI's default, --mutual 1): each
+    # unordered cell pair once, M2L and P2P applied to both sides
+    out.append(("c0_cube16k_mutual", datasets.cube(16384, seed=0), 64, 0.4, True, 1, False))
+    out.append(("cube_n200_c16_mutual", datasets.cube(200, seed=4), 16, 0.4, True, 1, True))
+    out.append(("plummer3000_c32_t05_mutual", datasets.plummer(3000, seed=5, charges="uniform"),
+                32, 0.5, True, 1, True))
+    out.append(("two_plummer4000_c16_mutual", datasets.two_plummer(4000, seed=7), 16, 0.3, True,
+                1, False))
+    x, y, z, q = datasets.cube(400, seed=8)
+    x[200:230] = x[0:30]
+    y[200:230] = y[0:30]
+    z[200:230] = z[0:30]
+    out.append(("cube_dups400_c8_mutual", (x, y, z, q), 8, 0.4, True, 1, True))
     return out
 
 
 def main(only=None):
-    for name, (x, y, z, q), ncrit, theta, full in cases():
+    for case in cases():
+        name, (x, y, z, q), ncrit, theta, full = case[:5]
+        mutual = case[5] if len(case) > 5 else 0
+        expansions = case[6] if len(case) > 6 else full
         if only and name not in only:
             continue
-        r = oracle.run_ref(x, y, z, q, ncrit, theta, count=True, dump_expansions=full)
+        r = oracle.run_ref(x, y, z, q, ncrit, theta, count=True, dump_expansions=expansions,
+                           mutual=mutual)
         cells = r["cells"]
         d = dict(
             ncrit=np.int64(ncrit), theta=np.float64(theta),
             rmin=r["rmin"], rmax=r["rmax"], max_depth=np.int64(r["max_depth"]),
             n_m2l=np.uint64(r["m2l"]), n_p2p_calls=np.uint64(r["p2p_calls"]),
-            ncells=np.int64(len(cells)),
+            ncells=np.int64(len(cells)), mutual=np.int64(mutual),
         )
         if full:
             d.update(x=x, y=y, z=z, q=q)
             d.update(keys=r["keys"], ibody=r["ibody"], cell_key=cells["key"], cell_nb=cells["nb"],
                      cell_bid=cells["bid"], cell_leaf=cells["leaf"], cell_center=cells["center"],
-                     cell_width=cells["width"], trg=r["trg"], M=r["M"], L=r["L"])
+                     cell_width=cells["width"], trg=r["trg"])
+            if expansions:
+                d.update(M=r["M"], L=r["L"])
         else:
             # large case: the bodies are regenerated from the seeded generator
             # by the tests; only their digest is kept

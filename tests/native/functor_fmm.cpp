@@ -4,7 +4,7 @@
 // MI355X library through tapas_gpu/tapas_exafmm_gpu.h, against the library's
 // own GPU traversal on the same bodies.
 //
-//   functor_fmm <in> <out> <ncrit> <theta>
+//   functor_fmm <in> <out> <ncrit> <theta> [mutual]
 // in: int64 n, then n doubles each of x, y, z, q.  out: n * 4 doubles, the
 // fields of the functor run in input order.  Prints the counts of both runs;
 // exit status 0 when both runs have the same interaction lists and fields
@@ -99,8 +99,10 @@ static void collect(TapasFMM::Cell *root, int64_t n, Run &r) {
   const size_t nc = (size_t)r.st.ncells;
   r.m2l_off.resize(nc + 1);
   r.p2p_off.resize(nc + 1);
-  r.m2l_src.resize((size_t)r.st.n_m2l_exec);
-  r.p2p_src.resize((size_t)r.st.n_p2p_leafpairs);
+  t->check(tfmm_get_lists(t->ctx, r.m2l_off.data(), nullptr, r.p2p_off.data(), nullptr),
+           "tfmm_get_lists");
+  r.m2l_src.resize((size_t)r.m2l_off[nc] + 1);
+  r.p2p_src.resize((size_t)r.p2p_off[nc] + 1);
   t->check(tfmm_get_lists(t->ctx, r.m2l_off.data(), r.m2l_src.data(), r.p2p_off.data(),
                           r.p2p_src.data()),
            "tfmm_get_lists");
@@ -112,12 +114,13 @@ static void collect(TapasFMM::Cell *root, int64_t n, Run &r) {
 }
 
 int main(int argc, char **argv) {
-  if (argc != 5) {
-    fprintf(stderr, "usage: functor_fmm <in> <out> <ncrit> <theta>\n");
+  if (argc != 5 && argc != 6) {
+    fprintf(stderr, "usage: functor_fmm <in> <out> <ncrit> <theta> [mutual]\n");
     return 2;
   }
   const int ncrit = atoi(argv[3]);
   const double theta = atof(argv[4]);
+  const int mutual = argc > 5 ? atoi(argv[5]) : 0;
   FILE *fi = fopen(argv[1], "rb");
   if (!fi) return 2;
   int64_t n = 0;
@@ -135,15 +138,17 @@ int main(int argc, char **argv) {
   try {
     // the library's own traversal
     TapasFMM::Cell *r0 = TapasFMM::Partition(bodies.data(), n, ncrit);
+    r0->SetOptMutual(mutual);
     TapasFMM::Map(tapas_gpu::FMM_Upward(), *r0, theta);
-    TapasFMM::Map(tapas_gpu::FMM_DTT(), tapas::Product(*r0, *r0), 0, 1000, theta);
+    TapasFMM::Map(tapas_gpu::FMM_DTT(), tapas::Product(*r0, *r0), mutual, 1000, theta);
     TapasFMM::Map(tapas_gpu::FMM_Downward(), *r0);
     collect(r0, n, gpu);
     delete r0;
     // the functors above, recorded on host cells and run by the GPU kernels
     TapasFMM::Cell *r1 = TapasFMM::Partition(bodies.data(), n, ncrit);
+    r1->SetOptMutual(mutual);
     TapasFMM::Map(FMM_Upward(), *r1, theta);
-    TapasFMM::Map(FMM_DTT(), tapas::Product(*r1, *r1), 0, 1000, theta);
+    TapasFMM::Map(FMM_DTT(), tapas::Product(*r1, *r1), mutual, 1000, theta);
     TapasFMM::Map(FMM_Downward(), *r1);
     collect(r1, n, fun);
     delete r1;

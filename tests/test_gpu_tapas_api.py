@@ -16,18 +16,19 @@ def run(args, cwd):
     return subprocess.run([EXE] + args, cwd=cwd, capture_output=True, text=True, timeout=120)
 
 
-def test_driver_rejects_mutual(tmp_path):
-    """mutual = 1 is refused before any device work, as on the reference's
-    CUDA path (tapas_exafmm.cxx:507-515)."""
-    r = run(["-n", "1000", "-m", "1"], str(tmp_path))
-    assert r.returncode == 1 and "Mutual is not supported" in r.stderr
+def test_driver_rejects_bad_mutual(tmp_path):
+    """mutual must be 0 or 1; refused before any device work."""
+    r = run(["-n", "1000", "-m", "2"], str(tmp_path))
+    assert r.returncode == 1 and "mutual must be 0 or 1" in r.stderr
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dist,prec", [("cube", "fp64"), ("plummer", "fp64"), ("cube", "mixed")])
-def test_driver_runs_reference_sequence(tmp_path, dist, prec):
-    print("driver", dist, prec, flush=True)
-    r = run(["-n", "30000", "-c", "64", "-t", "0.4", "-m", "0", "-d", dist, "-p", prec],
+@pytest.mark.parametrize("dist,prec,mutual", [("cube", "fp64", 0), ("plummer", "fp64", 0),
+                                              ("cube", "mixed", 0), ("plummer", "fp64", 1),
+                                              ("cube", "mixed", 1)])
+def test_driver_runs_reference_sequence(tmp_path, dist, prec, mutual):
+    print("driver", dist, prec, "mutual", mutual, flush=True)
+    r = run(["-n", "30000", "-c", "64", "-t", "0.4", "-m", str(mutual), "-d", dist, "-p", prec],
             str(tmp_path))
     assert r.returncode == 0, r.stderr
     errs = [float(v) for v in re.findall(r"Rel\. L2 Error \((?:pot|acc)\)\s*:\s*(\S+)", r.stdout)]
@@ -52,11 +53,13 @@ def test_reference_shaped_functors_compile_against_the_surface():
 
 
 @pytest.mark.gpu
-def test_gpu_functor_traversal_matches_gpu_traversal(tmp_path):
+@pytest.mark.parametrize("mutual", [0, 1])
+def test_gpu_functor_traversal_matches_gpu_traversal(tmp_path, mutual):
     """The functors run on host cells with their M2L and P2P recorded and
     launched on the GPU: the same interaction lists (CSR, bit for bit) and
     fields as the library's own GPU traversal (checked in the program), and
-    within 1e-10 of the CPU oracle, on the 16,384-body uniform cube."""
+    within 1e-10 of the CPU oracle, on the 16,384-body uniform cube, without
+    and with mutual interaction."""
     import numpy as np
     import datasets
     import oracle
@@ -66,13 +69,13 @@ def test_gpu_functor_traversal_matches_gpu_traversal(tmp_path):
         f.write(np.int64(len(x)).tobytes())
         for a in (x, y, z, q):
             f.write(np.ascontiguousarray(a, np.float64).tobytes())
-    print("functor_fmm n=%d ncrit 64 theta 0.4" % len(x), flush=True)
-    r = subprocess.run([FUNCTOR_EXE, inp, out, "64", "0.4"], capture_output=True, text=True,
-                       timeout=120)
+    print("functor_fmm n=%d ncrit 64 theta 0.4 mutual %d" % (len(x), mutual), flush=True)
+    r = subprocess.run([FUNCTOR_EXE, inp, out, "64", "0.4", str(mutual)], capture_output=True,
+                       text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     fields = np.fromfile(out, np.float64).reshape(-1, 4)
-    o = oracle.Run(x, y, z, q, 64, 0.4)
+    o = oracle.Run(x, y, z, q, 64, 0.4, mutual=mutual)
     c = o.counts()
     assert "m2l %d p2p leaf pairs %d" % (c["m2l"], c["p2p_leafpairs"]) in r.stdout
     ep, ea = oracle.rel_l2(fields, o.fields())
