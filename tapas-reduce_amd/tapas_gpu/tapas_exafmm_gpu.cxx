@@ -2,8 +2,8 @@
 // against the same user surface as the reference application
 // (sample/exafmm-dev-13274dd4ac68/examples/tapas_exafmm.cxx:448-712):
 // Partition, Map(FMM_Upward), Map(FMM_DTT, Product(root, root)),
-// Map(FMM_Downward), CopyBackResult, the main.csv timing report and the
-// 100-target direct-sum check.
+// Map(FMM_Downward), CopyBackResult, the main.csv timing report, the
+// 100-target direct-sum check and root->Report()'s CSV files.
 //
 // Options (as the reference's EX/include/args.h): -n bodies, -c ncrit,
 // -t theta, -m mutual (0 or 1), -d cube|plummer|sphere, -r repeat,
@@ -110,10 +110,17 @@ int main(int argc, char **argv) {
       for (int d = 0; d < 4; d++) out[i].TRG[d] = root->local_body_attr(i)[d];
     }
     const tfmm_stats_t st = TapasFMM::Stats(*root);
-    std::ofstream csv("main.csv");
-    csv << "Rank,total,upward,traverse,downward,tree,numP2P,numM2L\n";
-    csv << 0 << "," << (t4 - t0) << "," << (t2 - t1) << "," << (t3 - t2) << "," << (t4 - t3)
-        << "," << (t1 - t0) << "," << st.n_p2p_calls << "," << st.n_m2l << "\n";
+    // the reference's main.csv (tapas_exafmm.cxx:651-677, built with COUNT)
+    tapas_gpu::util::RankCSV csv({"total", "upward", "traverse", "downward", "tree", "numP2P",
+                                  "numM2L"});
+    csv.At("total", t4 - t0);
+    csv.At("upward", t2 - t1);
+    csv.At("traverse", t3 - t2);
+    csv.At("downward", t4 - t3);
+    csv.At("tree", t1 - t0);
+    csv.At("numP2P", (uint64_t)st.n_p2p_calls);
+    csv.At("numM2L", (uint64_t)st.n_m2l);
+    csv.Dump(tapas_gpu::util::ReportPath("main"));
     printf("bodies %d cells %lld depth %d  tree %.4f upward %.4f traverse %.4f downward %.4f"
            " total %.4f s (device: tree %.2f up %.2f trav %.2f down %.2f ms)\n",
            n, (long long)st.ncells, st.max_depth, t1 - t0, t2 - t1, t3 - t2, t4 - t3, t4 - t0,
@@ -139,6 +146,7 @@ int main(int argc, char **argv) {
       printf("Rel. L2 Error (pot)            : %e\n", std::sqrt(pd / pn));
       printf("Rel. L2 Error (acc)            : %e\n", std::sqrt(ad / an));
     }
+    root->Report();  // the reference's report files (tapas_exafmm.cxx:700)
     delete root;
   }
   return 0;

@@ -43,11 +43,16 @@
 #ifndef TAPAS_GPU_TAPAS_FMM_H_
 #define TAPAS_GPU_TAPAS_FMM_H_
 
+#include <unistd.h>  // gethostname
+
 #include <complex>
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <memory>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -216,6 +221,69 @@ struct Tree {
 
 }  // namespace detail
 
+// Report files in the reference's RankCSV format (TP/util.h:114-317): one
+// header line, then one row per rank, every column right-aligned in 14
+// characters (a value that fills its column gets one space after it), doubles
+// in scientific notation.  One GPU is one rank.
+namespace util {
+class RankCSV {
+ public:
+  explicit RankCSV(std::vector<std::string> cols) : cols_() {
+    cols_.push_back("Rank");
+    for (auto &c : cols) cols_.push_back(c);
+    vals_.assign(cols_.size(), std::string("N/A"));
+    At("Rank", 0);
+  }
+  void At(const std::string &col, double v) {
+    std::ostringstream ss;
+    ss << std::scientific << v;
+    set(col, ss.str());
+  }
+  void At(const std::string &col, int64_t v) { set(col, std::to_string(v)); }
+  void At(const std::string &col, uint64_t v) { set(col, std::to_string(v)); }
+  void At(const std::string &col, int v) { set(col, std::to_string(v)); }
+  void At(const std::string &col, const std::string &v) { set(col, v); }
+  std::string Text() const {
+    std::string out;
+    for (auto &c : cols_) out += pad(c);
+    out += "\n";
+    for (size_t i = 0; i < vals_.size(); i++) {
+      out += pad(vals_[i]);
+      if (i + 1 < vals_.size() && vals_[i].size() >= kWidth) out += " ";
+    }
+    out += "\n";
+    return out;
+  }
+  void Dump(const std::string &fname) const {
+    std::ofstream f(fname);
+    if (!f) throw Error("cannot write " + fname);
+    f << Text();
+  }
+
+ private:
+  static const size_t kWidth = 14;
+  static std::string pad(const std::string &v) {
+    return v.size() < kWidth ? std::string(kWidth - v.size(), ' ') + v : v;
+  }
+  void set(const std::string &col, const std::string &v) {
+    for (size_t i = 0; i < cols_.size(); i++)
+      if (cols_[i] == col) {
+        vals_[i] = v;
+        return;
+      }
+    throw Error("unknown report column " + col);
+  }
+  std::vector<std::string> cols_, vals_;
+};
+
+// TAPAS_REPORT_PREFIX + name + TAPAS_REPORT_SUFFIX + ".csv", as the
+// reference names its report files (TP/hot/report.h:48-58)
+inline std::string ReportPath(const std::string &name) {
+  const char *p = getenv("TAPAS_REPORT_PREFIX"), *s = getenv("TAPAS_REPORT_SUFFIX");
+  return std::string(p ? p : "") + name + std::string(s ? s : "") + ".csv";
+}
+}  // namespace util
+
 class SubCellRange;
 class BodyRange;
 
@@ -285,12 +353,68 @@ class Cell {
   std::shared_ptr<detail::Tree> tree() const { return t_; }
   index_t index() const { return idx_; }
 
+  // root->Report(): the reference's report files (TP/hot/report.h:39-152)
+  // for the last evaluation.  One GPU is one rank: no sampling, body
+  // exchange, global tree or LET, whose columns are 0; the tree build (key
+  // sort included) is the local growth; map2's net traversal and device call
+  // are the GPU traversal with its M2L and P2P kernels.
+  inline void Report() const;
+
   Cell(std::shared_ptr<detail::Tree> t, index_t i) : t_(std::move(t)), idx_(i) {}
 
  private:
   std::shared_ptr<detail::Tree> t_;
   index_t idx_;
 };
+
+inline void Cell::Report() const {
+  tfmm_stats_t st;
+  t_->check(tfmm_get_stats(t_->ctx, &st), "tfmm_get_stats");
+  char host[200] = {0};
+  gethostname(host, sizeof(host) - 1);
+  util::RankCSV hn({"Hostname"});
+  hn.At("Hostname", std::string(host));
+  hn.Dump(util::ReportPath("hostnames"));
+  {  // misc (rank 0): no sampling on one GPU
+    util::RankCSV misc({"SamplingRate", "NB"});
+    misc.At("SamplingRate", 0.0);
+    misc.At("NB", (int64_t)t_->n);
+    // the reference's misc.csv is a CSV, not a RankCSV: no Rank column
+    std::string txt = misc.Text();
+    const size_t cut = std::string("          Rank").size();
+    std::string out;
+    std::istringstream lines(txt);
+    for (std::string l; std::getline(lines, l);) out += l.substr(cut) + "\n";
+    std::ofstream f(util::ReportPath("misc"));
+    if (!f) throw Error("cannot write misc report");
+    f << out;
+  }
+  util::RankCSV lb({"NumBodies1", "NumBodies2", "NumLeaves", "NumCells"});
+  lb.At("NumBodies1", (int64_t)t_->n);
+  lb.At("NumBodies2", (int64_t)t_->n);
+  lb.At("NumLeaves", (int64_t)st.nleaves);
+  lb.At("NumCells", (int64_t)st.ncells);
+  lb.Dump(util::ReportPath("load_balancing"));
+  util::RankCSV tc({"all", "sample", "exchange", "grow-local", "grow-global"});
+  tc.At("all", st.ms_tree * 1e-3);
+  tc.At("sample", 0.0);
+  tc.At("exchange", 0.0);
+  tc.At("grow-local", st.ms_tree * 1e-3);
+  tc.At("grow-global", 0.0);
+  tc.Dump(util::ReportPath("tree_construction"));
+  util::RankCSV let({"LET-All", "LET-Trav-main", "LET-Trav-sub", "LET-Req-all", "LET-Req-comm",
+                     "LET-Res-all", "LET-Res-acomm", "LET-Res-bcomm", "LET-Reg"});
+  for (const char *c : {"LET-All", "LET-Trav-main", "LET-Trav-sub", "LET-Req-all", "LET-Req-comm",
+                        "LET-Res-all", "LET-Res-acomm", "LET-Res-bcomm", "LET-Reg"})
+    let.At(c, 0.0);
+  let.Dump(util::ReportPath("let_construction"));
+  util::RankCSV m2({"all", "let", "net_traverse", "device_call"});
+  m2.At("all", st.ms_traverse * 1e-3);
+  m2.At("let", 0.0);
+  m2.At("net_traverse", st.ms_traverse * 1e-3);
+  m2.At("device_call", (st.ms_m2l + st.ms_p2p) * 1e-3);
+  m2.Dump(util::ReportPath("map2"));
+}
 
 // Cell::subcells(): the 8 children of an internal cell
 class SubCellRange {

@@ -16,6 +16,13 @@ def run(args, cwd):
     return subprocess.run([EXE] + args, cwd=cwd, capture_output=True, text=True, timeout=120)
 
 
+def read_rank_csv(path):
+    """A one-rank file in the reference's RankCSV format: header and row of
+    14-character right-aligned columns."""
+    head, row = open(path).read().splitlines()[:2]
+    return dict(zip(head.split(), row.split()))
+
+
 def test_driver_rejects_bad_mutual(tmp_path):
     """mutual must be 0 or 1; refused before any device work."""
     r = run(["-n", "1000", "-m", "2"], str(tmp_path))
@@ -33,9 +40,24 @@ def test_driver_runs_reference_sequence(tmp_path, dist, prec, mutual):
     assert r.returncode == 0, r.stderr
     errs = [float(v) for v in re.findall(r"Rel\. L2 Error \((?:pot|acc)\)\s*:\s*(\S+)", r.stdout)]
     assert len(errs) == 2 and max(errs) < 1e-3, r.stdout
-    lines = open(os.path.join(str(tmp_path), "main.csv")).read().splitlines()
-    assert lines[0] == "Rank,total,upward,traverse,downward,tree,numP2P,numM2L"
-    assert int(lines[1].split(",")[-1]) > 0
+    main = read_rank_csv(os.path.join(str(tmp_path), "main.csv"))
+    assert list(main) == ["Rank", "total", "upward", "traverse", "downward", "tree", "numP2P",
+                          "numM2L"]
+    assert int(main["numM2L"]) > 0 and float(main["total"]) > 0
+    # root->Report(): the reference's report files (TP/hot/report.h:39-152)
+    lb = read_rank_csv(os.path.join(str(tmp_path), "load_balancing.csv"))
+    assert int(lb["NumBodies1"]) == 30000 and int(lb["NumCells"]) > int(lb["NumLeaves"]) > 0
+    tc = read_rank_csv(os.path.join(str(tmp_path), "tree_construction.csv"))
+    assert list(tc) == ["Rank", "all", "sample", "exchange", "grow-local", "grow-global"]
+    assert float(tc["all"]) > 0
+    m2 = read_rank_csv(os.path.join(str(tmp_path), "map2.csv"))
+    assert list(m2) == ["Rank", "all", "let", "net_traverse", "device_call"]
+    assert float(m2["net_traverse"]) > 0
+    let = read_rank_csv(os.path.join(str(tmp_path), "let_construction.csv"))
+    assert len(let) == 10 and float(let["LET-All"]) == 0
+    assert read_rank_csv(os.path.join(str(tmp_path), "hostnames.csv"))["Hostname"]
+    misc = open(os.path.join(str(tmp_path), "misc.csv")).read().split("\n")
+    assert misc[0].split() == ["SamplingRate", "NB"] and int(misc[1].split()[1]) == 30000
 
 
 FUNCTOR_SRC = os.path.join(ROOT, "tests", "native", "functor_fmm.cpp")
@@ -81,3 +103,22 @@ def test_gpu_functor_traversal_matches_gpu_traversal(tmp_path, mutual):
     ep, ea = oracle.rel_l2(fields, o.fields())
     print("functor run vs oracle: %.2e %.2e" % (ep, ea))
     assert ep < 1e-10 and ea < 1e-10
+
+
+def test_report_files_use_the_reference_rankcsv_format(tmp_path):
+    """RankCSV lines as the reference writes them (cpp/include/tapas/util.h:
+    114-317): a Rank column first, every column right-aligned in 14
+    characters, doubles in scientific notation, and one space after a value
+    that fills its column (except the last)."""
+    exe = os.path.join(str(tmp_path), "report_format")
+    src = os.path.join(ROOT, "tests", "native", "report_format.cpp")
+    r = subprocess.run(["g++", "-std=c++11", "-O1", "-o", exe, src], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60).stdout
+    assert out.split("\n") == [
+        "          Rank         total        numP2P      Hostname",
+        "             0  1.500000e+00           123a-very-long-hostname",
+        "          Rank LET-Res-acomm   grow-global",
+        "             0  0.000000e+00 -2.500000e-07",
+        ""]
