@@ -226,6 +226,13 @@ int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p
  * (tfmm_evaluate without a graph only) starts the P2P as soon as its list
  * is sorted, beside the M2L batch building. */
 #define TFMM_TUNE_P2P_WAVES 5
+/* TFMM_TUNE_DEVICE_LIMIT: a limit in bytes (0 = none, the default) on the
+ * device memory the library's buffers hold, over all contexts of the
+ * process; a call whose buffers would exceed it fails with TFMM_ENOMEM,
+ * leaving the context usable for a smaller problem or after the limit is
+ * raised.  For deployments that share the GPU, and to test the allocation
+ * failure path. */
+#define TFMM_TUNE_DEVICE_LIMIT 6
 int tfmm_set_tuning(tfmm_ctx *ctx, int knob, double value);
 int tfmm_get_tuning(tfmm_ctx *ctx, int knob, double *value);
 /* Autotuner: evaluates the current bodies with params under every candidate

@@ -122,6 +122,7 @@ struct tfmm_ctx {
 
   // this device's copy of the static scratch of tfmm_direct_noalloc
   void *noalloc = nullptr;
+  DevBuf finite_flag;  // the finiteness check of device-resident bodies
 
   // host staging (pinned)
   uint32_t *h_u32 = nullptr;  // pinned staging, H_U32_WORDS words
@@ -154,16 +155,20 @@ hipError_t grow_copy(DevBuf &b, size_t bytes, size_t used, hipStream_t s) {
   if (bytes <= b.cap) return hipSuccess;
   size_t want = b.cap ? b.cap : 256;
   while (want < bytes) want *= 2;
+  if (!dev_fits(0, want)) return hipErrorOutOfMemory;
   void *p = nullptr;
   hipError_t e = hipMalloc(&p, want);
   if (e != hipSuccess) return e;
+  dev_account(0, want);
   if (b.ptr && used) {
     e = hipMemcpyAsync(p, b.ptr, used, hipMemcpyDeviceToDevice, s);
-    if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return e;
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      if (hipFree(p) == hipSuccess) dev_account(want, 0);
+      return e;
+    }
   }
-  if (b.ptr) (void)hipFree(b.ptr);
+  if (b.ptr && hipFree(b.ptr) == hipSuccess) dev_account(b.cap, 0);
   b.ptr = p;
   b.cap = want;
   return hipSuccess;
@@ -1129,7 +1134,7 @@ void tfmm_destroy(tfmm_ctx *c) {
                     &c->feat_acc, &c->feat_x, &c->feat_out, &c->d_prob_off, &c->bprob,
                     &c->pkey, &c->pkey_alt, &c->ppos, &c->ppos_alt, &c->brec, &c->lev,
                     &c->lev_bsum, &c->lev_status, &c->dtt_geo, &c->dtt_topo, &c->u_cnt,
-                    &c->u_off, &c->units, &c->u_part, &c->p2p_next};
+                    &c->u_off, &c->units, &c->u_part, &c->p2p_next, &c->finite_flag};
   for (DevBuf *b : bufs) dev_release(*b);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   for (int i = 0; i < EV_COUNT; i++)
@@ -1168,6 +1173,11 @@ int tfmm_set_tuning(tfmm_ctx *c, int knob, double value) {
       if (!(value >= 0) || !(value <= 8) || value != std::floor(value))
         return fail(c, TFMM_EINVAL, "P2P waves per SIMD must be an integer in [0, 8]");
       c->p2p_waves = (int)value;
+      return TFMM_OK;
+    case TFMM_TUNE_DEVICE_LIMIT:
+      if (!(value >= 0) || !(value < 9.0e18) || value != std::floor(value))
+        return fail(c, TFMM_EINVAL, "device limit must be a whole number of bytes >= 0");
+      dev_set_limit((size_t)value);
       return TFMM_OK;
     default:
       return fail(c, TFMM_EINVAL, "unknown tuning knob");
@@ -1241,6 +1251,7 @@ int tfmm_get_tuning(tfmm_ctx *c, int knob, double *value) {
     case TFMM_TUNE_P2P_SCHEDULE: *value = c->p2p_schedule; return TFMM_OK;
     case TFMM_TUNE_GRAPH: *value = c->use_graph; return TFMM_OK;
     case TFMM_TUNE_P2P_WAVES: *value = c->p2p_waves; return TFMM_OK;
+    case TFMM_TUNE_DEVICE_LIMIT: *value = (double)dev_limit(); return TFMM_OK;
     default: return fail(c, TFMM_EINVAL, "unknown tuning knob");
   }
 }
@@ -1258,7 +1269,7 @@ namespace {
 int check_desc(tfmm_ctx *c, const void *host, const tfmm_body_desc *d) {
   if (!d || !host) return fail(c, TFMM_EINVAL, "null bodies or descriptor");
   if (d->n < 1 || d->n > INT32_MAX) return fail(c, TFMM_EINVAL, "n must be in [1, 2^31)");
-  if (d->stride < 8 || d->off_x + 24 > d->stride || d->off_q + 8 > d->stride)
+  if (d->stride < 24 || d->off_x > d->stride - 24 || d->off_q > d->stride - 8)
     return fail(c, TFMM_EINVAL, "body descriptor offsets exceed the stride");
   return TFMM_OK;
 }
@@ -1359,6 +1370,19 @@ int tfmm_set_batch(tfmm_ctx *c, int nprob, const tfmm_body_desc *descs,
   return set_problems(c, nprob, off.data());
 }
 
+// device-resident bodies: every coordinate and charge must be finite, as the
+// host path requires (one small reduction and a read-back per set call)
+int check_device_bodies(tfmm_ctx *c, int64_t n, const double *x, const double *y,
+                        const double *z, const double *q) {
+  TRY(dev_reserve(c->finite_flag, 4));
+  TRY(launch_fill_zero(c->finite_flag.ptr, 16, c->stream));
+  TRY(launch_check_finite(x, y, z, q, n, dp<uint32_t>(c->finite_flag), c->stream));
+  TRY(hipMemcpyAsync(c->h_u32, c->finite_flag.ptr, 4, hipMemcpyDeviceToHost, c->stream));
+  TRY(hipStreamSynchronize(c->stream));
+  if (c->h_u32[0]) return fail(c, TFMM_EINVAL, "a body coordinate or charge is not finite");
+  return TFMM_OK;
+}
+
 int tfmm_set_batch_device(tfmm_ctx *c, int nprob, const int64_t *offsets, const double *x,
                           const double *y, const double *z, const double *q) {
   if (!c) return TFMM_EINVAL;
@@ -1371,6 +1395,8 @@ int tfmm_set_batch_device(tfmm_ctx *c, int nprob, const int64_t *offsets, const 
     if (offsets[p + 1] <= offsets[p]) return fail(c, TFMM_EINVAL, "every problem needs a body");
   if (offsets[nprob] > INT32_MAX) return fail(c, TFMM_EINVAL, "total body count must be < 2^31");
   TRY(hipSetDevice(c->device));
+  int rc = check_device_bodies(c, offsets[nprob], x, y, z, q);
+  if (rc) return rc;
   c->n = offsets[nprob];
   c->bx = x;
   c->by = y;
@@ -1398,6 +1424,9 @@ int tfmm_set_bodies_device(tfmm_ctx *c, int64_t n, const double *x, const double
   if (!c) return TFMM_EINVAL;
   if (n < 1 || n > INT32_MAX) return fail(c, TFMM_EINVAL, "n must be in [1, 2^31)");
   if (!x || !y || !z || !q) return fail(c, TFMM_EINVAL, "null body array");
+  TRY(hipSetDevice(c->device));
+  int rc = check_device_bodies(c, n, x, y, z, q);
+  if (rc) return rc;
   c->n = n;
   c->bx = x;
   c->by = y;

@@ -5,6 +5,8 @@
 // (cpp/include/tapas/common.h:248-273, TP/hot/buildtree.h:299-324); the
 // reference sort is unstable, so only the key order is specified and equal
 // keys may come out in any order (this sort keeps input order for them).
+#include <atomic>
+
 #include "tfmm_internal.h"
 
 namespace tfmm {
@@ -223,23 +225,45 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
 
 }  // namespace
 
+// Device bytes held by the library's buffers (all contexts), and the limit
+// TFMM_TUNE_DEVICE_LIMIT sets on them (0: none).
+std::atomic<size_t> g_held{0}, g_limit{0};
+
+bool dev_fits(size_t release, size_t want) {
+  const size_t lim = g_limit.load();
+  return lim == 0 || g_held.load() - release + want <= lim;
+}
+
+void dev_account(size_t freed, size_t allocated) {
+  g_held -= freed;
+  g_held += allocated;
+}
+
+void dev_set_limit(size_t bytes) { g_limit = bytes; }
+size_t dev_limit() { return g_limit.load(); }
+
 hipError_t dev_reserve(DevBuf &b, size_t bytes) {
   if (bytes <= b.cap) return hipSuccess;
   size_t want = b.cap ? b.cap : 256;
   while (want < bytes) want *= 2;
+  if (!dev_fits(b.cap, want)) return hipErrorOutOfMemory;
   if (b.ptr) {
     hipError_t e = hipFree(b.ptr);
     if (e != hipSuccess) return e;
+    dev_account(b.cap, 0);
     b.ptr = nullptr;
     b.cap = 0;
   }
   hipError_t e = hipMalloc(&b.ptr, want);
-  if (e == hipSuccess) b.cap = want;
+  if (e == hipSuccess) {
+    b.cap = want;
+    dev_account(0, want);
+  }
   return e;
 }
 
 void dev_release(DevBuf &b) {
-  if (b.ptr) (void)hipFree(b.ptr);
+  if (b.ptr && hipFree(b.ptr) == hipSuccess) dev_account(b.cap, 0);
   b.ptr = nullptr;
   b.cap = 0;
 }

@@ -18,6 +18,13 @@ struct DevBuf {
 
 hipError_t dev_reserve(DevBuf &b, size_t bytes);
 void dev_release(DevBuf &b);
+// accounting of the device bytes the library holds, against the limit set by
+// TFMM_TUNE_DEVICE_LIMIT (0: none): whether releasing `release` bytes and
+// allocating `want` stays within it, and the bookkeeping of a change
+bool dev_fits(size_t release, size_t want);
+void dev_account(size_t freed, size_t allocated);
+void dev_set_limit(size_t bytes);
+size_t dev_limit();
 
 template <class T>
 inline T *dp(DevBuf &b) { return reinterpret_cast<T *>(b.ptr); }

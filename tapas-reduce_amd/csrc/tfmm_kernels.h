@@ -32,6 +32,9 @@ struct Csr {
 };
 
 // tree.hip
+// *flag |= 1 when any x, y, z or q of the n bodies is not finite
+hipError_t launch_check_finite(const double *x, const double *y, const double *z,
+                               const double *q, int64_t n, uint32_t *flag, hipStream_t s);
 hipError_t launch_bbox(const double *x, const double *y, const double *z, int64_t n,
                        double *partial, int nparts, double *region, hipStream_t s);
 // keys, and the bodies packed as (x, y, z, q) records for the permutation

@@ -14,6 +14,8 @@
 // one scan over the "split" flags, so cells of one level are contiguous and
 // the 8 children of a cell are adjacent, which the level-synchronous upward
 // and downward passes rely on.
+#include <algorithm>
+
 #include "tfmm_device.h"
 #include "tfmm_kernels.h"
 
@@ -384,7 +386,27 @@ __global__ void k_level_children(Cells c, const int32_t *lev, int d, const uint3
   make_child(c, lbeg + i, ch, next_beg + 8 * (int32_t)slot, keys, regions, sprob);
 }
 
+// flag = 1 when a coordinate or charge of the n bodies is not finite (the
+// host path rejects such bodies while unpacking them; device arrays are
+// checked here)
+__global__ void k_check_finite(const double *x, const double *y, const double *z,
+                               const double *q, int64_t n, uint32_t *flag) {
+  bool bad = false;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    bad |= !(isfinite(x[i]) && isfinite(y[i]) && isfinite(z[i]) && isfinite(q[i]));
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
 }  // namespace
+
+hipError_t launch_check_finite(const double *x, const double *y, const double *z,
+                               const double *q, int64_t n, uint32_t *flag, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t blocks = std::min<int64_t>(2048, (n + 255) / 256);
+  k_check_finite<<<(unsigned)blocks, 256, 0, s>>>(x, y, z, q, n, flag);
+  return hipGetLastError();
+}
 
 hipError_t launch_bbox(const double *x, const double *y, const double *z, int64_t n,
                        double *partial, int nparts, double *region, hipStream_t s) {

@@ -22,6 +22,7 @@ TUNE_M2L_XCD_RUN = 2
 TUNE_P2P_SCHEDULE = 3
 TUNE_GRAPH = 4
 TUNE_P2P_WAVES = 5
+TUNE_DEVICE_LIMIT = 6
 DIRECT_NOALLOC_MAX = 65536
 P = 10
 NTERM = 55

@@ -161,3 +161,63 @@ def test_gpu_batch_device_matches_host_copy(ctx):
         ctx.sync()
         for d in ptrs:
             hip.hipFree(d)
+
+
+def test_gpu_body_descriptor_offsets_checked_without_overflow(ctx):
+    """Offsets near SIZE_MAX do not wrap around the stride check: the call is
+    refused before any body record is read."""
+    import ctypes
+    body = np.zeros((4, 12), np.float64)
+    for off_x, off_q in ((2 ** 64 - 8, 24), (0, 2 ** 64 - 4), (80, 24)):
+        d = tfmm.BodyDesc(4, 96, off_x, off_q)
+        rc = tfmm.lib().tfmm_set_bodies(ctx._h, body.ctypes.data_as(ctypes.c_void_p),
+                                         ctypes.byref(d))
+        assert rc == -22, (off_x, off_q, rc)
+
+
+def test_gpu_device_bodies_must_be_finite(ctx):
+    """Device-resident bodies with a NaN coordinate or an infinite charge are
+    refused as the host path refuses them (TFMM_EINVAL)."""
+    import torch
+    x, y, z, q = datasets.cube(5000, seed=3)
+    for arr, val in ((1, float("nan")), (3, float("inf"))):
+        a = [np.array(v) for v in (x, y, z, q)]
+        a[arr][1234] = val
+        t = [torch.from_numpy(v).to("cuda:0") for v in a]
+        torch.cuda.synchronize()
+        with pytest.raises(tfmm.TfmmError, match="not finite"):
+            ctx.set_bodies_device(len(x), *[v.data_ptr() for v in t])
+        with pytest.raises(tfmm.TfmmError, match="not finite"):
+            ctx.set_batch_device(np.array([0, 2000, len(x)], np.int64),
+                                 *[v.data_ptr() for v in t])
+    t = [torch.from_numpy(np.array(v)).to("cuda:0") for v in (x, y, z, q)]
+    torch.cuda.synchronize()
+    ctx.set_bodies_device(len(x), *[v.data_ptr() for v in t])
+    ctx.evaluate(tfmm.make_params(64, 0.4))
+    assert np.isfinite(ctx.fields()).all()
+
+
+def test_gpu_allocation_failure_is_reported_and_recoverable():
+    """A device-memory limit below what an evaluation needs makes it fail
+    with TFMM_ENOMEM (no abort, no partial state used later); with the limit
+    lifted the same context evaluates and matches the oracle."""
+    x, y, z, q = datasets.plummer(40000, seed=12, charges="uniform")
+    c = tfmm.Context(0)
+    try:
+        c.set_bodies(x, y, z, q)
+        # the limit counts every context's buffers: 1 MiB is below what the
+        # process already holds, so the evaluation's first new buffer fails
+        c.set_tuning(tfmm.TUNE_DEVICE_LIMIT, 1 << 20)
+        assert c.get_tuning(tfmm.TUNE_DEVICE_LIMIT) == 1 << 20
+        with pytest.raises(tfmm.TfmmError) as e:
+            c.evaluate(tfmm.make_params(32, 0.4))
+        print("with a 1 MiB limit:", e.value)
+        assert "(-12)" in str(e.value) or "out of memory" in str(e.value).lower()
+        c.set_tuning(tfmm.TUNE_DEVICE_LIMIT, 0)
+        c.evaluate(tfmm.make_params(32, 0.4))
+        o = oracle.Run(x, y, z, q, 32, 0.4)
+        ep, ea = oracle.rel_l2(c.fields(), o.fields())
+        assert ep < 1e-10 and ea < 1e-10
+    finally:
+        c.set_tuning(tfmm.TUNE_DEVICE_LIMIT, 0)
+        c.close()
