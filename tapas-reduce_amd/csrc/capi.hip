@@ -89,7 +89,7 @@ struct tfmm_ctx {
   uint32_t dtt_steps = 0;
 
   // batched M2L (offset groups)
-  DevBuf g_used, g_gid, g_y64, g_y32, g_dl, g_lt, g_cnt;
+  DevBuf g_used, g_gid, g_y64, g_y32, g_af, g_dl, g_lt, g_cnt;
   DevBuf gkeys, gvals, gkeys_alt, gvals_alt, fkeys, fvals, fkeys_alt, fvals_alt;
   // it_flag: position in segment per grouped pair; it_iflag/it_iexcl: item starts per block
   DevBuf it_flag, it_iflag, it_iexcl, items, cols, tile_off, mq, codes;
@@ -648,6 +648,12 @@ int build_m2l_batches(tfmm_ctx *c) {
     TRY(e);
     c->codes_ready = true;
   }
+  // every group's A operands in the tile kernel's order
+  const bool mixed = c->prm.precision == TFMM_MIXED;
+  TRY(dev_reserve(c->g_af, m2l_afrag_bytes(ng, mixed)));
+  TRY(launch_m2l_group_afrag(mixed ? (const void *)c->g_y32.ptr : (const void *)c->g_y64.ptr,
+                             ng, dp<uint32_t>(c->codes), dp<int8_t>(c->g_dl), c->g_af.ptr,
+                             mixed, s));
   return TFMM_OK;
 }
 
@@ -717,8 +723,7 @@ int run_lists(tfmm_ctx *c) {
                        dp<uint32_t>(c->u_off), ntiles, c->unit_chunk, c->m2l_xcd_run,
                        c->units_bound,
                        dp<uint2>(c->cols),
-                       mixed ? (const void *)c->g_y32.ptr : (const void *)c->g_y64.ptr,
-                       c->mq.ptr, dp<uint32_t>(c->codes), k, (int32_t)c->ncells,
+                       c->g_af.ptr, c->mq.ptr, k, (int32_t)c->ncells,
                        dp<double>(c->L), dp<double>(c->u_part), mixed, s));
   TRY(hipEventRecord(c->ev[EV_M2LT], s));
   if (c->n_fallback) {
@@ -971,8 +976,7 @@ int fused_numeric(tfmm_ctx *c) {
   TRY(launch_m2l_tiles(dp<int32_t>(c->tile_off), dp<int2>(c->units), dp<uint32_t>(c->u_cnt),
                        dp<uint32_t>(c->u_off), ntiles, c->unit_chunk, c->m2l_xcd_run,
                        c->units_bound, dp<uint2>(c->cols),
-                       mixed ? (const void *)c->g_y32.ptr : (const void *)c->g_y64.ptr,
-                       c->mq.ptr, dp<uint32_t>(c->codes), k, (int32_t)c->ncells,
+                       c->g_af.ptr, c->mq.ptr, k, (int32_t)c->ncells,
                        dp<double>(c->L), dp<double>(c->u_part), mixed, s));
   TRY(rec(c, EV_M2LT, s));
   if (c->n_fallback) {
@@ -1126,7 +1130,7 @@ void tfmm_destroy(tfmm_ctx *c) {
                     &c->scan_tmp, &c->M, &c->L, &c->front_a, &c->front_b, &c->m2l_pairs,
                     &c->p2p_pairs, &c->cnt, &c->lkeys, &c->lkeys_alt, &c->lidx, &c->lidx_alt,
                     &c->m2l_off, &c->m2l_src, &c->p2p_off, &c->p2p_src, &c->acc,
-                    &c->trg, &c->out, &c->g_used, &c->g_gid, &c->g_y64, &c->g_y32, &c->g_dl,
+                    &c->trg, &c->out, &c->g_used, &c->g_gid, &c->g_y64, &c->g_y32, &c->g_af, &c->g_dl,
                     &c->g_lt, &c->g_cnt, &c->gkeys, &c->gvals, &c->gkeys_alt, &c->gvals_alt,
                     &c->fkeys, &c->fvals, &c->fkeys_alt, &c->fvals_alt, &c->it_flag,
                     &c->it_iflag, &c->it_iexcl, &c->items,

@@ -38,8 +38,9 @@
 //
 // Per-group factors: the level factor 2^(n dl), n the source degree and
 // dl = l_t - l_s, is split as 2^(p dl) 2^(-j dl) with p = j + n, the first
-// folded into the group's harmonics (index p) and the second applied to the
-// output row (degree j); both are exact powers of two.
+// folded into the group's harmonics (index p) and the second into the rows
+// (degree j) of the group's A operands, which are formed once per group
+// before the tile kernel; both are exact powers of two.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -60,7 +61,6 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 using m2lc::BLK_CB;
 using m2lc::BLK_RB;
 using m2lc::NBLK;
-using m2lc::NMULT;
 using m2lc::NR;
 constexpr int NCODE = NBLK * 4 * 64;
 constexpr int SRC_BITS = 25;  // source cell bits of a grouped pair's value
@@ -408,33 +408,40 @@ __device__ inline void load_b(T (&b)[7][4], const T *mq) {
   }
 }
 
-// A entry of code cw: sum of two elements of the wave's operand table
-template <class T>
-__device__ inline T a_value(const T *tbl, uint32_t cw) {
-  const char *base = reinterpret_cast<const char *>(tbl);
-  return *reinterpret_cast<const T *>(base + (cw & 0xffffu)) +
-         *reinterpret_cast<const T *>(base + (cw >> 16));
+// A operands of every group, in the order the tile kernel's lanes read them:
+// Af[g][AF_SLOT(w, e)][lane][kk] is the entry of block WAVE_BLK[w][e] at
+// k-step kk that lane supplies (a 16-byte vector per lane and block in FP32,
+// 32 in FP64), so a wave reads its operands of a new group with one
+// contiguous, conflict-free load per block.  Slots number the (wave, block)
+// pairs wave by wave: slot = sum of WAVE_NBLK[0 .. w-1] + e.
+constexpr int AF_PER_GROUP = NBLK * 64 * 4;  // elements per group
+__host__ __device__ constexpr int af_slot(int w, int e) {
+  int s = e;
+  for (int i = 0; i < w; i++) s += WAVE_NBLK[i];
+  return s;
 }
 
-// the operand table of one item: harmonics at multiples +1, -1, +2, -2
 template <class T>
-__device__ inline void write_table(T *tbl, T y0, T y1, int lane) {
-  tbl[lane] = y0;
-  tbl[NR + lane] = -y0;
-  tbl[2 * NR + lane] = y0 + y0;
-  tbl[3 * NR + lane] = -(y0 + y0);
-  if (lane < NR - 64) {
-    tbl[64 + lane] = y1;
-    tbl[NR + 64 + lane] = -y1;
-    tbl[2 * NR + 64 + lane] = y1 + y1;
-    tbl[3 * NR + 64 + lane] = -(y1 + y1);
-  }
-}
+struct Vec4;
+template <>
+struct Vec4<float> {
+  typedef f4 V;
+};
+template <>
+struct Vec4<double> {
+  typedef d4 V;
+};
 
+// A operands of wave W for the group whose fragments start at af
 template <class T, int W>
-__device__ inline void read_a(T (&a)[WAVE_NBLK[W] * 4], const T *tbl, const uint32_t *code) {
+__device__ inline void load_a(T (&a)[WAVE_NBLK[W] * 4], const T *af, int lane) {
+  typedef typename Vec4<T>::V V;
 #pragma unroll
-  for (int x = 0; x < WAVE_NBLK[W] * 4; x++) a[x] = a_value(tbl, code[x]);
+  for (int e = 0; e < WAVE_NBLK[W]; e++) {
+    const V v = *reinterpret_cast<const V *>(af + (af_slot(W, e) * 64 + lane) * 4);
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) a[e * 4 + kk] = v[kk];
+  }
 }
 
 // Pipeline state of one wave at item it: everything was loaded during item
@@ -443,29 +450,24 @@ __device__ inline void read_a(T (&a)[WAVE_NBLK[W] * 4], const T *tbl, const uint
 template <class T, int W>
 struct WaveState {
   uint2 c_cur;  // columns of it (sources of its B operands were c_b of it - 1)
-  uint2 c_b;    // columns of it + 1 (sources for the next B)
-  uint2 c_y;    // columns of it + 2 (group for the next harmonics)
+  uint2 c_b;    // columns of it + 1 (sources and group of the next operands)
   uint2 c_st;   // columns of it - 1 (target slots and level difference of acc)
   T b[7][4];    // B operands of it
-  T y0, y1;     // harmonics of it + 1
-  T a[WAVE_NBLK[W] * 4];  // A operands of it
   typename Mfma<T>::V acc[WAVE_NRB[W]];  // products of it - 1
 };
 
 // add the products of one item (columns c) into the tile's expansions
 template <class T, int W>
 __device__ inline void store_item(const typename Mfma<T>::V (&acc)[WAVE_NRB[W]], uint2 c,
-                                  const int (&jd)[WAVE_NRB[W]][4], T (*Lt)[NR + 1],
-                                  T (*Lt0b)[17], int q) {
+                                  T (*Lt)[NR + 1], T (*Lt0b)[17], int q) {
   if (!(c.y & 128u)) return;
   const uint32_t ts = c.y & 127u;
-  const int dl = (int)((c.y >> COL_DL) & 7u) - DL_MAX;
 #pragma unroll
   for (int f = 0; f < WAVE_NRB[W]; f++) {
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const int r = Mfma<T>::row(q, i);
-      const T v = ldexp(acc[f][i], -jd[f][i] * dl);
+      const T v = acc[f][i];
       if (W == 1 && WAVE_RBS[W][f] == 0) {
         Lt0b[ts][r] += v;
       } else {
@@ -477,21 +479,18 @@ __device__ inline void store_item(const typename Mfma<T>::V (&acc)[WAVE_NRB[W]],
 }
 
 // One item of wave W with the state of it in cur; fills nxt for it + 1.
-// Order: the item's MFMAs are issued first; the loads, operand table and A
-// reads for the next item and the store of the previous item's products
-// are independent of them and overlap their execution.  Loads issued here
-// are consumed by the next call.  The state is double buffered by
-// unrolling the caller's loop by two (a register rotation would wait for
-// the loads just issued).
+// Order: the item's MFMAs are issued first; the loads of the next item's
+// operands and the store of the previous item's products are independent of
+// them and overlap their execution.  Loads issued here are consumed by the
+// next call.  The state is double buffered by unrolling the caller's loop by
+// two (a register rotation would wait for the loads just issued).
 template <class T, int W>
-__device__ inline void wave_item(WaveState<T, W> &cur, WaveState<T, W> &nxt, int32_t it,
-                                 int32_t ib, int32_t last, const uint2 *cols, const T *Yg,
-                                 const T *Mq, const uint32_t *code,
-                                 const int (&jd)[WAVE_NRB[W]][4], T (*Lt)[NR + 1],
-                                 T (*Lt0b)[17], T *tbl, int lane) {
+__device__ inline void wave_item(WaveState<T, W> &cur, WaveState<T, W> &nxt,
+                                 T (&a)[WAVE_NBLK[W] * 4], int32_t it, int32_t ib, int32_t last,
+                                 const uint2 *cols, const T *Af, const T *Mq, T (*Lt)[NR + 1],
+                                 T (*Lt0b)[17], int lane) {
   constexpr int NB = WAVE_NBLK[W], NF = WAVE_NRB[W];
   const int q = lane >> 4, j = lane & 15;
-  const bool hi = lane < NR - 64;
   // the matrix work of it
 #pragma unroll
   for (int f = 0; f < NF; f++) nxt.acc[f] = (typename Mfma<T>::V){0, 0, 0, 0};
@@ -499,30 +498,22 @@ __device__ inline void wave_item(WaveState<T, W> &cur, WaveState<T, W> &nxt, int
   for (int kk = 0; kk < 4; kk++) {
 #pragma unroll
     for (int e = 0; e < NB; e++)
-      nxt.acc[WAVE_F[W][e]] = Mfma<T>::step(cur.a[e * 4 + kk], cur.b[BLK_CB[WAVE_BLK[W][e]]][kk],
+      nxt.acc[WAVE_F[W][e]] = Mfma<T>::step(a[e * 4 + kk], cur.b[BLK_CB[WAVE_BLK[W][e]]][kk],
                                             nxt.acc[WAVE_F[W][e]]);
   }
   nxt.c_st = cur.c_cur;
   // loads for the next items (indices clamped: loads stay in range)
   nxt.c_cur = cur.c_b;
   nxt.c_b = cols[(int64_t)min(it + 2, last) * 16 + j];
-  nxt.c_y = cols[(int64_t)min(it + 3, last) * 16 + j];
   load_b<T, W>(nxt.b, Mq + (int64_t)cur.c_b.x * NR + q * 28);
-  nxt.y0 = Yg[(int64_t)(cur.c_y.y >> COL_G) * NR + lane];
-  nxt.y1 = hi ? Yg[(int64_t)(cur.c_y.y >> COL_G) * NR + 64 + lane] : (T)0;
-  // operand table and A operands of it + 1 (shared by consecutive items of
-  // one group)
-  if ((cur.c_b.y >> COL_G) != (cur.c_cur.y >> COL_G)) {
-    __builtin_amdgcn_wave_barrier();
-    write_table<T>(tbl, cur.y0, cur.y1, lane);
-    __builtin_amdgcn_wave_barrier();
-    read_a<T, W>(nxt.a, tbl, code);
-  } else {
-#pragma unroll
-    for (int x = 0; x < NB * 4; x++) nxt.a[x] = cur.a[x];
-  }
+  // A operands of it + 1, loaded over those of it (the MFMAs above have
+  // read them) when the group changes: consecutive items of one group share
+  // them, and every column of an item carries its group, so the branch is
+  // uniform
+  const uint32_t gn = cur.c_b.y >> COL_G;
+  if (gn != (cur.c_cur.y >> COL_G)) load_a<T, W>(a, Af + (int64_t)gn * AF_PER_GROUP, lane);
   // the products of it - 1
-  store_item<T, W>(cur.acc, cur.c_st, jd, Lt, Lt0b, q);
+  store_item<T, W>(cur.acc, cur.c_st, Lt, Lt0b, q);
   // keep the 4 waves within a few items of each other: they read the same
   // source multipoles, which then come from L2 once instead of 4 times
   if (((it - ib) & (WAVE_SYNC - 1)) == WAVE_SYNC - 1) __syncthreads();
@@ -530,51 +521,29 @@ __device__ inline void wave_item(WaveState<T, W> &cur, WaveState<T, W> &nxt, int
 
 // The item loop of wave W over the tile's items [ib, ie).
 template <class T, int W>
-__device__ inline void run_wave(const uint2 *cols, int32_t ib, int32_t ie, const T *Yg,
-                                const T *Mq, const uint32_t *codes,
-                                T (*Lt)[NR + 1], T (*Lt0b)[17], T *tbl, int lane) {
-  constexpr int NB = WAVE_NBLK[W], NF = WAVE_NRB[W];
+__device__ inline void run_wave(const uint2 *cols, int32_t ib, int32_t ie, const T *Af,
+                                const T *Mq, T (*Lt)[NR + 1], T (*Lt0b)[17], int lane) {
+  constexpr int NF = WAVE_NRB[W];
   if (ib >= ie) return;
-  uint32_t code[NB * 4];
-#pragma unroll
-  for (int e = 0; e < NB; e++)
-#pragma unroll
-    for (int kk = 0; kk < 4; kk++) code[e * 4 + kk] = codes[(WAVE_BLK[W][e] * 4 + kk) * 64 + lane];
   const int q = lane >> 4, j = lane & 15;
-  int jd[NF][4];  // degree of each output row this lane holds
-#pragma unroll
-  for (int f = 0; f < NF; f++)
-#pragma unroll
-    for (int i = 0; i < 4; i++) jd[f][i] = row_degree(WAVE_RBS[W][f] * 16 + Mfma<T>::row(q, i));
-  const bool hi = lane < NR - 64;
   const int32_t last = ie - 1;
 
   WaveState<T, W> s0, s1;
   s0.c_cur = cols[(int64_t)ib * 16 + j];
   s0.c_b = cols[(int64_t)min(ib + 1, last) * 16 + j];
-  s0.c_y = cols[(int64_t)min(ib + 2, last) * 16 + j];
   s0.c_st = make_uint2(0u, 0u);  // no previous item
 #pragma unroll
   for (int f = 0; f < NF; f++) s0.acc[f] = (typename Mfma<T>::V){0, 0, 0, 0};
   load_b<T, W>(s0.b, Mq + (int64_t)s0.c_cur.x * NR + q * 28);
-  {
-    const uint32_t g = s0.c_cur.y >> COL_G;
-    const T y0 = Yg[(int64_t)g * NR + lane];
-    const T y1 = hi ? Yg[(int64_t)g * NR + 64 + lane] : (T)0;
-    write_table<T>(tbl, y0, y1, lane);
-  }
-  s0.y0 = Yg[(int64_t)(s0.c_b.y >> COL_G) * NR + lane];
-  s0.y1 = hi ? Yg[(int64_t)(s0.c_b.y >> COL_G) * NR + 64 + lane] : (T)0;
-  __builtin_amdgcn_wave_barrier();
-  read_a<T, W>(s0.a, tbl, code);
+  T a[WAVE_NBLK[W] * 4];  // A operands of the current item
+  load_a<T, W>(a, Af + (int64_t)(s0.c_cur.y >> COL_G) * AF_PER_GROUP, lane);
   for (int32_t it = ib; it < ie; it += 2) {
-    wave_item<T, W>(s0, s1, it, ib, last, cols, Yg, Mq, code, jd, Lt, Lt0b, tbl, lane);
-    if (it + 1 < ie)
-      wave_item<T, W>(s1, s0, it + 1, ib, last, cols, Yg, Mq, code, jd, Lt, Lt0b, tbl, lane);
+    wave_item<T, W>(s0, s1, a, it, ib, last, cols, Af, Mq, Lt, Lt0b, lane);
+    if (it + 1 < ie) wave_item<T, W>(s1, s0, a, it + 1, ib, last, cols, Af, Mq, Lt, Lt0b, lane);
   }
   // the products of the last item
-  if ((ie - ib) & 1) store_item<T, W>(s1.acc, s1.c_st, jd, Lt, Lt0b, q);
-  else store_item<T, W>(s0.acc, s0.c_st, jd, Lt, Lt0b, q);
+  if ((ie - ib) & 1) store_item<T, W>(s1.acc, s1.c_st, Lt, Lt0b, q);
+  else store_item<T, W>(s0.acc, s0.c_st, Lt, Lt0b, q);
 }
 
 // Workgroups are dispatched to the 8 XCDs round robin (workgroup b to XCD
@@ -640,14 +609,51 @@ __global__ void k_unit_combine(const uint32_t *ucnt, const uint32_t *uoff, int32
   }
 }
 
+// A fragments of every group from its harmonics: the entry that lane
+// supplies at (block, k-step) is the sum of two table elements addressed by
+// the operand code (byte offsets into the harmonics at multiples +1, -1, +2,
+// -2); the multiples are exact, so each entry is formed with one rounding.
+template <class T>
+__device__ inline T table_elem(const T *y, uint32_t byte_off) {
+  const uint32_t i = byte_off / (uint32_t)sizeof(T);
+  const uint32_t m = i / NR;
+  const T v = y[i - m * NR];
+  return m == 0 ? v : m == 1 ? -v : m == 2 ? v + v : -(v + v);
+}
+
+template <class T>
+__global__ void k_group_afrag(const T *Yg, uint32_t ng, const uint32_t *codes, const int8_t *gdl,
+                              T *Af) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)ng * (NBLK * 64)) return;
+  const int64_t g = t / (NBLK * 64);
+  const int r = (int)(t - g * (NBLK * 64));
+  const int slot = r >> 6, lane = r & 63;
+  int w = 0, e = slot;
+  while (w < TILE_WAVES - 1 && e >= WAVE_NBLK[w]) {
+    e -= WAVE_NBLK[w];
+    w++;
+  }
+  const int b = WAVE_BLK[w][e];
+  const T *y = Yg + g * NR;
+  // row factor 2^(-j dl) of the entry's row (degree j)
+  const int rexp = -row_degree(BLK_RB[b] * 16 + (lane & 15)) * (int)gdl[g];
+  typename Vec4<T>::V out;
+#pragma unroll
+  for (int kk = 0; kk < 4; kk++) {
+    const uint32_t cw = codes[(b * 4 + kk) * 64 + lane];
+    out[kk] = ldexp(table_elem(y, cw & 0xffffu) + table_elem(y, cw >> 16), rexp);
+  }
+  *reinterpret_cast<typename Vec4<T>::V *>(Af + g * AF_PER_GROUP + (slot * 64 + lane) * 4) = out;
+}
+
 template <class T>
 __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
     const int32_t *tile_off, const int2 *units, const uint32_t *ucnt, const uint32_t *uoff,
-    int32_t ntiles, int32_t chunk, int xcd_run, const uint2 *cols, const T *Yg, const T *Mq,
-    const uint32_t *codes, Cells c, int32_t ncells, double *L, double *part) {
+    int32_t ntiles, int32_t chunk, int xcd_run, const uint2 *cols, const T *Af, const T *Mq,
+    Cells c, int32_t ncells, double *L, double *part) {
   __shared__ T Lt[TILE][NR + 1];
   __shared__ T Lt0b[TILE][17];
-  __shared__ T Tb[TILE_WAVES][NMULT * NR];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int u = tile_of_block(blockIdx.x, gridDim.x, xcd_run);
   if ((uint32_t)u >= uoff[ntiles - 1] + ucnt[ntiles - 1]) return;  // grid is an upper bound
@@ -670,10 +676,10 @@ __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
   const int32_t ie = seg_start((int64_t)tb + (int64_t)(k + 1) * chunk);
   // every wave walks all items of the tile, in order, for its own blocks
   switch (wave) {
-    case 0: run_wave<T, 0>(cols, ib, ie, Yg, Mq, codes, Lt, Lt0b, Tb[0], lane); break;
-    case 1: run_wave<T, 1>(cols, ib, ie, Yg, Mq, codes, Lt, Lt0b, Tb[1], lane); break;
-    case 2: run_wave<T, 2>(cols, ib, ie, Yg, Mq, codes, Lt, Lt0b, Tb[2], lane); break;
-    default: run_wave<T, 3>(cols, ib, ie, Yg, Mq, codes, Lt, Lt0b, Tb[3], lane); break;
+    case 0: run_wave<T, 0>(cols, ib, ie, Af, Mq, Lt, Lt0b, lane); break;
+    case 1: run_wave<T, 1>(cols, ib, ie, Af, Mq, Lt, Lt0b, lane); break;
+    case 2: run_wave<T, 2>(cols, ib, ie, Af, Mq, Lt, Lt0b, lane); break;
+    default: run_wave<T, 3>(cols, ib, ie, Af, Mq, Lt, Lt0b, lane); break;
   }
   __syncthreads();
   for (int e = threadIdx.x; e < TILE * 16; e += 64 * TILE_WAVES) {
@@ -824,21 +830,37 @@ hipError_t launch_m2l_units(const int32_t *tile_off, int32_t ntiles, int32_t chu
   return hipGetLastError();
 }
 
+size_t m2l_afrag_bytes(uint32_t ngroups, bool fp32) {
+  return (size_t)std::max<uint32_t>(ngroups, 1) * AF_PER_GROUP * (fp32 ? 4 : 8);
+}
+
+hipError_t launch_m2l_group_afrag(const void *Yg, uint32_t ngroups, const uint32_t *codes,
+                                  const int8_t *gdl, void *Af, bool fp32, hipStream_t s) {
+  if (ngroups == 0) return hipSuccess;
+  const int64_t n = (int64_t)ngroups * NBLK * 64;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (fp32)
+    k_group_afrag<float><<<nb, 256, 0, s>>>((const float *)Yg, ngroups, codes, gdl, (float *)Af);
+  else
+    k_group_afrag<double><<<nb, 256, 0, s>>>((const double *)Yg, ngroups, codes + NCODE, gdl,
+                                             (double *)Af);
+  return hipGetLastError();
+}
+
 hipError_t launch_m2l_tiles(const int32_t *tile_off, const int2 *units, const uint32_t *ucnt,
                             const uint32_t *uoff, int32_t ntiles, int32_t chunk, int xcd_run,
-                            uint32_t units_bound, const uint2 *cols, const void *Yg,
-                            const void *Mq, const uint32_t *codes, Cells c, int32_t ncells,
-                            double *L, double *part, bool fp32, hipStream_t s) {
+                            uint32_t units_bound, const uint2 *cols, const void *Af,
+                            const void *Mq, Cells c, int32_t ncells, double *L, double *part,
+                            bool fp32, hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
   if (fp32)
     k_m2l_tiles<float><<<units_bound, 64 * TILE_WAVES, 0, s>>>(
-        tile_off, units, ucnt, uoff, ntiles, chunk, xcd_run, cols, (const float *)Yg,
-        (const float *)Mq,
-        codes, c, ncells, L, part);
+        tile_off, units, ucnt, uoff, ntiles, chunk, xcd_run, cols, (const float *)Af,
+        (const float *)Mq, c, ncells, L, part);
   else
     k_m2l_tiles<double><<<units_bound, 64 * TILE_WAVES, 0, s>>>(
-        tile_off, units, ucnt, uoff, ntiles, chunk, xcd_run, cols, (const double *)Yg,
-        (const double *)Mq, codes + NCODE, c, ncells, L, part);
+        tile_off, units, ucnt, uoff, ntiles, chunk, xcd_run, cols, (const double *)Af,
+        (const double *)Mq, c, ncells, L, part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   k_unit_combine<<<ntiles, 256, 0, s>>>(ucnt, uoff, ntiles, part, ncells, L);

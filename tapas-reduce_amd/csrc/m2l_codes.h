@@ -15,11 +15,12 @@
 // each with a coefficient in {-2, ..., 2}; only 18 of the 49 16 x 16 blocks
 // are non-zero.
 //
-// Kernel operand codes: the kernel keeps, per work item, the item's harmonics
-// at four multiples {+1, -1, +2, -2} in LDS (element m * 112 + i), and the
-// A-operand entry that lane l supplies at k-step kk of block b is the sum of
-// two table elements whose byte offsets are packed into one 32-bit code
-// (low 16 bits, high 16 bits).  Absent terms point at element 110, a zero.
+// Operand codes: with a group's harmonics laid out at four multiples
+// {+1, -1, +2, -2} (element m * 112 + i), the A-operand entry that lane l
+// supplies at k-step kk of block b is the sum of two table elements whose
+// byte offsets are packed into one 32-bit code (low 16 bits, high 16 bits).
+// Absent terms point at element 110, a zero.  The batched M2L forms every
+// group's A operands from these codes once, before the tile kernel.
 #pragma once
 #include <stddef.h>
 #include <stdint.h>

@@ -171,11 +171,17 @@ size_t m2l_unit_part_bytes(uint32_t units_bound, int32_t ntiles);
 hipError_t launch_m2l_units(const int32_t *tile_off, int32_t ntiles, int32_t chunk,
                             uint32_t *ucnt, uint32_t *uoff, int2 *units, void *scan_tmp,
                             hipStream_t s);
+// A fragments of every group (the tile kernel's operand order, row factors
+// 2^(-j dl) included) from the group harmonics (FP32 or FP64), the operand
+// codes of both types and the groups' level differences
+size_t m2l_afrag_bytes(uint32_t ngroups, bool fp32);
+hipError_t launch_m2l_group_afrag(const void *Yg, uint32_t ngroups, const uint32_t *codes,
+                                  const int8_t *gdl, void *Af, bool fp32, hipStream_t s);
 hipError_t launch_m2l_tiles(const int32_t *tile_off, const int2 *units, const uint32_t *ucnt,
                             const uint32_t *uoff, int32_t ntiles, int32_t chunk, int xcd_run,
-                            uint32_t units_bound, const uint2 *cols, const void *Yg,
-                            const void *Mq, const uint32_t *codes, Cells c, int32_t ncells,
-                            double *L, double *part, bool fp32, hipStream_t s);
+                            uint32_t units_bound, const uint2 *cols, const void *Af,
+                            const void *Mq, Cells c, int32_t ncells, double *L, double *part,
+                            bool fp32, hipStream_t s);
 
 // p2p.hip (mixed precision)
 hipError_t launch_leaf_relpos(Cells c, int32_t ncells, Bodies b, float4 *rel, hipStream_t s);

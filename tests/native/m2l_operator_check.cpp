@@ -1,8 +1,9 @@
 // CPU check of the batched M2L formulation used by the MFMA tile kernel:
 // operator codes (m2l_codes.h), per-group harmonic scaling with the level
-// factor folded into the harmonics and the output rows, and the M~/L~
-// scalings, applied in FP64 exactly as the kernel applies them (block by
-// block, k-step by k-step, lane by lane), against the oracle's per-pair M2L
+// factor folded into the harmonics (2^(p dl)) and into the rows of the A
+// operands (2^(-j dl)), and the M~/L~ scalings, applied in FP64 exactly as the
+// kernel applies them (A fragment by A fragment: block, k-step, lane), against
+// the oracle's per-pair M2L
 // (orc_m2l, the restatement of LaplaceSphericalCPU_tapas.h:208-273).
 // Prints "max_rel <value>" and exits 0 when the codes build.
 #include <cmath>
@@ -68,21 +69,20 @@ int main(int argc, char **argv) {
     for (int i = 0; i < 110; i++) M[i] = u(rng);
     for (int n = 0; n < P; n++) M[2 * tri(n, 0) + 1] = 0;  // m = 0 terms are real
     for (int i = 0; i < 110; i++) Mt[i] = M[i] / std::pow(sig_s, degree_of(i));
-    // the kernel's block-sparse product, then row factor and unscaling
+    // the kernel's A fragments (row factor 2^(-j dl) included), its
+    // block-sparse product, then unscaling
     double Lt[NR] = {0};
     for (int b = 0; b < NBLK; b++)
       for (int kk = 0; kk < 4; kk++)
         for (int l = 0; l < 64; l++) {
           const uint32_t cw = c8[(size_t)(b * 4 + kk) * 64 + l];
-          const double a = tbl[(cw & 0xffffu) / 8] + tbl[(cw >> 16) / 8];
           const int row = 16 * BLK_RB[b] + l % 16, col = 16 * BLK_CB[b] + 4 * kk + l / 16;
+          const double a = std::ldexp(tbl[(cw & 0xffffu) / 8] + tbl[(cw >> 16) / 8],
+                                      -degree_of(row) * dl);
           Lt[row] += a * Mt[col];
         }
     double L[110];
-    for (int i = 0; i < 110; i++) {
-      const int j = degree_of(i);
-      L[i] = std::ldexp(Lt[i], -j * dl) / std::pow(sig_t, j + 1);
-    }
+    for (int i = 0; i < 110; i++) L[i] = Lt[i] / std::pow(sig_t, degree_of(i) + 1);
     double Lref[110] = {0};
     orc_m2l(Lref, M, dX);
     double num = 0, den = 0;

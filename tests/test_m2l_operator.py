@@ -2,7 +2,8 @@
 
 tests/native/m2l_operator_check.cpp applies the kernel's operand codes
 (tapas-reduce_amd/csrc/m2l_codes.h), its table of harmonic multiples, the
-level factor split between harmonics and output rows, and the M~/L~ scalings
+level factor split between the harmonics and the rows of the A operands, and
+the M~/L~ scalings
 in FP64, block by block and lane by lane as the kernel does, for pairs up to
 three levels apart either way (and pairs on the z axis), and compares the result
 with the oracle's per-pair M2L (LaplaceSphericalCPU_tapas.h:208-273).
