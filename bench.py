@@ -77,7 +77,8 @@ def parse():
     ap.add_argument("--p2p-schedule", type=int, default=None, choices=[0, 1],
                     help="0: P2P beside the M2L batch building, 1: beside the M2L tiles")
     ap.add_argument("--graph", action="store_true",
-                    help="launch the numeric phase of every evaluation as one hipGraph")
+                    help="launch every evaluation as one hipGraph (after a first evaluation "
+                         "that sets its capacities)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256 MiB device buffer three times (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -174,6 +175,28 @@ def time_config(ctx, prm, steps, warmup):
     return elapsed * 1e3 / steps, {k: float(np.mean(v)) for k, v in phase.items()}, st
 
 
+def one_graph(ctx, prm, steps, n):
+    """The same evaluation as one hipGraph launch per step (TFMM_TUNE_GRAPH =
+    2): one ordinary evaluation makes the plan, then `steps` launches are
+    timed; their fields must equal the ordinary evaluation's bit for bit."""
+    import numpy as np
+    import tfmm
+    ref = ctx.fields()
+    was = ctx.get_tuning(tfmm.TUNE_GRAPH)
+    ctx.set_tuning(tfmm.TUNE_GRAPH, 2)
+    try:
+        ctx.evaluate(prm)  # the plan
+        ms, phases, st = time_config(ctx, prm, steps, 1)
+        same = bool(np.array_equal(ctx.fields(), ref))
+    finally:
+        ctx.set_tuning(tfmm.TUNE_GRAPH, was)
+    return {"value": n / (ms * 1e-3), "ms_per_step": ms, "steps": steps,
+            "launches_one_graph": bool(st["one_graph"]), "fields_equal_ordinary": same,
+            "note": "every step one hipGraph launch of the whole evaluation (tree, traversal, "
+                    "lists, M2L batches, numeric phase), sizes read on the device; no host "
+                    "read-back inside a step"}
+
+
 def errors(ctx, targets, direct):
     import numpy as np
     f = ctx.fields()[targets]
@@ -238,7 +261,7 @@ def set_knobs(ctx, args):
     import tfmm
     fixed = {}
     if args.graph:
-        ctx.set_tuning(tfmm.TUNE_GRAPH, 1)
+        ctx.set_tuning(tfmm.TUNE_GRAPH, 2)
     if args.unit_factor is not None:
         ctx.set_tuning(tfmm.TUNE_M2L_UNIT, args.unit_factor)
         fixed["m2l_unit_factor"] = args.unit_factor
@@ -320,6 +343,8 @@ def bench_single(args, dist, n, precision, use_picker, steps, warmup, fp64_steps
                          "against": "FP64 direct sum"},
                "picker": picked, "autotune": tuned, "phases_ms": phases, "counts": counts(st),
                "roofline": kernels(st, phases, precision, workload)}
+        if not args.graph:
+            out["one_graph"] = one_graph(ctx, prm, steps, n)
         if fallback_steps > 0 and picked:
             prm_fb = tfmm.make_params(64, 0.4, precision)
             msf, phf, stf = time_config(ctx, prm_fb, fallback_steps, 1)
@@ -383,6 +408,7 @@ def bench_batch(args, nprob, precision, steps, warmup):
         direct = ctx.direct(targets)  # per problem: a target sees only its own problem
         ms, phases, st = time_config(ctx, prm, steps, warmup)
         ep, ea = errors(ctx, targets, direct)
+        graph_run = None if args.graph else one_graph(ctx, prm, steps, n)
         prec_name = "fp64" if precision == tfmm.FP64 else "mixed"
         workload = ("batch of %d problems (cube/plummer/ellipsoid, N log-uniform in [2^10, 2^16])"
                     " total N=%d P=10 ncrit=%d theta=%g %s" % (nprob, n, ncrit, round(theta, 4),
@@ -394,7 +420,8 @@ def bench_batch(args, nprob, precision, steps, warmup):
                 "error": {"rel_l2_pot": ep, "rel_l2_acc": ea, "targets": len(targets),
                           "against": "FP64 direct sum within each problem"},
                 "picker": picked, "autotune": tuned, "phases_ms": phases, "counts": counts(st),
-                "roofline": kernels(st, phases, precision, workload)}, probs
+                "roofline": kernels(st, phases, precision, workload),
+                "one_graph": graph_run}, probs
     finally:
         ctx.close()
         del dx
@@ -437,6 +464,7 @@ def line(metric, head, args, cpu):
         "counts": head["counts"],
         "roofline": head["roofline"],
         "fp64_plain": head.get("fp64_plain"),
+        "one_graph": head.get("one_graph"),
         "cpu_baseline": cpu,
     }
 

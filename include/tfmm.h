@@ -111,6 +111,9 @@ typedef struct {
   /* problems in the forest (1 unless the bodies were set as a batch);
    * region_min/max above are those of problem 0 */
   int64_t n_problems;
+  /* 1 when the last tfmm_evaluate ran as one graph launch (TFMM_TUNE_GRAPH
+   * = 2), 0 when it ran the ordinary way */
+  int64_t one_graph;
 } tfmm_stats_t;
 
 int tfmm_create(int hip_device, tfmm_ctx **out);

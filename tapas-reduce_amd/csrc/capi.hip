@@ -41,6 +41,34 @@ enum Ev {
   EV_COUNT
 };
 
+// Capacities of an evaluation recorded as one graph (TFMM_TUNE_GRAPH = 2),
+// taken from an earlier evaluation of the context with headroom, and the
+// executable graph recorded with them.
+struct Plan {
+  bool valid = false;
+  // what the plan was made for
+  int64_t n = 0;
+  int nprob = 0;
+  tfmm_params prm{};
+  double unit_factor = 0;
+  int xcd_run = 0, p2p_schedule = 0, p2p_waves = 0;
+  // capacities
+  int L = 0;                        // tree levels
+  std::vector<int32_t> grid;        // cells per level
+  int64_t cells = 0;                // cells in all
+  int max_depth = 0, nsteps = 0, cwords = 0;
+  int32_t nslots = 0, ntiles = 0;
+  std::vector<uint32_t> step_grid;  // pairs per traversal step
+  PlanCaps caps{};
+  uint32_t groups = 0;
+  int gbits = 0, sbits = 0, tbits = 0;
+  // sort parities (fixed by the bit ranges)
+  bool keys_alt = false, gkeys_alt = false;
+  // the graph and the buffer addresses it was recorded with
+  hipGraphExec_t exec = nullptr;
+  uint64_t signature = 0;
+};
+
 }  // namespace
 
 struct tfmm_ctx {
@@ -108,6 +136,10 @@ struct tfmm_ctx {
   bool p2p_forked = false;       // this evaluation's P2P already started in phase A
   bool inner_timed = true;       // the last evaluation recorded its per-phase events
   hipGraphExec_t gexec = nullptr;
+  Plan plan;                      // the whole-evaluation graph (use_graph == 2)
+  DevBuf gsz, gstep;              // its device sizes and traversal step grids
+  uint32_t *h_plan = nullptr;     // pinned read-back of the whole graph (PLAN_HOST_BYTES)
+  bool one_graph = false;         // the last tfmm_evaluate was one launch of it
   int32_t unit_chunk = 0;
   uint32_t units_bound = 0;
   bool codes_ready = false;
@@ -195,6 +227,27 @@ Bodies sorted_bodies(tfmm_ctx *c) {
   b.q = dp<double>(c->sq);
   b.n = c->n;
   return b;
+}
+
+// every device buffer of the context
+std::vector<DevBuf *> all_buffers(tfmm_ctx *c) {
+  return {&c->own_x, &c->own_y, &c->own_z, &c->own_q, &c->keys, &c->keys_alt, &c->idx,
+          &c->idx_alt, &c->sort_tmp, &c->sx, &c->sy, &c->sz, &c->sq, &c->perm,
+          &c->region, &c->bbox_part, &c->c_key, &c->c_beg, &c->c_nb, &c->c_parent,
+          &c->c_child, &c->c_depth, &c->c_center, &c->c_width, &c->flag, &c->slot,
+          &c->scan_tmp, &c->M, &c->L, &c->front_a, &c->front_b, &c->m2l_pairs,
+          &c->p2p_pairs, &c->cnt, &c->lkeys, &c->lkeys_alt, &c->lidx, &c->lidx_alt,
+          &c->m2l_off, &c->m2l_src, &c->p2p_off, &c->p2p_src, &c->acc,
+          &c->trg, &c->out, &c->g_used, &c->g_gid, &c->g_y64, &c->g_y32, &c->g_af, &c->g_dl,
+          &c->g_lt, &c->g_cnt, &c->gkeys, &c->gvals, &c->gkeys_alt, &c->gvals_alt,
+          &c->fkeys, &c->fvals, &c->fkeys_alt, &c->fvals_alt, &c->it_flag,
+          &c->it_iflag, &c->it_iexcl, &c->items,
+          &c->tile_off, &c->cols, &c->mq, &c->codes, &c->fb_off, &c->fb_src, &c->fb_part,
+          &c->relpos, &c->leaf_of, &c->tops, &c->slot_of,
+          &c->feat_acc, &c->feat_x, &c->feat_out, &c->d_prob_off, &c->bprob,
+          &c->pkey, &c->pkey_alt, &c->ppos, &c->ppos_alt, &c->brec, &c->lev,
+          &c->lev_bsum, &c->lev_status, &c->dtt_geo, &c->dtt_topo, &c->u_cnt,
+          &c->u_off, &c->units, &c->u_part, &c->p2p_next, &c->finite_flag, &c->gsz, &c->gstep};
 }
 
 hipError_t reserve_cells(tfmm_ctx *c, size_t need, size_t used, hipStream_t s) {
@@ -883,19 +936,20 @@ void probe_external_events(tfmm_ctx *c) {
 
 // The P2P on the second stream (forked from and later joined to the context
 // stream by events).
-int fork_p2p(tfmm_ctx *c) {
+int fork_p2p(tfmm_ctx *c, int64_t ncells = -1) {
   hipStream_t s = c->stream;
   const Cells k = cells_of(c);
+  if (ncells < 0) ncells = c->ncells;
   const bool mixed = c->prm.precision == TFMM_MIXED;
   TRY(hipEventRecord(c->ev[EV_P2PREADY], s));
   TRY(hipStreamWaitEvent(c->side, c->ev[EV_P2PREADY], 0));
   TRY(rec(c, EV_P2P0, c->side));
   Csr pl{dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src)};
   if (mixed)
-    TRY(launch_p2p_fp32(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
+    TRY(launch_p2p_fp32(k, (int32_t)ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
                         dp<double>(c->acc), p2p_resident(c), dp<uint32_t>(c->p2p_next), c->side));
   else
-    TRY(launch_p2p_fp64(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<double>(c->acc),
+    TRY(launch_p2p_fp64(k, (int32_t)ncells, pl, sorted_bodies(c), dp<double>(c->acc),
                         c->side));
   TRY(rec(c, EV_P2P1, c->side));
   TRY(hipEventRecord(c->ev[EV_P2PJOIN], c->side));
@@ -1036,6 +1090,517 @@ int fused_numeric_graph(tfmm_ctx *c) {
   return TFMM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// The whole evaluation as one hipGraph (TFMM_TUNE_GRAPH = 2).  The first
+// evaluation of a context runs the ordinary way and leaves a plan:
+// capacities from its sizes with headroom.  Later evaluations with the same
+// bodies' count, parameters and knobs launch one graph recorded with those
+// capacities: every kernel reads the evaluation's actual sizes on the device
+// (plan.hip), nothing is read back before the end, and the graph's last
+// kernel checks the sizes against the capacities.  An evaluation that
+// exceeded one runs again the ordinary way and makes a new plan; the graph
+// is recorded again when a buffer moved.
+// Pinned read-back of the graph: sizes, group counters, traversal counters,
+// level table, pair counts and region.
+constexpr int PH_SZ = 0, PH_GCNT = 16, PH_CNT = 32, PH_LEV = 96, PH_WORDS = 160;
+constexpr size_t PH_U64_BYTE = PH_WORDS * 4, PH_REGION_BYTE = PH_U64_BYTE + 16;
+constexpr size_t PLAN_HOST_BYTES = PH_REGION_BYTE + 48;
+constexpr int PLAN_MAX_LEVELS = 30;  // 2 (L + 1) words of PH_LEV
+constexpr int PLAN_MAX_CWORDS = 64;  // 16 + steps + 1 words of PH_CNT
+
+uint64_t buffers_signature(tfmm_ctx *c) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+  for (DevBuf *b : all_buffers(c)) mix((uint64_t)(uintptr_t)b->ptr);
+  mix((uint64_t)(uintptr_t)c->bx);
+  mix((uint64_t)(uintptr_t)c->by);
+  mix((uint64_t)(uintptr_t)c->bz);
+  mix((uint64_t)(uintptr_t)c->bq);
+  return h;
+}
+
+bool plan_fits(tfmm_ctx *c, const tfmm_params *p) {
+  const Plan &pl = c->plan;
+  return pl.valid && pl.n == c->n && pl.nprob == c->nprob && pl.prm.ncrit == p->ncrit &&
+         pl.prm.theta == p->theta && pl.prm.precision == p->precision &&
+         pl.prm.mutual == p->mutual && pl.unit_factor == c->m2l_unit_factor &&
+         pl.xcd_run == c->m2l_xcd_run && pl.p2p_schedule == c->p2p_schedule &&
+         pl.p2p_waves == c->p2p_waves;
+}
+
+void plan_drop(tfmm_ctx *c) {
+  if (c->plan.exec) (void)hipGraphExecDestroy(c->plan.exec);
+  c->plan = Plan();
+}
+
+// Capacities from the sizes of the evaluation that just ran (its tree, its
+// traversal, its M2L batches): one more level than it had, an eighth more
+// of everything else.  No plan when the sizes leave no room in the keys.
+void plan_make(tfmm_ctx *c) {
+  plan_drop(c);
+  Plan pl;
+  const int L0 = (int)c->tree_hint.size();
+  if (L0 < 1 || L0 + 1 > PLAN_MAX_LEVELS || c->dtt_hint.empty()) return;
+  auto more = [](int64_t v, int64_t extra) { return v + v / 8 + extra; };
+  pl.n = c->n;
+  pl.nprob = c->nprob;
+  pl.prm = c->prm;
+  pl.unit_factor = c->m2l_unit_factor;
+  pl.xcd_run = c->m2l_xcd_run;
+  pl.p2p_schedule = c->p2p_schedule;
+  pl.p2p_waves = c->p2p_waves;
+  pl.L = L0 + 1;
+  pl.grid.resize(pl.L);
+  pl.cells = 4096;
+  for (int d = 0; d < pl.L; d++) {
+    const int64_t h = d < L0 ? c->tree_hint[d] : std::max<int64_t>(c->tree_hint[L0 - 1], 1024);
+    int64_t g = more(h, 256);
+    if (d == 0) g = std::max<int64_t>(g, c->nprob);
+    pl.grid[d] = (int32_t)std::min<int64_t>(g, INT32_MAX / 16);
+    pl.cells += pl.grid[d];
+  }
+  if (pl.cells > INT32_MAX / 4) return;
+  pl.max_depth = pl.L - 1;
+  pl.nsteps = 2 * pl.max_depth + 2;
+  pl.cwords = 16 + pl.nsteps + 1;
+  if (pl.cwords > PLAN_MAX_CWORDS) return;
+  pl.nslots = m2l_group_slots(pl.max_depth);
+  int64_t fmax = 0;
+  pl.step_grid.resize(pl.nsteps);
+  for (int st = 0; st < pl.nsteps; st++) {
+    const int64_t h = st < (int)c->dtt_hint.size() ? c->dtt_hint[st] : 0;
+    pl.step_grid[st] = (uint32_t)std::min<int64_t>(UINT32_MAX / 4, more(h, 4096));
+    fmax = std::max<int64_t>(fmax, pl.step_grid[st]);
+  }
+  auto cap32 = [](int64_t v) { return (uint32_t)std::min<int64_t>(UINT32_MAX / 4, v); };
+  pl.caps.front = cap32(fmax);
+  pl.caps.m2l = cap32(more(c->n_m2l_exec, 65536));
+  pl.caps.p2p = cap32(more(c->n_p2p, 65536));
+  pl.caps.items = cap32(more(c->n_items, 4096));
+  pl.caps.fallback = cap32(more(c->n_fallback, 4096));
+  pl.groups = cap32(more(c->n_groups, 256));
+  pl.ntiles = (int32_t)((pl.cells + m2l_tile_size() - 1) / m2l_tile_size());
+  const double f = c->m2l_unit_factor;
+  pl.caps.units =
+      cap32(f > 0 ? pl.ntiles + (int64_t)std::ceil(pl.ntiles / f) + 1 : (int64_t)pl.ntiles);
+  pl.gbits = bits_for(pl.groups);
+  pl.sbits = bits_for(pl.cells);
+  pl.tbits = bits_for(pl.ntiles);
+  if (pl.tbits + pl.gbits > 32 || pl.gbits > m2l_group_bits() ||
+      pl.cells >= ((int64_t)1 << m2l_src_bits()) || 2 * pl.sbits > 64 ||
+      (int64_t)pl.caps.fallback > (1 << 22))
+    return;
+  pl.valid = true;
+  c->plan = pl;
+}
+
+// every buffer of the graph, at the plan's capacities
+int plan_reserve(tfmm_ctx *c) {
+  const Plan &pl = c->plan;
+  hipStream_t s = c->stream;
+  const int64_t n = c->n;
+  const bool mixed = pl.prm.precision == TFMM_MIXED;
+  const size_t cells = (size_t)pl.cells;
+  const size_t nm = pl.caps.m2l, npp = pl.caps.p2p;
+  // tree
+  TRY(dev_reserve(c->keys, n * 8));
+  TRY(dev_reserve(c->keys_alt, n * 8));
+  TRY(dev_reserve(c->idx, n * 4));
+  TRY(dev_reserve(c->idx_alt, n * 4));
+  const size_t sort_n = std::max<size_t>({(size_t)n, nm, npp});
+  TRY(dev_reserve(c->sort_tmp, radix_tmp_bytes((int64_t)sort_n)));
+  TRY(dev_reserve(c->sx, n * 8));
+  TRY(dev_reserve(c->sy, n * 8));
+  TRY(dev_reserve(c->sz, n * 8));
+  TRY(dev_reserve(c->sq, n * 8));
+  TRY(dev_reserve(c->perm, n * 4));
+  TRY(dev_reserve(c->region, 6 * 8 * (size_t)c->nprob));
+  TRY(dev_reserve(c->brec, n * 32));
+  if (c->nprob > 1) {
+    TRY(dev_reserve(c->bprob, n * 4));
+    TRY(dev_reserve(c->pkey, n * 4));
+    TRY(dev_reserve(c->pkey_alt, n * 4));
+    TRY(dev_reserve(c->ppos, n * 4));
+    TRY(dev_reserve(c->ppos_alt, n * 4));
+  } else {
+    const int nparts = (int)std::min<int64_t>(1024, (n + 255) / 256);
+    TRY(dev_reserve(c->bbox_part, 2 * 8 * (size_t)nparts));
+  }
+  TRY(reserve_cells(c, cells, 0, s));
+  int32_t gmax = 0;
+  for (int32_t g : pl.grid) gmax = std::max(gmax, g);
+  TRY(dev_reserve(c->lev, (size_t)2 * (pl.L + 1) * 4));
+  TRY(dev_reserve(c->lev_status, 4));
+  TRY(dev_reserve(c->flag, (size_t)gmax * 4 + 1024));
+  TRY(dev_reserve(c->slot, (size_t)gmax * 4 + 1024));
+  TRY(dev_reserve(c->lev_bsum, (size_t)level_blocks(gmax) * 4 + 4));
+  // traversal and P2P list
+  TRY(dev_reserve(c->cnt, (size_t)pl.cwords * 4));
+  TRY(dev_reserve(c->dtt_geo, cells * 32));
+  TRY(dev_reserve(c->dtt_topo, cells * 8));
+  TRY(dev_reserve(c->g_used, (size_t)(pl.nslots + 1) * 4));
+  TRY(dev_reserve(c->m2l_pairs, nm * 8));
+  TRY(dev_reserve(c->slot_of, nm * 4));
+  TRY(dev_reserve(c->p2p_pairs, npp * 8));
+  TRY(dev_reserve(c->front_a, (size_t)pl.caps.front * 8));
+  TRY(dev_reserve(c->front_b, (size_t)pl.caps.front * 8));
+  TRY(dev_reserve(c->lkeys, npp * 4 + 4));
+  TRY(dev_reserve(c->lkeys_alt, npp * 4 + 4));
+  TRY(dev_reserve(c->lidx, npp * 4 + 4));
+  TRY(dev_reserve(c->lidx_alt, npp * 4 + 4));
+  TRY(dev_reserve(c->p2p_off, (cells + 1) * 4));
+  TRY(dev_reserve(c->p2p_src, npp * 4 + 4));
+  if (mixed) TRY(dev_reserve(c->relpos, (size_t)n * 16));
+  TRY(dev_reserve(c->acc, (size_t)n * 32));
+  // M2L batches
+  const size_t ng = pl.groups;
+  const size_t item_blocks = (nm + 1023) / 1024;
+  TRY(dev_reserve(c->g_gid, (size_t)pl.nslots * 4));
+  TRY(dev_reserve(c->scan_tmp,
+                  scan_tmp_bytes(std::max<int64_t>({(int64_t)pl.nslots, (int64_t)pl.ntiles,
+                                                    (int64_t)item_blocks}))));
+  TRY(dev_reserve(c->g_y64, ng * 112 * 8));
+  TRY(dev_reserve(c->g_y32, ng * 112 * 4));
+  TRY(dev_reserve(c->g_dl, ng));
+  TRY(dev_reserve(c->g_lt, ng));
+  TRY(dev_reserve(c->g_af, m2l_afrag_bytes((uint32_t)ng, mixed)));
+  TRY(dev_reserve(c->gkeys, nm * 4));
+  TRY(dev_reserve(c->gvals, nm * 4));
+  TRY(dev_reserve(c->gkeys_alt, nm * 4));
+  TRY(dev_reserve(c->gvals_alt, nm * 4));
+  TRY(dev_reserve(c->fkeys, nm * 8));
+  TRY(dev_reserve(c->fvals, nm * 4));
+  TRY(dev_reserve(c->fkeys_alt, nm * 8));
+  TRY(dev_reserve(c->fvals_alt, nm * 4));
+  TRY(dev_reserve(c->g_cnt, 16));
+  TRY(dev_reserve(c->it_flag, nm * 4));
+  TRY(dev_reserve(c->it_iflag, item_blocks * 4));
+  TRY(dev_reserve(c->it_iexcl, item_blocks * 4));
+  TRY(dev_reserve(c->items, (size_t)pl.caps.items * 16));
+  TRY(dev_reserve(c->cols, (size_t)pl.caps.items * 16 * 8));
+  TRY(dev_reserve(c->tile_off, (size_t)(pl.ntiles + 1) * 4));
+  TRY(dev_reserve(c->u_cnt, (size_t)pl.ntiles * 4));
+  TRY(dev_reserve(c->u_off, (size_t)pl.ntiles * 4));
+  TRY(dev_reserve(c->units, (size_t)pl.caps.units * 8));
+  TRY(dev_reserve(c->u_part, m2l_unit_part_bytes(pl.caps.units, pl.ntiles)));
+  TRY(dev_reserve(c->fb_off, (cells + 1) * 4));
+  TRY(dev_reserve(c->fb_src, (size_t)pl.caps.fallback * 4 + 4));
+  TRY(dev_reserve(c->fb_part, m2l_pairs_scratch_bytes((int32_t)pl.caps.fallback)));
+  if (!c->codes_ready) {
+    TRY(dev_reserve(c->codes, m2l_codes_bytes()));
+    const hipError_t e = m2l_upload_codes(dp<uint32_t>(c->codes), s);
+    if (e == hipErrorInvalidValue) return fail(c, TFMM_EHIP, "M2L operator table is inconsistent");
+    TRY(e);
+    c->codes_ready = true;
+  }
+  // expansions and fields
+  const size_t ebytes = cells * NTERM * 16;
+  TRY(dev_reserve(c->M, ebytes));
+  TRY(dev_reserve(c->L, ebytes));
+  if (!c->tops_ready) {
+    TRY(dev_reserve(c->tops, translation_ops_doubles() * 8));
+    TRY(launch_translation_ops(dp<double>(c->tops), s));
+    c->tops_ready = true;
+  }
+  TRY(dev_reserve(c->mq, cells * 112 * (mixed ? 4 : 8)));
+  TRY(dev_reserve(c->trg, (size_t)n * 32));
+  TRY(dev_reserve(c->out, (size_t)n * 32));
+  TRY(dev_reserve(c->leaf_of, (size_t)n * 4));
+  // the graph's own words
+  TRY(dev_reserve(c->gsz, SZ_WORDS * 4));
+  TRY(dev_reserve(c->gstep, (size_t)pl.nsteps * 4));
+  TRY(hipMemcpyAsync(c->gstep.ptr, pl.step_grid.data(), (size_t)pl.nsteps * 4,
+                     hipMemcpyHostToDevice, s));
+  if (!c->h_plan) TRY(hipHostMalloc((void **)&c->h_plan, PLAN_HOST_BYTES, 0));
+  TRY(hipStreamSynchronize(s));
+  return TFMM_OK;
+}
+
+// The evaluation with the plan's capacities and the device's sizes, enqueued
+// on the context stream (recorded into the graph by the caller).
+int whole_record(tfmm_ctx *c) {
+  Plan &pl = c->plan;
+  hipStream_t s = c->stream;
+  const int64_t n = c->n;
+  const int np = c->nprob;
+  const bool forest = np > 1;
+  const bool mixed = pl.prm.precision == TFMM_MIXED;
+  const Cells k = cells_of(c);
+  const int32_t cells = (int32_t)pl.cells;
+  uint32_t *sz = dp<uint32_t>(c->gsz);
+  TRY(launch_fill_zero(c->gsz.ptr, SZ_WORDS * 4, s));
+  // bodies to sorted keys
+  double *region = dp<double>(c->region);
+  const int64_t *off = forest ? dp<int64_t>(c->d_prob_off) : nullptr;
+  if (forest) {
+    TRY(launch_bbox_forest(c->bx, c->by, c->bz, off, np, region, s));
+  } else {
+    const int nparts = (int)std::min<int64_t>(1024, (n + 255) / 256);
+    TRY(launch_bbox(c->bx, c->by, c->bz, n, dp<double>(c->bbox_part), nparts, region, s));
+  }
+  TRY(launch_keys(c->bx, c->by, c->bz, c->bq, n, region, off, np, dp<uint64_t>(c->keys),
+                  dp<uint32_t>(c->idx), forest ? dp<uint32_t>(c->bprob) : nullptr,
+                  dp<double4>(c->brec), s));
+  bool alt = false;
+  TRY(radix_sort_pairs(dp<uint64_t>(c->keys), dp<uint32_t>(c->idx), dp<uint64_t>(c->keys_alt),
+                       dp<uint32_t>(c->idx_alt), n, DEPTHBITS, DEPTHBITS + 3 * MAXDEPTH,
+                       c->sort_tmp.ptr, s, &alt));
+  const uint32_t *sprob = nullptr;
+  if (forest) {
+    const uint64_t *k1 = dp<uint64_t>(alt ? c->keys_alt : c->keys);
+    const uint32_t *o1 = dp<uint32_t>(alt ? c->idx_alt : c->idx);
+    TRY(launch_problem_keys(o1, dp<uint32_t>(c->bprob), n, dp<uint32_t>(c->pkey),
+                            dp<uint32_t>(c->ppos), s));
+    bool palt = false;
+    TRY(radix_sort_pairs32(dp<uint32_t>(c->pkey), dp<uint32_t>(c->ppos), dp<uint32_t>(c->pkey_alt),
+                           dp<uint32_t>(c->ppos_alt), n, 0, bits_for(np - 1), c->sort_tmp.ptr, s,
+                           &palt));
+    sprob = dp<uint32_t>(palt ? c->pkey_alt : c->pkey);
+    TRY(launch_problem_gather(dp<uint32_t>(palt ? c->ppos_alt : c->ppos), k1, o1, n,
+                              dp<uint64_t>(alt ? c->keys : c->keys_alt),
+                              dp<uint32_t>(alt ? c->idx : c->idx_alt), s));
+    alt = !alt;
+  }
+  pl.keys_alt = alt;
+  const uint64_t *skeys = dp<uint64_t>(alt ? c->keys_alt : c->keys);
+  TRY(launch_permute(dp<uint32_t>(alt ? c->idx_alt : c->idx), dp<double4>(c->brec), n,
+                     dp<double>(c->sx), dp<double>(c->sy), dp<double>(c->sz), dp<double>(c->sq),
+                     dp<int32_t>(c->perm), s));
+  // cells, level by level, every size on the device
+  int32_t *lev = dp<int32_t>(c->lev);
+  TRY(launch_fill_zero(c->lev.ptr, (size_t)2 * (pl.L + 1) * 4, s));
+  TRY(launch_fill_zero(c->lev_status.ptr, 4, s));
+  TRY(launch_roots(k, n, off, np, region, lev, s));
+  for (int d = 0; d < pl.L; d++)
+    TRY(launch_level_step(k, lev, d, pl.grid[d], pl.prm.ncrit, cells, dp<uint32_t>(c->flag),
+                          dp<uint32_t>(c->slot), dp<uint32_t>(c->lev_bsum),
+                          dp<uint32_t>(c->lev_status), skeys, region, sprob, s, d == pl.L - 1));
+  TRY(launch_tree_final(lev, pl.L, dp<uint32_t>(c->lev_status), m2l_tile_size(), sz, s));
+  TRY(launch_pad_cells(k, sz, cells, s));
+  TRY(rec(c, EV_TREE, s));
+  TRY(rec(c, EV_DTT0, s));
+  // traversal
+  uint32_t *cnt = dp<uint32_t>(c->cnt);
+  uint32_t *used = dp<uint32_t>(c->g_used);
+  TRY(launch_dtt_cells(k, cells, pl.prm.theta, dp<double4>(c->dtt_geo), dp<int2>(c->dtt_topo),
+                       s));
+  TRY(launch_fill_zero(used, (size_t)(pl.nslots + 1) * 4, s));
+  TRY(launch_fill_zero(cnt, (size_t)pl.cwords * 4, s));
+  TRY(launch_root_pairs(dp<int2>(c->front_a), np, cnt + 16, s));
+  DevBuf *fa = &c->front_a, *fb = &c->front_b;
+  for (int st = 0; st < pl.nsteps; st++) {
+    TRY(launch_dtt_step(dp<int2>(*fa), cnt + 16 + st, pl.step_grid[st], dp<int2>(*fb),
+                        pl.caps.front, cnt + 16 + st + 1, dp<int2>(c->m2l_pairs), pl.caps.m2l,
+                        dp<int2>(c->p2p_pairs), pl.caps.p2p, cnt, dp<double4>(c->dtt_geo),
+                        dp<int2>(c->dtt_topo), k.depth, k.width, dp<int32_t>(c->slot_of), used,
+                        used + pl.nslots, pl.prm.mutual, s));
+    std::swap(fa, fb);
+  }
+  // P2P list: targets sorted, each target's sources sorted
+  {
+    const uint32_t npp = pl.caps.p2p;
+    TRY(launch_split_pairs(dp<int2>(c->p2p_pairs), npp, dp<uint32_t>(c->lkeys),
+                           dp<uint32_t>(c->lidx), s, cnt + 2));
+    bool lalt = false;
+    TRY(radix_sort_pairs32(dp<uint32_t>(c->lkeys), dp<uint32_t>(c->lidx),
+                           dp<uint32_t>(c->lkeys_alt), dp<uint32_t>(c->lidx_alt), npp, 0,
+                           pl.sbits, c->sort_tmp.ptr, s, &lalt, cnt + 2));
+    TRY(launch_csr_sorted_sources(dp<uint32_t>(lalt ? c->lkeys_alt : c->lkeys),
+                                  dp<uint32_t>(lalt ? c->lidx_alt : c->lidx), npp, cells,
+                                  dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src), s, cnt + 2));
+  }
+  if (mixed)
+    TRY(launch_leaf_relpos(k, cells, sorted_bodies(c), dp<float4>(c->relpos), s));
+  // M2L batches
+  {
+    const uint32_t nm = pl.caps.m2l;
+    uint32_t *gcnt = dp<uint32_t>(c->g_cnt);
+    TRY(scan_exclusive_u32(used, dp<uint32_t>(c->g_gid), pl.nslots, c->scan_tmp.ptr, s));
+    TRY(launch_groups_count(dp<uint32_t>(c->g_gid), used, pl.nslots, pl.groups, sz, s));
+    TRY(launch_m2l_group_harmonics(used, dp<uint32_t>(c->g_gid), pl.nslots,
+                                   dp<double>(c->g_y64), dp<float>(c->g_y32),
+                                   dp<int8_t>(c->g_dl), dp<int8_t>(c->g_lt), s, pl.groups));
+    TRY(launch_fill_zero(gcnt, 16, s));
+    TRY(launch_m2l_classify(dp<int2>(c->m2l_pairs), dp<int32_t>(c->slot_of), nm,
+                            dp<uint32_t>(c->g_gid), pl.gbits, pl.sbits, dp<uint32_t>(c->gkeys),
+                            dp<uint32_t>(c->gvals), dp<uint64_t>(c->fkeys),
+                            dp<uint32_t>(c->fvals), gcnt, s, cnt + 1, pl.groups));
+    bool galt = false;
+    TRY(radix_sort_pairs32(dp<uint32_t>(c->gkeys), dp<uint32_t>(c->gvals),
+                           dp<uint32_t>(c->gkeys_alt), dp<uint32_t>(c->gvals_alt), nm, 0,
+                           pl.tbits + pl.gbits, c->sort_tmp.ptr, s, &galt, gcnt));
+    pl.gkeys_alt = galt;
+    const uint32_t *gk = dp<uint32_t>(galt ? c->gkeys_alt : c->gkeys);
+    const uint32_t *gv = dp<uint32_t>(galt ? c->gvals_alt : c->gvals);
+    TRY(launch_m2l_count_items_dev(gk, nm, gcnt, dp<uint32_t>(c->it_flag),
+                                   dp<uint32_t>(c->it_iflag), dp<uint32_t>(c->it_iexcl),
+                                   c->scan_tmp.ptr, sz + SZ_NITEMS, s));
+    TRY(launch_m2l_write_items(gk, gv, nm, pl.gbits, dp<uint32_t>(c->it_flag),
+                               dp<uint32_t>(c->it_iexcl), dp<int8_t>(c->g_dl), dp<int4>(c->items),
+                               dp<uint2>(c->cols), s, gcnt, pl.caps.items));
+    TRY(launch_m2l_tile_offsets(dp<int4>(c->items), pl.caps.items, pl.ntiles,
+                                dp<int32_t>(c->tile_off), s, sz + SZ_NITEMS));
+    int32_t *chunk = reinterpret_cast<int32_t *>(sz + SZ_CHUNK);
+    TRY(launch_m2l_unit_chunk(sz + SZ_NITEMS, sz + SZ_NTILES, c->m2l_unit_factor, chunk, s));
+    TRY(launch_m2l_units(dp<int32_t>(c->tile_off), pl.ntiles, 0, dp<uint32_t>(c->u_cnt),
+                         dp<uint32_t>(c->u_off), dp<int2>(c->units), c->scan_tmp.ptr, s,
+                         pl.caps.units, chunk));
+    // the pairs outside the groups (appended compactly): at most the
+    // fallback capacity of them are sorted
+    const uint32_t nfb = std::min<uint32_t>(nm, pl.caps.fallback);
+    bool falt = false;
+    TRY(radix_sort_pairs(dp<uint64_t>(c->fkeys), dp<uint32_t>(c->fvals), dp<uint64_t>(c->fkeys_alt),
+                         dp<uint32_t>(c->fvals_alt), nfb, 0, 2 * pl.sbits, c->sort_tmp.ptr, s,
+                         &falt, gcnt + 1));
+    const uint64_t *fk = dp<uint64_t>(falt ? c->fkeys_alt : c->fkeys);
+    TRY(launch_csr_offsets(fk, nfb, pl.sbits, cells, dp<int32_t>(c->fb_off), s, gcnt + 1));
+    TRY(launch_unpack_pairs(fk, nfb, pl.sbits, dp<int32_t>(c->fb_src), s, gcnt + 1));
+    TRY(launch_m2l_group_afrag(mixed ? (const void *)c->g_y32.ptr : (const void *)c->g_y64.ptr,
+                               pl.groups, dp<uint32_t>(c->codes), dp<int8_t>(c->g_dl),
+                               c->g_af.ptr, mixed, s, sz + SZ_NG));
+  }
+  TRY(rec(c, EV_DTT1, s));
+  // the numeric phase
+  int rc;
+  if (c->p2p_schedule != 1 && (rc = fork_p2p(c, cells))) return rc;
+  const size_t ebytes = (size_t)cells * NTERM * 16;
+  TRY(launch_fill_zero(c->M.ptr, ebytes, s));
+  TRY(launch_p2m(k, cells, sorted_bodies(c), dp<double>(c->M), s));
+  for (int d = pl.L - 2; d >= 0; d--)
+    TRY(launch_m2m(k, 0, pl.grid[d + 1] / 8 + 1, dp<double>(c->tops), dp<double>(c->M), s, lev,
+                   d + 1));
+  TRY(rec(c, EV_UP, s));
+  TRY(launch_m2l_scale_m(k, cells, dp<double>(c->M), c->mq.ptr, mixed, s));
+  TRY(launch_fill_zero(c->L.ptr, ebytes, s));
+  TRY(rec(c, EV_M2L0, s));
+  if (c->p2p_schedule == 1 && (rc = fork_p2p(c, cells))) return rc;
+  TRY(launch_m2l_tiles(dp<int32_t>(c->tile_off), dp<int2>(c->units), dp<uint32_t>(c->u_cnt),
+                       dp<uint32_t>(c->u_off), pl.ntiles, 0, c->m2l_xcd_run, pl.caps.units,
+                       dp<uint2>(c->cols), c->g_af.ptr, c->mq.ptr, k, cells, dp<double>(c->L),
+                       dp<double>(c->u_part), mixed, s,
+                       reinterpret_cast<const int32_t *>(sz + SZ_CHUNK)));
+  TRY(rec(c, EV_M2LT, s));
+  if (pl.caps.fallback > 0) {
+    Csr fl{dp<int32_t>(c->fb_off), dp<int32_t>(c->fb_src)};
+    TRY(launch_m2l_fp64(k, cells, fl, (int32_t)pl.caps.fallback, dp<double>(c->M),
+                        dp<double>(c->L), c->fb_part.ptr, s, dp<uint32_t>(c->g_cnt) + 1));
+  }
+  TRY(rec(c, EV_M2L1, s));
+  TRY(hipStreamWaitEvent(s, c->ev[EV_P2PJOIN], 0));
+  TRY(rec(c, EV_TRAV, s));
+  for (int d = 0; d + 1 < pl.L; d++)
+    TRY(launch_l2l(k, 0, pl.grid[d + 1] / 8 + 1, dp<double>(c->tops), dp<double>(c->L), s, lev,
+                   d + 1));
+  TRY(launch_l2p(k, cells, sorted_bodies(c), dp<double>(c->L), dp<double>(c->acc),
+                 dp<int32_t>(c->perm), dp<int32_t>(c->leaf_of), dp<double>(c->trg),
+                 dp<double>(c->out), s));
+  // capacities, then the read-back
+  TRY(launch_plan_check(cnt, pl.nsteps, dp<uint32_t>(c->gstep), pl.caps, dp<uint32_t>(c->g_cnt),
+                        dp<uint32_t>(c->u_cnt), dp<uint32_t>(c->u_off), pl.ntiles, sz, s));
+  char *h = reinterpret_cast<char *>(c->h_plan);
+  TRY(hipMemcpyAsync(h + PH_SZ * 4, sz, SZ_WORDS * 4, hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(h + PH_GCNT * 4, c->g_cnt.ptr, 8, hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(h + PH_CNT * 4, cnt, (size_t)pl.cwords * 4, hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(h + PH_LEV * 4, lev, (size_t)2 * (pl.L + 1) * 4, hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(h + PH_U64_BYTE, static_cast<char *>(c->cnt.ptr) + 32, 16,
+                     hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(h + PH_REGION_BYTE, region, 48, hipMemcpyDeviceToHost, s));
+  return TFMM_OK;
+}
+
+// reserve, record and instantiate the plan's graph
+int whole_capture(tfmm_ctx *c) {
+  Plan &pl = c->plan;
+  if (pl.exec) (void)hipGraphExecDestroy(pl.exec);
+  pl.exec = nullptr;
+  int rc = plan_reserve(c);
+  if (rc) return rc;
+  if (c->ext_events < 0) probe_external_events(c);
+  hipStream_t s = c->stream;
+  TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+  c->capturing = true;
+  rc = whole_record(c);
+  c->capturing = false;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return hipfail(c, e, "hipStreamEndCapture");
+  const hipError_t e2 = hipGraphInstantiate(&pl.exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e2 != hipSuccess) {
+    pl.exec = nullptr;
+    return hipfail(c, e2, "hipGraphInstantiate");
+  }
+  pl.signature = buffers_signature(c);
+  return TFMM_OK;
+}
+
+// the host's view of the evaluation the graph just ran
+void whole_finish(tfmm_ctx *c) {
+  const Plan &pl = c->plan;
+  const uint32_t *h = c->h_plan;
+  const uint32_t *hs = h + PH_SZ, *gc = h + PH_GCNT, *cn = h + PH_CNT;
+  const int32_t *hl = reinterpret_cast<const int32_t *>(h + PH_LEV);
+  c->ncells = hs[SZ_NCELLS];
+  c->nleaves = hs[SZ_NLEAVES];
+  c->lvl.assign(1, 0);
+  c->tree_hint.clear();
+  for (int d = 0; d < pl.L && hl[2 * d + 1] > 0; d++) {
+    c->lvl.push_back(hl[2 * d] + hl[2 * d + 1]);
+    c->tree_hint.push_back(hl[2 * d + 1]);
+  }
+  c->keys_in_alt = pl.keys_alt;
+  c->grouped_keys_in_alt = pl.gkeys_alt;
+  c->n_m2l_exec = cn[1];
+  c->n_p2p = cn[2];
+  c->n_m2l_all = cn[3];
+  c->n_p2p_calls_leaf = cn[4];
+  c->dtt_steps = 0;
+  for (int st = 0; st <= pl.nsteps; st++)
+    if (cn[16 + st]) c->dtt_steps = st + 1;
+  memcpy(c->h_u64, reinterpret_cast<const char *>(h) + PH_U64_BYTE, 16);
+  memcpy(c->h_region, reinterpret_cast<const char *>(h) + PH_REGION_BYTE, 48);
+  c->n_groups = hs[SZ_NG];
+  c->n_grouped = gc[0];
+  c->n_fallback = gc[1];
+  c->n_items = hs[SZ_NITEMS];
+  c->unit_chunk = (int32_t)hs[SZ_CHUNK];
+  c->units_bound = pl.caps.units;
+}
+
+// One launch of the plan's graph (recorded again when a buffer moved).
+// 0: done; 1: a capacity was exceeded (the caller evaluates the ordinary
+// way); < 0: error.
+int whole_run(tfmm_ctx *c, const tfmm_params *p) {
+  TRY(hipSetDevice(c->device));
+  c->prm = *p;
+  Plan &pl = c->plan;
+  if (!pl.exec || pl.signature != buffers_signature(c)) {
+    const int rc = whole_capture(c);
+    if (rc) return rc;
+  }
+  hipStream_t s = c->stream;
+  c->stage = 0;
+  TRY(hipEventRecord(c->ev[EV_BEGIN], s));
+  if (c->ext_events != 1) TRY(hipEventRecord(c->ev[EV_TREE], s));  // no inner times
+  TRY(hipGraphLaunch(pl.exec, s));
+  TRY(hipEventRecord(c->ev[EV_DOWN], s));
+  TRY(hipStreamSynchronize(s));
+  if (c->h_plan[PH_SZ + SZ_STATUS] != 0) {
+    plan_drop(c);
+    return 1;
+  }
+  whole_finish(c);
+  c->one_graph = true;
+  c->fused = true;
+  c->inner_timed = c->ext_events == 1;
+  c->stage = 4;
+  return TFMM_OK;
+}
+
 int need_stage(tfmm_ctx *c, int st, const char *what) {
   if (!c) return TFMM_EINVAL;
   if (c->stage < st) return fail(c, TFMM_ESTATE, std::string(what) + " called out of order");
@@ -1123,24 +1688,10 @@ void tfmm_destroy(tfmm_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->side) (void)hipStreamSynchronize(c->side);
-  DevBuf *bufs[] = {&c->own_x, &c->own_y, &c->own_z, &c->own_q, &c->keys, &c->keys_alt, &c->idx,
-                    &c->idx_alt, &c->sort_tmp, &c->sx, &c->sy, &c->sz, &c->sq, &c->perm,
-                    &c->region, &c->bbox_part, &c->c_key, &c->c_beg, &c->c_nb, &c->c_parent,
-                    &c->c_child, &c->c_depth, &c->c_center, &c->c_width, &c->flag, &c->slot,
-                    &c->scan_tmp, &c->M, &c->L, &c->front_a, &c->front_b, &c->m2l_pairs,
-                    &c->p2p_pairs, &c->cnt, &c->lkeys, &c->lkeys_alt, &c->lidx, &c->lidx_alt,
-                    &c->m2l_off, &c->m2l_src, &c->p2p_off, &c->p2p_src, &c->acc,
-                    &c->trg, &c->out, &c->g_used, &c->g_gid, &c->g_y64, &c->g_y32, &c->g_af, &c->g_dl,
-                    &c->g_lt, &c->g_cnt, &c->gkeys, &c->gvals, &c->gkeys_alt, &c->gvals_alt,
-                    &c->fkeys, &c->fvals, &c->fkeys_alt, &c->fvals_alt, &c->it_flag,
-                    &c->it_iflag, &c->it_iexcl, &c->items,
-                    &c->tile_off, &c->cols, &c->mq, &c->codes, &c->fb_off, &c->fb_src, &c->fb_part, &c->relpos, &c->leaf_of, &c->tops, &c->slot_of,
-                    &c->feat_acc, &c->feat_x, &c->feat_out, &c->d_prob_off, &c->bprob,
-                    &c->pkey, &c->pkey_alt, &c->ppos, &c->ppos_alt, &c->brec, &c->lev,
-                    &c->lev_bsum, &c->lev_status, &c->dtt_geo, &c->dtt_topo, &c->u_cnt,
-                    &c->u_off, &c->units, &c->u_part, &c->p2p_next, &c->finite_flag};
-  for (DevBuf *b : bufs) dev_release(*b);
+  for (DevBuf *b : all_buffers(c)) dev_release(*b);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->plan.exec) (void)hipGraphExecDestroy(c->plan.exec);
+  if (c->h_plan) (void)hipHostFree(c->h_plan);
   for (int i = 0; i < EV_COUNT; i++)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   if (c->h_u32) (void)hipHostFree(c->h_u32);
@@ -1170,7 +1721,8 @@ int tfmm_set_tuning(tfmm_ctx *c, int knob, double value) {
       c->p2p_schedule = (int)value;
       return TFMM_OK;
     case TFMM_TUNE_GRAPH:
-      if (value != 0 && value != 1) return fail(c, TFMM_EINVAL, "graph must be 0 or 1");
+      if (value != 0 && value != 1 && value != 2)
+        return fail(c, TFMM_EINVAL, "graph must be 0, 1 or 2");
       c->use_graph = (int)value;
       return TFMM_OK;
     case TFMM_TUNE_P2P_WAVES:
@@ -1449,6 +2001,7 @@ int tfmm_build_tree(tfmm_ctx *c, const tfmm_params *p) {
   TRY(hipSetDevice(c->device));
   c->prm = *p;
   c->stage = 0;
+  c->one_graph = false;
   TRY(hipEventRecord(c->ev[EV_BEGIN], c->stream));
   rc = do_build_tree(c);
   if (rc) return rc;
@@ -1513,15 +2066,28 @@ int tfmm_downward(tfmm_ctx *c) {
 }
 
 int tfmm_evaluate(tfmm_ctx *c, const tfmm_params *p) {
+  const bool whole = c && c->use_graph == 2;
+  if (whole) {
+    int rc = check_params(c, p);
+    if (rc) return rc;
+    if (c->n < 1) return fail(c, TFMM_ESTATE, "no bodies set");
+    if (plan_fits(c, p)) {
+      rc = whole_run(c, p);
+      if (rc <= 0) return rc;
+    }
+  }
   int rc = tfmm_build_tree(c, p);
   if (rc) return rc;
   c->fused = true;
+  c->one_graph = false;
   rc = fused_prepare(c);
   if (rc) return rc;
-  rc = c->use_graph ? fused_numeric_graph(c) : fused_numeric(c);
+  const bool graph = c->use_graph == 1;
+  rc = graph ? fused_numeric_graph(c) : fused_numeric(c);
   if (rc) return rc;
-  c->inner_timed = !c->use_graph || c->ext_events == 1;
+  c->inner_timed = !graph || c->ext_events == 1;
   c->stage = 4;
+  if (whole) plan_make(c);  // from this evaluation's sizes (already on the host)
   return TFMM_OK;
 }
 
@@ -1546,6 +2112,7 @@ int tfmm_get_stats(tfmm_ctx *c, tfmm_stats_t *o) {
   o->nlevels = c->stage >= 1 ? (int32_t)c->lvl.size() - 1 : 0;
   o->max_depth = o->nlevels - 1;
   o->n_problems = c->nprob;
+  o->one_graph = c->one_graph && c->stage >= 4 ? 1 : 0;
   for (int d = 0; d < 3; d++) {
     o->region_min[d] = c->h_region[d];
     o->region_max[d] = c->h_region[3 + d];

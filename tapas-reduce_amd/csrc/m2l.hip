@@ -23,11 +23,13 @@ constexpr int PAIR_WAVES = 4;
 __global__ __launch_bounds__(64 * PAIR_WAVES) void k_m2l_pairs_fp64(Cells c, Csr list,
                                                                    int32_t p0, int32_t npairs,
                                                                    const int32_t *tgt,
-                                                                   const double *M, cd *part) {
+                                                                   const double *M, cd *part,
+                                                                   const uint32_t *dn) {
   __shared__ cd Ys[PAIR_WAVES][NTERM];
   __shared__ cd Ms[PAIR_WAVES][NTERM];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int32_t pl = blockIdx.x * PAIR_WAVES + w;  // pair within the chunk
+  if (dn) npairs = min(npairs, (int32_t)min(*dn, (uint32_t)INT32_MAX) - p0);
   if (pl >= npairs) return;  // whole wave leaves together; no block barrier follows
   const int32_t pi = p0 + pl;
   const int32_t ti = tgt[pi], sj = list.src[pi];
@@ -67,11 +69,12 @@ __global__ __launch_bounds__(64 * PAIR_WAVES) void k_m2l_pairs_fp64(Cells c, Csr
   part[(int64_t)pl * NTERM + lane] = acc;
 }
 
-// target of every list entry
-__global__ void k_list_targets(const int32_t *off, int32_t ncells, int32_t *tgt) {
+// target of every list entry (of the first npairs)
+__global__ void k_list_targets(const int32_t *off, int32_t ncells, int32_t npairs, int32_t *tgt) {
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ncells) return;
-  for (int32_t e = off[t]; e < off[t + 1]; e++) tgt[e] = t;
+  const int32_t e1 = min(off[t + 1], npairs);
+  for (int32_t e = off[t]; e < e1; e++) tgt[e] = t;
 }
 
 // L_t += partials of t's pairs within the chunk [p0, p1), in list order; one
@@ -101,16 +104,16 @@ size_t m2l_pairs_scratch_bytes(int32_t npairs) {
 // scratch: m2l_pairs_scratch_bytes(npairs) bytes (partials of one chunk of
 // pairs, then the target of every pair); chunks are added in list order
 hipError_t launch_m2l_fp64(Cells c, int32_t ncells, Csr list, int32_t npairs, const double *M,
-                           double *L, void *scratch, hipStream_t s) {
+                           double *L, void *scratch, hipStream_t s, const uint32_t *dn) {
   if (ncells <= 0 || npairs <= 0) return hipSuccess;
   const int32_t chunk = std::min(npairs, PAIR_CHUNK);
   cd *part = static_cast<cd *>(scratch);
   int32_t *tgt = reinterpret_cast<int32_t *>(part + (size_t)chunk * NTERM);
-  k_list_targets<<<(ncells + 255) / 256, 256, 0, s>>>(list.off, ncells, tgt);
+  k_list_targets<<<(ncells + 255) / 256, 256, 0, s>>>(list.off, ncells, npairs, tgt);
   for (int32_t p0 = 0; p0 < npairs; p0 += chunk) {
     const int32_t n = std::min(chunk, npairs - p0);
     k_m2l_pairs_fp64<<<(n + PAIR_WAVES - 1) / PAIR_WAVES, 64 * PAIR_WAVES, 0, s>>>(
-        c, list, p0, n, tgt, M, part);
+        c, list, p0, n, tgt, M, part, dn);
     k_m2l_reduce<<<(ncells + 3) / 4, 256, 0, s>>>(list, ncells, p0, p0 + n, part, L);
   }
   return hipGetLastError();

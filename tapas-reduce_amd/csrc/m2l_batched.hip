@@ -112,13 +112,17 @@ __device__ inline void slot_decode(int32_t slot, int &lt, int &ls, int o[3]) {
 // target | source).  Within one (tile, group) segment every target occurs at
 // most once, so the order inside a segment does not affect any sum and the
 // key needs no slot bits.
+// With dn the pair count is *dn (n its capacity), and group indices are
+// clamped below gcap (a capacity the caller checks after the launch).
 __global__ __launch_bounds__(1024) void k_classify(const int2 *pairs, const int32_t *slot_of,
                                                   uint32_t n, const uint32_t *gid_of_slot, int gbits,
                                                   int sbits, uint32_t *gkeys, uint32_t *gvals,
                                                   uint64_t *fkeys, uint32_t *fvals,
-                                                  uint32_t *counters) {
+                                                  uint32_t *counters, const uint32_t *dn,
+                                                  uint32_t gcap) {
   __shared__ uint32_t scratch[17];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (dn) n = min(n, *dn);
   const bool valid = i < n;
   int2 p = make_int2(0, 0);
   int32_t slot = -1;
@@ -131,7 +135,7 @@ __global__ __launch_bounds__(1024) void k_classify(const int2 *pairs, const int3
   const uint32_t at = block_append<16>(&counters[0], grouped, 1, scratch);
   if (grouped) {
     const uint64_t tile = (uint32_t)p.x / TILE, in_tile = (uint32_t)p.x % TILE;
-    gkeys[at] = (uint32_t)(tile << gbits) | gid_of_slot[slot];
+    gkeys[at] = (uint32_t)(tile << gbits) | min(gid_of_slot[slot], gcap - 1u);
     gvals[at] = (uint32_t)p.y | (uint32_t)(in_tile << SRC_BITS);
   }
   const uint32_t af = block_append<16>(&counters[1], fallback, 1, scratch);
@@ -159,10 +163,11 @@ __global__ void k_classify_all(const int2 *pairs, const int32_t *slot_of, uint32
 // in FP64 and FP32.
 __global__ void k_group_harmonics(const uint32_t *used, const uint32_t *gid_of_slot,
                                   int32_t nslots, double *Y64, float *Y32, int8_t *gdl,
-                                  int8_t *glt) {
+                                  int8_t *glt, uint32_t gcap) {
   const int32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= nslots || !used[slot]) return;
   const uint32_t g = gid_of_slot[slot];
+  if (g >= gcap) return;  // more groups than the buffers hold (checked by the caller)
   int lt, ls, o[3];
   slot_decode(slot, lt, ls, o);
   const int lf = lt > ls ? lt : ls;
@@ -232,10 +237,12 @@ constexpr int IP_WAVES = 16;
 constexpr int IP_ROWS = IP_WAVES + 3;  // 2 rows before the block's, 1 after
 
 __global__ __launch_bounds__(64 * IP_WAVES) void k_item_pos(const uint32_t *keys, uint32_t n,
-                                                            uint32_t *ipos, uint32_t *bcount) {
+                                                            uint32_t *ipos, uint32_t *bcount,
+                                                            const uint32_t *dn) {
   __shared__ uint32_t sk[IP_ROWS * 64];
   __shared__ uint64_t mark[IP_ROWS];
   __shared__ uint32_t wcount[IP_WAVES];
+  if (dn) n = min(n, *dn);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int64_t w0 = (int64_t)blockIdx.x * (64 * IP_WAVES) - 128;  // key of window slot 0
   for (int t = threadIdx.x; t < IP_ROWS * 64; t += 64 * IP_WAVES) {
@@ -286,15 +293,19 @@ __global__ __launch_bounds__(64 * IP_WAVES) void k_item_pos(const uint32_t *keys
 // Every column carries the group, so that every lane of the tile kernel can
 // fetch the group's data.
 // Blocks as in k_item_pos; bstart: exclusive prefix of its block counts.
+// With dn the pair count is *dn (n its capacity); items at or past
+// items_cap are not written (a capacity the caller checks).
 __global__ __launch_bounds__(64 * IP_WAVES) void k_items(const uint32_t *keys,
                                                          const uint32_t *vals,
                                                          const uint32_t *ipos,
                                                          const uint32_t *bstart, uint32_t n,
                                                          int gbits, const int8_t *gdl,
-                                                         int4 *items, uint2 *cols) {
+                                                         int4 *items, uint2 *cols,
+                                                         const uint32_t *dn, uint32_t items_cap) {
   __shared__ uint32_t wcount[IP_WAVES];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (dn) n = min(n, *dn);
   const bool valid = i < n;
   const uint32_t ip = valid ? ipos[i] : 1u;
   const bool first = valid && (ip & 15u) == 0;
@@ -311,6 +322,7 @@ __global__ __launch_bounds__(64 * IP_WAVES) void k_items(const uint32_t *keys,
   const uint32_t k = keys[i], v = vals[i];
   const uint32_t g = k & ((1u << gbits) - 1u);
   const uint32_t gw = (uint32_t)(gdl[g] + DL_MAX) << COL_DL | (g << COL_G);
+  if (item >= items_cap) return;
   cols[(int64_t)item * 16 + jj] =
       make_uint2(v & ((1u << SRC_BITS) - 1u), (v >> SRC_BITS) | 128u | gw);
   if (first) {  // first pair of the item: descriptor and padding columns
@@ -326,9 +338,10 @@ __global__ __launch_bounds__(64 * IP_WAVES) void k_items(const uint32_t *keys,
 }
 
 __global__ void k_tile_offsets(const int4 *items, uint32_t nitems, int32_t ntiles,
-                               int32_t *off) {
+                               int32_t *off, const uint32_t *dnitems) {
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t > ntiles) return;
+  if (dnitems) nitems = min(nitems, *dnitems);
   uint32_t lo = 0, hi = nitems;
   while (lo < hi) {
     const uint32_t mid = lo + ((hi - lo) >> 1);
@@ -573,18 +586,20 @@ __device__ inline int tile_of_block(int b, int ntiles, int run) {
 // which k_unit_combine adds in unit order after the launch, so the sums do
 // not depend on timing.
 __global__ void k_unit_counts(const int32_t *tile_off, int32_t ntiles, int32_t chunk,
-                              uint32_t *ucnt) {
+                              uint32_t *ucnt, const int32_t *dchunk) {
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
+  if (dchunk) chunk = *dchunk;
   const int32_t n = tile_off[t + 1] - tile_off[t];
   ucnt[t] = n > chunk ? (uint32_t)((n + chunk - 1) / chunk) : 1u;
 }
 
 __global__ void k_unit_map(const uint32_t *ucnt, const uint32_t *uoff, int32_t ntiles,
-                           int2 *units) {
+                           int2 *units, uint32_t units_cap) {
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
-  for (uint32_t k = 0; k < ucnt[t]; k++) units[uoff[t] + k] = make_int2(t, (int)k);
+  for (uint32_t k = 0; k < ucnt[t] && uoff[t] + k < units_cap; k++)
+    units[uoff[t] + k] = make_int2(t, (int)k);
 }
 
 // slab of extra unit k >= 1 of tile t: every tile has at least one unit
@@ -592,10 +607,13 @@ __device__ inline int64_t extra_slab(const uint32_t *uoff, int32_t t, int k) {
   return (int64_t)uoff[t] - t + k - 1;
 }
 
+// nslab: slabs the part buffer holds (units past them were not run; the
+// caller checks the unit count against its capacity)
 __global__ void k_unit_combine(const uint32_t *ucnt, const uint32_t *uoff, int32_t ntiles,
-                               const double *part, int32_t ncells, double *L) {
+                               const double *part, int32_t ncells, double *L, int64_t nslab) {
   const int32_t t = blockIdx.x;
-  const int n = (int)ucnt[t];
+  int n = (int)ucnt[t];
+  while (n > 1 && extra_slab(uoff, t, n - 1) >= nslab) n--;
   if (n <= 1) return;
   const int32_t first = t * TILE;
   for (int e = threadIdx.x; e < TILE * 110; e += blockDim.x) {
@@ -623,8 +641,9 @@ __device__ inline T table_elem(const T *y, uint32_t byte_off) {
 
 template <class T>
 __global__ void k_group_afrag(const T *Yg, uint32_t ng, const uint32_t *codes, const int8_t *gdl,
-                              T *Af) {
+                              T *Af, const uint32_t *dng) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (dng) ng = min(ng, *dng);
   if (t >= (int64_t)ng * (NBLK * 64)) return;
   const int64_t g = t / (NBLK * 64);
   const int r = (int)(t - g * (NBLK * 64));
@@ -651,7 +670,7 @@ template <class T>
 __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
     const int32_t *tile_off, const int2 *units, const uint32_t *ucnt, const uint32_t *uoff,
     int32_t ntiles, int32_t chunk, int xcd_run, const uint2 *cols, const T *Af, const T *Mq,
-    Cells c, int32_t ncells, double *L, double *part) {
+    Cells c, int32_t ncells, double *L, double *part, const int32_t *dchunk, int64_t nslab) {
   __shared__ T Lt[TILE][NR + 1];
   __shared__ T Lt0b[TILE][17];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -659,6 +678,8 @@ __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
   if ((uint32_t)u >= uoff[ntiles - 1] + ucnt[ntiles - 1]) return;  // grid is an upper bound
   const int2 tu = units[u];
   const int tile = tu.x, k = tu.y;
+  if (k > 0 && extra_slab(uoff, tile, k) >= nslab) return;  // no slab (capacity exceeded)
+  if (dchunk) chunk = *dchunk;
   for (int i = threadIdx.x; i < TILE * (NR + 1); i += 64 * TILE_WAVES) (&Lt[0][0])[i] = 0;
   for (int i = threadIdx.x; i < TILE * 17; i += 64 * TILE_WAVES) (&Lt0b[0][0])[i] = 0;
   __syncthreads();
@@ -745,19 +766,21 @@ hipError_t launch_m2l_classify_all(const int2 *pairs, const int32_t *slot_of, ui
 hipError_t launch_m2l_classify(const int2 *pairs, const int32_t *slot_of, uint32_t n,
                                const uint32_t *gid_of_slot, int gbits, int sbits,
                                uint32_t *gkeys, uint32_t *gvals, uint64_t *fkeys,
-                               uint32_t *fvals, uint32_t *counters, hipStream_t s) {
+                               uint32_t *fvals, uint32_t *counters, hipStream_t s,
+                               const uint32_t *dn, uint32_t gcap) {
   if (n == 0) return hipSuccess;
   k_classify<<<(n + 1023) / 1024, 1024, 0, s>>>(pairs, slot_of, n, gid_of_slot, gbits, sbits,
-                                                  gkeys, gvals, fkeys, fvals, counters);
+                                                  gkeys, gvals, fkeys, fvals, counters, dn,
+                                                  gcap);
   return hipGetLastError();
 }
 
 hipError_t launch_m2l_group_harmonics(const uint32_t *used, const uint32_t *gid_of_slot,
                                       int32_t nslots, double *Y64, float *Y32, int8_t *gdl,
-                                      int8_t *glt, hipStream_t s) {
+                                      int8_t *glt, hipStream_t s, uint32_t gcap) {
   if (nslots <= 0) return hipSuccess;
   k_group_harmonics<<<(nslots + 127) / 128, 128, 0, s>>>(used, gid_of_slot, nslots, Y64, Y32,
-                                                         gdl, glt);
+                                                         gdl, glt, gcap);
   return hipGetLastError();
 }
 
@@ -775,7 +798,7 @@ hipError_t launch_m2l_count_items(const uint32_t *keys, uint32_t n, uint32_t *ip
   *nitems_out = 0;
   if (n == 0) return hipSuccess;
   const unsigned per = 64 * IP_WAVES, nb = (n + per - 1) / per;
-  k_item_pos<<<nb, per, 0, s>>>(keys, n, ipos, bcount);
+  k_item_pos<<<nb, per, 0, s>>>(keys, n, ipos, bcount, nullptr);
   hipError_t e = scan_exclusive_u32(bcount, bstart, nb, scan_tmp, s);
   if (e != hipSuccess) return e;
   e = hipMemcpyAsync(h_pinned, bstart + (nb - 1), 4, hipMemcpyDeviceToHost, s);
@@ -788,19 +811,55 @@ hipError_t launch_m2l_count_items(const uint32_t *keys, uint32_t n, uint32_t *ip
   return hipGetLastError();
 }
 
+// the item count on the device: the block counts of k_item_pos summed
+__global__ void k_items_total(const uint32_t *bcount, const uint32_t *bstart, uint32_t nb,
+                              uint32_t *nitems) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *nitems = bstart[nb - 1] + bcount[nb - 1];
+}
+
+hipError_t launch_m2l_count_items_dev(const uint32_t *keys, uint32_t n, const uint32_t *dn,
+                                      uint32_t *ipos, uint32_t *bcount, uint32_t *bstart,
+                                      void *scan_tmp, uint32_t *nitems, hipStream_t s) {
+  const unsigned per = 64 * IP_WAVES, nb = (std::max<uint32_t>(n, 1) + per - 1) / per;
+  k_item_pos<<<nb, per, 0, s>>>(keys, n, ipos, bcount, dn);
+  hipError_t e = scan_exclusive_u32(bcount, bstart, nb, scan_tmp, s);
+  if (e != hipSuccess) return e;
+  k_items_total<<<1, 64, 0, s>>>(bcount, bstart, nb, nitems);
+  return hipGetLastError();
+}
+
 hipError_t launch_m2l_write_items(const uint32_t *keys, const uint32_t *vals, uint32_t n,
                                   int gbits, const uint32_t *ipos, const uint32_t *bstart,
-                                  const int8_t *gdl, int4 *items, uint2 *cols, hipStream_t s) {
+                                  const int8_t *gdl, int4 *items, uint2 *cols, hipStream_t s,
+                                  const uint32_t *dn, uint32_t items_cap) {
   if (n == 0) return hipSuccess;
   const unsigned per = 64 * IP_WAVES;
   k_items<<<(n + per - 1) / per, per, 0, s>>>(keys, vals, ipos, bstart, n, gbits, gdl, items,
-                                             cols);
+                                             cols, dn, items_cap);
   return hipGetLastError();
 }
 
 hipError_t launch_m2l_tile_offsets(const int4 *items, uint32_t nitems, int32_t ntiles,
-                                   int32_t *off, hipStream_t s) {
-  k_tile_offsets<<<(ntiles + 1 + 255) / 256, 256, 0, s>>>(items, nitems, ntiles, off);
+                                   int32_t *off, hipStream_t s, const uint32_t *dnitems) {
+  k_tile_offsets<<<(ntiles + 1 + 255) / 256, 256, 0, s>>>(items, nitems, ntiles, off, dnitems);
+  return hipGetLastError();
+}
+
+// m2l_unit_chunk on the device, from the item and tile counts there
+__global__ void k_unit_chunk(const uint32_t *nitems, const uint32_t *ntiles, double factor,
+                             int32_t *chunk) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (factor <= 0) {
+    *chunk = INT32_MAX / 2;
+    return;
+  }
+  const double mean = *ntiles > 0 ? (double)*nitems / (double)*ntiles : 1.0;
+  *chunk = (int32_t)fmin((double)(INT32_MAX / 2), fmax(64.0, ceil(mean * factor)));
+}
+
+hipError_t launch_m2l_unit_chunk(const uint32_t *nitems, const uint32_t *ntiles, double factor,
+                                 int32_t *chunk, hipStream_t s) {
+  k_unit_chunk<<<1, 64, 0, s>>>(nitems, ntiles, factor, chunk);
   return hipGetLastError();
 }
 
@@ -821,12 +880,12 @@ size_t m2l_unit_part_bytes(uint32_t units_bound, int32_t ntiles) {
 
 hipError_t launch_m2l_units(const int32_t *tile_off, int32_t ntiles, int32_t chunk,
                             uint32_t *ucnt, uint32_t *uoff, int2 *units, void *scan_tmp,
-                            hipStream_t s) {
+                            hipStream_t s, uint32_t units_cap, const int32_t *dchunk) {
   if (ntiles <= 0) return hipSuccess;
-  k_unit_counts<<<(ntiles + 255) / 256, 256, 0, s>>>(tile_off, ntiles, chunk, ucnt);
+  k_unit_counts<<<(ntiles + 255) / 256, 256, 0, s>>>(tile_off, ntiles, chunk, ucnt, dchunk);
   hipError_t e = scan_exclusive_u32(ucnt, uoff, ntiles, scan_tmp, s);
   if (e != hipSuccess) return e;
-  k_unit_map<<<(ntiles + 255) / 256, 256, 0, s>>>(ucnt, uoff, ntiles, units);
+  k_unit_map<<<(ntiles + 255) / 256, 256, 0, s>>>(ucnt, uoff, ntiles, units, units_cap);
   return hipGetLastError();
 }
 
@@ -835,15 +894,17 @@ size_t m2l_afrag_bytes(uint32_t ngroups, bool fp32) {
 }
 
 hipError_t launch_m2l_group_afrag(const void *Yg, uint32_t ngroups, const uint32_t *codes,
-                                  const int8_t *gdl, void *Af, bool fp32, hipStream_t s) {
+                                  const int8_t *gdl, void *Af, bool fp32, hipStream_t s,
+                                  const uint32_t *dng) {
   if (ngroups == 0) return hipSuccess;
   const int64_t n = (int64_t)ngroups * NBLK * 64;
   const unsigned nb = (unsigned)((n + 255) / 256);
   if (fp32)
-    k_group_afrag<float><<<nb, 256, 0, s>>>((const float *)Yg, ngroups, codes, gdl, (float *)Af);
+    k_group_afrag<float><<<nb, 256, 0, s>>>((const float *)Yg, ngroups, codes, gdl, (float *)Af,
+                                            dng);
   else
     k_group_afrag<double><<<nb, 256, 0, s>>>((const double *)Yg, ngroups, codes + NCODE, gdl,
-                                             (double *)Af);
+                                             (double *)Af, dng);
   return hipGetLastError();
 }
 
@@ -851,19 +912,21 @@ hipError_t launch_m2l_tiles(const int32_t *tile_off, const int2 *units, const ui
                             const uint32_t *uoff, int32_t ntiles, int32_t chunk, int xcd_run,
                             uint32_t units_bound, const uint2 *cols, const void *Af,
                             const void *Mq, Cells c, int32_t ncells, double *L, double *part,
-                            bool fp32, hipStream_t s) {
+                            bool fp32, hipStream_t s, const int32_t *dchunk) {
   if (ntiles <= 0) return hipSuccess;
+  // slabs of the part buffer (m2l_unit_part_bytes)
+  const int64_t nslab = units_bound > (uint32_t)ntiles ? (int64_t)units_bound - ntiles : 0;
   if (fp32)
     k_m2l_tiles<float><<<units_bound, 64 * TILE_WAVES, 0, s>>>(
         tile_off, units, ucnt, uoff, ntiles, chunk, xcd_run, cols, (const float *)Af,
-        (const float *)Mq, c, ncells, L, part);
+        (const float *)Mq, c, ncells, L, part, dchunk, nslab);
   else
     k_m2l_tiles<double><<<units_bound, 64 * TILE_WAVES, 0, s>>>(
         tile_off, units, ucnt, uoff, ntiles, chunk, xcd_run, cols, (const double *)Af,
-        (const double *)Mq, c, ncells, L, part);
+        (const double *)Mq, c, ncells, L, part, dchunk, nslab);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_unit_combine<<<ntiles, 256, 0, s>>>(ucnt, uoff, ntiles, part, ncells, L);
+  k_unit_combine<<<ntiles, 256, 0, s>>>(ucnt, uoff, ntiles, part, ncells, L, nslab);
   return hipGetLastError();
 }
 

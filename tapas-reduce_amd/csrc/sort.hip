@@ -105,8 +105,9 @@ template <class K>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const K *__restrict__ keys, int64_t n,
                                                            int shift,
                                                            uint32_t mask, uint32_t *counts,
-                                                           int nblocks) {
+                                                           int nblocks, const uint32_t *dn) {
   __shared__ uint32_t hist[RS_BINS];
+  if (dn) n = min(n, (int64_t)*dn);  // the count on the device, at most the capacity n
   for (int i = threadIdx.x; i < RS_BINS; i += RS_THREADS) hist[i] = 0;
   __syncthreads();
   const int64_t tile = (int64_t)blockIdx.x * RS_TILE;
@@ -123,7 +124,7 @@ template <class K>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const K *__restrict__ keys_in, const uint32_t *__restrict__ vals_in, K *__restrict__ keys_out,
     uint32_t *__restrict__ vals_out, int64_t n, int shift, int nbits,
-    const uint32_t *__restrict__ offsets, int nblocks) {
+    const uint32_t *__restrict__ offsets, int nblocks, const uint32_t *dn) {
   // The wave histograms are dead once every key knows its place in the tile,
   // so the tile's staging arrays reuse their LDS (more blocks per CU).
   constexpr size_t HIST_BYTES = sizeof(uint32_t) * RS_WAVES * RS_BINS;
@@ -136,6 +137,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
   __shared__ uint32_t gshift[RS_BINS];  // global position of the tile's run of digit d, less dstart[d]
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const uint32_t mask = (1u << nbits) - 1u;
+  if (dn) n = min(n, (int64_t)*dn);
   for (int i = lane; i < RS_BINS; i += 64) whist[wid][i] = 0;
   __syncthreads();
 
@@ -213,7 +215,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
   }
   __syncthreads();
   const int64_t tbase = (int64_t)blockIdx.x * RS_TILE;
-  const int tn = (int)((n - tbase) < RS_TILE ? (n - tbase) : RS_TILE);
+  const int tn = (int)((n - tbase) < RS_TILE ? (n - tbase) : RS_TILE);  // <= 0 past the count
   for (int i = threadIdx.x; i < tn; i += RS_THREADS) {
     const K key = sk[i];
     const uint32_t d = digit_of(key, shift, mask);
@@ -305,12 +307,14 @@ size_t radix_tmp_bytes(int64_t n) {
 
 // Sorts the bits [bit_lo, bit_hi) of the keys with as few passes of at most
 // RS_MAXBITS bits as possible, digit widths balanced.
+// With dn the number of pairs is read on the device (*dn, at most the
+// capacity n): grids and passes depend on n and the bit range only.
 template <class K>
 static hipError_t radix_sort_impl(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt,
                                   int64_t n, int bit_lo, int bit_hi, void *tmp, hipStream_t s,
-                                  bool *result_in_alt) {
+                                  bool *result_in_alt, const uint32_t *dn) {
   *result_in_alt = false;
-  if (n <= 1 || bit_hi <= bit_lo) return hipSuccess;
+  if ((n <= 1 && !dn) || n <= 0 || bit_hi <= bit_lo) return hipSuccess;
   const int nb = (int)ceil_div(n, RS_TILE);
   const int64_t ncount = (int64_t)nb * RS_BINS;
   uint32_t *counts = reinterpret_cast<uint32_t *>(tmp);
@@ -326,10 +330,11 @@ static hipError_t radix_sort_impl(K *keys, uint32_t *vals, K *keys_alt, uint32_t
     const int nbits = (bit_hi - lo + (passes - ps) - 1) / (passes - ps);  // balanced widths
     const uint32_t mask = (1u << nbits) - 1u;
     const int64_t nc = (int64_t)nb * (mask + 1);
-    k_radix_hist<K><<<nb, RS_THREADS, 0, s>>>(kin, n, lo, mask, counts, nb);
+    k_radix_hist<K><<<nb, RS_THREADS, 0, s>>>(kin, n, lo, mask, counts, nb, dn);
     hipError_t e = scan_exclusive_u32(counts, offs, nc, stmp, s);
     if (e != hipSuccess) return e;
-    k_radix_scatter<K><<<nb, RS_THREADS, 0, s>>>(kin, vin, kout, vout, n, lo, nbits, offs, nb);
+    k_radix_scatter<K><<<nb, RS_THREADS, 0, s>>>(kin, vin, kout, vout, n, lo, nbits, offs, nb,
+                                                  dn);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     K *tk = kin; kin = kout; kout = tk;
@@ -342,16 +347,16 @@ static hipError_t radix_sort_impl(K *keys, uint32_t *vals, K *keys_alt, uint32_t
 
 hipError_t radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
                             uint32_t *vals_alt, int64_t n, int bit_lo, int bit_hi, void *tmp,
-                            hipStream_t s, bool *result_in_alt) {
+                            hipStream_t s, bool *result_in_alt, const uint32_t *dn) {
   return radix_sort_impl<uint64_t>(keys, vals, keys_alt, vals_alt, n, bit_lo, bit_hi, tmp, s,
-                                   result_in_alt);
+                                   result_in_alt, dn);
 }
 
 hipError_t radix_sort_pairs32(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt,
                               uint32_t *vals_alt, int64_t n, int bit_lo, int bit_hi, void *tmp,
-                              hipStream_t s, bool *result_in_alt) {
+                              hipStream_t s, bool *result_in_alt, const uint32_t *dn) {
   return radix_sort_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, bit_lo, bit_hi, tmp, s,
-                                   result_in_alt);
+                                   result_in_alt, dn);
 }
 
 }  // namespace tfmm

@@ -37,14 +37,15 @@ size_t scan_tmp_bytes(int64_t n);
 hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, void *tmp,
                               hipStream_t s);
 
-// stable LSD radix sort of (key, value) pairs on key bits [bit_lo, bit_hi)
+// stable LSD radix sort of (key, value) pairs on key bits [bit_lo, bit_hi);
+// with dn, the number of pairs is *dn on the device and n its capacity
 size_t radix_tmp_bytes(int64_t n);
 hipError_t radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
                             uint32_t *vals_alt, int64_t n, int bit_lo, int bit_hi, void *tmp,
-                            hipStream_t s, bool *result_in_alt);
+                            hipStream_t s, bool *result_in_alt, const uint32_t *dn = nullptr);
 hipError_t radix_sort_pairs32(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt,
                               uint32_t *vals_alt, int64_t n, int bit_lo, int bit_hi, void *tmp,
-                              hipStream_t s, bool *result_in_alt);
+                              hipStream_t s, bool *result_in_alt, const uint32_t *dn = nullptr);
 
 }  // namespace tfmm
 

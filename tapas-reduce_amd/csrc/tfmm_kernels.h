@@ -31,6 +31,34 @@ struct Csr {
   const int32_t *src;
 };
 
+// Sizes of an evaluation recorded as one graph (plan.hip): words of the
+// device sizes buffer, and the status bits of the capacities it exceeded.
+enum : int {
+  SZ_STATUS = 0,
+  SZ_NCELLS,
+  SZ_NLEAVES,
+  SZ_DEPTH,
+  SZ_NTILES,
+  SZ_NG,
+  SZ_NITEMS,
+  SZ_CHUNK,
+  SZ_NUNITS,
+  SZ_WORDS = 16
+};
+enum : uint32_t {
+  ST_CELLS = 1,       // cell capacity (level steps)
+  ST_LEVEL_GRID = 2,  // a level larger than its grid (level steps)
+  ST_DEPTH = 4,       // a tree deeper than the planned levels
+  ST_TRAVERSAL = 8,   // lists, frontiers or step grids of the traversal
+  ST_GROUPS = 16,
+  ST_ITEMS = 32,
+  ST_UNITS = 64,
+  ST_FALLBACK = 128
+};
+struct PlanCaps {
+  uint32_t m2l, p2p, front, items, units, fallback;
+};
+
 // tree.hip
 // *flag |= 1 when any x, y, z or q of the n bodies is not finite
 hipError_t launch_check_finite(const double *x, const double *y, const double *z,
@@ -59,10 +87,20 @@ hipError_t launch_roots(Cells c, int64_t n, const int64_t *off, int nprob,
 // count per level); grid_cells bounds the level's size; status bit 1: cell
 // capacity cap exceeded, bit 2: level larger than grid_cells
 int32_t level_blocks(int32_t cells);
+// last: the planned tree has no level after d (splits there set ST_DEPTH)
 hipError_t launch_level_step(Cells c, int32_t *lev, int d, int32_t grid_cells, int ncrit,
                              int32_t cap, uint32_t *flag, uint32_t *loc, uint32_t *bsum,
                              uint32_t *status, const uint64_t *keys, const double *regions,
-                             const uint32_t *sprob, hipStream_t s);
+                             const uint32_t *sprob, hipStream_t s, bool last = false);
+// plan.hip
+hipError_t launch_tree_final(int32_t *lev, int L, const uint32_t *lev_status, int32_t tile,
+                             uint32_t *sz, hipStream_t s);
+hipError_t launch_pad_cells(Cells c, const uint32_t *sz, int32_t cap, hipStream_t s);
+hipError_t launch_groups_count(const uint32_t *gid, const uint32_t *used, int32_t nslots,
+                               uint32_t gcap, uint32_t *sz, hipStream_t s);
+hipError_t launch_plan_check(const uint32_t *cnt, int nsteps, const uint32_t *step_grid,
+                             const PlanCaps &caps, const uint32_t *gcnt, const uint32_t *ucnt,
+                             const uint32_t *uoff, int32_t ntiles, uint32_t *sz, hipStream_t s);
 hipError_t launch_split_flags(Cells c, int32_t lbeg, int32_t lcnt, int ncrit, uint32_t *flag,
                               hipStream_t s);
 hipError_t launch_children(Cells c, int32_t lbeg, int32_t lcnt, const uint32_t *flag,
@@ -76,10 +114,12 @@ hipError_t launch_p2m(Cells c, int32_t ncells, Bodies b, double *M, hipStream_t 
 // translate.hip (FP64 M2M / L2L on MFMA)
 size_t translation_ops_doubles();
 hipError_t launch_translation_ops(double *ops, hipStream_t s);
+// one level of parents (npar, children from first_child); with lev the
+// children's level dl is read on the device and npar is the capacity
 hipError_t launch_m2m(Cells c, int32_t first_child, int32_t npar, const double *ops, double *M,
-                      hipStream_t s);
+                      hipStream_t s, const int32_t *lev = nullptr, int dl = 0);
 hipError_t launch_l2l(Cells c, int32_t first_child, int32_t npar, const double *ops, double *L,
-                      hipStream_t s);
+                      hipStream_t s, const int32_t *lev = nullptr, int dl = 0);
 hipError_t launch_l2p(Cells c, int32_t ncells, Bodies b, const double *L, const double *p2p,
                       const int32_t *perm, int32_t *leaf_of, double *trg, double *out,
                       hipStream_t s);
@@ -106,19 +146,23 @@ hipError_t launch_pair_slots(const int2 *pairs, uint32_t n, const double4 *geo,
                              const int32_t *depth, const double *width, int32_t *slot,
                              uint32_t *used, uint32_t *n_outside, hipStream_t s);
 // leaf-pair list: targets and sources of the pairs; then, from the
-// target-sorted list, the CSR offsets and every target's sources ascending
+// target-sorted list, the CSR offsets and every target's sources ascending.
+// With dn the pair count is *dn on the device (n its capacity).
 hipError_t launch_split_pairs(const int2 *pairs, uint32_t n, uint32_t *tgt, uint32_t *src,
-                              hipStream_t s);
+                              hipStream_t s, const uint32_t *dn = nullptr);
 hipError_t launch_csr_sorted_sources(const uint32_t *tgt, const uint32_t *in, uint32_t n,
-                                     int32_t ncells, int32_t *off, int32_t *out, hipStream_t s);
+                                     int32_t ncells, int32_t *off, int32_t *out, hipStream_t s,
+                                     const uint32_t *dn = nullptr);
 hipError_t launch_unpack_pairs(const uint64_t *keys, uint32_t n, int sbits, int32_t *src,
-                               hipStream_t s);
+                               hipStream_t s, const uint32_t *dn = nullptr);
 hipError_t launch_csr_offsets(const uint64_t *sorted_keys, uint32_t n, int sbits,
-                              int32_t ncells, int32_t *off, hipStream_t s);
+                              int32_t ncells, int32_t *off, hipStream_t s,
+                              const uint32_t *dn = nullptr);
 // m2l.hip
 size_t m2l_pairs_scratch_bytes(int32_t npairs);
+// with dn the pair count is *dn on the device (npairs its capacity)
 hipError_t launch_m2l_fp64(Cells c, int32_t ncells, Csr list, int32_t npairs, const double *M,
-                           double *L, void *scratch, hipStream_t s);
+                           double *L, void *scratch, hipStream_t s, const uint32_t *dn = nullptr);
 
 // p2p.hip
 hipError_t launch_p2p_fp64(Cells c, int32_t ncells, Csr list, Bodies b, double *acc,
@@ -144,13 +188,16 @@ size_t m2l_codes_bytes();
 hipError_t launch_m2l_classify_all(const int2 *pairs, const int32_t *slot_of, uint32_t n,
                                    const uint32_t *gid_of_slot, int gbits, uint32_t *gkeys,
                                    uint32_t *gvals, hipStream_t s);
+// With dn the pair count is *dn on the device (n its capacity); group
+// indices are clamped below gcap, groups at or past gcap get no harmonics.
 hipError_t launch_m2l_classify(const int2 *pairs, const int32_t *slot_of, uint32_t n,
                                const uint32_t *gid_of_slot, int gbits, int sbits,
                                uint32_t *gkeys, uint32_t *gvals, uint64_t *fkeys,
-                               uint32_t *fvals, uint32_t *counters, hipStream_t s);
+                               uint32_t *fvals, uint32_t *counters, hipStream_t s,
+                               const uint32_t *dn = nullptr, uint32_t gcap = UINT32_MAX);
 hipError_t launch_m2l_group_harmonics(const uint32_t *used, const uint32_t *gid_of_slot,
                                       int32_t nslots, double *Y64, float *Y32, int8_t *gdl,
-                                      int8_t *glt, hipStream_t s);
+                                      int8_t *glt, hipStream_t s, uint32_t gcap = UINT32_MAX);
 hipError_t launch_m2l_scale_m(Cells c, int32_t ncells, const double *M, void *Mq, bool fp32,
                               hipStream_t s);
 // items of the (tile, group)-sorted grouped pairs: every pair's position in
@@ -159,29 +206,44 @@ hipError_t launch_m2l_scale_m(Cells c, int32_t ncells, const double *M, void *Mq
 hipError_t launch_m2l_count_items(const uint32_t *keys, uint32_t n, uint32_t *ipos,
                                   uint32_t *bcount, uint32_t *bstart, void *scan_tmp,
                                   uint32_t *h_pinned, uint32_t *nitems_out, hipStream_t s);
+// the same with the pair count *dn on the device (n its capacity) and the
+// item count left on the device in *nitems
+hipError_t launch_m2l_count_items_dev(const uint32_t *keys, uint32_t n, const uint32_t *dn,
+                                      uint32_t *ipos, uint32_t *bcount, uint32_t *bstart,
+                                      void *scan_tmp, uint32_t *nitems, hipStream_t s);
+// items at or past items_cap are not written
 hipError_t launch_m2l_write_items(const uint32_t *keys, const uint32_t *vals, uint32_t n,
                                   int gbits, const uint32_t *ipos, const uint32_t *bstart,
-                                  const int8_t *gdl, int4 *items, uint2 *cols, hipStream_t s);
+                                  const int8_t *gdl, int4 *items, uint2 *cols, hipStream_t s,
+                                  const uint32_t *dn = nullptr, uint32_t items_cap = UINT32_MAX);
 hipError_t launch_m2l_tile_offsets(const int4 *items, uint32_t nitems, int32_t ntiles,
-                                   int32_t *off, hipStream_t s);
+                                   int32_t *off, hipStream_t s,
+                                   const uint32_t *dnitems = nullptr);
+// m2l_unit_chunk of the device's item and tile counts, into *chunk
+hipError_t launch_m2l_unit_chunk(const uint32_t *nitems, const uint32_t *ntiles, double factor,
+                                 int32_t *chunk, hipStream_t s);
 // work units of the tile kernel: runs of at most `chunk` items of one tile
 int32_t m2l_unit_chunk(uint32_t nitems, int32_t ntiles, double factor);
 uint32_t m2l_units_bound(uint32_t nitems, int32_t ntiles, int32_t chunk);
 size_t m2l_unit_part_bytes(uint32_t units_bound, int32_t ntiles);
+// units at or past units_cap are not written; with dchunk the chunk is read
+// on the device
 hipError_t launch_m2l_units(const int32_t *tile_off, int32_t ntiles, int32_t chunk,
                             uint32_t *ucnt, uint32_t *uoff, int2 *units, void *scan_tmp,
-                            hipStream_t s);
+                            hipStream_t s, uint32_t units_cap = UINT32_MAX,
+                            const int32_t *dchunk = nullptr);
 // A fragments of every group (the tile kernel's operand order, row factors
 // 2^(-j dl) included) from the group harmonics (FP32 or FP64), the operand
 // codes of both types and the groups' level differences
 size_t m2l_afrag_bytes(uint32_t ngroups, bool fp32);
 hipError_t launch_m2l_group_afrag(const void *Yg, uint32_t ngroups, const uint32_t *codes,
-                                  const int8_t *gdl, void *Af, bool fp32, hipStream_t s);
+                                  const int8_t *gdl, void *Af, bool fp32, hipStream_t s,
+                                  const uint32_t *dng = nullptr);
 hipError_t launch_m2l_tiles(const int32_t *tile_off, const int2 *units, const uint32_t *ucnt,
                             const uint32_t *uoff, int32_t ntiles, int32_t chunk, int xcd_run,
                             uint32_t units_bound, const uint2 *cols, const void *Af,
                             const void *Mq, Cells c, int32_t ncells, double *L, double *part,
-                            bool fp32, hipStream_t s);
+                            bool fp32, hipStream_t s, const int32_t *dchunk = nullptr);
 
 // p2p.hip (mixed precision)
 hipError_t launch_leaf_relpos(Cells c, int32_t ncells, Bodies b, float4 *rel, hipStream_t s);

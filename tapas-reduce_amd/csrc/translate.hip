@@ -20,6 +20,8 @@
 // formulas to unit coefficient vectors; the products run on
 // v_mfma_f64_16x16x4 in FP64.  (dX of the reference comes from the cell
 // centres, which equal sigma_p u_o up to rounding.)
+#include <algorithm>
+
 #include "tfmm_device.h"
 #include "tfmm_kernels.h"
 
@@ -180,11 +182,31 @@ __device__ inline void parent_powers(double (*f)[NV + 2], const double *sig, int
 // rb only takes child columns below m2m_kend(rb) (child degree <= parent
 // degree), L2L row block rb only parent columns from l2l_kbeg(rb) on
 // (parent degree >= child degree; rounded down to the k-step of 4).
+// With lev, the children's level dl is read on the device (first cell
+// lev[2 dl], lev[2 dl + 1] cells, 8 per parent) and the launch's npar is only
+// the capacity its grid was sized by.  Both variants of a level are then
+// launched and the level's size picks the one that runs, as the host picks
+// it from a known size: sel > 0 runs only for fewer than sel blocks of 16
+// parents, sel < 0 only for at least -sel.
+__device__ inline bool level_extent(const int32_t *lev, int dl, int sel, int32_t &first_child,
+                                    int32_t &npar) {
+  if (!lev) return true;
+  first_child = lev[2 * dl];
+  const int32_t act = lev[2 * dl + 1] >> 3;
+  const int32_t nb = (act + 15) / 16;
+  if ((sel > 0 && nb >= sel) || (sel < 0 && nb < -sel)) return false;
+  npar = min(npar, act);
+  return true;
+}
+
 // M2M of one level: columns are the level's parents with children, in the
 // order of their children (the t-th such parent has children at
 // first_child + 8 t); block = 16 parents, wave rb = row block rb.
 __global__ __launch_bounds__(7 * 64) void k_m2m_mfma(Cells c, int32_t first_child, int32_t npar,
-                                                     const double *Fm, double *M) {
+                                                     const double *Fm, double *M,
+                                                     const int32_t *lev, int dl, int sel) {
+  if (!level_extent(lev, dl, sel, first_child, npar)) return;
+  if ((int32_t)blockIdx.x * 16 >= npar) return;
   const int lane = threadIdx.x & 63, rb = threadIdx.x >> 6;
   const int col = lane & 15, kq = lane >> 4;
   __shared__ double sig[16];
@@ -259,7 +281,10 @@ __global__ __launch_bounds__(7 * 64) void k_m2m_mfma(Cells c, int32_t first_chil
 // 7 row blocks of the child's coefficients at once (one B fragment per
 // k-step feeds 7 independent accumulators).
 __global__ __launch_bounds__(8 * 64) void k_l2l_mfma(Cells c, int32_t first_child, int32_t npar,
-                                                     const double *Fl, double *L) {
+                                                     const double *Fl, double *L,
+                                                     const int32_t *lev, int dl, int sel) {
+  if (!level_extent(lev, dl, sel, first_child, npar)) return;
+  if ((int32_t)blockIdx.x * 16 >= npar) return;
   const int lane = threadIdx.x & 63, oct = threadIdx.x >> 6;
   const int col = lane & 15, kq = lane >> 4;
   __shared__ double sig[16];
@@ -324,7 +349,10 @@ __global__ __launch_bounds__(8 * 64) void k_l2l_mfma(Cells c, int32_t first_chil
 // wave walking all 28 k-steps of a row block waits out one load latency per
 // step, which on the upper levels (a handful of blocks) is the whole launch.
 __global__ __launch_bounds__(8 * 64) void k_m2m_small(Cells c, int32_t first_child, int32_t npar,
-                                                      const double *Fm, double *M) {
+                                                      const double *Fm, double *M,
+                                                      const int32_t *lev, int dl, int sel) {
+  if (!level_extent(lev, dl, sel, first_child, npar)) return;
+  if ((int32_t)blockIdx.x * 16 >= npar) return;
   const int lane = threadIdx.x & 63, o = threadIdx.x >> 6, rb = blockIdx.y;
   const int col = lane & 15, kq = lane >> 4;
   __shared__ double sig[16];
@@ -378,7 +406,10 @@ constexpr int L2L_KPW = (KS + L2L_SPLIT - 1) / L2L_SPLIT;  // k-steps per wave
 
 __global__ __launch_bounds__(L2L_SPLIT * 64) void k_l2l_small(Cells c, int32_t first_child,
                                                               int32_t npar, const double *Fl,
-                                                              double *L) {
+                                                              double *L, const int32_t *lev,
+                                                              int dl, int sel) {
+  if (!level_extent(lev, dl, sel, first_child, npar)) return;
+  if ((int32_t)blockIdx.x * 16 >= npar) return;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int oct = blockIdx.y, rb = blockIdx.z;
   const int col = lane & 15, kq = lane >> 4;
@@ -453,26 +484,39 @@ hipError_t launch_translation_ops(double *ops, hipStream_t s) {
 // parents of one level: those whose children are the cells
 // [first_child, first_child + 8 npar) of the next level
 hipError_t launch_m2m(Cells c, int32_t first_child, int32_t npar, const double *ops, double *M,
-                      hipStream_t s) {
+                      hipStream_t s, const int32_t *lev, int dl) {
   if (npar <= 0) return hipSuccess;
   const double *Fm = ops + (size_t)NV * KM + (size_t)8 * NV * NV;
   const int nb = (npar + 15) / 16;
-  if (nb < SMALL_BLOCKS_M2M)
-    k_m2m_small<<<dim3(nb, 7), 8 * 64, 0, s>>>(c, first_child, npar, Fm, M);
-  else
-    k_m2m_mfma<<<nb, 7 * 64, 0, s>>>(c, first_child, npar, Fm, M);
+  if (lev) {  // the level's size on the device picks the variant
+    k_m2m_small<<<dim3(std::min(nb, SMALL_BLOCKS_M2M - 1), 7), 8 * 64, 0, s>>>(
+        c, first_child, npar, Fm, M, lev, dl, SMALL_BLOCKS_M2M);
+    if (nb >= SMALL_BLOCKS_M2M)
+      k_m2m_mfma<<<nb, 7 * 64, 0, s>>>(c, first_child, npar, Fm, M, lev, dl, -SMALL_BLOCKS_M2M);
+  } else if (nb < SMALL_BLOCKS_M2M) {
+    k_m2m_small<<<dim3(nb, 7), 8 * 64, 0, s>>>(c, first_child, npar, Fm, M, lev, dl, 0);
+  } else {
+    k_m2m_mfma<<<nb, 7 * 64, 0, s>>>(c, first_child, npar, Fm, M, lev, dl, 0);
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_l2l(Cells c, int32_t first_child, int32_t npar, const double *ops, double *L,
-                      hipStream_t s) {
+                      hipStream_t s, const int32_t *lev, int dl) {
   if (npar <= 0) return hipSuccess;
   const double *Fl = ops + (size_t)NV * KM + (size_t)8 * NV * NV + FRAG_M;
   const int nb = (npar + 15) / 16;
-  if (nb < SMALL_BLOCKS_L2L)
-    k_l2l_small<<<dim3(nb, 8, 7), L2L_SPLIT * 64, 0, s>>>(c, first_child, npar, Fl, L);
-  else
-    k_l2l_mfma<<<nb, 8 * 64, 0, s>>>(c, first_child, npar, Fl, L);
+  if (lev) {  // the level's size on the device picks the variant
+    k_l2l_small<<<dim3(std::min(nb, SMALL_BLOCKS_L2L - 1), 8, 7), L2L_SPLIT * 64, 0, s>>>(
+        c, first_child, npar, Fl, L, lev, dl, SMALL_BLOCKS_L2L);
+    if (nb >= SMALL_BLOCKS_L2L)
+      k_l2l_mfma<<<nb, 8 * 64, 0, s>>>(c, first_child, npar, Fl, L, lev, dl, -SMALL_BLOCKS_L2L);
+  } else if (nb < SMALL_BLOCKS_L2L) {
+    k_l2l_small<<<dim3(nb, 8, 7), L2L_SPLIT * 64, 0, s>>>(c, first_child, npar, Fl, L, lev, dl,
+                                                          0);
+  } else {
+    k_l2l_mfma<<<nb, 8 * 64, 0, s>>>(c, first_child, npar, Fl, L, lev, dl, 0);
+  }
   return hipGetLastError();
 }
 

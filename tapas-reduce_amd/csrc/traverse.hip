@@ -264,17 +264,19 @@ __global__ void k_pair_slots(const int2 *pairs, uint32_t n, const double4 *geo,
   else atomicAdd(n_outside, 1u);
 }
 
-__global__ void k_unpack_pairs(const uint64_t *keys, uint32_t n, int sbits, int32_t *src) {
+__global__ void k_unpack_pairs(const uint64_t *keys, uint32_t n, int sbits, int32_t *src,
+                               const uint32_t *dn) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+  if (t >= n || (dn && t >= *dn)) return;
   src[t] = (int32_t)(keys[t] & ((1ull << sbits) - 1ull));
 }
 
 // off[t] = first entry whose target is >= t (t = 0..ncells)
 __global__ void k_csr_offsets(const uint64_t *keys, uint32_t n, int sbits, int32_t ncells,
-                              int32_t *off) {
+                              int32_t *off, const uint32_t *dn) {
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t > ncells) return;
+  if (dn) n = min(n, *dn);
   const uint64_t v = (uint64_t)(uint32_t)t << sbits;
   uint32_t lo = 0, hi = n;
   while (lo < hi) {
@@ -290,18 +292,21 @@ __global__ void k_csr_offsets(const uint64_t *keys, uint32_t n, int sbits, int32
 // (rank = number of smaller sources, ties by position).  The result is the
 // (target, source) order of a sort of the whole pair, with half the radix
 // passes.
-__global__ void k_split_pairs(const int2 *pairs, uint32_t n, uint32_t *tgt, uint32_t *src) {
+__global__ void k_split_pairs(const int2 *pairs, uint32_t n, uint32_t *tgt, uint32_t *src,
+                              const uint32_t *dn) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+  if (t >= n || (dn && t >= *dn)) return;
   const int2 p = pairs[t];
   tgt[t] = (uint32_t)p.x;
   src[t] = (uint32_t)p.y;
 }
 
 // off[t] = first entry whose target is >= t (t = 0..ncells)
-__global__ void k_csr_offsets32(const uint32_t *tgt, uint32_t n, int32_t ncells, int32_t *off) {
+__global__ void k_csr_offsets32(const uint32_t *tgt, uint32_t n, int32_t ncells, int32_t *off,
+                                const uint32_t *dn) {
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t > ncells) return;
+  if (dn) n = min(n, *dn);
   uint32_t lo = 0, hi = n;
   while (lo < hi) {
     const uint32_t mid = lo + ((hi - lo) >> 1);
@@ -357,17 +362,18 @@ __global__ __launch_bounds__(64) void k_seg_sort(const int32_t *off, const uint3
 }  // namespace
 
 hipError_t launch_split_pairs(const int2 *pairs, uint32_t n, uint32_t *tgt, uint32_t *src,
-                              hipStream_t s) {
+                              hipStream_t s, const uint32_t *dn) {
   if (n == 0) return hipSuccess;
-  k_split_pairs<<<(n + 255) / 256, 256, 0, s>>>(pairs, n, tgt, src);
+  k_split_pairs<<<(n + 255) / 256, 256, 0, s>>>(pairs, n, tgt, src, dn);
   return hipGetLastError();
 }
 
 // off (ncells + 1) from the target-sorted list; out: each target's sources
 // (in, in the target-sorted order) in ascending order
 hipError_t launch_csr_sorted_sources(const uint32_t *tgt, const uint32_t *in, uint32_t n,
-                                     int32_t ncells, int32_t *off, int32_t *out, hipStream_t s) {
-  k_csr_offsets32<<<(ncells + 1 + 255) / 256, 256, 0, s>>>(tgt, n, ncells, off);
+                                     int32_t ncells, int32_t *off, int32_t *out, hipStream_t s,
+                                     const uint32_t *dn) {
+  k_csr_offsets32<<<(ncells + 1 + 255) / 256, 256, 0, s>>>(tgt, n, ncells, off, dn);
   if (n > 0 && ncells > 0) k_seg_sort<<<ncells, 64, 0, s>>>(off, in, out);
   return hipGetLastError();
 }
@@ -402,15 +408,15 @@ hipError_t launch_pair_slots(const int2 *pairs, uint32_t n, const double4 *geo,
 }
 
 hipError_t launch_unpack_pairs(const uint64_t *keys, uint32_t n, int sbits, int32_t *src,
-                               hipStream_t s) {
+                               hipStream_t s, const uint32_t *dn) {
   if (n == 0) return hipSuccess;
-  k_unpack_pairs<<<(n + 255) / 256, 256, 0, s>>>(keys, n, sbits, src);
+  k_unpack_pairs<<<(n + 255) / 256, 256, 0, s>>>(keys, n, sbits, src, dn);
   return hipGetLastError();
 }
 
 hipError_t launch_csr_offsets(const uint64_t *sorted_keys, uint32_t n, int sbits,
-                              int32_t ncells, int32_t *off, hipStream_t s) {
-  k_csr_offsets<<<(ncells + 1 + 255) / 256, 256, 0, s>>>(sorted_keys, n, sbits, ncells, off);
+                              int32_t ncells, int32_t *off, hipStream_t s, const uint32_t *dn) {
+  k_csr_offsets<<<(ncells + 1 + 255) / 256, 256, 0, s>>>(sorted_keys, n, sbits, ncells, off, dn);
   return hipGetLastError();
 }
 

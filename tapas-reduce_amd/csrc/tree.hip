@@ -329,10 +329,11 @@ __global__ __launch_bounds__(LV_BLOCK) void k_level_flags(Cells c, const int32_t
 }
 
 // one block: exclusive scan of the block totals, the next level's extent;
-// status bit 1: cell capacity exceeded, bit 2: level larger than its grid
+// status bit 1: cell capacity exceeded, bit 2: level larger than its grid,
+// bit 4: splits on the last planned level
 __global__ __launch_bounds__(1024) void k_level_scan(int32_t *lev, int d, uint32_t *bsum,
                                                      int32_t nblk, int32_t cap,
-                                                     uint32_t *status) {
+                                                     uint32_t *status, int last) {
   const int32_t lbeg = lev[2 * d], lcnt = lev[2 * d + 1];
   const int32_t nb = min(nblk, (lcnt + LV_BLOCK - 1) / LV_BLOCK);
   __shared__ uint32_t part[1024];
@@ -362,6 +363,7 @@ __global__ __launch_bounds__(1024) void k_level_scan(int32_t *lev, int d, uint32
     uint32_t st = 0;
     if (lcnt > nblk * LV_BLOCK) st |= 2u;
     if (next_beg + next_cnt > cap) st |= 1u;
+    if (last && next_cnt > 0) st |= 4u;
     if (st) {
       atomicOr(status, st);
       next_cnt = 0;  // stop: this tree is built again on the host's path
@@ -471,11 +473,11 @@ int32_t level_blocks(int32_t cells) { return (cells + LV_BLOCK - 1) / LV_BLOCK; 
 hipError_t launch_level_step(Cells c, int32_t *lev, int d, int32_t grid_cells, int ncrit,
                              int32_t cap, uint32_t *flag, uint32_t *loc, uint32_t *bsum,
                              uint32_t *status, const uint64_t *keys, const double *regions,
-                             const uint32_t *sprob, hipStream_t s) {
+                             const uint32_t *sprob, hipStream_t s, bool last) {
   const int32_t nblk = level_blocks(grid_cells);
   if (nblk <= 0) return hipSuccess;
   k_level_flags<<<nblk, LV_BLOCK, 0, s>>>(c, lev, d, ncrit, flag, loc, bsum);
-  k_level_scan<<<1, 1024, 0, s>>>(lev, d, bsum, nblk, cap, status);
+  k_level_scan<<<1, 1024, 0, s>>>(lev, d, bsum, nblk, cap, status, last ? 1 : 0);
   const int64_t nt = (int64_t)nblk * LV_BLOCK * 8;
   k_level_children<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(c, lev, d, flag, loc, bsum, keys,
                                                                  regions, sprob);

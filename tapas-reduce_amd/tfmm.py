@@ -58,7 +58,7 @@ class Stats(ctypes.Structure):
                 ("ms_m2l", ctypes.c_double), ("ms_dtt", ctypes.c_double),
                 ("n_m2l_groups", ctypes.c_uint64), ("n_m2l_items", ctypes.c_uint64),
                 ("n_m2l_fallback", ctypes.c_uint64), ("ms_m2l_tiles", ctypes.c_double),
-                ("n_problems", ctypes.c_int64)]
+                ("n_problems", ctypes.c_int64), ("one_graph", ctypes.c_int64)]
 
     def as_dict(self):
         out = {}

@@ -62,7 +62,7 @@ def test_gpu_autotune(ctx):
 
 
 def test_gpu_invalid_knobs(ctx):
-    for knob, value in ((tfmm.TUNE_M2L_UNIT, -1.0), (tfmm.TUNE_M2L_XCD_RUN, 0), (tfmm.TUNE_GRAPH, 2), (tfmm.TUNE_P2P_WAVES, 9), (tfmm.TUNE_P2P_SCHEDULE, 3),
+    for knob, value in ((tfmm.TUNE_M2L_UNIT, -1.0), (tfmm.TUNE_M2L_XCD_RUN, 0), (tfmm.TUNE_GRAPH, 3), (tfmm.TUNE_P2P_WAVES, 9), (tfmm.TUNE_P2P_SCHEDULE, 3),
                         (tfmm.TUNE_M2L_XCD_RUN, 2.5),
                         (99, 1.0)):
         with pytest.raises(tfmm.TfmmError):
