@@ -217,10 +217,19 @@ int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p
  * batches are built) into one hipGraph and launches it, updating the
  * executable graph in place between evaluations; 0 (default) launches the
  * same kernels one by one (measured equal in time: the GPU is never short of
- * queued work).  With a HIP runtime that cannot record timing events inside
- * a capture, the per-phase times inside the graph read 0.
- * tfmm_upward / _traverse / _downward called separately always launch one
- * by one. */
+ * queued work).  2 makes every tfmm_evaluate one hipGraph launch of the
+ * whole evaluation (tree build, traversal, list sorts, M2L batching and the
+ * numeric phase): the first evaluation with given bodies' count, parameters
+ * and knobs runs the ordinary way and sets capacities (its sizes plus an
+ * eighth, one tree level more); later ones are one launch whose kernels read
+ * the evaluation's sizes on the device, with no host read-back before the
+ * end, where tfmm_evaluate waits for the launch and checks the capacities.
+ * An evaluation that exceeded one runs again the ordinary way and sets new
+ * capacities; the fields are those of the ordinary evaluation bit for bit
+ * either way (tfmm_stats_t.one_graph tells which ran).  With a HIP runtime
+ * that cannot record timing events inside a capture, the per-phase times
+ * inside a graph read 0.  tfmm_upward / _traverse / _downward called
+ * separately always launch one by one. */
 #define TFMM_TUNE_GRAPH 4
 /* TFMM_TUNE_P2P_WAVES: 0 (default) runs the FP32 P2P as one workgroup per
  * cell; k in 1..8 runs it as k resident waves per SIMD that take target
