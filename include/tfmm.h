@@ -150,6 +150,26 @@ int tfmm_batch_evaluate(tfmm_ctx *ctx, int nprob, const tfmm_body_desc *descs,
 /* number of problems; offsets (nprob + 1 entries) when not null */
 int tfmm_num_problems(const tfmm_ctx *ctx, int64_t *offsets);
 
+/* Periodic boundary conditions (ExaFMM's images, EX/include/traversal.h:
+ * 126-185 and 247-268; Tapas takes the Xperiodic argument of M2L and P2P,
+ * EX/examples/LaplaceSphericalCPU_tapas.h:209-220, LaplaceP2PCPU_tapas.h:50-54,
+ * and checks against the periodic direct sum, tapas_exafmm.cxx:288-345, but
+ * its traversal passes Xperiodic = 0, tapas_exafmm.cxx:224).  images = 0
+ * (default): none.  images >= 1: the root cell becomes the periodic cell, a
+ * cube of side cycle centred on the bodies' bounding cube (the bodies must
+ * span at most one period); the traversal starts from the root and each of
+ * its 27 images at (ix, iy, iz) cycle, ix, iy, iz in {-1, 0, 1}, with
+ * mutual = 0; for images >= 2 the copies farther out, up to the
+ * (3^images)^3 copies of the periodic cell around it, add their far field
+ * through multipoles of ever larger blocks of copies into the root.
+ * tfmm_direct then sums over the same (3^images)^3 copies (the reference's
+ * prange): 27^images times the work of the plain direct sum.  One problem
+ * only (not a batch), no mutual interaction, no
+ * recorded lists (tfmm_traverse_lists); TFMM_TUNE_GRAPH 2 evaluates the
+ * ordinary way.  Takes effect at the next tree build. */
+#define TFMM_PERIODIC_MAX_IMAGES 6
+int tfmm_set_periodic(tfmm_ctx *ctx, int images, double cycle);
+
 int tfmm_build_tree(tfmm_ctx *ctx, const tfmm_params *params);
 /* Change theta or the precision of the following phases without rebuilding
  * the tree (the reference passes theta to Map(FMM_DTT, ...), not to
@@ -193,7 +213,10 @@ int tfmm_get_cells(tfmm_ctx *ctx, uint64_t *key, int32_t *beg, int32_t *nb, int3
 /* M and L of every cell: ncells * 55 complex (re, im) doubles each. */
 int tfmm_get_expansions(tfmm_ctx *ctx, double *M, double *L);
 /* Interaction lists in CSR form over target cells: off has ncells + 1
- * entries; src the source cell of each entry (sorted by target, then source). */
+ * entries; src the source cell of each entry (sorted by target, then source).
+ * With periodic images a source word is cell | image << 27 (bits of a
+ * uint32), image 9 (ix + 1) + 3 (iy + 1) + (iz + 1) for the source seen at
+ * (ix, iy, iz) * cycle. */
 int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p2p_off,
                    int32_t *p2p_src);
 
@@ -257,7 +280,8 @@ int tfmm_autotune(tfmm_ctx *ctx, const tfmm_params *params, int reps, double *ch
 
 /* FP64 direct sum on the current bodies for the given target indices (caller
  * order), normalised by the target charge; out: nt * 4 doubles (host).  In
- * a batch each target sums over the bodies of its own problem. */
+ * a batch each target sums over the bodies of its own problem; with periodic
+ * images over the (3^images)^3 copies of the bodies (tfmm_set_periodic). */
 int tfmm_direct(tfmm_ctx *ctx, const int64_t *targets, int64_t nt, double *out);
 
 /* tfmm_direct without any device allocation.  No counterpart in the

@@ -457,6 +457,11 @@ struct orc_ctx {
   int record;
   int64_t *m2l_list, n_m2l_cap;
   int64_t *p2p_list, n_p2p_cap;
+  /* periodic images: the image the traversal is in (xp = its Xperiodic,
+   * img = its code 9 (ix+1) + 3 (iy+1) + (iz+1), recorded in the top bits
+   * of list sources as the library's lists hold it) */
+  double xp[3];
+  int64_t img;
 };
 
 static int64_t lower_bound_u64(const uint64_t *a, int64_t lo, int64_t hi, uint64_t v) {
@@ -543,9 +548,11 @@ static void push_pair(int64_t **list, int64_t *cap, uint64_t cnt, int64_t a, int
 }
 
 /* P2P functor on one body pair (LaplaceP2PCPU_tapas.h:48-74): target i, and
- * with mutual the source j too when the positions differ */
+ * with mutual the source j too when the positions differ; dX = Xi - Xj -
+ * Xperiodic (:54) */
 static void p2p_pair(orc_ctx *c, int64_t i, int64_t j, int mutual) {
-  double dX[3] = {c->x[i] - c->x[j], c->y[i] - c->y[j], c->z[i] - c->z[j]};
+  double dX[3] = {c->x[i] - c->x[j] - c->xp[0], c->y[i] - c->y[j] - c->xp[1],
+                  c->z[i] - c->z[j] - c->xp[2]};
   double R2 = 0;
   for (int d = 0; d < 3; d++) R2 += dX[d] * dX[d];
   R2 += 0.0; /* softening EPS2 = 0 */
@@ -579,11 +586,13 @@ static void p2p_leaves(orc_ctx *c, const ocell *ci, const ocell *cj) {
   }
 }
 
+/* FMM_DTT (tapas_exafmm.cxx:209-286), in the periodic image c->xp
+ * (EX/include/traversal.h:36-62: dX = Xi - Xj - Xperiodic) */
 static void dtt(orc_ctx *c, int64_t ii, int64_t jj) {
   const ocell *Ci = &c->cells[ii], *Cj = &c->cells[jj];
-  double d0 = Ci->center[0] - Cj->center[0];
-  double d1 = Ci->center[1] - Cj->center[1];
-  double d2 = Ci->center[2] - Cj->center[2];
+  double d0 = Ci->center[0] - Cj->center[0] - c->xp[0];
+  double d1 = Ci->center[1] - Cj->center[1] - c->xp[1];
+  double d2 = Ci->center[2] - Cj->center[2] - c->xp[2];
   double R2 = 0;
   R2 += d0 * d0;
   R2 += d1 * d1;
@@ -603,12 +612,12 @@ static void dtt(orc_ctx *c, int64_t ii, int64_t jj) {
     else
       m2l(c->L + ii * NTERM, c->M + jj * NTERM, dX);
     /* lists hold directed (target, source) pairs: a mutual call is two */
-    if (c->record) push_pair(&c->m2l_list, &c->n_m2l_cap, c->n_m2l_rec++, ii, jj);
+    if (c->record) push_pair(&c->m2l_list, &c->n_m2l_cap, c->n_m2l_rec++, ii, jj | (c->img << 27));
     if (c->record && c->mutual) push_pair(&c->m2l_list, &c->n_m2l_cap, c->n_m2l_rec++, jj, ii);
     c->n_m2l++;
   } else if (Ci->leaf && Cj->leaf) {
     if (Ci->nb > 0 && Cj->nb > 0) {
-      if (c->record) push_pair(&c->p2p_list, &c->n_p2p_cap, c->n_p2p_rec++, ii, jj);
+      if (c->record) push_pair(&c->p2p_list, &c->n_p2p_cap, c->n_p2p_rec++, ii, jj | (c->img << 27));
       if (c->record && c->mutual && ii != jj)
         push_pair(&c->p2p_list, &c->n_p2p_cap, c->n_p2p_rec++, jj, ii);
       c->n_p2p_leafpairs++;
@@ -653,9 +662,12 @@ orc_ctx *orc_run(const double *x, const double *y, const double *z, const double
   return orc_run_mutual(x, y, z, q, n, ncrit, theta, phases, record_lists, 0);
 }
 
-orc_ctx *orc_run_mutual(const double *x, const double *y, const double *z, const double *q,
-                        int64_t n, int ncrit, double theta, int phases, int record_lists,
-                        int mutual) {
+/* The tree over the bodies (SamplingOctree::Build on one rank); with
+ * cycle > 0 the root region is the periodic cell: a cube of side cycle
+ * centred on the bodies' scalar bounding cube (NULL if they span more). */
+static orc_ctx *setup(const double *x, const double *y, const double *z, const double *q,
+                      int64_t n, int ncrit, double theta, int record_lists, int mutual,
+                      double cycle) {
   if (n <= 0 || ncrit <= 0 || !(theta > 0)) return NULL;
   orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
   c->mutual = mutual ? 1 : 0;
@@ -664,6 +676,18 @@ orc_ctx *orc_run_mutual(const double *x, const double *y, const double *z, const
   c->theta = theta;
   c->record = record_lists;
   orc_bbox(x, y, z, n, c->rmin, c->rmax);
+  if (cycle > 0) {
+    const double lo = c->rmin[0], hi = c->rmax[0];
+    if (!(hi - lo <= cycle)) {
+      free(c);
+      return NULL;
+    }
+    const double mid = 0.5 * (lo + hi), h = 0.5 * cycle;
+    for (int d = 0; d < 3; d++) {
+      c->rmin[d] = mid - h;
+      c->rmax[d] = mid + h;
+    }
+  }
   uint64_t *k0 = (uint64_t *)malloc(sizeof(uint64_t) * n);
   orc_keys(x, y, z, n, c->rmin, c->rmax, k0);
   kidx *ki = (kidx *)malloc(sizeof(kidx) * n);
@@ -686,8 +710,82 @@ orc_ctx *orc_run_mutual(const double *x, const double *y, const double *z, const
   generate_cell(c, 0, 0, n, -1);
   c->M = (cx *)calloc((size_t)c->ncells * NTERM, sizeof(cx));
   c->L = (cx *)calloc((size_t)c->ncells * NTERM, sizeof(cx));
+  return c;
+}
+
+orc_ctx *orc_run_mutual(const double *x, const double *y, const double *z, const double *q,
+                        int64_t n, int ncrit, double theta, int phases, int record_lists,
+                        int mutual) {
+  orc_ctx *c = setup(x, y, z, q, n, ncrit, theta, record_lists, mutual, 0.0);
+  if (!c) return NULL;
   if (phases >= 1) upward(c, 0);
   if (phases >= 2) dtt(c, 0, 0);
+  if (phases >= 3) downward(c, 0);
+  return c;
+}
+
+/* The far field of the periodic images beyond the first shell
+ * (EX/include/traversal.h:126-185): Ms, the multipole of a block of copies
+ * (about the root's centre), starts as the root's; per sublevel the 26 x 27
+ * copies at (3 i + c) cycle (i in {-1,0,1}^3 without 0, c in {-1,0,1}^3) add
+ * their M2L into the root's local expansion (dX = Xroot - Xroot - Xperiodic),
+ * then the 27 copies i cycle shift into the block three times larger
+ * (M2M, dX = parent - child centre; the reference's loop resets the parent
+ * before this M2M, leaving the centre copy out: the sum over all
+ * (3^images)^3 copies, which its direct check computes, needs it). */
+static void periodic_far(orc_ctx *c, int images, double cycle) {
+  cx Ms[NTERM], Mn[NTERM];
+  memcpy(Ms, c->M, sizeof(Ms));
+  for (int level = 0; level < images - 1; level++) {
+    for (int ix = -1; ix <= 1; ix++)
+      for (int iy = -1; iy <= 1; iy++)
+        for (int iz = -1; iz <= 1; iz++) {
+          if (ix == 0 && iy == 0 && iz == 0) continue;
+          for (int cx_ = -1; cx_ <= 1; cx_++)
+            for (int cy = -1; cy <= 1; cy++)
+              for (int cz = -1; cz <= 1; cz++) {
+                const double dX[3] = {-((ix * 3 + cx_) * cycle), -((iy * 3 + cy) * cycle),
+                                      -((iz * 3 + cz) * cycle)};
+                m2l(c->L, Ms, dX);
+              }
+        }
+    memset(Mn, 0, sizeof(Mn));
+    for (int ix = -1; ix <= 1; ix++)
+      for (int iy = -1; iy <= 1; iy++)
+        for (int iz = -1; iz <= 1; iz++) {
+          const double dX[3] = {-(ix * cycle), -(iy * cycle), -(iz * cycle)};
+          m2m(Mn, Ms, dX);
+        }
+    memcpy(Ms, Mn, sizeof(Ms));
+    cycle *= 3;
+  }
+}
+
+/* The FMM with periodic images (EX/include/traversal.h:247-268, images >= 1):
+ * the traversal from the root and each of its 27 images (Xperiodic =
+ * (ix, iy, iz) cycle, the reference's loop order, mutual = 0), then the
+ * farther copies (images >= 2) into the root's local expansion. */
+orc_ctx *orc_run_periodic(const double *x, const double *y, const double *z, const double *q,
+                          int64_t n, int ncrit, double theta, int images, double cycle,
+                          int phases, int record_lists) {
+  if (images < 1 || !(cycle > 0)) return NULL;
+  orc_ctx *c = setup(x, y, z, q, n, ncrit, theta, record_lists, 0, cycle);
+  if (!c) return NULL;
+  if (phases >= 1) upward(c, 0);
+  if (phases >= 2) {
+    for (int ix = -1; ix <= 1; ix++)
+      for (int iy = -1; iy <= 1; iy++)
+        for (int iz = -1; iz <= 1; iz++) {
+          c->xp[0] = ix * cycle;
+          c->xp[1] = iy * cycle;
+          c->xp[2] = iz * cycle;
+          c->img = 9 * (ix + 1) + 3 * (iy + 1) + (iz + 1);
+          dtt(c, 0, 0);
+        }
+    c->xp[0] = c->xp[1] = c->xp[2] = 0;
+    c->img = 0;
+    periodic_far(c, images, cycle);
+  }
   if (phases >= 3) downward(c, 0);
   return c;
 }
@@ -779,6 +877,41 @@ void orc_direct(const double *x, const double *y, const double *z, const double 
         acc[3] -= dX[2];
       }
     }
+    for (int d = 0; d < 4; d++) trg4[4 * t + d] = acc[d] / q[i];
+  }
+}
+
+/* The direct sum over the copies of the bodies at (ix, iy, iz) cycle, |ix|,
+ * |iy|, |iz| <= prange, prange = sum_{i < images} 3^i, in the reference's
+ * CheckResult order (tapas_exafmm.cxx:314-345), normalised by the target
+ * charge. */
+void orc_direct_periodic(const double *x, const double *y, const double *z, const double *q,
+                         int64_t n, const int64_t *targets, int64_t nt, int images, double cycle,
+                         double *trg4) {
+  int prange = 0, p3 = 1;
+  for (int i = 0; i < images; i++, p3 *= 3) prange += p3;
+  for (int64_t t = 0; t < nt; t++) {
+    int64_t i = targets[t];
+    double acc[4] = {0, 0, 0, 0};
+    for (int ix = -prange; ix <= prange; ix++)
+      for (int iy = -prange; iy <= prange; iy++)
+        for (int iz = -prange; iz <= prange; iz++) {
+          const double xp[3] = {ix * cycle, iy * cycle, iz * cycle};
+          for (int64_t j = 0; j < n; j++) {
+            double dX[3] = {x[i] - x[j] - xp[0], y[i] - y[j] - xp[1], z[i] - z[j] - xp[2]};
+            double R2 = 0;
+            for (int d = 0; d < 3; d++) R2 += dX[d] * dX[d];
+            if (R2 != 0) {
+              double invR2 = 1.0 / R2;
+              double invR = q[i] * q[j] * sqrt(invR2);
+              for (int d = 0; d < 3; d++) dX[d] *= invR2 * invR;
+              acc[0] += invR;
+              acc[1] -= dX[0];
+              acc[2] -= dX[1];
+              acc[3] -= dX[2];
+            }
+          }
+        }
     for (int d = 0; d < 4; d++) trg4[4 * t + d] = acc[d] / q[i];
   }
 }

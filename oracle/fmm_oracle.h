@@ -61,6 +61,14 @@ void orc_list_sizes(const orc_ctx *c, int64_t *out2);
 
 void orc_direct(const double *x, const double *y, const double *z, const double *q, int64_t n,
                 const int64_t *targets, int64_t nt, double *trg4);
+/* periodic images (images >= 1, period cycle): the FMM and the direct sum
+ * over the (3^images)^3 copies of the bodies */
+orc_ctx *orc_run_periodic(const double *x, const double *y, const double *z, const double *q,
+                          int64_t n, int ncrit, double theta, int images, double cycle,
+                          int phases, int record_lists);
+void orc_direct_periodic(const double *x, const double *y, const double *z, const double *q,
+                         int64_t n, const int64_t *targets, int64_t nt, int images, double cycle,
+                         double *trg4);
 void orc_rel_l2(const double *trg, const double *ref, int64_t n, double *err2);
 
 #ifdef __cplusplus

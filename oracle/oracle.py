@@ -41,6 +41,10 @@ def lib():
         L.orc_run_mutual.restype = ctypes.c_void_p
         L.orc_run_mutual.argtypes = [_d, _d, _d, _d, ctypes.c_int64, ctypes.c_int,
                                      ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_run_periodic.restype = ctypes.c_void_p
+        L.orc_run_periodic.argtypes = [_d, _d, _d, _d, ctypes.c_int64, ctypes.c_int,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_int, ctypes.c_int]
         for name in ("orc_free", "orc_region", "orc_ncells", "orc_max_depth", "orc_sorted",
                      "orc_cells", "orc_expansions", "orc_fields", "orc_counts", "orc_lists",
                      "orc_list_sizes"):
@@ -133,16 +137,24 @@ def l2p(trg, bx, q, L, center):
 
 class Run:
     """One oracle evaluation.  phases: 0 tree, 1 +upward, 2 +traversal,
-    3 +downward (full FMM); mutual as the reference's --mutual."""
+    3 +downward (full FMM); mutual as the reference's --mutual; images > 0:
+    periodic images of period cycle (mutual 0)."""
 
-    def __init__(self, x, y, z, q, ncrit=64, theta=0.4, phases=3, record_lists=False, mutual=0):
+    def __init__(self, x, y, z, q, ncrit=64, theta=0.4, phases=3, record_lists=False, mutual=0,
+                 images=0, cycle=0.0):
         L = lib()
         self.x, self.y, self.z, self.q = [np.ascontiguousarray(a, np.float64) for a in (x, y, z, q)]
         self.n = len(self.x)
-        h = L.orc_run_mutual(self.x.ctypes.data_as(_d), self.y.ctypes.data_as(_d),
-                             self.z.ctypes.data_as(_d), self.q.ctypes.data_as(_d), self.n,
-                             int(ncrit), float(theta), int(phases), int(bool(record_lists)),
-                             int(bool(mutual)))
+        if images > 0:
+            h = L.orc_run_periodic(self.x.ctypes.data_as(_d), self.y.ctypes.data_as(_d),
+                                   self.z.ctypes.data_as(_d), self.q.ctypes.data_as(_d), self.n,
+                                   int(ncrit), float(theta), int(images), float(cycle),
+                                   int(phases), int(bool(record_lists)))
+        else:
+            h = L.orc_run_mutual(self.x.ctypes.data_as(_d), self.y.ctypes.data_as(_d),
+                                 self.z.ctypes.data_as(_d), self.q.ctypes.data_as(_d), self.n,
+                                 int(ncrit), float(theta), int(phases), int(bool(record_lists)),
+                                 int(bool(mutual)))
         if not h:
             raise ValueError("oracle rejected the input")
         self._h = ctypes.c_void_p(h)
@@ -217,6 +229,17 @@ def direct(x, y, z, q, targets):
     out = np.zeros((len(t), 4))
     lib().orc_direct(_p(x), _p(y), _p(z), _p(q), ctypes.c_int64(len(x)), _p(t),
                      ctypes.c_int64(len(t)), _p(out))
+    return out
+
+
+def direct_periodic(x, y, z, q, targets, images, cycle):
+    """direct sum over the (3^images)^3 copies of the bodies (period cycle)"""
+    x, y, z, q = [np.ascontiguousarray(a, np.float64) for a in (x, y, z, q)]
+    t = np.ascontiguousarray(targets, np.int64)
+    out = np.zeros((len(t), 4))
+    lib().orc_direct_periodic(_p(x), _p(y), _p(z), _p(q), ctypes.c_int64(len(x)), _p(t),
+                              ctypes.c_int64(len(t)), ctypes.c_int(images),
+                              ctypes.c_double(cycle), _p(out))
     return out
 
 

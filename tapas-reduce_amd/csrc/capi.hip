@@ -140,6 +140,11 @@ struct tfmm_ctx {
   DevBuf gsz, gstep;              // its device sizes and traversal step grids
   uint32_t *h_plan = nullptr;     // pinned read-back of the whole graph (PLAN_HOST_BYTES)
   bool one_graph = false;         // the last tfmm_evaluate was one launch of it
+  // periodic images (tfmm_set_periodic): image sublevels, 0 = none, and the
+  // period; the root cell is then the periodic cell
+  int images = 0;
+  double cycle = 0;
+  DevBuf pflag;  // bodies spanning more than one period
   int32_t unit_chunk = 0;
   uint32_t units_bound = 0;
   bool codes_ready = false;
@@ -247,7 +252,8 @@ std::vector<DevBuf *> all_buffers(tfmm_ctx *c) {
           &c->feat_acc, &c->feat_x, &c->feat_out, &c->d_prob_off, &c->bprob,
           &c->pkey, &c->pkey_alt, &c->ppos, &c->ppos_alt, &c->brec, &c->lev,
           &c->lev_bsum, &c->lev_status, &c->dtt_geo, &c->dtt_topo, &c->u_cnt,
-          &c->u_off, &c->units, &c->u_part, &c->p2p_next, &c->finite_flag, &c->gsz, &c->gstep};
+          &c->u_off, &c->units, &c->u_part, &c->p2p_next, &c->finite_flag, &c->gsz, &c->gstep,
+          &c->pflag};
 }
 
 hipError_t reserve_cells(tfmm_ctx *c, size_t need, size_t used, hipStream_t s) {
@@ -268,6 +274,20 @@ hipError_t reserve_cells(tfmm_ctx *c, size_t need, size_t used, hipStream_t s) {
 }
 
 constexpr int H_U32_WORDS = 256;
+
+// the period the kernels displace images by (0 without periodic images)
+double period(const tfmm_ctx *c) { return c->images > 0 ? c->cycle : 0.0; }
+
+// the direct sum's image range for the image sublevels (the reference's
+// prange, tapas_exafmm.cxx:314-317)
+int image_range(const tfmm_ctx *c) {
+  int r = 0, p3 = 1;
+  for (int i = 0; i < c->images; i++, p3 *= 3) r += p3;
+  return r;
+}
+// source field of the fallback M2L keys (target << FB_SRC_BITS | source word):
+// the whole 32-bit source word, cell and periodic image
+constexpr int FB_SRC_BITS = 32;
 
 int bits_for(int64_t v) {
   int b = 1;
@@ -365,6 +385,11 @@ int do_build_tree(tfmm_ctx *c) {
     TRY(dev_reserve(c->bbox_part, 2 * 8 * (size_t)nparts));
     TRY(launch_bbox(c->bx, c->by, c->bz, n, dp<double>(c->bbox_part), nparts,
                     dp<double>(c->region), s));
+    if (c->images > 0) {
+      TRY(dev_reserve(c->pflag, 4));
+      TRY(launch_fill_zero(c->pflag.ptr, 4, s));
+      TRY(launch_periodic_region(dp<double>(c->region), c->cycle, dp<uint32_t>(c->pflag), s));
+    }
   }
   TRY(dev_reserve(c->brec, n * 32));
   TRY(launch_keys(c->bx, c->by, c->bz, c->bq, n, dp<double>(c->region), off, np,
@@ -527,7 +552,7 @@ int do_dtt(tfmm_ctx *c) {
     TRY(dev_reserve(c->front_b, (size_t)c->front_cap * 8));
     uint32_t *cnt = dp<uint32_t>(c->cnt);
     TRY(hipMemsetAsync(cnt, 0, (size_t)cwords * 4, s));
-    TRY(launch_root_pairs(dp<int2>(c->front_a), c->nprob, cnt + 16, s));
+    TRY(launch_root_pairs(dp<int2>(c->front_a), c->nprob, cnt + 16, s, c->images > 0));
     // grid of step st: the frontier size of the previous traversal of the same
     // tree shape when there is one (checked below), else the capacity
     const bool hint = (int)c->dtt_hint.size() == nsteps + 1 && c->dtt_hint_cells == c->ncells;
@@ -539,7 +564,7 @@ int do_dtt(tfmm_ctx *c) {
                           dp<int2>(c->p2p_pairs), c->p2p_cap, cnt, dp<double4>(c->dtt_geo),
                           dp<int2>(c->dtt_topo), k.depth, k.width, dp<int32_t>(c->slot_of),
                           dp<uint32_t>(c->g_used), dp<uint32_t>(c->g_used) + nslots,
-                          c->prm.mutual, s));
+                          c->prm.mutual, s, period(c)));
       std::swap(c->front_a, c->front_b);
     }
     TRY(hipMemcpyAsync(c->h_u32, cnt, (size_t)cwords * 4, hipMemcpyDeviceToHost, s));
@@ -636,7 +661,7 @@ int build_m2l_batches(tfmm_ctx *c) {
                                 dp<uint32_t>(c->g_gid), gbits, dp<uint32_t>(c->gkeys),
                                 dp<uint32_t>(c->gvals), s));
   } else {
-    TRY(launch_m2l_classify(dp<int2>(c->m2l_pairs), dp<int32_t>(c->slot_of), n, dp<uint32_t>(c->g_gid), gbits, sbits,
+    TRY(launch_m2l_classify(dp<int2>(c->m2l_pairs), dp<int32_t>(c->slot_of), n, dp<uint32_t>(c->g_gid), gbits, FB_SRC_BITS,
                             dp<uint32_t>(c->gkeys), dp<uint32_t>(c->gvals), dp<uint64_t>(c->fkeys),
                             dp<uint32_t>(c->fvals), dp<uint32_t>(c->g_cnt), s));
     TRY(hipMemcpyAsync(c->h_u32, c->g_cnt.ptr, 8, hipMemcpyDeviceToHost, s));
@@ -688,12 +713,13 @@ int build_m2l_batches(tfmm_ctx *c) {
   // fallback pairs: per-target list sorted by (target, source)
   bool falt = false;
   TRY(radix_sort_pairs(dp<uint64_t>(c->fkeys), dp<uint32_t>(c->fvals), dp<uint64_t>(c->fkeys_alt),
-                       dp<uint32_t>(c->fvals_alt), nfb, 0, 2 * sbits, c->sort_tmp.ptr, s, &falt));
+                       dp<uint32_t>(c->fvals_alt), nfb, 0, FB_SRC_BITS + sbits, c->sort_tmp.ptr, s,
+                       &falt));
   const uint64_t *fk = dp<uint64_t>(falt ? c->fkeys_alt : c->fkeys);
   TRY(dev_reserve(c->fb_off, (size_t)(c->ncells + 1) * 4));
   TRY(dev_reserve(c->fb_src, (size_t)std::max<uint32_t>(nfb, 1) * 4));
-  TRY(launch_csr_offsets(fk, nfb, sbits, (int32_t)c->ncells, dp<int32_t>(c->fb_off), s));
-  TRY(launch_unpack_pairs(fk, nfb, sbits, dp<int32_t>(c->fb_src), s));
+  TRY(launch_csr_offsets(fk, nfb, FB_SRC_BITS, (int32_t)c->ncells, dp<int32_t>(c->fb_off), s));
+  TRY(launch_unpack_pairs(fk, nfb, FB_SRC_BITS, dp<int32_t>(c->fb_src), s));
   if (!c->codes_ready) {
     TRY(dev_reserve(c->codes, m2l_codes_bytes()));
     hipError_t e = m2l_upload_codes(dp<uint32_t>(c->codes), s);
@@ -741,10 +767,10 @@ int run_lists(tfmm_ctx *c) {
     if (mixed)
       TRY(launch_p2p_fp32(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
                           dp<double>(c->acc), p2p_resident(c), dp<uint32_t>(c->p2p_next),
-                          c->side));
+                          c->side, period(c)));
     else
       TRY(launch_p2p_fp64(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<double>(c->acc),
-                          c->side));
+                          c->side, period(c)));
     TRY(hipEventRecord(c->ev[EV_P2P1], c->side));
     return TFMM_OK;
   };
@@ -783,8 +809,9 @@ int run_lists(tfmm_ctx *c) {
     Csr fl{dp<int32_t>(c->fb_off), dp<int32_t>(c->fb_src)};
     TRY(dev_reserve(c->fb_part, m2l_pairs_scratch_bytes((int32_t)c->n_fallback)));
     TRY(launch_m2l_fp64(k, (int32_t)c->ncells, fl, (int32_t)c->n_fallback, dp<double>(c->M),
-                        dp<double>(c->L), c->fb_part.ptr, s));
+                        dp<double>(c->L), c->fb_part.ptr, s, nullptr, period(c)));
   }
+  TRY(launch_periodic_far(dp<double>(c->M), dp<double>(c->L), c->cycle, c->images, s));
   TRY(hipEventRecord(c->ev[EV_M2L1], s));
 
   TRY(hipStreamWaitEvent(s, c->ev[EV_P2P1], 0));
@@ -806,6 +833,8 @@ int do_traverse(tfmm_ctx *c) {
 // reference's numM2L; P2P pairs of leaves.  Pairs with an empty cell are
 // counted but not executed, as in the traversal.
 int traverse_given(tfmm_ctx *c, const int32_t *m2l, int64_t nm, const int32_t *p2p, int64_t np) {
+  if (c->images > 0)
+    return fail(c, TFMM_EINVAL, "tfmm_traverse_lists: lists of periodic images are not taken");
   hipStream_t s = c->stream;
   const Cells k = cells_of(c);
   if (nm < 0 || np < 0 || (nm && !m2l) || (np && !p2p))
@@ -947,10 +976,11 @@ int fork_p2p(tfmm_ctx *c, int64_t ncells = -1) {
   Csr pl{dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src)};
   if (mixed)
     TRY(launch_p2p_fp32(k, (int32_t)ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
-                        dp<double>(c->acc), p2p_resident(c), dp<uint32_t>(c->p2p_next), c->side));
+                        dp<double>(c->acc), p2p_resident(c), dp<uint32_t>(c->p2p_next), c->side,
+                        period(c)));
   else
     TRY(launch_p2p_fp64(k, (int32_t)ncells, pl, sorted_bodies(c), dp<double>(c->acc),
-                        c->side));
+                        c->side, period(c)));
   TRY(rec(c, EV_P2P1, c->side));
   TRY(hipEventRecord(c->ev[EV_P2PJOIN], c->side));
   return TFMM_OK;
@@ -1036,8 +1066,9 @@ int fused_numeric(tfmm_ctx *c) {
   if (c->n_fallback) {
     Csr fl{dp<int32_t>(c->fb_off), dp<int32_t>(c->fb_src)};
     TRY(launch_m2l_fp64(k, (int32_t)c->ncells, fl, (int32_t)c->n_fallback, dp<double>(c->M),
-                        dp<double>(c->L), c->fb_part.ptr, s));
+                        dp<double>(c->L), c->fb_part.ptr, s, nullptr, period(c)));
   }
+  TRY(launch_periodic_far(dp<double>(c->M), dp<double>(c->L), c->cycle, c->images, s));
   TRY(rec(c, EV_M2L1, s));
   TRY(hipStreamWaitEvent(s, c->ev[EV_P2PJOIN], 0));
   TRY(rec(c, EV_TRAV, s));
@@ -1187,7 +1218,7 @@ void plan_make(tfmm_ctx *c) {
   pl.sbits = bits_for(pl.cells);
   pl.tbits = bits_for(pl.ntiles);
   if (pl.tbits + pl.gbits > 32 || pl.gbits > m2l_group_bits() ||
-      pl.cells >= ((int64_t)1 << m2l_src_bits()) || 2 * pl.sbits > 64 ||
+      pl.cells >= ((int64_t)1 << m2l_src_bits()) || FB_SRC_BITS + pl.sbits > 64 ||
       (int64_t)pl.caps.fallback > (1 << 22))
     return;
   pl.valid = true;
@@ -1422,7 +1453,7 @@ int whole_record(tfmm_ctx *c) {
                                    dp<int8_t>(c->g_dl), dp<int8_t>(c->g_lt), s, pl.groups));
     TRY(launch_fill_zero(gcnt, 16, s));
     TRY(launch_m2l_classify(dp<int2>(c->m2l_pairs), dp<int32_t>(c->slot_of), nm,
-                            dp<uint32_t>(c->g_gid), pl.gbits, pl.sbits, dp<uint32_t>(c->gkeys),
+                            dp<uint32_t>(c->g_gid), pl.gbits, FB_SRC_BITS, dp<uint32_t>(c->gkeys),
                             dp<uint32_t>(c->gvals), dp<uint64_t>(c->fkeys),
                             dp<uint32_t>(c->fvals), gcnt, s, cnt + 1, pl.groups));
     bool galt = false;
@@ -1450,11 +1481,11 @@ int whole_record(tfmm_ctx *c) {
     const uint32_t nfb = std::min<uint32_t>(nm, pl.caps.fallback);
     bool falt = false;
     TRY(radix_sort_pairs(dp<uint64_t>(c->fkeys), dp<uint32_t>(c->fvals), dp<uint64_t>(c->fkeys_alt),
-                         dp<uint32_t>(c->fvals_alt), nfb, 0, 2 * pl.sbits, c->sort_tmp.ptr, s,
+                         dp<uint32_t>(c->fvals_alt), nfb, 0, FB_SRC_BITS + pl.sbits, c->sort_tmp.ptr, s,
                          &falt, gcnt + 1));
     const uint64_t *fk = dp<uint64_t>(falt ? c->fkeys_alt : c->fkeys);
-    TRY(launch_csr_offsets(fk, nfb, pl.sbits, cells, dp<int32_t>(c->fb_off), s, gcnt + 1));
-    TRY(launch_unpack_pairs(fk, nfb, pl.sbits, dp<int32_t>(c->fb_src), s, gcnt + 1));
+    TRY(launch_csr_offsets(fk, nfb, FB_SRC_BITS, cells, dp<int32_t>(c->fb_off), s, gcnt + 1));
+    TRY(launch_unpack_pairs(fk, nfb, FB_SRC_BITS, dp<int32_t>(c->fb_src), s, gcnt + 1));
     TRY(launch_m2l_group_afrag(mixed ? (const void *)c->g_y32.ptr : (const void *)c->g_y64.ptr,
                                pl.groups, dp<uint32_t>(c->codes), dp<int8_t>(c->g_dl),
                                c->g_af.ptr, mixed, s, sz + SZ_NG));
@@ -1702,6 +1733,19 @@ void tfmm_destroy(tfmm_ctx *c) {
 }
 
 const char *tfmm_last_error(const tfmm_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int tfmm_set_periodic(tfmm_ctx *c, int images, double cycle) {
+  if (!c) return TFMM_EINVAL;
+  if (images < 0 || images > TFMM_PERIODIC_MAX_IMAGES)
+    return fail(c, TFMM_EINVAL, "images must be in [0, TFMM_PERIODIC_MAX_IMAGES]");
+  if (images > 0 && !(cycle > 0 && cycle < 1e300))
+    return fail(c, TFMM_EINVAL, "the period must be a positive finite number");
+  c->images = images;
+  c->cycle = images > 0 ? cycle : 0.0;
+  c->stage = 0;  // a new tree and traversal
+  plan_drop(c);
+  return TFMM_OK;
+}
 
 int tfmm_set_tuning(tfmm_ctx *c, int knob, double value) {
   if (!c) return TFMM_EINVAL;
@@ -1998,6 +2042,10 @@ int tfmm_build_tree(tfmm_ctx *c, const tfmm_params *p) {
   int rc = check_params(c, p);
   if (rc) return rc;
   if (c->n < 1) return fail(c, TFMM_ESTATE, "no bodies set");
+  if (c->images > 0 && c->nprob > 1)
+    return fail(c, TFMM_EINVAL, "periodic images apply to one problem, not a batch");
+  if (c->images > 0 && p->mutual)
+    return fail(c, TFMM_EINVAL, "periodic images are traversed with mutual = 0");
   TRY(hipSetDevice(c->device));
   c->prm = *p;
   c->stage = 0;
@@ -2005,6 +2053,11 @@ int tfmm_build_tree(tfmm_ctx *c, const tfmm_params *p) {
   TRY(hipEventRecord(c->ev[EV_BEGIN], c->stream));
   rc = do_build_tree(c);
   if (rc) return rc;
+  if (c->images > 0) {
+    TRY(hipMemcpyAsync(c->h_u32, c->pflag.ptr, 4, hipMemcpyDeviceToHost, c->stream));
+    TRY(hipStreamSynchronize(c->stream));
+    if (c->h_u32[0]) return fail(c, TFMM_EINVAL, "the bodies span more than one period");
+  }
   TRY(hipEventRecord(c->ev[EV_TREE], c->stream));
   c->stage = 1;
   return TFMM_OK;
@@ -2066,7 +2119,7 @@ int tfmm_downward(tfmm_ctx *c) {
 }
 
 int tfmm_evaluate(tfmm_ctx *c, const tfmm_params *p) {
-  const bool whole = c && c->use_graph == 2;
+  const bool whole = c && c->use_graph == 2 && c->images == 0;
   if (whole) {
     int rc = check_params(c, p);
     if (rc) return rc;
@@ -2274,7 +2327,7 @@ int tfmm_direct(tfmm_ctx *c, const int64_t *targets, int64_t nt, double *out) {
     if (cnt[p + 1] > cnt[p])
       e = launch_direct_fp64(c->bx + b, c->by + b, c->bz + b, c->bq + b, m,
                              dp<int64_t>(t) + cnt[p], 0, cnt[p + 1] - cnt[p],
-                             dp<double>(o) + 4 * cnt[p], c->stream);
+                             dp<double>(o) + 4 * cnt[p], c->stream, image_range(c), period(c));
   }
   if (e == hipSuccess)
     e = hipMemcpyAsync(grouped.data(), o.ptr, (size_t)nt * 32, hipMemcpyDeviceToHost, c->stream);
@@ -2311,7 +2364,7 @@ int tfmm_direct_noalloc(tfmm_ctx *c, const int64_t *targets, int64_t nt, double 
     int64_t j = i + 1;
     while (j < nt && targets[j] >= b && targets[j] < end) j++;
     e = launch_direct_fp64(c->bx + b, c->by + b, c->bz + b, c->bq + b, end - b, d_idx + i, b,
-                           j - i, d_out + 4 * i, c->stream);
+                           j - i, d_out + 4 * i, c->stream, image_range(c), period(c));
     i = j;
   }
   if (e == hipSuccess)

@@ -24,7 +24,8 @@ __global__ __launch_bounds__(64 * PAIR_WAVES) void k_m2l_pairs_fp64(Cells c, Csr
                                                                    int32_t p0, int32_t npairs,
                                                                    const int32_t *tgt,
                                                                    const double *M, cd *part,
-                                                                   const uint32_t *dn) {
+                                                                   const uint32_t *dn,
+                                                                   double cycle) {
   __shared__ cd Ys[PAIR_WAVES][NTERM];
   __shared__ cd Ms[PAIR_WAVES][NTERM];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -32,7 +33,9 @@ __global__ __launch_bounds__(64 * PAIR_WAVES) void k_m2l_pairs_fp64(Cells c, Csr
   if (dn) npairs = min(npairs, (int32_t)min(*dn, (uint32_t)INT32_MAX) - p0);
   if (pl >= npairs) return;  // whole wave leaves together; no block barrier follows
   const int32_t pi = p0 + pl;
-  const int32_t ti = tgt[pi], sj = list.src[pi];
+  const int32_t ti = tgt[pi];
+  const uint32_t sw = (uint32_t)list.src[pi];  // source cell and periodic image
+  const int32_t sj = (int32_t)(sw & CELL_MASK);
   int j = 0, k = 0;
   if (lane < NTERM) {
     while (tri(j + 1, 0) <= lane) j++;
@@ -41,9 +44,12 @@ __global__ __launch_bounds__(64 * PAIR_WAVES) void k_m2l_pairs_fp64(Cells c, Csr
   const cd *Mall = reinterpret_cast<const cd *>(M);
   if (lane < NTERM) Ms[w][lane] = Mall[(int64_t)sj * NTERM + lane];
   if (lane < P) {
-    const double dx = c.center[3 * (int64_t)ti] - c.center[3 * (int64_t)sj];
-    const double dy = c.center[3 * (int64_t)ti + 1] - c.center[3 * (int64_t)sj + 1];
-    const double dz = c.center[3 * (int64_t)ti + 2] - c.center[3 * (int64_t)sj + 2];
+    double sx, sy, sz;
+    image_shift(sw >> IMG_SHIFT, cycle, sx, sy, sz);
+    // dX = Xi - Xj - Xperiodic (LaplaceSphericalCPU_tapas.h:218-220)
+    const double dx = c.center[3 * (int64_t)ti] - c.center[3 * (int64_t)sj] - sx;
+    const double dy = c.center[3 * (int64_t)ti + 1] - c.center[3 * (int64_t)sj + 1] - sy;
+    const double dz = c.center[3 * (int64_t)ti + 2] - c.center[3 * (int64_t)sj + 2] - sz;
     double rho, ca, sa;
     cd ei;
     sph_frame(dx, dy, dz, rho, ca, sa, ei);
@@ -104,7 +110,8 @@ size_t m2l_pairs_scratch_bytes(int32_t npairs) {
 // scratch: m2l_pairs_scratch_bytes(npairs) bytes (partials of one chunk of
 // pairs, then the target of every pair); chunks are added in list order
 hipError_t launch_m2l_fp64(Cells c, int32_t ncells, Csr list, int32_t npairs, const double *M,
-                           double *L, void *scratch, hipStream_t s, const uint32_t *dn) {
+                           double *L, void *scratch, hipStream_t s, const uint32_t *dn,
+                           double cycle) {
   if (ncells <= 0 || npairs <= 0) return hipSuccess;
   const int32_t chunk = std::min(npairs, PAIR_CHUNK);
   cd *part = static_cast<cd *>(scratch);
@@ -113,7 +120,7 @@ hipError_t launch_m2l_fp64(Cells c, int32_t ncells, Csr list, int32_t npairs, co
   for (int32_t p0 = 0; p0 < npairs; p0 += chunk) {
     const int32_t n = std::min(chunk, npairs - p0);
     k_m2l_pairs_fp64<<<(n + PAIR_WAVES - 1) / PAIR_WAVES, 64 * PAIR_WAVES, 0, s>>>(
-        c, list, p0, n, tgt, M, part, dn);
+        c, list, p0, n, tgt, M, part, dn, cycle);
     k_m2l_reduce<<<(ncells + 3) / 4, 256, 0, s>>>(list, ncells, p0, p0 + n, part, L);
   }
   return hipGetLastError();

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(1024) void k_classify(const int2 *pairs, const int3
   if (grouped) {
     const uint64_t tile = (uint32_t)p.x / TILE, in_tile = (uint32_t)p.x % TILE;
     gkeys[at] = (uint32_t)(tile << gbits) | min(gid_of_slot[slot], gcap - 1u);
-    gvals[at] = (uint32_t)p.y | (uint32_t)(in_tile << SRC_BITS);
+    gvals[at] = ((uint32_t)p.y & CELL_MASK) | (uint32_t)(in_tile << SRC_BITS);
   }
   const uint32_t af = block_append<16>(&counters[1], fallback, 1, scratch);
   if (fallback) {
@@ -154,7 +154,7 @@ __global__ void k_classify_all(const int2 *pairs, const int32_t *slot_of, uint32
   const int2 p = pairs[i];
   const uint64_t tile = (uint32_t)p.x / TILE, in_tile = (uint32_t)p.x % TILE;
   gkeys[i] = (uint32_t)(tile << gbits) | gid_of_slot[slot_of[i]];
-  gvals[i] = (uint32_t)p.y | (uint32_t)(in_tile << SRC_BITS);
+  gvals[i] = ((uint32_t)p.y & CELL_MASK) | (uint32_t)(in_tile << SRC_BITS);
 }
 
 // Harmonics of every used group, FP64, scaled: Y~_p^q = Y_p^q sigma_t^(p+1)

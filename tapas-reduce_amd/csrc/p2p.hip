@@ -151,8 +151,8 @@ template <typename T, int RING, bool TWO>
 __device__ inline void p2p_pass(const Cells &c, const Csr &list, const Bodies &b,
                                 const float4 *rel, double *acc, typename Vec4<T>::type *ring,
                                 int32_t tb, int32_t lb, int32_t le, double tcx, double tcy,
-                                double tcz, T tiny, int G, int32_t t, int32_t t2, bool tvalid,
-                                bool tvalid2) {
+                                double tcz, double cycle, T tiny, int G, int32_t t, int32_t t2,
+                                bool tvalid, bool tvalid2) {
   using V4 = typename Vec4<T>::type;
   constexpr bool MIX = sizeof(T) == 4;
   const int lane = threadIdx.x;
@@ -187,18 +187,28 @@ __device__ inline void p2p_pass(const Cells &c, const Csr &list, const Bodies &b
   for (int32_t base = lb; base < le; base += 64) {
     // metadata of source leaves base .. base + 63, one per lane
     const int nsl = min(64, le - base);
-    int32_t mbeg = 0, mnb = 0;
+    int32_t mbeg = 0, mnb = 0, mimg = 0;
     float mox = 0, moy = 0, moz = 0;
     if (lane < nsl) {
-      const int32_t sj = list.src[base + lane];
+      // source word: leaf and periodic image (the leaf's bodies displaced by
+      // the image's Xperiodic, LaplaceP2PCPU_tapas.h:54)
+      const uint32_t sw = (uint32_t)list.src[base + lane];
+      const int32_t sj = (int32_t)(sw & CELL_MASK);
+      mimg = (int32_t)(sw >> IMG_SHIFT);
       mbeg = c.beg[sj];
       mnb = c.nb[sj];
       if constexpr (MIX) {
-        mox = (float)(tcx - c.center[3 * (int64_t)sj]);
-        moy = (float)(tcy - c.center[3 * (int64_t)sj + 1]);
-        moz = (float)(tcz - c.center[3 * (int64_t)sj + 2]);
+        double sx, sy, sz;
+        image_shift((uint32_t)mimg, cycle, sx, sy, sz);
+        mox = (float)(tcx - c.center[3 * (int64_t)sj] - sx);
+        moy = (float)(tcy - c.center[3 * (int64_t)sj + 1] - sy);
+        moz = (float)(tcz - c.center[3 * (int64_t)sj + 2] - sz);
       }
     }
+    // FP64: the image displacement of the current source leaf
+    double ksx = 0, ksy = 0, ksz = 0;
+    if constexpr (!MIX)
+      image_shift((uint32_t)__builtin_amdgcn_readlane(mimg, 0), cycle, ksx, ksy, ksz);
     // piece (k, off): bodies off .. off + 63 of source leaf k
     int k = 0, off = 0;
     int32_t kb = __builtin_amdgcn_readlane(mbeg, 0), kn = __builtin_amdgcn_readlane(mnb, 0);
@@ -217,6 +227,8 @@ __device__ inline void p2p_pass(const Cells &c, const Csr &list, const Bodies &b
       if (lane < n) {
         if constexpr (MIX)
           ring[ring_slot<RING>(wpos + lane)] = make_float4(v.x - kox, v.y - koy, v.z - koz, v.w);
+        else if (cycle != 0)
+          ring[ring_slot<RING>(wpos + lane)] = make_double4(v.x + ksx, v.y + ksy, v.z + ksz, v.w);
         else
           ring[ring_slot<RING>(wpos + lane)] = v;
       }
@@ -237,6 +249,8 @@ __device__ inline void p2p_pass(const Cells &c, const Csr &list, const Bodies &b
             kox = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mox), k));
             koy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(moy), k));
             koz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(moz), k));
+          } else {
+            image_shift((uint32_t)__builtin_amdgcn_readlane(mimg, k), cycle, ksx, ksy, ksz);
           }
         }
       }
@@ -290,7 +304,8 @@ __device__ inline void p2p_pass(const Cells &c, const Csr &list, const Bodies &b
 // the end of every pass.
 template <typename T, int RING>
 __device__ inline void p2p_leaf(int32_t ti, const Cells &c, const Csr &list, const Bodies &b,
-                                const float4 *rel, double *acc, typename Vec4<T>::type *ring) {
+                                const float4 *rel, double *acc, typename Vec4<T>::type *ring,
+                                double cycle) {
   constexpr bool MIX = sizeof(T) == 4;
   constexpr int GMAX = RING / (2 * SEG);  // a chunk (16 G) plus a 64-body piece fit the ring
   if (c.child[ti] >= 0) return;
@@ -318,11 +333,11 @@ __device__ inline void p2p_leaf(int32_t ti, const Cells &c, const Csr &list, con
     const int32_t t = t0 + tloc, t2 = t + tpp;
     const bool tvalid = tloc < tpp && t < nt, tvalid2 = tloc < tpp && t2 < nt;
     if (t0 + tpp < nt)
-      p2p_pass<T, RING, true>(c, list, b, rel, acc, ring, tb, lb, le, tcx, tcy, tcz, tiny, G, t,
-                              t2, tvalid, tvalid2);
+      p2p_pass<T, RING, true>(c, list, b, rel, acc, ring, tb, lb, le, tcx, tcy, tcz, cycle, tiny,
+                              G, t, t2, tvalid, tvalid2);
     else
-      p2p_pass<T, RING, false>(c, list, b, rel, acc, ring, tb, lb, le, tcx, tcy, tcz, tiny, G,
-                               t, t2, tvalid, false);
+      p2p_pass<T, RING, false>(c, list, b, rel, acc, ring, tb, lb, le, tcx, tcy, tcz, cycle, tiny,
+                               G, t, t2, tvalid, false);
   }
 }
 
@@ -330,17 +345,18 @@ constexpr int RING32 = 256;  // float4 entries: 4.25 KB with the pads
 constexpr int RING64 = 128;  // double4 entries: 4.25 KB with the pads
 constexpr int ring_entries(int ring) { return ring + ring / 16; }
 
-__global__ __launch_bounds__(64) void k_p2p_fp64(Cells c, Csr list, Bodies b, double *acc) {
+__global__ __launch_bounds__(64) void k_p2p_fp64(Cells c, Csr list, Bodies b, double *acc,
+                                                 double cycle) {
   __shared__ double4 ring[ring_entries(RING64)];
-  p2p_leaf<double, RING64>(blockIdx.x, c, list, b, nullptr, acc, ring);
+  p2p_leaf<double, RING64>(blockIdx.x, c, list, b, nullptr, acc, ring, cycle);
 }
 
 }  // namespace
 
 hipError_t launch_p2p_fp64(Cells c, int32_t ncells, Csr list, Bodies b, double *acc,
-                           hipStream_t s) {
+                           hipStream_t s, double cycle) {
   if (ncells <= 0) return hipSuccess;
-  k_p2p_fp64<<<ncells, 64, 0, s>>>(c, list, b, acc);
+  k_p2p_fp64<<<ncells, 64, 0, s>>>(c, list, b, acc, cycle);
   return hipGetLastError();
 }
 
@@ -365,9 +381,9 @@ __global__ __launch_bounds__(64) void k_leaf_relpos(Cells c, Bodies b, float4 *r
 }
 
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8)))
-void k_p2p_fp32(Cells c, Csr list, Bodies b, const float4 *rel, double *acc) {
+void k_p2p_fp32(Cells c, Csr list, Bodies b, const float4 *rel, double *acc, double cycle) {
   __shared__ float4 ring[ring_entries(RING32)];
-  p2p_leaf<float, RING32>(blockIdx.x, c, list, b, rel, acc, ring);
+  p2p_leaf<float, RING32>(blockIdx.x, c, list, b, rel, acc, ring, cycle);
 }
 
 // A fixed number of resident waves that take target cells from a counter
@@ -377,7 +393,8 @@ void k_p2p_fp32(Cells c, Csr list, Bodies b, const float4 *rel, double *acc) {
 // in which the waves take them does not change any result.
 __global__ __launch_bounds__(64) void k_p2p_fp32_resident(Cells c, int32_t ncells, Csr list,
                                                          Bodies b, const float4 *rel,
-                                                         double *acc, uint32_t *next) {
+                                                         double *acc, uint32_t *next,
+                                                         double cycle) {
   __shared__ float4 ring[ring_entries(RING32)];
   __shared__ uint32_t take;
   while (true) {
@@ -386,7 +403,7 @@ __global__ __launch_bounds__(64) void k_p2p_fp32_resident(Cells c, int32_t ncell
     __syncthreads();
     const uint32_t ti = take;
     if (ti >= (uint32_t)ncells) return;
-    p2p_leaf<float, RING32>((int32_t)ti, c, list, b, rel, acc, ring);
+    p2p_leaf<float, RING32>((int32_t)ti, c, list, b, rel, acc, ring, cycle);
   }
 }
 }  // namespace
@@ -398,15 +415,16 @@ hipError_t launch_leaf_relpos(Cells c, int32_t ncells, Bodies b, float4 *rel, hi
 }
 
 hipError_t launch_p2p_fp32(Cells c, int32_t ncells, Csr list, Bodies b, const float4 *rel,
-                           double *acc, int resident, uint32_t *next, hipStream_t s) {
+                           double *acc, int resident, uint32_t *next, hipStream_t s,
+                           double cycle) {
   if (ncells <= 0) return hipSuccess;
   if (resident > 0 && next) {
     hipError_t e = launch_fill_zero(next, 4, s);
     if (e != hipSuccess) return e;
     const int grid = resident < ncells ? resident : ncells;
-    k_p2p_fp32_resident<<<grid, 64, 0, s>>>(c, ncells, list, b, rel, acc, next);
+    k_p2p_fp32_resident<<<grid, 64, 0, s>>>(c, ncells, list, b, rel, acc, next, cycle);
   } else {
-    k_p2p_fp32<<<ncells, 64, 0, s>>>(c, list, b, rel, acc);
+    k_p2p_fp32<<<ncells, 64, 0, s>>>(c, list, b, rel, acc, cycle);
   }
   return hipGetLastError();
 }

@@ -21,6 +21,26 @@ constexpr int MAXDEPTH = 19;            // 64-bit keys, 3-D (sfc_morton.h:154-18
 constexpr int DEPTHBITS = 5;
 constexpr int WAVE = 64;
 
+// Periodic images (EX/include/traversal.h:247-268): the source side of a
+// traversal pair, list entry or P2P/M2L source carries the image it is seen
+// through in its top bits: word = cell | v << IMG_SHIFT, image (ix, iy, iz)
+// in {-1, 0, 1}^3 as v = 9 (ix + 1) + 3 (iy + 1) + (iz + 1) (the reference's
+// loop order), the source displaced by (ix, iy, iz) * cycle.  Without
+// periodic images cycle = 0 and v = 0: no displacement.
+constexpr int IMG_SHIFT = 27;
+constexpr uint32_t CELL_MASK = (1u << IMG_SHIFT) - 1u;
+constexpr uint32_t IMG_CENTRE = 13;  // (0, 0, 0)
+__host__ __device__ inline void image_shift(uint32_t v, double cycle, double &sx, double &sy,
+                                            double &sz) {
+  if (cycle == 0) {  // +0: subtracting it changes no value (nor the sign of a zero)
+    sx = sy = sz = 0;
+    return;
+  }
+  sx = cycle * (double)((int)(v / 9) - 1);
+  sy = cycle * (double)((int)(v / 3 % 3) - 1);
+  sz = cycle * (double)((int)(v % 3) - 1);
+}
+
 __host__ __device__ inline int tri(int n, int m) { return n * (n + 1) / 2 + m; }
 
 // ---------------------------------------------------------------------------

@@ -78,7 +78,10 @@ hipError_t launch_problem_keys(const uint32_t *order, const uint32_t *bprob, int
 hipError_t launch_problem_gather(const uint32_t *pos, const uint64_t *keys1,
                                  const uint32_t *order1, int64_t n, uint64_t *keys,
                                  uint32_t *order, hipStream_t s);
-hipError_t launch_root_pairs(int2 *front, int nroots, uint32_t *count, hipStream_t s);
+// the first traversal frontier: each root with itself, or (periodic) the one
+// root with its 27 images
+hipError_t launch_root_pairs(int2 *front, int nroots, uint32_t *count, hipStream_t s,
+                             bool periodic = false);
 hipError_t launch_permute(const uint32_t *order, const double4 *rec, int64_t n, double *sx,
                           double *sy, double *sz, double *sq, int32_t *perm, hipStream_t s);
 hipError_t launch_roots(Cells c, int64_t n, const int64_t *off, int nprob,
@@ -87,6 +90,9 @@ hipError_t launch_roots(Cells c, int64_t n, const int64_t *off, int nprob,
 // count per level); grid_cells bounds the level's size; status bit 1: cell
 // capacity cap exceeded, bit 2: level larger than grid_cells
 int32_t level_blocks(int32_t cells);
+// periodic images: the region (one problem) becomes the periodic cell of
+// side cycle around the bodies' bounding cube; *flag = 1 if they span more
+hipError_t launch_periodic_region(double *region, double cycle, uint32_t *flag, hipStream_t s);
 // last: the planned tree has no level after d (splits there set ST_DEPTH)
 hipError_t launch_level_step(Cells c, int32_t *lev, int d, int32_t grid_cells, int ncrit,
                              int32_t cap, uint32_t *flag, uint32_t *loc, uint32_t *bsum,
@@ -114,6 +120,10 @@ hipError_t launch_p2m(Cells c, int32_t ncells, Bodies b, double *M, hipStream_t 
 // translate.hip (FP64 M2M / L2L on MFMA)
 size_t translation_ops_doubles();
 hipError_t launch_translation_ops(double *ops, hipStream_t s);
+// periodic images beyond the first shell (images > 1): their far field into
+// the root's local expansion, from the root's multipole
+hipError_t launch_periodic_far(const double *Mroot, double *Lroot, double cycle, int images,
+                               hipStream_t s);
 // one level of parents (npar, children from first_child); with lev the
 // children's level dl is read on the device and npar is the capacity
 hipError_t launch_m2m(Cells c, int32_t first_child, int32_t npar, const double *ops, double *M,
@@ -140,7 +150,7 @@ hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin
                            int2 *p2p, uint32_t p2p_cap, uint32_t *cnt, const double4 *geo,
                            const int2 *topo, const int32_t *depth, const double *width,
                            int32_t *m2l_slot, uint32_t *used, uint32_t *n_outside, int mutual,
-                           hipStream_t s);
+                           hipStream_t s, double cycle = 0);
 // group-table slots of M2L pairs given from outside the traversal
 hipError_t launch_pair_slots(const int2 *pairs, uint32_t n, const double4 *geo,
                              const int32_t *depth, const double *width, int32_t *slot,
@@ -161,18 +171,24 @@ hipError_t launch_csr_offsets(const uint64_t *sorted_keys, uint32_t n, int sbits
 // m2l.hip
 size_t m2l_pairs_scratch_bytes(int32_t npairs);
 // with dn the pair count is *dn on the device (npairs its capacity)
+// (list sources are source words: cell and periodic image, cycle the period)
 hipError_t launch_m2l_fp64(Cells c, int32_t ncells, Csr list, int32_t npairs, const double *M,
-                           double *L, void *scratch, hipStream_t s, const uint32_t *dn = nullptr);
+                           double *L, void *scratch, hipStream_t s, const uint32_t *dn = nullptr,
+                           double cycle = 0);
 
 // p2p.hip
+// (list sources are source words: leaf and periodic image, cycle the period)
 hipError_t launch_p2p_fp64(Cells c, int32_t ncells, Csr list, Bodies b, double *acc,
-                           hipStream_t s);
+                           hipStream_t s, double cycle = 0);
 
 // direct.hip
 // target t sums over the n bodies at x, y, z, q: index targets[t] - tbase
+// with prange > 0, also over the images of every body at offsets (ix, iy, iz)
+// cycle, |ix|, |iy|, |iz| <= prange
 hipError_t launch_direct_fp64(const double *x, const double *y, const double *z,
                               const double *q, int64_t n, const int64_t *targets, int64_t tbase,
-                              int64_t nt, double *out, hipStream_t s);
+                              int64_t nt, double *out, hipStream_t s, int prange = 0,
+                              double cycle = 0);
 
 }  // namespace tfmm
 
@@ -250,7 +266,8 @@ hipError_t launch_leaf_relpos(Cells c, int32_t ncells, Bodies b, float4 *rel, hi
 // resident > 0: that many resident waves take target cells from the counter
 // next (4 bytes, zeroed here); 0: one workgroup per cell
 hipError_t launch_p2p_fp32(Cells c, int32_t ncells, Csr list, Bodies b, const float4 *rel,
-                           double *acc, int resident, uint32_t *next, hipStream_t s);
+                           double *acc, int resident, uint32_t *next, hipStream_t s,
+                           double cycle = 0);
 }  // namespace tfmm
 
 namespace tfmm {

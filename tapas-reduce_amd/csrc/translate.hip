@@ -472,6 +472,118 @@ size_t translation_ops_doubles() {
   return (size_t)NV * KM + (size_t)8 * NV * NV + FRAG_M + FRAG_L;
 }
 
+namespace {
+
+// M2L output coefficient (j, k) from source coefficients Ms and the singular
+// harmonics Ys of dX (LaplaceSphericalCPU_tapas.h:227-250, mutual = 0)
+__device__ inline cd m2l_entry(int j, int k, const cd *Ms, const cd *Ys) {
+  const double Cnm = sgn(j);
+  cd acc = mk(0, 0);
+  for (int n = 0; n < P - j; n++) {
+    for (int m = -n; m < 0; m++)
+      acc = cadd(acc, cmul(cscale(cconj(Ms[tri(n, -m)]), Cnm), yfull(Ys, j + n, m - k)));
+    for (int m = 0; m <= n; m++) {
+      const double Cnm2 = Cnm * sgn((k - m) * (k < m) + m);
+      acc = cadd(acc, cmul(cscale(Ms[tri(n, m)], Cnm2), yfull(Ys, j + n, m - k)));
+    }
+  }
+  return acc;
+}
+
+// image (ix, iy, iz) in {-1, 0, 1}^3 of index a in the reference's loop order
+__device__ inline void image_of(int a, int &ix, int &iy, int &iz) {
+  ix = a / 9 - 1;
+  iy = a / 3 % 3 - 1;
+  iz = a % 3 - 1;
+}
+
+// The far field of the periodic images beyond the first shell
+// (EX/include/traversal.h:126-185, images > 1): at sublevel l the copies of
+// the periodic cell form cells of side cycle_l = cycle 3^l whose multipole
+// (about the root's centre) is M_l, M_0 the root's.  The 26 x 27 copies
+// (3 i + c) cycle_l, i in {-1,0,1}^3 without 0, c in {-1,0,1}^3, add their M2L
+// into the root's local expansion (dX = -Xperiodic: the root with an image of
+// itself), and M_{l+1} is the M2M of the 27 copies i cycle_l, i in {-1,0,1}^3,
+// into the cell three times larger.  (The copy at the centre is part of that
+// cell: the reference's loop resets the parent before its M2M, which would
+// leave it out; this follows the sum over all (3^images)^3 copies that its
+// direct check computes.)  One workgroup: 4 waves share the M2L offsets, each
+// in a fixed order, their partial sums added in wave order.
+__global__ __launch_bounds__(256) void k_periodic_far(const double *Mroot, double *Lroot,
+                                                      double cycle, int images) {
+  __shared__ cd Ms[NTERM], Mn[NTERM], Yl[NTERM];
+  __shared__ cd Ys[4][NTERM];
+  __shared__ cd part[4][NTERM];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int j = 0, k = 0;
+  if (lane < NTERM) untri(lane, j, k);
+  const cd *M0 = reinterpret_cast<const cd *>(Mroot);
+  if (threadIdx.x < NTERM) Ms[threadIdx.x] = M0[threadIdx.x];
+  cd acc = mk(0, 0);
+  for (int level = 0; level + 1 < images; level++) {
+    __syncthreads();
+    for (int o = w; o < 26 * 27; o += 4) {
+      const int a = o / 27, ac = a < 13 ? a : a + 1;  // the 26 images other than the centre
+      int ix, iy, iz, cx, cy, cz;
+      image_of(ac, ix, iy, iz);
+      image_of(o % 27, cx, cy, cz);
+      __builtin_amdgcn_wave_barrier();
+      if (lane < P) {
+        const double dx = -((ix * 3 + cx) * cycle), dy = -((iy * 3 + cy) * cycle),
+                     dz = -((iz * 3 + cz) * cycle);
+        double rho, ca, sa;
+        cd ei;
+        sph_frame(dx, dy, dz, rho, ca, sa, ei);
+        local_column(lane, rho, ca, sa, ei, [&](int n, int m, cd Y) { Ys[w][tri(n, m)] = Y; });
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      if (lane < NTERM) acc = cadd(acc, m2l_entry(j, k, Ms, Ys[w]));
+    }
+    __syncthreads();
+    if (w == 0) {
+      cd mn = mk(0, 0);
+      for (int a = 0; a < 27; a++) {
+        int ix, iy, iz;
+        image_of(a, ix, iy, iz);
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) {  // dX = parent - child centre = -(i cycle_l)
+          double rho, ca, sa;
+          cd ei;
+          sph_frame(-(ix * cycle), -(iy * cycle), -(iz * cycle), rho, ca, sa, ei);
+          multipole_all<false>(rho, ca, sa, ei, [&](int n, int m, cd v, cd) { Yl[tri(n, m)] = v; });
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        if (lane < NTERM) mn = cadd(mn, m2m_entry(j, k, Ms, Yl));
+      }
+      if (lane < NTERM) Mn[lane] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x < NTERM) Ms[threadIdx.x] = Mn[threadIdx.x];
+    cycle *= 3;
+  }
+  if (lane < NTERM) part[w][lane] = acc;
+  __syncthreads();
+  if (threadIdx.x < NTERM) {
+    cd s = part[0][threadIdx.x];
+    for (int u = 1; u < 4; u++) s = cadd(s, part[u][threadIdx.x]);
+    cd *L0 = reinterpret_cast<cd *>(Lroot);
+    L0[threadIdx.x] = cadd(L0[threadIdx.x], s);
+  }
+}
+
+}  // namespace
+
+hipError_t launch_periodic_far(const double *Mroot, double *Lroot, double cycle, int images,
+                               hipStream_t s) {
+  if (images < 2) return hipSuccess;
+  k_periodic_far<<<1, 256, 0, s>>>(Mroot, Lroot, cycle, images);
+  return hipGetLastError();
+}
+
 // The unit operators (built once per context); ops: translation_ops_doubles()
 // doubles: the M2M operator, the 8 L2L operators, then both in fragment order.
 hipError_t launch_translation_ops(double *ops, hipStream_t s) {

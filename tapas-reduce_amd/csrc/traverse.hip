@@ -67,7 +67,7 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
                                                   const double4 *geo, const int2 *topo,
                                                   const int32_t *depth, const double *width,
                                                   int32_t *m2l_slot, uint32_t *used,
-                                                  uint32_t *n_outside, int mutual) {
+                                                  uint32_t *n_outside, int mutual, double cycle) {
 #pragma clang fp contract(off)
   const uint32_t nin = nin_dev ? *nin_dev : nin_max;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -92,12 +92,18 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
     if (valid) {
       const int2 pr = in[t];
       i = pr.x;
-      j = pr.y;
-      const double4 gi = geo[i], gj = geo[j];
-      const int2 ti = topo[i], tj = topo[j];
-      const double d0 = gi.x - gj.x;
-      const double d1 = gi.y - gj.y;
-      const double d2 = gi.z - gj.z;
+      j = pr.y;  // source cell and image (periodic images)
+      const int32_t jc = (int32_t)((uint32_t)j & CELL_MASK);
+      const uint32_t img = (uint32_t)j >> IMG_SHIFT;
+      double sx, sy, sz;
+      image_shift(img, cycle, sx, sy, sz);
+      const bool same = i == jc && (cycle == 0 || img == IMG_CENTRE);  // a cell with itself
+      const double4 gi = geo[i], gj = geo[jc];
+      const int2 ti = topo[i], tj = topo[jc];
+      // dX = Xi - Xj - Xperiodic (EX/include/traversal.h:37)
+      const double d0 = gi.x - gj.x - sx;
+      const double d1 = gi.y - gj.y - sy;
+      const double d2 = gi.z - gj.z - sz;
       double R2 = 0;
       R2 += d0 * d0;
       R2 += d1 * d1;
@@ -111,28 +117,28 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
         if (nonempty) {
           // the pair's M2L offset group (same arithmetic on the same centres
           // as a separate pass over the list would do)
-          const double ci[3] = {gi.x, gi.y, gi.z}, cj[3] = {gj.x, gj.y, gj.z};
-          g = m2lg::pair_slot(depth[i], depth[j], width[i], width[j], ci, cj);
-          if (two) g2 = m2lg::pair_slot(depth[j], depth[i], width[j], width[i], cj, ci);
+          const double ci[3] = {gi.x, gi.y, gi.z}, cj[3] = {gj.x + sx, gj.y + sy, gj.z + sz};
+          g = m2lg::pair_slot(depth[i], depth[jc], width[i], width[jc], ci, cj);
+          if (two) g2 = m2lg::pair_slot(depth[jc], depth[i], width[jc], width[i], cj, ci);
         }
       } else if (leaf_i && leaf_j) {
         if (nonempty) {
           k = DK_P2P;
-          two = mutual != 0 && i != j;
-          if (mutual != 0 && i == j) {
+          two = mutual != 0 && !same;
+          if (mutual != 0 && same) {
             v += (unsigned long long)ti.y * (unsigned long long)(ti.y + 1) / 2;
           } else {
             v += (unsigned long long)ti.y * (unsigned long long)tj.y;
           }
-          if (i == j) self += (unsigned long long)ti.y;
+          if (same) self += (unsigned long long)ti.y;
         }
       } else if (leaf_j) {
         k = DK_SPLIT_I;
         f = ti.x;
       } else if (leaf_i) {
         k = DK_SPLIT_J;
-        f = tj.x;
-      } else if (mutual != 0 && i == j) {
+        f = tj.x | (int32_t)(img << IMG_SHIFT);
+      } else if (mutual != 0 && same) {
         k = DK_SPLIT_BOTH;
         f = ti.x;
       } else if (Ri >= Rj) {
@@ -140,7 +146,7 @@ __global__ __launch_bounds__(DTT_THREADS) void k_dtt_step(const int2 *in, const 
         f = ti.x;
       } else {
         k = DK_SPLIT_J;
-        f = tj.x;
+        f = tj.x | (int32_t)(img << IMG_SHIFT);
       }
     }
     pi[r] = i;
@@ -390,12 +396,12 @@ hipError_t launch_dtt_step(const int2 *in, const uint32_t *nin_dev, uint32_t nin
                            int2 *p2p, uint32_t p2p_cap, uint32_t *cnt, const double4 *geo,
                            const int2 *topo, const int32_t *depth, const double *width,
                            int32_t *m2l_slot, uint32_t *used, uint32_t *n_outside, int mutual,
-                           hipStream_t s) {
+                           hipStream_t s, double cycle) {
   if (nin_max == 0) return hipSuccess;
   const uint32_t per = DTT_THREADS * DTT_PPT;
   k_dtt_step<<<(nin_max + per - 1) / per, DTT_THREADS, 0, s>>>(
       in, nin_dev, nin_max, out, out_cap, nout, m2l, m2l_cap, p2p, p2p_cap, cnt, geo, topo, depth,
-      width, m2l_slot, used, n_outside, mutual);
+      width, m2l_slot, used, n_outside, mutual, cycle);
   return hipGetLastError();
 }
 

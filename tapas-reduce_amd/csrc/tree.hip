@@ -221,10 +221,14 @@ __global__ void k_roots(Cells c, int64_t n, const int64_t *off, int nprob,
 }
 
 // Traversal start: the pair (root, root) of every problem.
-__global__ void k_root_pairs(int2 *front, int nroots, uint32_t *count) {
+// The traversal's first frontier: every root with itself, or with periodic
+// images the one root with each of its 27 images (EX/include/traversal.h:
+// 256-266).
+__global__ void k_root_pairs(int2 *front, int nroots, uint32_t *count, int periodic) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nroots) front[i] = make_int2(i, i);
-  if (i == 0) *count = (uint32_t)nroots;
+  const int n = periodic ? 27 : nroots;
+  if (i < n) front[i] = periodic ? make_int2(0, (int32_t)((uint32_t)i << IMG_SHIFT)) : make_int2(i, i);
+  if (i == 0) *count = (uint32_t)n;
 }
 
 __global__ void k_split_flags(Cells c, int32_t lbeg, int32_t lcnt, int ncrit, uint32_t *flag) {
@@ -400,7 +404,26 @@ __global__ void k_check_finite(const double *x, const double *y, const double *z
   if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
 }
 
+// Periodic images: the root cell is the periodic cell, a cube of side cycle
+// centred on the bodies' scalar bounding cube (region: min x, y, z, max x, y,
+// z); flag = 1 when the bodies span more than one period.
+__global__ void k_periodic_region(double *region, double cycle, uint32_t *flag) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double lo = region[0], hi = region[3];
+  if (!(hi - lo <= cycle)) *flag = 1u;
+  const double c = 0.5 * (lo + hi), h = 0.5 * cycle;
+  for (int d = 0; d < 3; d++) {
+    region[d] = c - h;
+    region[3 + d] = c + h;
+  }
+}
+
 }  // namespace
+
+hipError_t launch_periodic_region(double *region, double cycle, uint32_t *flag, hipStream_t s) {
+  k_periodic_region<<<1, 64, 0, s>>>(region, cycle, flag);
+  return hipGetLastError();
+}
 
 hipError_t launch_check_finite(const double *x, const double *y, const double *z,
                                const double *q, int64_t n, uint32_t *flag, hipStream_t s) {
@@ -450,8 +473,10 @@ hipError_t launch_problem_gather(const uint32_t *pos, const uint64_t *keys1,
   return hipGetLastError();
 }
 
-hipError_t launch_root_pairs(int2 *front, int nroots, uint32_t *count, hipStream_t s) {
-  k_root_pairs<<<(nroots + 255) / 256, 256, 0, s>>>(front, nroots, count);
+hipError_t launch_root_pairs(int2 *front, int nroots, uint32_t *count, hipStream_t s,
+                             bool periodic) {
+  const int n = periodic ? 27 : nroots;
+  k_root_pairs<<<(n + 255) / 256, 256, 0, s>>>(front, nroots, count, periodic ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -24,6 +24,7 @@ TUNE_GRAPH = 4
 TUNE_P2P_WAVES = 5
 TUNE_DEVICE_LIMIT = 6
 DIRECT_NOALLOC_MAX = 65536
+PERIODIC_MAX_IMAGES = 6
 P = 10
 NTERM = 55
 
@@ -77,7 +78,7 @@ EXPORTED = [
     "tfmm_upward",
     "tfmm_traverse", "tfmm_downward", "tfmm_evaluate", "tfmm_get_fields", "tfmm_get_stats",
     "tfmm_get_sorted", "tfmm_get_cells", "tfmm_get_expansions", "tfmm_get_lists", "tfmm_direct",
-    "tfmm_direct_noalloc", "tfmm_traverse_lists",
+    "tfmm_direct_noalloc", "tfmm_traverse_lists", "tfmm_set_periodic",
     "tfmm_picker_nparam", "tfmm_picker_create", "tfmm_picker_destroy", "tfmm_picker_last_error",
     "tfmm_picker_set_weights", "tfmm_picker_get_weights", "tfmm_picker_set_theta_max",
     "tfmm_picker_train",
@@ -112,6 +113,7 @@ def lib():
                                           ctypes.POINTER(vp), ctypes.POINTER(Params)]
         L.tfmm_num_problems.argtypes = [vp, vp]
         L.tfmm_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.tfmm_set_periodic.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.tfmm_get_tuning.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.tfmm_autotune.argtypes = [vp, ctypes.POINTER(Params), ctypes.c_int, vp]
         L.tfmm_build_tree.argtypes = [vp, ctypes.POINTER(Params)]
@@ -244,6 +246,11 @@ class Context:
         self._check(lib().tfmm_set_batch_device(self._h, len(off) - 1, _ptr(off), px, py, pz,
                                                 pq), "tfmm_set_batch_device")
         self.n = int(off[-1])
+
+    def set_periodic(self, images, cycle=0.0):
+        """periodic images (0: none) of period cycle; see tfmm_set_periodic"""
+        self._check(lib().tfmm_set_periodic(self._h, int(images), float(cycle)),
+                    "tfmm_set_periodic")
 
     def set_tuning(self, knob, value):
         self._check(lib().tfmm_set_tuning(self._h, int(knob), float(value)), "tfmm_set_tuning")
