@@ -7,7 +7,9 @@
 //
 // Options (as the reference's EX/include/args.h): -n bodies, -c ncrit,
 // -t theta, -m mutual (0 or 1), -d cube|plummer|sphere, -r repeat,
-// -k check (0/1), plus -p fp64|mixed.  Bodies follow the reference's
+// -k check (0/1), -i images (periodic images of period 2 pi, which the
+// reference's args take but its Tapas traversal leaves at 0; with -m 0),
+// plus -p fp64|mixed.  Bodies follow the reference's
 // dataset semantics (EX/include/dataset.h:131-271): cube U[-pi,pi)^3 or the
 // Plummer sphere scaled by 3pi/16, charges drand48() - 0.5, srand48(0).
 #include <unistd.h>
@@ -69,11 +71,11 @@ static std::vector<Body> init_bodies(int n, const std::string &dist) {
 }
 
 int main(int argc, char **argv) {
-  int n = 1000000, ncrit = 16, mutual = 1, repeat = 1, check = 1;
+  int n = 1000000, ncrit = 16, mutual = 1, repeat = 1, check = 1, images = 0;
   double theta = 0.4;
   std::string dist = "cube", prec = "fp64";
   int opt;
-  while ((opt = getopt(argc, argv, "n:c:t:m:d:r:k:p:")) != -1) {
+  while ((opt = getopt(argc, argv, "n:c:t:m:d:r:k:p:i:")) != -1) {
     switch (opt) {
       case 'n': n = atoi(optarg); break;
       case 'c': ncrit = atoi(optarg); break;
@@ -83,6 +85,7 @@ int main(int argc, char **argv) {
       case 'r': repeat = atoi(optarg); break;
       case 'k': check = atoi(optarg); break;
       case 'p': prec = optarg; break;
+      case 'i': images = atoi(optarg); break;
       default: fprintf(stderr, "unknown option\n"); return 2;
     }
   }
@@ -90,11 +93,17 @@ int main(int argc, char **argv) {
     fprintf(stderr, "TapasFMM: [Error] mutual must be 0 or 1.\n");
     return 1;
   }
+  if (images > 0 && mutual) {
+    fprintf(stderr, "TapasFMM: [Error] periodic images are traversed with -m 0.\n");
+    return 1;
+  }
   const Precision precision = prec == "mixed" ? Precision::Mixed : Precision::FP64;
+  const double cycle = 2 * M_PI;  // the reference's period (tapas_exafmm.cxx:537)
   std::vector<Body> bodies = init_bodies(n, dist);
   for (int t = 0; t < repeat; t++) {
     const double t0 = now();
-    Cell *root = TapasFMM::Partition(bodies.data(), (index_t)bodies.size(), ncrit, 0, precision);
+    Cell *root = TapasFMM::Partition(bodies.data(), (index_t)bodies.size(), ncrit, 0, precision,
+                                     images, cycle);
     root->SetOptMutual(mutual);
     const double t1 = now();
     TapasFMM::Map(FMM_Upward(), *root, theta);
@@ -126,7 +135,8 @@ int main(int argc, char **argv) {
            n, (long long)st.ncells, st.max_depth, t1 - t0, t2 - t1, t3 - t2, t4 - t3, t4 - t0,
            st.ms_tree, st.ms_upward, st.ms_traverse, st.ms_downward);
     if (check) {
-      // 100 stride-sampled targets (sorted order) against the direct sum
+      // 100 stride-sampled targets (sorted order) against the direct sum (with
+      // periodic images over the same copies, CheckResult's prange loops)
       const int ns = std::min<int>(100, (int)out.size());
       const int stride = (int)out.size() / ns;
       std::vector<int64_t> tg(ns);

@@ -542,10 +542,14 @@ struct TapasFMM {
 
   // Copies the bodies and builds the octree on the GPU.  theta and the
   // precision are taken at Map(FMM_DTT, ...); the tree does not depend on them.
+  // images > 0: periodic images of period cycle (tfmm_set_periodic; the
+  // reference's --images, EX/include/args.h, with its cycle 2 pi).
   static Cell *Partition(Body *b, index_t nb, int ncrit, int device = 0,
-                         Precision precision = Precision::FP64) {
+                         Precision precision = Precision::FP64, int images = 0,
+                         double cycle = 0) {
     auto t = std::make_shared<detail::Tree>();
     t->check(tfmm_create(device, &t->ctx), "tfmm_create");
+    if (images > 0) t->check(tfmm_set_periodic(t->ctx, images, cycle), "tfmm_set_periodic");
     tfmm_body_desc d;
     d.n = nb;
     d.stride = sizeof(Body);

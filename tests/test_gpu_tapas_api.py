@@ -27,6 +27,21 @@ def test_driver_rejects_bad_mutual(tmp_path):
     """mutual must be 0 or 1; refused before any device work."""
     r = run(["-n", "1000", "-m", "2"], str(tmp_path))
     assert r.returncode == 1 and "mutual must be 0 or 1" in r.stderr
+    r = run(["-n", "1000", "-m", "1", "-i", "1"], str(tmp_path))  # images with mutual
+    assert r.returncode == 1 and "-m 0" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("images", [1, 2])
+def test_driver_periodic_images(tmp_path, images):
+    """-i images (period 2 pi, the reference's cycle): the FMM against the
+    direct sum over the same copies of the bodies, the reference's 1e-3 gate."""
+    print("driver periodic images", images, flush=True)
+    r = run(["-n", "4000", "-c", "32", "-t", "0.4", "-m", "0", "-d", "cube", "-i", str(images)],
+            str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    errs = [float(v) for v in re.findall(r"Rel\. L2 Error \((?:pot|acc)\)\s*:\s*(\S+)", r.stdout)]
+    assert len(errs) == 2 and max(errs) < 1e-3, r.stdout
 
 
 @pytest.mark.gpu
