@@ -24,6 +24,9 @@
  *   downward pass (L2L, L2P)       EX/examples/tapas_exafmm.cxx:124-140,
  *                                  EX/examples/LaplaceSphericalCPU_tapas.h:275-345
  *   solid harmonics                EX/examples/LaplaceSphericalCPU_tapas.h:43-135
+ *   Cartesian basis (P2M, M2M, M2L, EX/kernels/LaplaceCartesianCPU.cxx:3-704
+ *     L2L, L2P)                    (the reference's BASIS = Cartesian build)
+ *   periodic images                EX/include/traversal.h:126-185, :247-268
  *   direct sum + error metric      EX/examples/tapas_exafmm.cxx:288-383,
  *                                  EX/include/verify.h:18-92
  *
@@ -421,6 +424,205 @@ void orc_l2p(double *trg, const double *bx, double q, const double *L, const dou
 }
 
 /* ------------------------------------------------------------------ */
+/* Cartesian Taylor basis (EX/kernels/LaplaceCartesianCPU.cxx, the    */
+/* reference's BASIS = Cartesian build, EX/include/types.h:39-41)      */
+/* ------------------------------------------------------------------ */
+/* Terms of degree 0 .. P-1 in three variables: CN = P (P+1) (P+2) / 6.
+ * Term k = (kx, ky, kz) of degree d sits at
+ *   index(k) = d (d+1) (d+2) / 6 + (d-kx) (d-kx+1) / 2 + kz,
+ * i.e. degree by degree, kx falling, then ky falling (the order of the
+ * reference's Index<nx,ny,nz>::I, LaplaceCartesianCPU.cxx:3-25).
+ *   multipole  M_k = sum_b q_b (X - x_b)^k / k!             (P2M, :636-644)
+ *   local      phi(X + e) = sum_n L_n e^n / n!
+ *   M2L        L_n += sum_{|k| <= P-1-|n|} D^{n+k}(1/r)(R) M_k,
+ *              R = Xi - Xj - Xperiodic                        (:659-673)
+ *   M2M        M_n += sum_{k <= n} (Xp - Xc)^{n-k} / (n-k)! M_k  (:646-657)
+ *   L2L        L_n += sum_{|k| <= P-1-|n|} (Xc - Xp)^k / k! L_{n+k}  (:675-689)
+ *   L2P        trg /= q; pot += sum_k e^k/k! L_k, grad_d += sum_{|k| <= P-2}
+ *              e^k/k! L_{k+e_d}, e = x - X                    (:691-704)
+ * D^k(1/r) comes from the Taylor coefficients a_k = D^k(1/r) / k! by the
+ * recurrence of 1/r (:45-104, :312-324):
+ *   |k| r^2 a_k = (1 - 2|k|) sum_i x_i a_{k-e_i} + (1 - |k|) sum_i a_{k-2e_i}. */
+#define CN (P * (P + 1) * (P + 2) / 6)
+
+static int cart_index(int x, int y, int z) {
+  const int d = x + y + z;
+  return d * (d + 1) * (d + 2) / 6 + (d - x) * (d - x + 1) / 2 + z;
+}
+
+/* multi-index of every term, in index order */
+static void cart_terms(int mi[CN][3]) {
+  int t = 0;
+  for (int d = 0; d < P; d++)
+    for (int x = d; x >= 0; x--)
+      for (int y = d - x; y >= 0; y--) {
+        mi[t][0] = x;
+        mi[t][1] = y;
+        mi[t][2] = d - x - y;
+        t++;
+      }
+}
+
+/* C_k = c0 dX^k / k!, each term from its predecessor along z, else y, else x
+ * (the reference's power, :167-171, :198-201, :229-232) */
+static void cart_powers(double *C, const double dX[3], double c0) {
+  int mi[CN][3];
+  cart_terms(mi);
+  C[0] = c0;
+  for (int t = 1; t < CN; t++) {
+    const int x = mi[t][0], y = mi[t][1], z = mi[t][2];
+    if (z > 0) C[t] = C[cart_index(x, y, z - 1)] * dX[2] / z;
+    else if (y > 0) C[t] = C[cart_index(x, y - 1, 0)] * dX[1] / y;
+    else C[t] = C[cart_index(x - 1, 0, 0)] * dX[0] / x;
+  }
+}
+
+/* C_k = D^k (1/r) at dX */
+static void cart_derivs(double *C, const double dX[3]) {
+  int mi[CN][3];
+  cart_terms(mi);
+  const double r2 = dX[0] * dX[0] + dX[1] * dX[1] + dX[2] * dX[2];
+  const double invR2 = 1.0 / r2;
+  double a[CN];
+  a[0] = sqrt(invR2);
+  for (int t = 1; t < CN; t++) {
+    const int k[3] = {mi[t][0], mi[t][1], mi[t][2]};
+    const int n = k[0] + k[1] + k[2];
+    double s1 = 0, s2 = 0;
+    for (int i = 0; i < 3; i++) {
+      int km[3] = {k[0], k[1], k[2]};
+      if (k[i] >= 1) {
+        km[i] -= 1;
+        s1 += dX[i] * a[cart_index(km[0], km[1], km[2])];
+      }
+      if (k[i] >= 2) {
+        km[i] -= 1;
+        s2 += a[cart_index(km[0], km[1], km[2])];
+      }
+    }
+    a[t] = ((1 - 2 * n) * s1 + (1 - n) * s2) / n * invR2;
+  }
+  for (int t = 0; t < CN; t++) {
+    double f = 1;
+    for (int i = 0; i < 3; i++)
+      for (int j = 2; j <= mi[t][i]; j++) f *= j;
+    C[t] = a[t] * f;
+  }
+}
+
+static void cart_p2m(double *M, const double *cen, const double *x, const double *y,
+                     const double *z, const double *q, int64_t nb) {
+  double C[CN];
+  for (int64_t b = 0; b < nb; b++) {
+    const double dX[3] = {cen[0] - x[b], cen[1] - y[b], cen[2] - z[b]};
+    cart_powers(C, dX, q[b]);
+    for (int t = 0; t < CN; t++) M[t] += C[t];
+  }
+}
+
+/* dX = parent centre - child centre */
+static void cart_m2m(double *Mp, const double *Mc, const double dX[3]) {
+  int mi[CN][3];
+  cart_terms(mi);
+  double C[CN];
+  cart_powers(C, dX, 1.0);
+  for (int t = 0; t < CN; t++) {
+    const int nx = mi[t][0], ny = mi[t][1], nz = mi[t][2];
+    double s = 0;
+    for (int kx = 0; kx <= nx; kx++)
+      for (int ky = 0; ky <= ny; ky++)
+        for (int kz = 0; kz <= nz; kz++)
+          s += C[cart_index(nx - kx, ny - ky, nz - kz)] * Mc[cart_index(kx, ky, kz)];
+    Mp[t] += s;
+  }
+}
+
+/* L_n += sum_{|k| <= P-1-|n|} C_{n+k} M_k */
+static void cart_contract(double *L, const double *C, const double *M) {
+  int mi[CN][3];
+  cart_terms(mi);
+  for (int t = 0; t < CN; t++) {
+    const int nx = mi[t][0], ny = mi[t][1], nz = mi[t][2];
+    const int left = P - 1 - (nx + ny + nz);
+    double s = 0;
+    for (int u = 0; u < CN; u++) {
+      if (mi[u][0] + mi[u][1] + mi[u][2] > left) break;
+      s += C[cart_index(nx + mi[u][0], ny + mi[u][1], nz + mi[u][2])] * M[u];
+    }
+    L[t] += s;
+  }
+}
+
+/* dX = target centre - source centre (- Xperiodic) */
+static void cart_m2l(double *Li, const double *Mj, const double dX[3]) {
+  double C[CN];
+  cart_derivs(C, dX);
+  cart_contract(Li, C, Mj);
+}
+
+/* both directions of one pair: D^k(1/r) at -dX is (-1)^|k| D^k(1/r) at dX
+ * (the reference's Coefs::negate, :612-633) */
+static void cart_m2l_mutual(double *Li, double *Lj, const double *Mi, const double *Mj,
+                            const double dX[3]) {
+  int mi[CN][3];
+  cart_terms(mi);
+  double C[CN];
+  cart_derivs(C, dX);
+  cart_contract(Li, C, Mj);
+  for (int t = 0; t < CN; t++)
+    if ((mi[t][0] + mi[t][1] + mi[t][2]) & 1) C[t] = -C[t];
+  cart_contract(Lj, C, Mi);
+}
+
+/* dX = child centre - parent centre */
+static void cart_l2l(double *Lc, const double *Lp, const double dX[3]) {
+  double C[CN];
+  cart_powers(C, dX, 1.0);
+  cart_contract(Lc, Lp, C);
+}
+
+static void cart_l2p(double *trg, const double bx[3], double q, const double *L,
+                     const double cen[3]) {
+  int mi[CN][3];
+  cart_terms(mi);
+  const double dX[3] = {bx[0] - cen[0], bx[1] - cen[1], bx[2] - cen[2]};
+  double C[CN];
+  cart_powers(C, dX, 1.0);
+  for (int d = 0; d < 4; d++) trg[d] /= q;
+  double s[4] = {0, 0, 0, 0};
+  for (int t = 0; t < CN; t++) {
+    const int x = mi[t][0], y = mi[t][1], z = mi[t][2];
+    s[0] += C[t] * L[t];
+    if (x + y + z <= P - 2) {
+      s[1] += C[t] * L[cart_index(x + 1, y, z)];
+      s[2] += C[t] * L[cart_index(x, y + 1, z)];
+      s[3] += C[t] * L[cart_index(x, y, z + 1)];
+    }
+  }
+  for (int d = 0; d < 4; d++) trg[d] += s[d];
+}
+
+/* Raw-array Cartesian entry points (CN doubles per expansion). */
+int orc_cart_terms(void) { return CN; }
+int orc_cart_index(int x, int y, int z) { return cart_index(x, y, z); }
+void orc_cart_derivs(double *C, const double *dX) { cart_derivs(C, dX); }
+void orc_cart_powers(double *C, const double *dX, double c0) { cart_powers(C, dX, c0); }
+void orc_cart_p2m(double *M, const double *cen, const double *x, const double *y,
+                  const double *z, const double *q, int64_t nb) {
+  cart_p2m(M, cen, x, y, z, q, nb);
+}
+void orc_cart_m2m(double *Mp, const double *Mc, const double *dX) { cart_m2m(Mp, Mc, dX); }
+void orc_cart_m2l(double *Li, const double *Mj, const double *dX) { cart_m2l(Li, Mj, dX); }
+void orc_cart_m2l_mutual(double *Li, double *Lj, const double *Mi, const double *Mj,
+                         const double *dX) {
+  cart_m2l_mutual(Li, Lj, Mi, Mj, dX);
+}
+void orc_cart_l2l(double *Lc, const double *Lp, const double *dX) { cart_l2l(Lc, Lp, dX); }
+void orc_cart_l2p(double *trg, const double *bx, double q, const double *L, const double *cen) {
+  cart_l2p(trg, bx, q, L, cen);
+}
+
+/* ------------------------------------------------------------------ */
 /* Tree, traversal and full evaluation                                  */
 /* ------------------------------------------------------------------ */
 
@@ -462,6 +664,10 @@ struct orc_ctx {
    * of list sources as the library's lists hold it) */
   double xp[3];
   int64_t img;
+  /* expansion basis: 0 spherical (M, L: NTERM complex per cell), 1 Cartesian
+   * (Mc, Lc: CN doubles per cell) */
+  int basis;
+  double *Mc, *Lc;
 };
 
 static int64_t lower_bound_u64(const uint64_t *a, int64_t lo, int64_t hi, uint64_t v) {
@@ -520,7 +726,29 @@ static int cmp_kidx(const void *a, const void *b) {
 
 static double cell_width(const ocell *c, int d) { return c->cmax[d] - c->cmin[d]; }
 
+static void upward_cart(orc_ctx *c, int64_t ci) {
+  ocell *cell = &c->cells[ci];
+  double *M = c->Mc + ci * CN;
+  for (int i = 0; i < CN; i++) M[i] = c->Lc[ci * CN + i] = 0;
+  if (cell->leaf) {
+    int64_t b = cell->bid;
+    cart_p2m(M, cell->center, c->x + b, c->y + b, c->z + b, c->q + b, cell->nb);
+  } else {
+    for (int i = 0; i < 8; i++) upward_cart(c, cell->child[i]);
+    for (int i = 0; i < 8; i++) {
+      const ocell *ch = &c->cells[cell->child[i]];
+      double dX[3];
+      for (int d = 0; d < 3; d++) dX[d] = cell->center[d] - ch->center[d];
+      cart_m2m(M, c->Mc + (int64_t)cell->child[i] * CN, dX);
+    }
+  }
+}
+
 static void upward(orc_ctx *c, int64_t ci) {
+  if (c->basis) {
+    upward_cart(c, ci);
+    return;
+  }
   ocell *cell = &c->cells[ci];
   cx *M = c->M + ci * NTERM;
   for (int i = 0; i < NTERM; i++) { M[i] = cx_make(0, 0); c->L[ci * NTERM + i] = cx_make(0, 0); }
@@ -607,7 +835,11 @@ static void dtt(orc_ctx *c, int64_t ii, int64_t jj) {
   Rj = (Rj / 2 * 1.00001) / c->theta;
   if (R2 > (Ri + Rj) * (Ri + Rj)) {
     double dX[3] = {d0, d1, d2};
-    if (c->mutual)
+    if (c->basis && c->mutual)
+      cart_m2l_mutual(c->Lc + ii * CN, c->Lc + jj * CN, c->Mc + ii * CN, c->Mc + jj * CN, dX);
+    else if (c->basis)
+      cart_m2l(c->Lc + ii * CN, c->Mc + jj * CN, dX);
+    else if (c->mutual)
       m2l_mutual(c->L + ii * NTERM, c->L + jj * NTERM, c->M + ii * NTERM, c->M + jj * NTERM, dX);
     else
       m2l(c->L + ii * NTERM, c->M + jj * NTERM, dX);
@@ -645,12 +877,18 @@ static void downward(orc_ctx *c, int64_t ci) {
     const ocell *par = &c->cells[cell->parent];
     double dX[3];
     for (int d = 0; d < 3; d++) dX[d] = cell->center[d] - par->center[d];
-    l2l(c->L + ci * NTERM, c->L + (int64_t)cell->parent * NTERM, dX);
+    if (c->basis)
+      cart_l2l(c->Lc + ci * CN, c->Lc + (int64_t)cell->parent * CN, dX);
+    else
+      l2l(c->L + ci * NTERM, c->L + (int64_t)cell->parent * NTERM, dX);
   }
   if (cell->leaf) {
     for (int64_t b = cell->bid; b < cell->bid + cell->nb; b++) {
       double bx[3] = {c->x[b], c->y[b], c->z[b]};
-      l2p(c->trg + 4 * b, bx, c->q[b], c->L + ci * NTERM, cell->center);
+      if (c->basis)
+        cart_l2p(c->trg + 4 * b, bx, c->q[b], c->Lc + ci * CN, cell->center);
+      else
+        l2p(c->trg + 4 * b, bx, c->q[b], c->L + ci * NTERM, cell->center);
     }
   } else {
     for (int i = 0; i < 8; i++) downward(c, cell->child[i]);
@@ -667,10 +905,11 @@ orc_ctx *orc_run(const double *x, const double *y, const double *z, const double
  * centred on the bodies' scalar bounding cube (NULL if they span more). */
 static orc_ctx *setup(const double *x, const double *y, const double *z, const double *q,
                       int64_t n, int ncrit, double theta, int record_lists, int mutual,
-                      double cycle) {
-  if (n <= 0 || ncrit <= 0 || !(theta > 0)) return NULL;
+                      double cycle, int basis) {
+  if (n <= 0 || ncrit <= 0 || !(theta > 0) || basis < 0 || basis > 1) return NULL;
   orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
   c->mutual = mutual ? 1 : 0;
+  c->basis = basis;
   c->n = n;
   c->ncrit = ncrit;
   c->theta = theta;
@@ -708,15 +947,26 @@ static orc_ctx *setup(const double *x, const double *y, const double *z, const d
   }
   free(ki);
   generate_cell(c, 0, 0, n, -1);
-  c->M = (cx *)calloc((size_t)c->ncells * NTERM, sizeof(cx));
-  c->L = (cx *)calloc((size_t)c->ncells * NTERM, sizeof(cx));
+  if (basis) {
+    c->Mc = (double *)calloc((size_t)c->ncells * CN, sizeof(double));
+    c->Lc = (double *)calloc((size_t)c->ncells * CN, sizeof(double));
+  } else {
+    c->M = (cx *)calloc((size_t)c->ncells * NTERM, sizeof(cx));
+    c->L = (cx *)calloc((size_t)c->ncells * NTERM, sizeof(cx));
+  }
   return c;
 }
 
 orc_ctx *orc_run_mutual(const double *x, const double *y, const double *z, const double *q,
                         int64_t n, int ncrit, double theta, int phases, int record_lists,
                         int mutual) {
-  orc_ctx *c = setup(x, y, z, q, n, ncrit, theta, record_lists, mutual, 0.0);
+  return orc_run_basis(x, y, z, q, n, ncrit, theta, phases, record_lists, mutual, 0);
+}
+
+orc_ctx *orc_run_basis(const double *x, const double *y, const double *z, const double *q,
+                       int64_t n, int ncrit, double theta, int phases, int record_lists,
+                       int mutual, int basis) {
+  orc_ctx *c = setup(x, y, z, q, n, ncrit, theta, record_lists, mutual, 0.0, basis);
   if (!c) return NULL;
   if (phases >= 1) upward(c, 0);
   if (phases >= 2) dtt(c, 0, 0);
@@ -733,7 +983,39 @@ orc_ctx *orc_run_mutual(const double *x, const double *y, const double *z, const
  * (M2M, dX = parent - child centre; the reference's loop resets the parent
  * before this M2M, leaving the centre copy out: the sum over all
  * (3^images)^3 copies, which its direct check computes, needs it). */
+static void periodic_far_cart(orc_ctx *c, int images, double cycle) {
+  double Ms[CN], Mn[CN];
+  memcpy(Ms, c->Mc, sizeof(Ms));
+  for (int level = 0; level < images - 1; level++) {
+    for (int ix = -1; ix <= 1; ix++)
+      for (int iy = -1; iy <= 1; iy++)
+        for (int iz = -1; iz <= 1; iz++) {
+          if (ix == 0 && iy == 0 && iz == 0) continue;
+          for (int cx_ = -1; cx_ <= 1; cx_++)
+            for (int cy = -1; cy <= 1; cy++)
+              for (int cz = -1; cz <= 1; cz++) {
+                const double dX[3] = {-((ix * 3 + cx_) * cycle), -((iy * 3 + cy) * cycle),
+                                      -((iz * 3 + cz) * cycle)};
+                cart_m2l(c->Lc, Ms, dX);
+              }
+        }
+    memset(Mn, 0, sizeof(Mn));
+    for (int ix = -1; ix <= 1; ix++)
+      for (int iy = -1; iy <= 1; iy++)
+        for (int iz = -1; iz <= 1; iz++) {
+          const double dX[3] = {-(ix * cycle), -(iy * cycle), -(iz * cycle)};
+          cart_m2m(Mn, Ms, dX);
+        }
+    memcpy(Ms, Mn, sizeof(Ms));
+    cycle *= 3;
+  }
+}
+
 static void periodic_far(orc_ctx *c, int images, double cycle) {
+  if (c->basis) {
+    periodic_far_cart(c, images, cycle);
+    return;
+  }
   cx Ms[NTERM], Mn[NTERM];
   memcpy(Ms, c->M, sizeof(Ms));
   for (int level = 0; level < images - 1; level++) {
@@ -768,8 +1050,15 @@ static void periodic_far(orc_ctx *c, int images, double cycle) {
 orc_ctx *orc_run_periodic(const double *x, const double *y, const double *z, const double *q,
                           int64_t n, int ncrit, double theta, int images, double cycle,
                           int phases, int record_lists) {
+  return orc_run_periodic_basis(x, y, z, q, n, ncrit, theta, images, cycle, phases,
+                                record_lists, 0);
+}
+
+orc_ctx *orc_run_periodic_basis(const double *x, const double *y, const double *z,
+                                const double *q, int64_t n, int ncrit, double theta, int images,
+                                double cycle, int phases, int record_lists, int basis) {
   if (images < 1 || !(cycle > 0)) return NULL;
-  orc_ctx *c = setup(x, y, z, q, n, ncrit, theta, record_lists, 0, cycle);
+  orc_ctx *c = setup(x, y, z, q, n, ncrit, theta, record_lists, 0, cycle, basis);
   if (!c) return NULL;
   if (phases >= 1) upward(c, 0);
   if (phases >= 2) {
@@ -793,7 +1082,8 @@ orc_ctx *orc_run_periodic(const double *x, const double *y, const double *z, con
 void orc_free(orc_ctx *c) {
   if (!c) return;
   free(c->keys); free(c->perm); free(c->x); free(c->y); free(c->z); free(c->q); free(c->trg);
-  free(c->cells); free(c->M); free(c->L); free(c->m2l_list); free(c->p2p_list);
+  free(c->cells); free(c->M); free(c->L); free(c->Mc); free(c->Lc);
+  free(c->m2l_list); free(c->p2p_list);
   free(c);
 }
 
@@ -823,10 +1113,12 @@ void orc_cells(const orc_ctx *c, uint64_t *keys, int64_t *nb, int64_t *bid, int3
   }
 }
 
+/* NTERM complex (spherical) or CN doubles (Cartesian) per cell */
 void orc_expansions(const orc_ctx *c, double *M, double *L) {
-  size_t bytes = sizeof(cx) * (size_t)c->ncells * NTERM;
-  if (M) memcpy(M, c->M, bytes);
-  if (L) memcpy(L, c->L, bytes);
+  const size_t bytes = c->basis ? sizeof(double) * (size_t)c->ncells * CN
+                                : sizeof(cx) * (size_t)c->ncells * NTERM;
+  if (M) memcpy(M, c->basis ? (const void *)c->Mc : (const void *)c->M, bytes);
+  if (L) memcpy(L, c->basis ? (const void *)c->Lc : (const void *)c->L, bytes);
 }
 
 /* fields in the caller's original body order */

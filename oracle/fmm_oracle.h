@@ -39,6 +39,21 @@ void orc_m2l(double *Li, const double *Mj, const double *dX);
 void orc_l2l(double *Lc, const double *Lp, const double *dX);
 void orc_l2p(double *trg, const double *bx, double q, const double *L, const double *cen);
 
+/* Cartesian Taylor basis (orc_cart_terms() = P (P+1) (P+2) / 6 doubles per
+ * expansion; term (x, y, z) at orc_cart_index(x, y, z)) */
+int orc_cart_terms(void);
+int orc_cart_index(int x, int y, int z);
+void orc_cart_derivs(double *C, const double *dX);
+void orc_cart_powers(double *C, const double *dX, double c0);
+void orc_cart_p2m(double *M, const double *cen, const double *x, const double *y,
+                  const double *z, const double *q, int64_t nb);
+void orc_cart_m2m(double *Mp, const double *Mc, const double *dX);
+void orc_cart_m2l(double *Li, const double *Mj, const double *dX);
+void orc_cart_m2l_mutual(double *Li, double *Lj, const double *Mi, const double *Mj,
+                         const double *dX);
+void orc_cart_l2l(double *Lc, const double *Lp, const double *dX);
+void orc_cart_l2p(double *trg, const double *bx, double q, const double *L, const double *cen);
+
 /* full evaluation; phases: 0 tree, 1 +upward, 2 +traversal, 3 +downward */
 orc_ctx *orc_run(const double *x, const double *y, const double *z, const double *q, int64_t n,
                  int ncrit, double theta, int phases, int record_lists);
@@ -46,6 +61,10 @@ orc_ctx *orc_run(const double *x, const double *y, const double *z, const double
 orc_ctx *orc_run_mutual(const double *x, const double *y, const double *z, const double *q,
                         int64_t n, int ncrit, double theta, int phases, int record_lists,
                         int mutual);
+/* either basis: 0 spherical, 1 Cartesian */
+orc_ctx *orc_run_basis(const double *x, const double *y, const double *z, const double *q,
+                       int64_t n, int ncrit, double theta, int phases, int record_lists,
+                       int mutual, int basis);
 void orc_free(orc_ctx *c);
 void orc_region(const orc_ctx *c, double *rmin, double *rmax);
 int64_t orc_ncells(const orc_ctx *c);
@@ -66,6 +85,9 @@ void orc_direct(const double *x, const double *y, const double *z, const double 
 orc_ctx *orc_run_periodic(const double *x, const double *y, const double *z, const double *q,
                           int64_t n, int ncrit, double theta, int images, double cycle,
                           int phases, int record_lists);
+orc_ctx *orc_run_periodic_basis(const double *x, const double *y, const double *z,
+                                const double *q, int64_t n, int ncrit, double theta, int images,
+                                double cycle, int phases, int record_lists, int basis);
 void orc_direct_periodic(const double *x, const double *y, const double *z, const double *q,
                          int64_t n, const int64_t *targets, int64_t nt, int images, double cycle,
                          double *trg4);

@@ -45,6 +45,16 @@ def lib():
         L.orc_run_periodic.argtypes = [_d, _d, _d, _d, ctypes.c_int64, ctypes.c_int,
                                        ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_int, ctypes.c_int]
+        L.orc_run_basis.restype = ctypes.c_void_p
+        L.orc_run_basis.argtypes = [_d, _d, _d, _d, ctypes.c_int64, ctypes.c_int,
+                                    ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int]
+        L.orc_run_periodic_basis.restype = ctypes.c_void_p
+        L.orc_run_periodic_basis.argtypes = [_d, _d, _d, _d, ctypes.c_int64, ctypes.c_int,
+                                             ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_cart_terms.restype = ctypes.c_int
+        L.orc_cart_index.restype = ctypes.c_int
         for name in ("orc_free", "orc_region", "orc_ncells", "orc_max_depth", "orc_sorted",
                      "orc_cells", "orc_expansions", "orc_fields", "orc_counts", "orc_lists",
                      "orc_list_sizes"):
@@ -135,26 +145,102 @@ def l2p(trg, bx, q, L, center):
     return t
 
 
+# ---------------------------------------------------------------------------
+# Cartesian Taylor basis (CN real terms per expansion)
+# ---------------------------------------------------------------------------
+SPHERICAL, CARTESIAN = 0, 1
+CN = P * (P + 1) * (P + 2) // 6
+
+
+def cart_terms():
+    """multi-index (x, y, z) of every Cartesian term, in index order"""
+    out = []
+    for d in range(P):
+        for x in range(d, -1, -1):
+            for y in range(d - x, -1, -1):
+                out.append((x, y, d - x - y))
+    return out
+
+
+def cart_index(x, y, z):
+    return int(lib().orc_cart_index(int(x), int(y), int(z)))
+
+
+def cart_derivs(dX):
+    C = np.zeros(CN)
+    lib().orc_cart_derivs(_p(C), _p(np.asarray(dX, float)))
+    return C
+
+
+def cart_powers(dX, c0=1.0):
+    C = np.zeros(CN)
+    lib().orc_cart_powers(_p(C), _p(np.asarray(dX, float)), ctypes.c_double(c0))
+    return C
+
+
+def cart_p2m(center, x, y, z, q):
+    M = np.zeros(CN)
+    x, y, z, q = [np.ascontiguousarray(a, np.float64) for a in (x, y, z, q)]
+    lib().orc_cart_p2m(_p(M), _p(np.asarray(center, float)), _p(x), _p(y), _p(z), _p(q),
+                       ctypes.c_int64(len(x)))
+    return M
+
+
+def cart_m2m(Mc, dX, Mp=None):
+    Mp = np.zeros(CN) if Mp is None else np.array(Mp, float)
+    lib().orc_cart_m2m(_p(Mp), _p(np.ascontiguousarray(Mc, float)), _p(np.asarray(dX, float)))
+    return Mp
+
+
+def cart_m2l(Mj, dX, Li=None):
+    Li = np.zeros(CN) if Li is None else np.array(Li, float)
+    lib().orc_cart_m2l(_p(Li), _p(np.ascontiguousarray(Mj, float)), _p(np.asarray(dX, float)))
+    return Li
+
+
+def cart_m2l_mutual(Mi, Mj, dX, Li=None, Lj=None):
+    Li = np.zeros(CN) if Li is None else np.array(Li, float)
+    Lj = np.zeros(CN) if Lj is None else np.array(Lj, float)
+    lib().orc_cart_m2l_mutual(_p(Li), _p(Lj), _p(np.ascontiguousarray(Mi, float)),
+                              _p(np.ascontiguousarray(Mj, float)), _p(np.asarray(dX, float)))
+    return Li, Lj
+
+
+def cart_l2l(Lp, dX, Lc=None):
+    Lc = np.zeros(CN) if Lc is None else np.array(Lc, float)
+    lib().orc_cart_l2l(_p(Lc), _p(np.ascontiguousarray(Lp, float)), _p(np.asarray(dX, float)))
+    return Lc
+
+
+def cart_l2p(trg, bx, q, L, center):
+    t = np.array(trg, float).copy()
+    lib().orc_cart_l2p(_p(t), _p(np.asarray(bx, float)), ctypes.c_double(q),
+                       _p(np.ascontiguousarray(L, float)), _p(np.asarray(center, float)))
+    return t
+
+
 class Run:
     """One oracle evaluation.  phases: 0 tree, 1 +upward, 2 +traversal,
     3 +downward (full FMM); mutual as the reference's --mutual; images > 0:
-    periodic images of period cycle (mutual 0)."""
+    periodic images of period cycle (mutual 0); basis SPHERICAL or CARTESIAN."""
 
     def __init__(self, x, y, z, q, ncrit=64, theta=0.4, phases=3, record_lists=False, mutual=0,
-                 images=0, cycle=0.0):
+                 images=0, cycle=0.0, basis=SPHERICAL):
         L = lib()
         self.x, self.y, self.z, self.q = [np.ascontiguousarray(a, np.float64) for a in (x, y, z, q)]
         self.n = len(self.x)
+        self.basis = int(basis)
         if images > 0:
-            h = L.orc_run_periodic(self.x.ctypes.data_as(_d), self.y.ctypes.data_as(_d),
-                                   self.z.ctypes.data_as(_d), self.q.ctypes.data_as(_d), self.n,
-                                   int(ncrit), float(theta), int(images), float(cycle),
-                                   int(phases), int(bool(record_lists)))
+            h = L.orc_run_periodic_basis(self.x.ctypes.data_as(_d), self.y.ctypes.data_as(_d),
+                                         self.z.ctypes.data_as(_d), self.q.ctypes.data_as(_d),
+                                         self.n, int(ncrit), float(theta), int(images),
+                                         float(cycle), int(phases), int(bool(record_lists)),
+                                         self.basis)
         else:
-            h = L.orc_run_mutual(self.x.ctypes.data_as(_d), self.y.ctypes.data_as(_d),
-                                 self.z.ctypes.data_as(_d), self.q.ctypes.data_as(_d), self.n,
-                                 int(ncrit), float(theta), int(phases), int(bool(record_lists)),
-                                 int(bool(mutual)))
+            h = L.orc_run_basis(self.x.ctypes.data_as(_d), self.y.ctypes.data_as(_d),
+                                self.z.ctypes.data_as(_d), self.q.ctypes.data_as(_d), self.n,
+                                int(ncrit), float(theta), int(phases), int(bool(record_lists)),
+                                int(bool(mutual)), self.basis)
         if not h:
             raise ValueError("oracle rejected the input")
         self._h = ctypes.c_void_p(h)
@@ -195,9 +281,14 @@ class Run:
         return out
 
     def expansions(self):
+        """(M, L): NTERM complex per cell (spherical) or CN real (Cartesian)"""
         nc = self.ncells
-        M = np.zeros((nc, NTERM), np.complex128)
-        Lx = np.zeros((nc, NTERM), np.complex128)
+        if self.basis == CARTESIAN:
+            M = np.zeros((nc, CN))
+            Lx = np.zeros((nc, CN))
+        else:
+            M = np.zeros((nc, NTERM), np.complex128)
+            Lx = np.zeros((nc, NTERM), np.complex128)
         lib().orc_expansions(self._h, _p(M), _p(Lx))
         return M, Lx
 
@@ -347,3 +438,32 @@ def run_ref(x, y, z, q, ncrit=64, theta=0.4, count=True, dump_expansions=False, 
                        check=True,
                        stdout=subprocess.PIPE, stderr=subprocess.PIPE, env=env, timeout=timeout)
         return read_ref_output(fout)
+
+
+def run_ref_cartesian(seed=12345):
+    """Run the reference's Cartesian-basis kernels on the fixed cells and
+    bodies of oracle/ref_cartesian.cxx; returns inputs and outputs as arrays."""
+    exe = os.path.join(REF_DIR, "ref_cartesian")
+    if not os.path.exists(exe):
+        raise FileNotFoundError("oracle/_ref/ref_cartesian not built")
+    with tempfile.TemporaryDirectory() as td:
+        fout = os.path.join(td, "out.bin")
+        subprocess.run([exe, fout, str(int(seed))], check=True, stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE)
+        a = np.fromfile(fout, np.float64)
+    nc, nb, nt = int(a[0]), int(a[1]), int(a[2])
+    p = 3
+    w = 3 + 2 * nt + 5
+    cells = a[p:p + nc * w].reshape(nc, w)
+    p += nc * w
+    bod = a[p:p + nb * 12].reshape(nb, 12)
+    p += nb * 12
+    xp = a[p:p + 6].reshape(2, 3)
+    return dict(center=cells[:, :3].copy(), M=cells[:, 3:3 + nt].copy(),
+                L=cells[:, 3 + nt:3 + 2 * nt].copy(),
+                cell_ibody=cells[:, -5].astype(np.int64), cell_nbody=cells[:, -4].astype(np.int64),
+                cell_ichild=cells[:, -3].astype(np.int64),
+                cell_nchild=cells[:, -2].astype(np.int64),
+                cell_iparent=cells[:, -1].astype(np.int64),
+                x=bod[:, :3].copy(), q=bod[:, 3].copy(), trg_in=bod[:, 4:8].copy(),
+                trg_out=bod[:, 8:12].copy(), xperiodic=xp.copy())

@@ -11,6 +11,12 @@ keys and body order, cells (key, nb, first body, leaf, centre, width), the
 -DCOUNT counters numM2L/numP2P, fields per body (caller order) and, for the
 small cases, M and L of every cell.  For the 1M-body case only a digest of
 keys and cells and the fields of 4096 sampled bodies are kept.
+
+cartesian_ops.npz holds the reference's Cartesian-basis kernels
+(EX/kernels/LaplaceCartesianCPU.cxx, built as oracle/_ref/ref_cartesian) on
+the fixed cells and bodies of oracle/ref_cartesian.cxx: inputs, multipoles
+after P2M / M2M, locals after M2L (mutual and in a periodic image) and L2L,
+and body fields after L2P.
 """
 import hashlib
 import os
@@ -76,7 +82,18 @@ def cases():
     return out
 
 
+def cartesian_ops():
+    r = oracle.run_ref_cartesian(12345)
+    path = os.path.join(HERE, "cartesian_ops.npz")
+    np.savez_compressed(path, **r)
+    print("%-24s cells=%d bodies=%d -> %s (%d KB)" % (
+        "cartesian_ops", len(r["M"]), len(r["q"]), os.path.basename(path),
+        os.path.getsize(path) // 1024), flush=True)
+
+
 def main(only=None):
+    if not only or "cartesian_ops" in only:
+        cartesian_ops()
     for case in cases():
         name, (x, y, z, q), ncrit, theta, full = case[:5]
         mutual = case[5] if len(case) > 5 else 0
