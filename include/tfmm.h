@@ -170,6 +170,27 @@ int tfmm_num_problems(const tfmm_ctx *ctx, int64_t *offsets);
 #define TFMM_PERIODIC_MAX_IMAGES 6
 int tfmm_set_periodic(tfmm_ctx *ctx, int images, double cycle);
 
+/* Expansion basis (the reference's compile-time BASIS, EX/Makefile.include:13
+ * and EX/include/types.h:39-46, as a run-time choice).
+ * TFMM_BASIS_SPHERICAL (default): spherical harmonics, the Tapas kernels
+ * (EX/examples/LaplaceSphericalCPU_tapas.h), 55 complex terms per expansion.
+ * TFMM_BASIS_CARTESIAN: Cartesian Taylor expansions of ExaFMM's
+ * LaplaceCartesianCPU.cxx, TFMM_CART_TERMS real terms per expansion (terms
+ * of degree 0 .. 9; term (x, y, z) of degree d = x + y + z at
+ * d (d+1) (d+2) / 6 + (y + z) (y + z + 1) / 2 + z), every M2L pair by one
+ * FP64 kernel (no offset groups: the stats' group, item and fallback counts
+ * are 0).  Both bases truncate the same series at total degree 9, so their
+ * fields agree to rounding.  The P2P follows the precision parameter; the
+ * Cartesian expansions are FP64 in either precision.  Mutual interaction,
+ * batches and periodic images work in either basis; TFMM_TUNE_GRAPH 2
+ * evaluates the ordinary way.  A change keeps the tree: the next tfmm_upward
+ * (or tfmm_evaluate) forms the expansions in the new basis. */
+#define TFMM_BASIS_SPHERICAL 0
+#define TFMM_BASIS_CARTESIAN 1
+#define TFMM_CART_TERMS 220
+int tfmm_set_basis(tfmm_ctx *ctx, int basis);
+int tfmm_get_basis(tfmm_ctx *ctx, int *basis);
+
 int tfmm_build_tree(tfmm_ctx *ctx, const tfmm_params *params);
 /* Change theta or the precision of the following phases without rebuilding
  * the tree (the reference passes theta to Map(FMM_DTT, ...), not to
@@ -210,7 +231,8 @@ int tfmm_get_sorted(tfmm_ctx *ctx, uint64_t *keys, int32_t *perm);
  * pointer may be null. */
 int tfmm_get_cells(tfmm_ctx *ctx, uint64_t *key, int32_t *beg, int32_t *nb, int32_t *leaf,
                    double *center, double *width);
-/* M and L of every cell: ncells * 55 complex (re, im) doubles each. */
+/* M and L of every cell: ncells * 55 complex (re, im) doubles each, or with
+ * the Cartesian basis ncells * TFMM_CART_TERMS doubles each. */
 int tfmm_get_expansions(tfmm_ctx *ctx, double *M, double *L);
 /* Interaction lists in CSR form over target cells: off has ncells + 1
  * entries; src the source cell of each entry (sorted by target, then source).

@@ -145,6 +145,11 @@ struct tfmm_ctx {
   int images = 0;
   double cycle = 0;
   DevBuf pflag;  // bodies spanning more than one period
+  // expansion basis (tfmm_set_basis): TFMM_BASIS_SPHERICAL or _CARTESIAN,
+  // and the Cartesian M2L's schedule tables (built once per context)
+  int basis = TFMM_BASIS_SPHERICAL;
+  DevBuf cart_tab;
+  bool cart_ready = false;
   int32_t unit_chunk = 0;
   uint32_t units_bound = 0;
   bool codes_ready = false;
@@ -253,7 +258,7 @@ std::vector<DevBuf *> all_buffers(tfmm_ctx *c) {
           &c->pkey, &c->pkey_alt, &c->ppos, &c->ppos_alt, &c->brec, &c->lev,
           &c->lev_bsum, &c->lev_status, &c->dtt_geo, &c->dtt_topo, &c->u_cnt,
           &c->u_off, &c->units, &c->u_part, &c->p2p_next, &c->finite_flag, &c->gsz, &c->gstep,
-          &c->pflag};
+          &c->pflag, &c->cart_tab};
 }
 
 hipError_t reserve_cells(tfmm_ctx *c, size_t need, size_t used, hipStream_t s) {
@@ -277,6 +282,24 @@ constexpr int H_U32_WORDS = 256;
 
 // the period the kernels displace images by (0 without periodic images)
 double period(const tfmm_ctx *c) { return c->images > 0 ? c->cycle : 0.0; }
+
+bool cartesian(const tfmm_ctx *c) { return c->basis == TFMM_BASIS_CARTESIAN; }
+
+// bytes of one expansion array (M or L) of every cell
+size_t expansion_bytes(const tfmm_ctx *c, int64_t cells) {
+  return (size_t)cells * (cartesian(c) ? (size_t)CART_TERMS * 8 : (size_t)NTERM * 16);
+}
+
+// the Cartesian M2L's tables, on the device once per context
+int cart_tables(tfmm_ctx *c) {
+  if (c->cart_ready) return TFMM_OK;
+  TRY(dev_reserve(c->cart_tab, cart_tables_bytes()));
+  hipError_t e = cart_upload_tables(c->cart_tab.ptr);
+  if (e == hipErrorInvalidValue) return fail(c, TFMM_EHIP, "Cartesian M2L schedule is inconsistent");
+  TRY(e);
+  c->cart_ready = true;
+  return TFMM_OK;
+}
 
 // the direct sum's image range for the image sublevels (the reference's
 // prange, tapas_exafmm.cxx:314-317)
@@ -473,11 +496,65 @@ int do_build_tree(tfmm_ctx *c) {
   return TFMM_OK;
 }
 
+hipError_t rec(tfmm_ctx *c, int ev, hipStream_t st);
+int sort_list(tfmm_ctx *c, DevBuf &pairs, uint32_t npairs, DevBuf &off, DevBuf &src);
+
+// The Cartesian basis's upward pass: P2M, then M2M level by level.
+int cart_upward(tfmm_ctx *c) {
+  hipStream_t s = c->stream;
+  const Cells k = cells_of(c);
+  TRY(launch_fill_zero(c->M.ptr, expansion_bytes(c, c->ncells), s));
+  TRY(launch_cart_p2m(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->M), s));
+  const int nl = (int)c->lvl.size() - 1;
+  for (int d = nl - 2; d >= 0; d--)
+    TRY(launch_cart_m2m(k, c->lvl[d + 1], (c->lvl[d + 2] - c->lvl[d + 1]) / 8, dp<double>(c->M),
+                        s));
+  return TFMM_OK;
+}
+
+// The Cartesian basis's M2L: every pair of the traversal from one
+// per-target list (sorted by sort_list into m2l_off / m2l_src), then the
+// far field of periodic images.
+int cart_m2l(tfmm_ctx *c) {
+  hipStream_t s = c->stream;
+  const Cells k = cells_of(c);
+  TRY(launch_fill_zero(c->L.ptr, expansion_bytes(c, c->ncells), s));
+  TRY(rec(c, EV_M2L0, s));
+  Csr ml{dp<int32_t>(c->m2l_off), dp<int32_t>(c->m2l_src)};
+  TRY(launch_cart_m2l(k, ml, (int32_t)c->ncells, dp<double>(c->M), dp<double>(c->L),
+                      c->cart_tab.ptr, period(c), s));
+  TRY(rec(c, EV_M2LT, s));
+  TRY(launch_cart_periodic_far(dp<double>(c->M), dp<double>(c->L), c->cycle, c->images,
+                               c->cart_tab.ptr, s));
+  return TFMM_OK;
+}
+
+// The list of the Cartesian M2L (no offset groups, no MFMA batches).
+int cart_lists(tfmm_ctx *c) {
+  int rc = sort_list(c, c->m2l_pairs, c->n_m2l_exec, c->m2l_off, c->m2l_src);
+  if (rc) return rc;
+  c->n_groups = c->n_grouped = c->n_fallback = c->n_items = 0;
+  return cart_tables(c);
+}
+
+int cart_downward(tfmm_ctx *c) {
+  hipStream_t s = c->stream;
+  const Cells k = cells_of(c);
+  const int nl = (int)c->lvl.size() - 1;
+  for (int d = 0; d + 1 < nl; d++)
+    TRY(launch_cart_l2l(k, c->lvl[d + 1], c->lvl[d + 2] - c->lvl[d + 1], dp<double>(c->L), s));
+  TRY(launch_cart_l2p(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->L),
+                      dp<double>(c->acc), dp<int32_t>(c->perm), dp<double>(c->trg),
+                      dp<double>(c->out), s));
+  return TFMM_OK;
+}
+
 int do_upward(tfmm_ctx *c) {
   hipStream_t s = c->stream;
-  const size_t bytes = (size_t)c->ncells * NTERM * 16;
+  const size_t bytes = expansion_bytes(c, c->ncells);
   TRY(dev_reserve(c->M, bytes));
   TRY(dev_reserve(c->L, bytes));
+  if (cartesian(c)) return cart_upward(c);
   TRY(hipMemsetAsync(c->M.ptr, 0, bytes, s));
   const Cells k = cells_of(c);
   TRY(launch_p2m(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->M), s));
@@ -779,6 +856,20 @@ int run_lists(tfmm_ctx *c) {
     rc = launch_p2p();
     if (rc) return rc;
   }
+  if (cartesian(c)) {
+    rc = cart_lists(c);
+    if (!rc) {
+      TRY(hipEventRecord(c->ev[EV_DTT1], s));
+      if (!p2p_early) rc = launch_p2p();
+    }
+    if (!rc) rc = cart_m2l(c);
+    if (!rc) TRY(hipEventRecord(c->ev[EV_M2L1], s));
+    TRY(hipStreamWaitEvent(s, c->ev[EV_P2P1], 0));
+    if (rc) return rc;
+    TRY(hipMemcpyAsync(c->h_u64, static_cast<char *>(c->cnt.ptr) + 32, 16,
+                       hipMemcpyDeviceToHost, s));
+    return TFMM_OK;
+  }
   rc = build_m2l_batches(c);
   if (rc) {
     if (p2p_early) (void)hipStreamWaitEvent(s, c->ev[EV_P2P1], 0);
@@ -915,6 +1006,11 @@ int traverse_given(tfmm_ctx *c, const int32_t *m2l, int64_t nm, const int32_t *p
 int do_downward(tfmm_ctx *c) {
   hipStream_t s = c->stream;
   const Cells k = cells_of(c);
+  if (cartesian(c)) {
+    TRY(dev_reserve(c->trg, (size_t)c->n * 32));
+    TRY(dev_reserve(c->out, (size_t)c->n * 32));
+    return cart_downward(c);
+  }
   const int nl = (int)c->lvl.size() - 1;
   for (int d = 0; d + 1 < nl; d++)
     TRY(launch_l2l(k, c->lvl[d + 1], (c->lvl[d + 2] - c->lvl[d + 1]) / 8, dp<double>(c->tops),
@@ -1008,15 +1104,21 @@ int fused_prepare(tfmm_ctx *c) {
     if (rc) return rc;
     c->p2p_forked = true;
   }
-  rc = build_m2l_batches(c);
+  rc = cartesian(c) ? cart_lists(c) : build_m2l_batches(c);
   if (rc) {
     if (c->p2p_forked) (void)hipStreamWaitEvent(s, c->ev[EV_P2PJOIN], 0);
     return rc;
   }
   TRY(hipMemcpyAsync(c->h_u64, static_cast<char *>(c->cnt.ptr) + 32, 16, hipMemcpyDeviceToHost, s));
-  const size_t ebytes = (size_t)c->ncells * NTERM * 16;
+  const size_t ebytes = expansion_bytes(c, c->ncells);
   TRY(dev_reserve(c->M, ebytes));
   TRY(dev_reserve(c->L, ebytes));
+  if (cartesian(c)) {
+    TRY(dev_reserve(c->trg, (size_t)c->n * 32));
+    TRY(dev_reserve(c->out, (size_t)c->n * 32));
+    TRY(hipEventRecord(c->ev[EV_DTT1], s));
+    return TFMM_OK;
+  }
   if (!c->tops_ready) {
     TRY(dev_reserve(c->tops, translation_ops_doubles() * 8));
     TRY(launch_translation_ops(dp<double>(c->tops), s));
@@ -1041,6 +1143,18 @@ int fused_numeric(tfmm_ctx *c) {
   // phase A (2)
   int rc;
   if (!c->p2p_forked && c->p2p_schedule != 1 && (rc = fork_p2p(c))) return rc;
+  if (cartesian(c)) {
+    if ((rc = cart_upward(c))) return rc;
+    TRY(rec(c, EV_UP, s));
+    if (!c->p2p_forked && c->p2p_schedule == 1 && (rc = fork_p2p(c))) return rc;
+    if ((rc = cart_m2l(c))) return rc;
+    TRY(rec(c, EV_M2L1, s));
+    TRY(hipStreamWaitEvent(s, c->ev[EV_P2PJOIN], 0));
+    TRY(rec(c, EV_TRAV, s));
+    if ((rc = cart_downward(c))) return rc;
+    TRY(rec(c, EV_DOWN, s));
+    return TFMM_OK;
+  }
   // upward
   const size_t ebytes = (size_t)c->ncells * NTERM * 16;
   TRY(launch_fill_zero(c->M.ptr, ebytes, s));
@@ -1747,6 +1861,24 @@ int tfmm_set_periodic(tfmm_ctx *c, int images, double cycle) {
   return TFMM_OK;
 }
 
+int tfmm_set_basis(tfmm_ctx *c, int basis) {
+  if (!c) return TFMM_EINVAL;
+  if (basis != TFMM_BASIS_SPHERICAL && basis != TFMM_BASIS_CARTESIAN)
+    return fail(c, TFMM_EINVAL, "basis must be TFMM_BASIS_SPHERICAL or TFMM_BASIS_CARTESIAN");
+  if (basis != c->basis) {
+    c->basis = basis;
+    if (c->stage > 1) c->stage = 1;  // the tree stands; expansions are made again
+    plan_drop(c);
+  }
+  return TFMM_OK;
+}
+
+int tfmm_get_basis(tfmm_ctx *c, int *basis) {
+  if (!c || !basis) return TFMM_EINVAL;
+  *basis = c->basis;
+  return TFMM_OK;
+}
+
 int tfmm_set_tuning(tfmm_ctx *c, int knob, double value) {
   if (!c) return TFMM_EINVAL;
   switch (knob) {
@@ -2119,7 +2251,7 @@ int tfmm_downward(tfmm_ctx *c) {
 }
 
 int tfmm_evaluate(tfmm_ctx *c, const tfmm_params *p) {
-  const bool whole = c && c->use_graph == 2 && c->images == 0;
+  const bool whole = c && c->use_graph == 2 && c->images == 0 && !cartesian(c);
   if (whole) {
     int rc = check_params(c, p);
     if (rc) return rc;
@@ -2250,7 +2382,7 @@ int tfmm_get_cells(tfmm_ctx *c, uint64_t *key, int32_t *beg, int32_t *nb, int32_
 int tfmm_get_expansions(tfmm_ctx *c, double *M, double *L) {
   int rc = need_stage(c, 2, "tfmm_get_expansions");
   if (rc) return rc;
-  const size_t bytes = (size_t)c->ncells * NTERM * 16;
+  const size_t bytes = expansion_bytes(c, c->ncells);
   if (M) TRY(hipMemcpyAsync(M, c->M.ptr, bytes, hipMemcpyDeviceToHost, c->stream));
   if (L) {
     if (c->stage < 3) return fail(c, TFMM_ESTATE, "L is formed by tfmm_traverse");

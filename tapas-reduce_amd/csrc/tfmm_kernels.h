@@ -133,6 +133,27 @@ hipError_t launch_l2l(Cells c, int32_t first_child, int32_t npar, const double *
 hipError_t launch_l2p(Cells c, int32_t ncells, Bodies b, const double *L, const double *p2p,
                       const int32_t *perm, int32_t *leaf_of, double *trg, double *out,
                       hipStream_t s);
+
+// cartesian.hip: the Cartesian Taylor basis, CART_TERMS doubles per
+// expansion (FP64); the M2L and the periodic far field read tables built
+// by cart_upload_tables (cart_tables_bytes of device memory, synchronous)
+constexpr int CART_TERMS = 220;  // P (P+1) (P+2) / 6 for P = 10
+constexpr int CART_BINS = 6;
+size_t cart_tables_bytes();
+hipError_t cart_upload_tables(void *dst);
+hipError_t launch_cart_p2m(Cells c, int32_t ncells, Bodies b, double *M, hipStream_t s);
+// one level: npar groups of 8 siblings from first_child into their parents
+hipError_t launch_cart_m2m(Cells c, int32_t first_child, int32_t npar, double *M, hipStream_t s);
+// list: per-target CSR of source words (cell | image << IMG_SHIFT)
+hipError_t launch_cart_m2l(Cells c, Csr list, int32_t ncells, const double *M, double *L,
+                           const void *tables, double cycle, hipStream_t s);
+// one level: nchild cells from first_child, from their parents
+hipError_t launch_cart_l2l(Cells c, int32_t first_child, int32_t nchild, double *L,
+                           hipStream_t s);
+hipError_t launch_cart_l2p(Cells c, int32_t ncells, Bodies b, const double *L, const double *p2p,
+                           const int32_t *perm, double *trg, double *out, hipStream_t s);
+hipError_t launch_cart_periodic_far(const double *Mroot, double *Lroot, double cycle, int images,
+                                    const void *tables, hipStream_t s);
 // traverse.hip
 struct DttCounters {
   // [0] next-frontier size, [1] executed M2L pairs, [2] executed P2P leaf pairs,

@@ -27,6 +27,8 @@ DIRECT_NOALLOC_MAX = 65536
 PERIODIC_MAX_IMAGES = 6
 P = 10
 NTERM = 55
+BASIS_SPHERICAL, BASIS_CARTESIAN = 0, 1
+CART_TERMS = 220  # P (P+1) (P+2) / 6 real terms of a Cartesian expansion
 
 _lib = None
 
@@ -78,7 +80,8 @@ EXPORTED = [
     "tfmm_upward",
     "tfmm_traverse", "tfmm_downward", "tfmm_evaluate", "tfmm_get_fields", "tfmm_get_stats",
     "tfmm_get_sorted", "tfmm_get_cells", "tfmm_get_expansions", "tfmm_get_lists", "tfmm_direct",
-    "tfmm_direct_noalloc", "tfmm_traverse_lists", "tfmm_set_periodic",
+    "tfmm_direct_noalloc", "tfmm_traverse_lists", "tfmm_set_periodic", "tfmm_set_basis",
+    "tfmm_get_basis",
     "tfmm_picker_nparam", "tfmm_picker_create", "tfmm_picker_destroy", "tfmm_picker_last_error",
     "tfmm_picker_set_weights", "tfmm_picker_get_weights", "tfmm_picker_set_theta_max",
     "tfmm_picker_train",
@@ -114,6 +117,8 @@ def lib():
         L.tfmm_num_problems.argtypes = [vp, vp]
         L.tfmm_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.tfmm_set_periodic.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.tfmm_set_basis.argtypes = [vp, ctypes.c_int]
+        L.tfmm_get_basis.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.tfmm_get_tuning.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.tfmm_autotune.argtypes = [vp, ctypes.POINTER(Params), ctypes.c_int, vp]
         L.tfmm_build_tree.argtypes = [vp, ctypes.POINTER(Params)]
@@ -252,6 +257,15 @@ class Context:
         self._check(lib().tfmm_set_periodic(self._h, int(images), float(cycle)),
                     "tfmm_set_periodic")
 
+    def set_basis(self, basis):
+        """BASIS_SPHERICAL or BASIS_CARTESIAN; see tfmm_set_basis"""
+        self._check(lib().tfmm_set_basis(self._h, int(basis)), "tfmm_set_basis")
+
+    def basis(self):
+        v = ctypes.c_int()
+        self._check(lib().tfmm_get_basis(self._h, ctypes.byref(v)), "tfmm_get_basis")
+        return v.value
+
     def set_tuning(self, knob, value):
         self._check(lib().tfmm_set_tuning(self._h, int(knob), float(value)), "tfmm_set_tuning")
 
@@ -334,9 +348,15 @@ class Context:
         return out
 
     def expansions(self, want_l=True):
+        """(M, L) of every cell: NTERM complex (spherical basis) or
+        CART_TERMS real (Cartesian basis) per cell"""
         nc = self.stats()["ncells"]
-        M = np.zeros((nc, NTERM), np.complex128)
-        Lx = np.zeros((nc, NTERM), np.complex128) if want_l else None
+        if self.basis() == BASIS_CARTESIAN:
+            M = np.zeros((nc, CART_TERMS))
+            Lx = np.zeros((nc, CART_TERMS)) if want_l else None
+        else:
+            M = np.zeros((nc, NTERM), np.complex128)
+            Lx = np.zeros((nc, NTERM), np.complex128) if want_l else None
         self._check(lib().tfmm_get_expansions(self._h, _ptr(M), _ptr(Lx)),
                     "tfmm_get_expansions")
         return M, Lx
