@@ -20,7 +20,10 @@ The default run also measures, each in a block of its own under "configs":
       as one forest, mixed precision, theta 0.4;
   c4  8,388,608-body two-cluster Plummer (separation 10a, mass 1:3), mixed
       precision, learned parameters, beside the fallback ncrit 64 / theta 0.4.
---only-headline skips them; --batch K / --dist / --n measure one workload.
+  c1_cartesian  c1 in the Cartesian Taylor basis (ExaFMM's
+      LaplaceCartesianCPU.cxx; FP64 expansions, every M2L pair by k_cart_m2l).
+--only-headline skips them; --batch K / --dist / --n measure one workload
+(--basis cartesian: in the Cartesian basis).
 
 roofline: the P2P kernel (20 flops per evaluated body pair against the FP32
 vector peak; the FP64 kernel against the FP64 vector peak), timed by HIP
@@ -40,6 +43,9 @@ PEAK_FP64_TFLOPS = 78.6
 P2P_FLOPS_PER_PAIR = 20
 # complex multiply-adds of one M2L pair at P = 10 (sum_j (j+1)(P-j)^2 = 1210) x 8
 M2L_FLOPS_PER_PAIR = 9680
+# one Cartesian M2L pair at P = 10: the contraction's 5005 multiply-adds (x 2),
+# the derivative recurrence's 219 terms (10 flops each) and the k! scaling (220)
+CART_M2L_FLOPS_PER_PAIR = 2 * 5005 + 10 * 219 + 220
 HEADLINE_METRIC = "FMM bodies/s at rel-L2 err <=1e-4, 4M Plummer; P2P % FP32 peak"
 
 
@@ -79,6 +85,8 @@ def parse():
     ap.add_argument("--graph", action="store_true",
                     help="launch every evaluation as one hipGraph (after a first evaluation "
                          "that sets its capacities)")
+    ap.add_argument("--basis", default="spherical", choices=["spherical", "cartesian"],
+                    help="expansion basis of a single-workload run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256 MiB device buffer three times (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -206,10 +214,11 @@ def errors(ctx, targets, direct):
     return rel(f[:, 0], direct[:, 0]), rel(f[:, 1:], direct[:, 1:])
 
 
-def kernels(st, phases, precision, workload):
+def kernels(st, phases, precision, workload, basis="spherical"):
     """Achieved rates of the P2P kernel (20 flops per evaluated body pair)
-    and the M2L tile kernel (9,680 flops per grouped pair) over their HIP-event
-    times on the streams they run on."""
+    and the M2L tile kernel (9,680 flops per grouped pair; in the Cartesian
+    basis k_cart_m2l, CART_M2L_FLOPS_PER_PAIR per pair, FP64 vector) over
+    their HIP-event times on the streams they run on."""
     import tfmm
     mixed = precision == tfmm.MIXED
     pairs = st["n_p2p_calls"] - st["n_p2p_self"]
@@ -224,6 +233,26 @@ def kernels(st, phases, precision, workload):
     tile_peak = PEAK_FP32_TFLOPS if mixed else PEAK_FP64_TFLOPS
     p2p_traffic = measured_traffic(workload, p2p_name)
     tile_traffic = measured_traffic(workload, tile_name)
+    if basis == "cartesian":
+        pairs_m2l = st["n_m2l_exec"]
+        cart_tf = (CART_M2L_FLOPS_PER_PAIR * pairs_m2l / (tiles_ms * 1e-3) / 1e12
+                   if tiles_ms > 0 else 0.0)
+        tile_traffic = measured_traffic(workload, "k_cart_m2l")
+        m2l_entry = {"kernel": "k_cart_m2l", "bound": "valu", "achieved": cart_tf,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": cart_tf / PEAK_FP64_TFLOPS,
+                     "traffic": tile_traffic["bytes"] if tile_traffic else None,
+                     "traffic_source": tile_traffic["source"] if tile_traffic else None,
+                     "flops_per_pair": CART_M2L_FLOPS_PER_PAIR, "pairs_per_launch": pairs_m2l,
+                     "ms_per_launch": tiles_ms}
+    else:
+        m2l_entry = {"kernel": tile_name, "bound": "mfma", "achieved": m2l_tf,
+                     "peak": tile_peak, "unit": "TFLOP/s", "frac": m2l_tf / tile_peak,
+                     "traffic": tile_traffic["bytes"] if tile_traffic else None,
+                     "traffic_source": tile_traffic["source"] if tile_traffic else None,
+                     "traffic_knobs": tile_traffic["knobs"] if tile_traffic else None,
+                     "flops_per_pair": M2L_FLOPS_PER_PAIR, "pairs_per_launch": grouped,
+                     "ms_per_launch": tiles_ms}
     return {
         "kernel": p2p_name, "bound": "valu", "achieved": p2p_tf, "peak": p2p_peak,
         "unit": "TFLOP/s", "frac": p2p_tf / p2p_peak,
@@ -231,13 +260,7 @@ def kernels(st, phases, precision, workload):
         "traffic_source": p2p_traffic["source"] if p2p_traffic else None,
         "flops_per_pair": P2P_FLOPS_PER_PAIR, "body_pairs": pairs, "ms_per_launch": p2p_ms,
         "pct_fp32_peak": 100.0 * p2p_tf / PEAK_FP32_TFLOPS,
-        "m2l_tiles": {"kernel": tile_name, "bound": "mfma", "achieved": m2l_tf,
-                      "peak": tile_peak, "unit": "TFLOP/s", "frac": m2l_tf / tile_peak,
-                      "traffic": tile_traffic["bytes"] if tile_traffic else None,
-                      "traffic_source": tile_traffic["source"] if tile_traffic else None,
-                      "traffic_knobs": tile_traffic["knobs"] if tile_traffic else None,
-                      "flops_per_pair": M2L_FLOPS_PER_PAIR, "pairs_per_launch": grouped,
-                      "ms_per_launch": tiles_ms},
+        "m2l_tiles": m2l_entry,
     }
 
 
@@ -260,6 +283,8 @@ def to_device(arrays):
 def set_knobs(ctx, args):
     import tfmm
     fixed = {}
+    if getattr(args, "basis", "spherical") == "cartesian":
+        ctx.set_basis(tfmm.BASIS_CARTESIAN)
     if args.graph:
         ctx.set_tuning(tfmm.TUNE_GRAPH, 2)
     if args.unit_factor is not None:
@@ -323,7 +348,9 @@ def bench_single(args, dist, n, precision, use_picker, steps, warmup, fp64_steps
     try:
         fixed = set_knobs(ctx, args)
         ctx.set_bodies_device(n, *[a.data_ptr() for a in dx])
-        picked = learned(ctx, args.picker, precision) if use_picker else None
+        cart = args.basis == "cartesian"
+        # the picker was trained on the spherical basis's timings
+        picked = learned(ctx, args.picker, precision) if use_picker and not cart else None
         ncrit = picked["ncrit"] if picked else (args.ncrit or 64)
         theta = picked["theta"] if picked else args.theta
         prm = tfmm.make_params(ncrit, theta, precision)
@@ -334,16 +361,18 @@ def bench_single(args, dist, n, precision, use_picker, steps, warmup, fp64_steps
         ms, phases, st = time_config(ctx, prm, steps, warmup)
         ep, ea = errors(ctx, targets, direct)
         prec_name = "fp64" if precision == tfmm.FP64 else "mixed"
-        workload = "%s N=%d P=10 ncrit=%d theta=%g %s%s" % (
-            dist, n, ncrit, round(theta, 4), prec_name, " (learned)" if picked else "")
+        workload = "%s N=%d P=10 ncrit=%d theta=%g %s%s%s" % (
+            dist, n, ncrit, round(theta, 4), prec_name, " (learned)" if picked else "",
+            " cartesian basis" if cart else "")
         out = {"workload": workload, "value": n / (ms * 1e-3), "unit": "bodies/s",
                "ms_per_step": ms, "steps": steps, "n_bodies": n, "distribution": dist,
                "ncrit": ncrit, "theta": theta, "precision": prec_name,
                "error": {"rel_l2_pot": ep, "rel_l2_acc": ea, "targets": len(targets),
                          "against": "FP64 direct sum"},
                "picker": picked, "autotune": tuned, "phases_ms": phases, "counts": counts(st),
-               "roofline": kernels(st, phases, precision, workload)}
-        if not args.graph:
+               "roofline": kernels(st, phases, precision, workload, args.basis),
+               "basis": args.basis}
+        if not args.graph and not cart:
             out["one_graph"] = one_graph(ctx, prm, steps, n)
         if fallback_steps > 0 and picked:
             prm_fb = tfmm.make_params(64, 0.4, precision)
@@ -501,8 +530,14 @@ def main():
         ks = args.config_steps
         c1_args = argparse.Namespace(**vars(args))
         c1_args.ncrit, c1_args.theta = 64, 0.4
+        c1_args.basis = "spherical"
+        c1_cart = argparse.Namespace(**vars(c1_args))
+        c1_cart.basis = "cartesian"
+        c1_cart.no_autotune = True  # the knobs are the spherical tile kernel's
         configs = {
             "c1": bench_single(c1_args, "cube", 1 << 20, tfmm.FP64, False, ks, 1, 0, seed=0),
+            "c1_cartesian": bench_single(c1_cart, "cube", 1 << 20, tfmm.FP64, False, ks, 1, 0,
+                                         seed=0),
             "c3": bench_batch(args, 256, tfmm.MIXED, ks, 1)[0],
             "c4": bench_single(args, "two_plummer", 1 << 23, tfmm.MIXED, not args.no_learned,
                                ks, 1, 2, fallback_steps=3),
