@@ -177,14 +177,16 @@ int tfmm_set_periodic(tfmm_ctx *ctx, int images, double cycle);
  * TFMM_BASIS_CARTESIAN: Cartesian Taylor expansions of ExaFMM's
  * LaplaceCartesianCPU.cxx, TFMM_CART_TERMS real terms per expansion (terms
  * of degree 0 .. 9; term (x, y, z) of degree d = x + y + z at
- * d (d+1) (d+2) / 6 + (y + z) (y + z + 1) / 2 + z), every M2L pair by one
- * FP64 kernel (no offset groups: the stats' group, item and fallback counts
- * are 0).  Both bases truncate the same series at total degree 9, so their
- * fields agree to rounding.  The P2P follows the precision parameter; the
- * Cartesian expansions are FP64 in either precision.  Mutual interaction,
- * batches and periodic images work in either basis; TFMM_TUNE_GRAPH 2
- * evaluates the ordinary way.  A change keeps the tree: the next tfmm_upward
- * (or tfmm_evaluate) forms the expansions in the new basis. */
+ * d (d+1) (d+2) / 6 + (y + z) (y + z + 1) / 2 + z).  Both bases truncate
+ * the same series at total degree 9, so their fields agree to rounding.
+ * TFMM_TUNE_CART_M2L picks the route of the Cartesian M2L (by default the
+ * FP64 tile kernel between exact basis changes; the stats' group, item and
+ * fallback counts are then the tile kernel's, and 0 for the direct kernel).
+ * The P2P follows the precision parameter; the Cartesian expansions and
+ * their M2L are FP64 in either precision.  Mutual interaction, batches and
+ * periodic images work in either basis; TFMM_TUNE_GRAPH 2 evaluates the
+ * ordinary way.  A change keeps the tree: the next tfmm_upward (or
+ * tfmm_evaluate) forms the expansions in the new basis. */
 #define TFMM_BASIS_SPHERICAL 0
 #define TFMM_BASIS_CARTESIAN 1
 #define TFMM_CART_TERMS 220
@@ -290,6 +292,17 @@ int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p
  * raised.  For deployments that share the GPU, and to test the allocation
  * failure path. */
 #define TFMM_TUNE_DEVICE_LIMIT 6
+/* TFMM_TUNE_CART_M2L: the M2L of the Cartesian basis (tfmm_set_basis).
+ * 0 (default): through the spherical tile kernel, exactly -- the Cartesian
+ * multipoles turned into spherical ones per degree (T1), the FP64 tile
+ * kernel (and its per-pair FP64 fallback, and the far field of periodic
+ * images), the spherical locals turned back (T2); both bases truncate the
+ * same series and a contraction with derivatives of 1/r sees only the
+ * traceless part of a moment tensor, so this is the Cartesian M2L up to
+ * rounding.  1: the direct Cartesian kernel, the reference's formula
+ * (5005 multiply-adds per pair) on every pair of the list.  A change makes
+ * the expansions again (as tfmm_set_basis). */
+#define TFMM_TUNE_CART_M2L 7
 int tfmm_set_tuning(tfmm_ctx *ctx, int knob, double value);
 int tfmm_get_tuning(tfmm_ctx *ctx, int knob, double *value);
 /* Autotuner: evaluates the current bodies with params under every candidate

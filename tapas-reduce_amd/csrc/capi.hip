@@ -150,6 +150,10 @@ struct tfmm_ctx {
   int basis = TFMM_BASIS_SPHERICAL;
   DevBuf cart_tab;
   bool cart_ready = false;
+  // Cartesian expansions (CART_TERMS doubles per cell); M and L then hold
+  // the spherical images the tile kernel works on (TFMM_TUNE_CART_M2L 0)
+  DevBuf Mc, Lc;
+  int cart_direct = 0;  // TFMM_TUNE_CART_M2L: 1 = the direct Cartesian M2L kernel
   int32_t unit_chunk = 0;
   uint32_t units_bound = 0;
   bool codes_ready = false;
@@ -258,7 +262,7 @@ std::vector<DevBuf *> all_buffers(tfmm_ctx *c) {
           &c->pkey, &c->pkey_alt, &c->ppos, &c->ppos_alt, &c->brec, &c->lev,
           &c->lev_bsum, &c->lev_status, &c->dtt_geo, &c->dtt_topo, &c->u_cnt,
           &c->u_off, &c->units, &c->u_part, &c->p2p_next, &c->finite_flag, &c->gsz, &c->gstep,
-          &c->pflag, &c->cart_tab};
+          &c->pflag, &c->cart_tab, &c->Mc, &c->Lc};
 }
 
 hipError_t reserve_cells(tfmm_ctx *c, size_t need, size_t used, hipStream_t s) {
@@ -284,10 +288,27 @@ constexpr int H_U32_WORDS = 256;
 double period(const tfmm_ctx *c) { return c->images > 0 ? c->cycle : 0.0; }
 
 bool cartesian(const tfmm_ctx *c) { return c->basis == TFMM_BASIS_CARTESIAN; }
+// the Cartesian basis with its own M2L kernel (no spherical images)
+bool cart_direct(const tfmm_ctx *c) { return cartesian(c) && c->cart_direct; }
+// the tile kernel in FP32 (mixed precision, spherical basis); the Cartesian
+// basis keeps its expansions and their M2L in FP64
+bool m2l_fp32(const tfmm_ctx *c) { return c->prm.precision == TFMM_MIXED && !cartesian(c); }
 
-// bytes of one expansion array (M or L) of every cell
-size_t expansion_bytes(const tfmm_ctx *c, int64_t cells) {
-  return (size_t)cells * (cartesian(c) ? (size_t)CART_TERMS * 8 : (size_t)NTERM * 16);
+// bytes of a spherical expansion array (M or L) of every cell
+size_t sph_bytes(int64_t cells) { return (size_t)cells * NTERM * 16; }
+// bytes of a Cartesian expansion array (Mc or Lc) of every cell
+size_t cart_bytes(int64_t cells) { return (size_t)cells * CART_TERMS * 8; }
+
+// the expansion arrays of the basis and route
+hipError_t reserve_expansions(tfmm_ctx *c) {
+  hipError_t e;
+  if (cartesian(c)) {
+    if ((e = dev_reserve(c->Mc, cart_bytes(c->ncells))) != hipSuccess) return e;
+    if ((e = dev_reserve(c->Lc, cart_bytes(c->ncells))) != hipSuccess) return e;
+    if (c->cart_direct) return hipSuccess;
+  }
+  if ((e = dev_reserve(c->M, sph_bytes(c->ncells))) != hipSuccess) return e;
+  return dev_reserve(c->L, sph_bytes(c->ncells));
 }
 
 // the Cartesian M2L's tables, on the device once per context
@@ -499,37 +520,41 @@ int do_build_tree(tfmm_ctx *c) {
 hipError_t rec(tfmm_ctx *c, int ev, hipStream_t st);
 int sort_list(tfmm_ctx *c, DevBuf &pairs, uint32_t npairs, DevBuf &off, DevBuf &src);
 
-// The Cartesian basis's upward pass: P2M, then M2M level by level.
+// The Cartesian basis's upward pass: P2M, then M2M level by level (Mc);
+// for the tile kernel's route also the spherical images of the multipoles (M).
 int cart_upward(tfmm_ctx *c) {
   hipStream_t s = c->stream;
   const Cells k = cells_of(c);
-  TRY(launch_fill_zero(c->M.ptr, expansion_bytes(c, c->ncells), s));
-  TRY(launch_cart_p2m(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->M), s));
+  TRY(launch_fill_zero(c->Mc.ptr, cart_bytes(c->ncells), s));
+  TRY(launch_cart_p2m(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->Mc), s));
   const int nl = (int)c->lvl.size() - 1;
   for (int d = nl - 2; d >= 0; d--)
-    TRY(launch_cart_m2m(k, c->lvl[d + 1], (c->lvl[d + 2] - c->lvl[d + 1]) / 8, dp<double>(c->M),
+    TRY(launch_cart_m2m(k, c->lvl[d + 1], (c->lvl[d + 2] - c->lvl[d + 1]) / 8, dp<double>(c->Mc),
                         s));
+  if (!c->cart_direct)
+    TRY(launch_cart_to_sph((int32_t)c->ncells, dp<double>(c->Mc), dp<double>(c->M),
+                           c->cart_tab.ptr, s));
   return TFMM_OK;
 }
 
-// The Cartesian basis's M2L: every pair of the traversal from one
-// per-target list (sorted by sort_list into m2l_off / m2l_src), then the
-// far field of periodic images.
+// The direct Cartesian M2L: every pair of the traversal from one per-target
+// list (sorted by sort_list into m2l_off / m2l_src), then the far field of
+// periodic images.
 int cart_m2l(tfmm_ctx *c) {
   hipStream_t s = c->stream;
   const Cells k = cells_of(c);
-  TRY(launch_fill_zero(c->L.ptr, expansion_bytes(c, c->ncells), s));
+  TRY(launch_fill_zero(c->Lc.ptr, cart_bytes(c->ncells), s));
   TRY(rec(c, EV_M2L0, s));
   Csr ml{dp<int32_t>(c->m2l_off), dp<int32_t>(c->m2l_src)};
-  TRY(launch_cart_m2l(k, ml, (int32_t)c->ncells, dp<double>(c->M), dp<double>(c->L),
+  TRY(launch_cart_m2l(k, ml, (int32_t)c->ncells, dp<double>(c->Mc), dp<double>(c->Lc),
                       c->cart_tab.ptr, period(c), s));
   TRY(rec(c, EV_M2LT, s));
-  TRY(launch_cart_periodic_far(dp<double>(c->M), dp<double>(c->L), c->cycle, c->images,
+  TRY(launch_cart_periodic_far(dp<double>(c->Mc), dp<double>(c->Lc), c->cycle, c->images,
                                c->cart_tab.ptr, s));
   return TFMM_OK;
 }
 
-// The list of the Cartesian M2L (no offset groups, no MFMA batches).
+// The list of the direct Cartesian M2L (no offset groups, no MFMA batches).
 int cart_lists(tfmm_ctx *c) {
   int rc = sort_list(c, c->m2l_pairs, c->n_m2l_exec, c->m2l_off, c->m2l_src);
   if (rc) return rc;
@@ -537,13 +562,20 @@ int cart_lists(tfmm_ctx *c) {
   return cart_tables(c);
 }
 
+// The tile kernel's route: its spherical locals (L) into the Cartesian ones.
+hipError_t cart_from_tiles(tfmm_ctx *c) {
+  if (!cartesian(c) || c->cart_direct) return hipSuccess;
+  return launch_sph_to_cart((int32_t)c->ncells, dp<double>(c->L), dp<double>(c->Lc),
+                            c->cart_tab.ptr, c->stream);
+}
+
 int cart_downward(tfmm_ctx *c) {
   hipStream_t s = c->stream;
   const Cells k = cells_of(c);
   const int nl = (int)c->lvl.size() - 1;
   for (int d = 0; d + 1 < nl; d++)
-    TRY(launch_cart_l2l(k, c->lvl[d + 1], c->lvl[d + 2] - c->lvl[d + 1], dp<double>(c->L), s));
-  TRY(launch_cart_l2p(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->L),
+    TRY(launch_cart_l2l(k, c->lvl[d + 1], c->lvl[d + 2] - c->lvl[d + 1], dp<double>(c->Lc), s));
+  TRY(launch_cart_l2p(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->Lc),
                       dp<double>(c->acc), dp<int32_t>(c->perm), dp<double>(c->trg),
                       dp<double>(c->out), s));
   return TFMM_OK;
@@ -551,10 +583,13 @@ int cart_downward(tfmm_ctx *c) {
 
 int do_upward(tfmm_ctx *c) {
   hipStream_t s = c->stream;
-  const size_t bytes = expansion_bytes(c, c->ncells);
-  TRY(dev_reserve(c->M, bytes));
-  TRY(dev_reserve(c->L, bytes));
-  if (cartesian(c)) return cart_upward(c);
+  TRY(reserve_expansions(c));
+  if (cartesian(c)) {
+    int rc = cart_tables(c);
+    if (rc) return rc;
+    return cart_upward(c);
+  }
+  const size_t bytes = sph_bytes(c->ncells);
   TRY(hipMemsetAsync(c->M.ptr, 0, bytes, s));
   const Cells k = cells_of(c);
   TRY(launch_p2m(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->M), s));
@@ -805,7 +840,7 @@ int build_m2l_batches(tfmm_ctx *c) {
     c->codes_ready = true;
   }
   // every group's A operands in the tile kernel's order
-  const bool mixed = c->prm.precision == TFMM_MIXED;
+  const bool mixed = m2l_fp32(c);
   TRY(dev_reserve(c->g_af, m2l_afrag_bytes(ng, mixed)));
   TRY(launch_m2l_group_afrag(mixed ? (const void *)c->g_y32.ptr : (const void *)c->g_y64.ptr,
                              ng, dp<uint32_t>(c->codes), dp<int8_t>(c->g_dl), c->g_af.ptr,
@@ -856,7 +891,7 @@ int run_lists(tfmm_ctx *c) {
     rc = launch_p2p();
     if (rc) return rc;
   }
-  if (cartesian(c)) {
+  if (cart_direct(c)) {
     rc = cart_lists(c);
     if (!rc) {
       TRY(hipEventRecord(c->ev[EV_DTT1], s));
@@ -876,11 +911,12 @@ int run_lists(tfmm_ctx *c) {
     return rc;
   }
   // M~ in the MFMA operand layout
-  TRY(dev_reserve(c->mq, (size_t)c->ncells * 112 * (mixed ? 4 : 8)));
-  TRY(launch_m2l_scale_m(k, (int32_t)c->ncells, dp<double>(c->M), c->mq.ptr, mixed, s));
+  const bool tf32 = m2l_fp32(c);
+  TRY(dev_reserve(c->mq, (size_t)c->ncells * 112 * (tf32 ? 4 : 8)));
+  TRY(launch_m2l_scale_m(k, (int32_t)c->ncells, dp<double>(c->M), c->mq.ptr, tf32, s));
   TRY(hipEventRecord(c->ev[EV_DTT1], s));
 
-  const size_t lbytes = (size_t)c->ncells * NTERM * 16;
+  const size_t lbytes = sph_bytes(c->ncells);
   TRY(hipMemsetAsync(c->L.ptr, 0, lbytes, s));
   TRY(hipEventRecord(c->ev[EV_M2L0], s));
   if (!p2p_early) {
@@ -894,7 +930,7 @@ int run_lists(tfmm_ctx *c) {
                        c->units_bound,
                        dp<uint2>(c->cols),
                        c->g_af.ptr, c->mq.ptr, k, (int32_t)c->ncells,
-                       dp<double>(c->L), dp<double>(c->u_part), mixed, s));
+                       dp<double>(c->L), dp<double>(c->u_part), tf32, s));
   TRY(hipEventRecord(c->ev[EV_M2LT], s));
   if (c->n_fallback) {
     Csr fl{dp<int32_t>(c->fb_off), dp<int32_t>(c->fb_src)};
@@ -903,6 +939,7 @@ int run_lists(tfmm_ctx *c) {
                         dp<double>(c->L), c->fb_part.ptr, s, nullptr, period(c)));
   }
   TRY(launch_periodic_far(dp<double>(c->M), dp<double>(c->L), c->cycle, c->images, s));
+  TRY(cart_from_tiles(c));
   TRY(hipEventRecord(c->ev[EV_M2L1], s));
 
   TRY(hipStreamWaitEvent(s, c->ev[EV_P2P1], 0));
@@ -1104,16 +1141,15 @@ int fused_prepare(tfmm_ctx *c) {
     if (rc) return rc;
     c->p2p_forked = true;
   }
-  rc = cartesian(c) ? cart_lists(c) : build_m2l_batches(c);
+  rc = cart_direct(c) ? cart_lists(c) : build_m2l_batches(c);
+  if (!rc && cartesian(c)) rc = cart_tables(c);
   if (rc) {
     if (c->p2p_forked) (void)hipStreamWaitEvent(s, c->ev[EV_P2PJOIN], 0);
     return rc;
   }
   TRY(hipMemcpyAsync(c->h_u64, static_cast<char *>(c->cnt.ptr) + 32, 16, hipMemcpyDeviceToHost, s));
-  const size_t ebytes = expansion_bytes(c, c->ncells);
-  TRY(dev_reserve(c->M, ebytes));
-  TRY(dev_reserve(c->L, ebytes));
-  if (cartesian(c)) {
+  TRY(reserve_expansions(c));
+  if (cart_direct(c)) {
     TRY(dev_reserve(c->trg, (size_t)c->n * 32));
     TRY(dev_reserve(c->out, (size_t)c->n * 32));
     TRY(hipEventRecord(c->ev[EV_DTT1], s));
@@ -1124,7 +1160,7 @@ int fused_prepare(tfmm_ctx *c) {
     TRY(launch_translation_ops(dp<double>(c->tops), s));
     c->tops_ready = true;
   }
-  TRY(dev_reserve(c->mq, (size_t)c->ncells * 112 * (mixed ? 4 : 8)));
+  TRY(dev_reserve(c->mq, (size_t)c->ncells * 112 * (m2l_fp32(c) ? 4 : 8)));
   TRY(dev_reserve(c->trg, (size_t)c->n * 32));
   TRY(dev_reserve(c->out, (size_t)c->n * 32));
   TRY(dev_reserve(c->leaf_of, (size_t)c->n * 4));
@@ -1137,13 +1173,12 @@ int fused_prepare(tfmm_ctx *c) {
 int fused_numeric(tfmm_ctx *c) {
   hipStream_t s = c->stream;
   const Cells k = cells_of(c);
-  const bool mixed = c->prm.precision == TFMM_MIXED;
   // P2P on the second stream, joined before the downward pass; forked at the
   // start of the phase (schedule 0), at the tile kernel (1), or already in
   // phase A (2)
   int rc;
   if (!c->p2p_forked && c->p2p_schedule != 1 && (rc = fork_p2p(c))) return rc;
-  if (cartesian(c)) {
+  if (cart_direct(c)) {
     if ((rc = cart_upward(c))) return rc;
     TRY(rec(c, EV_UP, s));
     if (!c->p2p_forked && c->p2p_schedule == 1 && (rc = fork_p2p(c))) return rc;
@@ -1155,16 +1190,21 @@ int fused_numeric(tfmm_ctx *c) {
     TRY(rec(c, EV_DOWN, s));
     return TFMM_OK;
   }
-  // upward
-  const size_t ebytes = (size_t)c->ncells * NTERM * 16;
-  TRY(launch_fill_zero(c->M.ptr, ebytes, s));
-  TRY(launch_p2m(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->M), s));
+  // upward (the Cartesian basis: in its own arrays, then the spherical images)
+  const size_t ebytes = sph_bytes(c->ncells);
   const int nl = (int)c->lvl.size() - 1;
-  for (int d = nl - 2; d >= 0; d--)
-    TRY(launch_m2m(k, c->lvl[d + 1], (c->lvl[d + 2] - c->lvl[d + 1]) / 8, dp<double>(c->tops),
-                   dp<double>(c->M), s));
+  if (cartesian(c)) {
+    if ((rc = cart_upward(c))) return rc;
+  } else {
+    TRY(launch_fill_zero(c->M.ptr, ebytes, s));
+    TRY(launch_p2m(k, (int32_t)c->ncells, sorted_bodies(c), dp<double>(c->M), s));
+    for (int d = nl - 2; d >= 0; d--)
+      TRY(launch_m2m(k, c->lvl[d + 1], (c->lvl[d + 2] - c->lvl[d + 1]) / 8,
+                     dp<double>(c->tops), dp<double>(c->M), s));
+  }
   TRY(rec(c, EV_UP, s));
   // M2L
+  const bool mixed = m2l_fp32(c);
   TRY(launch_m2l_scale_m(k, (int32_t)c->ncells, dp<double>(c->M), c->mq.ptr, mixed, s));
   TRY(launch_fill_zero(c->L.ptr, ebytes, s));
   TRY(rec(c, EV_M2L0, s));
@@ -1183,10 +1223,16 @@ int fused_numeric(tfmm_ctx *c) {
                         dp<double>(c->L), c->fb_part.ptr, s, nullptr, period(c)));
   }
   TRY(launch_periodic_far(dp<double>(c->M), dp<double>(c->L), c->cycle, c->images, s));
+  TRY(cart_from_tiles(c));
   TRY(rec(c, EV_M2L1, s));
   TRY(hipStreamWaitEvent(s, c->ev[EV_P2PJOIN], 0));
   TRY(rec(c, EV_TRAV, s));
   // downward
+  if (cartesian(c)) {
+    if ((rc = cart_downward(c))) return rc;
+    TRY(rec(c, EV_DOWN, s));
+    return TFMM_OK;
+  }
   for (int d = 0; d + 1 < nl; d++)
     TRY(launch_l2l(k, c->lvl[d + 1], (c->lvl[d + 2] - c->lvl[d + 1]) / 8, dp<double>(c->tops),
                    dp<double>(c->L), s));
@@ -1911,6 +1957,13 @@ int tfmm_set_tuning(tfmm_ctx *c, int knob, double value) {
         return fail(c, TFMM_EINVAL, "device limit must be a whole number of bytes >= 0");
       dev_set_limit((size_t)value);
       return TFMM_OK;
+    case TFMM_TUNE_CART_M2L:
+      if (value != 0 && value != 1) return fail(c, TFMM_EINVAL, "Cartesian M2L route must be 0 or 1");
+      if ((int)value != c->cart_direct) {
+        c->cart_direct = (int)value;
+        if (cartesian(c) && c->stage > 1) c->stage = 1;  // the expansions are made again
+      }
+      return TFMM_OK;
     default:
       return fail(c, TFMM_EINVAL, "unknown tuning knob");
   }
@@ -1984,6 +2037,7 @@ int tfmm_get_tuning(tfmm_ctx *c, int knob, double *value) {
     case TFMM_TUNE_GRAPH: *value = c->use_graph; return TFMM_OK;
     case TFMM_TUNE_P2P_WAVES: *value = c->p2p_waves; return TFMM_OK;
     case TFMM_TUNE_DEVICE_LIMIT: *value = (double)dev_limit(); return TFMM_OK;
+    case TFMM_TUNE_CART_M2L: *value = c->cart_direct; return TFMM_OK;
     default: return fail(c, TFMM_EINVAL, "unknown tuning knob");
   }
 }
@@ -2382,11 +2436,12 @@ int tfmm_get_cells(tfmm_ctx *c, uint64_t *key, int32_t *beg, int32_t *nb, int32_
 int tfmm_get_expansions(tfmm_ctx *c, double *M, double *L) {
   int rc = need_stage(c, 2, "tfmm_get_expansions");
   if (rc) return rc;
-  const size_t bytes = expansion_bytes(c, c->ncells);
-  if (M) TRY(hipMemcpyAsync(M, c->M.ptr, bytes, hipMemcpyDeviceToHost, c->stream));
+  const bool cart = cartesian(c);
+  const size_t bytes = cart ? cart_bytes(c->ncells) : sph_bytes(c->ncells);
+  if (M) TRY(hipMemcpyAsync(M, (cart ? c->Mc : c->M).ptr, bytes, hipMemcpyDeviceToHost, c->stream));
   if (L) {
     if (c->stage < 3) return fail(c, TFMM_ESTATE, "L is formed by tfmm_traverse");
-    TRY(hipMemcpyAsync(L, c->L.ptr, bytes, hipMemcpyDeviceToHost, c->stream));
+    TRY(hipMemcpyAsync(L, (cart ? c->Lc : c->L).ptr, bytes, hipMemcpyDeviceToHost, c->stream));
   }
   TRY(hipStreamSynchronize(c->stream));
   return TFMM_OK;

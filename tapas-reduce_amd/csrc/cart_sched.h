@@ -179,5 +179,149 @@ inline bool build_tables(CartTables &tb) {
 }
 
 
+// ---------------------------------------------------------------------------
+// Basis change between the Cartesian expansions and the spherical ones of
+// the library (55 complex terms, (n, m) at n (n+1) / 2 + m, m >= 0).
+// Both bases truncate the same double series at total degree ORDER - 1, and
+// a contraction with derivatives of 1/r sees only the traceless part of a
+// moment tensor, so the Cartesian M2L is exactly
+//   L_cart = T2 (M2L_spherical (T1 M_cart))
+// with T1, T2 block diagonal by degree:
+//   M_sph[n,m] = sum_{|k|=n} conj(y_{nm,k}) (-1)^n k! M_cart[k]
+//     (P2M stores q conj(Y_n^m(x - X)); M_cart[k] = sum q (X - x)^k / k!)
+//   L_cart[k]  = k! sum_{m=0..n} w_m Re(L_sph[n,m] y_{nm,k}),  w_0 = 1, w_m = 2
+//     (L2P sums w_m Re(L Y_n^m(e)); the Cartesian local is sum L e^k / k!)
+// where y_{nm,k} are the coefficients of the regular solid harmonic
+//   Y_n^m(e) = (-1)^n / (n+m)! rho^n P_n^m(cos alpha) e^{i m beta}
+// (the normalisation of the library's harmonics recurrence,
+// LaplaceSphericalCPU_tapas.h:43-100) as a polynomial in e:
+//   R_m^m = (-1)^m (2m-1)!! (x + i y)^m,  R_{m+1}^m = (2m+1) z R_m^m,
+//   R_n^m = ((2n-1) z R_{n-1}^m - (n+m-1) rho^2 R_{n-2}^m) / (n-m).
+constexpr int SPH_TERMS = ORDER * (ORDER + 1) / 2;  // 55
+constexpr int BASIS_ENTRIES = 1705;  // sum_n (n+1) (n+1) (n+2) / 2, either map
+
+struct BasisTables {
+  // T1: spherical term t from Cartesian terms t1_k[t1_off[t] .. t1_off[t+1])
+  uint32_t t1_off[SPH_TERMS + 1];
+  uint32_t t1_k[BASIS_ENTRIES];
+  double t1_re[BASIS_ENTRIES], t1_im[BASIS_ENTRIES];
+  // T2: Cartesian term k from spherical terms t2_t[t2_off[k] .. t2_off[k+1]]:
+  //   L_cart[k] = sum a Re L_sph[t] + b Im L_sph[t]
+  uint32_t t2_off[CN + 1];
+  uint32_t t2_t[BASIS_ENTRIES];
+  double t2_a[BASIS_ENTRIES], t2_b[BASIS_ENTRIES];
+};
+
+inline double factorial(int n) {
+  double f = 1;
+  for (int i = 2; i <= n; i++) f *= i;
+  return f;
+}
+
+// coefficients of Y_n^m over all Cartesian terms (zero off degree n)
+inline void solid_harmonics(std::vector<std::vector<double>> &re,
+                            std::vector<std::vector<double>> &im) {
+  re.assign(SPH_TERMS, std::vector<double>(CN, 0.0));
+  im.assign(SPH_TERMS, std::vector<double>(CN, 0.0));
+  using Poly = std::pair<std::vector<double>, std::vector<double>>;  // re, im over terms
+  auto zero = [] { return Poly(std::vector<double>(CN, 0.0), std::vector<double>(CN, 0.0)); };
+  auto shift = [&](const Poly &p, int ax, int times) {  // p * e_ax^times
+    Poly q = zero();
+    for (int t = 0; t < CN; t++) {
+      if (p.first[t] == 0 && p.second[t] == 0) continue;
+      int e[3] = {term_part(t, 0), term_part(t, 1), term_part(t, 2)};
+      e[ax] += times;
+      if (e[0] + e[1] + e[2] >= ORDER) continue;
+      const int u = cidx(e[0], e[1], e[2]);
+      q.first[u] += p.first[t];
+      q.second[u] += p.second[t];
+    }
+    return q;
+  };
+  for (int m = 0; m < ORDER; m++) {
+    // R_m^m = (-1)^m (2m-1)!! (x + i y)^m
+    Poly r = zero();
+    r.first[0] = 1;
+    for (int j = 0; j < m; j++) {  // multiply by (x + i y)
+      const Poly px = shift(r, 0, 1), py = shift(r, 1, 1);
+      for (int t = 0; t < CN; t++) {
+        r.first[t] = px.first[t] - py.second[t];
+        r.second[t] = px.second[t] + py.first[t];
+      }
+    }
+    double df = 1;
+    for (int j = 1; j <= 2 * m - 1; j += 2) df *= j;
+    const double s = (m & 1 ? -1.0 : 1.0) * df;
+    for (int t = 0; t < CN; t++) {
+      r.first[t] *= s;
+      r.second[t] *= s;
+    }
+    Poly prev = zero(), cur = r;  // R_{n-1}^m, R_n^m
+    for (int n = m; n < ORDER; n++) {
+      if (n > m) {
+        const Poly zc = shift(cur, 2, 1);
+        Poly nx = zero();
+        const Poly rx = shift(prev, 0, 2), ry = shift(prev, 1, 2), rz = shift(prev, 2, 2);
+        for (int t = 0; t < CN; t++) {
+          const double r2re = rx.first[t] + ry.first[t] + rz.first[t];
+          const double r2im = rx.second[t] + ry.second[t] + rz.second[t];
+          nx.first[t] = ((2 * n - 1) * zc.first[t] - (n + m - 1) * r2re) / (n - m);
+          nx.second[t] = ((2 * n - 1) * zc.second[t] - (n + m - 1) * r2im) / (n - m);
+        }
+        prev = cur;
+        cur = nx;
+      }
+      const int st = n * (n + 1) / 2 + m;
+      const double f = (n & 1 ? -1.0 : 1.0) / factorial(n + m);
+      for (int t = 0; t < CN; t++) {
+        re[st][t] = cur.first[t] * f;
+        im[st][t] = cur.second[t] * f;
+      }
+    }
+  }
+}
+
+inline bool build_basis_tables(BasisTables &bt) {
+  std::vector<std::vector<double>> yre, yim;
+  solid_harmonics(yre, yim);
+  auto kfact = [](int t) {
+    return factorial(term_part(t, 0)) * factorial(term_part(t, 1)) * factorial(term_part(t, 2));
+  };
+  auto degree = [](int t) { return term_part(t, 0) + term_part(t, 1) + term_part(t, 2); };
+  int e = 0;
+  for (int n = 0; n < ORDER; n++)
+    for (int m = 0; m <= n; m++) {
+      const int st = n * (n + 1) / 2 + m;
+      bt.t1_off[st] = (uint32_t)e;
+      for (int k = 0; k < CN; k++) {
+        if (degree(k) != n) continue;
+        if (e >= BASIS_ENTRIES) return false;
+        const double s = (n & 1 ? -1.0 : 1.0) * kfact(k);
+        bt.t1_k[e] = (uint32_t)k;
+        bt.t1_re[e] = yre[st][k] * s;   // conj(y) (-1)^n k!
+        bt.t1_im[e] = -yim[st][k] * s;
+        e++;
+      }
+    }
+  bt.t1_off[SPH_TERMS] = (uint32_t)e;
+  if (e != BASIS_ENTRIES) return false;
+  e = 0;
+  for (int k = 0; k < CN; k++) {
+    bt.t2_off[k] = (uint32_t)e;
+    const int n = degree(k);
+    for (int m = 0; m <= n; m++) {
+      const int st = n * (n + 1) / 2 + m;
+      if (e >= BASIS_ENTRIES) return false;
+      const double w = (m == 0 ? 1.0 : 2.0) * kfact(k);
+      bt.t2_t[e] = (uint32_t)st;
+      bt.t2_a[e] = w * yre[st][k];    // Re(L y) = Re L Re y - Im L Im y
+      bt.t2_b[e] = -w * yim[st][k];
+      e++;
+    }
+  }
+  bt.t2_off[CN] = (uint32_t)e;
+  return e == BASIS_ENTRIES;
+}
+
 }  // namespace cart
 }  // namespace tfmm

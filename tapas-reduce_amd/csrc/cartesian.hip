@@ -33,8 +33,10 @@
 namespace tfmm {
 namespace {
 
+using cart::BasisTables;
 using cart::BL;
 using cart::CartTables;
+using cart::SPH_TERMS;
 using cart::cdeg_off;
 using cart::cidx;
 using cart::CN;
@@ -300,6 +302,51 @@ __global__ __launch_bounds__(64 * MW) void k_cart_m2l(Cells c, Csr list, int32_t
 }
 
 // ---------------------------------------------------------------------------
+// Basis change (cart_sched.h): the Cartesian M2L computed exactly as
+// T2 (spherical M2L (T1 M)), the spherical M2L being the library's MFMA
+// tile kernel.  One wave per cell; the source expansion staged in LDS.
+constexpr int BW = 4;
+
+// spherical multipoles (NTERM complex per cell) from Cartesian ones
+__global__ __launch_bounds__(64 * BW) void k_cart_to_sph(int32_t ncells, const double *Mc,
+                                                         double *Ms, const BasisTables *bt) {
+  __shared__ double sm[BW][CN];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int32_t ci = blockIdx.x * BW + w;
+  if (ci >= ncells) return;
+  for (int t = lane; t < CN; t += 64) sm[w][t] = Mc[(int64_t)ci * CN + t];
+  __builtin_amdgcn_wave_barrier();
+  if (lane >= SPH_TERMS) return;
+  double re = 0, im = 0;
+  for (uint32_t e = bt->t1_off[lane]; e < bt->t1_off[lane + 1]; e++) {
+    const double v = sm[w][bt->t1_k[e]];
+    re += bt->t1_re[e] * v;
+    im += bt->t1_im[e] * v;
+  }
+  Ms[((int64_t)ci * SPH_TERMS + lane) * 2] = re;
+  Ms[((int64_t)ci * SPH_TERMS + lane) * 2 + 1] = im;
+}
+
+// Cartesian locals from spherical ones
+__global__ __launch_bounds__(64 * BW) void k_sph_to_cart(int32_t ncells, const double *Ls,
+                                                         double *Lc, const BasisTables *bt) {
+  __shared__ double sl[BW][2 * SPH_TERMS];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int32_t ci = blockIdx.x * BW + w;
+  if (ci >= ncells) return;
+  for (int t = lane; t < 2 * SPH_TERMS; t += 64) sl[w][t] = Ls[(int64_t)ci * 2 * SPH_TERMS + t];
+  __builtin_amdgcn_wave_barrier();
+  for (int k = lane; k < CN; k += 64) {
+    double s = 0;
+    for (uint32_t e = bt->t2_off[k]; e < bt->t2_off[k + 1]; e++) {
+      const int t = (int)bt->t2_t[e];
+      s += bt->t2_a[e] * sl[w][2 * t] + bt->t2_b[e] * sl[w][2 * t + 1];
+    }
+    Lc[(int64_t)ci * CN + k] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // L2L: one block per child cell; thread t owns term t of the child.
 __global__ __launch_bounds__(256) void k_cart_l2l(Cells c, int32_t first_child, int32_t nchild,
                                                   double *L) {
@@ -481,16 +528,40 @@ __global__ __launch_bounds__(256) void k_cart_periodic_far(const double *Mroot, 
     Lroot[t] += ((Lsum[0][t] + Lsum[1][t]) + Lsum[2][t]) + Lsum[3][t];
 }
 
+// the device copy of both table sets
+struct DeviceTables {
+  CartTables sched;
+  BasisTables basis;
+};
+
+const CartTables *sched_of(const void *p) { return &static_cast<const DeviceTables *>(p)->sched; }
+const BasisTables *basis_of(const void *p) { return &static_cast<const DeviceTables *>(p)->basis; }
+
 }  // namespace
 
-size_t cart_tables_bytes() { return sizeof(CartTables); }
+size_t cart_tables_bytes() { return sizeof(DeviceTables); }
 
 hipError_t cart_upload_tables(void *dst) {
-  static CartTables host;
+  static DeviceTables host;
   static int state = 0;  // 0 not built, 1 built, -1 failed
-  if (state == 0) state = cart::build_tables(host) ? 1 : -1;
+  if (state == 0)
+    state = cart::build_tables(host.sched) && cart::build_basis_tables(host.basis) ? 1 : -1;
   if (state < 0) return hipErrorInvalidValue;
   return hipMemcpy(dst, &host, sizeof(host), hipMemcpyHostToDevice);
+}
+
+hipError_t launch_cart_to_sph(int32_t ncells, const double *Mc, double *Ms, const void *tables,
+                              hipStream_t s) {
+  if (ncells <= 0) return hipSuccess;
+  k_cart_to_sph<<<(ncells + BW - 1) / BW, 64 * BW, 0, s>>>(ncells, Mc, Ms, basis_of(tables));
+  return hipGetLastError();
+}
+
+hipError_t launch_sph_to_cart(int32_t ncells, const double *Ls, double *Lc, const void *tables,
+                              hipStream_t s) {
+  if (ncells <= 0) return hipSuccess;
+  k_sph_to_cart<<<(ncells + BW - 1) / BW, 64 * BW, 0, s>>>(ncells, Ls, Lc, basis_of(tables));
+  return hipGetLastError();
 }
 
 hipError_t launch_cart_p2m(Cells c, int32_t ncells, Bodies b, double *M, hipStream_t s) {
@@ -508,8 +579,8 @@ hipError_t launch_cart_m2m(Cells c, int32_t first_child, int32_t npar, double *M
 hipError_t launch_cart_m2l(Cells c, Csr list, int32_t ncells, const double *M, double *L,
                            const void *tables, double cycle, hipStream_t s) {
   if (ncells <= 0) return hipSuccess;
-  k_cart_m2l<<<(ncells + MW - 1) / MW, 64 * MW, 0, s>>>(
-      c, list, ncells, M, L, static_cast<const CartTables *>(tables), cycle);
+  k_cart_m2l<<<(ncells + MW - 1) / MW, 64 * MW, 0, s>>>(c, list, ncells, M, L,
+                                                       sched_of(tables), cycle);
   return hipGetLastError();
 }
 
@@ -530,8 +601,7 @@ hipError_t launch_cart_l2p(Cells c, int32_t ncells, Bodies b, const double *L, c
 hipError_t launch_cart_periodic_far(const double *Mroot, double *Lroot, double cycle, int images,
                                     const void *tables, hipStream_t s) {
   if (images < 2 || !(cycle > 0)) return hipSuccess;
-  k_cart_periodic_far<<<1, 256, 0, s>>>(Mroot, Lroot, cycle, images,
-                                        static_cast<const CartTables *>(tables));
+  k_cart_periodic_far<<<1, 256, 0, s>>>(Mroot, Lroot, cycle, images, sched_of(tables));
   return hipGetLastError();
 }
 

@@ -154,6 +154,12 @@ hipError_t launch_cart_l2p(Cells c, int32_t ncells, Bodies b, const double *L, c
                            const int32_t *perm, double *trg, double *out, hipStream_t s);
 hipError_t launch_cart_periodic_far(const double *Mroot, double *Lroot, double cycle, int images,
                                     const void *tables, hipStream_t s);
+// basis change of every cell: Cartesian multipoles into spherical ones (the
+// library's NTERM complex layout) and spherical locals into Cartesian ones
+hipError_t launch_cart_to_sph(int32_t ncells, const double *Mc, double *Ms, const void *tables,
+                              hipStream_t s);
+hipError_t launch_sph_to_cart(int32_t ncells, const double *Ls, double *Lc, const void *tables,
+                              hipStream_t s);
 // traverse.hip
 struct DttCounters {
   // [0] next-frontier size, [1] executed M2L pairs, [2] executed P2P leaf pairs,

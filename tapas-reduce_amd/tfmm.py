@@ -23,6 +23,7 @@ TUNE_P2P_SCHEDULE = 3
 TUNE_GRAPH = 4
 TUNE_P2P_WAVES = 5
 TUNE_DEVICE_LIMIT = 6
+TUNE_CART_M2L = 7
 DIRECT_NOALLOC_MAX = 65536
 PERIODIC_MAX_IMAGES = 6
 P = 10

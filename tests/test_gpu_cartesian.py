@@ -1,7 +1,9 @@
 """The Cartesian basis on the GPU (tfmm_set_basis, TFMM_BASIS_CARTESIAN),
 through the C ABI, against the CPU oracle's restatement of ExaFMM's
 LaplaceCartesianCPU.cxx (itself pinned to the reference's own Cartesian
-kernels and by the mathematics, tests/test_oracle_cartesian.py):
+kernels and by the mathematics, tests/test_oracle_cartesian.py), on both
+routes of the Cartesian M2L (TFMM_TUNE_CART_M2L: 0 the FP64 tile kernel
+between exact basis changes, 1 the direct Cartesian kernel):
 
 * FP64, mutual 0 and 1: fields within relative L2 1e-10 of the oracle's
   Cartesian FMM; M of every cell and L of every non-empty cell within 1e-10
@@ -38,28 +40,34 @@ def rel_max(a, b):
     return np.abs(a - b).max() / scale if scale > 0 else np.abs(a - b).max()
 
 
-def run(ctx, b, ncrit, theta, precision=tfmm.FP64, mutual=0, basis=tfmm.BASIS_CARTESIAN):
-    print("basis %d n=%d ncrit=%d theta=%g precision=%d mutual=%d" % (
-        basis, len(b[0]), ncrit, theta, precision, mutual), flush=True)
+def run(ctx, b, ncrit, theta, precision=tfmm.FP64, mutual=0, basis=tfmm.BASIS_CARTESIAN,
+        route=0):
+    print("basis %d route %d n=%d ncrit=%d theta=%g precision=%d mutual=%d" % (
+        basis, route, len(b[0]), ncrit, theta, precision, mutual), flush=True)
     ctx.set_basis(basis)
+    ctx.set_tuning(tfmm.TUNE_CART_M2L, route)
     ctx.set_bodies(*b)
     ctx.evaluate(tfmm.make_params(ncrit, theta, precision, mutual))
     return ctx.fields(), ctx.stats()
 
 
+@pytest.mark.parametrize("route", [0, 1])
 @pytest.mark.parametrize("name,ncrit,theta,mutual", [
     ("cube", 32, 0.4, 0), ("plummer", 64, 0.5, 0), ("cube", 32, 0.4, 1),
     ("plummer", 32, 0.5, 1)])
-def test_gpu_cartesian_fp64_matches_oracle(ctx, name, ncrit, theta, mutual):
+def test_gpu_cartesian_fp64_matches_oracle(ctx, name, ncrit, theta, mutual, route):
     b = datasets.make(name, 6000, seed=11)
-    f, st = run(ctx, b, ncrit, theta, mutual=mutual)
+    f, st = run(ctx, b, ncrit, theta, mutual=mutual, route=route)
     o = oracle.Run(*b, ncrit, theta, mutual=mutual, basis=oracle.CARTESIAN)
     ep, ea = oracle.rel_l2(f, o.fields())
     assert ep < TOL and ea < TOL, (ep, ea)
     cnt = o.counts()
     assert st["n_m2l"] == cnt["m2l"]
     assert st["n_p2p_calls"] == cnt["p2p_bodypairs"] + cnt["p2p_selfpairs"]
-    assert st["n_m2l_groups"] == 0 and st["n_m2l_items"] == 0 and st["n_m2l_fallback"] == 0
+    if route == 1:
+        assert st["n_m2l_groups"] == 0 and st["n_m2l_items"] == 0 and st["n_m2l_fallback"] == 0
+    else:
+        assert st["n_m2l_items"] > 0
     # expansions of every cell (cells compared in key order)
     M, L = ctx.expansions()
     assert M.shape == (st["ncells"], tfmm.CART_TERMS)
@@ -88,12 +96,13 @@ def test_gpu_cartesian_mixed(ctx):
     assert ep < 1e-4 and ea < 1e-3, (ep, ea)
 
 
-def test_gpu_cartesian_paths_agree(ctx):
+@pytest.mark.parametrize("route", [0, 1])
+def test_gpu_cartesian_paths_agree(ctx, route):
     """phase-wise calls, tfmm_evaluate and its captured numeric phase: the
     same kernels on the same lists, bit for bit"""
     b = datasets.cube(30000, seed=5)
     prm = tfmm.make_params(64, 0.4, tfmm.FP64)
-    f_eval, s_eval = run(ctx, b, 64, 0.4)
+    f_eval, s_eval = run(ctx, b, 64, 0.4, route=route)
     ctx.build_tree(prm)
     ctx.upward()
     ctx.traverse()
@@ -116,10 +125,12 @@ def test_gpu_cartesian_paths_agree(ctx):
         ctx.set_tuning(tfmm.TUNE_GRAPH, 0)
 
 
-def test_gpu_cartesian_batch(ctx):
+@pytest.mark.parametrize("route", [0, 1])
+def test_gpu_cartesian_batch(ctx, route):
     probs, _ = datasets.batch(5, seed=8, nmin=500, nmax=4000)
     probs.append((np.array([0.3]), np.array([0.2]), np.array([0.9]), np.array([1.0])))
     ctx.set_basis(tfmm.BASIS_CARTESIAN)
+    ctx.set_tuning(tfmm.TUNE_CART_M2L, route)
     ctx.set_batch(probs)
     print("forest of %d problems" % len(probs), flush=True)
     ctx.evaluate(tfmm.make_params(64, 0.4, tfmm.FP64))
@@ -135,14 +146,15 @@ def test_gpu_cartesian_batch(ctx):
         assert ep < TOL and ea < TOL, (i, ep, ea)
 
 
+@pytest.mark.parametrize("route", [0, 1])
 @pytest.mark.parametrize("images", [1, 2])
-def test_gpu_cartesian_periodic(ctx, images):
+def test_gpu_cartesian_periodic(ctx, images, route):
     r = np.random.default_rng(images)
     b = tuple(r.uniform(-np.pi, np.pi, 3000) for _ in range(3)) + (r.random(3000) - 0.5,)
     cyc = 2 * np.pi
     ctx.set_periodic(images, cyc)
     try:
-        f, _ = run(ctx, b, 32, 0.4)
+        f, _ = run(ctx, b, 32, 0.4, route=route)
         o = oracle.Run(*b, 32, 0.4, images=images, cycle=cyc, basis=oracle.CARTESIAN)
         ep, ea = oracle.rel_l2(f, o.fields())
         assert ep < TOL and ea < TOL, (images, ep, ea)
@@ -188,10 +200,18 @@ def test_gpu_cartesian_switch_keeps_tree(ctx):
 
 def test_gpu_cartesian_1m_equals_spherical(ctx):
     b = datasets.cube(1 << 20, seed=0)
-    fc, st = run(ctx, b, 64, 0.4)
     fs, ss = run(ctx, b, 64, 0.4, basis=tfmm.BASIS_SPHERICAL)
-    assert st["n_m2l"] == ss["n_m2l"]
-    ep, ea = oracle.rel_l2(fc, fs)
-    assert ep < TOL and ea < TOL, (ep, ea)
-    print("1M cube: cartesian %.2f ms, spherical %.2f ms" % (st["ms_total"], ss["ms_total"]),
-          flush=True)
+    for route in (0, 1):
+        fc, st = run(ctx, b, 64, 0.4, route=route)
+        assert st["n_m2l"] == ss["n_m2l"]
+        ep, ea = oracle.rel_l2(fc, fs)
+        assert ep < TOL and ea < TOL, (route, ep, ea)
+        print("1M cube: cartesian route %d %.2f ms, spherical %.2f ms" % (
+            route, st["ms_total"], ss["ms_total"]), flush=True)
+    ctx.set_tuning(tfmm.TUNE_CART_M2L, 0)
+
+
+def test_gpu_cartesian_route_knob(ctx):
+    assert ctx.get_tuning(tfmm.TUNE_CART_M2L) == 0
+    with pytest.raises(tfmm.TfmmError):
+        ctx.set_tuning(tfmm.TUNE_CART_M2L, 2)
