@@ -21,7 +21,9 @@ The default run also measures, each in a block of its own under "configs":
   c4  8,388,608-body two-cluster Plummer (separation 10a, mass 1:3), mixed
       precision, learned parameters, beside the fallback ncrit 64 / theta 0.4.
   c1_cartesian  c1 in the Cartesian Taylor basis (ExaFMM's
-      LaplaceCartesianCPU.cxx; FP64 expansions, every M2L pair by k_cart_m2l).
+      LaplaceCartesianCPU.cxx; FP64 expansions, the M2L through the FP64
+      tile kernel between exact basis changes).
+      --cart-m2l 1: the direct Cartesian M2L kernel instead.
 --only-headline skips them; --batch K / --dist / --n measure one workload
 (--basis cartesian: in the Cartesian basis).
 
@@ -87,6 +89,9 @@ def parse():
                          "that sets its capacities)")
     ap.add_argument("--basis", default="spherical", choices=["spherical", "cartesian"],
                     help="expansion basis of a single-workload run")
+    ap.add_argument("--cart-m2l", type=int, default=0, choices=[0, 1],
+                    help="route of the Cartesian M2L: 0 the FP64 tile kernel between basis "
+                         "changes, 1 the direct Cartesian kernel")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256 MiB device buffer three times (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -214,11 +219,11 @@ def errors(ctx, targets, direct):
     return rel(f[:, 0], direct[:, 0]), rel(f[:, 1:], direct[:, 1:])
 
 
-def kernels(st, phases, precision, workload, basis="spherical"):
+def kernels(st, phases, precision, workload, basis="spherical", cart_direct=False):
     """Achieved rates of the P2P kernel (20 flops per evaluated body pair)
-    and the M2L tile kernel (9,680 flops per grouped pair; in the Cartesian
-    basis k_cart_m2l, CART_M2L_FLOPS_PER_PAIR per pair, FP64 vector) over
-    their HIP-event times on the streams they run on."""
+    and the M2L tile kernel (9,680 flops per grouped pair; the Cartesian
+    basis's direct kernel k_cart_m2l: CART_M2L_FLOPS_PER_PAIR per pair, FP64
+    vector) over their HIP-event times on the streams they run on."""
     import tfmm
     mixed = precision == tfmm.MIXED
     pairs = st["n_p2p_calls"] - st["n_p2p_self"]
@@ -233,7 +238,7 @@ def kernels(st, phases, precision, workload, basis="spherical"):
     tile_peak = PEAK_FP32_TFLOPS if mixed else PEAK_FP64_TFLOPS
     p2p_traffic = measured_traffic(workload, p2p_name)
     tile_traffic = measured_traffic(workload, tile_name)
-    if basis == "cartesian":
+    if basis == "cartesian" and cart_direct:
         pairs_m2l = st["n_m2l_exec"]
         cart_tf = (CART_M2L_FLOPS_PER_PAIR * pairs_m2l / (tiles_ms * 1e-3) / 1e12
                    if tiles_ms > 0 else 0.0)
@@ -285,6 +290,7 @@ def set_knobs(ctx, args):
     fixed = {}
     if getattr(args, "basis", "spherical") == "cartesian":
         ctx.set_basis(tfmm.BASIS_CARTESIAN)
+        ctx.set_tuning(tfmm.TUNE_CART_M2L, getattr(args, "cart_m2l", 0))
     if args.graph:
         ctx.set_tuning(tfmm.TUNE_GRAPH, 2)
     if args.unit_factor is not None:
@@ -363,14 +369,15 @@ def bench_single(args, dist, n, precision, use_picker, steps, warmup, fp64_steps
         prec_name = "fp64" if precision == tfmm.FP64 else "mixed"
         workload = "%s N=%d P=10 ncrit=%d theta=%g %s%s%s" % (
             dist, n, ncrit, round(theta, 4), prec_name, " (learned)" if picked else "",
-            " cartesian basis" if cart else "")
+            (" cartesian basis" + (" direct M2L" if args.cart_m2l else "")) if cart else "")
         out = {"workload": workload, "value": n / (ms * 1e-3), "unit": "bodies/s",
                "ms_per_step": ms, "steps": steps, "n_bodies": n, "distribution": dist,
                "ncrit": ncrit, "theta": theta, "precision": prec_name,
                "error": {"rel_l2_pot": ep, "rel_l2_acc": ea, "targets": len(targets),
                          "against": "FP64 direct sum"},
                "picker": picked, "autotune": tuned, "phases_ms": phases, "counts": counts(st),
-               "roofline": kernels(st, phases, precision, workload, args.basis),
+               "roofline": kernels(st, phases, precision, workload, args.basis,
+                                   cart and args.cart_m2l == 1),
                "basis": args.basis}
         if not args.graph and not cart:
             out["one_graph"] = one_graph(ctx, prm, steps, n)
@@ -533,7 +540,6 @@ def main():
         c1_args.basis = "spherical"
         c1_cart = argparse.Namespace(**vars(c1_args))
         c1_cart.basis = "cartesian"
-        c1_cart.no_autotune = True  # the knobs are the spherical tile kernel's
         configs = {
             "c1": bench_single(c1_args, "cube", 1 << 20, tfmm.FP64, False, ks, 1, 0, seed=0),
             "c1_cartesian": bench_single(c1_cart, "cube", 1 << 20, tfmm.FP64, False, ks, 1, 0,

@@ -62,6 +62,41 @@ constexpr TermTab make_terms() {
 }
 __constant__ TermTab c_terms = make_terms();
 
+// 1 / e for the power chains x^e / e! (a product in place of a division)
+struct Inv {
+  double v[P];
+};
+constexpr Inv make_inv() {
+  Inv r{};
+  for (int e = 1; e < P; e++) r.v[e] = 1.0 / e;
+  return r;
+}
+__constant__ Inv c_inv = make_inv();
+#define kInv c_inv.v
+
+// M2M products: for each parent term n, the pairs (k, n - k), k <= n
+// componentwise (k | (n - k) << 8), term by term from m2m_off[n]
+constexpr int M2M_ENTRIES = 5005;
+struct M2mTab {
+  uint16_t off[CN + 1];
+  uint16_t ent[M2M_ENTRIES];
+};
+constexpr M2mTab make_m2m() {
+  M2mTab r{};
+  int e = 0;
+  for (int n = 0; n < CN; n++) {
+    r.off[n] = (uint16_t)e;
+    const int nx = term_part(n, 0), ny = term_part(n, 1), nz = term_part(n, 2);
+    for (int kx = 0; kx <= nx; kx++)
+      for (int ky = 0; ky <= ny; ky++)
+        for (int kz = 0; kz <= nz; kz++)
+          r.ent[e++] = (uint16_t)(cidx(kx, ky, kz) | cidx(nx - kx, ny - ky, nz - kz) << 8);
+  }
+  r.off[CN] = (uint16_t)e;
+  return r;
+}
+__constant__ M2mTab c_m2m = make_m2m();
+
 // ---------------------------------------------------------------------------
 // P2M: one wave per leaf, lane l owns terms l, l+64, l+128, l+192.  Every
 // body of a chunk of 64 writes its powers (X - x)^a / a! per axis (the
@@ -100,7 +135,7 @@ __global__ __launch_bounds__(64 * PW) void k_cart_p2m(Cells c, int32_t ncells, B
         tb[lane][a * P] = p;
 #pragma unroll
         for (int e = 1; e < P; e++) {
-          p = p * d[a] / e;
+          p = p * d[a] * kInv[e];
           tb[lane][a * P + e] = p;
         }
       }
@@ -121,38 +156,43 @@ __global__ __launch_bounds__(64 * PW) void k_cart_p2m(Cells c, int32_t ncells, B
 
 // ---------------------------------------------------------------------------
 // M2M: one block per group of 8 siblings (children [c0, c0 + 8) of one
-// parent); thread t owns term t of the parent.
+// parent); thread t owns term t of the parent.  Each child's multipole and
+// its shift powers (Xp - Xc)^m / m! (all terms) are staged in LDS; the
+// products of a term come from the (k, n - k) table.
 __global__ __launch_bounds__(256) void k_cart_m2m(Cells c, int32_t first_child, int32_t npar,
                                                   double *M) {
-  __shared__ double Mc[8 * CN];
+  __shared__ double Mc[8][CN];
+  __shared__ double Pw[8][CN];
   __shared__ double pw[8][PWR];
   const int32_t g = blockIdx.x;
   if (g >= npar) return;
   const int32_t c0 = first_child + 8 * g;
   const int32_t par = c.parent[c0];
-  for (int e = threadIdx.x; e < 8 * CN; e += 256) Mc[e] = M[(int64_t)c0 * CN + e];
+  for (int e = threadIdx.x; e < 8 * CN; e += 256) Mc[e / CN][e % CN] = M[(int64_t)c0 * CN + e];
   if (threadIdx.x < 24) {
     const int j = threadIdx.x / 3, a = threadIdx.x % 3;
     const double d = c.center[3 * (int64_t)par + a] - c.center[3 * (int64_t)(c0 + j) + a];
     double p = 1;
     pw[j][a * P] = p;
     for (int e = 1; e < P; e++) {
-      p = p * d / e;
+      p = p * d * kInv[e];
       pw[j][a * P + e] = p;
     }
   }
   __syncthreads();
   const int t = threadIdx.x;
+  if (t < CN) {
+    const int x = c_terms.x[t], y = c_terms.y[t], z = c_terms.z[t];
+    for (int j = 0; j < 8; j++) Pw[j][t] = pw[j][x] * pw[j][P + y] * pw[j][2 * P + z];
+  }
+  __syncthreads();
   if (t >= CN) return;
-  const int nx = c_terms.x[t], ny = c_terms.y[t], nz = c_terms.z[t];
   double s = 0;
-  for (int j = 0; j < 8; j++) {
-    if (c.nb[c0 + j] == 0) continue;  // an empty child's multipole is zero
-    for (int kx = 0; kx <= nx; kx++)
-      for (int ky = 0; ky <= ny; ky++)
-        for (int kz = 0; kz <= nz; kz++)
-          s += pw[j][nx - kx] * pw[j][P + ny - ky] * pw[j][2 * P + nz - kz] *
-               Mc[j * CN + cidx(kx, ky, kz)];
+  const int e1 = c_m2m.off[t + 1];
+  for (int e = c_m2m.off[t]; e < e1; e++) {
+    const int en = c_m2m.ent[e], k = en & 0xff, m = en >> 8;
+#pragma unroll
+    for (int j = 0; j < 8; j++) s += Pw[j][m] * Mc[j][k];
   }
   M[(int64_t)par * CN + t] += s;
 }
@@ -362,7 +402,7 @@ __global__ __launch_bounds__(256) void k_cart_l2l(Cells c, int32_t first_child, 
     double p = 1;
     pw[a * P] = p;
     for (int e = 1; e < P; e++) {
-      p = p * d / e;
+      p = p * d * kInv[e];
       pw[a * P + e] = p;
     }
   }
@@ -430,9 +470,9 @@ __global__ __launch_bounds__(64 * LW) void k_cart_l2p(Cells c, int32_t ncells, B
     px[0] = py[0] = pz[0] = 1;
 #pragma unroll
     for (int e = 1; e < P; e++) {
-      px[e] = px[e - 1] * d[0] / e;
-      py[e] = py[e - 1] * d[1] / e;
-      pz[e] = pz[e - 1] * d[2] / e;
+      px[e] = px[e - 1] * d[0] * kInv[e];
+      py[e] = py[e - 1] * d[1] * kInv[e];
+      pz[e] = pz[e - 1] * d[2] * kInv[e];
     }
     double s[4] = {0, 0, 0, 0};
     l2p_all(std::make_integer_sequence<int, CN>{}, Ls, px, py, pz, s);
@@ -506,7 +546,7 @@ __global__ __launch_bounds__(256) void k_cart_periodic_far(const double *Mroot, 
         double p = 1;
         pw[a * P] = p;
         for (int e = 1; e < P; e++) {
-          p = p * dd[a] / e;
+          p = p * dd[a] * kInv[e];
           pw[a * P + e] = p;
         }
       }
