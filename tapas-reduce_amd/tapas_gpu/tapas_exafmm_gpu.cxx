@@ -9,7 +9,8 @@
 // -t theta, -m mutual (0 or 1), -d cube|plummer|sphere, -r repeat,
 // -k check (0/1), -i images (periodic images of period 2 pi, which the
 // reference's args take but its Tapas traversal leaves at 0; with -m 0),
-// plus -p fp64|mixed.  Bodies follow the reference's
+// plus -p fp64|mixed and -b spherical|cartesian (the reference's
+// compile-time BASIS, EX/Makefile.include:13).  Bodies follow the reference's
 // dataset semantics (EX/include/dataset.h:131-271): cube U[-pi,pi)^3 or the
 // Plummer sphere scaled by 3pi/16, charges drand48() - 0.5, srand48(0).
 #include <unistd.h>
@@ -75,7 +76,8 @@ int main(int argc, char **argv) {
   double theta = 0.4;
   std::string dist = "cube", prec = "fp64";
   int opt;
-  while ((opt = getopt(argc, argv, "n:c:t:m:d:r:k:p:i:")) != -1) {
+  std::string basis_name = "spherical";
+  while ((opt = getopt(argc, argv, "n:c:t:m:d:r:k:p:i:b:")) != -1) {
     switch (opt) {
       case 'n': n = atoi(optarg); break;
       case 'c': ncrit = atoi(optarg); break;
@@ -86,6 +88,7 @@ int main(int argc, char **argv) {
       case 'k': check = atoi(optarg); break;
       case 'p': prec = optarg; break;
       case 'i': images = atoi(optarg); break;
+      case 'b': basis_name = optarg; break;
       default: fprintf(stderr, "unknown option\n"); return 2;
     }
   }
@@ -98,12 +101,17 @@ int main(int argc, char **argv) {
     return 1;
   }
   const Precision precision = prec == "mixed" ? Precision::Mixed : Precision::FP64;
+  if (basis_name != "spherical" && basis_name != "cartesian") {
+    fprintf(stderr, "TapasFMM: [Error] basis must be spherical or cartesian.\n");
+    return 1;
+  }
+  const Basis basis = basis_name == "cartesian" ? Basis::Cartesian : Basis::Spherical;
   const double cycle = 2 * M_PI;  // the reference's period (tapas_exafmm.cxx:537)
   std::vector<Body> bodies = init_bodies(n, dist);
   for (int t = 0; t < repeat; t++) {
     const double t0 = now();
     Cell *root = TapasFMM::Partition(bodies.data(), (index_t)bodies.size(), ncrit, 0, precision,
-                                     images, cycle);
+                                     images, cycle, basis);
     root->SetOptMutual(mutual);
     const double t1 = now();
     TapasFMM::Map(FMM_Upward(), *root, theta);

@@ -148,6 +148,8 @@ struct FMM_Downward {};
 struct FMM_DTT {};
 
 enum class Precision { FP64 = TFMM_FP64, Mixed = TFMM_MIXED };
+// expansion basis (the reference's compile-time BASIS, EX/Makefile.include:13)
+enum class Basis { Spherical = TFMM_BASIS_SPHERICAL, Cartesian = TFMM_BASIS_CARTESIAN };
 
 // tapas::Center: Distance between cell centres
 struct CenterTag {};
@@ -543,13 +545,15 @@ struct TapasFMM {
   // Copies the bodies and builds the octree on the GPU.  theta and the
   // precision are taken at Map(FMM_DTT, ...); the tree does not depend on them.
   // images > 0: periodic images of period cycle (tfmm_set_periodic; the
-  // reference's --images, EX/include/args.h, with its cycle 2 pi).
+  // reference's --images, EX/include/args.h, with its cycle 2 pi).  basis:
+  // the expansions' basis (tfmm_set_basis).
   static Cell *Partition(Body *b, index_t nb, int ncrit, int device = 0,
                          Precision precision = Precision::FP64, int images = 0,
-                         double cycle = 0) {
+                         double cycle = 0, Basis basis = Basis::Spherical) {
     auto t = std::make_shared<detail::Tree>();
     t->check(tfmm_create(device, &t->ctx), "tfmm_create");
     if (images > 0) t->check(tfmm_set_periodic(t->ctx, images, cycle), "tfmm_set_periodic");
+    t->check(tfmm_set_basis(t->ctx, (int)basis), "tfmm_set_basis");
     tfmm_body_desc d;
     d.n = nb;
     d.stride = sizeof(Body);

@@ -29,6 +29,27 @@ def test_driver_rejects_bad_mutual(tmp_path):
     assert r.returncode == 1 and "mutual must be 0 or 1" in r.stderr
     r = run(["-n", "1000", "-m", "1", "-i", "1"], str(tmp_path))  # images with mutual
     assert r.returncode == 1 and "-m 0" in r.stderr
+    r = run(["-n", "1000", "-b", "legendre"], str(tmp_path))
+    assert r.returncode == 1 and "basis must be" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mutual", [0, 1])
+def test_driver_cartesian_basis(tmp_path, mutual):
+    """-b cartesian (the reference's BASIS = Cartesian build): the driver's
+    check against the direct sum passes the reference's 1e-3 gate, and the
+    errors equal the spherical run's to the printed digits."""
+    errs = {}
+    for basis in ("spherical", "cartesian"):
+        print("driver basis", basis, "mutual", mutual, flush=True)
+        r = run(["-n", "20000", "-c", "32", "-t", "0.4", "-m", str(mutual), "-d", "plummer",
+                 "-b", basis], str(tmp_path))
+        assert r.returncode == 0, r.stderr
+        e = [float(v) for v in re.findall(r"Rel\. L2 Error \((?:pot|acc)\)\s*:\s*(\S+)", r.stdout)]
+        assert len(e) == 2 and max(e) < 1e-3, r.stdout
+        errs[basis] = e
+    for a, b in zip(errs["spherical"], errs["cartesian"]):
+        assert abs(a - b) <= 1e-3 * a, errs  # printed to 6 digits
 
 
 @pytest.mark.gpu
