@@ -108,6 +108,7 @@ struct tfmm_ctx {
   DevBuf front_a, front_b, m2l_pairs, p2p_pairs, cnt, lkeys, lkeys_alt, lidx, lidx_alt;
   DevBuf dtt_geo, dtt_topo;
   DevBuf m2l_off, m2l_src, p2p_off, p2p_src;
+  DevBuf seg_long;  // queue of the per-target lists too long for one wave's sort
   uint32_t m2l_cap = 0, p2p_cap = 0, front_cap = 0;
   std::vector<uint32_t> dtt_hint;  // frontier sizes of the last traversal
   int64_t dtt_hint_cells = -1;
@@ -262,7 +263,7 @@ std::vector<DevBuf *> all_buffers(tfmm_ctx *c) {
           &c->pkey, &c->pkey_alt, &c->ppos, &c->ppos_alt, &c->brec, &c->lev,
           &c->lev_bsum, &c->lev_status, &c->dtt_geo, &c->dtt_topo, &c->u_cnt,
           &c->u_off, &c->units, &c->u_part, &c->p2p_next, &c->finite_flag, &c->gsz, &c->gstep,
-          &c->pflag, &c->cart_tab, &c->Mc, &c->Lc};
+          &c->pflag, &c->cart_tab, &c->Mc, &c->Lc, &c->seg_long};
 }
 
 hipError_t reserve_cells(tfmm_ctx *c, size_t need, size_t used, hipStream_t s) {
@@ -623,9 +624,11 @@ int sort_list(tfmm_ctx *c, DevBuf &pairs, uint32_t npairs, DevBuf &off, DevBuf &
   bool alt = false;
   TRY(radix_sort_pairs32(dp<uint32_t>(c->lkeys), dp<uint32_t>(c->lidx), dp<uint32_t>(c->lkeys_alt),
                          dp<uint32_t>(c->lidx_alt), npairs, 0, sbits, c->sort_tmp.ptr, s, &alt));
+  TRY(dev_reserve(c->seg_long, (size_t)(c->ncells + 1) * 4));
   TRY(launch_csr_sorted_sources(dp<uint32_t>(alt ? c->lkeys_alt : c->lkeys),
                                 dp<uint32_t>(alt ? c->lidx_alt : c->lidx), npairs,
-                                (int32_t)c->ncells, dp<int32_t>(off), dp<int32_t>(src), s));
+                                (int32_t)c->ncells, dp<int32_t>(off), dp<int32_t>(src),
+                                dp<uint32_t>(c->seg_long), s));
   return TFMM_OK;
 }
 
@@ -1440,6 +1443,7 @@ int plan_reserve(tfmm_ctx *c) {
   TRY(dev_reserve(c->lidx, npp * 4 + 4));
   TRY(dev_reserve(c->lidx_alt, npp * 4 + 4));
   TRY(dev_reserve(c->p2p_off, (cells + 1) * 4));
+  TRY(dev_reserve(c->seg_long, (cells + 1) * 4));
   TRY(dev_reserve(c->p2p_src, npp * 4 + 4));
   if (mixed) TRY(dev_reserve(c->relpos, (size_t)n * 16));
   TRY(dev_reserve(c->acc, (size_t)n * 32));
@@ -1598,7 +1602,8 @@ int whole_record(tfmm_ctx *c) {
                            pl.sbits, c->sort_tmp.ptr, s, &lalt, cnt + 2));
     TRY(launch_csr_sorted_sources(dp<uint32_t>(lalt ? c->lkeys_alt : c->lkeys),
                                   dp<uint32_t>(lalt ? c->lidx_alt : c->lidx), npp, cells,
-                                  dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src), s, cnt + 2));
+                                  dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src),
+                                  dp<uint32_t>(c->seg_long), s, cnt + 2));
   }
   if (mixed)
     TRY(launch_leaf_relpos(k, cells, sorted_bodies(c), dp<float4>(c->relpos), s));

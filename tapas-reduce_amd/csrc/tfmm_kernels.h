@@ -187,9 +187,10 @@ hipError_t launch_pair_slots(const int2 *pairs, uint32_t n, const double4 *geo,
 // With dn the pair count is *dn on the device (n its capacity).
 hipError_t launch_split_pairs(const int2 *pairs, uint32_t n, uint32_t *tgt, uint32_t *src,
                               hipStream_t s, const uint32_t *dn = nullptr);
+// longq: ncells + 1 words of scratch (lists longer than one wave sorts)
 hipError_t launch_csr_sorted_sources(const uint32_t *tgt, const uint32_t *in, uint32_t n,
-                                     int32_t ncells, int32_t *off, int32_t *out, hipStream_t s,
-                                     const uint32_t *dn = nullptr);
+                                     int32_t ncells, int32_t *off, int32_t *out, uint32_t *longq,
+                                     hipStream_t s, const uint32_t *dn = nullptr);
 hipError_t launch_unpack_pairs(const uint64_t *keys, uint32_t n, int sbits, int32_t *src,
                                hipStream_t s, const uint32_t *dn = nullptr);
 hipError_t launch_csr_offsets(const uint64_t *sorted_keys, uint32_t n, int sbits,

@@ -338,13 +338,18 @@ __device__ inline void rank_sources(P v, int32_t n, int lane, int32_t *out) {
 }
 
 // one wave per target: its sources in ascending order (up to 64 sorted in
-// registers, longer lists ranked)
+// registers, up to SEG_CAP ranked from LDS); a longer list is queued in
+// longq (word 0 the count, then the targets) for k_seg_sort_long
 __global__ __launch_bounds__(64) void k_seg_sort(const int32_t *off, const uint32_t *in,
-                                                 int32_t *out) {
+                                                 int32_t *out, uint32_t *longq) {
   __shared__ uint32_t v[SEG_CAP];
   const int lane = threadIdx.x;
   const int32_t b = off[blockIdx.x], n = off[blockIdx.x + 1] - b;
   if (n <= 0) return;
+  if (n > SEG_CAP) {
+    if (lane == 0) longq[1 + atomicAdd(longq, 1u)] = blockIdx.x;
+    return;
+  }
   if (n <= 64) {
     // one source per lane: bitonic sort across the wave, ascending (lanes
     // past n hold a value above every cell index)
@@ -356,12 +361,60 @@ __global__ __launch_bounds__(64) void k_seg_sort(const int32_t *off, const uint3
         x = keep_min ? min(x, y) : max(x, y);
       }
     if (lane < n) out[b + lane] = (int32_t)x;
-  } else if (n <= SEG_CAP) {
+  } else {
     for (int32_t i = lane; i < n; i += 64) v[i] = in[b + i];
     __syncthreads();
     rank_sources(static_cast<const uint32_t *>(v), n, lane, out + b);
-  } else {
-    rank_sources(in + b, n, lane, out + b);  // long lists rank from global memory
+  }
+}
+
+// The queued long lists, one workgroup of 1024 threads at a time: up to
+// LONG_CAP sources bitonic-sorted in LDS (padded with the largest word),
+// longer ones ranked by the whole workgroup from global memory.  Sources of
+// one target are distinct words, so the order is unique.
+constexpr int LONG_CAP = 16384;
+constexpr int LONG_THREADS = 1024;
+
+__global__ __launch_bounds__(LONG_THREADS) void k_seg_sort_long(const int32_t *off,
+                                                                const uint32_t *in, int32_t *out,
+                                                                const uint32_t *longq) {
+  __shared__ uint32_t v[LONG_CAP];
+  const uint32_t nq = longq[0];
+  for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+    const int32_t t = (int32_t)longq[1 + q];
+    const int32_t b = off[t], n = off[t + 1] - b;
+    if (n <= LONG_CAP) {
+      int32_t m = 1;
+      while (m < n) m <<= 1;
+      for (int32_t i = threadIdx.x; i < m; i += LONG_THREADS) v[i] = i < n ? in[b + i] : ~0u;
+      __syncthreads();
+      for (int32_t k = 2; k <= m; k <<= 1)
+        for (int32_t j = k >> 1; j > 0; j >>= 1) {
+          for (int32_t i = threadIdx.x; i < m; i += LONG_THREADS) {
+            const int32_t p = i ^ j;
+            if (p > i) {
+              const uint32_t a = v[i], c = v[p];
+              if ((a > c) == ((i & k) == 0)) {
+                v[i] = c;
+                v[p] = a;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      for (int32_t i = threadIdx.x; i < n; i += LONG_THREADS) out[b + i] = (int32_t)v[i];
+    } else {
+      for (int32_t i = threadIdx.x; i < n; i += LONG_THREADS) {
+        const uint32_t x = in[b + i];
+        int32_t r = 0;
+        for (int32_t k = 0; k < n; k++) {
+          const uint32_t y = in[b + k];
+          r += (y < x || (y == x && k < i)) ? 1 : 0;
+        }
+        out[b + r] = (int32_t)x;
+      }
+    }
+    __syncthreads();  // v is reused by the next list
   }
 }
 
@@ -375,12 +428,18 @@ hipError_t launch_split_pairs(const int2 *pairs, uint32_t n, uint32_t *tgt, uint
 }
 
 // off (ncells + 1) from the target-sorted list; out: each target's sources
-// (in, in the target-sorted order) in ascending order
+// (in, in the target-sorted order) in ascending order; longq: ncells + 1
+// words of scratch (the queue of lists longer than one wave sorts)
 hipError_t launch_csr_sorted_sources(const uint32_t *tgt, const uint32_t *in, uint32_t n,
-                                     int32_t ncells, int32_t *off, int32_t *out, hipStream_t s,
-                                     const uint32_t *dn) {
+                                     int32_t ncells, int32_t *off, int32_t *out, uint32_t *longq,
+                                     hipStream_t s, const uint32_t *dn) {
   k_csr_offsets32<<<(ncells + 1 + 255) / 256, 256, 0, s>>>(tgt, n, ncells, off, dn);
-  if (n > 0 && ncells > 0) k_seg_sort<<<ncells, 64, 0, s>>>(off, in, out);
+  if (n > 0 && ncells > 0) {
+    hipError_t e = launch_fill_zero(longq, 4, s);
+    if (e != hipSuccess) return e;
+    k_seg_sort<<<ncells, 64, 0, s>>>(off, in, out, longq);
+    k_seg_sort_long<<<64, LONG_THREADS, 0, s>>>(off, in, out, longq);
+  }
   return hipGetLastError();
 }
 

@@ -155,6 +155,46 @@ def test_gpu_long_p2p_list_matches_oracle(ctx):
     assert ep < TOL and ea < TOL, (ep, ea)
 
 
+def test_gpu_very_long_p2p_lists(ctx):
+    """Caller-recorded P2P lists with targets of ~3,000, ~15,000 and every
+    nonempty leaf (~30,000) as sources: the per-target sort queues lists
+    longer than one wave's 2,048 for the workgroup sort (bitonic in LDS up
+    to 16,384, ranked by the workgroup beyond).  The lists come back sorted
+    and equal to the given sets, and a target whose sources are every leaf
+    gets the direct sum over all bodies."""
+    rng = np.random.default_rng(21)
+    n = 30000
+    x, y, z = [rng.random(n) for _ in range(3)]
+    q = rng.uniform(-1, 1, n)
+    ctx.set_bodies(x, y, z, q)
+    ctx.build_tree(tfmm.make_params(1, 0.4, tfmm.FP64))
+    c = ctx.cells()
+    leaves = np.flatnonzero((c["leaf"] == 1) & (c["nb"] > 0)).astype(np.int32)
+    assert len(leaves) > 16384
+    t0, t1, t2 = leaves[0], leaves[1], leaves[2]
+    src = {t0: leaves, t1: leaves[::2], t2: leaves[::10]}
+    pairs = [np.stack([np.full(len(s), t, np.int32), s], axis=1) for t, s in src.items()]
+    others = np.setdiff1d(leaves, [t0, t1, t2])
+    pairs.append(np.stack([others, others], axis=1))  # every other leaf with itself
+    p2p = np.concatenate(pairs)
+    rng.shuffle(p2p)
+    print("long lists: %d, %d, %d sources" % tuple(len(s) for s in src.values()), flush=True)
+    ctx.upward()
+    ctx.traverse_lists(np.zeros((0, 2), np.int32), p2p)
+    ctx.downward()
+    (_, _), (po, ps) = ctx.lists()
+    for t, s in src.items():
+        got = ps[po[t]:po[t + 1]]
+        assert np.array_equal(got, np.sort(s)), t
+    assert po[others[0] + 1] - po[others[0]] == 1
+    # target t0 sees every body through P2P (no M2L given): the direct sum
+    _, perm = ctx.sorted()
+    mine = perm[c["beg"][t0]:c["beg"][t0] + c["nb"][t0]].astype(np.int64)
+    f = ctx.fields()
+    ep, ea = oracle.rel_l2(f[mine], ctx.direct(mine))
+    assert ep < 1e-12 and ea < 1e-12, (ep, ea)
+
+
 def test_gpu_c1_matches_reference(ctx):
     """1M uniform bodies: key/cell digests, counters and 4096 sampled fields
     against the reference, and the FMM against the GPU FP64 direct sum."""
