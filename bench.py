@@ -310,9 +310,11 @@ def default_knobs(ctx):
     ctx.set_tuning(tfmm.TUNE_M2L_UNIT, 0)
     ctx.set_tuning(tfmm.TUNE_M2L_XCD_RUN, 32)
     ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, 0)
+    ctx.set_tuning(tfmm.TUNE_P2P_WAVES, 0)
 
 
 def autotune(ctx, prm, fixed, enabled):
+    import tfmm
     if fixed:
         return {"fixed": fixed}
     if not enabled:
@@ -320,8 +322,10 @@ def autotune(ctx, prm, fixed, enabled):
     t0 = time.perf_counter()
     unit, run, sched, tms = ctx.autotune(prm, reps=3)
     return {"m2l_unit_factor": unit, "m2l_xcd_run": run, "p2p_schedule": sched,
+            "p2p_waves": int(ctx.get_tuning(tfmm.TUNE_P2P_WAVES)),
             "ms_per_evaluation": tms, "seconds": time.perf_counter() - t0,
-            "note": "16 candidate settings x 3 evaluations, outside the timed steps"}
+            "note": "16 tile-kernel settings, then up to 8 P2P schedule/resident-wave "
+                    "settings, x 3 evaluations each, outside the timed steps"}
 
 
 def learned(ctx, path, precision):

@@ -305,12 +305,14 @@ int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p
 #define TFMM_TUNE_CART_M2L 7
 int tfmm_set_tuning(tfmm_ctx *ctx, int knob, double value);
 int tfmm_get_tuning(tfmm_ctx *ctx, int knob, double *value);
-/* Autotuner: evaluates the current bodies with params under every candidate
- * setting of the tile-kernel knobs (unit factor {0, 1.5, 2, 3} x XCD run
- * {8, 16, 32, 64}; the P2P schedule stays as set), reps times each, and
- * keeps the setting whose evaluation was fastest (median).  chosen (4
+/* Autotuner: evaluates the current bodies with params, reps times per
+ * candidate, in two stages -- the tile-kernel knobs (unit factor {0, 1.5,
+ * 2, 3} x XCD run {8, 16, 32, 64}) at the current P2P setting, then the
+ * P2P's schedule {0, 1, 2} x resident waves {0, 3, 4} (waves for mixed
+ * precision only, schedule 2 without a graph) at the best tile setting --
+ * and keeps the setting whose evaluation was fastest (median).  chosen (4
  * doubles, may be null): unit factor, XCD run, P2P schedule, milliseconds
- * per evaluation. */
+ * per evaluation; the P2P waves chosen stay set (tfmm_get_tuning). */
 int tfmm_autotune(tfmm_ctx *ctx, const tfmm_params *params, int reps, double *chosen);
 
 /* FP64 direct sum on the current bodies for the given target indices (caller

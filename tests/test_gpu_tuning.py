@@ -50,15 +50,25 @@ def test_gpu_autotune(ctx):
     x, y, z, q = datasets.cube(60000, seed=2)
     ctx.set_bodies(x, y, z, q)
     prm = tfmm.make_params(64, 0.4, tfmm.MIXED)
-    unit, run, sched, ms = ctx.autotune(prm, reps=1)
-    assert unit in (0.0, 1.5, 2.0, 3.0) and run in (8, 16, 32, 64) and sched in (0, 1)
-    assert ms > 0
-    assert ctx.get_tuning(tfmm.TUNE_M2L_UNIT) == unit
-    assert ctx.get_tuning(tfmm.TUNE_M2L_XCD_RUN) == run
-    assert ctx.get_tuning(tfmm.TUNE_P2P_SCHEDULE) == sched
-    ctx.set_tuning(tfmm.TUNE_M2L_UNIT, 0)
-    ctx.set_tuning(tfmm.TUNE_M2L_XCD_RUN, 32)
-    ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, 0)
+    f0 = None
+    try:
+        ctx.evaluate(prm)
+        f0 = ctx.fields()
+        unit, run, sched, ms = ctx.autotune(prm, reps=1)
+        assert unit in (0.0, 1.5, 2.0, 3.0) and run in (8, 16, 32, 64) and sched in (0, 1, 2)
+        assert ms > 0
+        assert ctx.get_tuning(tfmm.TUNE_M2L_UNIT) == unit
+        assert ctx.get_tuning(tfmm.TUNE_M2L_XCD_RUN) == run
+        assert ctx.get_tuning(tfmm.TUNE_P2P_SCHEDULE) == sched
+        assert ctx.get_tuning(tfmm.TUNE_P2P_WAVES) in (0, 3, 4)
+        ctx.evaluate(prm)  # the chosen setting: the same fields up to summation order
+        ep, ea = oracle.rel_l2(ctx.fields(), f0)
+        assert ep < 1e-6 and ea < 1e-6, (ep, ea)
+    finally:
+        ctx.set_tuning(tfmm.TUNE_M2L_UNIT, 0)
+        ctx.set_tuning(tfmm.TUNE_M2L_XCD_RUN, 32)
+        ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, 0)
+        ctx.set_tuning(tfmm.TUNE_P2P_WAVES, 0)
 
 
 def test_gpu_invalid_knobs(ctx):
