@@ -82,8 +82,12 @@ def parse():
                     help="M2L work-unit size in mean items per tile (0: one unit per tile)")
     ap.add_argument("--xcd-run", type=int, default=None,
                     help="M2L work units dealt to one XCD in a row")
-    ap.add_argument("--p2p-schedule", type=int, default=None, choices=[0, 1],
-                    help="0: P2P beside the M2L batch building, 1: beside the M2L tiles")
+    ap.add_argument("--p2p-schedule", type=int, default=None, choices=[0, 1, 2, 3],
+                    help="0: P2P from the numeric phase's start, 1: beside the M2L tiles, "
+                         "2: from phase A, beside the M2L batch building, 3: as 2, drained "
+                         "by a full-occupancy launch after the tiles")
+    ap.add_argument("--p2p-waves", type=int, default=None, choices=[0, 1, 2, 3, 4, 5, 6, 7, 8],
+                    help="fix the FP32 P2P's resident waves per SIMD (0: one workgroup per cell)")
     ap.add_argument("--graph", action="store_true",
                     help="launch every evaluation as one hipGraph (after a first evaluation "
                          "that sets its capacities)")
@@ -302,6 +306,9 @@ def set_knobs(ctx, args):
     if args.p2p_schedule is not None:
         ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, args.p2p_schedule)
         fixed["p2p_schedule"] = args.p2p_schedule
+    if getattr(args, "p2p_waves", None) is not None:
+        ctx.set_tuning(tfmm.TUNE_P2P_WAVES, args.p2p_waves)
+        fixed["p2p_waves"] = args.p2p_waves
     return fixed
 
 

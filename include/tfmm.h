@@ -255,9 +255,15 @@ int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p
 #define TFMM_TUNE_M2L_XCD_RUN 2
 /* TFMM_TUNE_P2P_SCHEDULE: where the P2P kernel (second stream) starts.  In
  * tfmm_evaluate, 0 starts it with the numeric phase, beside the upward pass
- * and the M2L kernels; 1 starts it with the M2L tile kernel.  In the
- * separate tfmm_traverse, 0 starts it as soon as its list is sorted, beside
- * the M2L batch building; 1 again with the tile kernel. */
+ * and the M2L kernels; 1 starts it with the M2L tile kernel; 2 as soon as
+ * its list is sorted, beside the M2L batch building (without a graph; with
+ * one as 0); 3 as 2 with the resident FP32 P2P (TFMM_TUNE_P2P_WAVES > 0,
+ * mixed precision) holding its few waves until the tile kernel is done, when
+ * a full-occupancy launch of the same kernel on the context stream takes the
+ * target cells still left on its counter (each cell is computed by one
+ * wave either way: the fields do not change).  In the separate
+ * tfmm_traverse, 1 starts it with the tile kernel, any other value as soon
+ * as its list is sorted. */
 #define TFMM_TUNE_P2P_SCHEDULE 3
 /* TFMM_TUNE_GRAPH: 1 records tfmm_evaluate's numeric phase (P2M, M2M, P2P,
  * M2L, L2L, L2P, whose sizes are fixed once the traversal and the M2L
@@ -308,8 +314,9 @@ int tfmm_get_tuning(tfmm_ctx *ctx, int knob, double *value);
 /* Autotuner: evaluates the current bodies with params, reps times per
  * candidate, in two stages -- the tile-kernel knobs (unit factor {0, 1.5,
  * 2, 3} x XCD run {8, 16, 32, 64}) at the current P2P setting, then the
- * P2P's schedule {0, 1, 2} x resident waves {0, 3, 4} (waves for mixed
- * precision only, schedule 2 without a graph) at the best tile setting --
+ * P2P's schedule {0, 1, 2} x resident waves {0, 3, 4} and schedule 3 with
+ * 1 or 2 waves (waves and schedule 3 for mixed precision only, schedule 2
+ * without a graph) at the best tile setting --
  * and keeps the setting whose evaluation was fastest (median).  chosen (4
  * doubles, may be null): unit factor, XCD run, P2P schedule, milliseconds
  * per evaluation; the P2P waves chosen stay set (tfmm_get_tuning). */

@@ -889,7 +889,7 @@ int run_lists(tfmm_ctx *c) {
     TRY(hipEventRecord(c->ev[EV_P2P1], c->side));
     return TFMM_OK;
   };
-  const bool p2p_early = c->p2p_schedule == 0;
+  const bool p2p_early = c->p2p_schedule != 1;
   if (p2p_early) {
     rc = launch_p2p();
     if (rc) return rc;
@@ -1101,16 +1101,40 @@ void probe_external_events(tfmm_ctx *c) {
 
 // The P2P on the second stream (forked from and later joined to the context
 // stream by events).
+// Schedule 3: the resident FP32 P2P starts early with its few waves, and a
+// full-occupancy launch of the same kernel on the context stream takes the
+// target cells still left on its counter once the tile kernel is done.
+bool p2p_drains(const tfmm_ctx *c) {
+  return c->p2p_schedule == 3 && c->p2p_waves > 0 && c->prm.precision == TFMM_MIXED;
+}
+
+int drain_p2p(tfmm_ctx *c, int64_t ncells) {
+  if (!p2p_drains(c)) return TFMM_OK;
+  const Cells k = cells_of(c);
+  Csr pl{dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src)};
+  TRY(launch_p2p_fp32_drain(k, (int32_t)ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
+                            dp<double>(c->acc), 8 * 4 * c->n_cu, dp<uint32_t>(c->p2p_next),
+                            c->stream, period(c)));
+  return TFMM_OK;
+}
+
 int fork_p2p(tfmm_ctx *c, int64_t ncells = -1) {
   hipStream_t s = c->stream;
   const Cells k = cells_of(c);
   if (ncells < 0) ncells = c->ncells;
   const bool mixed = c->prm.precision == TFMM_MIXED;
+  // the drained schedule clears the shared counter on this stream, before
+  // either launch can take from it
+  if (p2p_drains(c)) TRY(launch_fill_zero(c->p2p_next.ptr, 4, s));
   TRY(hipEventRecord(c->ev[EV_P2PREADY], s));
   TRY(hipStreamWaitEvent(c->side, c->ev[EV_P2PREADY], 0));
   TRY(rec(c, EV_P2P0, c->side));
   Csr pl{dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src)};
-  if (mixed)
+  if (p2p_drains(c))
+    TRY(launch_p2p_fp32_drain(k, (int32_t)ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
+                              dp<double>(c->acc), p2p_resident(c), dp<uint32_t>(c->p2p_next),
+                              c->side, period(c)));
+  else if (mixed)
     TRY(launch_p2p_fp32(k, (int32_t)ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
                         dp<double>(c->acc), p2p_resident(c), dp<uint32_t>(c->p2p_next), c->side,
                         period(c)));
@@ -1137,7 +1161,7 @@ int fused_prepare(tfmm_ctx *c) {
   }
   TRY(dev_reserve(c->acc, (size_t)c->n * 32));
   c->p2p_forked = false;
-  if (c->p2p_schedule == 2 && !c->use_graph) {
+  if (c->p2p_schedule >= 2 && !c->use_graph) {
     // the P2P list is complete: start the P2P now, beside the M2L batch
     // building (with resident waves it leaves that work CU slots)
     rc = fork_p2p(c);
@@ -1187,6 +1211,7 @@ int fused_numeric(tfmm_ctx *c) {
     if (!c->p2p_forked && c->p2p_schedule == 1 && (rc = fork_p2p(c))) return rc;
     if ((rc = cart_m2l(c))) return rc;
     TRY(rec(c, EV_M2L1, s));
+    if ((rc = drain_p2p(c, c->ncells))) return rc;
     TRY(hipStreamWaitEvent(s, c->ev[EV_P2PJOIN], 0));
     TRY(rec(c, EV_TRAV, s));
     if ((rc = cart_downward(c))) return rc;
@@ -1228,6 +1253,7 @@ int fused_numeric(tfmm_ctx *c) {
   TRY(launch_periodic_far(dp<double>(c->M), dp<double>(c->L), c->cycle, c->images, s));
   TRY(cart_from_tiles(c));
   TRY(rec(c, EV_M2L1, s));
+  if ((rc = drain_p2p(c, c->ncells))) return rc;
   TRY(hipStreamWaitEvent(s, c->ev[EV_P2PJOIN], 0));
   TRY(rec(c, EV_TRAV, s));
   // downward
@@ -1682,6 +1708,7 @@ int whole_record(tfmm_ctx *c) {
                         dp<double>(c->L), c->fb_part.ptr, s, dp<uint32_t>(c->g_cnt) + 1));
   }
   TRY(rec(c, EV_M2L1, s));
+  if ((rc = drain_p2p(c, cells))) return rc;
   TRY(hipStreamWaitEvent(s, c->ev[EV_P2PJOIN], 0));
   TRY(rec(c, EV_TRAV, s));
   for (int d = 0; d + 1 < pl.L; d++)
@@ -1943,8 +1970,8 @@ int tfmm_set_tuning(tfmm_ctx *c, int knob, double value) {
       c->m2l_xcd_run = (int)value;
       return TFMM_OK;
     case TFMM_TUNE_P2P_SCHEDULE:
-      if (value != 0 && value != 1 && value != 2)
-        return fail(c, TFMM_EINVAL, "P2P schedule must be 0, 1 or 2");
+      if (value != 0 && value != 1 && value != 2 && value != 3)
+        return fail(c, TFMM_EINVAL, "P2P schedule must be 0, 1, 2 or 3");
       c->p2p_schedule = (int)value;
       return TFMM_OK;
     case TFMM_TUNE_GRAPH:
@@ -2046,10 +2073,13 @@ int tfmm_autotune(tfmm_ctx *c, const tfmm_params *p, int reps, double *chosen) {
   // schedule 2 to evaluations without a graph)
   std::vector<Cand> p2p;
   const bool fp32 = p->precision == TFMM_MIXED;
-  for (int sched = 0; sched <= 2; sched++) {
+  for (int sched = 0; sched <= 3; sched++) {
     if (sched == 2 && c->use_graph) continue;
-    for (int waves : {0, 3, 4}) {
+    if (sched == 3 && !fp32) continue;
+    for (int waves : {0, 1, 2, 3, 4}) {
       if (waves && !fp32) continue;
+      // resident waves: 3 or 4 beside everything, 1 or 2 when the tiles drain
+      if (sched == 3 ? (waves == 0 || waves > 2) : (waves == 1 || waves == 2)) continue;
       if (sched == sched0 && waves == waves0) continue;  // timed in the first stage
       p2p.push_back({tiles[bt].unit, tiles[bt].run, sched, waves, {}});
     }

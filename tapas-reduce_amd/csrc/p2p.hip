@@ -414,6 +414,18 @@ hipError_t launch_leaf_relpos(Cells c, int32_t ncells, Bodies b, float4 *rel, hi
   return hipGetLastError();
 }
 
+// resident waves taking target cells from *next without clearing it first:
+// several such launches (on different streams) share the cells of one
+// cleared counter, each cell computed once by one wave
+hipError_t launch_p2p_fp32_drain(Cells c, int32_t ncells, Csr list, Bodies b, const float4 *rel,
+                                 double *acc, int waves, uint32_t *next, hipStream_t s,
+                                 double cycle) {
+  if (ncells <= 0 || waves <= 0) return hipSuccess;
+  const int grid = waves < ncells ? waves : ncells;
+  k_p2p_fp32_resident<<<grid, 64, 0, s>>>(c, ncells, list, b, rel, acc, next, cycle);
+  return hipGetLastError();
+}
+
 hipError_t launch_p2p_fp32(Cells c, int32_t ncells, Csr list, Bodies b, const float4 *rel,
                            double *acc, int resident, uint32_t *next, hipStream_t s,
                            double cycle) {

@@ -293,6 +293,11 @@ hipError_t launch_m2l_tiles(const int32_t *tile_off, const int2 *units, const ui
 hipError_t launch_leaf_relpos(Cells c, int32_t ncells, Bodies b, float4 *rel, hipStream_t s);
 // resident > 0: that many resident waves take target cells from the counter
 // next (4 bytes, zeroed here); 0: one workgroup per cell
+// the resident FP32 P2P on a counter cleared by the caller (waves
+// workgroups of one wave each); launches sharing the counter split the cells
+hipError_t launch_p2p_fp32_drain(Cells c, int32_t ncells, Csr list, Bodies b, const float4 *rel,
+                                 double *acc, int waves, uint32_t *next, hipStream_t s,
+                                 double cycle);
 hipError_t launch_p2p_fp32(Cells c, int32_t ncells, Csr list, Bodies b, const float4 *rel,
                            double *acc, int resident, uint32_t *next, hipStream_t s,
                            double cycle = 0);

@@ -55,12 +55,12 @@ def test_gpu_autotune(ctx):
         ctx.evaluate(prm)
         f0 = ctx.fields()
         unit, run, sched, ms = ctx.autotune(prm, reps=1)
-        assert unit in (0.0, 1.5, 2.0, 3.0) and run in (8, 16, 32, 64) and sched in (0, 1, 2)
+        assert unit in (0.0, 1.5, 2.0, 3.0) and run in (8, 16, 32, 64) and sched in (0, 1, 2, 3)
         assert ms > 0
         assert ctx.get_tuning(tfmm.TUNE_M2L_UNIT) == unit
         assert ctx.get_tuning(tfmm.TUNE_M2L_XCD_RUN) == run
         assert ctx.get_tuning(tfmm.TUNE_P2P_SCHEDULE) == sched
-        assert ctx.get_tuning(tfmm.TUNE_P2P_WAVES) in (0, 3, 4)
+        assert ctx.get_tuning(tfmm.TUNE_P2P_WAVES) in (0, 1, 2, 3, 4)
         ctx.evaluate(prm)  # the chosen setting: the same fields up to summation order
         ep, ea = oracle.rel_l2(ctx.fields(), f0)
         assert ep < 1e-6 and ea < 1e-6, (ep, ea)
@@ -72,7 +72,7 @@ def test_gpu_autotune(ctx):
 
 
 def test_gpu_invalid_knobs(ctx):
-    for knob, value in ((tfmm.TUNE_M2L_UNIT, -1.0), (tfmm.TUNE_M2L_XCD_RUN, 0), (tfmm.TUNE_GRAPH, 3), (tfmm.TUNE_P2P_WAVES, 9), (tfmm.TUNE_P2P_SCHEDULE, 3),
+    for knob, value in ((tfmm.TUNE_M2L_UNIT, -1.0), (tfmm.TUNE_M2L_XCD_RUN, 0), (tfmm.TUNE_GRAPH, 3), (tfmm.TUNE_P2P_WAVES, 9), (tfmm.TUNE_P2P_SCHEDULE, 4),
                         (tfmm.TUNE_M2L_XCD_RUN, 2.5),
                         (99, 1.0)):
         with pytest.raises(tfmm.TfmmError):
@@ -124,10 +124,12 @@ def test_gpu_evaluate_matches_phase_calls(ctx):
     assert np.array_equal(ctx.fields(), f1)
 
 
-@pytest.mark.parametrize("schedule", [0, 1, 2])
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3])
 def test_gpu_resident_p2p_and_schedules(ctx, schedule):
     """The FP32 P2P as resident waves taking target leaves from a counter, and
-    every P2P schedule, give bit for bit the fields of the default launch."""
+    every P2P schedule (3: the resident waves drained by a second launch on
+    the same counter after the tiles), give bit for bit the fields of the
+    default launch, also inside both graph modes."""
     x, y, z, q = datasets.plummer(60000, seed=12, charges="uniform")
     ctx.set_bodies(x, y, z, q)
     prm = tfmm.make_params(32, 0.4, tfmm.MIXED)
@@ -136,11 +138,24 @@ def test_gpu_resident_p2p_and_schedules(ctx, schedule):
     ctx.evaluate(prm)
     ref = ctx.fields()
     ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, schedule)
-    for waves in (0, 1, 3):
-        print("schedule %d waves %d" % (schedule, waves), flush=True)
-        ctx.set_tuning(tfmm.TUNE_P2P_WAVES, waves)
-        for _ in range(2):
-            ctx.evaluate(prm)
-            assert np.array_equal(ctx.fields(), ref), (schedule, waves)
-    ctx.set_tuning(tfmm.TUNE_P2P_WAVES, 0)
-    ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, 0)
+    try:
+        for waves in (0, 1, 3):
+            print("schedule %d waves %d" % (schedule, waves), flush=True)
+            ctx.set_tuning(tfmm.TUNE_P2P_WAVES, waves)
+            for _ in range(2):
+                ctx.evaluate(prm)
+                assert np.array_equal(ctx.fields(), ref), (schedule, waves)
+        if schedule == 3:
+            ctx.set_tuning(tfmm.TUNE_P2P_WAVES, 2)
+            for graph in (1, 2):
+                ctx.set_tuning(tfmm.TUNE_GRAPH, graph)
+                for rep in range(3):
+                    print("schedule 3 waves 2 graph %d launch %d" % (graph, rep), flush=True)
+                    ctx.evaluate(prm)
+                    assert np.array_equal(ctx.fields(), ref), (graph, rep)
+                    assert np.all(np.isfinite(ctx.direct(np.array([0, 30000]))))
+                ctx.set_tuning(tfmm.TUNE_GRAPH, 0)
+    finally:
+        ctx.set_tuning(tfmm.TUNE_GRAPH, 0)
+        ctx.set_tuning(tfmm.TUNE_P2P_WAVES, 0)
+        ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, 0)
