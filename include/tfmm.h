@@ -314,9 +314,8 @@ int tfmm_get_tuning(tfmm_ctx *ctx, int knob, double *value);
 /* Autotuner: evaluates the current bodies with params, reps times per
  * candidate, in two stages -- the tile-kernel knobs (unit factor {0, 1.5,
  * 2, 3} x XCD run {8, 16, 32, 64}) at the current P2P setting, then the
- * P2P's schedule {0, 1, 2} x resident waves {0, 3, 4} and schedule 3 with
- * 1 or 2 waves (waves and schedule 3 for mixed precision only, schedule 2
- * without a graph) at the best tile setting --
+ * P2P's schedule {0, 1, 2} x resident waves {0, 3, 4} (waves for mixed
+ * precision only, schedule 2 without a graph) at the best tile setting --
  * and keeps the setting whose evaluation was fastest (median).  chosen (4
  * doubles, may be null): unit factor, XCD run, P2P schedule, milliseconds
  * per evaluation; the P2P waves chosen stay set (tfmm_get_tuning). */
