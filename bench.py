@@ -223,7 +223,8 @@ def errors(ctx, targets, direct):
     return rel(f[:, 0], direct[:, 0]), rel(f[:, 1:], direct[:, 1:])
 
 
-def kernels(st, phases, precision, workload, basis="spherical", cart_direct=False):
+def kernels(st, phases, precision, workload, basis="spherical", cart_direct=False,
+            resident=False):
     """Achieved rates of the P2P kernel (20 flops per evaluated body pair)
     and the M2L tile kernel (9,680 flops per grouped pair; the Cartesian
     basis's direct kernel k_cart_m2l: CART_M2L_FLOPS_PER_PAIR per pair, FP64
@@ -234,7 +235,7 @@ def kernels(st, phases, precision, workload, basis="spherical", cart_direct=Fals
     p2p_ms = phases["ms_p2p"]
     p2p_tf = P2P_FLOPS_PER_PAIR * pairs / (p2p_ms * 1e-3) / 1e12 if p2p_ms > 0 else 0.0
     p2p_peak = PEAK_FP32_TFLOPS if mixed else PEAK_FP64_TFLOPS
-    p2p_name = "k_p2p_fp32" if mixed else "k_p2p_fp64"
+    p2p_name = ("k_p2p_fp32_resident" if resident else "k_p2p_fp32") if mixed else "k_p2p_fp64"
     grouped = st["n_m2l_exec"] - st["n_m2l_fallback"]
     tiles_ms = phases["ms_m2l_tiles"]
     m2l_tf = M2L_FLOPS_PER_PAIR * grouped / (tiles_ms * 1e-3) / 1e12 if tiles_ms > 0 else 0.0
@@ -388,7 +389,8 @@ def bench_single(args, dist, n, precision, use_picker, steps, warmup, fp64_steps
                          "against": "FP64 direct sum"},
                "picker": picked, "autotune": tuned, "phases_ms": phases, "counts": counts(st),
                "roofline": kernels(st, phases, precision, workload, args.basis,
-                                   cart and args.cart_m2l == 1),
+                                   cart and args.cart_m2l == 1,
+                                   ctx.get_tuning(tfmm.TUNE_P2P_WAVES) > 0),
                "basis": args.basis}
         if not args.graph and not cart:
             out["one_graph"] = one_graph(ctx, prm, steps, n)
@@ -467,7 +469,8 @@ def bench_batch(args, nprob, precision, steps, warmup):
                 "error": {"rel_l2_pot": ep, "rel_l2_acc": ea, "targets": len(targets),
                           "against": "FP64 direct sum within each problem"},
                 "picker": picked, "autotune": tuned, "phases_ms": phases, "counts": counts(st),
-                "roofline": kernels(st, phases, precision, workload),
+                "roofline": kernels(st, phases, precision, workload, resident=ctx.get_tuning(
+                    tfmm.TUNE_P2P_WAVES) > 0),
                 "one_graph": graph_run}, probs
     finally:
         ctx.close()
