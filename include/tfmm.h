@@ -256,8 +256,8 @@ int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p
 /* TFMM_TUNE_P2P_SCHEDULE: where the P2P kernel (second stream) starts.  In
  * tfmm_evaluate, 0 starts it with the numeric phase, beside the upward pass
  * and the M2L kernels; 1 starts it with the M2L tile kernel; 2 as soon as
- * its list is sorted, beside the M2L batch building (without a graph; with
- * one as 0); 3 as 2 with the resident FP32 P2P (TFMM_TUNE_P2P_WAVES > 0,
+ * its list is sorted, beside the M2L batch building (also inside the
+ * one-launch graph of TFMM_TUNE_GRAPH 2; with TFMM_TUNE_GRAPH 1 as 0); 3 as 2 with the resident FP32 P2P (TFMM_TUNE_P2P_WAVES > 0,
  * mixed precision) holding its few waves until the tile kernel is done, when
  * a full-occupancy launch of the same kernel on the context stream takes the
  * target cells still left on its counter (each cell is computed by one
@@ -287,9 +287,7 @@ int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p
 /* TFMM_TUNE_P2P_WAVES: 0 (default) runs the FP32 P2P as one workgroup per
  * cell; k in 1..8 runs it as k resident waves per SIMD that take target
  * leaves from a counter, so it holds a bounded share of the CUs and the
- * kernels beside it keep slots.  Schedule 2 of TFMM_TUNE_P2P_SCHEDULE
- * (tfmm_evaluate without a graph only) starts the P2P as soon as its list
- * is sorted, beside the M2L batch building. */
+ * kernels beside it keep slots (see TFMM_TUNE_P2P_SCHEDULE 2 and 3). */
 #define TFMM_TUNE_P2P_WAVES 5
 /* TFMM_TUNE_DEVICE_LIMIT: a limit in bytes (0 = none, the default) on the
  * device memory the library's buffers hold, over all contexts of the
@@ -315,7 +313,8 @@ int tfmm_get_tuning(tfmm_ctx *ctx, int knob, double *value);
  * candidate, in two stages -- the tile-kernel knobs (unit factor {0, 1.5,
  * 2, 3} x XCD run {8, 16, 32, 64}) at the current P2P setting, then the
  * P2P's schedule {0, 1, 2} x resident waves {0, 3, 4} (waves for mixed
- * precision only, schedule 2 without a graph) at the best tile setting --
+ * precision only, schedule 2 not with TFMM_TUNE_GRAPH 1) at the best tile
+ * setting --
  * and keeps the setting whose evaluation was fastest (median).  chosen (4
  * doubles, may be null): unit factor, XCD run, P2P schedule, milliseconds
  * per evaluation; the P2P waves chosen stay set (tfmm_get_tuning). */

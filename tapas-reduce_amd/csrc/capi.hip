@@ -1633,6 +1633,10 @@ int whole_record(tfmm_ctx *c) {
   }
   if (mixed)
     TRY(launch_leaf_relpos(k, cells, sorted_bodies(c), dp<float4>(c->relpos), s));
+  // schedules 2 and 3: the P2P branches off here, beside the M2L batching
+  int rc;
+  const bool early = c->p2p_schedule >= 2;
+  if (early && (rc = fork_p2p(c, cells))) return rc;
   // M2L batches
   {
     const uint32_t nm = pl.caps.m2l;
@@ -1683,8 +1687,7 @@ int whole_record(tfmm_ctx *c) {
   }
   TRY(rec(c, EV_DTT1, s));
   // the numeric phase
-  int rc;
-  if (c->p2p_schedule != 1 && (rc = fork_p2p(c, cells))) return rc;
+  if (!early && c->p2p_schedule != 1 && (rc = fork_p2p(c, cells))) return rc;
   const size_t ebytes = (size_t)cells * NTERM * 16;
   TRY(launch_fill_zero(c->M.ptr, ebytes, s));
   TRY(launch_p2m(k, cells, sorted_bodies(c), dp<double>(c->M), s));
@@ -2077,7 +2080,7 @@ int tfmm_autotune(tfmm_ctx *c, const tfmm_params *p, int reps, double *chosen) {
   // Plummer, 12.70 against 12.71 ms, and splits the P2P into two launches:
   // it stays a knob, not a candidate)
   for (int sched = 0; sched <= 2; sched++) {
-    if (sched == 2 && c->use_graph) continue;
+    if (sched == 2 && c->use_graph == 1) continue;
     for (int waves : {0, 3, 4}) {
       if (waves && !fp32) continue;
       if (sched == sched0 && waves == waves0) continue;  // timed in the first stage

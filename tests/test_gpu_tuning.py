@@ -145,12 +145,13 @@ def test_gpu_resident_p2p_and_schedules(ctx, schedule):
             for _ in range(2):
                 ctx.evaluate(prm)
                 assert np.array_equal(ctx.fields(), ref), (schedule, waves)
-        if schedule == 3:
+        if schedule >= 2:
             ctx.set_tuning(tfmm.TUNE_P2P_WAVES, 2)
             for graph in (1, 2):
                 ctx.set_tuning(tfmm.TUNE_GRAPH, graph)
                 for rep in range(3):
-                    print("schedule 3 waves 2 graph %d launch %d" % (graph, rep), flush=True)
+                    print("schedule %d waves 2 graph %d launch %d" % (schedule, graph, rep),
+                          flush=True)
                     ctx.evaluate(prm)
                     assert np.array_equal(ctx.fields(), ref), (graph, rep)
                     assert np.all(np.isfinite(ctx.direct(np.array([0, 30000]))))
