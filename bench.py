@@ -82,10 +82,10 @@ def parse():
                     help="M2L work-unit size in mean items per tile (0: one unit per tile)")
     ap.add_argument("--xcd-run", type=int, default=None,
                     help="M2L work units dealt to one XCD in a row")
-    ap.add_argument("--p2p-schedule", type=int, default=None, choices=[0, 1, 2, 3],
+    ap.add_argument("--p2p-schedule", type=int, default=None, choices=[0, 1, 2, 3, 4],
                     help="0: P2P from the numeric phase's start, 1: beside the M2L tiles, "
                          "2: from phase A, beside the M2L batch building, 3: as 2, drained "
-                         "by a full-occupancy launch after the tiles")
+                         "by a full-occupancy launch after the tiles, 4: serially, alone")
     ap.add_argument("--p2p-waves", type=int, default=None, choices=[0, 1, 2, 3, 4, 5, 6, 7, 8],
                     help="fix the FP32 P2P's resident waves per SIMD (0: one workgroup per cell)")
     ap.add_argument("--graph", action="store_true",
@@ -212,6 +212,34 @@ def one_graph(ctx, prm, steps, n):
             "note": "every step one hipGraph launch of the whole evaluation (tree, traversal, "
                     "lists, M2L batches, numeric phase), sizes read on the device; no host "
                     "read-back inside a step"}
+
+
+def p2p_isolated(ctx, prm, st, roof):
+    """The P2P kernel's own rate: three more evaluations with the P2P at full
+    occupancy (no resident-wave cap) on the context stream with nothing
+    beside it (TFMM_TUNE_P2P_SCHEDULE 4), outside the timed steps.  The
+    headline's figure is over the P2P's span while the upward pass, the M2L
+    batching and the tiles share the GPU with it."""
+    import tfmm
+    was = ctx.get_tuning(tfmm.TUNE_P2P_SCHEDULE), ctx.get_tuning(tfmm.TUNE_P2P_WAVES)
+    ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, 4)
+    ctx.set_tuning(tfmm.TUNE_P2P_WAVES, 0)
+    try:
+        ms = []
+        for _ in range(3):
+            ctx.evaluate(prm)
+            ms.append(ctx.stats()["ms_p2p"])
+    finally:
+        ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, was[0])
+        ctx.set_tuning(tfmm.TUNE_P2P_WAVES, was[1])
+    t = sorted(ms)[1]
+    pairs = st["n_p2p_calls"] - st["n_p2p_self"]
+    tf = P2P_FLOPS_PER_PAIR * pairs / (t * 1e-3) / 1e12 if t > 0 else 0.0
+    kernel = roof["kernel"].replace("_resident", "")
+    return {"kernel": kernel, "ms_per_launch": t, "achieved": tf, "peak": roof["peak"],
+            "unit": "TFLOP/s", "frac": tf / roof["peak"],
+            "note": "the P2P kernel at full occupancy with nothing beside it (P2P schedule 4, "
+                    "no resident-wave cap), median of 3 evaluations outside the timed steps"}
 
 
 def errors(ctx, targets, direct):
@@ -394,6 +422,7 @@ def bench_single(args, dist, n, precision, use_picker, steps, warmup, fp64_steps
                "basis": args.basis}
         if not args.graph and not cart:
             out["one_graph"] = one_graph(ctx, prm, steps, n)
+        out["roofline"]["isolated"] = p2p_isolated(ctx, prm, st, out["roofline"])
         if fallback_steps > 0 and picked:
             prm_fb = tfmm.make_params(64, 0.4, precision)
             msf, phf, stf = time_config(ctx, prm_fb, fallback_steps, 1)

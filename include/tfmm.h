@@ -261,7 +261,9 @@ int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p
  * mixed precision) holding its few waves until the tile kernel is done, when
  * a full-occupancy launch of the same kernel on the context stream takes the
  * target cells still left on its counter (each cell is computed by one
- * wave either way: the fields do not change).  In the separate
+ * wave either way: the fields do not change); 4 runs it on the context
+ * stream with nothing beside it (from phase A, serially: for timing the
+ * kernel alone).  In the separate
  * tfmm_traverse, 1 starts it with the tile kernel, any other value as soon
  * as its list is sorted. */
 #define TFMM_TUNE_P2P_SCHEDULE 3

@@ -1126,23 +1126,27 @@ int fork_p2p(tfmm_ctx *c, int64_t ncells = -1) {
   // the drained schedule clears the shared counter on this stream, before
   // either launch can take from it
   if (p2p_drains(c)) TRY(launch_fill_zero(c->p2p_next.ptr, 4, s));
-  TRY(hipEventRecord(c->ev[EV_P2PREADY], s));
-  TRY(hipStreamWaitEvent(c->side, c->ev[EV_P2PREADY], 0));
-  TRY(rec(c, EV_P2P0, c->side));
+  // schedule 4: on the context stream, alone (the kernel's own time)
+  hipStream_t ps = c->p2p_schedule == 4 ? s : c->side;
+  if (ps != s) {
+    TRY(hipEventRecord(c->ev[EV_P2PREADY], s));
+    TRY(hipStreamWaitEvent(ps, c->ev[EV_P2PREADY], 0));
+  }
+  TRY(rec(c, EV_P2P0, ps));
   Csr pl{dp<int32_t>(c->p2p_off), dp<int32_t>(c->p2p_src)};
   if (p2p_drains(c))
     TRY(launch_p2p_fp32_drain(k, (int32_t)ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
                               dp<double>(c->acc), p2p_resident(c), dp<uint32_t>(c->p2p_next),
-                              c->side, period(c)));
+                              ps, period(c)));
   else if (mixed)
     TRY(launch_p2p_fp32(k, (int32_t)ncells, pl, sorted_bodies(c), dp<float4>(c->relpos),
-                        dp<double>(c->acc), p2p_resident(c), dp<uint32_t>(c->p2p_next), c->side,
+                        dp<double>(c->acc), p2p_resident(c), dp<uint32_t>(c->p2p_next), ps,
                         period(c)));
   else
-    TRY(launch_p2p_fp64(k, (int32_t)ncells, pl, sorted_bodies(c), dp<double>(c->acc),
-                        c->side, period(c)));
-  TRY(rec(c, EV_P2P1, c->side));
-  TRY(hipEventRecord(c->ev[EV_P2PJOIN], c->side));
+    TRY(launch_p2p_fp64(k, (int32_t)ncells, pl, sorted_bodies(c), dp<double>(c->acc), ps,
+                        period(c)));
+  TRY(rec(c, EV_P2P1, ps));
+  TRY(hipEventRecord(c->ev[EV_P2PJOIN], ps));
   return TFMM_OK;
 }
 
@@ -1973,8 +1977,8 @@ int tfmm_set_tuning(tfmm_ctx *c, int knob, double value) {
       c->m2l_xcd_run = (int)value;
       return TFMM_OK;
     case TFMM_TUNE_P2P_SCHEDULE:
-      if (value != 0 && value != 1 && value != 2 && value != 3)
-        return fail(c, TFMM_EINVAL, "P2P schedule must be 0, 1, 2 or 3");
+      if (value != 0 && value != 1 && value != 2 && value != 3 && value != 4)
+        return fail(c, TFMM_EINVAL, "P2P schedule must be 0 ... 4");
       c->p2p_schedule = (int)value;
       return TFMM_OK;
     case TFMM_TUNE_GRAPH:

@@ -72,7 +72,7 @@ def test_gpu_autotune(ctx):
 
 
 def test_gpu_invalid_knobs(ctx):
-    for knob, value in ((tfmm.TUNE_M2L_UNIT, -1.0), (tfmm.TUNE_M2L_XCD_RUN, 0), (tfmm.TUNE_GRAPH, 3), (tfmm.TUNE_P2P_WAVES, 9), (tfmm.TUNE_P2P_SCHEDULE, 4),
+    for knob, value in ((tfmm.TUNE_M2L_UNIT, -1.0), (tfmm.TUNE_M2L_XCD_RUN, 0), (tfmm.TUNE_GRAPH, 3), (tfmm.TUNE_P2P_WAVES, 9), (tfmm.TUNE_P2P_SCHEDULE, 5),
                         (tfmm.TUNE_M2L_XCD_RUN, 2.5),
                         (99, 1.0)):
         with pytest.raises(tfmm.TfmmError):
@@ -124,7 +124,7 @@ def test_gpu_evaluate_matches_phase_calls(ctx):
     assert np.array_equal(ctx.fields(), f1)
 
 
-@pytest.mark.parametrize("schedule", [0, 1, 2, 3])
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3, 4])
 def test_gpu_resident_p2p_and_schedules(ctx, schedule):
     """The FP32 P2P as resident waves taking target leaves from a counter, and
     every P2P schedule (3: the resident waves drained by a second launch on
