@@ -790,10 +790,11 @@ int build_m2l_batches(tfmm_ctx *c) {
   const int32_t tile = m2l_tile_size();
   const int32_t ntiles = (int32_t)((c->ncells + tile - 1) / tile);
   const int tbits = bits_for(ntiles);
-  TRY(dev_reserve(c->sort_tmp, radix_tmp_bytes(std::max<uint32_t>(ngp, nfb))));
+  TRY(dev_reserve(c->sort_tmp, radix_tmp_bytes(n)));
   bool alt = false;
+  // all n keys: the grouped ones sort to the first ngp places
   TRY(radix_sort_pairs32(dp<uint32_t>(c->gkeys), dp<uint32_t>(c->gvals), dp<uint32_t>(c->gkeys_alt),
-                         dp<uint32_t>(c->gvals_alt), ngp, 0, tbits + gbits, c->sort_tmp.ptr, s,
+                         dp<uint32_t>(c->gvals_alt), n, 0, tbits + gbits, c->sort_tmp.ptr, s,
                          &alt));
   c->grouped_keys_in_alt = alt;
   const uint32_t *gk = dp<uint32_t>(alt ? c->gkeys_alt : c->gkeys);
@@ -1656,9 +1657,10 @@ int whole_record(tfmm_ctx *c) {
                             dp<uint32_t>(c->gvals), dp<uint64_t>(c->fkeys),
                             dp<uint32_t>(c->fvals), gcnt, s, cnt + 1, pl.groups));
     bool galt = false;
+    // every pair's key (the grouped ones sort first, gcnt[0] of them)
     TRY(radix_sort_pairs32(dp<uint32_t>(c->gkeys), dp<uint32_t>(c->gvals),
                            dp<uint32_t>(c->gkeys_alt), dp<uint32_t>(c->gvals_alt), nm, 0,
-                           pl.tbits + pl.gbits, c->sort_tmp.ptr, s, &galt, gcnt));
+                           pl.tbits + pl.gbits, c->sort_tmp.ptr, s, &galt, cnt + 1));
     pl.gkeys_alt = galt;
     const uint32_t *gk = dp<uint32_t>(galt ? c->gkeys_alt : c->gkeys);
     const uint32_t *gv = dp<uint32_t>(galt ? c->gvals_alt : c->gvals);

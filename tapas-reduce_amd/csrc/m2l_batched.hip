@@ -112,8 +112,16 @@ __device__ inline void slot_decode(int32_t slot, int &lt, int &ls, int o[3]) {
 // target | source).  Within one (tile, group) segment every target occurs at
 // most once, so the order inside a segment does not affect any sum and the
 // key needs no slot bits.
+// Grouped keys stay at the pair's index; a pair outside the groups leaves
+// the key NO_GROUP there, which sorts after every grouped key (the tile
+// field of a key is at most 2^tbits - 2), so a sort of all n keys puts the
+// grouped pairs first.  The pairs outside the groups are appended compactly
+// (counters[1], one atomic per block that has any); k_grouped_count then
+// sets counters[0] to the grouped count.
 // With dn the pair count is *dn (n its capacity), and group indices are
 // clamped below gcap (a capacity the caller checks after the launch).
+constexpr uint32_t NO_GROUP = 0xffffffffu;
+
 __global__ __launch_bounds__(1024) void k_classify(const int2 *pairs, const int32_t *slot_of,
                                                   uint32_t n, const uint32_t *gid_of_slot, int gbits,
                                                   int sbits, uint32_t *gkeys, uint32_t *gvals,
@@ -132,17 +140,24 @@ __global__ __launch_bounds__(1024) void k_classify(const int2 *pairs, const int3
   }
   const bool grouped = valid && slot >= 0;
   const bool fallback = valid && slot < 0;
-  const uint32_t at = block_append<16>(&counters[0], grouped, 1, scratch);
-  if (grouped) {
+  if (valid) {
     const uint64_t tile = (uint32_t)p.x / TILE, in_tile = (uint32_t)p.x % TILE;
-    gkeys[at] = (uint32_t)(tile << gbits) | min(gid_of_slot[slot], gcap - 1u);
-    gvals[at] = ((uint32_t)p.y & CELL_MASK) | (uint32_t)(in_tile << SRC_BITS);
+    gkeys[i] = grouped ? (uint32_t)(tile << gbits) | min(gid_of_slot[slot], gcap - 1u) : NO_GROUP;
+    gvals[i] = grouped ? ((uint32_t)p.y & CELL_MASK) | (uint32_t)(in_tile << SRC_BITS) : 0u;
   }
+  if (!__syncthreads_or(fallback)) return;  // the usual block: nothing outside the groups
   const uint32_t af = block_append<16>(&counters[1], fallback, 1, scratch);
   if (fallback) {
     fkeys[af] = ((uint64_t)(uint32_t)p.x << sbits) | (uint64_t)(uint32_t)p.y;
     fvals[af] = 0;
   }
+}
+
+// counters[0] = the grouped pairs: the pair count less those outside
+__global__ void k_grouped_count(uint32_t n, const uint32_t *dn, uint32_t *counters) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (dn) n = min(n, *dn);
+  counters[0] = n - min(n, counters[1]);
 }
 
 // k_classify when every pair is in the table: keys and values in pair order
@@ -772,6 +787,7 @@ hipError_t launch_m2l_classify(const int2 *pairs, const int32_t *slot_of, uint32
   k_classify<<<(n + 1023) / 1024, 1024, 0, s>>>(pairs, slot_of, n, gid_of_slot, gbits, sbits,
                                                   gkeys, gvals, fkeys, fvals, counters, dn,
                                                   gcap);
+  k_grouped_count<<<1, 64, 0, s>>>(n, dn, counters);
   return hipGetLastError();
 }
 

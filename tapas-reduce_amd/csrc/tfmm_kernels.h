@@ -232,6 +232,10 @@ size_t m2l_codes_bytes();
 hipError_t launch_m2l_classify_all(const int2 *pairs, const int32_t *slot_of, uint32_t n,
                                    const uint32_t *gid_of_slot, int gbits, uint32_t *gkeys,
                                    uint32_t *gvals, hipStream_t s);
+// Grouped keys and values at the pair's index, the key 0xffffffff (after
+// every grouped key) for a pair outside the groups, which is appended to
+// fkeys/fvals; counters[1] = pairs outside, counters[0] = grouped pairs.  A
+// sort of all n keys puts the grouped pairs first.
 // With dn the pair count is *dn on the device (n its capacity); group
 // indices are clamped below gcap, groups at or past gcap get no harmonics.
 hipError_t launch_m2l_classify(const int2 *pairs, const int32_t *slot_of, uint32_t n,

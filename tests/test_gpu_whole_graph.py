@@ -167,3 +167,38 @@ def test_gpu_whole_graph_plan_follows_parameters(ctx, ref):
         ctx.set_tuning(tfmm.TUNE_GRAPH, 0)
         ctx.set_tuning(tfmm.TUNE_M2L_UNIT, 0)
         ref.set_tuning(tfmm.TUNE_M2L_UNIT, 0)
+
+
+def test_gpu_pairs_outside_the_groups(ctx, ref):
+    """An adaptive problem whose M2L pairs partly fall outside the offset
+    groups (level difference above 3, or an offset beyond the table) runs the
+    split into grouped keys and the per-pair list: FP64 within relative L2
+    1e-10 of the CPU oracle, and as one graph launch bit for bit the ordinary
+    evaluation, with the same counters."""
+    prm = tfmm.make_params(16, 0.4, tfmm.FP64)
+    found = None
+    for name, n, seed in (("two_plummer", 20000, 3), ("plummer", 30000, 7),
+                          ("two_plummer", 60000, 5)):
+        b = datasets.make(name, n, seed=seed)
+        f_ref, s_ref = ordinary(ref, b, prm)
+        print("%s n=%d: %d of %d M2L pairs outside the groups" % (
+            name, n, s_ref["n_m2l_fallback"], s_ref["n_m2l_exec"]), flush=True)
+        if s_ref["n_m2l_fallback"] > 0:
+            found = (b, f_ref, s_ref)
+            break
+    assert found is not None, "no case with pairs outside the groups"
+    b, f_ref, s_ref = found
+    o = oracle.Run(*b, 16, 0.4)
+    ep, ea = oracle.rel_l2(f_ref, o.fields())
+    assert ep < 1e-10 and ea < 1e-10, (ep, ea)
+    ctx.set_tuning(tfmm.TUNE_GRAPH, 2)
+    try:
+        ctx.set_bodies(*b)
+        ctx.evaluate(prm)
+        for rep in range(2):
+            print("pairs outside the groups, graph launch %d" % rep, flush=True)
+            ctx.evaluate(prm)
+            s = check(ctx, f_ref, s_ref, ("outside", rep))
+            assert s["one_graph"] == 1
+    finally:
+        ctx.set_tuning(tfmm.TUNE_GRAPH, 0)
