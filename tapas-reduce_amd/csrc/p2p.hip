@@ -68,13 +68,15 @@ __device__ inline float rsq_guarded(float r2) {
   return __builtin_fmaxf(__builtin_amdgcn_rsqf(r2), 0.f);
 }
 
-// 1/sqrt(x) in FP64: hardware estimate (about 23 bits) and two Newton steps
-// y <- y + y/2 (1 - x y^2); a negative x gives NaN, max(NaN, 0) = 0
+// 1/sqrt(x) in FP64: the hardware estimate (relative error 5.2e-8 at most,
+// about 24 bits) and one Newton step y <- y + y/2 (1 - x y^2), which squares
+// the error: 4.1e-15 at most, a few ulp (both measured on gfx950 over 2^20
+// arguments spread over [1e-12, 1e12]; a second step gives 1.4e-16 and costs
+// 4 of the pair's 23 VALU instructions).  A negative x gives NaN, and
+// max(NaN, 0) = 0.
 __device__ inline double rsq_guarded(double x) {
   double y = __builtin_amdgcn_rsq(x);
-  double e = __builtin_fma(-x * y, y, 1.0);
-  y = __builtin_fma(0.5 * y, e, y);
-  e = __builtin_fma(-x * y, y, 1.0);
+  const double e = __builtin_fma(-x * y, y, 1.0);
   y = __builtin_fma(0.5 * y, e, y);
   return __builtin_fmax(y, 0.0);
 }
