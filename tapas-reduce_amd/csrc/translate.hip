@@ -344,16 +344,21 @@ __global__ __launch_bounds__(8 * 64) void k_l2l_mfma(Cells c, int32_t first_chil
 }
 
 // Levels with few parents: one block per (16 parents, row block) and one
-// wave per octant (M2M) or per quarter of the k-steps (L2L), every wave's
-// operands loaded at once; the partial products meet in LDS.  A single
-// wave walking all 28 k-steps of a row block waits out one load latency per
-// step, which on the upper levels (a handful of blocks) is the whole launch.
-__global__ __launch_bounds__(8 * 64) void k_m2m_small(Cells c, int32_t first_child, int32_t npar,
-                                                      const double *Fm, double *M,
-                                                      const int32_t *lev, int dl, int sel) {
+// wave per two octants (M2M) or per quarter of the k-steps (L2L), every
+// wave's operands of an octant loaded at once; the partial products meet in
+// LDS.  A single wave walking all 28 k-steps of a row block waits out one
+// load latency per step, which on the upper levels (a handful of blocks) is
+// the whole launch.  The M2M block is 4 waves (one per SIMD) so that it
+// fits beside the resident P2P's waves (184 VGPRs a wave: an 8-wave block,
+// two waves a SIMD, waited for the P2P to end).
+constexpr int M2M_SMALL_WAVES = 4;
+
+__global__ __launch_bounds__(M2M_SMALL_WAVES * 64) void k_m2m_small(
+    Cells c, int32_t first_child, int32_t npar, const double *Fm, double *M, const int32_t *lev,
+    int dl, int sel) {
   if (!level_extent(lev, dl, sel, first_child, npar)) return;
   if ((int32_t)blockIdx.x * 16 >= npar) return;
-  const int lane = threadIdx.x & 63, o = threadIdx.x >> 6, rb = blockIdx.y;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, rb = blockIdx.y;
   const int col = lane & 15, kq = lane >> 4;
   __shared__ double sig[16];
   __shared__ double cf[16][NV + 2];
@@ -371,25 +376,36 @@ __global__ __launch_bounds__(8 * 64) void k_m2m_small(Cells c, int32_t first_chi
   const int32_t t = (int32_t)blockIdx.x * 16 + col;
   const bool valid = t < npar;
   const int32_t c0 = first_child + 8 * (valid ? t : 0);
-  const double *B = M + (int64_t)c0 * NV + o * NV;
-  const double *A = Fm + (int64_t)rb * KS * 8 * 64 + o * 64 + lane;
   const int kend = m2m_kend(rb);
-  double a[KS], b[KS];
+  // per octant, the k-steps in two batches of KH, each batch's operands
+  // loaded at once (one batch of every operand would not fit beside the P2P)
+  constexpr int KH = KS / 2;
+  static_assert(2 * KH == KS, "two equal batches of k-steps");
+#pragma unroll 1
+  for (int o = w; o < 8; o += M2M_SMALL_WAVES) {
+    const double *B = M + (int64_t)c0 * NV + o * NV;
+    const double *A = Fm + (int64_t)rb * KS * 8 * 64 + o * 64 + lane;
+    d4 acc = {0, 0, 0, 0};
+#pragma unroll 1
+    for (int k0 = 0; k0 < KS; k0 += KH) {
+      double a[KH], b[KH];
 #pragma unroll
-  for (int ks = 0; ks < KS; ks++) {
-    const int kc = 4 * ks + kq;
-    const bool in = kc < kend;
-    const int kk = in ? kc : 0;
-    a[ks] = A[ks * 8 * 64];
-    b[ks] = B[kk] * (in ? cf[col][kk] : 0.0);
+      for (int j = 0; j < KH; j++) {
+        const int kc = 4 * (k0 + j) + kq;
+        const bool in = kc < kend;
+        const int kk = in ? kc : 0;
+        a[j] = A[(k0 + j) * 8 * 64];
+        b[j] = B[kk] * (in ? cf[col][kk] : 0.0);
+      }
+#pragma unroll
+      for (int j = 0; j < KH; j++)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j], b[j], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) red[o][lane][i] = acc[i];
   }
-  d4 acc = {0, 0, 0, 0};
-#pragma unroll
-  for (int ks = 0; ks < KS; ks++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ks], b[ks], acc, 0, 0, 0);
-#pragma unroll
-  for (int i = 0; i < 4; i++) red[o][lane][i] = acc[i];
   __syncthreads();
-  if (o != 0 || !valid) return;
+  if (w != 0 || !valid) return;
   const int32_t p = c.parent[c0];
 #pragma unroll
   for (int i = 0; i < 4; i++) {
@@ -601,12 +617,13 @@ hipError_t launch_m2m(Cells c, int32_t first_child, int32_t npar, const double *
   const double *Fm = ops + (size_t)NV * KM + (size_t)8 * NV * NV;
   const int nb = (npar + 15) / 16;
   if (lev) {  // the level's size on the device picks the variant
-    k_m2m_small<<<dim3(std::min(nb, SMALL_BLOCKS_M2M - 1), 7), 8 * 64, 0, s>>>(
+    k_m2m_small<<<dim3(std::min(nb, SMALL_BLOCKS_M2M - 1), 7), M2M_SMALL_WAVES * 64, 0, s>>>(
         c, first_child, npar, Fm, M, lev, dl, SMALL_BLOCKS_M2M);
     if (nb >= SMALL_BLOCKS_M2M)
       k_m2m_mfma<<<nb, 7 * 64, 0, s>>>(c, first_child, npar, Fm, M, lev, dl, -SMALL_BLOCKS_M2M);
   } else if (nb < SMALL_BLOCKS_M2M) {
-    k_m2m_small<<<dim3(nb, 7), 8 * 64, 0, s>>>(c, first_child, npar, Fm, M, lev, dl, 0);
+    k_m2m_small<<<dim3(nb, 7), M2M_SMALL_WAVES * 64, 0, s>>>(c, first_child, npar, Fm, M, lev,
+                                                             dl, 0);
   } else {
     k_m2m_mfma<<<nb, 7 * 64, 0, s>>>(c, first_child, npar, Fm, M, lev, dl, 0);
   }
