@@ -107,6 +107,7 @@ constexpr int PWR = 3 * P;
 
 __global__ __launch_bounds__(64 * PW) void k_cart_p2m(Cells c, int32_t ncells, Bodies b,
                                                       double *M) {
+  critical_path_priority();
   __shared__ double tab[PW][64][PWR + 1];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int32_t ci = blockIdx.x * PW + w;
@@ -161,6 +162,7 @@ __global__ __launch_bounds__(64 * PW) void k_cart_p2m(Cells c, int32_t ncells, B
 // products of a term come from the (k, n - k) table.
 __global__ __launch_bounds__(256) void k_cart_m2m(Cells c, int32_t first_child, int32_t npar,
                                                   double *M) {
+  critical_path_priority();
   __shared__ double Mc[8][CN];
   __shared__ double Pw[8][CN];
   __shared__ double pw[8][PWR];
@@ -305,6 +307,7 @@ constexpr int MW = 4;  // M2L waves (target cells) per block
 __global__ __launch_bounds__(64 * MW) void k_cart_m2l(Cells c, Csr list, int32_t ncells,
                                                       const double *M, double *L,
                                                       const CartTables *tb, double cycle) {
+  critical_path_priority();
   __shared__ CartShared sh;
   __shared__ double sMs[MW][MS_LEN];
   __shared__ double sCs[MW][CN + 4];
@@ -350,6 +353,7 @@ constexpr int BW = 4;
 // spherical multipoles (NTERM complex per cell) from Cartesian ones
 __global__ __launch_bounds__(64 * BW) void k_cart_to_sph(int32_t ncells, const double *Mc,
                                                          double *Ms, const BasisTables *bt) {
+  critical_path_priority();
   __shared__ double sm[BW][CN];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int32_t ci = blockIdx.x * BW + w;
@@ -370,6 +374,7 @@ __global__ __launch_bounds__(64 * BW) void k_cart_to_sph(int32_t ncells, const d
 // Cartesian locals from spherical ones
 __global__ __launch_bounds__(64 * BW) void k_sph_to_cart(int32_t ncells, const double *Ls,
                                                          double *Lc, const BasisTables *bt) {
+  critical_path_priority();
   __shared__ double sl[BW][2 * SPH_TERMS];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int32_t ci = blockIdx.x * BW + w;

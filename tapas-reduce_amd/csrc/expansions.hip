@@ -43,6 +43,7 @@ __device__ inline int visit_term(int v) {
 
 __global__ __launch_bounds__(64 * P2M_WAVES) void k_p2m(Cells c, int32_t ncells, Bodies b,
                                                         double *M) {
+  critical_path_priority();
   __shared__ double stage[P2M_WAVES][64][17];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int32_t ci = blockIdx.x * P2M_WAVES + w;
@@ -195,6 +196,7 @@ __global__ __launch_bounds__(256) void k_l2p(Cells c, Bodies b, const int32_t *l
 // Zero fill (in place of hipMemsetAsync, which must not be recorded into a
 // captured graph): 16-byte stores, the tail by single bytes.
 __global__ void k_fill_zero(uint4 *p, size_t n16, unsigned char *tail, int ntail) {
+  critical_path_priority();
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n16) p[i] = make_uint4(0, 0, 0, 0);
   if (i < (size_t)ntail) tail[i] = 0;

@@ -26,6 +26,7 @@ __global__ __launch_bounds__(64 * PAIR_WAVES) void k_m2l_pairs_fp64(Cells c, Csr
                                                                    const double *M, cd *part,
                                                                    const uint32_t *dn,
                                                                    double cycle) {
+  critical_path_priority();
   __shared__ cd Ys[PAIR_WAVES][NTERM];
   __shared__ cd Ms[PAIR_WAVES][NTERM];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -87,6 +88,7 @@ __global__ void k_list_targets(const int32_t *off, int32_t ncells, int32_t npair
 // wave per target
 __global__ __launch_bounds__(256) void k_m2l_reduce(Csr list, int32_t ncells, int32_t p0,
                                                     int32_t p1, const cd *part, double *L) {
+  critical_path_priority();
   const int lane = threadIdx.x & 63;
   const int32_t ti = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (ti >= ncells || lane >= NTERM) return;

@@ -128,6 +128,7 @@ __global__ __launch_bounds__(1024) void k_classify(const int2 *pairs, const int3
                                                   uint64_t *fkeys, uint32_t *fvals,
                                                   uint32_t *counters, const uint32_t *dn,
                                                   uint32_t gcap) {
+  critical_path_priority();
   __shared__ uint32_t scratch[17];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (dn) n = min(n, *dn);
@@ -164,6 +165,7 @@ __global__ void k_grouped_count(uint32_t n, const uint32_t *dn, uint32_t *counte
 __global__ void k_classify_all(const int2 *pairs, const int32_t *slot_of, uint32_t n,
                                const uint32_t *gid_of_slot, int gbits, uint32_t *gkeys,
                                uint32_t *gvals) {
+  critical_path_priority();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int2 p = pairs[i];
@@ -179,6 +181,7 @@ __global__ void k_classify_all(const int2 *pairs, const int32_t *slot_of, uint32
 __global__ void k_group_harmonics(const uint32_t *used, const uint32_t *gid_of_slot,
                                   int32_t nslots, double *Y64, float *Y32, int8_t *gdl,
                                   int8_t *glt, uint32_t gcap) {
+  critical_path_priority();
   const int32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= nslots || !used[slot]) return;
   const uint32_t g = gid_of_slot[slot];
@@ -216,6 +219,7 @@ __global__ void k_group_harmonics(const uint32_t *used, const uint32_t *gid_of_s
 // Mq[cell][q][t] = M~[cell][4 t + q], row r = 2 tri(n, m) + (0 re | 1 im).
 template <class T>
 __global__ void k_scale_m(Cells c, int32_t ncells, const double *M, T *Mq) {
+  critical_path_priority();
   const int32_t ci = blockIdx.x;
   if (ci >= ncells) return;
   const int r = threadIdx.x;  // 0..111
@@ -254,6 +258,7 @@ constexpr int IP_ROWS = IP_WAVES + 3;  // 2 rows before the block's, 1 after
 __global__ __launch_bounds__(64 * IP_WAVES) void k_item_pos(const uint32_t *keys, uint32_t n,
                                                             uint32_t *ipos, uint32_t *bcount,
                                                             const uint32_t *dn) {
+  critical_path_priority();
   __shared__ uint32_t sk[IP_ROWS * 64];
   __shared__ uint64_t mark[IP_ROWS];
   __shared__ uint32_t wcount[IP_WAVES];
@@ -317,6 +322,7 @@ __global__ __launch_bounds__(64 * IP_WAVES) void k_items(const uint32_t *keys,
                                                          int gbits, const int8_t *gdl,
                                                          int4 *items, uint2 *cols,
                                                          const uint32_t *dn, uint32_t items_cap) {
+  critical_path_priority();
   __shared__ uint32_t wcount[IP_WAVES];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -354,6 +360,7 @@ __global__ __launch_bounds__(64 * IP_WAVES) void k_items(const uint32_t *keys,
 
 __global__ void k_tile_offsets(const int4 *items, uint32_t nitems, int32_t ntiles,
                                int32_t *off, const uint32_t *dnitems) {
+  critical_path_priority();
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t > ntiles) return;
   if (dnitems) nitems = min(nitems, *dnitems);
@@ -602,6 +609,7 @@ __device__ inline int tile_of_block(int b, int ntiles, int run) {
 // not depend on timing.
 __global__ void k_unit_counts(const int32_t *tile_off, int32_t ntiles, int32_t chunk,
                               uint32_t *ucnt, const int32_t *dchunk) {
+  critical_path_priority();
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
   if (dchunk) chunk = *dchunk;
@@ -611,6 +619,7 @@ __global__ void k_unit_counts(const int32_t *tile_off, int32_t ntiles, int32_t c
 
 __global__ void k_unit_map(const uint32_t *ucnt, const uint32_t *uoff, int32_t ntiles,
                            int2 *units, uint32_t units_cap) {
+  critical_path_priority();
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
   for (uint32_t k = 0; k < ucnt[t] && uoff[t] + k < units_cap; k++)
@@ -626,6 +635,7 @@ __device__ inline int64_t extra_slab(const uint32_t *uoff, int32_t t, int k) {
 // caller checks the unit count against its capacity)
 __global__ void k_unit_combine(const uint32_t *ucnt, const uint32_t *uoff, int32_t ntiles,
                                const double *part, int32_t ncells, double *L, int64_t nslab) {
+  critical_path_priority();
   const int32_t t = blockIdx.x;
   int n = (int)ucnt[t];
   while (n > 1 && extra_slab(uoff, t, n - 1) >= nslab) n--;
@@ -657,6 +667,7 @@ __device__ inline T table_elem(const T *y, uint32_t byte_off) {
 template <class T>
 __global__ void k_group_afrag(const T *Yg, uint32_t ng, const uint32_t *codes, const int8_t *gdl,
                               T *Af, const uint32_t *dng) {
+  critical_path_priority();
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (dng) ng = min(ng, *dng);
   if (t >= (int64_t)ng * (NBLK * 64)) return;
@@ -686,6 +697,7 @@ __global__ __launch_bounds__(64 * TILE_WAVES) void k_m2l_tiles(
     const int32_t *tile_off, const int2 *units, const uint32_t *ucnt, const uint32_t *uoff,
     int32_t ntiles, int32_t chunk, int xcd_run, const uint2 *cols, const T *Af, const T *Mq,
     Cells c, int32_t ncells, double *L, double *part, const int32_t *dchunk, int64_t nslab) {
+  critical_path_priority();
   __shared__ T Lt[TILE][NR + 1];
   __shared__ T Lt0b[TILE][17];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

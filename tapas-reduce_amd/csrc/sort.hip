@@ -52,6 +52,7 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *total) {
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t *in, int64_t n,
                                                               uint32_t *sums) {
+  __builtin_amdgcn_s_setprio(2);  // as the kernels beside the early P2P (critical_path_priority)
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
   uint32_t s = 0;
 #pragma unroll
@@ -67,6 +68,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t *in
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const uint32_t *in, int64_t n,
                                                             const uint32_t *block_off,
                                                             uint32_t *out) {
+  __builtin_amdgcn_s_setprio(2);  // as the kernels beside the early P2P (critical_path_priority)
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
   uint32_t v[SCAN_ITEMS];
   uint32_t s = 0;
@@ -106,6 +108,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const K *__restrict__
                                                            int shift,
                                                            uint32_t mask, uint32_t *counts,
                                                            int nblocks, const uint32_t *dn) {
+  __builtin_amdgcn_s_setprio(2);  // as the kernels beside the early P2P (critical_path_priority)
   __shared__ uint32_t hist[RS_BINS];
   if (dn) n = min(n, (int64_t)*dn);  // the count on the device, at most the capacity n
   for (int i = threadIdx.x; i < RS_BINS; i += RS_THREADS) hist[i] = 0;
@@ -125,6 +128,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const K *__restrict__ keys_in, const uint32_t *__restrict__ vals_in, K *__restrict__ keys_out,
     uint32_t *__restrict__ vals_out, int64_t n, int shift, int nbits,
     const uint32_t *__restrict__ offsets, int nblocks, const uint32_t *dn) {
+  __builtin_amdgcn_s_setprio(2);  // as the kernels beside the early P2P (critical_path_priority)
   // The wave histograms are dead once every key knows its place in the tile,
   // so the tile's staging arrays reuse their LDS (more blocks per CU).
   constexpr size_t HIST_BYTES = sizeof(uint32_t) * RS_WAVES * RS_BINS;

@@ -205,6 +205,7 @@ __device__ inline bool level_extent(const int32_t *lev, int dl, int sel, int32_t
 __global__ __launch_bounds__(7 * 64) void k_m2m_mfma(Cells c, int32_t first_child, int32_t npar,
                                                      const double *Fm, double *M,
                                                      const int32_t *lev, int dl, int sel) {
+  critical_path_priority();
   if (!level_extent(lev, dl, sel, first_child, npar)) return;
   if ((int32_t)blockIdx.x * 16 >= npar) return;
   const int lane = threadIdx.x & 63, rb = threadIdx.x >> 6;
@@ -356,6 +357,7 @@ constexpr int M2M_SMALL_WAVES = 4;
 __global__ __launch_bounds__(M2M_SMALL_WAVES * 64) void k_m2m_small(
     Cells c, int32_t first_child, int32_t npar, const double *Fm, double *M, const int32_t *lev,
     int dl, int sel) {
+  critical_path_priority();
   if (!level_extent(lev, dl, sel, first_child, npar)) return;
   if ((int32_t)blockIdx.x * 16 >= npar) return;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, rb = blockIdx.y;
@@ -527,6 +529,7 @@ __device__ inline void image_of(int a, int &ix, int &iy, int &iz) {
 // in a fixed order, their partial sums added in wave order.
 __global__ __launch_bounds__(256) void k_periodic_far(const double *Mroot, double *Lroot,
                                                       double cycle, int images) {
+  critical_path_priority();
   __shared__ cd Ms[NTERM], Mn[NTERM], Yl[NTERM];
   __shared__ cd Ys[4][NTERM];
   __shared__ cd part[4][NTERM];
