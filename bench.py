@@ -291,9 +291,15 @@ def kernels(st, phases, precision, workload, basis="spherical", cart_direct=Fals
                      "traffic_knobs": tile_traffic["knobs"] if tile_traffic else None,
                      "flops_per_pair": M2L_FLOPS_PER_PAIR, "pairs_per_launch": grouped,
                      "ms_per_launch": tiles_ms}
+    timing = ("HIP-event span of the launch in the timed evaluation, on the P2P's stream, "
+              "beside the kernels that run with it")
+    if resident:
+        timing += ("; as resident waves at the default issue priority beside kernels that "
+                   "raise theirs, the P2P takes the slots they leave, so the span is longer "
+                   "than the kernel's own time (roofline.isolated)")
     return {
         "kernel": p2p_name, "bound": "valu", "achieved": p2p_tf, "peak": p2p_peak,
-        "unit": "TFLOP/s", "frac": p2p_tf / p2p_peak,
+        "unit": "TFLOP/s", "frac": p2p_tf / p2p_peak, "timing": timing,
         "traffic": p2p_traffic["bytes"] if p2p_traffic else None,
         "traffic_source": p2p_traffic["source"] if p2p_traffic else None,
         "flops_per_pair": P2P_FLOPS_PER_PAIR, "body_pairs": pairs, "ms_per_launch": p2p_ms,
