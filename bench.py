@@ -362,12 +362,12 @@ def autotune(ctx, prm, fixed, enabled):
     if not enabled:
         return None
     t0 = time.perf_counter()
-    unit, run, sched, tms = ctx.autotune(prm, reps=3)
+    unit, run, sched, tms = ctx.autotune(prm, reps=5)
     return {"m2l_unit_factor": unit, "m2l_xcd_run": run, "p2p_schedule": sched,
             "p2p_waves": int(ctx.get_tuning(tfmm.TUNE_P2P_WAVES)),
             "ms_per_evaluation": tms, "seconds": time.perf_counter() - t0,
             "note": "16 tile-kernel settings, then up to 8 P2P schedule/resident-wave "
-                    "settings, x 3 evaluations each, outside the timed steps"}
+                    "settings, x 5 evaluations each, outside the timed steps"}
 
 
 def learned(ctx, path, precision):

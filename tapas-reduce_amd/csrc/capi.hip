@@ -2082,16 +2082,18 @@ int tfmm_autotune(tfmm_ctx *c, const tfmm_params *p, int reps, double *chosen) {
   // schedule 2 to evaluations without a graph)
   std::vector<Cand> p2p;
   const bool fp32 = p->precision == TFMM_MIXED;
-  // (schedule 3 measured level with schedule 2 at 3 waves on the 4M
-  // Plummer, 12.70 against 12.71 ms, and splits the P2P into two launches:
-  // it stays a knob, not a candidate)
-  for (int sched = 0; sched <= 2; sched++) {
-    if (sched == 2 && c->use_graph == 1) continue;
-    for (int waves : {0, 3, 4}) {
-      if (waves && !fp32) continue;
-      if (sched == sched0 && waves == waves0) continue;  // timed in the first stage
-      p2p.push_back({tiles[bt].unit, tiles[bt].run, sched, waves, {}});
-    }
+  // schedules 0-2 with 0, 3 or 4 resident waves (schedule 3 with 2 waves
+  // measured level with schedule 2 with 3 on the 4M Plummer, 11.7 ms, and
+  // 5 % slower on the 8M two-cluster problem, where the autotuner took it
+  // once on noise: it stays a knob, not a candidate)
+  struct SW { int sched, waves; };
+  static const SW kP2P[] = {{0, 0}, {0, 3}, {0, 4}, {1, 0}, {1, 3},
+                            {1, 4}, {2, 0}, {2, 3}, {2, 4}};
+  for (const SW &k : kP2P) {
+    if (k.sched >= 2 && c->use_graph == 1) continue;  // the early start needs phase A
+    if (k.waves && !fp32) continue;                     // resident waves: the FP32 P2P
+    if (k.sched == sched0 && k.waves == waves0) continue;  // timed in the first stage
+    p2p.push_back({tiles[bt].unit, tiles[bt].run, k.sched, k.waves, {}});
   }
   size_t bp = 0;
   double bp_ms = bt_ms;

@@ -55,12 +55,12 @@ def test_gpu_autotune(ctx):
         ctx.evaluate(prm)
         f0 = ctx.fields()
         unit, run, sched, ms = ctx.autotune(prm, reps=1)
-        assert unit in (0.0, 1.5, 2.0, 3.0) and run in (8, 16, 32, 64) and sched in (0, 1, 2)
+        assert unit in (0.0, 1.5, 2.0, 3.0) and run in (8, 16, 32, 64)
         assert ms > 0
         assert ctx.get_tuning(tfmm.TUNE_M2L_UNIT) == unit
         assert ctx.get_tuning(tfmm.TUNE_M2L_XCD_RUN) == run
         assert ctx.get_tuning(tfmm.TUNE_P2P_SCHEDULE) == sched
-        assert ctx.get_tuning(tfmm.TUNE_P2P_WAVES) in (0, 3, 4)
+        assert sched in (0, 1, 2) and ctx.get_tuning(tfmm.TUNE_P2P_WAVES) in (0, 3, 4)
         ctx.evaluate(prm)  # the chosen setting: the same fields up to summation order
         ep, ea = oracle.rel_l2(ctx.fields(), f0)
         assert ep < 1e-6 and ea < 1e-6, (ep, ea)
