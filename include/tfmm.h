@@ -312,7 +312,8 @@ int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p
 int tfmm_set_tuning(tfmm_ctx *ctx, int knob, double value);
 int tfmm_get_tuning(tfmm_ctx *ctx, int knob, double *value);
 /* Autotuner: evaluates the current bodies with params, reps times per
- * candidate, in two stages -- the tile-kernel knobs (unit factor {0, 1.5,
+ * candidate (after 2 uncounted evaluations of the candidate, which let the
+ * device settle to the setting), in two stages -- the tile-kernel knobs (unit factor {0, 1.5,
  * 2, 3} x XCD run {8, 16, 32, 64}) at the current P2P setting, then the
  * P2P's schedule {0, 1, 2} x resident waves {0, 3, 4} (waves for mixed
  * precision only, schedule 2 not with TFMM_TUNE_GRAPH 1) at the best tile

@@ -2042,17 +2042,26 @@ int tfmm_autotune(tfmm_ctx *c, const tfmm_params *p, int reps, double *chosen) {
     c->p2p_schedule = k.sched;
     c->p2p_waves = k.waves;
   };
+  // Each candidate is evaluated as a block: AT_SETTLE evaluations that are
+  // not counted, then reps counted ones.  The time of a setting depends on
+  // the few evaluations before it (measured: the P2P schedule 2 with 3
+  // resident waves took 12.5, 11.9, then 11.7 ms after a switch, and 13.0
+  // when the candidates were interleaved), so interleaving would rank the
+  // settings by their neighbours.
+  constexpr int AT_SETTLE = 2;
   auto run = [&](std::vector<Cand> &cand, size_t &best, double &best_ms) -> int {
-    for (int rep = 0; rep < reps; rep++)
-      for (Cand &k : cand) {
-        apply(k);
+    for (Cand &k : cand) {
+      apply(k);
+      for (int rep = -AT_SETTLE; rep < reps; rep++) {
         int rc = tfmm_evaluate(c, p);
         if (rc) return rc;
+        if (rep < 0) continue;
         tfmm_stats_t st;
         rc = tfmm_get_stats(c, &st);
         if (rc) return rc;
         k.ms.push_back(st.ms_total);
       }
+    }
     for (size_t i = 0; i < cand.size(); i++) {
       std::vector<double> v = cand[i].ms;
       std::sort(v.begin(), v.end());
