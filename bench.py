@@ -263,7 +263,8 @@ def kernels(st, phases, precision, workload, basis="spherical", cart_direct=Fals
     p2p_ms = phases["ms_p2p"]
     p2p_tf = P2P_FLOPS_PER_PAIR * pairs / (p2p_ms * 1e-3) / 1e12 if p2p_ms > 0 else 0.0
     p2p_peak = PEAK_FP32_TFLOPS if mixed else PEAK_FP64_TFLOPS
-    p2p_name = ("k_p2p_fp32_resident" if resident else "k_p2p_fp32") if mixed else "k_p2p_fp64"
+    p2p_name = ("k_p2p_fp32_resident" if resident else "k_p2p_fp32") if mixed else (
+        "k_p2p_fp64_resident" if resident else "k_p2p_fp64")
     grouped = st["n_m2l_exec"] - st["n_m2l_fallback"]
     tiles_ms = phases["ms_m2l_tiles"]
     m2l_tf = M2L_FLOPS_PER_PAIR * grouped / (tiles_ms * 1e-3) / 1e12 if tiles_ms > 0 else 0.0

@@ -286,10 +286,11 @@ int tfmm_get_lists(tfmm_ctx *ctx, int32_t *m2l_off, int32_t *m2l_src, int32_t *p
  * inside a graph read 0.  tfmm_upward / _traverse / _downward called
  * separately always launch one by one. */
 #define TFMM_TUNE_GRAPH 4
-/* TFMM_TUNE_P2P_WAVES: 0 (default) runs the FP32 P2P as one workgroup per
- * cell; k in 1..8 runs it as k resident waves per SIMD that take target
- * leaves from a counter, so it holds a bounded share of the CUs and the
- * kernels beside it keep slots (see TFMM_TUNE_P2P_SCHEDULE 2 and 3). */
+/* TFMM_TUNE_P2P_WAVES: 0 (default) runs the P2P (FP32 or FP64) as one
+ * workgroup per cell; k in 1..8 runs it as k resident waves per SIMD that
+ * take target leaves from a counter, so it holds a bounded share of the CUs
+ * and the kernels beside it keep slots (see TFMM_TUNE_P2P_SCHEDULE 2 and 3;
+ * the drain of schedule 3 is the FP32 P2P's). */
 #define TFMM_TUNE_P2P_WAVES 5
 /* TFMM_TUNE_DEVICE_LIMIT: a limit in bytes (0 = none, the default) on the
  * device memory the library's buffers hold, over all contexts of the

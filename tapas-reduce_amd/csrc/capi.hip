@@ -886,7 +886,7 @@ int run_lists(tfmm_ctx *c) {
                           c->side, period(c)));
     else
       TRY(launch_p2p_fp64(k, (int32_t)c->ncells, pl, sorted_bodies(c), dp<double>(c->acc),
-                          c->side, period(c)));
+                          p2p_resident(c), dp<uint32_t>(c->p2p_next), c->side, period(c)));
     TRY(hipEventRecord(c->ev[EV_P2P1], c->side));
     return TFMM_OK;
   };
@@ -1144,7 +1144,8 @@ int fork_p2p(tfmm_ctx *c, int64_t ncells = -1) {
                         dp<double>(c->acc), p2p_resident(c), dp<uint32_t>(c->p2p_next), ps,
                         period(c)));
   else
-    TRY(launch_p2p_fp64(k, (int32_t)ncells, pl, sorted_bodies(c), dp<double>(c->acc), ps,
+    TRY(launch_p2p_fp64(k, (int32_t)ncells, pl, sorted_bodies(c), dp<double>(c->acc),
+                        p2p_resident(c), dp<uint32_t>(c->p2p_next), ps,
                         period(c)));
   TRY(rec(c, EV_P2P1, ps));
   TRY(hipEventRecord(c->ev[EV_P2PJOIN], ps));
@@ -2087,10 +2088,9 @@ int tfmm_autotune(tfmm_ctx *c, const tfmm_params *p, int reps, double *chosen) {
     restore();
     return rc;
   }
-  // the P2P's start and resident waves (the waves apply to the FP32 P2P;
-  // schedule 2 to evaluations without a graph)
+  // the P2P's start and resident waves (schedule 2 to evaluations without
+  // a graph)
   std::vector<Cand> p2p;
-  const bool fp32 = p->precision == TFMM_MIXED;
   // schedules 0-2 with 0, 3 or 4 resident waves (schedule 3 with 2 waves
   // measured level with schedule 2 with 3 on the 4M Plummer, 11.7 ms, and
   // 5 % slower on the 8M two-cluster problem, where the autotuner took it
@@ -2100,7 +2100,6 @@ int tfmm_autotune(tfmm_ctx *c, const tfmm_params *p, int reps, double *chosen) {
                             {1, 4}, {2, 0}, {2, 3}, {2, 4}};
   for (const SW &k : kP2P) {
     if (k.sched >= 2 && c->use_graph == 1) continue;  // the early start needs phase A
-    if (k.waves && !fp32) continue;                     // resident waves: the FP32 P2P
     if (k.sched == sched0 && k.waves == waves0) continue;  // timed in the first stage
     p2p.push_back({tiles[bt].unit, tiles[bt].run, k.sched, k.waves, {}});
   }

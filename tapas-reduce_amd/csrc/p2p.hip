@@ -353,12 +353,37 @@ __global__ __launch_bounds__(64) void k_p2p_fp64(Cells c, Csr list, Bodies b, do
   p2p_leaf<double, RING64>(blockIdx.x, c, list, b, nullptr, acc, ring, cycle);
 }
 
+// The FP64 P2P as resident waves taking target cells from a counter, as
+// k_p2p_fp32_resident below (each target computed by one wave exactly as by
+// k_p2p_fp64, so the fields do not change).
+__global__ __launch_bounds__(64) void k_p2p_fp64_resident(Cells c, int32_t ncells, Csr list,
+                                                         Bodies b, double *acc, uint32_t *next,
+                                                         double cycle) {
+  __shared__ double4 ring[ring_entries(RING64)];
+  __shared__ uint32_t take;
+  while (true) {
+    __syncthreads();
+    if (threadIdx.x == 0) take = atomicAdd(next, 1u);
+    __syncthreads();
+    const uint32_t ti = take;
+    if (ti >= (uint32_t)ncells) return;
+    p2p_leaf<double, RING64>((int32_t)ti, c, list, b, nullptr, acc, ring, cycle);
+  }
+}
+
 }  // namespace
 
 hipError_t launch_p2p_fp64(Cells c, int32_t ncells, Csr list, Bodies b, double *acc,
-                           hipStream_t s, double cycle) {
+                           int resident, uint32_t *next, hipStream_t s, double cycle) {
   if (ncells <= 0) return hipSuccess;
-  k_p2p_fp64<<<ncells, 64, 0, s>>>(c, list, b, acc, cycle);
+  if (resident > 0 && next) {
+    hipError_t e = launch_fill_zero(next, 4, s);
+    if (e != hipSuccess) return e;
+    const int grid = resident < ncells ? resident : ncells;
+    k_p2p_fp64_resident<<<grid, 64, 0, s>>>(c, ncells, list, b, acc, next, cycle);
+  } else {
+    k_p2p_fp64<<<ncells, 64, 0, s>>>(c, list, b, acc, cycle);
+  }
   return hipGetLastError();
 }
 

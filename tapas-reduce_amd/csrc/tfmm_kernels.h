@@ -206,8 +206,10 @@ hipError_t launch_m2l_fp64(Cells c, int32_t ncells, Csr list, int32_t npairs, co
 
 // p2p.hip
 // (list sources are source words: leaf and periodic image, cycle the period)
+// resident > 0: that many resident waves take the target cells from *next
+// (cleared first), as launch_p2p_fp32
 hipError_t launch_p2p_fp64(Cells c, int32_t ncells, Csr list, Bodies b, double *acc,
-                           hipStream_t s, double cycle = 0);
+                           int resident, uint32_t *next, hipStream_t s, double cycle = 0);
 
 // direct.hip
 // target t sums over the n bodies at x, y, z, q: index targets[t] - tbase

@@ -160,3 +160,35 @@ def test_gpu_resident_p2p_and_schedules(ctx, schedule):
         ctx.set_tuning(tfmm.TUNE_GRAPH, 0)
         ctx.set_tuning(tfmm.TUNE_P2P_WAVES, 0)
         ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, 0)
+
+
+def test_gpu_resident_p2p_fp64(ctx):
+    """The FP64 P2P as resident waves (TFMM_TUNE_P2P_WAVES > 0) gives bit for
+    bit the fields of its one-workgroup-per-cell launch, started early or
+    late, and inside the one-launch graph."""
+    x, y, z, q = datasets.plummer(40000, seed=13, charges="uniform")
+    ctx.set_bodies(x, y, z, q)
+    prm = tfmm.make_params(32, 0.4, tfmm.FP64)
+    ctx.set_tuning(tfmm.TUNE_P2P_WAVES, 0)
+    ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, 0)
+    ctx.evaluate(prm)
+    ref = ctx.fields()
+    try:
+        for schedule in (0, 2, 3):
+            for waves in (1, 3):
+                print("fp64 schedule %d waves %d" % (schedule, waves), flush=True)
+                ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, schedule)
+                ctx.set_tuning(tfmm.TUNE_P2P_WAVES, waves)
+                ctx.evaluate(prm)
+                assert np.array_equal(ctx.fields(), ref), (schedule, waves)
+        ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, 2)
+        ctx.set_tuning(tfmm.TUNE_P2P_WAVES, 2)
+        ctx.set_tuning(tfmm.TUNE_GRAPH, 2)
+        for rep in range(3):
+            print("fp64 waves 2 one graph, launch %d" % rep, flush=True)
+            ctx.evaluate(prm)
+            assert np.array_equal(ctx.fields(), ref), rep
+    finally:
+        ctx.set_tuning(tfmm.TUNE_GRAPH, 0)
+        ctx.set_tuning(tfmm.TUNE_P2P_WAVES, 0)
+        ctx.set_tuning(tfmm.TUNE_P2P_SCHEDULE, 0)
